@@ -1,0 +1,229 @@
+/*
+ * range_solver_amd.h -- C ABI of the MI355X (gfx950) element-assembly + sparse-solve path.
+ *
+ * Plain C: integers, doubles, raw pointers and opaque handles; no C++ or torch types cross
+ * this boundary.  Every entry point returns an rs_status; rs_last_error() gives the text of
+ * the last failure on the calling thread.  Non-convergence is not an error (it is reported
+ * in rs_info, as the reference reports nothing and simply stops at nOuterIterations).
+ *
+ * Reference interfaces replaced (paths in Range-Software/range-solver-lib):
+ *   RMatrixSolver            include/rmatrixsolver.h:9-52, src/rmatrixsolver.cpp:123-581
+ *   RMatrixPreconditioner    include/rmatrixpreconditioner.h:7-64, src/rmatrixpreconditioner.cpp
+ *   SparseMatrixCache        src/rmatrixsolver.cpp:14-109 (the flat CSR + ||A||_F)
+ *   RIterationInfo           include/riterationinfo.h:9-91, src/riterationinfo.cpp:74-193
+ *   RSolverGeneric node book src/rsolvergeneric.cpp:498-582; RSolverStress src/rsolverstress.cpp:1564-1652
+ *   RSolverHeat assembly     src/rsolverheat.cpp:361-455 (element loop), :755-820 (assemblyMatrix)
+ *   RSolverStress assembly   src/rsolverstress.cpp:384-715 (surface + volume), :1654-1772
+ *   write-back               src/rsolverheat.cpp:473-480, src/rsolverstress.cpp:800-836
+ */
+#ifndef RANGE_SOLVER_AMD_H
+#define RANGE_SOLVER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RS_EOD 0xFFFFFFFFu /* disabled position in a node book (RConstants::eod) */
+
+typedef enum rs_status {
+    RS_OK = 0,
+    RS_ERR_INVALID_ARG = 1,
+    RS_ERR_UNKNOWN_SOLVER = 2,      /* src/rmatrixsolver.cpp:194-195 */
+    RS_ERR_INVALID_PRECOND = 3,     /* src/rmatrixpreconditioner.cpp:31-32, 73-74 */
+    RS_ERR_DEVICE = 4,
+    RS_ERR_NO_DEVICE = 5,
+    RS_ERR_OUT_OF_MEMORY = 6,
+    RS_ERR_NOT_INITIALIZED = 7,
+    RS_ERR_UNSUPPORTED = 8
+} rs_status;
+
+/* Same values as RMatrixPreconditionerType, include/rmatrixpreconditioner.h:7-16;
+ * LEARNED is new and follows BLOCK_JACOBI. */
+typedef enum rs_precond_type {
+    RS_PRECOND_NONE = 0,
+    RS_PRECOND_JACOBI = 1,
+    RS_PRECOND_BLOCK_JACOBI = 2,
+    RS_PRECOND_LEARNED = 3
+} rs_precond_type;
+
+/* RMatrixSolverConf::Type (range-model-lib): the two types RMatrixSolver::solve dispatches,
+ * src/rmatrixsolver.cpp:186-196. */
+typedef enum rs_solver_type { RS_SOLVER_CG = 0, RS_SOLVER_GMRES = 1 } rs_solver_type;
+
+/* Stopping rule.  REFERENCE is the reference's rule: error = ||r|| / (||A||_F ||x|| + ||b||)
+ * (src/rmatrixsolver.cpp:333-344) fed to RIterationInfo::hasConverged.  RELRES uses
+ * error = ||r|| / ||b|| with the same hasConverged tests. */
+typedef enum rs_stop_rule { RS_STOP_REFERENCE = 0, RS_STOP_RELRES = 1 } rs_stop_rule;
+
+/* FP64 is the drop-in.  MIXED: FP32 matrix values, FP32 SpMV and preconditioner, FP64 dot
+ * products, norms, solution and residual replacement (CG only, RELRES rule). */
+typedef enum rs_precision { RS_PRECISION_FP64 = 0, RS_PRECISION_MIXED = 1 } rs_precision;
+
+typedef enum rs_reason {
+    RS_REASON_RUNNING = 0,
+    RS_REASON_CONVERGED = 1,  /* error < cvg */
+    RS_REASON_STAGNATED = 2,  /* |trend| < eps^2 */
+    RS_REASON_NONNORMAL = 3,  /* error or trend not a normal number (0, NaN, Inf, subnormal) */
+    RS_REASON_MAXIT = 4       /* nOuterIterations reached */
+} rs_reason;
+
+/* Mirrors RMatrixSolverConf as used by the reference (type, nOuterIterations,
+ * nInnerIterations, solverCvgValue, outputFrequency, outputFileName). */
+typedef struct rs_conf {
+    int32_t solver_type;          /* rs_solver_type */
+    uint32_t n_outer_iterations;
+    uint32_t n_inner_iterations;  /* GMRES restart length */
+    uint32_t output_frequency;    /* 0 = silent (RIterationInfo::printIteration) */
+    double solver_cvg_value;
+    int32_t stop_rule;            /* rs_stop_rule */
+    int32_t precision;            /* rs_precision */
+    const char *output_file_name; /* convergence file appended as the reference does; NULL = none */
+} rs_conf;
+
+typedef struct rs_info {
+    uint32_t iterations;          /* CG iterations / GMRES outer iterations performed */
+    uint32_t inner_iterations;    /* GMRES inner steps in total; MIXED: FP64 residual replacements */
+    int32_t reason;               /* rs_reason */
+    int32_t converged;            /* reason is CONVERGED */
+    double error;                 /* last RIterationInfo error */
+    double trend;                 /* last RIterationInfo trend */
+    double equation_scale;        /* 1e9*||A||_F/||b|| (src/rmatrixsolver.cpp:166-171) */
+    double a_norm;                /* ||A||_F */
+    double b_norm;                /* ||b|| */
+    double true_relres;           /* ||b - A x|| / ||b||, recomputed in FP64 after the solve */
+    double solve_ms;              /* device time of the solve, host-visible stream events */
+} rs_info;
+
+/* ---------------------------------------------------------------- runtime ------------ */
+int rs_init(int device);          /* select the device, create the library stream */
+int rs_finalize(void);
+const char *rs_last_error(void);
+int rs_device_count(int *count);
+int rs_version(int *major, int *minor);
+
+/* Device memory helpers for device-resident callers. */
+int rs_dmalloc(void **ptr, size_t bytes);
+int rs_dfree(void *ptr);
+int rs_memcpy_htod(void *dst, const void *src, size_t bytes);
+int rs_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int rs_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int rs_synchronize(void);
+
+/* ---------------------------------------------------------------- matrix ------------- */
+/* Device CSR: the SparseMatrixCache of src/rmatrixsolver.cpp:14-54 (uint32 rowPtr and
+ * column indexes, double values, ||A||_F over stored entries), kept in HBM. */
+typedef struct rs_matrix rs_matrix;
+int rs_matrix_create(uint32_t n_rows, const uint32_t *row_ptr, const uint32_t *col,
+                     const double *val, rs_matrix **out);
+/* refreshSparseMatrixCache, src/rmatrixsolver.cpp:56-97: same pattern, new values. */
+int rs_matrix_update_values(rs_matrix *A, const double *val);
+int rs_matrix_update_values_device(rs_matrix *A, const double *d_val);
+int rs_matrix_info(const rs_matrix *A, uint32_t *n_rows, uint64_t *nnz, double *frobenius);
+int rs_matrix_download(const rs_matrix *A, uint32_t *row_ptr, uint32_t *col, double *val);
+int rs_matrix_spmv(rs_matrix *A, const double *x, double *y);               /* host vectors */
+int rs_matrix_spmv_device(rs_matrix *A, const double *d_x, double *d_y);
+/* Measurement: average device time (ms, stream events) of `reps` back-to-back SpMV launches
+ * (precision RS_PRECISION_FP64: FP64 values/vectors; RS_PRECISION_MIXED: FP32), with
+ * `lanes` lanes per row (0 = the library's choice). */
+int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t reps, uint32_t lanes,
+                         double *avg_ms);
+int rs_matrix_destroy(rs_matrix *A);
+
+/* ---------------------------------------------------------------- preconditioner ----- */
+/* RMatrixPreconditioner(const RSparseMatrix&, type, blockSize), include/rmatrixpreconditioner.h:36 */
+typedef struct rs_precond rs_precond;
+int rs_precond_create(const rs_matrix *A, int32_t type, uint32_t block_size, rs_precond **out);
+/* RMatrixPreconditioner::compute(x, y), include/rmatrixpreconditioner.h:48 (host vectors) */
+int rs_precond_compute(const rs_precond *P, const double *x, double *y);
+int rs_precond_width(const rs_precond *P, uint32_t *width);
+int rs_precond_download(const rs_precond *P, double *data);                 /* n*width values */
+int rs_precond_destroy(rs_precond *P);
+
+/* ---------------------------------------------------------------- solver ------------- */
+/* RMatrixSolver(const RMatrixSolverConf&), include/rmatrixsolver.h:27 */
+typedef struct rs_solver rs_solver;
+int rs_solver_create(const rs_conf *conf, rs_solver **out);
+/* RMatrixSolver copy constructor / assignment, include/rmatrixsolver.h:30-36 */
+int rs_solver_copy(const rs_solver *src, rs_solver **out);
+/* RMatrixSolver::disableConvergenceLogFile, include/rmatrixsolver.h:42 */
+int rs_solver_disable_convergence_log(rs_solver *S);
+/* RMatrixSolver::solve(A, b, x, type, blockSize), include/rmatrixsolver.h:39.
+ * b, x are host arrays of A's row count; x holds the initial guess and receives the
+ * solution (the reference resizes x to n with zeros when it is shorter). */
+int rs_solver_solve(rs_solver *S, rs_matrix *A, const double *b, double *x,
+                    int32_t precond_type, uint32_t block_size, rs_info *info);
+/* Same, with b and x already resident in device memory. */
+int rs_solver_solve_device(rs_solver *S, rs_matrix *A, const double *d_b, double *d_x,
+                           int32_t precond_type, uint32_t block_size, rs_info *info);
+/* (error, trend) per iteration of the last solve, 2*count doubles. */
+int rs_solver_history(const rs_solver *S, double *out, uint32_t cap, uint32_t *count);
+int rs_solver_destroy(rs_solver *S);
+
+/* ---------------------------------------------------------------- model preparation -- */
+/* Host-only (no device needed).  Structured box of nx*ny*nz hexahedral cells, each split
+ * into 6 positively oriented tetrahedra around the cell diagonal (Kuhn split).
+ * xyz: 3*(nx+1)*(ny+1)*(nz+1) doubles, node id = i + (nx+1)*(j + (ny+1)*k).
+ * tets: 4*6*nx*ny*nz node ids, tet id = 6*cell + t, cell = ci + nx*(cj + ny*ck). */
+int rs_mesh_kuhn_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly, double lz,
+                     double *xyz, uint32_t *tets);
+/* Triangular tet faces lying on the plane xyz[axis] == value (|.| <= tol).  Pass
+ * tris == NULL to count only. */
+int rs_mesh_plane_faces(uint32_t n_nodes, const double *xyz, uint32_t n_tets,
+                        const uint32_t *tets, int32_t axis, double value, double tol,
+                        uint32_t *tris, uint64_t cap, uint64_t *count);
+/* Counter-based seeded fields (SplitMix64; Box-Muller for normals); bit-identical on any
+ * host.  uniform: U(0,1); lognormal: exp(mu + sigma*N(0,1)). */
+int rs_field_uniform(uint64_t seed, uint64_t n, double *out);
+int rs_field_lognormal(uint64_t seed, uint64_t n, double mu, double sigma, double *out);
+/* RSolverGeneric::generateNodeBook/rebuildNodeBook (dofs_per_node 1) and
+ * RSolverStress::generateNodeBook (dofs_per_node 3): fixed[n_nodes*dofs_per_node] marks
+ * prescribed positions; positions of nodes in no element are disabled too.  book receives
+ * 0..n-1 in increasing position order or RS_EOD.  *n_enabled receives n. */
+int rs_node_book(uint32_t n_nodes, uint32_t n_elems, uint32_t nodes_per_elem,
+                 const uint32_t *conn, uint32_t dofs_per_node, const uint8_t *fixed,
+                 uint32_t *book, uint32_t *n_enabled);
+
+/* ---------------------------------------------------------------- assembly ----------- */
+/* An assembly owns the device mesh, the node book, the CSR pattern of the free DOFs (the
+ * matrix the reference builds with RSparseMatrix::addValue), the element -> CSR scatter map
+ * and a node-disjoint colouring of the elements (each colour is one conflict-free launch,
+ * so the result is deterministic without atomics). */
+typedef struct rs_assembly rs_assembly;
+int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t n_tets,
+                       const uint32_t *tets, uint32_t dofs_per_node, const uint8_t *fixed,
+                       rs_assembly **out);
+int rs_assembly_info(const rs_assembly *S, uint32_t *n_dof, uint64_t *nnz,
+                     uint32_t *n_colours);
+int rs_assembly_book(const rs_assembly *S, uint32_t *book);
+/* Device time (ms, stream events) of the last rs_assemble_* call, colour launches only. */
+int rs_assembly_last_ms(const rs_assembly *S, double *ms);
+/* The assembled matrix (owned by the assembly; do not destroy). */
+int rs_assembly_matrix(rs_assembly *S, rs_matrix **A);
+/* Heat (RSolverHeat::prepare + assemblyMatrix).  Per-element conduction k, volumetric heat
+ * q, and (transient only) density*capacity rhoc; t_node = nodeTemperature (prescribed values
+ * at fixed nodes, previous step elsewhere).  Host arrays; writes the matrix values and b
+ * (n_dof doubles, host or device per the _device variant). */
+int rs_assemble_heat(rs_assembly *S, const double *k, const double *q, const double *rhoc,
+                     const double *t_node, int32_t transient, double alpha, double dt,
+                     double *b);
+/* Device-resident variant: all arrays are device pointers. */
+int rs_assemble_heat_device(rs_assembly *S, const double *d_k, const double *d_q,
+                            const double *d_rhoc, const double *d_t_node, int32_t transient,
+                            double alpha, double dt, double *d_b);
+/* Stress (RSolverStress::prepare + assemblyMatrix, static): per-element E, nu, rho;
+ * gravity (3); u_node (3 per node: prescribed displacements); n_tri surface triangles
+ * with a force per unit area each (3 doubles).  Host arrays. */
+int rs_assemble_stress(rs_assembly *S, const double *E, const double *nu, const double *rho,
+                       const double *gravity, const double *u_node, uint32_t n_tri,
+                       const uint32_t *tris, const double *traction, double *b);
+/* Write-back (src/rsolverheat.cpp:473-480): node[i*dpn+c] = x[book] where enabled. */
+int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node);
+int rs_assembly_destroy(rs_assembly *S);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

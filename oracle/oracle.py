@@ -1,0 +1,210 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker or the timed CPU baseline.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+EOD = 0xFFFFFFFF
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI = 0, 1, 2
+SOLVER_CG, SOLVER_GMRES = 0, 1
+STOP_REFERENCE, STOP_RELRES = 0, 1
+EPS = 1.0e-14
+
+
+class OrConf(C.Structure):
+    _fields_ = [("solver_type", C.c_int), ("n_outer", C.c_uint), ("n_inner", C.c_uint),
+                ("cvg", C.c_double), ("stop_rule", C.c_int)]
+
+
+class OrInfo(C.Structure):
+    _fields_ = [("iterations", C.c_uint), ("inner_total", C.c_uint), ("error", C.c_double),
+                ("trend", C.c_double), ("equation_scale", C.c_double), ("a_norm", C.c_double),
+                ("b_norm", C.c_double), ("reason", C.c_int)]
+
+
+_lib = None
+_P = C.c_void_p
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    lib.or_has_converged.argtypes = [C.c_double, C.c_double, C.c_double]
+    lib.or_has_converged.restype = C.c_int
+    lib.or_converged_reason.argtypes = [C.c_double, C.c_double, C.c_double]
+    lib.or_converged_reason.restype = C.c_int
+    lib.or_precond_build.argtypes = [C.c_uint, _P, _P, _P, C.c_int, C.c_uint, _P, C.POINTER(C.c_uint)]
+    lib.or_precond_apply.argtypes = [C.c_int, C.c_uint, C.c_uint, _P, _P, _P]
+    lib.or_precond_apply.restype = None
+    lib.or_frobenius.argtypes = [C.c_uint, _P, _P]
+    lib.or_frobenius.restype = C.c_double
+    lib.or_solve.argtypes = [C.c_uint, _P, _P, _P, _P, _P, C.c_int, C.c_uint, C.POINTER(OrConf),
+                             C.c_double, C.POINTER(OrInfo), _P, C.c_uint]
+    lib.or_spmv.argtypes = [C.c_uint, _P, _P, _P, _P, _P]
+    lib.or_spmv.restype = None
+    lib.or_node_book.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint, _P, _P]
+    lib.or_node_book.restype = C.c_uint32
+    lib.or_sparse_create.argtypes = [C.c_uint32]
+    lib.or_sparse_create.restype = _P
+    lib.or_sparse_destroy.argtypes = [_P]
+    lib.or_sparse_destroy.restype = None
+    lib.or_sparse_nnz.argtypes = [_P]
+    lib.or_sparse_nnz.restype = C.c_uint64
+    lib.or_sparse_to_csr.argtypes = [_P, _P, _P, _P]
+    lib.or_sparse_to_csr.restype = None
+    lib.or_heat_assemble.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, C.c_uint32,
+                                     C.c_int, C.c_double, C.c_double, _P, _P]
+    lib.or_stress_assemble.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P,
+                                       C.c_uint32, _P, _P, _P, C.c_uint32, _P, _P]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def _csr(m):
+    rp = np.ascontiguousarray(m.indptr, np.uint32)
+    col = np.ascontiguousarray(m.indices, np.uint32)
+    val = np.ascontiguousarray(m.data, np.float64)
+    return rp, col, val
+
+
+def has_converged(error: float, trend: float, cvg: float) -> bool:
+    return bool(load().or_has_converged(error, trend, cvg))
+
+
+def converged_reason(error: float, trend: float, cvg: float) -> int:
+    return int(load().or_converged_reason(error, trend, cvg))
+
+
+def frobenius(m) -> float:
+    rp, _, val = _csr(m.tocsr())
+    return load().or_frobenius(m.shape[0], _p(rp), _p(val))
+
+
+def spmv(m, x) -> np.ndarray:
+    rp, col, val = _csr(m.tocsr())
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(m.shape[0])
+    load().or_spmv(m.shape[0], _p(rp), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
+def precond_build(m, ptype: int, block_size: int = 1):
+    m = m.tocsr()
+    n = m.shape[0]
+    rp, col, val = _csr(m)
+    data = np.zeros(max(1, n * max(1, block_size)))
+    w = C.c_uint()
+    rc = load().or_precond_build(n, _p(rp), _p(col), _p(val), ptype, block_size, _p(data), C.byref(w))
+    if rc != 0:
+        raise ValueError("invalid preconditioner")
+    return data[: n * max(1, w.value)], w.value
+
+
+def precond_apply(ptype: int, n: int, width: int, data, x) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(n)
+    data = np.ascontiguousarray(data, np.float64)
+    load().or_precond_apply(ptype, n, width, _p(data), _p(x), _p(y))
+    return y
+
+
+def solve(m, b, x0=None, precond=PRECOND_NONE, block_size=1, solver=SOLVER_CG, n_outer=1000,
+          n_inner=1000, cvg=1e-13, stop_rule=STOP_REFERENCE, prev_error=0.0, history=False):
+    """RMatrixSolver::solve restated (src/rmatrixsolver.cpp:158-581).  Returns (x, info[, hist])."""
+    m = m.tocsr()
+    n = m.shape[0]
+    rp, col, val = _csr(m)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64, copy=True)
+    conf = OrConf(solver, n_outer, n_inner, cvg, stop_rule)
+    info = OrInfo()
+    hist = np.zeros(2 * max(1, n_outer)) if history else None
+    rc = load().or_solve(n, _p(rp), _p(col), _p(val), _p(b), _p(x), precond, block_size,
+                         C.byref(conf), prev_error, C.byref(info), _p(hist),
+                         n_outer if history else 0)
+    if rc != 0:
+        raise ValueError(f"oracle solve failed ({rc})")
+    d = {k: getattr(info, k) for k, _ in info._fields_}
+    if history:
+        return x, d, hist[: 2 * info.iterations].reshape(-1, 2)
+    return x, d
+
+
+def node_book(n_nodes: int, conn, dofs_per_node: int, fixed):
+    conn = np.ascontiguousarray(conn, np.uint32)
+    npe = conn.shape[1]
+    fixed = np.ascontiguousarray(fixed, np.uint8).reshape(-1)
+    book = np.empty(n_nodes * dofs_per_node, np.uint32)
+    n = load().or_node_book(n_nodes, conn.shape[0], npe, _p(conn), dofs_per_node, _p(fixed), _p(book))
+    return book, int(n)
+
+
+def _export(A, n_dof):
+    import scipy.sparse as sp
+    lib = load()
+    nnz = lib.or_sparse_nnz(A)
+    rp = np.empty(n_dof + 1, np.uint32)
+    col = np.empty(nnz, np.uint32)
+    val = np.empty(nnz, np.float64)
+    lib.or_sparse_to_csr(A, _p(rp), _p(col), _p(val))
+    lib.or_sparse_destroy(A)
+    return sp.csr_matrix((val, col.astype(np.int64), rp.astype(np.int64)), shape=(n_dof, n_dof))
+
+
+def heat_assemble(xyz, tets, k, q, t_node, book, n_dof, rhoc=None, transient=False, alpha=1.0,
+                  dt=1.0):
+    """RSolverHeat prepare + assemblyMatrix restated; returns (A as scipy CSR, b)."""
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    k, q, t_node = (np.ascontiguousarray(a, np.float64) for a in (k, q, t_node))
+    rc = np.ascontiguousarray(rhoc, np.float64) if rhoc is not None else None
+    book = np.ascontiguousarray(book, np.uint32)
+    A = lib.or_sparse_create(n_dof)
+    b = np.zeros(max(1, n_dof))
+    lib.or_heat_assemble(xyz.size // 3, _p(xyz), tets.size // 4, _p(tets), _p(k), _p(q), _p(rc),
+                         _p(t_node), _p(book), n_dof, int(bool(transient)), alpha, dt, A, _p(b))
+    return _export(A, n_dof), b[:n_dof]
+
+
+def stress_assemble(xyz, tets, E, nu, rho, gravity, u_node, tris, traction, book, n_dof):
+    """RSolverStress prepare + assemblyMatrix (static) restated; returns (A, b)."""
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    E, nu, rho, u_node = (np.ascontiguousarray(a, np.float64) for a in (E, nu, rho, u_node))
+    g = np.ascontiguousarray(gravity, np.float64)
+    ntri = 0 if tris is None else int(np.asarray(tris).size // 3)
+    tr = np.ascontiguousarray(tris, np.uint32).reshape(-1) if ntri else None
+    tt = np.ascontiguousarray(traction, np.float64).reshape(-1) if ntri else None
+    book = np.ascontiguousarray(book, np.uint32)
+    A = lib.or_sparse_create(n_dof)
+    b = np.zeros(max(1, n_dof))
+    lib.or_stress_assemble(xyz.size // 3, _p(xyz), tets.size // 4, _p(tets), _p(E), _p(nu), _p(rho),
+                           _p(g), _p(u_node), ntri, _p(tr), _p(tt), _p(book), n_dof, A, _p(b))
+    return _export(A, n_dof), b[:n_dof]

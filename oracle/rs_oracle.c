@@ -1,0 +1,775 @@
+/*
+ * rs_oracle.c -- CPU restatement of Range's assembly + sparse-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle): loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py, never by the product.  Each function cites the reference
+ * file:line it restates; the operation order inside the solver loops is kept so that
+ * iteration counts and convergence histories can be compared with the GPU path.
+ *
+ * Parity unpinned: the value of RConstants::eps (OR_EPS), the behaviour of RRMatrix::invert
+ * on a singular block (here: the block becomes zero), the tet quadrature rule of
+ * range-model-lib (here: the exact 4-point degree-2 rule, which gives the closed-form
+ * consistent mass; stiffness and load do not depend on the rule).
+ */
+#include "rs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+/* ------------------------------------------------------------------------------------ */
+/* RIterationInfo::hasConverged -- src/riterationinfo.cpp:90-109                          */
+/* ------------------------------------------------------------------------------------ */
+int or_converged_reason(double error, double trend, double cvg)
+{
+    if (!isnormal(error)) return OR_REASON_NONNORMAL;
+    if (!isnormal(trend)) return OR_REASON_NONNORMAL;
+    if (error < cvg) return OR_REASON_CONVERGED;
+    if (fabs(trend) < pow(OR_EPS, 2)) return OR_REASON_STAGNATED;
+    return OR_REASON_RUNNING;
+}
+
+int or_has_converged(double error, double trend, double cvg)
+{
+    return or_converged_reason(error, trend, cvg) != OR_REASON_RUNNING;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* CSR helpers -- src/rmatrixsolver.cpp:23-54 (cache build, Frobenius norm), :111-119    */
+/* ------------------------------------------------------------------------------------ */
+double or_frobenius(unsigned n, const uint32_t *rowptr, const double *val)
+{
+    double norm2 = 0.0;
+    int64_t nnz = rowptr[n];
+#pragma omp parallel for reduction(+:norm2)
+    for (int64_t j = 0; j < nnz; j++) norm2 += val[j] * val[j];
+    return sqrt(norm2);
+}
+
+static inline double multiply_row(const uint32_t *rowptr, const uint32_t *col, const double *val,
+                                  uint32_t row, const double *x)
+{
+    double v = 0.0;
+    for (uint32_t j = rowptr[row]; j < rowptr[row + 1]; j++) v += val[j] * x[col[j]];
+    return v;
+}
+
+void or_spmv(unsigned n, const uint32_t *rowptr, const uint32_t *col, const double *val,
+             const double *x, double *y)
+{
+#pragma omp parallel for
+    for (int64_t i = 0; i < (int64_t)n; i++) y[i] = multiply_row(rowptr, col, val, (uint32_t)i, x);
+}
+
+static int find_column(const uint32_t *rowptr, const uint32_t *col, uint32_t row, uint32_t c,
+                       uint32_t *pos)
+{
+    for (uint32_t j = rowptr[row]; j < rowptr[row + 1]; j++) {
+        if (col[j] == c) { *pos = j; return 1; }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Preconditioner -- src/rmatrixpreconditioner.cpp:16-190                                */
+/* ------------------------------------------------------------------------------------ */
+
+/* Gauss-Jordan inverse with partial pivoting, in place, w <= 16.  Stand-in for
+ * RRMatrix::invert (range-base-lib); singular -> zero block (parity unpinned). */
+static void invert_block(double *a, unsigned w)
+{
+    double m[16][32];
+    for (unsigned i = 0; i < w; i++) {
+        for (unsigned j = 0; j < w; j++) { m[i][j] = a[i * w + j]; m[i][w + j] = (i == j) ? 1.0 : 0.0; }
+    }
+    for (unsigned c = 0; c < w; c++) {
+        unsigned piv = c;
+        for (unsigned r = c + 1; r < w; r++) if (fabs(m[r][c]) > fabs(m[piv][c])) piv = r;
+        if (m[piv][c] == 0.0) { for (unsigned i = 0; i < w * w; i++) a[i] = 0.0; return; }
+        if (piv != c) {
+            for (unsigned j = 0; j < 2 * w; j++) { double t = m[c][j]; m[c][j] = m[piv][j]; m[piv][j] = t; }
+        }
+        double inv = 1.0 / m[c][c];
+        for (unsigned j = 0; j < 2 * w; j++) m[c][j] *= inv;
+        for (unsigned r = 0; r < w; r++) {
+            if (r == c) continue;
+            double f = m[r][c];
+            if (f == 0.0) continue;
+            for (unsigned j = 0; j < 2 * w; j++) m[r][j] -= f * m[c][j];
+        }
+    }
+    for (unsigned i = 0; i < w; i++) for (unsigned j = 0; j < w; j++) a[i * w + j] = m[i][w + j];
+}
+
+int or_precond_build(unsigned n, const uint32_t *rowptr, const uint32_t *col, const double *val,
+                     int type, unsigned block_size, double *data, unsigned *width_out)
+{
+    if (type == OR_PRECOND_NONE) { *width_out = 0; return 0; }
+    if (type == OR_PRECOND_JACOBI) {
+        /* constructJacobi :78-94 (serial in the reference) */
+        for (unsigned i = 0; i < n; i++) {
+            uint32_t pos;
+            data[i] = 0.0;
+            if (find_column(rowptr, col, i, i, &pos)) {
+                double v = val[pos];
+                data[i] = (v == 0.0) ? 0.0 : 1.0 / v;
+            }
+        }
+        *width_out = 1;
+        return 0;
+    }
+    if (type == OR_PRECOND_BLOCK_JACOBI) {
+        /* constructBlockJacobi :96-148 -- width reduced until it divides n */
+        unsigned w = block_size;
+        if (w == 0 || w > 16) return -1;
+        while (n % w != 0) w--;
+        unsigned nb = n / w;
+#pragma omp parallel for
+        for (int64_t ib = 0; ib < (int64_t)nb; ib++) {
+            double blk[256];
+            for (unsigned k = 0; k < w; k++) {
+                uint32_t m = (uint32_t)ib * w + k;
+                for (unsigned l = 0; l < w; l++) {
+                    uint32_t c = (uint32_t)ib * w + l, pos;
+                    blk[k * w + l] = find_column(rowptr, col, m, c, &pos) ? val[pos] : 0.0;
+                }
+            }
+            invert_block(blk, w);
+            for (unsigned k = 0; k < w; k++)
+                for (unsigned l = 0; l < w; l++) data[((uint32_t)ib * w + k) * w + l] = blk[k * w + l];
+        }
+        *width_out = w;
+        return 0;
+    }
+    return -2;
+}
+
+void or_precond_apply(int type, unsigned n, unsigned width, const double *data,
+                      const double *x, double *y)
+{
+    if (type == OR_PRECOND_JACOBI) {
+        for (unsigned i = 0; i < n; i++) y[i] = data[i] * x[i];            /* :150-163 */
+    } else if (type == OR_PRECOND_BLOCK_JACOBI) {
+        unsigned nb = n / width;                                            /* :165-190 */
+        for (unsigned ib = 0; ib < nb; ib++) {
+            for (unsigned k = 0; k < width; k++) {
+                unsigned m = ib * width + k;
+                double v = 0.0;
+                for (unsigned l = 0; l < width; l++) v += data[m * width + l] * x[ib * width + l];
+                y[m] = v;
+            }
+        }
+    } else {
+        memcpy(y, x, sizeof(double) * n);                                  /* :55-66 */
+    }
+}
+
+/* Apply inside an enclosing OpenMP parallel region (orphaned worksharing, as the reference). */
+static void precond_apply_omp(int type, unsigned n, unsigned width, const double *data,
+                              const double *x, double *y)
+{
+    if (type == OR_PRECOND_JACOBI) {
+#pragma omp for
+        for (int64_t i = 0; i < (int64_t)n; i++) y[i] = data[i] * x[i];
+    } else if (type == OR_PRECOND_BLOCK_JACOBI) {
+        int64_t nb = n / width;
+#pragma omp for
+        for (int64_t ib = 0; ib < nb; ib++) {
+            for (unsigned k = 0; k < width; k++) {
+                uint64_t m = (uint64_t)ib * width + k;
+                double v = 0.0;
+                for (unsigned l = 0; l < width; l++) v += data[m * width + l] * x[(uint64_t)ib * width + l];
+                y[m] = v;
+            }
+        }
+    } else {
+#pragma omp for
+        for (int64_t i = 0; i < (int64_t)n; i++) y[i] = x[i];
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Solvers -- src/rmatrixsolver.cpp:158-581                                               */
+/* ------------------------------------------------------------------------------------ */
+typedef struct iterinfo {
+    double cvg, error, trend;
+    unsigned iteration;
+    double *history; unsigned history_cap;
+} iterinfo;
+
+/* RIterationInfo::setError -- src/riterationinfo.cpp:74-78 */
+static void set_error(iterinfo *it, double error)
+{
+    it->trend = error - it->error;
+    it->error = error;
+    if (it->history && it->iteration < it->history_cap) {
+        it->history[2 * it->iteration] = it->error;
+        it->history[2 * it->iteration + 1] = it->trend;
+    }
+}
+
+/* solveCG -- src/rmatrixsolver.cpp:208-355 */
+static unsigned solve_cg(unsigned m, const uint32_t *rowptr, const uint32_t *col, const double *val,
+                         double An, const double *b, double *x, int ptype, unsigned width,
+                         const double *pdata, unsigned n_outer, int stop_rule, iterinfo *info)
+{
+    double *r = calloc(m ? m : 1, sizeof(double));
+    double *z = calloc(m ? m : 1, sizeof(double));
+    double *p = calloc(m ? m : 1, sizeof(double));
+    double *q = calloc(m ? m : 1, sizeof(double));
+    double ro0 = 0.0, ro1 = 0.0, beta = 0.0, dot = 0.0, alpha = 0.0, xn = 0.0, rn = 0.0, bn = 0.0;
+    unsigned iterations = 0;
+    int converged = 0;
+
+#pragma omp parallel default(shared)
+    {
+#pragma omp for reduction(+:bn)
+        for (int64_t i = 0; i < (int64_t)m; i++) {
+            r[i] = b[i] - multiply_row(rowptr, col, val, (uint32_t)i, x);       /* :236-241 */
+            bn += b[i] * b[i];
+        }
+#pragma omp barrier
+#pragma omp master
+        bn = sqrt(bn);
+
+        for (unsigned it = 0; it < n_outer; it++) {
+#pragma omp barrier
+#pragma omp master
+            info->iteration = it;
+#pragma omp barrier
+            precond_apply_omp(ptype, m, width, pdata, r, z);                   /* :258 */
+#pragma omp master
+            ro0 = 0.0;
+#pragma omp barrier
+#pragma omp for reduction(+:ro0)
+            for (int64_t i = 0; i < (int64_t)m; i++) ro0 += r[i] * z[i];        /* :263-267 */
+#pragma omp master
+            {
+                if (it > 0) {                                                   /* :271-275 */
+                    beta = ro0 / ro1;
+                    ro1 = fmax(ro1, OR_EPS);
+                }
+            }
+#pragma omp barrier
+#pragma omp for
+            for (int64_t i = 0; i < (int64_t)m; i++) p[i] = (it == 0) ? z[i] : p[i] * beta + z[i];
+#pragma omp master
+            dot = 0.0;
+#pragma omp barrier
+#pragma omp for reduction(+:dot)
+            for (int64_t i = 0; i < (int64_t)m; i++) {                          /* :295-300 */
+                q[i] = multiply_row(rowptr, col, val, (uint32_t)i, p);
+                dot += p[i] * q[i];
+            }
+#pragma omp master
+            {
+                if (dot > 0.0) dot = fmax(dot, OR_EPS);                         /* :304-315 */
+                else if (dot < 0.0) dot = fmin(dot, -OR_EPS);
+                else dot = OR_EPS;
+                alpha = ro0 / dot;
+                ro1 = ro0;
+                xn = 0.0;
+                rn = 0.0;
+            }
+#pragma omp barrier
+#pragma omp for reduction(+:xn,rn)
+            for (int64_t i = 0; i < (int64_t)m; i++) {                          /* :324-331 */
+                x[i] += alpha * p[i];
+                r[i] -= alpha * q[i];
+                xn += x[i] * x[i];
+                rn += r[i] * r[i];
+            }
+#pragma omp master
+            {
+                xn = sqrt(xn);
+                rn = sqrt(rn);
+                double norm = An * xn + bn;                                     /* :338-344 */
+                if (fabs(norm) < OR_EPS) norm = OR_EPS;
+                double err = (stop_rule == OR_STOP_RELRES) ? rn / fmax(bn, OR_EPS) : rn / norm;
+                set_error(info, err);
+                iterations = it + 1;
+                converged = or_has_converged(info->error, info->trend, info->cvg);
+            }
+#pragma omp barrier
+            if (converged) break;                                               /* :349-352 */
+        }
+    }
+    free(r); free(z); free(p); free(q);
+    return iterations;
+}
+
+/* solveGMRES -- src/rmatrixsolver.cpp:357-581.  The classical Gram-Schmidt dot products of
+ * :461-476 are summed per thread and then in thread order (the reference uses a critical
+ * section, whose order is not fixed). */
+static unsigned solve_gmres(unsigned mA, const uint32_t *rowptr, const uint32_t *col,
+                            const double *val, double An, const double *b, double *x, int ptype,
+                            unsigned width, const double *pdata, unsigned nouter, unsigned ninner,
+                            int stop_rule, iterinfo *info, unsigned *inner_total)
+{
+    size_t N = mA ? mA : 1;
+    double *ro = calloc(N, sizeof(double));
+    double *w = calloc(N, sizeof(double));
+    double *p = calloc(ninner + 1, sizeof(double));
+    double *y = calloc(ninner + 1, sizeof(double));
+    double *c = calloc(ninner + 1, sizeof(double));
+    double *s = calloc(ninner + 1, sizeof(double));
+    double *v = calloc((size_t)(ninner + 1) * N, sizeof(double));
+    double *z = calloc((size_t)(ninner + 1) * N, sizeof(double));
+    double *h = calloc((size_t)(ninner + 1) * ninner, sizeof(double));
+    double *hv = calloc(ninner + 1, sizeof(double));
+    int nthr = omp_get_max_threads();
+    double *hloc = calloc((size_t)nthr * (ninner + 1), sizeof(double));
+#define H(i, j) h[(size_t)(i) * ninner + (j)]
+#define V(i) (v + (size_t)(i) * N)
+#define Z(i) (z + (size_t)(i) * N)
+    double bn = 0.0;
+    for (unsigned i = 0; i < mA; i++) bn += b[i] * b[i];
+    bn = sqrt(bn);
+    unsigned iterations = 0, total = 0;
+
+    for (unsigned ito = 0; ito < nouter; ito++) {
+        info->iteration = ito;
+        memset(h, 0, sizeof(double) * (size_t)(ninner + 1) * ninner);
+        double beta = 0.0, xn = 0.0;
+#pragma omp parallel for reduction(+:xn,beta)
+        for (int64_t i = 0; i < (int64_t)mA; i++) {                             /* :401-407 */
+            ro[i] = b[i] - multiply_row(rowptr, col, val, (uint32_t)i, x);
+            xn += x[i] * x[i];
+            beta += ro[i] * ro[i];
+        }
+        xn = sqrt(xn);
+        beta = sqrt(beta);
+        double norm = An * xn + bn;                                             /* :415-421 */
+        if (fabs(norm) < OR_EPS) norm = OR_EPS;
+        double err = (stop_rule == OR_STOP_RELRES) ? beta / fmax(bn, OR_EPS) : beta / norm;
+        set_error(info, err);
+        iterations = ito + 1;
+        if (or_has_converged(info->error, info->trend, info->cvg)) break;
+
+#pragma omp parallel for
+        for (int64_t i = 0; i < (int64_t)mA; i++) V(0)[i] = (beta == 0.0) ? 0.0 : ro[i] / beta;
+        memset(p, 0, sizeof(double) * (ninner + 1));
+        p[0] = beta;
+
+        unsigned iti;
+        for (iti = 0; iti < ninner; iti++) {
+            total++;
+#pragma omp parallel
+            precond_apply_omp(ptype, mA, width, pdata, V(iti), Z(iti));         /* :446 */
+#pragma omp parallel for
+            for (int64_t i = 0; i < (int64_t)mA; i++) w[i] = multiply_row(rowptr, col, val, (uint32_t)i, Z(iti));
+            memset(hloc, 0, sizeof(double) * (size_t)nthr * (ninner + 1));
+#pragma omp parallel
+            {
+                double *hl = hloc + (size_t)omp_get_thread_num() * (ninner + 1);
+#pragma omp for
+                for (int64_t j = 0; j < (int64_t)mA; j++) {                    /* :461-469 */
+                    double wj = w[j];
+                    for (unsigned i = 0; i <= iti; i++) hl[i] += wj * V(i)[j];
+                }
+            }
+            for (unsigned i = 0; i <= iti; i++) {
+                hv[i] = 0.0;
+                for (int t = 0; t < nthr; t++) hv[i] += hloc[(size_t)t * (ninner + 1) + i];
+                H(i, iti) = hv[i];                                              /* :480-483 */
+            }
+            double wnorm = 0.0;
+#pragma omp parallel for reduction(+:wnorm)
+            for (int64_t j = 0; j < (int64_t)mA; j++) {                         /* :487-504 */
+                double value = w[j];
+                for (unsigned i = 0; i <= iti; i++) value -= hv[i] * V(i)[j];
+                w[j] = value;
+                wnorm += value * value;
+            }
+            H(iti + 1, iti) = sqrt(wnorm);
+            double hn = H(iti + 1, iti);
+#pragma omp parallel for
+            for (int64_t i = 0; i < (int64_t)mA; i++) V(iti + 1)[i] = (hn == 0.0) ? 0.0 : w[i] / hn;
+            for (unsigned i = 1; i <= iti; i++) {                               /* :521-526 */
+                double hsave = H(i - 1, iti);
+                H(i - 1, iti) = c[i - 1] * hsave + s[i - 1] * H(i, iti);
+                H(i, iti) = -s[i - 1] * hsave + c[i - 1] * H(i, iti);
+            }
+            double g = sqrt(H(iti, iti) * H(iti, iti) + H(iti + 1, iti) * H(iti + 1, iti));
+            if (g == 0.0) { c[iti] = s[iti] = 0.0; }
+            else { c[iti] = H(iti, iti) / g; s[iti] = H(iti + 1, iti) / g; }
+            H(iti, iti) = g;
+            H(iti + 1, iti) = 0.0;
+            p[iti + 1] = -s[iti] * p[iti];
+            p[iti] = c[iti] * p[iti];
+            if (fabs(p[iti + 1]) < beta * 1.0e-4) { iti++; break; }             /* :546-550 */
+        }
+        /* back substitution :557-566 */
+        y[iti - 1] = (H(iti - 1, iti - 1) == 0.0) ? 0.0 : p[iti - 1] / H(iti - 1, iti - 1);
+        for (int i = (int)iti - 2; i >= 0; i--) {
+            for (int j = (int)iti - 1; j > i; j--) p[i] -= y[j] * H(i, j);
+            y[i] = (H(i, i) == 0.0) ? 0.0 : p[i] / H(i, i);
+        }
+#pragma omp parallel for
+        for (int64_t i = 0; i < (int64_t)mA; i++) {                             /* :571-578 */
+            for (unsigned j = 0; j <= iti - 1; j++) x[i] += Z(j)[i] * y[j];
+        }
+    }
+#undef H
+#undef V
+#undef Z
+    free(ro); free(w); free(p); free(y); free(c); free(s); free(v); free(z); free(h); free(hv); free(hloc);
+    *inner_total = total;
+    return iterations;
+}
+
+/* RMatrixSolver::solve -- src/rmatrixsolver.cpp:158-201 */
+int or_solve(unsigned n, const uint32_t *rowptr, const uint32_t *col, const double *val,
+             const double *b, double *x, int precond_type, unsigned block_size,
+             const or_conf *conf, double prev_error, or_info *out, double *history,
+             unsigned history_cap)
+{
+    unsigned width = 0;
+    double *pdata = NULL;
+    if (precond_type < 0 || precond_type > OR_PRECOND_BLOCK_JACOBI) return -2;
+    if (conf->solver_type != OR_SOLVER_CG && conf->solver_type != OR_SOLVER_GMRES) return -3;
+    if (precond_type != OR_PRECOND_NONE) {
+        unsigned w = precond_type == OR_PRECOND_BLOCK_JACOBI ? block_size : 1;
+        pdata = malloc(sizeof(double) * (size_t)(n ? n : 1) * (w ? w : 1));
+        if (or_precond_build(n, rowptr, col, val, precond_type, block_size, pdata, &width) != 0) {
+            free(pdata);
+            return -2;
+        }
+    }
+    double An = or_frobenius(n, rowptr, val);
+    double bn = 0.0;
+    for (unsigned i = 0; i < n; i++) bn += b[i] * b[i];
+    bn = sqrt(bn);
+    double scale = 1.0;
+    if (bn != 0.0 && An != 0.0) scale = 1.0e9 / fabs(bn / An);                  /* :166-171 */
+
+    double *y = malloc(sizeof(double) * (n ? n : 1));
+    for (unsigned i = 0; i < n; i++) { y[i] = b[i] * scale; x[i] *= scale; }   /* :181-184 */
+
+    iterinfo info = { conf->cvg, prev_error, 0.0, 0, history, history_cap };
+    unsigned iters, inner = 0;
+    if (conf->solver_type == OR_SOLVER_CG)
+        iters = solve_cg(n, rowptr, col, val, An, y, x, precond_type, width, pdata, conf->n_outer,
+                         conf->stop_rule, &info);
+    else
+        iters = solve_gmres(n, rowptr, col, val, An, y, x, precond_type, width, pdata, conf->n_outer,
+                            conf->n_inner, conf->stop_rule, &info, &inner);
+    double inv = 1.0 / scale;
+    for (unsigned i = 0; i < n; i++) x[i] *= inv;                              /* :198 */
+
+    if (out) {
+        out->iterations = iters;
+        out->inner_total = inner;
+        out->error = info.error;
+        out->trend = info.trend;
+        out->equation_scale = scale;
+        out->a_norm = An;
+        out->b_norm = bn;
+        int reason = or_converged_reason(info.error, info.trend, conf->cvg);
+        out->reason = reason != OR_REASON_RUNNING ? reason : OR_REASON_MAXIT;
+    }
+    free(y);
+    free(pdata);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Node book -- src/rsolvergeneric.cpp:498-582, src/rsolverstress.cpp:1564-1652           */
+/* ------------------------------------------------------------------------------------ */
+uint32_t or_node_book(uint32_t nn, uint32_t ne, uint32_t npe, const uint32_t *conn,
+                      unsigned dpn, const uint8_t *fixed, uint32_t *book)
+{
+    uint8_t *computable = calloc(nn ? nn : 1, 1);
+    for (uint64_t e = 0; e < ne; e++)
+        for (uint32_t k = 0; k < npe; k++) computable[conn[e * npe + k]] = 1;
+    uint32_t next = 0;
+    for (uint64_t i = 0; i < (uint64_t)nn * dpn; i++) {                         /* :565-582 */
+        int disabled = fixed[i] || !computable[i / dpn];
+        book[i] = disabled ? OR_EOD : next++;
+    }
+    free(computable);
+    return next;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Row-wise sparse matrix with RSparseMatrix::addValue (find-or-insert, accumulate)      */
+/* ------------------------------------------------------------------------------------ */
+typedef struct or_row { uint32_t n, cap; uint32_t *idx; double *val; } or_row;
+struct or_sparse { uint32_t n_rows; or_row *rows; };
+
+or_sparse *or_sparse_create(uint32_t n_rows)
+{
+    or_sparse *A = calloc(1, sizeof(or_sparse));
+    A->n_rows = n_rows;
+    A->rows = calloc(n_rows ? n_rows : 1, sizeof(or_row));
+    return A;
+}
+
+void or_sparse_destroy(or_sparse *A)
+{
+    if (!A) return;
+    for (uint32_t i = 0; i < A->n_rows; i++) { free(A->rows[i].idx); free(A->rows[i].val); }
+    free(A->rows);
+    free(A);
+}
+
+static void add_value(or_sparse *A, uint32_t r, uint32_t c, double v)
+{
+    or_row *row = &A->rows[r];
+    uint32_t lo = 0, hi = row->n;
+    while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (row->idx[mid] < c) lo = mid + 1; else hi = mid; }
+    if (lo < row->n && row->idx[lo] == c) { row->val[lo] += v; return; }
+    if (row->n == row->cap) {
+        row->cap = row->cap ? 2 * row->cap : 16;
+        row->idx = realloc(row->idx, sizeof(uint32_t) * row->cap);
+        row->val = realloc(row->val, sizeof(double) * row->cap);
+    }
+    memmove(row->idx + lo + 1, row->idx + lo, sizeof(uint32_t) * (row->n - lo));
+    memmove(row->val + lo + 1, row->val + lo, sizeof(double) * (row->n - lo));
+    row->idx[lo] = c;
+    row->val[lo] = v;
+    row->n++;
+}
+
+uint64_t or_sparse_nnz(const or_sparse *A)
+{
+    uint64_t s = 0;
+    for (uint32_t i = 0; i < A->n_rows; i++) s += A->rows[i].n;
+    return s;
+}
+
+void or_sparse_to_csr(const or_sparse *A, uint32_t *rowptr, uint32_t *col, double *val)
+{
+    rowptr[0] = 0;
+    for (uint32_t i = 0; i < A->n_rows; i++) rowptr[i + 1] = rowptr[i] + A->rows[i].n;
+#pragma omp parallel for
+    for (int64_t i = 0; i < (int64_t)A->n_rows; i++) {
+        memcpy(col + rowptr[i], A->rows[i].idx, sizeof(uint32_t) * A->rows[i].n);
+        memcpy(val + rowptr[i], A->rows[i].val, sizeof(double) * A->rows[i].n);
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Linear tetrahedron: shape functions, exact 4-point rule, RElement::findJacobian        */
+/* ------------------------------------------------------------------------------------ */
+static const double TET_DN[4][3] = { { -1, -1, -1 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+#define TQ_A 0.5854101966249685
+#define TQ_B 0.1381966011250105
+static const double TET_IP[4][3] = { { TQ_B, TQ_B, TQ_B }, { TQ_A, TQ_B, TQ_B },
+                                     { TQ_B, TQ_A, TQ_B }, { TQ_B, TQ_B, TQ_A } };
+static const double TET_W = 1.0 / 24.0;
+
+static void tet_shape(const double ip[3], double N[4])
+{
+    N[0] = 1.0 - ip[0] - ip[1] - ip[2];
+    N[1] = ip[0];
+    N[2] = ip[1];
+    N[3] = ip[2];
+}
+
+/* Returns detJ of dx/dxi and J[a][b] = (dx/dxi)^-1[b][a], so that
+ * B[m][a] = sum_b dN[m][b] * J[a][b] is grad N_m (src/rsolverheat.cpp:405-411). */
+static double tet_jacobian(const double X[4][3], double J[3][3])
+{
+    double G[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            G[a][b] = 0.0;
+            for (int m = 0; m < 4; m++) G[a][b] += X[m][a] * TET_DN[m][b];
+        }
+    double det = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1])
+               - G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0])
+               + G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+    double inv[3][3];
+    inv[0][0] = (G[1][1] * G[2][2] - G[1][2] * G[2][1]) / det;
+    inv[0][1] = (G[0][2] * G[2][1] - G[0][1] * G[2][2]) / det;
+    inv[0][2] = (G[0][1] * G[1][2] - G[0][2] * G[1][1]) / det;
+    inv[1][0] = (G[1][2] * G[2][0] - G[1][0] * G[2][2]) / det;
+    inv[1][1] = (G[0][0] * G[2][2] - G[0][2] * G[2][0]) / det;
+    inv[1][2] = (G[0][2] * G[1][0] - G[0][0] * G[1][2]) / det;
+    inv[2][0] = (G[1][0] * G[2][1] - G[1][1] * G[2][0]) / det;
+    inv[2][1] = (G[0][1] * G[2][0] - G[0][0] * G[2][1]) / det;
+    inv[2][2] = (G[0][0] * G[1][1] - G[0][1] * G[1][0]) / det;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) J[a][b] = inv[b][a];
+    return det;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Heat -- src/rsolverheat.cpp:361-455 (element loop), :755-820 (assemblyMatrix)          */
+/* ------------------------------------------------------------------------------------ */
+int or_heat_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                     const double *k, const double *q, const double *rhoc,
+                     const double *t_node, const uint32_t *book, uint32_t n_dof,
+                     int transient, double alpha, double dt, or_sparse *A, double *b)
+{
+    (void)nn;
+    memset(b, 0, sizeof(double) * n_dof);
+#pragma omp parallel for
+    for (int64_t e = 0; e < (int64_t)ne; e++) {
+        const uint32_t *nd = tets + 4 * e;
+        double X[4][3], J[3][3], B[4][3], N[4];
+        double Ke[4][4] = { { 0 } }, Me[4][4] = { { 0 } }, fe[4] = { 0 };
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) X[m][a] = xyz[3 * (size_t)nd[m] + a];
+        for (int ip = 0; ip < 4; ip++) {                                        /* :397-436 */
+            tet_shape(TET_IP[ip], N);
+            double detJ = tet_jacobian(X, J);
+            for (int m = 0; m < 4; m++)
+                for (int a = 0; a < 3; a++)
+                    B[m][a] = TET_DN[m][0] * J[a][0] + TET_DN[m][1] * J[a][1] + TET_DN[m][2] * J[a][2];
+            for (int m = 0; m < 4; m++) {
+                for (int n = 0; n < 4; n++) {
+                    Ke[m][n] += (B[m][0] * B[n][0] + B[m][1] * B[n][1] + B[m][2] * B[n][2])
+                              * k[e] * detJ * TET_W;
+                    if (transient) Me[m][n] += N[m] * N[n] * rhoc[e] * detJ * TET_W;
+                }
+                fe[m] += q[e] * N[m] * detJ * TET_W;
+            }
+        }
+        /* assemblyMatrix :755-820 */
+        double Ae[4][4], be[4];
+        if (transient) {
+            for (int m = 0; m < 4; m++) {
+                be[m] = dt * fe[m];
+                for (int n = 0; n < 4; n++) {
+                    Ae[m][n] = Me[m][n] + alpha * dt * Ke[m][n];
+                    be[m] += (Me[m][n] - (1.0 - alpha) * dt * Ke[m][n]) * t_node[nd[n]];
+                }
+            }
+        } else {
+            memcpy(Ae, Ke, sizeof(Ae));
+            memcpy(be, fe, sizeof(be));
+        }
+        for (int m = 0; m < 4; m++) {                                           /* :787-798 */
+            if (book[nd[m]] == OR_EOD)
+                for (int n = 0; n < 4; n++) be[n] -= Ae[n][m] * t_node[nd[m]];
+        }
+#pragma omp critical
+        {
+            for (int m = 0; m < 4; m++) {                                       /* :802-819 */
+                uint32_t mp = book[nd[m]];
+                if (mp == OR_EOD) continue;
+                b[mp] += be[m];
+                for (int n = 0; n < 4; n++) {
+                    uint32_t np = book[nd[n]];
+                    if (np != OR_EOD) add_value(A, mp, np, Ae[m][n]);
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Stress -- src/rsolverstress.cpp:573-715 (volume), :384-571 (traction), :1654-1772      */
+/* ------------------------------------------------------------------------------------ */
+static void stress_scatter(or_sparse *A, double *b, const uint32_t *book, const uint32_t *nd,
+                           int npe, const double *Ae, const double *be, int with_matrix)
+{
+    int nd3 = 3 * npe;
+    for (int m = 0; m < npe; m++) {
+        for (int i = 0; i < 3; i++) {
+            uint32_t mp = book[3 * (size_t)nd[m] + i];
+            if (mp == OR_EOD) continue;
+            b[mp] += be[3 * m + i];
+            if (!with_matrix) continue;
+            for (int n = 0; n < npe; n++)
+                for (int j = 0; j < 3; j++) {
+                    uint32_t np = book[3 * (size_t)nd[n] + j];
+                    if (np != OR_EOD) add_value(A, mp, np, Ae[(3 * m + i) * nd3 + 3 * n + j]);
+                }
+        }
+    }
+}
+
+int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                       const double *E, const double *nu, const double *rho,
+                       const double *gravity, const double *u_node,
+                       uint32_t ntri, const uint32_t *tris, const double *traction,
+                       const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b)
+{
+    (void)nn;
+    memset(b, 0, sizeof(double) * n_dof);
+    /* Surface elements: force per unit area, :445-515 with surfaceThickness == 0, so the
+     * element matrices are zero and only fe is scattered. */
+    for (uint32_t t = 0; t < ntri; t++) {
+        const uint32_t *nd = tris + 3 * (size_t)t;
+        double P[3][3];
+        for (int m = 0; m < 3; m++) for (int a = 0; a < 3; a++) P[m][a] = xyz[3 * (size_t)nd[m] + a];
+        double e1[3], e2[3], cr[3];
+        for (int a = 0; a < 3; a++) { e1[a] = P[1][a] - P[0][a]; e2[a] = P[2][a] - P[0][a]; }
+        cr[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        cr[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        cr[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        double detJ = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);    /* = 2*area */
+        static const double tip[3][2] = { { 1.0 / 6, 1.0 / 6 }, { 2.0 / 3, 1.0 / 6 }, { 1.0 / 6, 2.0 / 3 } };
+        double fe[9] = { 0 };
+        for (int ip = 0; ip < 3; ip++) {
+            double N[3] = { 1.0 - tip[ip][0] - tip[ip][1], tip[ip][0], tip[ip][1] };
+            for (int m = 0; m < 3; m++) {
+                double integ = N[m] * detJ * (1.0 / 6.0);
+                for (int c = 0; c < 3; c++) fe[3 * m + c] += traction[3 * (size_t)t + c] * integ;
+            }
+        }
+        stress_scatter(A, b, book, nd, 3, NULL, fe, 0);
+    }
+#pragma omp parallel for
+    for (int64_t e = 0; e < (int64_t)ne; e++) {
+        const uint32_t *nd = tets + 4 * e;
+        double X[4][3], J[3][3], B[4][3], N[4];
+        double De[6][6] = { { 0 } }, Be[12][6], BeD[12][6];
+        double Ke[144] = { 0 }, fe[12] = { 0 };
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) X[m][a] = xyz[3 * (size_t)nd[m] + a];
+        double Ee = E[e], v = nu[e];
+        De[0][0] = 1.0 - v; De[0][1] = v;       De[0][2] = v;                  /* :619-623 */
+        De[1][0] = v;       De[1][1] = 1.0 - v; De[1][2] = v;
+        De[2][0] = v;       De[2][1] = v;       De[2][2] = 1.0 - v;
+        De[3][3] = De[4][4] = De[5][5] = (1.0 - 2.0 * v) / 2.0;
+        double f = Ee / ((1.0 + v) * (1.0 - 2.0 * v));
+        for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) De[i][j] *= f;
+        for (int ip = 0; ip < 4; ip++) {
+            tet_shape(TET_IP[ip], N);
+            double detJ = tet_jacobian(X, J);
+            for (int m = 0; m < 4; m++)
+                for (int a = 0; a < 3; a++)
+                    B[m][a] = TET_DN[m][0] * J[a][0] + TET_DN[m][1] * J[a][1] + TET_DN[m][2] * J[a][2];
+            memset(Be, 0, sizeof(Be));
+            for (int m = 0; m < 4; m++) {                                       /* :643-651 */
+                Be[3 * m + 0][0] = B[m][0];
+                Be[3 * m + 1][1] = B[m][1];
+                Be[3 * m + 2][2] = B[m][2];
+                Be[3 * m + 1][3] = B[m][2]; Be[3 * m + 2][3] = B[m][1];
+                Be[3 * m + 0][4] = B[m][2]; Be[3 * m + 2][4] = B[m][0];
+                Be[3 * m + 0][5] = B[m][1]; Be[3 * m + 1][5] = B[m][0];
+            }
+            for (int i = 0; i < 12; i++)
+                for (int j = 0; j < 6; j++) {
+                    double s = 0.0;
+                    for (int l = 0; l < 6; l++) s += Be[i][l] * De[l][j];
+                    BeD[i][j] = s;
+                }
+            for (int i = 0; i < 12; i++)                                        /* :654-663 */
+                for (int j = 0; j < 12; j++) {
+                    double s = 0.0;
+                    for (int l = 0; l < 6; l++) s += BeD[i][l] * Be[j][l];
+                    Ke[i * 12 + j] += s * detJ * TET_W;
+                }
+            for (int m = 0; m < 4; m++)                                         /* :683-685 */
+                for (int c = 0; c < 3; c++) fe[3 * m + c] += gravity[c] * rho[e] * N[m] * detJ * TET_W;
+        }
+        /* assemblyMatrix, static branch :1710-1714, lifting :1720-1739 */
+        for (int m = 0; m < 4; m++)
+            for (int c = 0; c < 3; c++) {
+                uint32_t dof = 3 * nd[m] + c;
+                if (book[dof] != OR_EOD) continue;
+                double u = u_node[dof];
+                for (int n = 0; n < 12; n++) fe[n] -= Ke[n * 12 + 3 * m + c] * u;
+            }
+#pragma omp critical
+        stress_scatter(A, b, book, nd, 4, Ke, fe, 1);
+    }
+    return 0;
+}

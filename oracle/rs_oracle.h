@@ -1,0 +1,118 @@
+/*
+ * rs_oracle.h -- CPU restatement of Range's element-assembly + sparse-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle: tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only as the checker or as the timed CPU
+ * baseline.  The product (range-solver-lib_amd/) never links, loads or calls it.
+ *
+ * Every function restates a reference function; the citation names the file:line it
+ * follows (paths relative to the Range-Software/range-solver-lib tree).  Plain C11 +
+ * OpenMP, mirroring the reference's own OpenMP loops so the timing is representative.
+ *
+ * Parity pins (tests/test_oracle.py): tst_matrix_solver.cpp vectors, direct sparse solves,
+ * CG finite-termination on k distinct eigenvalues, closed-form tet matrices computed by a
+ * different formulation, K*1=0, rigid-body kernels, patch tests.  Functions with no pin:
+ * RConstants::eps (value chosen here, OR_EPS), RRMatrix::invert on singular blocks.
+ */
+#ifndef RS_ORACLE_H
+#define RS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_EOD 0xFFFFFFFFu
+/* RConstants::eps lives in range-base-lib (absent): value chosen here. */
+#define OR_EPS 1.0e-14
+
+enum { OR_PRECOND_NONE = 0, OR_PRECOND_JACOBI = 1, OR_PRECOND_BLOCK_JACOBI = 2 };
+enum { OR_SOLVER_CG = 0, OR_SOLVER_GMRES = 1 };
+enum { OR_STOP_REFERENCE = 0, OR_STOP_RELRES = 1 };
+enum { OR_REASON_RUNNING = 0, OR_REASON_CONVERGED = 1, OR_REASON_STAGNATED = 2,
+       OR_REASON_NONNORMAL = 3, OR_REASON_MAXIT = 4 };
+
+typedef struct or_conf {
+    int solver_type;          /* OR_SOLVER_* */
+    unsigned n_outer;         /* RMatrixSolverConf::getNOuterIterations */
+    unsigned n_inner;         /* RMatrixSolverConf::getNInnerIterations (GMRES restart) */
+    double cvg;               /* RMatrixSolverConf::getSolverCvgValue */
+    int stop_rule;            /* OR_STOP_REFERENCE: rn/(An*xn+bn); OR_STOP_RELRES: rn/bn */
+} or_conf;
+
+typedef struct or_info {
+    unsigned iterations;      /* iterations performed (CG) / outer iterations (GMRES) */
+    unsigned inner_total;     /* GMRES: total inner steps */
+    double error, trend;      /* last RIterationInfo error / trend */
+    double equation_scale, a_norm, b_norm;
+    int reason;
+} or_info;
+
+/* RIterationInfo::hasConverged, src/riterationinfo.cpp:90-109 */
+int or_has_converged(double error, double trend, double cvg);
+int or_converged_reason(double error, double trend, double cvg);
+
+/* RMatrixPreconditioner ctor + construct*, src/rmatrixpreconditioner.cpp:16-148.
+ * data must hold n*block_size doubles; *width_out receives the effective block width. */
+int or_precond_build(unsigned n, const uint32_t *rowptr, const uint32_t *col, const double *val,
+                     int type, unsigned block_size, double *data, unsigned *width_out);
+/* RMatrixPreconditioner::compute, src/rmatrixpreconditioner.cpp:51-76,150-190 */
+void or_precond_apply(int type, unsigned n, unsigned width, const double *data,
+                      const double *x, double *y);
+
+/* ||A||_F over stored entries, src/rmatrixsolver.cpp:37-51 */
+double or_frobenius(unsigned n, const uint32_t *rowptr, const double *val);
+
+/* RMatrixSolver::solve + solveCG / solveGMRES, src/rmatrixsolver.cpp:158-581.
+ * x is in/out (initial guess).  prev_error seeds RIterationInfo's stored error (0 for a
+ * fresh solver).  history (nullable) receives (error, trend) per iteration. */
+int or_solve(unsigned n, const uint32_t *rowptr, const uint32_t *col, const double *val,
+             const double *b, double *x, int precond_type, unsigned block_size,
+             const or_conf *conf, double prev_error, or_info *info, double *history,
+             unsigned history_cap);
+
+/* y = A x, src/rmatrixsolver.cpp:111-119 */
+void or_spmv(unsigned n, const uint32_t *rowptr, const uint32_t *col, const double *val,
+             const double *x, double *y);
+
+/* ---- DOF numbering (node book) ---------------------------------------------------- */
+/* RSolverGeneric::generateNodeBook + rebuildNodeBook, src/rsolvergeneric.cpp:498-582,
+ * and RSolverStress::generateNodeBook, src/rsolverstress.cpp:1564-1652.
+ * fixed[nn*dpn] marks disabled positions; positions of nodes touched by no element are
+ * disabled too.  book[nn*dpn] receives positions or OR_EOD.  Returns the enabled count. */
+uint32_t or_node_book(uint32_t nn, uint32_t ne, uint32_t npe, const uint32_t *conn,
+                      unsigned dpn, const uint8_t *fixed, uint32_t *book);
+
+/* ---- Assembly into a row-wise sparse matrix (RSparseMatrix::addValue semantics) ----- */
+typedef struct or_sparse or_sparse;
+or_sparse *or_sparse_create(uint32_t n_rows);
+void or_sparse_destroy(or_sparse *A);
+uint64_t or_sparse_nnz(const or_sparse *A);
+/* Export as CSR with sorted columns; arrays sized n+1 / nnz / nnz. */
+void or_sparse_to_csr(const or_sparse *A, uint32_t *rowptr, uint32_t *col, double *val);
+
+/* RSolverHeat::prepare volume loop + assemblyMatrix, src/rsolverheat.cpp:361-455, 755-820.
+ * rhoc (density*capacity per element) is used only when transient != 0.
+ * t_node: nodeTemperature (Dirichlet values at fixed nodes; T_old when transient).
+ * b must hold n_dof doubles (zeroed here). */
+int or_heat_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                     const double *k, const double *q, const double *rhoc,
+                     const double *t_node, const uint32_t *book, uint32_t n_dof,
+                     int transient, double alpha, double dt, or_sparse *A, double *b);
+
+/* RSolverStress::prepare volume loop + surface traction + assemblyMatrix,
+ * src/rsolverstress.cpp:573-715, 384-571, 1654-1772 (static, no local rotations).
+ * book: 3*nn positions; u_node: 3*nn prescribed displacements (x,y,z interleaved);
+ * gravity: 3 doubles (elementGravity, with rho); tris/traction: surface triangles and the
+ * force per unit area on each (3 doubles per triangle). */
+int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                       const double *E, const double *nu, const double *rho,
+                       const double *gravity, const double *u_node,
+                       uint32_t ntri, const uint32_t *tris, const double *traction,
+                       const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

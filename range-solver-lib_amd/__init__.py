@@ -1,0 +1,498 @@
+"""range_solver_lib_amd -- Python binding of the MI355X element-assembly + sparse-solve library.
+
+A thin ctypes layer over the C ABI declared in include/range_solver_amd.h (the product is the
+HIP/C++ library librangesolver_amd.so beside this file).  The classes mirror the reference's
+call surface (Range-Software/range-solver-lib):
+
+    RMatrixSolver(conf).solve(A, b, x, precond, block)   include/rmatrixsolver.h:27-42
+    RMatrixPreconditioner(A, type, block).compute(x)     include/rmatrixpreconditioner.h:36-48
+    SparseMatrix (the CSR cache of src/rmatrixsolver.cpp:14-109, resident in HBM)
+    Assembly (RSolverHeat/RSolverStress prepare + assemblyMatrix on the GPU)
+
+Loading fails loudly when the shared library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librangesolver_amd.so")
+
+RS_EOD = 0xFFFFFFFF
+
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2, 3
+SOLVER_CG, SOLVER_GMRES = 0, 1
+STOP_REFERENCE, STOP_RELRES = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+REASON_NAMES = {0: "running", 1: "converged", 2: "stagnated", 3: "nonnormal", 4: "maxit"}
+
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "UNKNOWN_SOLVER", 3: "INVALID_PRECOND",
+                4: "DEVICE", 5: "NO_DEVICE", 6: "OUT_OF_MEMORY", 7: "NOT_INITIALIZED",
+                8: "UNSUPPORTED"}
+
+
+class RSError(RuntimeError):
+    """Raised for a non-zero rs_status (the reference throws RError in the same cases)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+class Conf(C.Structure):
+    _fields_ = [("solver_type", C.c_int32), ("n_outer_iterations", C.c_uint32),
+                ("n_inner_iterations", C.c_uint32), ("output_frequency", C.c_uint32),
+                ("solver_cvg_value", C.c_double), ("stop_rule", C.c_int32),
+                ("precision", C.c_int32), ("output_file_name", C.c_char_p)]
+
+
+class Info(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("inner_iterations", C.c_uint32),
+                ("reason", C.c_int32), ("converged", C.c_int32), ("error", C.c_double),
+                ("trend", C.c_double), ("equation_scale", C.c_double), ("a_norm", C.c_double),
+                ("b_norm", C.c_double), ("true_relres", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["reason_name"] = REASON_NAMES.get(self.reason, str(self.reason))
+        return d
+
+
+_P = C.c_void_p
+_u32p = C.POINTER(C.c_uint32)
+_f64p = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+
+# name -> (argtypes); every entry point returns int (rs_status) except rs_last_error.
+_SIGS = {
+    "rs_init": [C.c_int], "rs_finalize": [], "rs_device_count": [C.POINTER(C.c_int)],
+    "rs_version": [C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "rs_dmalloc": [C.POINTER(_P), C.c_size_t], "rs_dfree": [_P],
+    "rs_memcpy_htod": [_P, _P, C.c_size_t], "rs_memcpy_dtoh": [_P, _P, C.c_size_t],
+    "rs_memcpy_dtod": [_P, _P, C.c_size_t], "rs_synchronize": [],
+    "rs_matrix_create": [C.c_uint32, _P, _P, _P, C.POINTER(_P)],
+    "rs_matrix_update_values": [_P, _P], "rs_matrix_update_values_device": [_P, _P],
+    "rs_matrix_info": [_P, _u32p, C.POINTER(C.c_uint64), _f64p],
+    "rs_matrix_download": [_P, _P, _P, _P], "rs_matrix_spmv": [_P, _P, _P],
+    "rs_matrix_spmv_device": [_P, _P, _P], "rs_matrix_destroy": [_P],
+    "rs_matrix_spmv_timed": [_P, C.c_int32, C.c_uint32, C.c_uint32, _f64p],
+    "rs_precond_create": [_P, C.c_int32, C.c_uint32, C.POINTER(_P)],
+    "rs_precond_compute": [_P, _P, _P], "rs_precond_width": [_P, _u32p],
+    "rs_precond_download": [_P, _P], "rs_precond_destroy": [_P],
+    "rs_solver_create": [C.POINTER(Conf), C.POINTER(_P)], "rs_solver_copy": [_P, C.POINTER(_P)],
+    "rs_solver_disable_convergence_log": [_P],
+    "rs_solver_solve": [_P, _P, _P, _P, C.c_int32, C.c_uint32, C.POINTER(Info)],
+    "rs_solver_solve_device": [_P, _P, _P, _P, C.c_int32, C.c_uint32, C.POINTER(Info)],
+    "rs_solver_history": [_P, _P, C.c_uint32, _u32p], "rs_solver_destroy": [_P],
+    "rs_mesh_kuhn_box": [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, _P, _P],
+    "rs_mesh_plane_faces": [C.c_uint32, _P, C.c_uint32, _P, C.c_int32, C.c_double, C.c_double, _P,
+                            C.c_uint64, C.POINTER(C.c_uint64)],
+    "rs_field_uniform": [C.c_uint64, C.c_uint64, _P],
+    "rs_field_lognormal": [C.c_uint64, C.c_uint64, C.c_double, C.c_double, _P],
+    "rs_node_book": [C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P, _u32p],
+    "rs_assembly_create": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P, C.POINTER(_P)],
+    "rs_assembly_info": [_P, _u32p, C.POINTER(C.c_uint64), _u32p],
+    "rs_assembly_book": [_P, _P], "rs_assembly_last_ms": [_P, _f64p],
+    "rs_assembly_matrix": [_P, C.POINTER(_P)],
+    "rs_assemble_heat": [_P, _P, _P, _P, _P, C.c_int32, C.c_double, C.c_double, _P],
+    "rs_assemble_heat_device": [_P, _P, _P, _P, _P, C.c_int32, C.c_double, C.c_double, _P],
+    "rs_assemble_stress": [_P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P],
+    "rs_assembly_dofs_to_nodes": [_P, _P, _P], "rs_assembly_destroy": [_P],
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load librangesolver_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `make -C range-solver-lib_amd` "
+                          "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(path)
+    for name, argtypes in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = C.c_int
+    lib.rs_last_error.argtypes = []
+    lib.rs_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return sorted(list(_SIGS) + ["rs_last_error"])
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = _lib.rs_last_error().decode(errors="replace")
+        raise RSError(rc, msg)
+
+
+def _ptr(a: np.ndarray | None) -> C.c_void_p | None:
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays crossing the C ABI must be contiguous"
+    return C.c_void_p(a.ctypes.data)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ------------------------------------------------------------------------------ runtime
+def init(device: int = 0) -> None:
+    load()
+    _check(_lib.rs_init(device))
+
+
+def device_count() -> int:
+    load()
+    n = C.c_int(0)
+    _check(_lib.rs_device_count(C.byref(n)))
+    return n.value
+
+
+class DeviceArray:
+    """A plain device allocation (rs_dmalloc) holding a 1-D numpy-typed array."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        load()
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        _check(_lib.rs_dmalloc(C.byref(self.ptr), max(1, self.n * self.dtype.itemsize)))
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _check(_lib.rs_memcpy_htod(self.ptr, _ptr(a), a.nbytes))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(_lib.rs_memcpy_dtoh(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "ptr", None) is not None and self.ptr.value:
+            _lib.rs_dfree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def synchronize() -> None:
+    _check(_lib.rs_synchronize())
+
+
+# ------------------------------------------------------------------------------ matrix
+class SparseMatrix:
+    """Device CSR with ||A||_F (SparseMatrixCache, src/rmatrixsolver.cpp:14-54)."""
+
+    def __init__(self, n: int, rowptr, col, val, _handle=None, _owner=None):
+        load()
+        self._owner = _owner
+        if _handle is not None:
+            self.h = _handle
+            self._borrowed = True
+        else:
+            self._borrowed = False
+            self.rowptr, self.col, self.val = _u32(rowptr), _u32(col), _f64(val)
+            self.h = C.c_void_p()
+            _check(_lib.rs_matrix_create(n, _ptr(self.rowptr), _ptr(self.col), _ptr(self.val),
+                                         C.byref(self.h)))
+
+    @classmethod
+    def from_scipy(cls, m) -> "SparseMatrix":
+        m = m.tocsr()
+        return cls(m.shape[0], m.indptr, m.indices, m.data)
+
+    def info(self):
+        n, nnz, fro = C.c_uint32(), C.c_uint64(), C.c_double()
+        _check(_lib.rs_matrix_info(self.h, C.byref(n), C.byref(nnz), C.byref(fro)))
+        return n.value, nnz.value, fro.value
+
+    @property
+    def n(self) -> int:
+        return self.info()[0]
+
+    @property
+    def frobenius(self) -> float:
+        return self.info()[2]
+
+    def update_values(self, val) -> None:
+        v = _f64(val)
+        _check(_lib.rs_matrix_update_values(self.h, _ptr(v)))
+
+    def download(self):
+        n, nnz, _ = self.info()
+        rp = np.empty(n + 1, np.uint32)
+        col = np.empty(nnz, np.uint32)
+        val = np.empty(nnz, np.float64)
+        _check(_lib.rs_matrix_download(self.h, _ptr(rp), _ptr(col), _ptr(val)))
+        return rp, col, val
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp, col, val = self.download()
+        n = rp.size - 1
+        return sp.csr_matrix((val, col, rp), shape=(n, n))
+
+    def spmv(self, x) -> np.ndarray:
+        x = _f64(x)
+        y = np.empty_like(x)
+        _check(_lib.rs_matrix_spmv(self.h, _ptr(x), _ptr(y)))
+        return y
+
+    def spmv_device(self, dx: DeviceArray, dy: DeviceArray) -> None:
+        _check(_lib.rs_matrix_spmv_device(self.h, dx.ptr, dy.ptr))
+
+    def spmv_timed(self, precision: int = PRECISION_FP64, reps: int = 50, lanes: int = 0) -> float:
+        """Average device milliseconds of one SpMV launch (stream events)."""
+        ms = C.c_double()
+        _check(_lib.rs_matrix_spmv_timed(self.h, precision, reps, lanes, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if _lib is not None and not getattr(self, "_borrowed", True) and getattr(self, "h", None):
+            _lib.rs_matrix_destroy(self.h)
+            self.h = None
+
+
+class RMatrixPreconditioner:
+    """RMatrixPreconditioner(A, type, blockSize).compute(x) -- include/rmatrixpreconditioner.h:36-48."""
+
+    def __init__(self, A: SparseMatrix, ptype: int = PRECOND_NONE, block_size: int = 1):
+        self.A = A
+        self.h = C.c_void_p()
+        _check(_lib.rs_precond_create(A.h, ptype, block_size, C.byref(self.h)))
+
+    @property
+    def width(self) -> int:
+        w = C.c_uint32()
+        _check(_lib.rs_precond_width(self.h, C.byref(w)))
+        return w.value
+
+    def data(self) -> np.ndarray:
+        out = np.empty(self.A.n * max(1, self.width), np.float64)
+        _check(_lib.rs_precond_download(self.h, _ptr(out)))
+        return out
+
+    def compute(self, x) -> np.ndarray:
+        x = _f64(x)
+        y = np.empty_like(x)
+        _check(_lib.rs_precond_compute(self.h, _ptr(x), _ptr(y)))
+        return y
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.rs_precond_destroy(self.h)
+            self.h = None
+
+
+@dataclass
+class MatrixSolverConf:
+    """RMatrixSolverConf as used by the reference (range-model-lib)."""
+    solver_type: int = SOLVER_CG
+    n_outer_iterations: int = 1000
+    n_inner_iterations: int = 1000
+    solver_cvg_value: float = 1e-13
+    output_frequency: int = 0
+    output_file_name: str | None = None
+    stop_rule: int = STOP_REFERENCE
+    precision: int = PRECISION_FP64
+
+    def to_c(self) -> Conf:
+        return Conf(self.solver_type, self.n_outer_iterations, self.n_inner_iterations,
+                    self.output_frequency, self.solver_cvg_value, self.stop_rule, self.precision,
+                    self.output_file_name.encode() if self.output_file_name else None)
+
+
+class RMatrixSolver:
+    """RMatrixSolver(conf).solve(A, b, x, precond, blockSize) -- include/rmatrixsolver.h:27-42."""
+
+    def __init__(self, conf: MatrixSolverConf):
+        load()
+        self.conf = conf
+        c = conf.to_c()
+        self.h = C.c_void_p()
+        _check(_lib.rs_solver_create(C.byref(c), C.byref(self.h)))
+        self.info: Info | None = None
+
+    def copy(self) -> "RMatrixSolver":
+        other = RMatrixSolver.__new__(RMatrixSolver)
+        other.conf = self.conf
+        other.info = None
+        other.h = C.c_void_p()
+        _check(_lib.rs_solver_copy(self.h, C.byref(other.h)))
+        return other
+
+    def disable_convergence_log_file(self) -> None:
+        _check(_lib.rs_solver_disable_convergence_log(self.h))
+
+    def solve(self, A: SparseMatrix, b, x=None, precond: int = PRECOND_NONE,
+              block_size: int = 1) -> np.ndarray:
+        b = _f64(b)
+        n = A.n
+        # the reference resizes x to n, keeping the leading entries as the initial guess
+        x0 = np.zeros(n) if x is None else _f64(x)
+        if x0.size != n:
+            x0 = np.concatenate([x0[:n], np.zeros(max(0, n - x0.size))])
+        x0 = np.array(x0, dtype=np.float64, copy=True)
+        info = Info()
+        _check(_lib.rs_solver_solve(self.h, A.h, _ptr(b), _ptr(x0), precond, block_size,
+                                    C.byref(info)))
+        self.info = info
+        return x0
+
+    def solve_device(self, A: SparseMatrix, db: DeviceArray, dx: DeviceArray,
+                     precond: int = PRECOND_NONE, block_size: int = 1) -> Info:
+        info = Info()
+        _check(_lib.rs_solver_solve_device(self.h, A.h, db.ptr, dx.ptr, precond, block_size,
+                                           C.byref(info)))
+        self.info = info
+        return info
+
+    def history(self) -> np.ndarray:
+        cnt = C.c_uint32()
+        _check(_lib.rs_solver_history(self.h, None, 0, C.byref(cnt)))
+        out = np.empty(2 * max(1, cnt.value), np.float64)
+        _check(_lib.rs_solver_history(self.h, _ptr(out), cnt.value, C.byref(cnt)))
+        return out[: 2 * cnt.value].reshape(-1, 2)
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.rs_solver_destroy(self.h)
+            self.h = None
+
+
+# ------------------------------------------------------------------------------ model prep (host)
+def kuhn_box(nx: int, ny: int, nz: int, lx: float = 1.0, ly: float = 1.0, lz: float = 1.0):
+    load()
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    xyz = np.empty(3 * nn, np.float64)
+    tets = np.empty(24 * nx * ny * nz, np.uint32)
+    _check(_lib.rs_mesh_kuhn_box(nx, ny, nz, lx, ly, lz, _ptr(xyz), _ptr(tets)))
+    return xyz.reshape(-1, 3), tets.reshape(-1, 4)
+
+
+def plane_faces(xyz, tets, axis: int, value: float, tol: float = 1e-12) -> np.ndarray:
+    load()
+    xyz, tets = _f64(xyz), _u32(tets)
+    cnt = C.c_uint64()
+    _check(_lib.rs_mesh_plane_faces(xyz.size // 3, _ptr(xyz), tets.size // 4, _ptr(tets), axis,
+                                    value, tol, None, 0, C.byref(cnt)))
+    out = np.empty(3 * cnt.value, np.uint32)
+    _check(_lib.rs_mesh_plane_faces(xyz.size // 3, _ptr(xyz), tets.size // 4, _ptr(tets), axis,
+                                    value, tol, _ptr(out), cnt.value, C.byref(cnt)))
+    return out.reshape(-1, 3)
+
+
+def field_uniform(seed: int, n: int) -> np.ndarray:
+    load()
+    out = np.empty(n, np.float64)
+    _check(_lib.rs_field_uniform(seed, n, _ptr(out)))
+    return out
+
+
+def field_lognormal(seed: int, n: int, mu: float, sigma: float) -> np.ndarray:
+    load()
+    out = np.empty(n, np.float64)
+    _check(_lib.rs_field_lognormal(seed, n, mu, sigma, _ptr(out)))
+    return out
+
+
+def node_book(n_nodes: int, conn, dofs_per_node: int, fixed) -> tuple[np.ndarray, int]:
+    load()
+    conn = _u32(conn)
+    npe = conn.shape[1] if conn.ndim == 2 else 4
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    book = np.empty(n_nodes * dofs_per_node, np.uint32)
+    n = C.c_uint32()
+    _check(_lib.rs_node_book(n_nodes, conn.size // npe, npe, _ptr(conn), dofs_per_node,
+                             _ptr(fixed), _ptr(book), C.byref(n)))
+    return book, n.value
+
+
+# ------------------------------------------------------------------------------ assembly
+class Assembly:
+    """Device mesh + node book + CSR pattern + scatter map + element colouring."""
+
+    def __init__(self, xyz, tets, dofs_per_node: int, fixed):
+        load()
+        self.xyz = _f64(xyz).reshape(-1)
+        self.tets = _u32(tets).reshape(-1)
+        self.nn = self.xyz.size // 3
+        self.ne = self.tets.size // 4
+        self.dpn = dofs_per_node
+        fixed = np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
+        assert fixed.size == self.nn * dofs_per_node
+        self.h = C.c_void_p()
+        _check(_lib.rs_assembly_create(self.nn, _ptr(self.xyz), self.ne, _ptr(self.tets),
+                                       dofs_per_node, _ptr(fixed), C.byref(self.h)))
+        nd, nnz, nc = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        _check(_lib.rs_assembly_info(self.h, C.byref(nd), C.byref(nnz), C.byref(nc)))
+        self.n_dof, self.nnz, self.n_colours = nd.value, nnz.value, nc.value
+        mh = C.c_void_p()
+        _check(_lib.rs_assembly_matrix(self.h, C.byref(mh)))
+        self.matrix = SparseMatrix(0, None, None, None, _handle=mh, _owner=self)
+
+    def book(self) -> np.ndarray:
+        out = np.empty(self.nn * self.dpn, np.uint32)
+        _check(_lib.rs_assembly_book(self.h, _ptr(out)))
+        return out
+
+    @property
+    def last_ms(self) -> float:
+        v = C.c_double()
+        _check(_lib.rs_assembly_last_ms(self.h, C.byref(v)))
+        return v.value
+
+    def assemble_heat(self, k, q, t_node, rhoc=None, transient=False, alpha=1.0, dt=1.0):
+        k, q, t_node = _f64(k), _f64(q), _f64(t_node)
+        rc = _f64(rhoc) if rhoc is not None else None
+        b = np.empty(self.n_dof, np.float64)
+        _check(_lib.rs_assemble_heat(self.h, _ptr(k), _ptr(q), _ptr(rc), _ptr(t_node),
+                                     int(bool(transient)), alpha, dt, _ptr(b)))
+        return b
+
+    def assemble_heat_device(self, dk, dq, dt_node, db, drhoc=None, transient=False, alpha=1.0,
+                             dt=1.0):
+        _check(_lib.rs_assemble_heat_device(self.h, dk.ptr, dq.ptr,
+                                            drhoc.ptr if drhoc is not None else None,
+                                            dt_node.ptr, int(bool(transient)), alpha, dt, db.ptr))
+
+    def assemble_stress(self, E, nu, rho, gravity, u_node, tris=None, traction=None):
+        E, nu, rho, u_node = _f64(E), _f64(nu), _f64(rho), _f64(u_node)
+        g = _f64(gravity)
+        ntri = 0 if tris is None else int(np.asarray(tris).size // 3)
+        tr = _u32(tris) if ntri else None
+        tt = _f64(traction) if ntri else None
+        b = np.empty(self.n_dof, np.float64)
+        _check(_lib.rs_assemble_stress(self.h, _ptr(E), _ptr(nu), _ptr(rho), _ptr(g),
+                                       _ptr(u_node), ntri, _ptr(tr), _ptr(tt), _ptr(b)))
+        return b
+
+    def dofs_to_nodes(self, dx: DeviceArray, dnode: DeviceArray) -> None:
+        _check(_lib.rs_assembly_dofs_to_nodes(self.h, dx.ptr, dnode.ptr))
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.rs_assembly_destroy(self.h)
+            self.h = None

@@ -1,0 +1,573 @@
+// rs_assembly.hip -- deterministic graph-coloured tetrahedral assembly on gfx950.
+//
+// Reference behaviour restated (Range-Software/range-solver-lib):
+//   RSolverHeat::prepare volume loop    src/rsolverheat.cpp:361-455
+//   RSolverHeat::assemblyMatrix         src/rsolverheat.cpp:755-820
+//   RSolverStress::prepare volume loop  src/rsolverstress.cpp:573-715
+//   RSolverStress surface traction      src/rsolverstress.cpp:384-571 (force per unit area)
+//   RSolverStress::assemblyMatrix       src/rsolverstress.cpp:1654-1772 (static, no rotation)
+//   node write-back                     src/rsolverheat.cpp:473-480, rsolverstress.cpp:800-836
+//
+// The reference serialises its scatter with `omp critical` around RSparseMatrix::addValue.
+// Here the CSR pattern and an element -> CSR-position scatter map are built once; elements
+// are sorted by a node-disjoint colour, and each colour is one launch whose threads own
+// disjoint rows, so they add into the CSR values and b without atomics and in a fixed
+// order (bit-reproducible from run to run).
+//
+// Element matrices use the closed forms of the linear tetrahedron (constant gradients):
+//   heat    Ke = k V grad(N_m).grad(N_n), fe = Q V / 4, Me = rho c V (1 + delta_mn) / 20
+//   stress  K_mn[i][j] = V (lambda g_mi g_nj + mu g_mj g_ni + mu delta_ij g_m.g_n)
+// which equal the reference's integration-point sums (B De B^T detJ w) exactly for linear
+// tets.  Stress uses 16 lanes per element (one 3x3 node block each) with the element's
+// vertex coordinates staged in LDS.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "rs_device.hpp"
+#include "rs_internal.hpp"
+
+namespace rs {
+
+struct HeatArgs {
+    const double *xyz;
+    const uint4 *conn;          // colour-sorted elements
+    const uint32_t *perm;       // colour-sorted index -> original element id
+    const uint32_t *map;        // 16 CSR positions per element (RS_EOD when row/col fixed)
+    const uint32_t *book;       // node -> dof
+    const double *k, *q, *rhoc; // per original element
+    const double *t_node;       // nodeTemperature
+    double *val, *b;
+    int transient;
+    double alpha, dt;
+};
+
+__global__ void k_zero(double *__restrict__ a, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) a[i] = 0.0;
+}
+
+// Gradients of the 4 linear shape functions and the volume of tet (X0..X3).
+__device__ __forceinline__ double tet_gradients(const double (&X)[4][3], double (&g)[4][3])
+{
+    double a[3], b[3], c[3];
+    for (int i = 0; i < 3; i++) {
+        a[i] = X[1][i] - X[0][i];
+        b[i] = X[2][i] - X[0][i];
+        c[i] = X[3][i] - X[0][i];
+    }
+    // J = [a b c] (columns); rows of J^-1 are grad N1..N3: (b x c, c x a, a x b) / det
+    double bc[3] = { b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0] };
+    double ca[3] = { c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0] };
+    double ab[3] = { a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0] };
+    double det = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    double inv = 1.0 / det;
+    for (int i = 0; i < 3; i++) {
+        g[1][i] = bc[i] * inv;
+        g[2][i] = ca[i] * inv;
+        g[3][i] = ab[i] * inv;
+        g[0][i] = -(g[1][i] + g[2][i] + g[3][i]);
+    }
+    return det / 6.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_heat_colour(HeatArgs a, uint32_t e0, uint32_t e1)
+{
+    uint32_t e = e0 + blockIdx.x * kBlock + threadIdx.x;
+    if (e >= e1) return;
+    uint4 nd4 = a.conn[e];
+    uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+    uint32_t orig = a.perm[e];
+    double X[4][3], g[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) X[m][i] = a.xyz[3 * (uint64_t)nd[m] + i];
+    double V = tet_gradients(X, g);
+    double kV = a.k[orig] * V;
+    double fq = a.q[orig] * V * 0.25;
+    double Ae[4][4], be[4];
+    if (a.transient) {
+        double mc = a.rhoc[orig] * V / 20.0;
+        double T[4];
+#pragma unroll
+        for (int n = 0; n < 4; n++) T[n] = a.t_node[nd[n]];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            be[m] = a.dt * fq;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                double K = kV * (g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]);
+                double M = mc * (m == n ? 2.0 : 1.0);
+                Ae[m][n] = M + a.alpha * a.dt * K;
+                be[m] += (M - (1.0 - a.alpha) * a.dt * K) * T[n];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            be[m] = fq;
+#pragma unroll
+            for (int n = 0; n < 4; n++) Ae[m][n] = kV * (g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]);
+        }
+    }
+    uint32_t dof[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) dof[m] = a.book[nd[m]];
+    // Dirichlet lifting (src/rsolverheat.cpp:787-798)
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (dof[m] != RS_EOD) continue;
+        double tm = a.t_node[nd[m]];
+#pragma unroll
+        for (int n = 0; n < 4; n++) be[n] -= Ae[n][m] * tm;
+    }
+    const uint4 *mp = reinterpret_cast<const uint4 *>(a.map + 16ull * e);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (dof[m] == RS_EOD) continue;
+        a.b[dof[m]] += be[m];
+        uint4 pos = mp[m];
+        uint32_t pp[4] = { pos.x, pos.y, pos.z, pos.w };
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+            if (pp[n] != RS_EOD) a.val[pp[n]] += Ae[m][n];
+    }
+}
+
+struct StressArgs {
+    const double *xyz;
+    const uint4 *conn;
+    const uint32_t *perm;
+    const uint32_t *map;        // 144 positions per element, [(4m+n)*9 + 3i+j]
+    const uint32_t *book;       // 3*node+c -> dof
+    const double *E, *nu, *rho;
+    double gx, gy, gz;
+    const double *u_node;       // prescribed displacements (3 per node)
+    double *val, *b;
+};
+
+// 16 lanes per element: lane (m, n) assembles the 3x3 block coupling node m to node n.
+__global__ __launch_bounds__(kBlock) void k_stress_colour(StressArgs a, uint32_t e0, uint32_t e1)
+{
+    constexpr int EPB = kBlock / 16;  // elements per workgroup
+    __shared__ double sX[EPB][12];
+    const int le = threadIdx.x / 16, lane = threadIdx.x % 16;
+    const int m = lane / 4, n = lane % 4;
+    uint32_t e = e0 + blockIdx.x * EPB + le;
+    bool live = e < e1;
+    uint4 nd4 = live ? a.conn[e] : make_uint4(0, 0, 0, 0);
+    uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+    if (live && lane < 12) sX[le][lane] = a.xyz[3 * (uint64_t)nd[lane / 3] + lane % 3];
+    __syncthreads();
+    if (!live) return;
+    double X[4][3], g[4][3];
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) X[p][i] = sX[le][3 * p + i];
+    double V = tet_gradients(X, g);
+    uint32_t orig = a.perm[e];
+    double E = a.E[orig], v = a.nu[orig];
+    double lam = E * v / ((1.0 + v) * (1.0 - 2.0 * v));
+    double mu = E / (2.0 * (1.0 + v));
+    double gmn = g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2];
+    double K[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            K[i][j] = V * (lam * g[m][i] * g[n][j] + mu * g[m][j] * g[n][i] + (i == j ? mu * gmn : 0.0));
+    // Dirichlet lifting of node n's prescribed components into rows of node m
+    double lift[3] = { 0.0, 0.0, 0.0 };
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (a.book[3 * (uint64_t)nd[n] + j] != RS_EOD) continue;
+        double u = a.u_node[3 * (uint64_t)nd[n] + j];
+#pragma unroll
+        for (int i = 0; i < 3; i++) lift[i] += K[i][j] * u;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) lift[i] = group_sum<4>(lift[i]);
+    const uint32_t *mp = a.map + 144ull * e + lane * 9;
+    double w = a.rho[orig] * V * 0.25;
+    double grav[3] = { a.gx * w, a.gy * w, a.gz * w };
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        uint32_t row = a.book[3 * (uint64_t)nd[m] + i];
+        if (row == RS_EOD) continue;
+        if (n == 0) a.b[row] += grav[i] - lift[i];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            uint32_t pos = mp[3 * i + j];
+            if (pos != RS_EOD) a.val[pos] += K[i][j];
+        }
+    }
+}
+
+// Surface force per unit area on linear triangles: each node gets t * area / 3
+// (src/rsolverstress.cpp:498-507 with N integrated exactly).
+__global__ void k_traction_colour(const double *__restrict__ xyz, const uint32_t *__restrict__ tris,
+                                  const double *__restrict__ t, const uint32_t *__restrict__ book,
+                                  double *__restrict__ b, uint32_t f0, uint32_t f1)
+{
+    uint32_t f = f0 + blockIdx.x * kBlock + threadIdx.x;
+    if (f >= f1) return;
+    uint32_t nd[3] = { tris[3 * f], tris[3 * f + 1], tris[3 * f + 2] };
+    double e1[3], e2[3];
+    for (int i = 0; i < 3; i++) {
+        e1[i] = xyz[3 * (uint64_t)nd[1] + i] - xyz[3 * (uint64_t)nd[0] + i];
+        e2[i] = xyz[3 * (uint64_t)nd[2] + i] - xyz[3 * (uint64_t)nd[0] + i];
+    }
+    double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+    double area3 = 0.5 * sqrt(cx * cx + cy * cy + cz * cz) / 3.0;
+    for (int m = 0; m < 3; m++)
+        for (int c = 0; c < 3; c++) {
+            uint32_t row = book[3 * (uint64_t)nd[m] + c];
+            if (row != RS_EOD) b[row] += t[3 * f + c] * area3;
+        }
+}
+
+// node[i*dpn+c] = x[book[i*dpn+c]] for enabled positions (src/rsolverheat.cpp:473-480)
+__global__ void k_dofs_to_nodes(const uint32_t *__restrict__ book, uint64_t npos, const double *__restrict__ x,
+                                double *__restrict__ node)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < npos; i += (uint64_t)gridDim.x * kBlock) {
+        uint32_t d = book[i];
+        if (d != RS_EOD) node[i] = x[d];
+    }
+}
+
+__global__ void k_gather(const double *__restrict__ src, const uint32_t *__restrict__ perm, double *__restrict__ dst,
+                         uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        dst[i] = src[perm[i]];
+}
+
+static uint32_t grid_for(uint64_t items, uint32_t per_block)
+{
+    return (uint32_t)std::max<uint64_t>(1, (items + per_block - 1) / per_block);
+}
+
+static uint32_t vblocks(uint64_t n)
+{
+    uint64_t want = (n + kBlock - 1) / kBlock;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)std::max(8, runtime().n_cu * 8)));
+}
+
+}  // namespace rs
+
+using namespace rs;
+
+struct rs_assembly {
+    uint32_t n_nodes = 0, n_elems = 0, dpn = 1, n_dof = 0, n_colours = 0;
+    std::vector<uint32_t> book;           // host copy
+    std::vector<uint32_t> colour_off;     // n_colours+1
+    DBuf<double> xyz;
+    DBuf<uint32_t> conn, perm, map, dbook;
+    rs_matrix A;                          // borrowed by callers
+    DBuf<double> fields, t_node, bvec;    // staging for the host-array entry points
+    DBuf<uint32_t> tris;
+    DBuf<double> trac;
+    double last_ms = 0.0;
+};
+
+static int timed_begin(hipEvent_t (&ev)[2])
+{
+    RS_HIP(hipEventCreate(&ev[0]));
+    RS_HIP(hipEventCreate(&ev[1]));
+    RS_HIP(hipEventRecord(ev[0], runtime().stream));
+    return RS_OK;
+}
+
+static int timed_end(hipEvent_t (&ev)[2], double &ms)
+{
+    RS_HIP(hipEventRecord(ev[1], runtime().stream));
+    RS_HIP(hipEventSynchronize(ev[1]));
+    float f = 0.0f;
+    RS_HIP(hipEventElapsedTime(&f, ev[0], ev[1]));
+    ms = f;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    return RS_OK;
+}
+
+extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t n_tets, const uint32_t *tets,
+                                  uint32_t dofs_per_node, const uint8_t *fixed, rs_assembly **out)
+{
+    if (!xyz || !tets || !out || (dofs_per_node != 1 && dofs_per_node != 3))
+        return fail(RS_ERR_INVALID_ARG, "rs_assembly_create: bad arguments (dofs_per_node must be 1 or 3)");
+    for (uint64_t i = 0; i < 4ull * n_tets; i++)
+        if (tets[i] >= n_nodes) return fail(RS_ERR_INVALID_ARG, "rs_assembly_create: node id out of range");
+    int rc = require_runtime();
+    if (rc) return rc;
+    std::unique_ptr<rs_assembly> S(new rs_assembly());
+    S->n_nodes = n_nodes;
+    S->n_elems = n_tets;
+    S->dpn = dofs_per_node;
+    const uint32_t dpn = dofs_per_node;
+    S->book.resize((uint64_t)n_nodes * dpn);
+    if ((rc = rs_node_book(n_nodes, n_tets, 4, tets, dpn, fixed, S->book.data(), &S->n_dof))) return rc;
+    Pattern pat;
+    if ((rc = build_pattern(n_nodes, n_tets, 4, tets, dpn, S->book.data(), S->n_dof, pat))) return rc;
+    std::vector<uint16_t> colour;
+    if ((rc = colour_elements(n_nodes, n_tets, 4, tets, colour, S->n_colours))) return rc;
+    // counting sort of the elements by colour (stable: spatial order kept inside a colour)
+    S->colour_off.assign(S->n_colours + 1, 0);
+    for (uint32_t e = 0; e < n_tets; e++) S->colour_off[colour[e] + 1]++;
+    for (uint32_t c = 0; c < S->n_colours; c++) S->colour_off[c + 1] += S->colour_off[c];
+    std::vector<uint32_t> perm(n_tets), conn_s(4ull * n_tets);
+    {
+        std::vector<uint32_t> fill(S->colour_off.begin(), S->colour_off.end() - 1);
+        for (uint32_t e = 0; e < n_tets; e++) perm[fill[colour[e]]++] = e;
+    }
+    const uint32_t per = dpn == 1 ? 16 : 144;
+    std::vector<uint32_t> map((uint64_t)per * n_tets);
+    const uint32_t *rp = pat.rowptr.data(), *cl = pat.col.data();
+    const std::vector<uint32_t> &book = S->book;
+    parallel_for(n_tets, [&](uint64_t b, uint64_t e_end) {
+        for (uint64_t s = b; s < e_end; s++) {
+            const uint32_t *nd = tets + 4ull * perm[s];
+            std::memcpy(&conn_s[4 * s], nd, 16);
+            auto find = [&](uint32_t row, uint32_t colv) -> uint32_t {
+                const uint32_t *lo = cl + rp[row], *hi = cl + rp[row + 1];
+                const uint32_t *it = std::lower_bound(lo, hi, colv);
+                return (it != hi && *it == colv) ? (uint32_t)(it - cl) : RS_EOD;
+            };
+            if (dpn == 1) {
+                for (int m = 0; m < 4; m++)
+                    for (int n = 0; n < 4; n++) {
+                        uint32_t r = book[nd[m]], c = book[nd[n]];
+                        map[16 * s + 4 * m + n] = (r == RS_EOD || c == RS_EOD) ? RS_EOD : find(r, c);
+                    }
+            } else {
+                for (int m = 0; m < 4; m++)
+                    for (int n = 0; n < 4; n++)
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++) {
+                                uint32_t r = book[3ull * nd[m] + i], c = book[3ull * nd[n] + j];
+                                map[144 * s + (4 * m + n) * 9 + 3 * i + j] =
+                                    (r == RS_EOD || c == RS_EOD) ? RS_EOD : find(r, c);
+                            }
+            }
+        }
+    });
+    hipStream_t st = runtime().stream;
+    uint64_t nnz = pat.col.size();
+    if ((rc = S->xyz.alloc(3ull * n_nodes)) || (rc = S->conn.alloc(4ull * n_tets)) || (rc = S->perm.alloc(n_tets)) ||
+        (rc = S->map.alloc(map.size())) || (rc = S->dbook.alloc(S->book.size())))
+        return rc;
+    rs_matrix &A = S->A;
+    A.n = S->n_dof;
+    A.nnz = nnz;
+    A.borrowed = true;
+    if ((rc = A.rowptr.alloc(S->n_dof + 1ull)) || (rc = A.col.alloc(nnz)) || (rc = A.val.alloc(nnz))) return rc;
+    RS_HIP(hipMemcpyAsync(S->xyz.p, xyz, sizeof(double) * 3 * n_nodes, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->conn.p, conn_s.data(), sizeof(uint32_t) * conn_s.size(), hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->perm.p, perm.data(), sizeof(uint32_t) * perm.size(), hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->map.p, map.data(), sizeof(uint32_t) * map.size(), hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->dbook.p, S->book.data(), sizeof(uint32_t) * S->book.size(), hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(A.rowptr.p, pat.rowptr.data(), sizeof(uint32_t) * pat.rowptr.size(), hipMemcpyHostToDevice, st));
+    if (nnz) {
+        RS_HIP(hipMemcpyAsync(A.col.p, pat.col.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
+        k_zero<<<vblocks(nnz), kBlock, 0, st>>>(A.val.p, nnz);
+    }
+    RS_HIP(hipStreamSynchronize(st));
+    *out = S.release();
+    return RS_OK;
+}
+
+extern "C" int rs_assembly_info(const rs_assembly *S, uint32_t *n_dof, uint64_t *nnz, uint32_t *n_colours)
+{
+    if (!S) return fail(RS_ERR_INVALID_ARG, "rs_assembly_info: null");
+    if (n_dof) *n_dof = S->n_dof;
+    if (nnz) *nnz = S->A.nnz;
+    if (n_colours) *n_colours = S->n_colours;
+    return RS_OK;
+}
+
+extern "C" int rs_assembly_last_ms(const rs_assembly *S, double *ms)
+{
+    if (!S || !ms) return fail(RS_ERR_INVALID_ARG, "rs_assembly_last_ms: null");
+    *ms = S->last_ms;
+    return RS_OK;
+}
+
+extern "C" int rs_assembly_book(const rs_assembly *S, uint32_t *book)
+{
+    if (!S || !book) return fail(RS_ERR_INVALID_ARG, "rs_assembly_book: null");
+    std::memcpy(book, S->book.data(), sizeof(uint32_t) * S->book.size());
+    return RS_OK;
+}
+
+extern "C" int rs_assembly_matrix(rs_assembly *S, rs_matrix **A)
+{
+    if (!S || !A) return fail(RS_ERR_INVALID_ARG, "rs_assembly_matrix: null");
+    *A = &S->A;
+    return RS_OK;
+}
+
+extern "C" int rs_assemble_heat_device(rs_assembly *S, const double *d_k, const double *d_q, const double *d_rhoc,
+                                       const double *d_t_node, int32_t transient, double alpha, double dt, double *d_b)
+{
+    if (!S || !d_k || !d_q || !d_t_node || !d_b || (transient && !d_rhoc))
+        return fail(RS_ERR_INVALID_ARG, "rs_assemble_heat_device: null argument");
+    if (S->dpn != 1) return fail(RS_ERR_INVALID_ARG, "rs_assemble_heat_device: assembly has 3 dofs per node");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    hipEvent_t ev[2];
+    if ((rc = timed_begin(ev))) return rc;
+    HeatArgs a;
+    a.xyz = S->xyz.p;
+    a.conn = reinterpret_cast<const uint4 *>(S->conn.p);
+    a.perm = S->perm.p;
+    a.map = S->map.p;
+    a.book = S->dbook.p;
+    a.k = d_k;
+    a.q = d_q;
+    a.rhoc = d_rhoc;
+    a.t_node = d_t_node;
+    a.val = S->A.val.p;
+    a.b = d_b;
+    a.transient = transient;
+    a.alpha = alpha;
+    a.dt = dt;
+    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+    if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
+    for (uint32_t c = 0; c < S->n_colours; c++) {
+        uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+        if (e1 > e0) k_heat_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+    }
+    RS_HIP(hipGetLastError());
+    if ((rc = timed_end(ev, S->last_ms))) return rc;
+    if ((rc = matrix_frobenius(S->A))) return rc;
+    S->A.val32_valid = false;
+    S->A.version++;
+    return RS_OK;
+}
+
+extern "C" int rs_assemble_heat(rs_assembly *S, const double *k, const double *q, const double *rhoc,
+                                const double *t_node, int32_t transient, double alpha, double dt, double *b)
+{
+    if (!S || !k || !q || !t_node || !b || (transient && !rhoc))
+        return fail(RS_ERR_INVALID_ARG, "rs_assemble_heat: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    uint64_t ne = S->n_elems;
+    if ((rc = S->fields.alloc(3 * ne + 1)) || (rc = S->t_node.alloc(S->n_nodes + 1ull)) || (rc = S->bvec.alloc(S->n_dof + 1ull)))
+        return rc;
+    RS_HIP(hipMemcpyAsync(S->fields.p, k, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->fields.p + ne, q, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    if (transient) RS_HIP(hipMemcpyAsync(S->fields.p + 2 * ne, rhoc, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->t_node.p, t_node, sizeof(double) * S->n_nodes, hipMemcpyHostToDevice, st));
+    rc = rs_assemble_heat_device(S, S->fields.p, S->fields.p + ne, S->fields.p + 2 * ne, S->t_node.p, transient, alpha,
+                                 dt, S->bvec.p);
+    if (rc) return rc;
+    RS_HIP(hipMemcpyAsync(b, S->bvec.p, sizeof(double) * S->n_dof, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
+
+extern "C" int rs_assemble_stress(rs_assembly *S, const double *E, const double *nu, const double *rho,
+                                  const double *gravity, const double *u_node, uint32_t n_tri, const uint32_t *tris,
+                                  const double *traction, double *b)
+{
+    if (!S || !E || !nu || !rho || !gravity || !u_node || !b || (n_tri && (!tris || !traction)))
+        return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress: null argument");
+    if (S->dpn != 3) return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress: assembly has 1 dof per node");
+    for (uint64_t i = 0; i < 3ull * n_tri; i++)
+        if (tris[i] >= S->n_nodes) return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress: triangle node out of range");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    uint64_t ne = S->n_elems;
+    if ((rc = S->fields.alloc(3 * ne + 1)) || (rc = S->t_node.alloc(3ull * S->n_nodes + 1)) ||
+        (rc = S->bvec.alloc(S->n_dof + 1ull)))
+        return rc;
+    RS_HIP(hipMemcpyAsync(S->fields.p, E, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->fields.p + ne, nu, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->fields.p + 2 * ne, rho, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->t_node.p, u_node, sizeof(double) * 3 * S->n_nodes, hipMemcpyHostToDevice, st));
+    // colour the traction triangles (node-disjoint) on the host
+    std::vector<uint16_t> fcol;
+    uint32_t nfc = 0;
+    std::vector<uint32_t> fsorted(3ull * n_tri), foff;
+    std::vector<double> tsorted(3ull * n_tri);
+    if (n_tri) {
+        if ((rc = colour_elements(S->n_nodes, n_tri, 3, tris, fcol, nfc))) return rc;
+        foff.assign(nfc + 1, 0);
+        for (uint32_t f = 0; f < n_tri; f++) foff[fcol[f] + 1]++;
+        for (uint32_t c = 0; c < nfc; c++) foff[c + 1] += foff[c];
+        std::vector<uint32_t> fill(foff.begin(), foff.end() - 1);
+        for (uint32_t f = 0; f < n_tri; f++) {
+            uint32_t s = fill[fcol[f]]++;
+            for (int i = 0; i < 3; i++) {
+                fsorted[3 * s + i] = tris[3 * f + i];
+                tsorted[3 * s + i] = traction[3 * f + i];
+            }
+        }
+        if ((rc = S->tris.alloc(3ull * n_tri)) || (rc = S->trac.alloc(3ull * n_tri))) return rc;
+        RS_HIP(hipMemcpyAsync(S->tris.p, fsorted.data(), sizeof(uint32_t) * 3 * n_tri, hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(S->trac.p, tsorted.data(), sizeof(double) * 3 * n_tri, hipMemcpyHostToDevice, st));
+    }
+    hipEvent_t ev[2];
+    if ((rc = timed_begin(ev))) return rc;
+    StressArgs a;
+    a.xyz = S->xyz.p;
+    a.conn = reinterpret_cast<const uint4 *>(S->conn.p);
+    a.perm = S->perm.p;
+    a.map = S->map.p;
+    a.book = S->dbook.p;
+    a.E = S->fields.p;
+    a.nu = S->fields.p + ne;
+    a.rho = S->fields.p + 2 * ne;
+    a.gx = gravity[0];
+    a.gy = gravity[1];
+    a.gz = gravity[2];
+    a.u_node = S->t_node.p;
+    a.val = S->A.val.p;
+    a.b = S->bvec.p;
+    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+    if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(S->bvec.p, S->n_dof);
+    // surfaces first, then volumes, as the reference's prepare() (src/rsolverstress.cpp:384, 573)
+    for (uint32_t c = 0; c < nfc; c++) {
+        uint32_t f0 = foff[c], f1 = foff[c + 1];
+        if (f1 > f0)
+            k_traction_colour<<<grid_for(f1 - f0, kBlock), kBlock, 0, st>>>(S->xyz.p, S->tris.p, S->trac.p, S->dbook.p,
+                                                                            S->bvec.p, f0, f1);
+    }
+    constexpr uint32_t EPB = kBlock / 16;
+    for (uint32_t c = 0; c < S->n_colours; c++) {
+        uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+        if (e1 > e0) k_stress_colour<<<grid_for(e1 - e0, EPB), kBlock, 0, st>>>(a, e0, e1);
+    }
+    RS_HIP(hipGetLastError());
+    if ((rc = timed_end(ev, S->last_ms))) return rc;
+    if ((rc = matrix_frobenius(S->A))) return rc;
+    S->A.val32_valid = false;
+    S->A.version++;
+    RS_HIP(hipMemcpyAsync(b, S->bvec.p, sizeof(double) * S->n_dof, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
+
+extern "C" int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node)
+{
+    if (!S || !d_x || !d_node) return fail(RS_ERR_INVALID_ARG, "rs_assembly_dofs_to_nodes: null");
+    int rc = require_runtime();
+    if (rc) return rc;
+    uint64_t npos = (uint64_t)S->n_nodes * S->dpn;
+    k_dofs_to_nodes<<<vblocks(npos), kBlock, 0, runtime().stream>>>(S->dbook.p, npos, d_x, d_node);
+    RS_HIP(hipGetLastError());
+    return RS_OK;
+}
+
+extern "C" int rs_assembly_destroy(rs_assembly *S)
+{
+    delete S;
+    return RS_OK;
+}

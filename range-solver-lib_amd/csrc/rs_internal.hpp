@@ -1,0 +1,112 @@
+// rs_internal.hpp -- shared internal declarations of the range-solver MI355X library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "range_solver_amd.h"
+
+namespace rs {
+
+// ----------------------------------------------------------------- errors / runtime
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+#define RS_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t _e = (call);                                             \
+        if (_e != hipSuccess) return ::rs::hip_fail(_e, #call);             \
+    } while (0)
+
+struct Runtime {
+    bool ready = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+};
+Runtime &runtime();
+int require_runtime();
+
+// RConstants::eps lives in range-base-lib (absent); value chosen here.
+constexpr double kEps = 1.0e-14;
+
+// ----------------------------------------------------------------- device arrays
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count);
+    void release();
+    ~DBuf() { release(); }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+};
+
+template <typename T>
+int DBuf<T>::alloc(size_t count)
+{
+    if (count == n && p) return RS_OK;
+    release();
+    if (count == 0) return RS_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+        p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    n = count;
+    return RS_OK;
+}
+
+template <typename T>
+void DBuf<T>::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+
+// ----------------------------------------------------------------- host preparation
+// Parallel loop over [0, n) on host threads.
+void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)> &fn);
+
+struct Pattern {
+    uint32_t n_dof = 0;
+    std::vector<uint32_t> book;      // n_nodes*dpn -> dof or RS_EOD
+    std::vector<uint32_t> rowptr;    // n_dof+1
+    std::vector<uint32_t> col;       // nnz, sorted per row
+};
+
+// CSR pattern of the free DOFs of a mesh whose elements couple all their nodes.
+int build_pattern(uint32_t n_nodes, uint32_t n_elems, uint32_t npe, const uint32_t *conn,
+                  uint32_t dpn, const uint32_t *book, uint32_t n_dof, Pattern &pat);
+
+// Greedy node-disjoint colouring; returns colour per element and the number of colours.
+int colour_elements(uint32_t n_nodes, uint32_t n_elems, uint32_t npe, const uint32_t *conn,
+                    std::vector<uint16_t> &colour, uint32_t &n_colours);
+
+// ----------------------------------------------------------------- device matrix
+struct Matrix {
+    uint32_t n = 0;
+    uint64_t nnz = 0;
+    double frob = 0.0;
+    DBuf<uint32_t> rowptr;
+    DBuf<uint32_t> col;
+    DBuf<double> val;
+    DBuf<float> val32;          // FP32 copy for the mixed-precision inner solve
+    bool val32_valid = false;
+    bool borrowed = false;      // owned by an assembly
+    uint64_t version = 0;       // bumped on every value update
+};
+
+int matrix_frobenius(Matrix &A);   // device reduction, sets A.frob
+int matrix_refresh_fp32(Matrix &A);
+
+}  // namespace rs
+
+// C-ABI handle types are the internal structs.
+struct rs_matrix : rs::Matrix {};

@@ -1,0 +1,1713 @@
+// rs_solve.hip -- device CSR matrix, preconditioners and the Krylov solvers for gfx950.
+//
+// Reference behaviour restated (Range-Software/range-solver-lib):
+//   SparseMatrixCache / ||A||_F       src/rmatrixsolver.cpp:14-109
+//   RMatrixSolver::solve              src/rmatrixsolver.cpp:158-201 (equation scaling)
+//   RMatrixSolver::solveCG            src/rmatrixsolver.cpp:208-355
+//   RMatrixSolver::solveGMRES         src/rmatrixsolver.cpp:357-581
+//   RMatrixPreconditioner             src/rmatrixpreconditioner.cpp:16-190
+//   RIterationInfo                    src/riterationinfo.cpp:74-193
+//
+// Layout in HBM: CSR (uint32 row pointers, uint32 columns, FP64 values, optional FP32
+// copy), vectors as contiguous FP64 arrays, Jacobi as n inverse diagonal values, block
+// Jacobi as n*w values (row m holds row m of its block's inverse, as the reference).
+//
+// One CG iteration is three kernels, each ending in a deterministic grid reduction whose
+// last workgroup updates the solver scalars in device memory:
+//   dir       p = P r + beta p                       (reads P, r, p; writes p)
+//   spmv_dot  q = A p, alpha = rho / (p.q)           (CSR stream + gather of p; writes q)
+//   update    x += alpha p, r -= alpha q, |x|, |r|, rho' = r.P r, stopping rule, beta
+// Iterations run as replays of a captured hipGraph of S iterations; every kernel returns at
+// once when the device-side stop flag is set, so the iteration count equals the reference's.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+#include "rs_device.hpp"
+#include "rs_internal.hpp"
+
+namespace rs {
+
+// ============================================================================ launch shapes
+struct RowShape {
+    uint32_t nb;     // workgroups (multiple of 8)
+    uint32_t chunk;  // rows per workgroup (multiple of rows per pass)
+    int g;           // lanes per row
+};
+
+static uint32_t round_up(uint32_t a, uint32_t m) { return (a + m - 1) / m * m; }
+
+// Lanes per row: the smallest power of two >= average row length / 1.5, in [2, 64].
+static int lanes_for(uint64_t nnz, uint32_t n)
+{
+    double avg = n ? (double)nnz / n : 1.0;
+    int g = 2;
+    while (g < 64 && g * 1.5 < avg) g *= 2;
+    return g;
+}
+
+static RowShape row_shape(uint32_t n, int g)
+{
+    RowShape s;
+    s.g = g;
+    uint32_t rpp = kBlock / g;
+    uint32_t passes = (n + rpp - 1) / rpp;
+    uint32_t target = (uint32_t)std::max(8, runtime().n_cu * 8);
+    s.nb = round_up(std::max(1u, std::min(target, passes)), kXcd);
+    s.chunk = round_up((n + s.nb - 1) / s.nb, rpp);
+    if (s.chunk == 0) s.chunk = rpp;
+    return s;
+}
+
+static uint32_t vec_blocks(uint64_t items)
+{
+    uint64_t want = (items + kBlock - 1) / kBlock;
+    uint32_t target = (uint32_t)std::max(8, runtime().n_cu * 8);
+    return round_up((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, target)), kXcd);
+}
+
+// ============================================================================ device state
+// Solver scalars in device memory (written only by the last workgroup of a reduction).
+struct SolveState {
+    double rho0, rho1, beta, alpha;
+    double bn, an, error, trend;
+    double cvg, scale, rmax, t0;
+    double t1, t2, t3, t4;
+    uint32_t it, done, reason, n_outer;
+    uint32_t stop_rule, hist_cap, mode, pad;
+    uint32_t ticket[8];
+};
+enum { kStopRunning = 0, kStopDone = 1, kStopPause = 2 };
+constexpr double kReliableDelta = 0.1;   // FP64 residual replacement when |r| drops 10x
+constexpr uint32_t kMaxReplacementsWithoutGain = 20;
+
+struct CsrView {
+    uint32_t n, chunk;
+    const uint32_t *rowptr, *col;
+    const double *val;
+    const float *val32;
+};
+
+// ============================================================================ row kernels
+template <int G, typename VT, typename XT>
+__device__ __forceinline__ double row_dot(const uint32_t *__restrict__ rowptr,
+                                          const uint32_t *__restrict__ col,
+                                          const VT *__restrict__ val, const XT *__restrict__ x,
+                                          uint32_t r, int gl)
+{
+    uint32_t b = rowptr[r], e = rowptr[r + 1];
+    double s = 0.0;
+    for (uint32_t j = b + gl; j < e; j += G) s += (double)val[j] * (double)x[col[j]];
+    return group_sum<G>(s);
+}
+
+template <int G>
+__device__ __forceinline__ float row_dot_f32(const uint32_t *__restrict__ rowptr,
+                                             const uint32_t *__restrict__ col,
+                                             const float *__restrict__ val,
+                                             const float *__restrict__ x, uint32_t r, int gl)
+{
+    uint32_t b = rowptr[r], e = rowptr[r + 1];
+    float s = 0.0f;
+    for (uint32_t j = b + gl; j < e; j += G) s = fmaf(val[j], x[col[j]], s);
+    return group_sum<G>(s);
+}
+
+// Iterate the rows owned by this workgroup (contiguous XCD-aware chunk), G lanes per row.
+#define RS_ROW_LOOP(G, n, chunk)                                                       \
+    const uint32_t _lb = xcd_remap(blockIdx.x, gridDim.x);                             \
+    const int gl = threadIdx.x % (G);                                                  \
+    const uint32_t _r0 = _lb * (chunk);                                                \
+    const uint32_t _r1 = min((n), _r0 + (chunk));                                      \
+    for (uint32_t _base = _r0; _base < _r1; _base += kBlock / (G))                    \
+        for (uint32_t r = _base + threadIdx.x / (G); r < _r1; r = _r1)
+
+// y = A x
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_spmv(CsrView A, const double *__restrict__ x,
+                                                 double *__restrict__ y)
+{
+    RS_ROW_LOOP(G, A.n, A.chunk)
+    {
+        double s = row_dot<G>(A.rowptr, A.col, A.val, x, r, gl);
+        if (gl == 0) y[r] = s;
+    }
+}
+
+// y = A x with FP32 values and vectors
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_spmv32(CsrView A, const float *__restrict__ x,
+                                                   float *__restrict__ y)
+{
+    RS_ROW_LOOP(G, A.n, A.chunk)
+    {
+        float s = row_dot_f32<G>(A.rowptr, A.col, A.val32, x, r, gl);
+        if (gl == 0) y[r] = s;
+    }
+}
+
+// res = b - A x; t0 = |res|, t1 = |b| (used for the FP64 true residual)
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_resid_norm(CsrView A, const double *__restrict__ b,
+                                                       const double *__restrict__ x,
+                                                       double *partials, SolveState *st)
+{
+    double v[2] = { 0.0, 0.0 };
+    RS_ROW_LOOP(G, A.n, A.chunk)
+    {
+        double s = row_dot<G>(A.rowptr, A.col, A.val, x, r, gl);
+        if (gl == 0) {
+            double bi = b[r], ri = bi - s;
+            v[0] += ri * ri;
+            v[1] += bi * bi;
+        }
+    }
+    double tot[2];
+    if (grid_reduce<2, kBlock>(v, partials, &st->ticket[0], tot) && threadIdx.x == 0) {
+        st->t0 = sqrt(tot[0]);
+        st->t1 = sqrt(tot[1]);
+    }
+}
+
+// sum of squares of an array -> st->t0 (Frobenius norm / vector norm)
+__global__ __launch_bounds__(kBlock) void k_norm2(const double *__restrict__ a, uint64_t n,
+                                                  double *partials, SolveState *st, int slot)
+{
+    double v[1] = { 0.0 };
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        v[0] += a[i] * a[i];
+    double tot[1];
+    if (grid_reduce<1, kBlock>(v, partials, &st->ticket[0], tot) && threadIdx.x == 0) {
+        double s = sqrt(tot[0]);
+        if (slot == 0) st->t0 = s;
+        else st->t1 = s;
+    }
+}
+
+__global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        b[i] = (float)a[i];
+}
+
+// ============================================================================ preconditioner
+// constructJacobi (src/rmatrixpreconditioner.cpp:78-94): 1/A_ii, 0 if absent or zero.
+__global__ void k_jacobi_build(CsrView A, double *__restrict__ d)
+{
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < A.n; i += gridDim.x * kBlock) {
+        double v = 0.0;
+        bool found = false;
+        for (uint32_t j = A.rowptr[i]; j < A.rowptr[i + 1]; j++)
+            if (A.col[j] == i) { v = A.val[j]; found = true; break; }
+        d[i] = (found && v != 0.0) ? 1.0 / v : 0.0;
+    }
+}
+
+// constructBlockJacobi (:96-148): gather the w x w diagonal block (missing -> 0) and invert
+// it by Gauss-Jordan with partial pivoting (a singular block becomes zero).
+template <int W>
+__device__ __forceinline__ void invert_small(double (&m)[W][W], int w)
+{
+    double inv[W][W];
+    for (int i = 0; i < w; i++)
+        for (int j = 0; j < w; j++) inv[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int c = 0; c < w; c++) {
+        int piv = c;
+        for (int r = c + 1; r < w; r++)
+            if (fabs(m[r][c]) > fabs(m[piv][c])) piv = r;
+        if (m[piv][c] == 0.0) {
+            for (int i = 0; i < w; i++)
+                for (int j = 0; j < w; j++) m[i][j] = 0.0;
+            return;
+        }
+        if (piv != c)
+            for (int j = 0; j < w; j++) {
+                double t = m[c][j]; m[c][j] = m[piv][j]; m[piv][j] = t;
+                t = inv[c][j]; inv[c][j] = inv[piv][j]; inv[piv][j] = t;
+            }
+        double s = 1.0 / m[c][c];
+        for (int j = 0; j < w; j++) { m[c][j] *= s; inv[c][j] *= s; }
+        for (int r = 0; r < w; r++) {
+            if (r == c) continue;
+            double f = m[r][c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < w; j++) { m[r][j] -= f * m[c][j]; inv[r][j] -= f * inv[c][j]; }
+        }
+    }
+    for (int i = 0; i < w; i++)
+        for (int j = 0; j < w; j++) m[i][j] = inv[i][j];
+}
+
+template <int WMAX>
+__global__ void k_blockjacobi_build(CsrView A, uint32_t w, double *__restrict__ data)
+{
+    uint32_t nb = A.n / w;
+    for (uint32_t ib = blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += gridDim.x * kBlock) {
+        double m[WMAX][WMAX];
+        for (uint32_t k = 0; k < w; k++) {
+            for (uint32_t l = 0; l < w; l++) m[k][l] = 0.0;
+            uint32_t row = ib * w + k;
+            for (uint32_t j = A.rowptr[row]; j < A.rowptr[row + 1]; j++) {
+                uint32_t c = A.col[j];
+                if (c >= ib * w && c < ib * w + w) m[k][c - ib * w] = A.val[j];
+            }
+        }
+        invert_small<WMAX>(m, (int)w);
+        for (uint32_t k = 0; k < w; k++)
+            for (uint32_t l = 0; l < w; l++) data[(uint64_t)(ib * w + k) * w + l] = m[k][l];
+    }
+}
+
+// z = P r on preconditioner block ib (W rows; W == 0: runtime width up to 16).
+template <int W, typename T, typename PT>
+__device__ __forceinline__ void pc_apply_block(int ptype, uint32_t w, const PT *__restrict__ pd,
+                                               uint64_t ib, const T (&r)[W == 0 ? 16 : W],
+                                               T (&z)[W == 0 ? 16 : W])
+{
+    constexpr int WM = W == 0 ? 16 : W;
+    const int ww = W == 0 ? (int)w : W;
+    if (ptype == RS_PRECOND_BLOCK_JACOBI) {
+        for (int k = 0; k < ww; k++) {
+            T s = 0;
+            for (int l = 0; l < ww; l++) s += (T)pd[(ib * ww + k) * ww + l] * r[l];
+            z[k] = s;
+        }
+    } else if (ptype == RS_PRECOND_JACOBI) {
+        for (int k = 0; k < ww && k < WM; k++) z[k] = (T)pd[ib * ww + k] * r[k];
+    } else {
+        for (int k = 0; k < ww && k < WM; k++) z[k] = r[k];
+    }
+}
+
+// y = P x on host-visible request (RMatrixPreconditioner::compute)
+template <int W>
+__global__ void k_pc_apply(int ptype, uint32_t w, const double *__restrict__ pd, uint32_t n,
+                           const double *__restrict__ x, double *__restrict__ y)
+{
+    constexpr int WM = W == 0 ? 16 : W;
+    const uint32_t ww = W == 0 ? w : W;
+    uint64_t nb = n / ww;
+    for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
+        double r[WM], z[WM];
+        for (uint32_t k = 0; k < ww; k++) r[k] = x[ib * ww + k];
+        pc_apply_block<W, double, double>(ptype, ww, pd, ib, r, z);
+        for (uint32_t k = 0; k < ww; k++) y[ib * ww + k] = z[k];
+    }
+}
+
+// ============================================================================ CG (FP64)
+struct CgArgs {
+    CsrView A;
+    const double *b;
+    double *x, *r, *p, *q;
+    int32_t ptype;
+    uint32_t pw;
+    const double *pd;
+    double *partials;
+    SolveState *st;
+    double *hist;
+};
+
+__global__ void k_state_init(SolveState *st, double an, double cvg, double prev_error,
+                             uint32_t n_outer, uint32_t stop_rule, uint32_t hist_cap, uint32_t mode)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    st->rho0 = st->rho1 = st->beta = st->alpha = 0.0;
+    st->bn = 0.0;
+    st->an = an;
+    st->error = prev_error;
+    st->trend = 0.0;
+    st->cvg = cvg;
+    st->scale = 1.0;
+    st->rmax = 0.0;
+    st->t0 = st->t1 = st->t2 = st->t3 = st->t4 = 0.0;
+    st->it = 0;
+    st->done = n_outer == 0 ? kStopDone : kStopRunning;
+    st->reason = n_outer == 0 ? RS_REASON_MAXIT : RS_REASON_RUNNING;
+    st->n_outer = n_outer;
+    st->stop_rule = stop_rule;
+    st->hist_cap = hist_cap;
+    st->mode = mode;
+    for (int i = 0; i < 8; i++) st->ticket[i] = 0;
+}
+
+// equationScale = 1e9 / |bn/An| if both nonzero (src/rmatrixsolver.cpp:164-171)
+__global__ void k_scale_from_norm(SolveState *st)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double bn = st->t0, an = st->an;
+    st->scale = (bn != 0.0 && an != 0.0) ? 1.0e9 / fabs(bn / an) : 1.0;
+}
+
+// y = b*s ; x *= s   (src/rmatrixsolver.cpp:181-184)
+__global__ void k_apply_scale(const double *__restrict__ b, double *__restrict__ y,
+                              double *__restrict__ x, uint64_t n, const SolveState *st)
+{
+    double s = st->scale;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        y[i] = b[i] * s;
+        x[i] *= s;
+    }
+}
+
+// x *= 1/s   (src/rmatrixsolver.cpp:198)
+__global__ void k_unscale(double *__restrict__ x, uint64_t n, const SolveState *st)
+{
+    double s = 1.0 / st->scale;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        x[i] *= s;
+}
+
+// r = b - A x, bn = |b|   (src/rmatrixsolver.cpp:236-246)
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs a)
+{
+    double v[1] = { 0.0 };
+    RS_ROW_LOOP(G, a.A.n, a.A.chunk)
+    {
+        double s = row_dot<G>(a.A.rowptr, a.A.col, a.A.val, a.x, r, gl);
+        if (gl == 0) {
+            double bi = a.b[r];
+            a.r[r] = bi - s;
+            v[0] += bi * bi;
+        }
+    }
+    double tot[1];
+    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0)
+        a.st->bn = sqrt(tot[0]);
+}
+
+// rho0 = r . P r for the first iteration (src/rmatrixsolver.cpp:258-267)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_cg_rho(CgArgs a)
+{
+    constexpr int WM = W == 0 ? 16 : W;
+    const uint32_t ww = W == 0 ? a.pw : W;
+    const uint64_t nb = a.A.n / ww;
+    double v[1] = { 0.0 };
+    for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
+        double r[WM], z[WM];
+        for (uint32_t k = 0; k < ww; k++) r[k] = a.r[ib * ww + k];
+        pc_apply_block<W, double, double>(a.ptype, ww, a.pd, ib, r, z);
+        for (uint32_t k = 0; k < ww; k++) v[0] += r[k] * z[k];
+    }
+    double tot[1];
+    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0)
+        a.st->rho0 = tot[0];
+}
+
+// p = z (first iteration) or p*beta + z   (src/rmatrixsolver.cpp:278-289)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs a)
+{
+    if (a.st->done) return;
+    constexpr int WM = W == 0 ? 16 : W;
+    const uint32_t ww = W == 0 ? a.pw : W;
+    const uint64_t nb = a.A.n / ww;
+    const bool first = a.st->it == 0;
+    const double beta = a.st->beta;
+    for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
+        double r[WM], z[WM];
+        for (uint32_t k = 0; k < ww; k++) r[k] = a.r[ib * ww + k];
+        pc_apply_block<W, double, double>(a.ptype, ww, a.pd, ib, r, z);
+        for (uint32_t k = 0; k < ww; k++) {
+            uint64_t i = ib * ww + k;
+            a.p[i] = first ? z[k] : a.p[i] * beta + z[k];
+        }
+    }
+}
+
+// q = A p, dot = p.q -> alpha = rho0/dot, rho1 = rho0   (src/rmatrixsolver.cpp:295-320)
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs a)
+{
+    if (a.st->done) return;
+    double v[1] = { 0.0 };
+    RS_ROW_LOOP(G, a.A.n, a.A.chunk)
+    {
+        double s = row_dot<G>(a.A.rowptr, a.A.col, a.A.val, a.p, r, gl);
+        if (gl == 0) {
+            a.q[r] = s;
+            v[0] += a.p[r] * s;
+        }
+    }
+    double tot[1];
+    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0) {
+        double dot = tot[0];
+        const double eps = kEps;
+        if (dot > 0.0) dot = fmax(dot, eps);
+        else if (dot < 0.0) dot = fmin(dot, -eps);
+        else dot = eps;
+        a.st->alpha = a.st->rho0 / dot;
+        a.st->rho1 = a.st->rho0;
+    }
+}
+
+// x += alpha p, r -= alpha q, norms, stopping rule, next rho and beta
+// (src/rmatrixsolver.cpp:324-352, next iteration's :258-275)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
+{
+    SolveState *st = a.st;
+    if (st->done) return;
+    constexpr int WM = W == 0 ? 16 : W;
+    const uint32_t ww = W == 0 ? a.pw : W;
+    const uint64_t nb = a.A.n / ww;
+    const double alpha = st->alpha;
+    double v[3] = { 0.0, 0.0, 0.0 };
+    for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
+        double r[WM], z[WM];
+        for (uint32_t k = 0; k < ww; k++) {
+            uint64_t i = ib * ww + k;
+            double xi = a.x[i] + alpha * a.p[i];
+            double ri = a.r[i] - alpha * a.q[i];
+            a.x[i] = xi;
+            a.r[i] = ri;
+            v[0] += xi * xi;
+            v[1] += ri * ri;
+            r[k] = ri;
+        }
+        pc_apply_block<W, double, double>(a.ptype, ww, a.pd, ib, r, z);
+        for (uint32_t k = 0; k < ww; k++) v[2] += r[k] * z[k];
+    }
+    double tot[3];
+    if (grid_reduce<3, kBlock>(v, a.partials, &st->ticket[0], tot) && threadIdx.x == 0) {
+        const double eps = kEps;
+        double xn = sqrt(tot[0]), rn = sqrt(tot[1]);
+        double norm = st->an * xn + st->bn;
+        if (fabs(norm) < eps) norm = eps;
+        double err = st->stop_rule == RS_STOP_RELRES ? rn / fmax(st->bn, eps) : rn / norm;
+        double trend = err - st->error;
+        st->trend = trend;
+        st->error = err;
+        uint32_t it = st->it;
+        if (it < st->hist_cap) {
+            a.hist[2 * it] = err;
+            a.hist[2 * it + 1] = trend;
+        }
+        st->it = it + 1;
+        int reason = converged_reason(err, trend, st->cvg, eps);
+        if (reason != RS_REASON_RUNNING) {
+            st->reason = reason;
+            st->done = kStopDone;
+        } else if (it + 1 >= st->n_outer) {
+            st->reason = RS_REASON_MAXIT;
+            st->done = kStopDone;
+        } else {
+            st->beta = tot[2] / st->rho1;
+            st->rho0 = tot[2];
+        }
+    }
+}
+
+// ============================================================================ CG (mixed)
+// FP32 matrix values, FP32 direction/residual/correction vectors and FP32 preconditioner;
+// FP64 accumulation of every dot product and norm; FP64 solution.  The recursive FP32
+// residual is replaced by the FP64 true residual b - A x whenever it has dropped by
+// kReliableDelta since the last replacement, or when it claims convergence (reliable
+// updates: Sleijpen & van der Vorst 1996); convergence is decided on the FP64 residual.
+struct MpArgs {
+    CsrView A;
+    const double *b;
+    double *x;            // FP64 solution (scaled)
+    float *r, *p, *q, *dx;
+    int32_t ptype;
+    uint32_t pw;
+    const float *pd;
+    double *partials;
+    SolveState *st;
+    double *hist;
+};
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_mp_dir(MpArgs a)
+{
+    if (a.st->done) return;
+    constexpr int WM = W == 0 ? 16 : W;
+    const uint32_t ww = W == 0 ? a.pw : W;
+    const uint64_t nb = a.A.n / ww;
+    const bool first = a.st->it == 0;
+    const float beta = (float)a.st->beta;
+    for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
+        float r[WM], z[WM];
+        for (uint32_t k = 0; k < ww; k++) r[k] = a.r[ib * ww + k];
+        pc_apply_block<W, float, float>(a.ptype, ww, a.pd, ib, r, z);
+        for (uint32_t k = 0; k < ww; k++) {
+            uint64_t i = ib * ww + k;
+            a.p[i] = first ? z[k] : fmaf(a.p[i], beta, z[k]);
+        }
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_mp_spmv_dot(MpArgs a)
+{
+    if (a.st->done) return;
+    double v[1] = { 0.0 };
+    RS_ROW_LOOP(G, a.A.n, a.A.chunk)
+    {
+        float s = row_dot_f32<G>(a.A.rowptr, a.A.col, a.A.val32, a.p, r, gl);
+        if (gl == 0) {
+            a.q[r] = s;
+            v[0] += (double)a.p[r] * (double)s;
+        }
+    }
+    double tot[1];
+    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0) {
+        double dot = tot[0];
+        const double eps = kEps;
+        if (dot > 0.0) dot = fmax(dot, eps);
+        else if (dot < 0.0) dot = fmin(dot, -eps);
+        else dot = eps;
+        a.st->alpha = a.st->rho0 / dot;
+        a.st->rho1 = a.st->rho0;
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_mp_update(MpArgs a)
+{
+    SolveState *st = a.st;
+    if (st->done) return;
+    constexpr int WM = W == 0 ? 16 : W;
+    const uint32_t ww = W == 0 ? a.pw : W;
+    const uint64_t nb = a.A.n / ww;
+    const float alpha = (float)st->alpha;
+    double v[2] = { 0.0, 0.0 };
+    for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
+        float r[WM], z[WM];
+        for (uint32_t k = 0; k < ww; k++) {
+            uint64_t i = ib * ww + k;
+            a.dx[i] = fmaf(alpha, a.p[i], a.dx[i]);
+            float ri = fmaf(-alpha, a.q[i], a.r[i]);
+            a.r[i] = ri;
+            v[0] += (double)ri * (double)ri;
+            r[k] = ri;
+        }
+        pc_apply_block<W, float, float>(a.ptype, ww, a.pd, ib, r, z);
+        for (uint32_t k = 0; k < ww; k++) v[1] += (double)r[k] * (double)z[k];
+    }
+    double tot[2];
+    if (grid_reduce<2, kBlock>(v, a.partials, &st->ticket[0], tot) && threadIdx.x == 0) {
+        const double eps = kEps;
+        double rn = sqrt(tot[0]);
+        double err = rn / fmax(st->bn, eps);
+        double trend = err - st->error;
+        st->trend = trend;
+        st->error = err;
+        uint32_t it = st->it;
+        if (it < st->hist_cap) {
+            a.hist[2 * it] = err;
+            a.hist[2 * it + 1] = trend;
+        }
+        st->it = it + 1;
+        bool bad = !dev_isnormal(err) || !isfinite(tot[1]);
+        if (bad) {
+            st->reason = RS_REASON_NONNORMAL;
+            st->done = kStopDone;
+        } else if (it + 1 >= st->n_outer) {
+            st->reason = RS_REASON_MAXIT;
+            st->done = kStopDone;
+        } else {
+            st->beta = tot[1] / st->rho1;
+            st->rho0 = tot[1];
+            if (rn < kReliableDelta * st->rmax || err < st->cvg) st->done = kStopPause;
+        }
+    }
+}
+
+// x += dx, dx = 0
+__global__ void k_mp_fold(double *__restrict__ x, float *__restrict__ dx, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        x[i] += (double)dx[i];
+        dx[i] = 0.0f;
+    }
+}
+
+// r32 = b - A x in FP64; |r| decides convergence (first: also sets bn and rmax)
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_mp_true_resid(MpArgs a, int first)
+{
+    double v[2] = { 0.0, 0.0 };
+    RS_ROW_LOOP(G, a.A.n, a.A.chunk)
+    {
+        double s = row_dot<G>(a.A.rowptr, a.A.col, a.A.val, a.x, r, gl);
+        if (gl == 0) {
+            double bi = a.b[r], ri = bi - s;
+            a.r[r] = (float)ri;
+            v[0] += ri * ri;
+            v[1] += bi * bi;
+        }
+    }
+    double tot[2];
+    SolveState *st = a.st;
+    if (grid_reduce<2, kBlock>(v, a.partials, &st->ticket[0], tot) && threadIdx.x == 0) {
+        const double eps = kEps;
+        double rn = sqrt(tot[0]);
+        if (first) st->bn = sqrt(tot[1]);
+        double err = rn / fmax(st->bn, eps);
+        st->rmax = rn;
+        st->t2 = err;     // last true relative residual
+        if (!first) {
+            st->error = err;
+            uint32_t it = st->it;
+            if (it > 0 && it - 1 < st->hist_cap) a.hist[2 * (it - 1)] = err;
+        }
+        if (!dev_isnormal(err) && err != 0.0) {
+            st->reason = RS_REASON_NONNORMAL;
+            st->done = kStopDone;
+        } else if (err < st->cvg) {
+            st->reason = RS_REASON_CONVERGED;
+            st->done = kStopDone;
+        } else {
+            st->done = kStopRunning;
+        }
+    }
+}
+
+// rho = r . P r after a residual replacement; the next direction uses beta = rho/rho1
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_mp_rho(MpArgs a, int first)
+{
+    constexpr int WM = W == 0 ? 16 : W;
+    const uint32_t ww = W == 0 ? a.pw : W;
+    const uint64_t nb = a.A.n / ww;
+    double v[1] = { 0.0 };
+    for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
+        float r[WM], z[WM];
+        for (uint32_t k = 0; k < ww; k++) r[k] = a.r[ib * ww + k];
+        pc_apply_block<W, float, float>(a.ptype, ww, a.pd, ib, r, z);
+        for (uint32_t k = 0; k < ww; k++) v[0] += (double)r[k] * (double)z[k];
+    }
+    double tot[1];
+    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0) {
+        if (first) {
+            a.st->rho0 = tot[0];
+        } else {
+            a.st->beta = tot[0] / a.st->rho1;
+            a.st->rho0 = tot[0];
+        }
+    }
+}
+
+// ============================================================================ GMRES kernels
+// h_i = w . v_i for i in [i0, i0+8) -> partials[i][block]; final sum in k_multidot_final
+__global__ __launch_bounds__(kBlock) void k_multidot(const double *__restrict__ w,
+                                                     const double *__restrict__ V, uint64_t ld,
+                                                     uint32_t n, uint32_t cnt, double *partials)
+{
+    __shared__ double lds[8 * (kBlock / kWave)];
+    uint32_t i0 = blockIdx.y * 8;
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = 0.0;
+    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (uint64_t)gridDim.x * kBlock) {
+        double wj = w[j];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (i0 + k < cnt) v[k] += wj * V[(i0 + k) * ld + j];
+    }
+    block_sum<8, kBlock>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (i0 + k < cnt) partials[(uint64_t)(i0 + k) * gridDim.x + blockIdx.x] = v[k];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_multidot_final(const double *partials, uint32_t nblk,
+                                                           double *out)
+{
+    __shared__ double lds[kBlock / kWave];
+    double v[1] = { 0.0 };
+    for (uint32_t j = threadIdx.x; j < nblk; j += kBlock) v[0] += partials[(uint64_t)blockIdx.x * nblk + j];
+    block_sum<1, kBlock>(v, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
+// w -= sum_i h_i v_i ; t0 = |w|  (classical Gram-Schmidt, src/rmatrixsolver.cpp:486-508)
+__global__ __launch_bounds__(kBlock) void k_gs_subtract(double *__restrict__ w,
+                                                        const double *__restrict__ V, uint64_t ld,
+                                                        uint32_t n, uint32_t cnt,
+                                                        const double *__restrict__ h,
+                                                        double *partials, SolveState *st)
+{
+    double v[1] = { 0.0 };
+    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (uint64_t)gridDim.x * kBlock) {
+        double value = w[j];
+        for (uint32_t i = 0; i < cnt; i++) value -= h[i] * V[i * ld + j];
+        w[j] = value;
+        v[0] += value * value;
+    }
+    double tot[1];
+    if (grid_reduce<1, kBlock>(v, partials, &st->ticket[0], tot) && threadIdx.x == 0) st->t0 = sqrt(tot[0]);
+}
+
+// v = (h == 0) ? 0 : w / h with h = st->t0
+__global__ void k_normalize(const double *__restrict__ w, double *__restrict__ v, uint64_t n,
+                            const SolveState *st)
+{
+    double h = st->t0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        v[i] = (h == 0.0) ? 0.0 : w[i] / h;
+}
+
+// ro = b - A x; t0 = |x|, t1 = |ro|   (src/rmatrixsolver.cpp:401-412)
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_gmres_resid(CsrView A, const double *__restrict__ b,
+                                                        const double *__restrict__ x,
+                                                        double *__restrict__ ro, double *partials,
+                                                        SolveState *st)
+{
+    double v[2] = { 0.0, 0.0 };
+    RS_ROW_LOOP(G, A.n, A.chunk)
+    {
+        double s = row_dot<G>(A.rowptr, A.col, A.val, x, r, gl);
+        if (gl == 0) {
+            double xi = x[r], ri = b[r] - s;
+            ro[r] = ri;
+            v[0] += xi * xi;
+            v[1] += ri * ri;
+        }
+    }
+    double tot[2];
+    if (grid_reduce<2, kBlock>(v, partials, &st->ticket[0], tot) && threadIdx.x == 0) {
+        st->t0 = sqrt(tot[0]);
+        st->t1 = sqrt(tot[1]);
+    }
+}
+
+// v0 = ro / beta (0 if beta == 0)
+__global__ void k_scale_by(const double *__restrict__ a, double *__restrict__ out, uint64_t n, double beta)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        out[i] = (beta == 0.0) ? 0.0 : a[i] / beta;
+}
+
+// x += sum_{j<cnt} Z_j y_j   (src/rmatrixsolver.cpp:571-578)
+__global__ void k_update_x(double *__restrict__ x, const double *__restrict__ Z, uint64_t ld,
+                           uint32_t n, uint32_t cnt, const double *__restrict__ y)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        double xi = x[i];
+        for (uint32_t j = 0; j < cnt; j++) xi += Z[j * ld + i] * y[j];
+        x[i] = xi;
+    }
+}
+
+// ============================================================================ dispatch helpers
+#define RS_DISPATCH_G(g, KERNEL, grid, ...)                                                   \
+    do {                                                                                      \
+        hipStream_t _s = runtime().stream;                                                    \
+        switch (g) {                                                                          \
+            case 2: KERNEL<2><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                   \
+            case 4: KERNEL<4><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                   \
+            case 8: KERNEL<8><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                   \
+            case 16: KERNEL<16><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                 \
+            case 32: KERNEL<32><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                 \
+            default: KERNEL<64><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                 \
+        }                                                                                     \
+    } while (0)
+
+#define RS_DISPATCH_W(w, KERNEL, grid, ...)                                                   \
+    do {                                                                                      \
+        hipStream_t _s = runtime().stream;                                                    \
+        if ((w) == 1) KERNEL<1><<<grid, kBlock, 0, _s>>>(__VA_ARGS__);                        \
+        else if ((w) == 3) KERNEL<3><<<grid, kBlock, 0, _s>>>(__VA_ARGS__);                   \
+        else KERNEL<0><<<grid, kBlock, 0, _s>>>(__VA_ARGS__);                                 \
+    } while (0)
+
+static CsrView view_of(const Matrix &A, const RowShape &sh)
+{
+    CsrView v;
+    v.n = A.n;
+    v.chunk = sh.chunk;
+    v.rowptr = A.rowptr.p;
+    v.col = A.col.p;
+    v.val = A.val.p;
+    v.val32 = A.val32.p;
+    return v;
+}
+
+// Scratch for standalone reductions (matrix norm, SpMV requests).
+struct Scratch {
+    DBuf<double> partials;
+    DBuf<SolveState> st;
+};
+static Scratch &scratch()
+{
+    static Scratch s;
+    return s;
+}
+static int ensure_scratch(size_t nparts)
+{
+    Scratch &s = scratch();
+    if (s.partials.n < nparts) {
+        int rc = s.partials.alloc(std::max<size_t>(nparts, 65536));
+        if (rc) return rc;
+    }
+    if (!s.st.p) {
+        int rc = s.st.alloc(1);
+        if (rc) return rc;
+        RS_HIP(hipMemsetAsync(s.st.p, 0, sizeof(SolveState), runtime().stream));
+    }
+    return RS_OK;
+}
+
+int matrix_frobenius(Matrix &A)
+{
+    int rc = ensure_scratch(8 * 4096);
+    if (rc) return rc;
+    Scratch &s = scratch();
+    uint32_t nb = vec_blocks(A.nnz);
+    hipStream_t st = runtime().stream;
+    k_norm2<<<nb, kBlock, 0, st>>>(A.val.p, A.nnz, s.partials.p, s.st.p, 0);
+    RS_HIP(hipGetLastError());
+    SolveState h;
+    RS_HIP(hipMemcpyAsync(&h, s.st.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    A.frob = h.t0;
+    return RS_OK;
+}
+
+int matrix_refresh_fp32(Matrix &A)
+{
+    if (A.val32_valid) return RS_OK;
+    int rc = A.val32.alloc(A.nnz);
+    if (rc) return rc;
+    k_to_f32<<<vec_blocks(A.nnz), kBlock, 0, runtime().stream>>>(A.val.p, A.val32.p, A.nnz);
+    RS_HIP(hipGetLastError());
+    A.val32_valid = true;
+    return RS_OK;
+}
+
+}  // namespace rs
+
+using namespace rs;
+
+// ============================================================================ C ABI: matrix
+extern "C" int rs_matrix_create(uint32_t n_rows, const uint32_t *row_ptr, const uint32_t *col,
+                                const double *val, rs_matrix **out)
+{
+    if (!out || !row_ptr || (n_rows && (!col || !val) && row_ptr[n_rows] > 0))
+        return fail(RS_ERR_INVALID_ARG, "rs_matrix_create: null argument");
+    if (row_ptr[0] != 0) return fail(RS_ERR_INVALID_ARG, "rs_matrix_create: row_ptr[0] != 0");
+    for (uint32_t i = 0; i < n_rows; i++)
+        if (row_ptr[i + 1] < row_ptr[i]) return fail(RS_ERR_INVALID_ARG, "rs_matrix_create: row_ptr not monotone");
+    uint64_t nnz = row_ptr[n_rows];
+    for (uint64_t j = 0; j < nnz; j++)
+        if (col[j] >= n_rows) return fail(RS_ERR_INVALID_ARG, "rs_matrix_create: column index out of range");
+    int rc = require_runtime();
+    if (rc) return rc;
+    std::unique_ptr<rs_matrix> A(new rs_matrix());
+    A->n = n_rows;
+    A->nnz = nnz;
+    if ((rc = A->rowptr.alloc(n_rows + 1ull))) return rc;
+    if ((rc = A->col.alloc(nnz))) return rc;
+    if ((rc = A->val.alloc(nnz))) return rc;
+    hipStream_t st = runtime().stream;
+    RS_HIP(hipMemcpyAsync(A->rowptr.p, row_ptr, sizeof(uint32_t) * (n_rows + 1ull), hipMemcpyHostToDevice, st));
+    if (nnz) {
+        RS_HIP(hipMemcpyAsync(A->col.p, col, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(A->val.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
+        if ((rc = matrix_frobenius(*A))) return rc;
+    }
+    RS_HIP(hipStreamSynchronize(st));
+    *out = A.release();
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_update_values(rs_matrix *A, const double *val)
+{
+    if (!A || (!val && A->nnz)) return fail(RS_ERR_INVALID_ARG, "rs_matrix_update_values: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    if (A->nnz) {
+        RS_HIP(hipMemcpyAsync(A->val.p, val, sizeof(double) * A->nnz, hipMemcpyHostToDevice, runtime().stream));
+        if ((rc = matrix_frobenius(*A))) return rc;
+    }
+    A->val32_valid = false;
+    A->version++;
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_update_values_device(rs_matrix *A, const double *d_val)
+{
+    if (!A || (!d_val && A->nnz)) return fail(RS_ERR_INVALID_ARG, "rs_matrix_update_values_device: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    if (A->nnz) {
+        if (d_val != A->val.p)
+            RS_HIP(hipMemcpyAsync(A->val.p, d_val, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, runtime().stream));
+        if ((rc = matrix_frobenius(*A))) return rc;
+    }
+    A->val32_valid = false;
+    A->version++;
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_info(const rs_matrix *A, uint32_t *n_rows, uint64_t *nnz, double *frobenius)
+{
+    if (!A) return fail(RS_ERR_INVALID_ARG, "rs_matrix_info: null matrix");
+    if (n_rows) *n_rows = A->n;
+    if (nnz) *nnz = A->nnz;
+    if (frobenius) *frobenius = A->frob;
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_download(const rs_matrix *A, uint32_t *row_ptr, uint32_t *col, double *val)
+{
+    if (!A) return fail(RS_ERR_INVALID_ARG, "rs_matrix_download: null matrix");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    if (row_ptr) RS_HIP(hipMemcpyAsync(row_ptr, A->rowptr.p, sizeof(uint32_t) * (A->n + 1ull), hipMemcpyDeviceToHost, st));
+    if (col && A->nnz) RS_HIP(hipMemcpyAsync(col, A->col.p, sizeof(uint32_t) * A->nnz, hipMemcpyDeviceToHost, st));
+    if (val && A->nnz) RS_HIP(hipMemcpyAsync(val, A->val.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_spmv_device(rs_matrix *A, const double *d_x, double *d_y)
+{
+    if (!A || !d_x || !d_y) return fail(RS_ERR_INVALID_ARG, "rs_matrix_spmv_device: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    if (A->n == 0) return RS_OK;
+    RowShape sh = row_shape(A->n, lanes_for(A->nnz, A->n));
+    CsrView v = view_of(*A, sh);
+    RS_DISPATCH_G(sh.g, k_spmv, sh.nb, v, d_x, d_y);
+    RS_HIP(hipGetLastError());
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_spmv(rs_matrix *A, const double *x, double *y)
+{
+    if (!A || !x || !y) return fail(RS_ERR_INVALID_ARG, "rs_matrix_spmv: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    DBuf<double> dx, dy;
+    if ((rc = dx.alloc(A->n)) || (rc = dy.alloc(A->n))) return rc;
+    hipStream_t st = runtime().stream;
+    RS_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * A->n, hipMemcpyHostToDevice, st));
+    if ((rc = rs_matrix_spmv_device(A, dx.p, dy.p))) return rc;
+    RS_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * A->n, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t reps, uint32_t lanes,
+                                    double *avg_ms)
+{
+    if (!A || !avg_ms || reps == 0) return fail(RS_ERR_INVALID_ARG, "rs_matrix_spmv_timed: bad argument");
+    if (lanes && (lanes & (lanes - 1)) != 0) return fail(RS_ERR_INVALID_ARG, "rs_matrix_spmv_timed: lanes not a power of two");
+    int rc = require_runtime();
+    if (rc) return rc;
+    if (A->n == 0) { *avg_ms = 0.0; return RS_OK; }
+    hipStream_t st = runtime().stream;
+    int g = lanes ? (int)lanes : lanes_for(A->nnz, A->n);
+    RowShape sh = row_shape(A->n, g);
+    DBuf<double> x, y;
+    if ((rc = x.alloc(A->n)) || (rc = y.alloc(A->n))) return rc;
+    RS_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * A->n, st));
+    if (precision == RS_PRECISION_MIXED && (rc = matrix_refresh_fp32(*A))) return rc;
+    CsrView v = view_of(*A, sh);
+    float *x32 = reinterpret_cast<float *>(x.p), *y32 = reinterpret_cast<float *>(y.p);
+    auto launch = [&]() {
+        if (precision == RS_PRECISION_MIXED) RS_DISPATCH_G(sh.g, k_spmv32, sh.nb, v, x32, y32);
+        else RS_DISPATCH_G(sh.g, k_spmv, sh.nb, v, x.p, y.p);
+    };
+    launch();  // warm
+    hipEvent_t e0, e1;
+    RS_HIP(hipEventCreate(&e0));
+    RS_HIP(hipEventCreate(&e1));
+    RS_HIP(hipEventRecord(e0, st));
+    for (uint32_t i = 0; i < reps; i++) launch();
+    RS_HIP(hipEventRecord(e1, st));
+    RS_HIP(hipEventSynchronize(e1));
+    RS_HIP(hipGetLastError());
+    float ms = 0.0f;
+    RS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_destroy(rs_matrix *A)
+{
+    if (!A) return RS_OK;
+    if (A->borrowed) return fail(RS_ERR_INVALID_ARG, "rs_matrix_destroy: matrix is owned by an assembly");
+    delete A;
+    return RS_OK;
+}
+
+// ============================================================================ C ABI: precond
+struct rs_precond {
+    int32_t type = RS_PRECOND_NONE;
+    uint32_t n = 0, width = 0;
+    DBuf<double> data;
+    DBuf<float> data32;
+    bool data32_valid = false;
+};
+
+namespace rs {
+static int precond_build(const Matrix &A, int32_t type, uint32_t block_size, rs_precond &P)
+{
+    P.type = type;
+    P.n = A.n;
+    P.data32_valid = false;
+    RowShape sh = row_shape(std::max(A.n, 1u), 2);
+    CsrView v = view_of(A, sh);
+    hipStream_t st = runtime().stream;
+    int rc;
+    if (type == RS_PRECOND_NONE) {
+        P.width = 1;
+        return RS_OK;
+    }
+    if (type == RS_PRECOND_JACOBI) {
+        P.width = 1;
+        if ((rc = P.data.alloc(std::max(A.n, 1u)))) return rc;
+        if (A.n) k_jacobi_build<<<vec_blocks(A.n), kBlock, 0, st>>>(v, P.data.p);
+        RS_HIP(hipGetLastError());
+        return RS_OK;
+    }
+    if (type == RS_PRECOND_BLOCK_JACOBI) {
+        uint32_t w = block_size;
+        if (w == 0) return fail(RS_ERR_INVALID_ARG, "block-Jacobi: block size 0");
+        if (w > 16) return fail(RS_ERR_UNSUPPORTED, "block-Jacobi: block size above 16");
+        while (A.n % w != 0) w--;  // src/rmatrixpreconditioner.cpp:98-105
+        P.width = w;
+        if ((rc = P.data.alloc(std::max<uint64_t>((uint64_t)A.n * w, 1)))) return rc;
+        if (A.n && w == 3) k_blockjacobi_build<3><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, P.data.p);
+        else if (A.n && w <= 4) k_blockjacobi_build<4><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, P.data.p);
+        else if (A.n) k_blockjacobi_build<16><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, P.data.p);
+        RS_HIP(hipGetLastError());
+        return RS_OK;
+    }
+    return fail(RS_ERR_INVALID_PRECOND, "Invalid matrix preconditioner type '" + std::to_string(type) + "'");
+}
+
+static int precond_fp32(rs_precond &P)
+{
+    if (P.type == RS_PRECOND_NONE || P.data32_valid) return RS_OK;
+    int rc = P.data32.alloc(P.data.n);
+    if (rc) return rc;
+    k_to_f32<<<vec_blocks(P.data.n), kBlock, 0, runtime().stream>>>(P.data.p, P.data32.p, P.data.n);
+    RS_HIP(hipGetLastError());
+    P.data32_valid = true;
+    return RS_OK;
+}
+}  // namespace rs
+
+extern "C" int rs_precond_create(const rs_matrix *A, int32_t type, uint32_t block_size, rs_precond **out)
+{
+    if (!A || !out) return fail(RS_ERR_INVALID_ARG, "rs_precond_create: null argument");
+    if (type < RS_PRECOND_NONE || type > RS_PRECOND_BLOCK_JACOBI)
+        return fail(RS_ERR_INVALID_PRECOND, "Invalid matrix preconditioner type '" + std::to_string(type) + "'");
+    int rc = require_runtime();
+    if (rc) return rc;
+    std::unique_ptr<rs_precond> P(new rs_precond());
+    if ((rc = precond_build(*A, type, block_size, *P))) return rc;
+    RS_HIP(hipStreamSynchronize(runtime().stream));
+    *out = P.release();
+    return RS_OK;
+}
+
+extern "C" int rs_precond_width(const rs_precond *P, uint32_t *width)
+{
+    if (!P || !width) return fail(RS_ERR_INVALID_ARG, "rs_precond_width: null argument");
+    *width = P->width;
+    return RS_OK;
+}
+
+extern "C" int rs_precond_download(const rs_precond *P, double *data)
+{
+    if (!P || !data) return fail(RS_ERR_INVALID_ARG, "rs_precond_download: null argument");
+    if (P->type == RS_PRECOND_NONE) return RS_OK;
+    int rc = require_runtime();
+    if (rc) return rc;
+    RS_HIP(hipMemcpyAsync(data, P->data.p, sizeof(double) * (uint64_t)P->n * P->width, hipMemcpyDeviceToHost, runtime().stream));
+    RS_HIP(hipStreamSynchronize(runtime().stream));
+    return RS_OK;
+}
+
+extern "C" int rs_precond_compute(const rs_precond *P, const double *x, double *y)
+{
+    if (!P || !x || !y) return fail(RS_ERR_INVALID_ARG, "rs_precond_compute: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    if (P->n == 0) return RS_OK;
+    DBuf<double> dx, dy;
+    if ((rc = dx.alloc(P->n)) || (rc = dy.alloc(P->n))) return rc;
+    hipStream_t st = runtime().stream;
+    RS_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * P->n, hipMemcpyHostToDevice, st));
+    uint32_t w = P->width ? P->width : 1;
+    RS_DISPATCH_W(w, k_pc_apply, vec_blocks(P->n / w), P->type, w, P->data.p, P->n, dx.p, dy.p);
+    RS_HIP(hipGetLastError());
+    RS_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * P->n, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
+
+extern "C" int rs_precond_destroy(rs_precond *P)
+{
+    delete P;
+    return RS_OK;
+}
+
+// ============================================================================ C ABI: solver
+struct rs_solver {
+    rs_conf conf{};
+    std::string out_file;
+    double error = 0.0;     // RIterationInfo::error, carried across solves of this object
+    double trend = 0.0;
+    uint32_t last_iterations = 0;
+    std::vector<double> history;
+    // device workspace
+    uint32_t ws_n = 0;
+    DBuf<double> y, r, p, q, ro, w, V, Z, hbuf;
+    DBuf<float> r32, p32, q32, dx32;
+    DBuf<double> partials, hist;
+    DBuf<SolveState> st;
+    // captured iteration graph
+    hipGraphExec_t graph = nullptr;
+    std::string graph_key;   // raw bytes of every kernel argument and launch shape captured
+    ~rs_solver()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+};
+
+namespace rs {
+
+static constexpr uint32_t kGraphIters = 16;
+static constexpr uint32_t kHistCap = 1u << 20;
+
+static int ensure_ws(rs_solver &S, uint32_t n, bool mixed, bool gmres)
+{
+    int rc;
+    size_t N = std::max(n, 1u);
+    if ((rc = S.y.alloc(N)) || (rc = S.partials.alloc(8 * 4096 * 4)) || (rc = S.st.alloc(1))) return rc;
+    uint32_t hc = std::min(kHistCap, std::max(1u, S.conf.n_outer_iterations));
+    if ((rc = S.hist.alloc(2ull * hc))) return rc;
+    if (!mixed && !gmres) {
+        if ((rc = S.r.alloc(N)) || (rc = S.p.alloc(N)) || (rc = S.q.alloc(N))) return rc;
+    }
+    if (mixed) {
+        if ((rc = S.r32.alloc(N)) || (rc = S.p32.alloc(N)) || (rc = S.q32.alloc(N)) || (rc = S.dx32.alloc(N))) return rc;
+    }
+    if (gmres) {
+        uint64_t k = S.conf.n_inner_iterations;
+        if ((rc = S.ro.alloc(N)) || (rc = S.w.alloc(N)) || (rc = S.V.alloc((k + 1) * N)) ||
+            (rc = S.Z.alloc((k + 1) * N)) || (rc = S.hbuf.alloc(2 * (k + 2))))
+            return rc;
+    }
+    S.ws_n = n;
+    return RS_OK;
+}
+
+// RIterationInfo::printIteration / writeToFile (src/riterationinfo.cpp:133-193), replayed
+// on the host from the device history after the solve.
+static void emit_log(const rs_solver &S, const char *title, uint32_t iters, uint32_t hist_n)
+{
+    const rs_conf &c = S.conf;
+    if (c.output_frequency == 0) return;
+    std::printf("-------------------------------------------------------------------------------\n");
+    std::printf("%s\n", title);
+    std::printf("-------------------------------------------------------------------------------\n");
+    std::printf(">    Iteration     |   Residual   |    Trend     |                             \n");
+    std::printf("-------------------------------------------------------------------------------\n");
+    FILE *f = nullptr;
+    bool first_write = false;
+    if (!S.out_file.empty()) {
+        FILE *probe = std::fopen(S.out_file.c_str(), "r");
+        first_write = probe == nullptr;
+        if (probe) std::fclose(probe);
+        f = std::fopen(S.out_file.c_str(), "a");
+        if (f && first_write) std::fprintf(f, "Iteration number|Solver residual|Solver convergence\n");
+    }
+    for (uint32_t it = 0; it < iters && it < hist_n; it++) {
+        double e = S.history[2 * it], t = S.history[2 * it + 1];
+        bool conv = std::isnan(e) || !std::isnormal(e) || !std::isnormal(t) || e < c.solver_cvg_value ||
+                    std::fabs(t) < kEps * kEps;
+        if (!conv && it % c.output_frequency != 0) continue;
+        std::printf("> %6u of %-6u |% 12e |% 12e |\n", it + 1, c.n_outer_iterations, e, t);
+        if (f) std::fprintf(f, "%u|%.7e|%.7e\n", it + 1, e, t);
+    }
+    if (f) std::fclose(f);
+    std::printf("-------------------------------------------------------------------------------\n");
+}
+
+struct StopFlags {
+    uint32_t it, done, reason, pad;
+};
+
+// Replay the captured iteration graph until the device-side stop flag is raised.  Two
+// replays are kept in flight so the host's flag check overlaps device work.
+static int run_graph_loop(rs_solver &S, uint32_t &done_out)
+{
+    hipStream_t st = runtime().stream;
+    StopFlags *hflags = nullptr;
+    RS_HIP(hipHostMalloc(reinterpret_cast<void **>(&hflags), 2 * sizeof(StopFlags), hipHostMallocDefault));
+    hipEvent_t ev[2];
+    RS_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    RS_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int rc = RS_OK;
+    auto enqueue = [&](int slot) -> int {
+        RS_HIP(hipGraphLaunch(S.graph, st));
+        RS_HIP(hipMemcpyAsync(&hflags[slot], &S.st.p->it, sizeof(StopFlags), hipMemcpyDeviceToHost, st));
+        RS_HIP(hipEventRecord(ev[slot], st));
+        return RS_OK;
+    };
+    uint64_t launched = 0;
+    rc = enqueue(0);
+    launched = 1;
+    while (rc == RS_OK) {
+        int slot = (int)(launched % 2);
+        if ((rc = enqueue(slot))) break;
+        launched++;
+        int prev = (int)((launched - 2) % 2);
+        hipError_t e = hipEventSynchronize(ev[prev]);
+        if (e != hipSuccess) { rc = hip_fail(e, "hipEventSynchronize"); break; }
+        if (hflags[prev].done != kStopRunning) {
+            done_out = hflags[prev].done;
+            break;
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    (void)hipHostFree(hflags);
+    return rc;
+}
+
+template <typename Args>
+static std::string graph_key_of(int kind, const Args &a, const RowShape &sh, uint32_t w, uint32_t vb)
+{
+    std::string k(reinterpret_cast<const char *>(&a), sizeof(Args));
+    uint32_t extra[6] = { (uint32_t)kind, sh.nb, sh.chunk, (uint32_t)sh.g, w, vb };
+    k.append(reinterpret_cast<const char *>(extra), sizeof(extra));
+    return k;
+}
+
+static int capture(rs_solver &S, const std::function<void()> &body, const std::string &key)
+{
+    if (S.graph && S.graph_key == key) return RS_OK;
+    if (S.graph) {
+        (void)hipGraphExecDestroy(S.graph);
+        S.graph = nullptr;
+    }
+    hipStream_t st = runtime().stream;
+    hipGraph_t g;
+    RS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    body();
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        S.graph = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    S.graph_key = key;
+    return RS_OK;
+}
+
+static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, uint32_t &done)
+{
+    RowShape sh = row_shape(A.n, lanes_for(A.nnz, A.n));
+    CgArgs a{};
+    a.A = view_of(A, sh);
+    a.b = S.y.p;
+    a.x = d_x;
+    a.r = S.r.p;
+    a.p = S.p.p;
+    a.q = S.q.p;
+    a.ptype = P.type;
+    a.pw = P.width ? P.width : 1;
+    a.pd = P.data.p;
+    a.partials = S.partials.p;
+    a.st = S.st.p;
+    a.hist = S.hist.p;
+    uint32_t w = a.pw;
+    uint32_t vb = vec_blocks(A.n / w);
+    RS_DISPATCH_G(sh.g, k_cg_init, sh.nb, a);
+    RS_DISPATCH_W(w, k_cg_rho, vb, a);
+    RS_HIP(hipGetLastError());
+    int rc = capture(S, [&] {
+        for (uint32_t i = 0; i < kGraphIters; i++) {
+            RS_DISPATCH_W(w, k_cg_dir, vb, a);
+            RS_DISPATCH_G(sh.g, k_cg_spmv_dot, sh.nb, a);
+            RS_DISPATCH_W(w, k_cg_update, vb, a);
+        }
+    }, graph_key_of(1, a, sh, w, vb));
+    if (rc) return rc;
+    return run_graph_loop(S, done);
+}
+
+static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, uint32_t &done,
+                          uint32_t &refinements)
+{
+    int rc;
+    if ((rc = matrix_refresh_fp32(A)) || (rc = precond_fp32(P))) return rc;
+    RowShape sh = row_shape(A.n, lanes_for(A.nnz, A.n));
+    MpArgs a{};
+    a.A = view_of(A, sh);
+    a.b = S.y.p;
+    a.x = d_x;
+    a.r = S.r32.p;
+    a.p = S.p32.p;
+    a.q = S.q32.p;
+    a.dx = S.dx32.p;
+    a.ptype = P.type;
+    a.pw = P.width ? P.width : 1;
+    a.pd = P.data32.p;
+    a.partials = S.partials.p;
+    a.st = S.st.p;
+    a.hist = S.hist.p;
+    uint32_t w = a.pw;
+    uint32_t vb = vec_blocks(A.n / w);
+    uint32_t nvb = vec_blocks(A.n);
+    hipStream_t stream = runtime().stream;
+    RS_HIP(hipMemsetAsync(S.dx32.p, 0, sizeof(float) * A.n, stream));
+    RS_DISPATCH_G(sh.g, k_mp_true_resid, sh.nb, a, 1);
+    RS_DISPATCH_W(w, k_mp_rho, vb, a, 1);
+    RS_HIP(hipGetLastError());
+    rc = capture(S, [&] {
+        for (uint32_t i = 0; i < kGraphIters; i++) {
+            RS_DISPATCH_W(w, k_mp_dir, vb, a);
+            RS_DISPATCH_G(sh.g, k_mp_spmv_dot, sh.nb, a);
+            RS_DISPATCH_W(w, k_mp_update, vb, a);
+        }
+    }, graph_key_of(2, a, sh, w, vb));
+    if (rc) return rc;
+    refinements = 0;
+    // the first true residual may already satisfy the tolerance
+    SolveState h;
+    RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    RS_HIP(hipStreamSynchronize(stream));
+    done = h.done;
+    double best = h.t2;
+    uint32_t no_gain = 0;
+    while (done == kStopRunning) {
+        if ((rc = run_graph_loop(S, done))) return rc;
+        if (done != kStopPause) break;
+        refinements++;
+        // residual replacements that stop lowering the FP64 residual mean the FP32 recursion
+        // has lost contact with the true residual: stop instead of spinning
+        if (no_gain >= kMaxReplacementsWithoutGain) {
+            k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
+            RS_HIP(hipGetLastError());
+            h.reason = RS_REASON_STAGNATED;
+            RS_HIP(hipMemcpyAsync(&S.st.p->reason, &h.reason, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+            RS_HIP(hipStreamSynchronize(stream));
+            return RS_OK;
+        }
+        k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
+        RS_DISPATCH_G(sh.g, k_mp_true_resid, sh.nb, a, 0);
+        RS_DISPATCH_W(w, k_mp_rho, vb, a, 0);
+        RS_HIP(hipGetLastError());
+        RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+        RS_HIP(hipStreamSynchronize(stream));
+        done = h.done;
+        if (h.t2 < 0.5 * best) {
+            best = h.t2;
+            no_gain = 0;
+        } else {
+            no_gain++;
+        }
+    }
+    // fold the last FP32 corrections into the FP64 solution
+    k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
+    RS_HIP(hipGetLastError());
+    return RS_OK;
+}
+
+static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, uint32_t &outer,
+                       uint32_t &inner_total)
+{
+    hipStream_t stream = runtime().stream;
+    const uint32_t n = A.n;
+    const uint32_t nouter = S.conf.n_outer_iterations, ninner = S.conf.n_inner_iterations;
+    if (ninner == 0) return fail(RS_ERR_INVALID_ARG, "GMRES: nInnerIterations is 0");
+    RowShape sh = row_shape(n, lanes_for(A.nnz, n));
+    CsrView v = view_of(A, sh);
+    uint32_t nvb = vec_blocks(n);
+    uint32_t w = P.width ? P.width : 1;
+    const uint64_t ld = n;
+    SolveState *dst = S.st.p;
+    SolveState h;
+    std::vector<double> H((size_t)(ninner + 1) * ninner), c(ninner + 1), s(ninner + 1), pp(ninner + 1), y(ninner + 1),
+        hv(ninner + 2);
+    auto Hm = [&](uint32_t i, uint32_t j) -> double & { return H[(size_t)i * ninner + j]; };
+    // bn = |b| (src/rmatrixsolver.cpp:375)
+    k_norm2<<<nvb, kBlock, 0, stream>>>(S.y.p, n, S.partials.p, dst, 0);
+    RS_HIP(hipGetLastError());
+    RS_HIP(hipMemcpyAsync(&h, dst, sizeof(h), hipMemcpyDeviceToHost, stream));
+    RS_HIP(hipStreamSynchronize(stream));
+    const double bn = h.t0, An = A.frob;
+    // multidot partial layout: (k+1) x nbm
+    uint32_t nbm = std::min<uint32_t>(nvb, 512);
+    DBuf<double> mpart;
+    int rc = mpart.alloc((size_t)(ninner + 1) * nbm);
+    if (rc) return rc;
+    outer = 0;
+    inner_total = 0;
+    for (uint32_t ito = 0; ito < nouter; ito++) {
+        std::fill(H.begin(), H.end(), 0.0);
+        RS_DISPATCH_G(sh.g, k_gmres_resid, sh.nb, v, S.y.p, d_x, S.ro.p, S.partials.p, dst);
+        RS_HIP(hipGetLastError());
+        RS_HIP(hipMemcpyAsync(&h, dst, sizeof(h), hipMemcpyDeviceToHost, stream));
+        RS_HIP(hipStreamSynchronize(stream));
+        double xn = h.t0, beta = h.t1;
+        double norm = An * xn + bn;                         // src/rmatrixsolver.cpp:415-421
+        if (std::fabs(norm) < kEps) norm = kEps;
+        double err = S.conf.stop_rule == RS_STOP_RELRES ? beta / std::max(bn, kEps) : beta / norm;
+        S.trend = err - S.error;
+        S.error = err;
+        if (ito < S.history.size() / 2) {
+            S.history[2 * ito] = err;
+            S.history[2 * ito + 1] = S.trend;
+        }
+        outer = ito + 1;
+        bool conv = !std::isnormal(S.error) || !std::isnormal(S.trend) || S.error < S.conf.solver_cvg_value ||
+                    std::fabs(S.trend) < kEps * kEps;
+        if (conv) break;
+        k_scale_by<<<nvb, kBlock, 0, stream>>>(S.ro.p, S.V.p, n, beta);
+        std::fill(pp.begin(), pp.end(), 0.0);
+        pp[0] = beta;
+        uint32_t iti;
+        for (iti = 0; iti < ninner; iti++) {
+            inner_total++;
+            double *vi = S.V.p + (uint64_t)iti * ld, *zi = S.Z.p + (uint64_t)iti * ld;
+            RS_DISPATCH_W(w, k_pc_apply, vec_blocks(n / w), P.type, w, P.data.p, n, vi, zi);
+            RS_DISPATCH_G(sh.g, k_spmv, sh.nb, v, zi, S.w.p);
+            uint32_t cnt = iti + 1;
+            dim3 g(nbm, (cnt + 7) / 8);
+            k_multidot<<<g, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, mpart.p);
+            k_multidot_final<<<cnt, kBlock, 0, stream>>>(mpart.p, nbm, S.hbuf.p);
+            k_gs_subtract<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, S.hbuf.p, S.partials.p, dst);
+            k_normalize<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p + (uint64_t)(iti + 1) * ld, n, dst);
+            RS_HIP(hipGetLastError());
+            RS_HIP(hipMemcpyAsync(hv.data(), S.hbuf.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, stream));
+            RS_HIP(hipMemcpyAsync(&h, dst, sizeof(h), hipMemcpyDeviceToHost, stream));
+            RS_HIP(hipStreamSynchronize(stream));
+            for (uint32_t i = 0; i <= iti; i++) Hm(i, iti) = hv[i];
+            Hm(iti + 1, iti) = h.t0;
+            for (uint32_t i = 1; i <= iti; i++) {           // src/rmatrixsolver.cpp:521-544
+                double hsave = Hm(i - 1, iti);
+                Hm(i - 1, iti) = c[i - 1] * hsave + s[i - 1] * Hm(i, iti);
+                Hm(i, iti) = -s[i - 1] * hsave + c[i - 1] * Hm(i, iti);
+            }
+            double gg = std::sqrt(Hm(iti, iti) * Hm(iti, iti) + Hm(iti + 1, iti) * Hm(iti + 1, iti));
+            if (gg == 0.0) {
+                c[iti] = s[iti] = 0.0;
+            } else {
+                c[iti] = Hm(iti, iti) / gg;
+                s[iti] = Hm(iti + 1, iti) / gg;
+            }
+            Hm(iti, iti) = gg;
+            Hm(iti + 1, iti) = 0.0;
+            pp[iti + 1] = -s[iti] * pp[iti];
+            pp[iti] = c[iti] * pp[iti];
+            if (std::fabs(pp[iti + 1]) < beta * 1.0e-4) {   // src/rmatrixsolver.cpp:546-550
+                iti++;
+                break;
+            }
+        }
+        // back substitution (src/rmatrixsolver.cpp:557-566)
+        y[iti - 1] = (Hm(iti - 1, iti - 1) == 0.0) ? 0.0 : pp[iti - 1] / Hm(iti - 1, iti - 1);
+        for (int i = (int)iti - 2; i >= 0; i--) {
+            for (int j = (int)iti - 1; j > i; j--) pp[i] -= y[j] * Hm(i, j);
+            y[i] = (Hm(i, i) == 0.0) ? 0.0 : pp[i] / Hm(i, i);
+        }
+        RS_HIP(hipMemcpyAsync(S.hbuf.p, y.data(), sizeof(double) * iti, hipMemcpyHostToDevice, stream));
+        k_update_x<<<nvb, kBlock, 0, stream>>>(d_x, S.Z.p, ld, n, iti, S.hbuf.p);
+        RS_HIP(hipGetLastError());
+        RS_HIP(hipStreamSynchronize(stream));
+    }
+    return RS_OK;
+}
+
+static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d_x, int32_t ptype,
+                        uint32_t block_size, rs_info *info)
+{
+    const rs_conf &c = S.conf;
+    if (c.solver_type != RS_SOLVER_CG && c.solver_type != RS_SOLVER_GMRES)
+        return fail(RS_ERR_UNKNOWN_SOLVER, "Unknown matrix solver type '" + std::to_string(c.solver_type) + "'");
+    if (ptype < RS_PRECOND_NONE || ptype > RS_PRECOND_BLOCK_JACOBI)
+        return fail(RS_ERR_INVALID_PRECOND, "Invalid matrix preconditioner type '" + std::to_string(ptype) + "'");
+    bool mixed = c.precision == RS_PRECISION_MIXED;
+    bool gmres = c.solver_type == RS_SOLVER_GMRES;
+    if (mixed && gmres) return fail(RS_ERR_UNSUPPORTED, "mixed precision is implemented for CG only");
+    int rc;
+    // RMatrixPreconditioner P(A, type, blockSize)  (src/rmatrixsolver.cpp:160)
+    rs_precond P;
+    if ((rc = precond_build(A, ptype, block_size, P))) return rc;
+    if ((rc = ensure_ws(S, A.n, mixed, gmres))) return rc;
+    hipStream_t stream = runtime().stream;
+    hipEvent_t e0, e1;
+    RS_HIP(hipEventCreate(&e0));
+    RS_HIP(hipEventCreate(&e1));
+    RS_HIP(hipEventRecord(e0, stream));
+    const uint32_t n = A.n;
+    uint32_t nvb = vec_blocks(std::max(n, 1u));
+    uint32_t hist_cap = (uint32_t)(S.hist.n / 2);
+    k_state_init<<<1, 64, 0, stream>>>(S.st.p, A.frob, c.solver_cvg_value, S.error, c.n_outer_iterations,
+                                       mixed ? RS_STOP_RELRES : (uint32_t)c.stop_rule, hist_cap, mixed ? 1 : 0);
+    // equation scaling (src/rmatrixsolver.cpp:164-184)
+    k_norm2<<<nvb, kBlock, 0, stream>>>(d_b, n, S.partials.p, S.st.p, 0);
+    k_scale_from_norm<<<1, 64, 0, stream>>>(S.st.p);
+    k_apply_scale<<<nvb, kBlock, 0, stream>>>(d_b, S.y.p, d_x, n, S.st.p);
+    RS_HIP(hipGetLastError());
+    uint32_t done = 0, iters = 0, inner = 0, refinements = 0;
+    S.history.assign(2ull * std::min(hist_cap, std::max(1u, c.n_outer_iterations)), 0.0);
+    if (n > 0) {
+        if (gmres) rc = solve_gmres(S, A, P, d_x, iters, inner);
+        else if (mixed) rc = solve_cg_mixed(S, A, P, d_x, done, refinements);
+        else rc = solve_cg64(S, A, P, d_x, done);
+        if (rc) return rc;
+    }
+    k_unscale<<<nvb, kBlock, 0, stream>>>(d_x, n, S.st.p);
+    RS_HIP(hipGetLastError());
+    RS_HIP(hipEventRecord(e1, stream));
+    RS_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    RS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SolveState h;
+    RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    RS_HIP(hipStreamSynchronize(stream));
+    int reason;
+    double equation_scale = h.scale;
+    if (!gmres) {
+        iters = h.it;
+        S.error = h.error;
+        S.trend = h.trend;
+        reason = (int)h.reason;
+        if (!S.history.empty() && iters)
+            RS_HIP(hipMemcpy(S.history.data(), S.hist.p, sizeof(double) * 2 * std::min<uint64_t>(iters, S.history.size() / 2),
+                             hipMemcpyDeviceToHost));
+    } else {
+        reason = RS_REASON_MAXIT;
+        bool conv = !std::isnormal(S.error) || !std::isnormal(S.trend) || S.error < c.solver_cvg_value ||
+                    std::fabs(S.trend) < kEps * kEps;
+        if (conv) {
+            if (!std::isnormal(S.error) || !std::isnormal(S.trend)) reason = RS_REASON_NONNORMAL;
+            else if (S.error < c.solver_cvg_value) reason = RS_REASON_CONVERGED;
+            else reason = RS_REASON_STAGNATED;
+        }
+    }
+    if (n == 0) iters = 0;
+    S.last_iterations = iters;
+    // true residual on the caller's (unscaled) system, FP64
+    double true_relres = 0.0, b_norm = 0.0;
+    if (n > 0) {
+        RowShape sh = row_shape(n, lanes_for(A.nnz, n));
+        CsrView v = view_of(A, sh);
+        RS_DISPATCH_G(sh.g, k_resid_norm, sh.nb, v, d_b, d_x, S.partials.p, S.st.p);
+        RS_HIP(hipGetLastError());
+        RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+        RS_HIP(hipStreamSynchronize(stream));
+        true_relres = h.t1 > 0.0 ? h.t0 / h.t1 : h.t0;
+        b_norm = h.t1;
+    }
+    emit_log(S, gmres ? "GMRES" : "CG", iters, (uint32_t)(S.history.size() / 2));
+    if (info) {
+        info->iterations = iters;
+        info->inner_iterations = gmres ? inner : refinements;
+        info->reason = reason;
+        info->converged = reason == RS_REASON_CONVERGED;
+        info->error = S.error;
+        info->trend = S.trend;
+        info->equation_scale = equation_scale;
+        info->a_norm = A.frob;
+        info->b_norm = b_norm;
+        info->true_relres = true_relres;
+        info->solve_ms = ms;
+    }
+    return RS_OK;
+}
+
+}  // namespace rs
+
+extern "C" int rs_solver_create(const rs_conf *conf, rs_solver **out)
+{
+    if (!conf || !out) return fail(RS_ERR_INVALID_ARG, "rs_solver_create: null argument");
+    rs_solver *S = new rs_solver();
+    S->conf = *conf;
+    if (conf->output_file_name) S->out_file = conf->output_file_name;
+    S->conf.output_file_name = nullptr;
+    *out = S;
+    return RS_OK;
+}
+
+extern "C" int rs_solver_copy(const rs_solver *src, rs_solver **out)
+{
+    if (!src || !out) return fail(RS_ERR_INVALID_ARG, "rs_solver_copy: null argument");
+    rs_solver *S = new rs_solver();
+    S->conf = src->conf;
+    S->out_file = src->out_file;
+    S->error = src->error;
+    S->trend = src->trend;
+    *out = S;
+    return RS_OK;
+}
+
+extern "C" int rs_solver_disable_convergence_log(rs_solver *S)
+{
+    if (!S) return fail(RS_ERR_INVALID_ARG, "rs_solver_disable_convergence_log: null solver");
+    S->out_file.clear();
+    return RS_OK;
+}
+
+extern "C" int rs_solver_solve_device(rs_solver *S, rs_matrix *A, const double *d_b, double *d_x,
+                                      int32_t precond_type, uint32_t block_size, rs_info *info)
+{
+    if (!S || !A || (A->n && (!d_b || !d_x))) return fail(RS_ERR_INVALID_ARG, "rs_solver_solve_device: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    return solve_device(*S, *A, d_b, d_x, precond_type, block_size, info);
+}
+
+extern "C" int rs_solver_solve(rs_solver *S, rs_matrix *A, const double *b, double *x,
+                               int32_t precond_type, uint32_t block_size, rs_info *info)
+{
+    if (!S || !A || (A->n && (!b || !x))) return fail(RS_ERR_INVALID_ARG, "rs_solver_solve: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    DBuf<double> db, dx;
+    if ((rc = db.alloc(A->n)) || (rc = dx.alloc(A->n))) return rc;
+    hipStream_t st = runtime().stream;
+    if (A->n) {
+        RS_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * A->n, hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * A->n, hipMemcpyHostToDevice, st));
+    }
+    rc = solve_device(*S, *A, db.p, dx.p, precond_type, block_size, info);
+    if (rc) return rc;
+    if (A->n) {
+        RS_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * A->n, hipMemcpyDeviceToHost, st));
+        RS_HIP(hipStreamSynchronize(st));
+    }
+    return RS_OK;
+}
+
+extern "C" int rs_solver_history(const rs_solver *S, double *out, uint32_t cap, uint32_t *count)
+{
+    if (!S || !count) return fail(RS_ERR_INVALID_ARG, "rs_solver_history: null argument");
+    uint32_t n = std::min<uint32_t>(S->last_iterations, (uint32_t)(S->history.size() / 2));
+    *count = n;
+    if (out) std::memcpy(out, S->history.data(), sizeof(double) * 2 * std::min(n, cap));
+    return RS_OK;
+}
+
+extern "C" int rs_solver_destroy(rs_solver *S)
+{
+    delete S;
+    return RS_OK;
+}

@@ -1,0 +1,212 @@
+"""GPU parity tests of the sparse-solve path through the C ABI against the oracle.
+
+Bars: integer/index results bit-exact; FP64 results within the stated tolerances (the
+reductions sum in a different order from the reference's OpenMP reductions, so values agree
+to rounding, not bit for bit).  CG iteration counts must match the oracle within +-1.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from helpers import random_spd
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "tst_matrix_solver.json")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_reference_known_answers_through_c_abi(gpu):
+    rs = gpu
+    with open(GOLDEN) as f:
+        g = json.load(f)
+    c = g["conf"]
+    for case in g["cases"]:
+        _say(f"case {case['name']}")
+        r, cc, v = zip(*case["entries"])
+        A = sp.csr_matrix((v, (r, cc)), shape=(case["n"], case["n"]))
+        M = rs.SparseMatrix.from_scipy(A)
+        conf = rs.MatrixSolverConf(n_outer_iterations=c["n_outer_iterations"],
+                                   n_inner_iterations=c["n_inner_iterations"],
+                                   solver_cvg_value=c["solver_cvg_value"], output_frequency=0)
+        x = rs.RMatrixSolver(conf).solve(M, case["b"])  # default preconditioner NONE
+        assert x.shape == (case["n"],)
+        assert np.all(np.abs(x - np.array(case["x"])) < g["tolerance"])
+
+
+@pytest.mark.parametrize("n,density,seed", [(1, 1.0, 0), (37, 0.2, 1), (2000, 0.004, 2), (50000, 0.0002, 3)])
+def test_spmv_matches_scipy(gpu, n, density, seed):
+    _say(f"spmv n={n}")
+    rs = gpu
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=density, random_state=rng, format="csr")
+    if n > 10:
+        A = A.tolil()
+        A[n // 2, :] = 0  # an empty row
+        A = A.tocsr()
+    A.sort_indices()
+    M = rs.SparseMatrix.from_scipy(A)
+    x = rng.standard_normal(n)
+    y = M.spmv(x)
+    ref = A @ x
+    np.testing.assert_allclose(y, ref, rtol=1e-13, atol=1e-13 * (abs(ref).max() + 1))
+    assert M.frobenius == pytest.approx(np.sqrt((A.data ** 2).sum()), rel=1e-13)
+    rp, col, val = M.download()
+    assert np.array_equal(rp, A.indptr) and np.array_equal(col, A.indices) and np.array_equal(val, A.data)
+
+
+@pytest.mark.parametrize("ptype,block", [(1, 1), (2, 3), (2, 5)])
+def test_preconditioner_matches_oracle(gpu, orc, ptype, block):
+    _say(f"precond type={ptype} block={block}")
+    rs = gpu
+    A = random_spd(300, 0.02, seed=5)
+    M = rs.SparseMatrix.from_scipy(A)
+    P = rs.RMatrixPreconditioner(M, ptype, block)
+    data_ref, w_ref = orc.precond_build(A, ptype, block)
+    assert P.width == w_ref
+    np.testing.assert_allclose(P.data(), data_ref, rtol=1e-12, atol=1e-14)
+    x = np.random.default_rng(6).standard_normal(300)
+    np.testing.assert_allclose(P.compute(x), orc.precond_apply(ptype, 300, w_ref, data_ref, x),
+                               rtol=1e-12, atol=1e-14)
+
+
+def test_invalid_types_raise(gpu):
+    rs = gpu
+    M = rs.SparseMatrix.from_scipy(sp.eye(4).tocsr())
+    with pytest.raises(rs.RSError) as e:
+        rs.RMatrixPreconditioner(M, 7, 1)
+    assert e.value.code == 3
+    with pytest.raises(rs.RSError) as e:
+        rs.RMatrixSolver(rs.MatrixSolverConf(solver_type=9)).solve(M, np.ones(4))
+    assert e.value.code == 2
+    with pytest.raises(rs.RSError) as e:
+        rs.RMatrixSolver(rs.MatrixSolverConf()).solve(M, np.ones(4), None, 5, 1)
+    assert e.value.code == 3
+
+
+def _cg_case(rs, orc, A, b, ptype, block, cvg, n_outer=20000, stop_rule=0, x0=None):
+    M = rs.SparseMatrix.from_scipy(A)
+    conf = rs.MatrixSolverConf(n_outer_iterations=n_outer, solver_cvg_value=cvg, stop_rule=stop_rule)
+    S = rs.RMatrixSolver(conf)
+    x = S.solve(M, b, x0, ptype, block)
+    x_ref, info_ref, hist_ref = orc.solve(A, b, x0=x0, precond=ptype, block_size=block, n_outer=n_outer,
+                                          cvg=cvg, stop_rule=stop_rule, history=True)
+    return S, x, x_ref, info_ref, hist_ref
+
+
+@pytest.mark.parametrize("ptype,block", [(0, 1), (1, 1), (2, 3)])
+def test_cg_matches_oracle(gpu, orc, ptype, block):
+    _say(f"cg precond={ptype} block={block}")
+    rs = gpu
+    A = random_spd(3000, 0.002, seed=7)
+    b = np.random.default_rng(8).standard_normal(3000)
+    S, x, x_ref, info_ref, hist_ref = _cg_case(rs, orc, A, b, ptype, block, 1e-12)
+    it = S.info.iterations
+    assert abs(it - info_ref["iterations"]) <= 1, (it, info_ref["iterations"])
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-9
+    assert S.info.equation_scale == pytest.approx(info_ref["equation_scale"], rel=1e-12)
+    hist = S.history()
+    m = min(len(hist), len(hist_ref)) - 1
+    # the error histories agree while they are well above rounding
+    k = int(np.argmax(hist_ref[:, 0] < 1e-9)) or m
+    np.testing.assert_allclose(hist[:min(k, m), 0], hist_ref[:min(k, m), 0], rtol=1e-6)
+    assert S.info.converged == 1
+
+
+def test_cg_warm_start_and_error_carry(gpu, orc):
+    """x holds the initial guess; a second solve on the same solver object starts with the
+    previous error stored (RIterationInfo persists across solve calls of one RMatrixSolver)."""
+    rs = gpu
+    A = random_spd(500, 0.01, seed=9)
+    b = np.random.default_rng(10).standard_normal(500)
+    x0 = np.random.default_rng(11).standard_normal(500) * 0.1
+    M = rs.SparseMatrix.from_scipy(A)
+    S = rs.RMatrixSolver(rs.MatrixSolverConf(n_outer_iterations=5000, solver_cvg_value=1e-12))
+    x1 = S.solve(M, b, x0, 1, 1)
+    e1 = S.info.error
+    x_ref, info_ref = orc.solve(A, b, x0=x0, precond=1, n_outer=5000, cvg=1e-12)
+    assert abs(S.info.iterations - info_ref["iterations"]) <= 1
+    x2 = S.solve(M, b, x1, 1, 1)
+    x_ref2, info_ref2 = orc.solve(A, b, x0=x1, precond=1, n_outer=5000, cvg=1e-12, prev_error=e1)
+    assert abs(S.info.iterations - info_ref2["iterations"]) <= 1
+    np.testing.assert_allclose(x2, x_ref2, rtol=1e-9, atol=1e-12)
+
+
+def test_cg_maxit_and_zero_rhs(gpu, orc):
+    rs = gpu
+    A = random_spd(400, 0.01, seed=12)
+    b = np.random.default_rng(13).standard_normal(400)
+    S, x, x_ref, info_ref, _ = _cg_case(rs, orc, A, b, 1, 1, 1e-30, n_outer=7)
+    assert S.info.iterations == 7 == info_ref["iterations"]
+    assert S.info.reason == 4
+    np.testing.assert_allclose(x, x_ref, rtol=1e-10, atol=1e-14)
+    M = rs.SparseMatrix.from_scipy(A)
+    S2 = rs.RMatrixSolver(rs.MatrixSolverConf())
+    x0 = S2.solve(M, np.zeros(400), None, 1, 1)
+    assert np.all(x0 == 0.0) and S2.info.iterations == 1 and S2.info.equation_scale == 1.0
+
+
+@pytest.mark.parametrize("ptype", [0, 1, 2])
+def test_gmres_matches_oracle(gpu, orc, ptype):
+    _say(f"gmres precond={ptype}")
+    rs = gpu
+    rng = np.random.default_rng(14)
+    n = 1500
+    A = (sp.random(n, n, density=0.004, random_state=rng) + sp.eye(n) * 3.0).tocsr()
+    A.sort_indices()
+    b = rng.standard_normal(n)
+    M = rs.SparseMatrix.from_scipy(A)
+    conf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=100,
+                               n_inner_iterations=20, solver_cvg_value=1e-12)
+    S = rs.RMatrixSolver(conf)
+    x = S.solve(M, b, None, ptype, 3)
+    x_ref, info_ref = orc.solve(A, b, precond=ptype, block_size=3, solver=orc.SOLVER_GMRES,
+                                n_outer=100, n_inner=20, cvg=1e-12)
+    assert abs(S.info.iterations - info_ref["iterations"]) <= 1
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-8
+    assert S.info.true_relres < 1e-6
+
+
+def test_convergence_log_format(gpu, tmp_path):
+    """RIterationInfo::writeToFile: header line, then `it+1|error|trend` in %.7e."""
+    rs = gpu
+    A = random_spd(200, 0.02, seed=15)
+    b = np.ones(200)
+    path = tmp_path / "cvg.txt"
+    conf = rs.MatrixSolverConf(n_outer_iterations=1000, solver_cvg_value=1e-10, output_frequency=5,
+                               output_file_name=str(path))
+    S = rs.RMatrixSolver(conf)
+    S.solve(rs.SparseMatrix.from_scipy(A), b, None, 1, 1)
+    lines = path.read_text().splitlines()
+    assert lines[0] == "Iteration number|Solver residual|Solver convergence"
+    its = [int(l.split("|")[0]) for l in lines[1:]]
+    assert its[0] == 1 and all((i - 1) % 5 == 0 for i in its[:-1])
+    assert its[-1] == S.info.iterations
+    hist = S.history()
+    e = float(lines[-1].split("|")[1])
+    assert e == pytest.approx(hist[-1, 0], rel=1e-6)
+    S.disable_convergence_log_file()
+    S.solve(rs.SparseMatrix.from_scipy(A), b, None, 1, 1)
+    assert len(path.read_text().splitlines()) == len(lines)
+
+
+def test_mixed_precision_reaches_fp64_accuracy(gpu, orc):
+    rs = gpu
+    A = random_spd(5000, 0.0015, seed=16, diag_shift=0.05)
+    b = np.random.default_rng(17).standard_normal(5000)
+    M = rs.SparseMatrix.from_scipy(A)
+    conf = rs.MatrixSolverConf(n_outer_iterations=50000, solver_cvg_value=1e-8,
+                               stop_rule=rs.STOP_RELRES, precision=rs.PRECISION_MIXED)
+    S = rs.RMatrixSolver(conf)
+    x = S.solve(M, b, None, 1, 1)
+    assert S.info.converged == 1, S.info.as_dict()
+    assert S.info.true_relres <= 1e-8
+    x_star, _ = orc.solve(A, b, precond=1, n_outer=50000, cvg=1e-14, stop_rule=1)
+    assert np.linalg.norm(x - x_star) / np.linalg.norm(x_star) < 1e-6
+    assert S.info.inner_iterations >= 1  # FP64 residual replacements happened
