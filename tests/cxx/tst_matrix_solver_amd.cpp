@@ -1,0 +1,92 @@
+// Drop-in check of the C++ call surface: the known-answer cases of the reference's
+// tests/tst_matrix_solver.cpp:33-77 (CG, preconditioner NONE, cvg 1e-13, 1000 iterations,
+// tolerance 1e-6) through rsamd::RMatrixSolver, plus the error behaviour of an unknown
+// preconditioner type.  Exit status 0 when every check passes.
+#include <cmath>
+#include <cstdio>
+
+#include "rmatrixsolver_amd.hpp"
+
+using namespace rsamd;
+
+static int failures = 0;
+#define EXPECT(cond)                                                       \
+    do {                                                                   \
+        if (!(cond)) {                                                     \
+            std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);  \
+            failures++;                                                    \
+        }                                                                  \
+    } while (0)
+
+static RMatrixSolverConf cg_conf()
+{
+    RMatrixSolverConf conf(RMatrixSolverConf::CG);
+    conf.setNInnerIterations(1000);
+    conf.setNOuterIterations(1000);
+    conf.setSolverCvgValue(1.0e-13);
+    conf.setOutputFrequency(0);
+    return conf;
+}
+
+int main()
+{
+    const double tol = 1.0e-6;
+    {
+        RSparseMatrix a;
+        a.setNRows(2);
+        a.addValue(0, 0, 4.0);
+        a.addValue(0, 1, 1.0);
+        a.addValue(1, 0, 1.0);
+        a.addValue(1, 1, 3.0);
+        RRVector b(2, 0.0);
+        b[0] = 1.0;
+        b[1] = 2.0;
+        RRVector x;
+        RMatrixSolver solver(cg_conf());
+        solver.solve(a, b, x);
+        EXPECT(x.getNRows() == 2u);
+        EXPECT(std::fabs(x[0] - 1.0 / 11.0) < tol);
+        EXPECT(std::fabs(x[1] - 7.0 / 11.0) < tol);
+        std::printf("spd 2x2: x = [%.12f, %.12f] in %u iterations\n", x[0], x[1], solver.lastInfo().iterations);
+    }
+    {
+        RSparseMatrix a;
+        a.setNRows(3);
+        a.addValue(0, 0, 2.0);
+        a.addValue(1, 1, 4.0);
+        a.addValue(2, 2, 5.0);
+        RRVector b(3, 0.0);
+        b[0] = 6.0;
+        b[1] = 8.0;
+        b[2] = 10.0;
+        RRVector x;
+        RMatrixSolver solver(cg_conf());
+        solver.solve(a, b, x);
+        EXPECT(std::fabs(x[0] - 3.0) < tol);
+        EXPECT(std::fabs(x[1] - 2.0) < tol);
+        EXPECT(std::fabs(x[2] - 2.0) < tol);
+        // same object, same pattern, new values: the cached CSR takes the refresh path
+        a.addValue(0, 0, 2.0);  // A_00 = 4
+        RRVector y;
+        solver.solve(a, b, y, R_MATRIX_PRECONDITIONER_JACOBI, 1);
+        EXPECT(std::fabs(y[0] - 1.5) < tol);
+        // copy constructor keeps the configuration
+        RMatrixSolver copy(solver);
+        RRVector z;
+        copy.solve(a, b, z, R_MATRIX_PRECONDITIONER_BLOCK_JACOBI, 3);
+        EXPECT(std::fabs(z[2] - 2.0) < tol);
+        RMatrixPreconditioner P(a, R_MATRIX_PRECONDITIONER_JACOBI, 1);
+        RRVector pz;
+        P.compute(b, pz);
+        EXPECT(std::fabs(pz[1] - 2.0) < 1e-15);
+        bool threw = false;
+        try {
+            solver.solve(a, b, y, (RMatrixPreconditionerType)7, 1);
+        } catch (const RError &e) {
+            threw = e.getType() == RError::Type::Application;
+        }
+        EXPECT(threw);
+    }
+    std::printf("%s (%d failures)\n", failures ? "FAIL" : "PASS", failures);
+    return failures ? 1 : 0;
+}

@@ -210,3 +210,15 @@ def test_mixed_precision_reaches_fp64_accuracy(gpu, orc):
     x_star, _ = orc.solve(A, b, precond=1, n_outer=50000, cvg=1e-14, stop_rule=1)
     assert np.linalg.norm(x - x_star) / np.linalg.norm(x_star) < 1e-6
     assert S.info.inner_iterations >= 1  # FP64 residual replacements happened
+
+
+def test_cxx_drop_in_replays_reference_test(gpu):
+    """The C++ mirror (include/rmatrixsolver_amd.hpp) replays tst_matrix_solver.cpp's cases."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cxx", "tst_matrix_solver_amd")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.dirname(exe)], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASS" in r.stdout
