@@ -125,11 +125,20 @@ int rs_matrix_info(const rs_matrix *A, uint32_t *n_rows, uint64_t *nnz, double *
 int rs_matrix_download(const rs_matrix *A, uint32_t *row_ptr, uint32_t *col, double *val);
 int rs_matrix_spmv(rs_matrix *A, const double *x, double *y);               /* host vectors */
 int rs_matrix_spmv_device(rs_matrix *A, const double *d_x, double *d_y);
+/* SpMV shapes: 0..4 = vector CSR with 4/8/16/32/64 lanes per row; 5, 6 = LDS-staged tile CSR
+ * with <= 2048 / <= 4096 nonzeros per tile.  The library times every shape once per matrix
+ * and precision (autotuner) and uses the fastest. */
+#define RS_SPMV_SHAPES 7
 /* Measurement: average device time (ms, stream events) of `reps` back-to-back SpMV launches
- * (precision RS_PRECISION_FP64: FP64 values/vectors; RS_PRECISION_MIXED: FP32), with
- * `lanes` lanes per row (0 = the library's choice). */
-int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t reps, uint32_t lanes,
+ * (precision RS_PRECISION_FP64: FP64 values/vectors; RS_PRECISION_MIXED: FP32) of shape
+ * `plan` (-1 = the autotuned shape). */
+int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t reps, int32_t plan,
                          double *avg_ms);
+/* The autotuned shape for `precision` and (optional) the tuning time of every shape
+ * (RS_SPMV_SHAPES doubles, ms). */
+int rs_matrix_spmv_plan(rs_matrix *A, int32_t precision, int32_t *plan, double *plan_ms);
+/* Override the autotuner for `precision` (plan -1: tune again on next use). */
+int rs_matrix_force_plan(rs_matrix *A, int32_t precision, int32_t plan);
 int rs_matrix_destroy(rs_matrix *A);
 
 /* ---------------------------------------------------------------- preconditioner ----- */

@@ -28,6 +28,9 @@ PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2, 3
 SOLVER_CG, SOLVER_GMRES = 0, 1
 STOP_REFERENCE, STOP_RELRES = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+SPMV_SHAPES = 7
+SPMV_SHAPE_NAMES = ["vector-csr g4", "vector-csr g8", "vector-csr g16", "vector-csr g32",
+                    "vector-csr g64", "tile-csr 2048", "tile-csr 4096"]
 REASON_NAMES = {0: "running", 1: "converged", 2: "stagnated", 3: "nonnormal", 4: "maxit"}
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "UNKNOWN_SOLVER", 3: "INVALID_PRECOND",
@@ -79,7 +82,9 @@ _SIGS = {
     "rs_matrix_info": [_P, _u32p, C.POINTER(C.c_uint64), _f64p],
     "rs_matrix_download": [_P, _P, _P, _P], "rs_matrix_spmv": [_P, _P, _P],
     "rs_matrix_spmv_device": [_P, _P, _P], "rs_matrix_destroy": [_P],
-    "rs_matrix_spmv_timed": [_P, C.c_int32, C.c_uint32, C.c_uint32, _f64p],
+    "rs_matrix_spmv_timed": [_P, C.c_int32, C.c_uint32, C.c_int32, _f64p],
+    "rs_matrix_spmv_plan": [_P, C.c_int32, C.POINTER(C.c_int32), _P],
+    "rs_matrix_force_plan": [_P, C.c_int32, C.c_int32],
     "rs_precond_create": [_P, C.c_int32, C.c_uint32, C.POINTER(_P)],
     "rs_precond_compute": [_P, _P, _P], "rs_precond_width": [_P, _u32p],
     "rs_precond_download": [_P, _P], "rs_precond_destroy": [_P],
@@ -263,11 +268,18 @@ class SparseMatrix:
     def spmv_device(self, dx: DeviceArray, dy: DeviceArray) -> None:
         _check(_lib.rs_matrix_spmv_device(self.h, dx.ptr, dy.ptr))
 
-    def spmv_timed(self, precision: int = PRECISION_FP64, reps: int = 50, lanes: int = 0) -> float:
-        """Average device milliseconds of one SpMV launch (stream events)."""
+    def spmv_timed(self, precision: int = PRECISION_FP64, reps: int = 50, plan: int = -1) -> float:
+        """Average device milliseconds of one SpMV launch (stream events); plan -1 = tuned."""
         ms = C.c_double()
-        _check(_lib.rs_matrix_spmv_timed(self.h, precision, reps, lanes, C.byref(ms)))
+        _check(_lib.rs_matrix_spmv_timed(self.h, precision, reps, plan, C.byref(ms)))
         return ms.value
+
+    def spmv_plan(self, precision: int = PRECISION_FP64):
+        """(autotuned shape id, tuning ms per shape)."""
+        plan = C.c_int32()
+        ms = np.zeros(SPMV_SHAPES)
+        _check(_lib.rs_matrix_spmv_plan(self.h, precision, C.byref(plan), _ptr(ms)))
+        return plan.value, ms
 
     def __del__(self):
         if _lib is not None and not getattr(self, "_borrowed", True) and getattr(self, "h", None):

@@ -374,6 +374,7 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
         RS_HIP(hipMemcpyAsync(A.col.p, pat.col.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
         k_zero<<<vblocks(nnz), kBlock, 0, st>>>(A.val.p, nnz);
     }
+    if ((rc = matrix_set_tiles(A, pat.rowptr.data()))) return rc;
     RS_HIP(hipStreamSynchronize(st));
     *out = S.release();
     return RS_OK;

@@ -72,46 +72,6 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds)
     __syncthreads();
 }
 
-// Deterministic grid reduction.  Each workgroup publishes its NV partial sums at
-// partials[k*gridDim.x + blockIdx.x]; the last workgroup to take a ticket sums them in
-// workgroup order (independent of which workgroup finished last), resets the ticket and
-// returns true with the totals in tot[] of its thread 0.  Stores/loads of the partials and
-// the ticket are agent-scope atomics so the exchange is coherent across the XCD L2s.
-template <int NV, int BLOCK>
-__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials, uint32_t *ticket,
-                                            double (&tot)[NV])
-{
-    __shared__ double lds[NV * (BLOCK / kWave)];
-    __shared__ uint32_t s_last;
-    block_sum<NV, BLOCK>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++)
-            __hip_atomic_store(&partials[k * gridDim.x + blockIdx.x], v[k], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == gridDim.x - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (!s_last) return false;
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) acc[k] = 0.0;
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += BLOCK) {
-#pragma unroll
-        for (int k = 0; k < NV; k++)
-            acc[k] += __hip_atomic_load(&partials[k * gridDim.x + i], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<NV, BLOCK>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) tot[k] = acc[k];
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
-}
-
 // std::isnormal for doubles: finite, nonzero and not subnormal.
 __device__ __forceinline__ bool dev_isnormal(double x)
 {

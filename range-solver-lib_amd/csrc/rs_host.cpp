@@ -241,6 +241,22 @@ int build_pattern(uint32_t n_nodes, uint32_t n_elems, uint32_t npe, const uint32
     return RS_OK;
 }
 
+// ---------------------------------------------------------------------------- SpMV tiles
+std::vector<uint32_t> build_tiles(const uint32_t *rowptr, uint32_t n, uint32_t cap, uint32_t max_rows)
+{
+    std::vector<uint32_t> tiles;
+    tiles.reserve(n / std::max(1u, max_rows) + 2);
+    tiles.push_back(0);
+    uint32_t r = 0;
+    while (r < n) {
+        uint32_t start = r;
+        while (r < n && r - start < max_rows && rowptr[r + 1] - rowptr[start] <= cap) r++;
+        if (r == start) r++;  // one row longer than cap
+        tiles.push_back(r);
+    }
+    return tiles;
+}
+
 // ---------------------------------------------------------------------------- colouring
 // First-fit greedy colouring in element order: an element takes the lowest colour that
 // none of its nodes has seen yet.  Elements of one colour share no node, so one launch per

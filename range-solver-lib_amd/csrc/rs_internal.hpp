@@ -85,6 +85,10 @@ struct Pattern {
 int build_pattern(uint32_t n_nodes, uint32_t n_elems, uint32_t npe, const uint32_t *conn,
                   uint32_t dpn, const uint32_t *book, uint32_t n_dof, Pattern &pat);
 
+// SpMV tiles: consecutive rows grouped so that a tile has <= max_rows rows and <= cap
+// nonzeros (a single longer row forms its own tile).  Returns ntiles+1 start rows.
+std::vector<uint32_t> build_tiles(const uint32_t *rowptr, uint32_t n, uint32_t cap, uint32_t max_rows);
+
 // Greedy node-disjoint colouring; returns colour per element and the number of colours.
 int colour_elements(uint32_t n_nodes, uint32_t n_elems, uint32_t npe, const uint32_t *conn,
                     std::vector<uint16_t> &colour, uint32_t &n_colours);
@@ -101,10 +105,16 @@ struct Matrix {
     bool val32_valid = false;
     bool borrowed = false;      // owned by an assembly
     uint64_t version = 0;       // bumped on every value update
+    DBuf<uint32_t> tiles[2];    // SpMV tile starts for the two tile capacities
+    uint32_t ntiles[2] = { 0, 0 };
+    int plan[2] = { -1, -1 };   // autotuned SpMV shape for FP64 / FP32
+    double plan_ms[2][8] = {};  // tuning times per shape
 };
 
 int matrix_frobenius(Matrix &A);   // device reduction, sets A.frob
 int matrix_refresh_fp32(Matrix &A);
+int matrix_set_tiles(Matrix &A, const uint32_t *host_rowptr);
+int matrix_tune(Matrix &A, bool fp32);
 
 }  // namespace rs
 

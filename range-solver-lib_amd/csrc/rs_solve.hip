@@ -8,68 +8,38 @@
 //   RMatrixPreconditioner             src/rmatrixpreconditioner.cpp:16-190
 //   RIterationInfo                    src/riterationinfo.cpp:74-193
 //
-// Layout in HBM: CSR (uint32 row pointers, uint32 columns, FP64 values, optional FP32
-// copy), vectors as contiguous FP64 arrays, Jacobi as n inverse diagonal values, block
-// Jacobi as n*w values (row m holds row m of its block's inverse, as the reference).
+// Layout in HBM: CSR (uint32 row pointers, uint32 columns, FP64 values, FP32 copy on demand),
+// SpMV tile boundaries, vectors as contiguous arrays, Jacobi as n inverse diagonal values,
+// block Jacobi as n*w values (row m holds row m of its block's inverse, as the reference).
 //
-// One CG iteration is three kernels, each ending in a deterministic grid reduction whose
-// last workgroup updates the solver scalars in device memory:
-//   dir       p = P r + beta p                       (reads P, r, p; writes p)
-//   spmv_dot  q = A p, alpha = rho / (p.q)           (CSR stream + gather of p; writes q)
-//   update    x += alpha p, r -= alpha q, |x|, |r|, rho' = r.P r, stopping rule, beta
-// Iterations run as replays of a captured hipGraph of S iterations; every kernel returns at
-// once when the device-side stop flag is set, so the iteration count equals the reference's.
+// Reductions are deterministic two-step sums: every workgroup of a producer kernel stores
+// its partial sums in its own slots (plain stores), then a one-workgroup finalize kernel sums
+// the slots in workgroup order and updates the solver scalars.  The kernel boundary makes the
+// partials visible across the XCD L2s without per-workgroup release fences (an agent-scope
+// release writes back the issuing XCD's L2, which serialised 2048 workgroups per reduction).
+//
+// One CG iteration = 5 launches:
+//   dir        p = P r + beta p
+//   spmv_dot   q = A p, partial p.q                 (SpMV shape picked by the autotuner)
+//   fin DOT    alpha = rho / p.q
+//   update     x += alpha p, r -= alpha q, partial |x|^2, |r|^2, r.P r
+//   fin UPDATE error, trend, stopping rule, beta
+// S iterations are captured once in a hipGraph and replayed; every kernel returns at once
+// when the device-side stop flag is set, so the iteration count equals the reference's.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
 #include "rs_device.hpp"
 #include "rs_internal.hpp"
+#include "rs_spmv.hpp"
 
 namespace rs {
 
-// ============================================================================ launch shapes
-struct RowShape {
-    uint32_t nb;     // workgroups (multiple of 8)
-    uint32_t chunk;  // rows per workgroup (multiple of rows per pass)
-    int g;           // lanes per row
-};
-
-static uint32_t round_up(uint32_t a, uint32_t m) { return (a + m - 1) / m * m; }
-
-// Lanes per row: the smallest power of two >= average row length / 1.5, in [2, 64].
-static int lanes_for(uint64_t nnz, uint32_t n)
-{
-    double avg = n ? (double)nnz / n : 1.0;
-    int g = 2;
-    while (g < 64 && g * 1.5 < avg) g *= 2;
-    return g;
-}
-
-static RowShape row_shape(uint32_t n, int g)
-{
-    RowShape s;
-    s.g = g;
-    uint32_t rpp = kBlock / g;
-    uint32_t passes = (n + rpp - 1) / rpp;
-    uint32_t target = (uint32_t)std::max(8, runtime().n_cu * 8);
-    s.nb = round_up(std::max(1u, std::min(target, passes)), kXcd);
-    s.chunk = round_up((n + s.nb - 1) / s.nb, rpp);
-    if (s.chunk == 0) s.chunk = rpp;
-    return s;
-}
-
-static uint32_t vec_blocks(uint64_t items)
-{
-    uint64_t want = (items + kBlock - 1) / kBlock;
-    uint32_t target = (uint32_t)std::max(8, runtime().n_cu * 8);
-    return round_up((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, target)), kXcd);
-}
-
 // ============================================================================ device state
-// Solver scalars in device memory (written only by the last workgroup of a reduction).
 struct SolveState {
     double rho0, rho1, beta, alpha;
     double bn, an, error, trend;
@@ -77,119 +47,287 @@ struct SolveState {
     double t1, t2, t3, t4;
     uint32_t it, done, reason, n_outer;
     uint32_t stop_rule, hist_cap, mode, pad;
-    uint32_t ticket[8];
 };
 enum { kStopRunning = 0, kStopDone = 1, kStopPause = 2 };
 constexpr double kReliableDelta = 0.1;   // FP64 residual replacement when |r| drops 10x
 constexpr uint32_t kMaxReplacementsWithoutGain = 20;
 
-struct CsrView {
-    uint32_t n, chunk;
-    const uint32_t *rowptr, *col;
-    const double *val;
-    const float *val32;
+// Block sum of NV values, thread 0 stores them at partials[k*gridDim.x + blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void store_partials(double (&v)[NV], double *partials)
+{
+    __shared__ double lds[NV * (kBlock / kWave)];
+    block_sum<NV, kBlock>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) partials[k * gridDim.x + blockIdx.x] = v[k];
+    }
+}
+
+// ============================================================================ finalize
+enum FinOp {
+    FIN_NORM0 = 0,      // t0 = sqrt(s0)
+    FIN_NORM01,         // t0 = sqrt(s0), t1 = sqrt(s1)
+    FIN_SCALE,          // t0 = |b|, scale = 1e9 / |bn / an|   (src/rmatrixsolver.cpp:164-171)
+    FIN_CG_INIT,        // bn = sqrt(s0)
+    FIN_RHO,            // rho0 = s0
+    FIN_DOT,            // alpha = rho0 / clamp(p.q), rho1 = rho0       (:302-321)
+    FIN_UPDATE,         // stopping rule, beta                          (:333-352, :258-275)
+    FIN_MP_UPDATE,      // mixed: recursive residual, pause for replacement
+    FIN_MP_TRUE,        // mixed: FP64 true residual (flag: first)
+    FIN_MP_RHO,         // mixed: rho after a replacement (flag: first)
 };
 
-// ============================================================================ row kernels
-template <int G, typename VT, typename XT>
-__device__ __forceinline__ double row_dot(const uint32_t *__restrict__ rowptr,
-                                          const uint32_t *__restrict__ col,
-                                          const VT *__restrict__ val, const XT *__restrict__ x,
-                                          uint32_t r, int gl)
+__global__ __launch_bounds__(kBlock) void k_fin(const double *__restrict__ partials, uint32_t nparts,
+                                                int nv, int op, int flag, SolveState *st,
+                                                double *__restrict__ hist)
 {
-    uint32_t b = rowptr[r], e = rowptr[r + 1];
-    double s = 0.0;
-    for (uint32_t j = b + gl; j < e; j += G) s += (double)val[j] * (double)x[col[j]];
-    return group_sum<G>(s);
-}
-
-template <int G>
-__device__ __forceinline__ float row_dot_f32(const uint32_t *__restrict__ rowptr,
-                                             const uint32_t *__restrict__ col,
-                                             const float *__restrict__ val,
-                                             const float *__restrict__ x, uint32_t r, int gl)
-{
-    uint32_t b = rowptr[r], e = rowptr[r + 1];
-    float s = 0.0f;
-    for (uint32_t j = b + gl; j < e; j += G) s = fmaf(val[j], x[col[j]], s);
-    return group_sum<G>(s);
-}
-
-// Iterate the rows owned by this workgroup (contiguous XCD-aware chunk), G lanes per row.
-#define RS_ROW_LOOP(G, n, chunk)                                                       \
-    const uint32_t _lb = xcd_remap(blockIdx.x, gridDim.x);                             \
-    const int gl = threadIdx.x % (G);                                                  \
-    const uint32_t _r0 = _lb * (chunk);                                                \
-    const uint32_t _r1 = min((n), _r0 + (chunk));                                      \
-    for (uint32_t _base = _r0; _base < _r1; _base += kBlock / (G))                    \
-        for (uint32_t r = _base + threadIdx.x / (G); r < _r1; r = _r1)
-
-// y = A x
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_spmv(CsrView A, const double *__restrict__ x,
-                                                 double *__restrict__ y)
-{
-    RS_ROW_LOOP(G, A.n, A.chunk)
-    {
-        double s = row_dot<G>(A.rowptr, A.col, A.val, x, r, gl);
-        if (gl == 0) y[r] = s;
-    }
-}
-
-// y = A x with FP32 values and vectors
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_spmv32(CsrView A, const float *__restrict__ x,
-                                                   float *__restrict__ y)
-{
-    RS_ROW_LOOP(G, A.n, A.chunk)
-    {
-        float s = row_dot_f32<G>(A.rowptr, A.col, A.val32, x, r, gl);
-        if (gl == 0) y[r] = s;
-    }
-}
-
-// res = b - A x; t0 = |res|, t1 = |b| (used for the FP64 true residual)
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_resid_norm(CsrView A, const double *__restrict__ b,
-                                                       const double *__restrict__ x,
-                                                       double *partials, SolveState *st)
-{
-    double v[2] = { 0.0, 0.0 };
-    RS_ROW_LOOP(G, A.n, A.chunk)
-    {
-        double s = row_dot<G>(A.rowptr, A.col, A.val, x, r, gl);
-        if (gl == 0) {
-            double bi = b[r], ri = bi - s;
-            v[0] += ri * ri;
-            v[1] += bi * bi;
+    if ((op == FIN_DOT || op == FIN_UPDATE || op == FIN_MP_UPDATE) && st->done) return;
+    __shared__ double lds[4 * (kBlock / kWave)];
+    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+    for (uint32_t i = threadIdx.x; i < nparts; i += kBlock)
+        for (int k = 0; k < nv; k++) v[k] += partials[(uint64_t)k * nparts + i];
+    block_sum<4, kBlock>(v, lds);
+    if (threadIdx.x != 0) return;
+    const double eps = kEps;
+    switch (op) {
+        case FIN_NORM0: st->t0 = sqrt(v[0]); break;
+        case FIN_NORM01: st->t0 = sqrt(v[0]); st->t1 = sqrt(v[1]); break;
+        case FIN_SCALE: {
+            double bn = sqrt(v[0]), an = st->an;
+            st->t0 = bn;
+            st->scale = (bn != 0.0 && an != 0.0) ? 1.0e9 / fabs(bn / an) : 1.0;
+            break;
         }
-    }
-    double tot[2];
-    if (grid_reduce<2, kBlock>(v, partials, &st->ticket[0], tot) && threadIdx.x == 0) {
-        st->t0 = sqrt(tot[0]);
-        st->t1 = sqrt(tot[1]);
+        case FIN_CG_INIT: st->bn = sqrt(v[0]); break;
+        case FIN_RHO: st->rho0 = v[0]; break;
+        case FIN_DOT: {
+            double dot = v[0];
+            if (dot > 0.0) dot = fmax(dot, eps);
+            else if (dot < 0.0) dot = fmin(dot, -eps);
+            else dot = eps;
+            st->alpha = st->rho0 / dot;
+            st->rho1 = st->rho0;
+            break;
+        }
+        case FIN_UPDATE: {
+            double xn = sqrt(v[0]), rn = sqrt(v[1]);
+            double norm = st->an * xn + st->bn;
+            if (fabs(norm) < eps) norm = eps;
+            double err = st->stop_rule == RS_STOP_RELRES ? rn / fmax(st->bn, eps) : rn / norm;
+            double trend = err - st->error;
+            st->trend = trend;
+            st->error = err;
+            uint32_t it = st->it;
+            if (it < st->hist_cap) {
+                hist[2 * it] = err;
+                hist[2 * it + 1] = trend;
+            }
+            st->it = it + 1;
+            int reason = converged_reason(err, trend, st->cvg, eps);
+            if (reason != RS_REASON_RUNNING) {
+                st->reason = reason;
+                st->done = kStopDone;
+            } else if (it + 1 >= st->n_outer) {
+                st->reason = RS_REASON_MAXIT;
+                st->done = kStopDone;
+            } else {
+                st->beta = v[2] / st->rho1;
+                st->rho0 = v[2];
+            }
+            break;
+        }
+        case FIN_MP_UPDATE: {
+            double rn = sqrt(v[0]);
+            double err = rn / fmax(st->bn, eps);
+            double trend = err - st->error;
+            st->trend = trend;
+            st->error = err;
+            uint32_t it = st->it;
+            if (it < st->hist_cap) {
+                hist[2 * it] = err;
+                hist[2 * it + 1] = trend;
+            }
+            st->it = it + 1;
+            if (!dev_isnormal(err) || !isfinite(v[1])) {
+                st->reason = RS_REASON_NONNORMAL;
+                st->done = kStopDone;
+            } else if (it + 1 >= st->n_outer) {
+                st->reason = RS_REASON_MAXIT;
+                st->done = kStopDone;
+            } else {
+                st->beta = v[1] / st->rho1;
+                st->rho0 = v[1];
+                if (rn < kReliableDelta * st->rmax || err < st->cvg) st->done = kStopPause;
+            }
+            break;
+        }
+        case FIN_MP_TRUE: {
+            double rn = sqrt(v[0]);
+            if (flag) st->bn = sqrt(v[1]);
+            double err = rn / fmax(st->bn, eps);
+            st->rmax = rn;
+            st->t2 = err;
+            if (!flag) {
+                st->error = err;
+                uint32_t it = st->it;
+                if (it > 0 && it - 1 < st->hist_cap) hist[2 * (it - 1)] = err;
+            }
+            if (!dev_isnormal(err) && err != 0.0) {
+                st->reason = RS_REASON_NONNORMAL;
+                st->done = kStopDone;
+            } else if (err < st->cvg) {
+                st->reason = RS_REASON_CONVERGED;
+                st->done = kStopDone;
+            } else {
+                st->done = kStopRunning;
+            }
+            break;
+        }
+        case FIN_MP_RHO:
+            if (!flag) st->beta = v[0] / st->rho1;
+            st->rho0 = v[0];
+            break;
     }
 }
 
-// sum of squares of an array -> st->t0 (Frobenius norm / vector norm)
-__global__ __launch_bounds__(kBlock) void k_norm2(const double *__restrict__ a, uint64_t n,
-                                                  double *partials, SolveState *st, int slot)
+// ============================================================================ epilogues
+struct EpiStore64 {
+    double *y;
+    __device__ void row(uint32_t r, double s) { y[r] = s; }
+};
+struct EpiStore32 {
+    float *y;
+    __device__ void row(uint32_t r, float s) { y[r] = s; }
+};
+// res = b - A x : accumulate |res|^2, |b|^2
+struct EpiResidNorm {
+    const double *b;
+    double acc[2];
+    __device__ void row(uint32_t r, double s)
+    {
+        double bi = b[r], ri = bi - s;
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+    }
+};
+// r = b - A x, |b|^2  (CG start)
+struct EpiCgInit {
+    const double *b;
+    double *r;
+    double acc[1];
+    __device__ void row(uint32_t i, double s)
+    {
+        double bi = b[i];
+        r[i] = bi - s;
+        acc[0] += bi * bi;
+    }
+};
+// q = A p, p.q
+struct EpiDot64 {
+    const double *p;
+    double *q;
+    double acc[1];
+    __device__ void row(uint32_t r, double s)
+    {
+        q[r] = s;
+        acc[0] += p[r] * s;
+    }
+};
+struct EpiDot32 {
+    const float *p;
+    float *q;
+    double acc[1];
+    __device__ void row(uint32_t r, float s)
+    {
+        q[r] = s;
+        acc[0] += (double)p[r] * (double)s;
+    }
+};
+// r32 = (float)(b - A x) in FP64, |r|^2, |b|^2
+struct EpiTrueResid32 {
+    const double *b;
+    float *r;
+    double acc[2];
+    __device__ void row(uint32_t i, double s)
+    {
+        double bi = b[i], ri = bi - s;
+        r[i] = (float)ri;
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+    }
+};
+// ro = b - A x, |x|^2, |ro|^2  (GMRES outer residual)
+struct EpiGmresResid {
+    const double *b, *x;
+    double *ro;
+    double acc[2];
+    __device__ void row(uint32_t i, double s)
+    {
+        double xi = x[i], ri = b[i] - s;
+        ro[i] = ri;
+        acc[0] += xi * xi;
+        acc[1] += ri * ri;
+    }
+};
+
+// ============================================================================ SpMV kernels
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_spmv(CsrView A, const double *__restrict__ x, double *__restrict__ y)
+{
+    EpiStore64 e{ y };
+    Shape::template run<double>(A, A.val, x, e);
+}
+
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_spmv32(CsrView A, const float *__restrict__ x, float *__restrict__ y)
+{
+    EpiStore32 e{ y };
+    Shape::template run<float>(A, A.val32, x, e);
+}
+
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_resid_norm(CsrView A, const double *__restrict__ b,
+                                                       const double *__restrict__ x, double *partials)
+{
+    EpiResidNorm e{ b, { 0.0, 0.0 } };
+    Shape::template run<double>(A, A.val, x, e);
+    store_partials<2>(e.acc, partials);
+}
+
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_gmres_resid(CsrView A, const double *__restrict__ b,
+                                                        const double *__restrict__ x, double *__restrict__ ro,
+                                                        double *partials)
+{
+    EpiGmresResid e{ b, x, ro, { 0.0, 0.0 } };
+    Shape::template run<double>(A, A.val, x, e);
+    store_partials<2>(e.acc, partials);
+}
+
+// ============================================================================ misc vector kernels
+__global__ __launch_bounds__(kBlock) void k_norm2(const double *__restrict__ a, uint64_t n, double *partials)
 {
     double v[1] = { 0.0 };
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
         v[0] += a[i] * a[i];
-    double tot[1];
-    if (grid_reduce<1, kBlock>(v, partials, &st->ticket[0], tot) && threadIdx.x == 0) {
-        double s = sqrt(tot[0]);
-        if (slot == 0) st->t0 = s;
-        else st->t1 = s;
-    }
+    store_partials<1>(v, partials);
 }
 
 __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, uint64_t n)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
         b[i] = (float)a[i];
+}
+
+__global__ void k_fill(double *__restrict__ a, uint64_t n, double v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) a[i] = v;
+}
+
+__global__ void k_fill32(float *__restrict__ a, uint64_t n, float v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) a[i] = v;
 }
 
 // ============================================================================ preconditioner
@@ -266,7 +404,6 @@ __device__ __forceinline__ void pc_apply_block(int ptype, uint32_t w, const PT *
                                                uint64_t ib, const T (&r)[W == 0 ? 16 : W],
                                                T (&z)[W == 0 ? 16 : W])
 {
-    constexpr int WM = W == 0 ? 16 : W;
     const int ww = W == 0 ? (int)w : W;
     if (ptype == RS_PRECOND_BLOCK_JACOBI) {
         for (int k = 0; k < ww; k++) {
@@ -275,13 +412,13 @@ __device__ __forceinline__ void pc_apply_block(int ptype, uint32_t w, const PT *
             z[k] = s;
         }
     } else if (ptype == RS_PRECOND_JACOBI) {
-        for (int k = 0; k < ww && k < WM; k++) z[k] = (T)pd[ib * ww + k] * r[k];
+        for (int k = 0; k < ww; k++) z[k] = (T)pd[ib * ww + k] * r[k];
     } else {
-        for (int k = 0; k < ww && k < WM; k++) z[k] = r[k];
+        for (int k = 0; k < ww; k++) z[k] = r[k];
     }
 }
 
-// y = P x on host-visible request (RMatrixPreconditioner::compute)
+// y = P x (RMatrixPreconditioner::compute, and GMRES's z_k = P v_k)
 template <int W>
 __global__ void k_pc_apply(int ptype, uint32_t w, const double *__restrict__ pd, uint32_t n,
                            const double *__restrict__ x, double *__restrict__ y)
@@ -330,15 +467,6 @@ __global__ void k_state_init(SolveState *st, double an, double cvg, double prev_
     st->stop_rule = stop_rule;
     st->hist_cap = hist_cap;
     st->mode = mode;
-    for (int i = 0; i < 8; i++) st->ticket[i] = 0;
-}
-
-// equationScale = 1e9 / |bn/An| if both nonzero (src/rmatrixsolver.cpp:164-171)
-__global__ void k_scale_from_norm(SolveState *st)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double bn = st->t0, an = st->an;
-    st->scale = (bn != 0.0 && an != 0.0) ? 1.0e9 / fabs(bn / an) : 1.0;
 }
 
 // y = b*s ; x *= s   (src/rmatrixsolver.cpp:181-184)
@@ -360,26 +488,16 @@ __global__ void k_unscale(double *__restrict__ x, uint64_t n, const SolveState *
         x[i] *= s;
 }
 
-// r = b - A x, bn = |b|   (src/rmatrixsolver.cpp:236-246)
-template <int G>
+// r = b - A x, partial |b|^2   (src/rmatrixsolver.cpp:236-246)
+template <class Shape>
 __global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs a)
 {
-    double v[1] = { 0.0 };
-    RS_ROW_LOOP(G, a.A.n, a.A.chunk)
-    {
-        double s = row_dot<G>(a.A.rowptr, a.A.col, a.A.val, a.x, r, gl);
-        if (gl == 0) {
-            double bi = a.b[r];
-            a.r[r] = bi - s;
-            v[0] += bi * bi;
-        }
-    }
-    double tot[1];
-    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0)
-        a.st->bn = sqrt(tot[0]);
+    EpiCgInit e{ a.b, a.r, { 0.0 } };
+    Shape::template run<double>(a.A, a.A.val, a.x, e);
+    store_partials<1>(e.acc, a.partials);
 }
 
-// rho0 = r . P r for the first iteration (src/rmatrixsolver.cpp:258-267)
+// partial rho0 = r . P r for the first iteration (src/rmatrixsolver.cpp:258-267)
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_cg_rho(CgArgs a)
 {
@@ -393,9 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_rho(CgArgs a)
         pc_apply_block<W, double, double>(a.ptype, ww, a.pd, ib, r, z);
         for (uint32_t k = 0; k < ww; k++) v[0] += r[k] * z[k];
     }
-    double tot[1];
-    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0)
-        a.st->rho0 = tot[0];
+    store_partials<1>(v, a.partials);
 }
 
 // p = z (first iteration) or p*beta + z   (src/rmatrixsolver.cpp:278-289)
@@ -419,34 +535,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs a)
     }
 }
 
-// q = A p, dot = p.q -> alpha = rho0/dot, rho1 = rho0   (src/rmatrixsolver.cpp:295-320)
-template <int G>
+// q = A p, partial p.q   (src/rmatrixsolver.cpp:295-300)
+template <class Shape>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs a)
 {
     if (a.st->done) return;
-    double v[1] = { 0.0 };
-    RS_ROW_LOOP(G, a.A.n, a.A.chunk)
-    {
-        double s = row_dot<G>(a.A.rowptr, a.A.col, a.A.val, a.p, r, gl);
-        if (gl == 0) {
-            a.q[r] = s;
-            v[0] += a.p[r] * s;
-        }
-    }
-    double tot[1];
-    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0) {
-        double dot = tot[0];
-        const double eps = kEps;
-        if (dot > 0.0) dot = fmax(dot, eps);
-        else if (dot < 0.0) dot = fmin(dot, -eps);
-        else dot = eps;
-        a.st->alpha = a.st->rho0 / dot;
-        a.st->rho1 = a.st->rho0;
-    }
+    EpiDot64 e{ a.p, a.q, { 0.0 } };
+    Shape::template run<double>(a.A, a.A.val, a.p, e);
+    store_partials<1>(e.acc, a.partials);
 }
 
-// x += alpha p, r -= alpha q, norms, stopping rule, next rho and beta
-// (src/rmatrixsolver.cpp:324-352, next iteration's :258-275)
+// x += alpha p, r -= alpha q, partial |x|^2, |r|^2 and r.P r  (src/rmatrixsolver.cpp:324-331)
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
 {
@@ -472,34 +571,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
         pc_apply_block<W, double, double>(a.ptype, ww, a.pd, ib, r, z);
         for (uint32_t k = 0; k < ww; k++) v[2] += r[k] * z[k];
     }
-    double tot[3];
-    if (grid_reduce<3, kBlock>(v, a.partials, &st->ticket[0], tot) && threadIdx.x == 0) {
-        const double eps = kEps;
-        double xn = sqrt(tot[0]), rn = sqrt(tot[1]);
-        double norm = st->an * xn + st->bn;
-        if (fabs(norm) < eps) norm = eps;
-        double err = st->stop_rule == RS_STOP_RELRES ? rn / fmax(st->bn, eps) : rn / norm;
-        double trend = err - st->error;
-        st->trend = trend;
-        st->error = err;
-        uint32_t it = st->it;
-        if (it < st->hist_cap) {
-            a.hist[2 * it] = err;
-            a.hist[2 * it + 1] = trend;
-        }
-        st->it = it + 1;
-        int reason = converged_reason(err, trend, st->cvg, eps);
-        if (reason != RS_REASON_RUNNING) {
-            st->reason = reason;
-            st->done = kStopDone;
-        } else if (it + 1 >= st->n_outer) {
-            st->reason = RS_REASON_MAXIT;
-            st->done = kStopDone;
-        } else {
-            st->beta = tot[2] / st->rho1;
-            st->rho0 = tot[2];
-        }
-    }
+    store_partials<3>(v, a.partials);
 }
 
 // ============================================================================ CG (mixed)
@@ -541,29 +613,13 @@ __global__ __launch_bounds__(kBlock) void k_mp_dir(MpArgs a)
     }
 }
 
-template <int G>
+template <class Shape>
 __global__ __launch_bounds__(kBlock) void k_mp_spmv_dot(MpArgs a)
 {
     if (a.st->done) return;
-    double v[1] = { 0.0 };
-    RS_ROW_LOOP(G, a.A.n, a.A.chunk)
-    {
-        float s = row_dot_f32<G>(a.A.rowptr, a.A.col, a.A.val32, a.p, r, gl);
-        if (gl == 0) {
-            a.q[r] = s;
-            v[0] += (double)a.p[r] * (double)s;
-        }
-    }
-    double tot[1];
-    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0) {
-        double dot = tot[0];
-        const double eps = kEps;
-        if (dot > 0.0) dot = fmax(dot, eps);
-        else if (dot < 0.0) dot = fmin(dot, -eps);
-        else dot = eps;
-        a.st->alpha = a.st->rho0 / dot;
-        a.st->rho1 = a.st->rho0;
-    }
+    EpiDot32 e{ a.p, a.q, { 0.0 } };
+    Shape::template run<float>(a.A, a.A.val32, a.p, e);
+    store_partials<1>(e.acc, a.partials);
 }
 
 template <int W>
@@ -589,33 +645,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_update(MpArgs a)
         pc_apply_block<W, float, float>(a.ptype, ww, a.pd, ib, r, z);
         for (uint32_t k = 0; k < ww; k++) v[1] += (double)r[k] * (double)z[k];
     }
-    double tot[2];
-    if (grid_reduce<2, kBlock>(v, a.partials, &st->ticket[0], tot) && threadIdx.x == 0) {
-        const double eps = kEps;
-        double rn = sqrt(tot[0]);
-        double err = rn / fmax(st->bn, eps);
-        double trend = err - st->error;
-        st->trend = trend;
-        st->error = err;
-        uint32_t it = st->it;
-        if (it < st->hist_cap) {
-            a.hist[2 * it] = err;
-            a.hist[2 * it + 1] = trend;
-        }
-        st->it = it + 1;
-        bool bad = !dev_isnormal(err) || !isfinite(tot[1]);
-        if (bad) {
-            st->reason = RS_REASON_NONNORMAL;
-            st->done = kStopDone;
-        } else if (it + 1 >= st->n_outer) {
-            st->reason = RS_REASON_MAXIT;
-            st->done = kStopDone;
-        } else {
-            st->beta = tot[1] / st->rho1;
-            st->rho0 = tot[1];
-            if (rn < kReliableDelta * st->rmax || err < st->cvg) st->done = kStopPause;
-        }
-    }
+    store_partials<2>(v, a.partials);
 }
 
 // x += dx, dx = 0
@@ -627,50 +657,18 @@ __global__ void k_mp_fold(double *__restrict__ x, float *__restrict__ dx, uint64
     }
 }
 
-// r32 = b - A x in FP64; |r| decides convergence (first: also sets bn and rmax)
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_mp_true_resid(MpArgs a, int first)
+// r32 = b - A x in FP64, partial |r|^2, |b|^2
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_mp_true_resid(MpArgs a)
 {
-    double v[2] = { 0.0, 0.0 };
-    RS_ROW_LOOP(G, a.A.n, a.A.chunk)
-    {
-        double s = row_dot<G>(a.A.rowptr, a.A.col, a.A.val, a.x, r, gl);
-        if (gl == 0) {
-            double bi = a.b[r], ri = bi - s;
-            a.r[r] = (float)ri;
-            v[0] += ri * ri;
-            v[1] += bi * bi;
-        }
-    }
-    double tot[2];
-    SolveState *st = a.st;
-    if (grid_reduce<2, kBlock>(v, a.partials, &st->ticket[0], tot) && threadIdx.x == 0) {
-        const double eps = kEps;
-        double rn = sqrt(tot[0]);
-        if (first) st->bn = sqrt(tot[1]);
-        double err = rn / fmax(st->bn, eps);
-        st->rmax = rn;
-        st->t2 = err;     // last true relative residual
-        if (!first) {
-            st->error = err;
-            uint32_t it = st->it;
-            if (it > 0 && it - 1 < st->hist_cap) a.hist[2 * (it - 1)] = err;
-        }
-        if (!dev_isnormal(err) && err != 0.0) {
-            st->reason = RS_REASON_NONNORMAL;
-            st->done = kStopDone;
-        } else if (err < st->cvg) {
-            st->reason = RS_REASON_CONVERGED;
-            st->done = kStopDone;
-        } else {
-            st->done = kStopRunning;
-        }
-    }
+    EpiTrueResid32 e{ a.b, a.r, { 0.0, 0.0 } };
+    Shape::template run<double>(a.A, a.A.val, a.x, e);
+    store_partials<2>(e.acc, a.partials);
 }
 
-// rho = r . P r after a residual replacement; the next direction uses beta = rho/rho1
+// partial rho = r . P r after a residual replacement
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_mp_rho(MpArgs a, int first)
+__global__ __launch_bounds__(kBlock) void k_mp_rho(MpArgs a)
 {
     constexpr int WM = W == 0 ? 16 : W;
     const uint32_t ww = W == 0 ? a.pw : W;
@@ -682,15 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_rho(MpArgs a, int first)
         pc_apply_block<W, float, float>(a.ptype, ww, a.pd, ib, r, z);
         for (uint32_t k = 0; k < ww; k++) v[0] += (double)r[k] * (double)z[k];
     }
-    double tot[1];
-    if (grid_reduce<1, kBlock>(v, a.partials, &a.st->ticket[0], tot) && threadIdx.x == 0) {
-        if (first) {
-            a.st->rho0 = tot[0];
-        } else {
-            a.st->beta = tot[0] / a.st->rho1;
-            a.st->rho0 = tot[0];
-        }
-    }
+    store_partials<1>(v, a.partials);
 }
 
 // ============================================================================ GMRES kernels
@@ -728,12 +718,11 @@ __global__ __launch_bounds__(kBlock) void k_multidot_final(const double *partial
     if (threadIdx.x == 0) out[blockIdx.x] = v[0];
 }
 
-// w -= sum_i h_i v_i ; t0 = |w|  (classical Gram-Schmidt, src/rmatrixsolver.cpp:486-508)
+// w -= sum_i h_i v_i ; partial |w|^2  (classical Gram-Schmidt, src/rmatrixsolver.cpp:486-508)
 __global__ __launch_bounds__(kBlock) void k_gs_subtract(double *__restrict__ w,
                                                         const double *__restrict__ V, uint64_t ld,
                                                         uint32_t n, uint32_t cnt,
-                                                        const double *__restrict__ h,
-                                                        double *partials, SolveState *st)
+                                                        const double *__restrict__ h, double *partials)
 {
     double v[1] = { 0.0 };
     for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (uint64_t)gridDim.x * kBlock) {
@@ -742,8 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_subtract(double *__restrict__ w,
         w[j] = value;
         v[0] += value * value;
     }
-    double tot[1];
-    if (grid_reduce<1, kBlock>(v, partials, &st->ticket[0], tot) && threadIdx.x == 0) st->t0 = sqrt(tot[0]);
+    store_partials<1>(v, partials);
 }
 
 // v = (h == 0) ? 0 : w / h with h = st->t0
@@ -753,31 +741,6 @@ __global__ void k_normalize(const double *__restrict__ w, double *__restrict__ v
     double h = st->t0;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
         v[i] = (h == 0.0) ? 0.0 : w[i] / h;
-}
-
-// ro = b - A x; t0 = |x|, t1 = |ro|   (src/rmatrixsolver.cpp:401-412)
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_gmres_resid(CsrView A, const double *__restrict__ b,
-                                                        const double *__restrict__ x,
-                                                        double *__restrict__ ro, double *partials,
-                                                        SolveState *st)
-{
-    double v[2] = { 0.0, 0.0 };
-    RS_ROW_LOOP(G, A.n, A.chunk)
-    {
-        double s = row_dot<G>(A.rowptr, A.col, A.val, x, r, gl);
-        if (gl == 0) {
-            double xi = x[r], ri = b[r] - s;
-            ro[r] = ri;
-            v[0] += xi * xi;
-            v[1] += ri * ri;
-        }
-    }
-    double tot[2];
-    if (grid_reduce<2, kBlock>(v, partials, &st->ticket[0], tot) && threadIdx.x == 0) {
-        st->t0 = sqrt(tot[0]);
-        st->t1 = sqrt(tot[1]);
-    }
 }
 
 // v0 = ro / beta (0 if beta == 0)
@@ -798,19 +761,53 @@ __global__ void k_update_x(double *__restrict__ x, const double *__restrict__ Z,
     }
 }
 
-// ============================================================================ dispatch helpers
-#define RS_DISPATCH_G(g, KERNEL, grid, ...)                                                   \
-    do {                                                                                      \
-        hipStream_t _s = runtime().stream;                                                    \
-        switch (g) {                                                                          \
-            case 2: KERNEL<2><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                   \
-            case 4: KERNEL<4><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                   \
-            case 8: KERNEL<8><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                   \
-            case 16: KERNEL<16><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                 \
-            case 32: KERNEL<32><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                 \
-            default: KERNEL<64><<<grid, kBlock, 0, _s>>>(__VA_ARGS__); break;                 \
-        }                                                                                     \
-    } while (0)
+// ============================================================================ launch helpers
+static uint32_t round_up(uint32_t a, uint32_t m) { return (a + m - 1) / m * m; }
+
+static uint32_t target_blocks() { return (uint32_t)std::max(8, runtime().n_cu * 8); }
+
+static uint32_t vec_blocks(uint64_t items)
+{
+    uint64_t want = (items + kBlock - 1) / kBlock;
+    return round_up((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, target_blocks())), kXcd);
+}
+
+static const int kVecLanes[5] = { 4, 8, 16, 32, 64 };
+static const uint32_t kTileCap[2] = { 2048, 4096 };
+
+// Workgroups of plan `id` for matrix A, and the view its kernels read.
+static CsrView plan_view(const Matrix &A, int id, uint32_t &nb)
+{
+    CsrView v{};
+    v.n = A.n;
+    v.rowptr = A.rowptr.p;
+    v.col = A.col.p;
+    v.val = A.val.p;
+    v.val32 = A.val32.p;
+    if (id < 5) {
+        uint32_t rpp = kBlock / kVecLanes[id];
+        uint32_t passes = (A.n + rpp - 1) / rpp;
+        nb = round_up(std::max(1u, std::min(target_blocks(), passes)), kXcd);
+        v.chunk = round_up((A.n + nb - 1) / nb, rpp);
+        if (v.chunk == 0) v.chunk = rpp;
+    } else {
+        int t = id - 5;
+        v.tiles = A.tiles[t].p;
+        v.ntiles = A.ntiles[t];
+        nb = round_up(std::max(1u, std::min(target_blocks(), v.ntiles)), kXcd);
+        v.tpb = (v.ntiles + nb - 1) / nb;
+    }
+    return v;
+}
+
+// Plan before tuning: vector rows with lanes ~ row length.
+static int default_plan(const Matrix &A)
+{
+    double avg = A.n ? (double)A.nnz / A.n : 1.0;
+    int g = 0;
+    while (g < 4 && kVecLanes[g] * 1.5 < avg) g++;
+    return g;
+}
 
 #define RS_DISPATCH_W(w, KERNEL, grid, ...)                                                   \
     do {                                                                                      \
@@ -820,19 +817,12 @@ __global__ void k_update_x(double *__restrict__ x, const double *__restrict__ Z,
         else KERNEL<0><<<grid, kBlock, 0, _s>>>(__VA_ARGS__);                                 \
     } while (0)
 
-static CsrView view_of(const Matrix &A, const RowShape &sh)
+static void fin(const double *partials, uint32_t nparts, int nv, int op, int flag, SolveState *st, double *hist)
 {
-    CsrView v;
-    v.n = A.n;
-    v.chunk = sh.chunk;
-    v.rowptr = A.rowptr.p;
-    v.col = A.col.p;
-    v.val = A.val.p;
-    v.val32 = A.val32.p;
-    return v;
+    k_fin<<<1, kBlock, 0, runtime().stream>>>(partials, nparts, nv, op, flag, st, hist);
 }
 
-// Scratch for standalone reductions (matrix norm, SpMV requests).
+// Scratch for standalone reductions (matrix norm).
 struct Scratch {
     DBuf<double> partials;
     DBuf<SolveState> st;
@@ -842,29 +832,24 @@ static Scratch &scratch()
     static Scratch s;
     return s;
 }
-static int ensure_scratch(size_t nparts)
+static int ensure_scratch()
 {
     Scratch &s = scratch();
-    if (s.partials.n < nparts) {
-        int rc = s.partials.alloc(std::max<size_t>(nparts, 65536));
-        if (rc) return rc;
-    }
-    if (!s.st.p) {
-        int rc = s.st.alloc(1);
-        if (rc) return rc;
-        RS_HIP(hipMemsetAsync(s.st.p, 0, sizeof(SolveState), runtime().stream));
-    }
+    int rc;
+    if (!s.partials.p && (rc = s.partials.alloc(8 * 4096))) return rc;
+    if (!s.st.p && (rc = s.st.alloc(1))) return rc;
     return RS_OK;
 }
 
 int matrix_frobenius(Matrix &A)
 {
-    int rc = ensure_scratch(8 * 4096);
+    int rc = ensure_scratch();
     if (rc) return rc;
     Scratch &s = scratch();
     uint32_t nb = vec_blocks(A.nnz);
     hipStream_t st = runtime().stream;
-    k_norm2<<<nb, kBlock, 0, st>>>(A.val.p, A.nnz, s.partials.p, s.st.p, 0);
+    k_norm2<<<nb, kBlock, 0, st>>>(A.val.p, A.nnz, s.partials.p);
+    fin(s.partials.p, nb, 1, FIN_NORM0, 0, s.st.p, nullptr);
     RS_HIP(hipGetLastError());
     SolveState h;
     RS_HIP(hipMemcpyAsync(&h, s.st.p, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -878,9 +863,83 @@ int matrix_refresh_fp32(Matrix &A)
     if (A.val32_valid) return RS_OK;
     int rc = A.val32.alloc(A.nnz);
     if (rc) return rc;
-    k_to_f32<<<vec_blocks(A.nnz), kBlock, 0, runtime().stream>>>(A.val.p, A.val32.p, A.nnz);
+    if (A.nnz) k_to_f32<<<vec_blocks(A.nnz), kBlock, 0, runtime().stream>>>(A.val.p, A.val32.p, A.nnz);
     RS_HIP(hipGetLastError());
     A.val32_valid = true;
+    return RS_OK;
+}
+
+int matrix_set_tiles(Matrix &A, const uint32_t *host_rowptr)
+{
+    for (int t = 0; t < 2; t++) {
+        std::vector<uint32_t> tiles = build_tiles(host_rowptr, A.n, kTileCap[t], kTileRows);
+        int rc = A.tiles[t].alloc(tiles.size());
+        if (rc) return rc;
+        A.ntiles[t] = (uint32_t)tiles.size() - 1;
+        RS_HIP(hipMemcpyAsync(A.tiles[t].p, tiles.data(), sizeof(uint32_t) * tiles.size(), hipMemcpyHostToDevice,
+                              runtime().stream));
+    }
+    RS_HIP(hipStreamSynchronize(runtime().stream));
+    A.plan[0] = A.plan[1] = -1;
+    return RS_OK;
+}
+
+// Average device time of `reps` launches of SpMV plan `id`.
+static int time_plan(Matrix &A, int id, bool fp32, uint32_t reps, const void *x, void *y, double &ms)
+{
+    hipStream_t st = runtime().stream;
+    uint32_t nb = 0;
+    CsrView v = plan_view(A, id, nb);
+    auto launch = [&]() {
+        if (fp32) RS_SHAPE_SWITCH(id, k_spmv32, nb, st, v, (const float *)x, (float *)y);
+        else RS_SHAPE_SWITCH(id, k_spmv, nb, st, v, (const double *)x, (double *)y);
+    };
+    launch();
+    hipEvent_t e0, e1;
+    RS_HIP(hipEventCreate(&e0));
+    RS_HIP(hipEventCreate(&e1));
+    RS_HIP(hipEventRecord(e0, st));
+    for (uint32_t i = 0; i < reps; i++) launch();
+    RS_HIP(hipEventRecord(e1, st));
+    RS_HIP(hipEventSynchronize(e1));
+    RS_HIP(hipGetLastError());
+    float f = 0.0f;
+    RS_HIP(hipEventElapsedTime(&f, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ms = f / reps;
+    return RS_OK;
+}
+
+// The SpMV autotuner: times every shape once per matrix and precision and keeps the fastest
+// (the pattern of a matrix handle never changes and values do not affect the choice).
+int matrix_tune(Matrix &A, bool fp32)
+{
+    int slot = fp32 ? 1 : 0;
+    if (A.plan[slot] >= 0) return RS_OK;
+    if (A.n == 0 || A.tiles[0].p == nullptr) {
+        A.plan[slot] = default_plan(A);
+        return RS_OK;
+    }
+    int rc;
+    if (fp32 && (rc = matrix_refresh_fp32(A))) return rc;
+    DBuf<double> x, y;
+    if ((rc = x.alloc(A.n)) || (rc = y.alloc(A.n))) return rc;
+    k_fill<<<vec_blocks(A.n), kBlock, 0, runtime().stream>>>(x.p, A.n, 1.0);
+    if (fp32) k_fill32<<<vec_blocks(A.n), kBlock, 0, runtime().stream>>>((float *)x.p, A.n, 1.0f);
+    uint32_t reps = A.nnz > 4000000 ? 4 : 10;
+    double best = 1e30;
+    int best_id = default_plan(A);
+    for (int id = 0; id < kNumShapes; id++) {
+        double ms = 0.0;
+        if ((rc = time_plan(A, id, fp32, reps, x.p, y.p, ms))) return rc;
+        A.plan_ms[slot][id] = ms;
+        if (ms < best) {
+            best = ms;
+            best_id = id;
+        }
+    }
+    A.plan[slot] = best_id;
     return RS_OK;
 }
 
@@ -915,6 +974,7 @@ extern "C" int rs_matrix_create(uint32_t n_rows, const uint32_t *row_ptr, const 
         RS_HIP(hipMemcpyAsync(A->val.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
         if ((rc = matrix_frobenius(*A))) return rc;
     }
+    if ((rc = matrix_set_tiles(*A, row_ptr))) return rc;
     RS_HIP(hipStreamSynchronize(st));
     *out = A.release();
     return RS_OK;
@@ -977,9 +1037,10 @@ extern "C" int rs_matrix_spmv_device(rs_matrix *A, const double *d_x, double *d_
     int rc = require_runtime();
     if (rc) return rc;
     if (A->n == 0) return RS_OK;
-    RowShape sh = row_shape(A->n, lanes_for(A->nnz, A->n));
-    CsrView v = view_of(*A, sh);
-    RS_DISPATCH_G(sh.g, k_spmv, sh.nb, v, d_x, d_y);
+    if ((rc = matrix_tune(*A, false))) return rc;
+    uint32_t nb = 0;
+    CsrView v = plan_view(*A, A->plan[0], nb);
+    RS_SHAPE_SWITCH(A->plan[0], k_spmv, nb, runtime().stream, v, d_x, d_y);
     RS_HIP(hipGetLastError());
     return RS_OK;
 }
@@ -999,41 +1060,42 @@ extern "C" int rs_matrix_spmv(rs_matrix *A, const double *x, double *y)
     return RS_OK;
 }
 
-extern "C" int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t reps, uint32_t lanes,
-                                    double *avg_ms)
+extern "C" int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t reps, int32_t plan, double *avg_ms)
 {
-    if (!A || !avg_ms || reps == 0) return fail(RS_ERR_INVALID_ARG, "rs_matrix_spmv_timed: bad argument");
-    if (lanes && (lanes & (lanes - 1)) != 0) return fail(RS_ERR_INVALID_ARG, "rs_matrix_spmv_timed: lanes not a power of two");
+    if (!A || !avg_ms || reps == 0 || plan >= kNumShapes) return fail(RS_ERR_INVALID_ARG, "rs_matrix_spmv_timed: bad argument");
     int rc = require_runtime();
     if (rc) return rc;
-    if (A->n == 0) { *avg_ms = 0.0; return RS_OK; }
-    hipStream_t st = runtime().stream;
-    int g = lanes ? (int)lanes : lanes_for(A->nnz, A->n);
-    RowShape sh = row_shape(A->n, g);
+    if (A->n == 0) {
+        *avg_ms = 0.0;
+        return RS_OK;
+    }
+    bool fp32 = precision == RS_PRECISION_MIXED;
+    if ((rc = matrix_tune(*A, fp32))) return rc;
+    int id = plan < 0 ? A->plan[fp32 ? 1 : 0] : plan;
     DBuf<double> x, y;
     if ((rc = x.alloc(A->n)) || (rc = y.alloc(A->n))) return rc;
-    RS_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * A->n, st));
-    if (precision == RS_PRECISION_MIXED && (rc = matrix_refresh_fp32(*A))) return rc;
-    CsrView v = view_of(*A, sh);
-    float *x32 = reinterpret_cast<float *>(x.p), *y32 = reinterpret_cast<float *>(y.p);
-    auto launch = [&]() {
-        if (precision == RS_PRECISION_MIXED) RS_DISPATCH_G(sh.g, k_spmv32, sh.nb, v, x32, y32);
-        else RS_DISPATCH_G(sh.g, k_spmv, sh.nb, v, x.p, y.p);
-    };
-    launch();  // warm
-    hipEvent_t e0, e1;
-    RS_HIP(hipEventCreate(&e0));
-    RS_HIP(hipEventCreate(&e1));
-    RS_HIP(hipEventRecord(e0, st));
-    for (uint32_t i = 0; i < reps; i++) launch();
-    RS_HIP(hipEventRecord(e1, st));
-    RS_HIP(hipEventSynchronize(e1));
-    RS_HIP(hipGetLastError());
-    float ms = 0.0f;
-    RS_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *avg_ms = ms / reps;
+    k_fill<<<vec_blocks(A->n), kBlock, 0, runtime().stream>>>(x.p, A->n, 1.0);
+    if (fp32) k_fill32<<<vec_blocks(A->n), kBlock, 0, runtime().stream>>>((float *)x.p, A->n, 1.0f);
+    return time_plan(*A, id, fp32, reps, x.p, y.p, *avg_ms);
+}
+
+extern "C" int rs_matrix_spmv_plan(rs_matrix *A, int32_t precision, int32_t *plan, double *plan_ms)
+{
+    if (!A || !plan) return fail(RS_ERR_INVALID_ARG, "rs_matrix_spmv_plan: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    bool fp32 = precision == RS_PRECISION_MIXED;
+    if ((rc = matrix_tune(*A, fp32))) return rc;
+    *plan = A->plan[fp32 ? 1 : 0];
+    if (plan_ms)
+        for (int i = 0; i < kNumShapes; i++) plan_ms[i] = A->plan_ms[fp32 ? 1 : 0][i];
+    return RS_OK;
+}
+
+extern "C" int rs_matrix_force_plan(rs_matrix *A, int32_t precision, int32_t plan)
+{
+    if (!A || plan < -1 || plan >= kNumShapes) return fail(RS_ERR_INVALID_ARG, "rs_matrix_force_plan: bad argument");
+    A->plan[precision == RS_PRECISION_MIXED ? 1 : 0] = plan;
     return RS_OK;
 }
 
@@ -1060,8 +1122,8 @@ static int precond_build(const Matrix &A, int32_t type, uint32_t block_size, rs_
     P.type = type;
     P.n = A.n;
     P.data32_valid = false;
-    RowShape sh = row_shape(std::max(A.n, 1u), 2);
-    CsrView v = view_of(A, sh);
+    uint32_t nb_unused = 0;
+    CsrView v = plan_view(A, 0, nb_unused);
     hipStream_t st = runtime().stream;
     int rc;
     if (type == RS_PRECOND_NONE) {
@@ -1168,14 +1230,16 @@ struct rs_solver {
     uint32_t last_iterations = 0;
     std::vector<double> history;
     // device workspace
-    uint32_t ws_n = 0;
     DBuf<double> y, r, p, q, ro, w, V, Z, hbuf;
     DBuf<float> r32, p32, q32, dx32;
     DBuf<double> partials, hist;
     DBuf<SolveState> st;
-    // captured iteration graph
+    // captured iteration graph (or, with RS_NO_GRAPH=1 in the environment, the same S
+    // iterations launched one kernel at a time)
     hipGraphExec_t graph = nullptr;
     std::string graph_key;   // raw bytes of every kernel argument and launch shape captured
+    std::function<void()> body;
+    bool eager = false;
     ~rs_solver()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -1191,7 +1255,7 @@ static int ensure_ws(rs_solver &S, uint32_t n, bool mixed, bool gmres)
 {
     int rc;
     size_t N = std::max(n, 1u);
-    if ((rc = S.y.alloc(N)) || (rc = S.partials.alloc(8 * 4096 * 4)) || (rc = S.st.alloc(1))) return rc;
+    if ((rc = S.y.alloc(N)) || (rc = S.partials.alloc(4 * 4096)) || (rc = S.st.alloc(1))) return rc;
     uint32_t hc = std::min(kHistCap, std::max(1u, S.conf.n_outer_iterations));
     if ((rc = S.hist.alloc(2ull * hc))) return rc;
     if (!mixed && !gmres) {
@@ -1206,7 +1270,6 @@ static int ensure_ws(rs_solver &S, uint32_t n, bool mixed, bool gmres)
             (rc = S.Z.alloc((k + 1) * N)) || (rc = S.hbuf.alloc(2 * (k + 2))))
             return rc;
     }
-    S.ws_n = n;
     return RS_OK;
 }
 
@@ -1232,8 +1295,7 @@ static void emit_log(const rs_solver &S, const char *title, uint32_t iters, uint
     }
     for (uint32_t it = 0; it < iters && it < hist_n; it++) {
         double e = S.history[2 * it], t = S.history[2 * it + 1];
-        bool conv = std::isnan(e) || !std::isnormal(e) || !std::isnormal(t) || e < c.solver_cvg_value ||
-                    std::fabs(t) < kEps * kEps;
+        bool conv = !std::isnormal(e) || !std::isnormal(t) || e < c.solver_cvg_value || std::fabs(t) < kEps * kEps;
         if (!conv && it % c.output_frequency != 0) continue;
         std::printf("> %6u of %-6u |% 12e |% 12e |\n", it + 1, c.n_outer_iterations, e, t);
         if (f) std::fprintf(f, "%u|%.7e|%.7e\n", it + 1, e, t);
@@ -1258,7 +1320,12 @@ static int run_graph_loop(rs_solver &S, uint32_t &done_out)
     RS_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int rc = RS_OK;
     auto enqueue = [&](int slot) -> int {
-        RS_HIP(hipGraphLaunch(S.graph, st));
+        if (S.eager) {
+            S.body();
+            RS_HIP(hipGetLastError());
+        } else {
+            RS_HIP(hipGraphLaunch(S.graph, st));
+        }
         RS_HIP(hipMemcpyAsync(&hflags[slot], &S.st.p->it, sizeof(StopFlags), hipMemcpyDeviceToHost, st));
         RS_HIP(hipEventRecord(ev[slot], st));
         return RS_OK;
@@ -1272,7 +1339,10 @@ static int run_graph_loop(rs_solver &S, uint32_t &done_out)
         launched++;
         int prev = (int)((launched - 2) % 2);
         hipError_t e = hipEventSynchronize(ev[prev]);
-        if (e != hipSuccess) { rc = hip_fail(e, "hipEventSynchronize"); break; }
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "hipEventSynchronize");
+            break;
+        }
         if (hflags[prev].done != kStopRunning) {
             done_out = hflags[prev].done;
             break;
@@ -1286,16 +1356,20 @@ static int run_graph_loop(rs_solver &S, uint32_t &done_out)
 }
 
 template <typename Args>
-static std::string graph_key_of(int kind, const Args &a, const RowShape &sh, uint32_t w, uint32_t vb)
+static std::string graph_key_of(int kind, const Args &a, int plan, uint32_t nb, uint32_t w, uint32_t vb)
 {
     std::string k(reinterpret_cast<const char *>(&a), sizeof(Args));
-    uint32_t extra[6] = { (uint32_t)kind, sh.nb, sh.chunk, (uint32_t)sh.g, w, vb };
+    uint32_t extra[5] = { (uint32_t)kind, (uint32_t)plan, nb, w, vb };
     k.append(reinterpret_cast<const char *>(extra), sizeof(extra));
     return k;
 }
 
 static int capture(rs_solver &S, const std::function<void()> &body, const std::string &key)
 {
+    S.body = body;
+    const char *ng = std::getenv("RS_NO_GRAPH");
+    S.eager = ng && ng[0] == '1';
+    if (S.eager) return RS_OK;
     if (S.graph && S.graph_key == key) return RS_OK;
     if (S.graph) {
         (void)hipGraphExecDestroy(S.graph);
@@ -1319,9 +1393,12 @@ static int capture(rs_solver &S, const std::function<void()> &body, const std::s
 
 static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, uint32_t &done)
 {
-    RowShape sh = row_shape(A.n, lanes_for(A.nnz, A.n));
+    int rc = matrix_tune(A, false);
+    if (rc) return rc;
+    const int plan = A.plan[0];
+    uint32_t nb = 0;
     CgArgs a{};
-    a.A = view_of(A, sh);
+    a.A = plan_view(A, plan, nb);
     a.b = S.y.p;
     a.x = d_x;
     a.r = S.r.p;
@@ -1335,16 +1412,21 @@ static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, ui
     a.hist = S.hist.p;
     uint32_t w = a.pw;
     uint32_t vb = vec_blocks(A.n / w);
-    RS_DISPATCH_G(sh.g, k_cg_init, sh.nb, a);
+    hipStream_t stream = runtime().stream;
+    RS_SHAPE_SWITCH(plan, k_cg_init, nb, stream, a);
+    fin(S.partials.p, nb, 1, FIN_CG_INIT, 0, S.st.p, S.hist.p);
     RS_DISPATCH_W(w, k_cg_rho, vb, a);
+    fin(S.partials.p, vb, 1, FIN_RHO, 0, S.st.p, S.hist.p);
     RS_HIP(hipGetLastError());
-    int rc = capture(S, [&] {
+    rc = capture(S, [&] {
         for (uint32_t i = 0; i < kGraphIters; i++) {
             RS_DISPATCH_W(w, k_cg_dir, vb, a);
-            RS_DISPATCH_G(sh.g, k_cg_spmv_dot, sh.nb, a);
+            RS_SHAPE_SWITCH(plan, k_cg_spmv_dot, nb, stream, a);
+            fin(S.partials.p, nb, 1, FIN_DOT, 0, S.st.p, S.hist.p);
             RS_DISPATCH_W(w, k_cg_update, vb, a);
+            fin(S.partials.p, vb, 3, FIN_UPDATE, 0, S.st.p, S.hist.p);
         }
-    }, graph_key_of(1, a, sh, w, vb));
+    }, graph_key_of(1, a, plan, nb, w, vb));
     if (rc) return rc;
     return run_graph_loop(S, done);
 }
@@ -1353,10 +1435,13 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
                           uint32_t &refinements)
 {
     int rc;
-    if ((rc = matrix_refresh_fp32(A)) || (rc = precond_fp32(P))) return rc;
-    RowShape sh = row_shape(A.n, lanes_for(A.nnz, A.n));
+    if ((rc = matrix_refresh_fp32(A)) || (rc = precond_fp32(P)) || (rc = matrix_tune(A, true)) ||
+        (rc = matrix_tune(A, false)))
+        return rc;
+    const int plan32 = A.plan[1], plan64 = A.plan[0];
+    uint32_t nb32 = 0, nb64 = 0;
     MpArgs a{};
-    a.A = view_of(A, sh);
+    a.A = plan_view(A, plan32, nb32);
     a.b = S.y.p;
     a.x = d_x;
     a.r = S.r32.p;
@@ -1369,24 +1454,29 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
     a.partials = S.partials.p;
     a.st = S.st.p;
     a.hist = S.hist.p;
+    MpArgs a64 = a;
+    a64.A = plan_view(A, plan64, nb64);
     uint32_t w = a.pw;
     uint32_t vb = vec_blocks(A.n / w);
     uint32_t nvb = vec_blocks(A.n);
     hipStream_t stream = runtime().stream;
-    RS_HIP(hipMemsetAsync(S.dx32.p, 0, sizeof(float) * A.n, stream));
-    RS_DISPATCH_G(sh.g, k_mp_true_resid, sh.nb, a, 1);
-    RS_DISPATCH_W(w, k_mp_rho, vb, a, 1);
+    k_fill32<<<nvb, kBlock, 0, stream>>>(S.dx32.p, A.n, 0.0f);
+    RS_SHAPE_SWITCH(plan64, k_mp_true_resid, nb64, stream, a64);
+    fin(S.partials.p, nb64, 2, FIN_MP_TRUE, 1, S.st.p, S.hist.p);
+    RS_DISPATCH_W(w, k_mp_rho, vb, a);
+    fin(S.partials.p, vb, 1, FIN_MP_RHO, 1, S.st.p, S.hist.p);
     RS_HIP(hipGetLastError());
     rc = capture(S, [&] {
         for (uint32_t i = 0; i < kGraphIters; i++) {
             RS_DISPATCH_W(w, k_mp_dir, vb, a);
-            RS_DISPATCH_G(sh.g, k_mp_spmv_dot, sh.nb, a);
+            RS_SHAPE_SWITCH(plan32, k_mp_spmv_dot, nb32, stream, a);
+            fin(S.partials.p, nb32, 1, FIN_DOT, 0, S.st.p, S.hist.p);
             RS_DISPATCH_W(w, k_mp_update, vb, a);
+            fin(S.partials.p, vb, 2, FIN_MP_UPDATE, 0, S.st.p, S.hist.p);
         }
-    }, graph_key_of(2, a, sh, w, vb));
+    }, graph_key_of(2, a, plan32, nb32, w, vb));
     if (rc) return rc;
     refinements = 0;
-    // the first true residual may already satisfy the tolerance
     SolveState h;
     RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     RS_HIP(hipStreamSynchronize(stream));
@@ -1397,8 +1487,8 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
         if ((rc = run_graph_loop(S, done))) return rc;
         if (done != kStopPause) break;
         refinements++;
-        // residual replacements that stop lowering the FP64 residual mean the FP32 recursion
-        // has lost contact with the true residual: stop instead of spinning
+        // replacements that stop lowering the FP64 residual mean the FP32 recursion has lost
+        // contact with the true residual: stop instead of spinning
         if (no_gain >= kMaxReplacementsWithoutGain) {
             k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
             RS_HIP(hipGetLastError());
@@ -1408,8 +1498,10 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
             return RS_OK;
         }
         k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
-        RS_DISPATCH_G(sh.g, k_mp_true_resid, sh.nb, a, 0);
-        RS_DISPATCH_W(w, k_mp_rho, vb, a, 0);
+        RS_SHAPE_SWITCH(plan64, k_mp_true_resid, nb64, stream, a64);
+        fin(S.partials.p, nb64, 2, FIN_MP_TRUE, 0, S.st.p, S.hist.p);
+        RS_DISPATCH_W(w, k_mp_rho, vb, a);
+        fin(S.partials.p, vb, 1, FIN_MP_RHO, 0, S.st.p, S.hist.p);
         RS_HIP(hipGetLastError());
         RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
         RS_HIP(hipStreamSynchronize(stream));
@@ -1434,8 +1526,11 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
     const uint32_t n = A.n;
     const uint32_t nouter = S.conf.n_outer_iterations, ninner = S.conf.n_inner_iterations;
     if (ninner == 0) return fail(RS_ERR_INVALID_ARG, "GMRES: nInnerIterations is 0");
-    RowShape sh = row_shape(n, lanes_for(A.nnz, n));
-    CsrView v = view_of(A, sh);
+    int rc = matrix_tune(A, false);
+    if (rc) return rc;
+    const int plan = A.plan[0];
+    uint32_t nb = 0;
+    CsrView v = plan_view(A, plan, nb);
     uint32_t nvb = vec_blocks(n);
     uint32_t w = P.width ? P.width : 1;
     const uint64_t ld = n;
@@ -1445,21 +1540,21 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
         hv(ninner + 2);
     auto Hm = [&](uint32_t i, uint32_t j) -> double & { return H[(size_t)i * ninner + j]; };
     // bn = |b| (src/rmatrixsolver.cpp:375)
-    k_norm2<<<nvb, kBlock, 0, stream>>>(S.y.p, n, S.partials.p, dst, 0);
+    k_norm2<<<nvb, kBlock, 0, stream>>>(S.y.p, n, S.partials.p);
+    fin(S.partials.p, nvb, 1, FIN_NORM0, 0, dst, nullptr);
     RS_HIP(hipGetLastError());
     RS_HIP(hipMemcpyAsync(&h, dst, sizeof(h), hipMemcpyDeviceToHost, stream));
     RS_HIP(hipStreamSynchronize(stream));
     const double bn = h.t0, An = A.frob;
-    // multidot partial layout: (k+1) x nbm
     uint32_t nbm = std::min<uint32_t>(nvb, 512);
     DBuf<double> mpart;
-    int rc = mpart.alloc((size_t)(ninner + 1) * nbm);
-    if (rc) return rc;
+    if ((rc = mpart.alloc((size_t)(ninner + 1) * nbm))) return rc;
     outer = 0;
     inner_total = 0;
     for (uint32_t ito = 0; ito < nouter; ito++) {
         std::fill(H.begin(), H.end(), 0.0);
-        RS_DISPATCH_G(sh.g, k_gmres_resid, sh.nb, v, S.y.p, d_x, S.ro.p, S.partials.p, dst);
+        RS_SHAPE_SWITCH(plan, k_gmres_resid, nb, stream, v, S.y.p, d_x, S.ro.p, S.partials.p);
+        fin(S.partials.p, nb, 2, FIN_NORM01, 0, dst, nullptr);
         RS_HIP(hipGetLastError());
         RS_HIP(hipMemcpyAsync(&h, dst, sizeof(h), hipMemcpyDeviceToHost, stream));
         RS_HIP(hipStreamSynchronize(stream));
@@ -1485,12 +1580,13 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
             inner_total++;
             double *vi = S.V.p + (uint64_t)iti * ld, *zi = S.Z.p + (uint64_t)iti * ld;
             RS_DISPATCH_W(w, k_pc_apply, vec_blocks(n / w), P.type, w, P.data.p, n, vi, zi);
-            RS_DISPATCH_G(sh.g, k_spmv, sh.nb, v, zi, S.w.p);
+            RS_SHAPE_SWITCH(plan, k_spmv, nb, stream, v, zi, S.w.p);
             uint32_t cnt = iti + 1;
             dim3 g(nbm, (cnt + 7) / 8);
             k_multidot<<<g, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, mpart.p);
             k_multidot_final<<<cnt, kBlock, 0, stream>>>(mpart.p, nbm, S.hbuf.p);
-            k_gs_subtract<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, S.hbuf.p, S.partials.p, dst);
+            k_gs_subtract<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, S.hbuf.p, S.partials.p);
+            fin(S.partials.p, nvb, 1, FIN_NORM0, 0, dst, nullptr);
             k_normalize<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p + (uint64_t)(iti + 1) * ld, n, dst);
             RS_HIP(hipGetLastError());
             RS_HIP(hipMemcpyAsync(hv.data(), S.hbuf.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, stream));
@@ -1545,23 +1641,26 @@ static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d
     bool gmres = c.solver_type == RS_SOLVER_GMRES;
     if (mixed && gmres) return fail(RS_ERR_UNSUPPORTED, "mixed precision is implemented for CG only");
     int rc;
+    // tune outside the timed region (once per matrix)
+    if (A.n && (rc = matrix_tune(A, false))) return rc;
+    if (A.n && mixed && (rc = matrix_tune(A, true))) return rc;
     // RMatrixPreconditioner P(A, type, blockSize)  (src/rmatrixsolver.cpp:160)
     rs_precond P;
-    if ((rc = precond_build(A, ptype, block_size, P))) return rc;
     if ((rc = ensure_ws(S, A.n, mixed, gmres))) return rc;
     hipStream_t stream = runtime().stream;
     hipEvent_t e0, e1;
     RS_HIP(hipEventCreate(&e0));
     RS_HIP(hipEventCreate(&e1));
     RS_HIP(hipEventRecord(e0, stream));
+    if ((rc = precond_build(A, ptype, block_size, P))) return rc;
     const uint32_t n = A.n;
     uint32_t nvb = vec_blocks(std::max(n, 1u));
     uint32_t hist_cap = (uint32_t)(S.hist.n / 2);
     k_state_init<<<1, 64, 0, stream>>>(S.st.p, A.frob, c.solver_cvg_value, S.error, c.n_outer_iterations,
                                        mixed ? RS_STOP_RELRES : (uint32_t)c.stop_rule, hist_cap, mixed ? 1 : 0);
     // equation scaling (src/rmatrixsolver.cpp:164-184)
-    k_norm2<<<nvb, kBlock, 0, stream>>>(d_b, n, S.partials.p, S.st.p, 0);
-    k_scale_from_norm<<<1, 64, 0, stream>>>(S.st.p);
+    k_norm2<<<nvb, kBlock, 0, stream>>>(d_b, n, S.partials.p);
+    fin(S.partials.p, nvb, 1, FIN_SCALE, 0, S.st.p, S.hist.p);
     k_apply_scale<<<nvb, kBlock, 0, stream>>>(d_b, S.y.p, d_x, n, S.st.p);
     RS_HIP(hipGetLastError());
     uint32_t done = 0, iters = 0, inner = 0, refinements = 0;
@@ -1570,6 +1669,7 @@ static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d
         if (gmres) rc = solve_gmres(S, A, P, d_x, iters, inner);
         else if (mixed) rc = solve_cg_mixed(S, A, P, d_x, done, refinements);
         else rc = solve_cg64(S, A, P, d_x, done);
+        S.body = nullptr;  // the iteration body refers to this call's locals
         if (rc) return rc;
     }
     k_unscale<<<nvb, kBlock, 0, stream>>>(d_x, n, S.st.p);
@@ -1591,8 +1691,8 @@ static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d
         S.trend = h.trend;
         reason = (int)h.reason;
         if (!S.history.empty() && iters)
-            RS_HIP(hipMemcpy(S.history.data(), S.hist.p, sizeof(double) * 2 * std::min<uint64_t>(iters, S.history.size() / 2),
-                             hipMemcpyDeviceToHost));
+            RS_HIP(hipMemcpy(S.history.data(), S.hist.p,
+                             sizeof(double) * 2 * std::min<uint64_t>(iters, S.history.size() / 2), hipMemcpyDeviceToHost));
     } else {
         reason = RS_REASON_MAXIT;
         bool conv = !std::isnormal(S.error) || !std::isnormal(S.trend) || S.error < c.solver_cvg_value ||
@@ -1608,9 +1708,11 @@ static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d
     // true residual on the caller's (unscaled) system, FP64
     double true_relres = 0.0, b_norm = 0.0;
     if (n > 0) {
-        RowShape sh = row_shape(n, lanes_for(A.nnz, n));
-        CsrView v = view_of(A, sh);
-        RS_DISPATCH_G(sh.g, k_resid_norm, sh.nb, v, d_b, d_x, S.partials.p, S.st.p);
+        uint32_t nb = 0;
+        const int plan = A.plan[0] >= 0 ? A.plan[0] : default_plan(A);
+        CsrView v = plan_view(A, plan, nb);
+        RS_SHAPE_SWITCH(plan, k_resid_norm, nb, stream, v, d_b, d_x, S.partials.p);
+        fin(S.partials.p, nb, 2, FIN_NORM01, 0, S.st.p, S.hist.p);
         RS_HIP(hipGetLastError());
         RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
         RS_HIP(hipStreamSynchronize(stream));

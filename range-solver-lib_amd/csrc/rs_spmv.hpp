@@ -1,0 +1,141 @@
+// rs_spmv.hpp -- CSR SpMV building blocks for gfx950, shared by every kernel that multiplies
+// by the matrix (plain SpMV, fused CG kernels, residuals).  Two row mappings:
+//
+//  VecRows<G>    "vector CSR": G lanes per row (G | 64, so 64/G rows per wavefront); each
+//                workgroup owns a contiguous, XCD-aware chunk of rows; lane-group sums by
+//                shuffles.  Good when rows are long.
+//  TileRows<CAP> "stream CSR": the rows are cut into tiles of <= 256 rows and <= CAP nonzeros
+//                (computed once per matrix).  A workgroup streams a tile's column/value
+//                entries with independent, fully coalesced loads, stages the products in LDS,
+//                then one thread per row sums its segment from LDS.  No dependent
+//                rowptr -> col -> x chain per row, so many more loads are in flight.
+//
+// Both call epi.row(r, s) exactly once per row with the row's dot product s (accumulated in
+// AT: double for the FP64 path, float for the FP32 path); the epilogue fuses whatever the
+// caller needs (store, residual, dot products).  Sums are in a fixed order (deterministic).
+#pragma once
+
+#include "rs_device.hpp"
+
+namespace rs {
+
+constexpr uint32_t kTileRows = 256;
+
+struct CsrView {
+    uint32_t n;
+    uint32_t chunk;            // VecRows: rows per workgroup
+    const uint32_t *rowptr, *col;
+    const double *val;
+    const float *val32;
+    const uint32_t *tiles;     // TileRows: tile start rows (ntiles + 1)
+    uint32_t ntiles, tpb;      // tiles, tiles per workgroup
+};
+
+template <int G>
+struct VecRows {
+    static constexpr bool kTile = false;
+    static constexpr int kParam = G;
+    template <typename AT, typename VT, typename XT, class Epi>
+    __device__ __forceinline__ static void run(const CsrView &A, const VT *__restrict__ val,
+                                               const XT *__restrict__ x, Epi &epi)
+    {
+        const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+        const int gl = threadIdx.x % G;
+        const uint32_t r0 = lb * A.chunk;
+        const uint32_t r1 = min(A.n, r0 + A.chunk);
+        const uint32_t *__restrict__ rowptr = A.rowptr;
+        const uint32_t *__restrict__ col = A.col;
+        for (uint32_t base = r0; base < r1; base += kBlock / G) {
+            const uint32_t r = base + threadIdx.x / G;
+            if (r < r1) {
+                const uint32_t b = rowptr[r], e = rowptr[r + 1];
+                AT s = 0;
+                for (uint32_t j = b + gl; j < e; j += G) s += (AT)val[j] * (AT)x[col[j]];
+                s = group_sum<G>(s);
+                if (gl == 0) epi.row(r, s);
+            }
+        }
+    }
+};
+
+template <int CAP>
+struct TileRows {
+    static constexpr bool kTile = true;
+    static constexpr int kParam = CAP;
+    template <typename AT, typename VT, typename XT, class Epi>
+    __device__ __forceinline__ static void run(const CsrView &A, const VT *__restrict__ val,
+                                               const XT *__restrict__ x, Epi &epi)
+    {
+        __shared__ AT prod[CAP];
+        __shared__ uint32_t rp[kTileRows + 1];
+        __shared__ AT red[kBlock / kWave];
+        const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+        const uint32_t t0 = lb * A.tpb;
+        const uint32_t t1 = min(A.ntiles, t0 + A.tpb);
+        const uint32_t *__restrict__ col = A.col;
+        for (uint32_t t = t0; t < t1; t++) {
+            const uint32_t r0 = A.tiles[t], nr = A.tiles[t + 1] - r0;
+            for (uint32_t i = threadIdx.x; i <= nr; i += kBlock) rp[i] = A.rowptr[r0 + i];
+            __syncthreads();
+            const uint32_t j0 = rp[0], j1 = rp[nr];
+            if (j1 - j0 <= (uint32_t)CAP) {
+                uint32_t j = j0 + threadIdx.x;
+                // 4 independent entries per thread in flight
+                for (; j + 3 * kBlock < j1; j += 4 * kBlock) {
+                    uint32_t c0 = col[j], c1 = col[j + kBlock], c2 = col[j + 2 * kBlock], c3 = col[j + 3 * kBlock];
+                    VT v0 = val[j], v1 = val[j + kBlock], v2 = val[j + 2 * kBlock], v3 = val[j + 3 * kBlock];
+                    XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+                    prod[j - j0] = (AT)v0 * (AT)x0;
+                    prod[j + kBlock - j0] = (AT)v1 * (AT)x1;
+                    prod[j + 2 * kBlock - j0] = (AT)v2 * (AT)x2;
+                    prod[j + 3 * kBlock - j0] = (AT)v3 * (AT)x3;
+                }
+                for (; j < j1; j += kBlock) prod[j - j0] = (AT)val[j] * (AT)x[col[j]];
+                __syncthreads();
+                if (threadIdx.x < nr) {
+                    const uint32_t a = rp[threadIdx.x] - j0, e = rp[threadIdx.x + 1] - j0;
+                    AT s = 0;
+                    for (uint32_t k = a; k < e; k++) s += prod[k];
+                    epi.row(r0 + threadIdx.x, s);
+                }
+            } else {
+                // a single row longer than CAP: the whole workgroup reduces it
+                AT s = 0;
+                for (uint32_t j = j0 + threadIdx.x; j < j1; j += kBlock) s += (AT)val[j] * (AT)x[col[j]];
+                s = group_sum<kWave>(s);
+                if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = s;
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    AT tot = 0;
+                    for (int w = 0; w < kBlock / kWave; w++) tot += red[w];
+                    epi.row(r0, tot);
+                }
+            }
+            __syncthreads();
+        }
+    }
+};
+
+// Launch plan of one matrix: which shape, and its grid.
+struct SpmvPlan {
+    int id = -1;           // index into kShapes
+    uint32_t nb = 0;       // workgroups
+};
+
+// Shape ids: 0..4 VecRows<4,8,16,32,64>, 5 TileRows<2048>, 6 TileRows<4096>
+constexpr int kNumShapes = 7;
+
+#define RS_SHAPE_SWITCH(id, KERNEL, grid, stream, ...)                                         \
+    do {                                                                                       \
+        switch (id) {                                                                          \
+            case 0: KERNEL<::rs::VecRows<4>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;   \
+            case 1: KERNEL<::rs::VecRows<8>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;   \
+            case 2: KERNEL<::rs::VecRows<16>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;  \
+            case 3: KERNEL<::rs::VecRows<32>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;  \
+            case 4: KERNEL<::rs::VecRows<64>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;  \
+            case 5: KERNEL<::rs::TileRows<2048>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
+            default: KERNEL<::rs::TileRows<4096>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
+        }                                                                                      \
+    } while (0)
+
+}  // namespace rs
