@@ -31,6 +31,16 @@ sys.path.insert(0, ROOT)
 METRIC = ("time-to-1e-8-residual per system: learned+mixed vs plain FP64 CG; SpMV GB/s")
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of each FP64 SpMV shape on the 128^3 heat matrix, from two rocprofv3
+# --pmc runs (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB copy); the
+# runs and their commands are summarised in profiles/spmv_heat128_pmc.md.
+MEASURED_TRAFFIC_128 = {
+    "vector-csr g4": 584.6e6, "vector-csr g8": 507.6e6, "vector-csr g16": 487.7e6,
+    "vector-csr g32": 456.4e6, "vector-csr g64": 456.9e6, "tile-csr 2048": 456.3e6,
+    "tile-csr 4096": 441.9e6,
+}
+MEASURED_TRAFFIC_128_SOURCE = "profiles/spmv_heat128_pmc.md"
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -213,8 +223,10 @@ def main() -> int:
                       "shape": rs.SPMV_SHAPE_NAMES[plan],
                       "tuning_us": {rs.SPMV_SHAPE_NAMES[i]: round(t * 1e3, 2) for i, t in enumerate(tune_ms)}}
     ach = spmv["fp64"]["gbs"]
+    traffic = MEASURED_TRAFFIC_128.get(spmv["fp64"]["shape"]) if (c == 128 and args.sigma == 2.0) else None
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": int(traffic) if traffic else None,
                 "kernel": f"rs::k_spmv FP64 CSR SpMV ({spmv['fp64']['shape']})",
                 "algorithmic_bytes": spmv["fp64"]["bytes"]}
 

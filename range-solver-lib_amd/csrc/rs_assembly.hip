@@ -374,7 +374,7 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
         RS_HIP(hipMemcpyAsync(A.col.p, pat.col.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
         k_zero<<<vblocks(nnz), kBlock, 0, st>>>(A.val.p, nnz);
     }
-    if ((rc = matrix_set_tiles(A, pat.rowptr.data()))) return rc;
+    if ((rc = matrix_set_layout(A, pat.rowptr.data()))) return rc;
     RS_HIP(hipStreamSynchronize(st));
     *out = S.release();
     return RS_OK;
@@ -445,8 +445,7 @@ extern "C" int rs_assemble_heat_device(rs_assembly *S, const double *d_k, const 
     RS_HIP(hipGetLastError());
     if ((rc = timed_end(ev, S->last_ms))) return rc;
     if ((rc = matrix_frobenius(S->A))) return rc;
-    S->A.val32_valid = false;
-    S->A.version++;
+    matrix_values_changed(S->A);
     return RS_OK;
 }
 
@@ -549,8 +548,7 @@ extern "C" int rs_assemble_stress(rs_assembly *S, const double *E, const double 
     RS_HIP(hipGetLastError());
     if ((rc = timed_end(ev, S->last_ms))) return rc;
     if ((rc = matrix_frobenius(S->A))) return rc;
-    S->A.val32_valid = false;
-    S->A.version++;
+    matrix_values_changed(S->A);
     RS_HIP(hipMemcpyAsync(b, S->bvec.p, sizeof(double) * S->n_dof, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
     return RS_OK;

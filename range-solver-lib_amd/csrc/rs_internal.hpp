@@ -107,13 +107,28 @@ struct Matrix {
     uint64_t version = 0;       // bumped on every value update
     DBuf<uint32_t> tiles[2];    // SpMV tile starts for the two tile capacities
     uint32_t ntiles[2] = { 0, 0 };
+    // SELL-64 copy: slice offsets, column-major columns (padding RS_EOD) and values
+    DBuf<uint32_t> sell_ptr, sell_col;
+    DBuf<double> sell_val;
+    DBuf<float> sell_val32;
+    uint32_t nslices = 0;
+    bool sell_valid = false, sell32_valid = false;
     int plan[2] = { -1, -1 };   // autotuned SpMV shape for FP64 / FP32
     double plan_ms[2][8] = {};  // tuning times per shape
 };
 
+// Every change of the FP64 values goes through here: derived copies become stale.
+inline void matrix_values_changed(Matrix &A)
+{
+    A.val32_valid = false;
+    A.sell_valid = false;
+    A.sell32_valid = false;
+    A.version++;
+}
+
 int matrix_frobenius(Matrix &A);   // device reduction, sets A.frob
 int matrix_refresh_fp32(Matrix &A);
-int matrix_set_tiles(Matrix &A, const uint32_t *host_rowptr);
+int matrix_set_layout(Matrix &A, const uint32_t *host_rowptr);
 int matrix_tune(Matrix &A, bool fp32);
 
 }  // namespace rs

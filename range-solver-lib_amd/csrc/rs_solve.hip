@@ -78,16 +78,30 @@ enum FinOp {
     FIN_MP_RHO,         // mixed: rho after a replacement (flag: first)
 };
 
-__global__ __launch_bounds__(kBlock) void k_fin(const double *__restrict__ partials, uint32_t nparts,
-                                                int nv, int op, int flag, SolveState *st,
-                                                double *__restrict__ hist)
+constexpr int kFinBlock = 1024;
+
+__global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ partials, uint32_t nparts,
+                                                   int nv, int op, int flag, SolveState *st,
+                                                   double *__restrict__ hist)
 {
     if ((op == FIN_DOT || op == FIN_UPDATE || op == FIN_MP_UPDATE) && st->done) return;
-    __shared__ double lds[4 * (kBlock / kWave)];
+    __shared__ double lds[4 * (kFinBlock / kWave)];
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
-    for (uint32_t i = threadIdx.x; i < nparts; i += kBlock)
+    // all loads of a thread are independent and issued before the sums (up to 4 partials per
+    // value and thread at the launch sizes used here)
+    double ld[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            uint32_t i = threadIdx.x + u * kFinBlock;
+            ld[k][u] = (k < nv && i < nparts) ? partials[(uint64_t)k * nparts + i] : 0.0;
+        }
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (ld[k][0] + ld[k][1]) + (ld[k][2] + ld[k][3]);
+    for (uint32_t i = threadIdx.x + 4 * kFinBlock; i < nparts; i += kFinBlock)
         for (int k = 0; k < nv; k++) v[k] += partials[(uint64_t)k * nparts + i];
-    block_sum<4, kBlock>(v, lds);
+    block_sum<4, kFinBlock>(v, lds);
     if (threadIdx.x != 0) return;
     const double eps = kEps;
     switch (op) {
@@ -328,6 +342,38 @@ __global__ void k_fill(double *__restrict__ a, uint64_t n, double v)
 __global__ void k_fill32(float *__restrict__ a, uint64_t n, float v)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) a[i] = v;
+}
+
+// SELL-64 columns: row r = 64 s + lane writes its entries at sell_ptr[s] + 64 j + lane.
+__global__ void k_sell_cols(CsrView A, uint32_t *__restrict__ sell_col)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < (uint64_t)A.nslices * kWave;
+         r += (uint64_t)gridDim.x * kBlock) {
+        uint32_t s = (uint32_t)(r / kWave), lane = (uint32_t)(r % kWave);
+        uint32_t base = A.sell_ptr[s], width = (A.sell_ptr[s + 1] - base) / kWave;
+        uint32_t b = 0, len = 0;
+        if (r < A.n) {
+            b = A.rowptr[r];
+            len = A.rowptr[r + 1] - b;
+        }
+        for (uint32_t j = 0; j < width; j++) sell_col[base + j * kWave + lane] = j < len ? A.col[b + j] : kSellPad;
+    }
+}
+
+// SELL-64 values from the CSR values (padding 0); FP32 copy from the FP64 SELL values.
+__global__ void k_sell_vals(CsrView A, double *__restrict__ sell_val)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < (uint64_t)A.nslices * kWave;
+         r += (uint64_t)gridDim.x * kBlock) {
+        uint32_t s = (uint32_t)(r / kWave), lane = (uint32_t)(r % kWave);
+        uint32_t base = A.sell_ptr[s], width = (A.sell_ptr[s + 1] - base) / kWave;
+        uint32_t b = 0, len = 0;
+        if (r < A.n) {
+            b = A.rowptr[r];
+            len = A.rowptr[r + 1] - b;
+        }
+        for (uint32_t j = 0; j < width; j++) sell_val[base + j * kWave + lane] = j < len ? A.val[b + j] : 0.0;
+    }
 }
 
 // ============================================================================ preconditioner
@@ -774,6 +820,7 @@ static uint32_t vec_blocks(uint64_t items)
 
 static const int kVecLanes[5] = { 4, 8, 16, 32, 64 };
 static const uint32_t kTileCap[2] = { 2048, 4096 };
+constexpr int kSellPlan = 7;
 
 // Workgroups of plan `id` for matrix A, and the view its kernels read.
 static CsrView plan_view(const Matrix &A, int id, uint32_t &nb)
@@ -790,12 +837,21 @@ static CsrView plan_view(const Matrix &A, int id, uint32_t &nb)
         nb = round_up(std::max(1u, std::min(target_blocks(), passes)), kXcd);
         v.chunk = round_up((A.n + nb - 1) / nb, rpp);
         if (v.chunk == 0) v.chunk = rpp;
-    } else {
+    } else if (id < 7) {
         int t = id - 5;
         v.tiles = A.tiles[t].p;
         v.ntiles = A.ntiles[t];
         nb = round_up(std::max(1u, std::min(target_blocks(), v.ntiles)), kXcd);
         v.tpb = (v.ntiles + nb - 1) / nb;
+    } else {
+        v.sell_ptr = A.sell_ptr.p;
+        v.sell_col = A.sell_col.p;
+        v.sell_val = A.sell_val.p;
+        v.sell_val32 = A.sell_val32.p;
+        v.nslices = A.nslices;
+        uint32_t waves = kBlock / kWave;
+        nb = round_up(std::max(1u, std::min(target_blocks(), (A.nslices + waves - 1) / waves)), kXcd);
+        v.spb = (A.nslices + nb - 1) / nb;
     }
     return v;
 }
@@ -819,7 +875,7 @@ static int default_plan(const Matrix &A)
 
 static void fin(const double *partials, uint32_t nparts, int nv, int op, int flag, SolveState *st, double *hist)
 {
-    k_fin<<<1, kBlock, 0, runtime().stream>>>(partials, nparts, nv, op, flag, st, hist);
+    k_fin<<<1, kFinBlock, 0, runtime().stream>>>(partials, nparts, nv, op, flag, st, hist);
 }
 
 // Scratch for standalone reductions (matrix norm).
@@ -869,8 +925,29 @@ int matrix_refresh_fp32(Matrix &A)
     return RS_OK;
 }
 
-int matrix_set_tiles(Matrix &A, const uint32_t *host_rowptr)
+int matrix_set_layout(Matrix &A, const uint32_t *host_rowptr)
 {
+    // SELL-64 slice offsets (host), columns (device)
+    A.nslices = (A.n + kWave - 1) / kWave;
+    std::vector<uint32_t> sp(A.nslices + 1ull, 0);
+    for (uint32_t s = 0; s < A.nslices; s++) {
+        uint32_t w = 0;
+        for (uint32_t r = s * kWave; r < std::min<uint64_t>(A.n, (uint64_t)(s + 1) * kWave); r++)
+            w = std::max(w, host_rowptr[r + 1] - host_rowptr[r]);
+        uint64_t next = (uint64_t)sp[s] + (uint64_t)w * kWave;
+        if (next > 0xFFFFFFF0ull) return fail(RS_ERR_UNSUPPORTED, "SELL-64 copy exceeds 2^32 entries");
+        sp[s + 1] = (uint32_t)next;
+    }
+    int rc;
+    if ((rc = A.sell_ptr.alloc(sp.size())) || (rc = A.sell_col.alloc(std::max<uint32_t>(sp.back(), 1)))) return rc;
+    RS_HIP(hipMemcpyAsync(A.sell_ptr.p, sp.data(), sizeof(uint32_t) * sp.size(), hipMemcpyHostToDevice, runtime().stream));
+    {
+        uint32_t nb = 0;
+        CsrView v = plan_view(A, kSellPlan, nb);
+        if (A.nslices) k_sell_cols<<<vec_blocks((uint64_t)A.nslices * kWave), kBlock, 0, runtime().stream>>>(v, A.sell_col.p);
+        RS_HIP(hipGetLastError());
+    }
+    A.sell_valid = A.sell32_valid = false;
     for (int t = 0; t < 2; t++) {
         std::vector<uint32_t> tiles = build_tiles(host_rowptr, A.n, kTileCap[t], kTileRows);
         int rc = A.tiles[t].alloc(tiles.size());
@@ -884,9 +961,38 @@ int matrix_set_tiles(Matrix &A, const uint32_t *host_rowptr)
     return RS_OK;
 }
 
+// Bring the copies a plan reads up to date with the FP64 CSR values.
+int matrix_sync_plan(Matrix &A, int plan, bool fp32)
+{
+    int rc;
+    if (fp32 && plan != kSellPlan && (rc = matrix_refresh_fp32(A))) return rc;
+    if (plan != kSellPlan) return RS_OK;
+    hipStream_t st = runtime().stream;
+    uint32_t nb = 0;
+    CsrView v = plan_view(A, kSellPlan, nb);
+    uint64_t total = A.sell_col.n;
+    if (!A.sell_valid) {
+        if ((rc = A.sell_val.alloc(total))) return rc;
+        v.sell_val = A.sell_val.p;
+        if (A.nslices) k_sell_vals<<<vec_blocks((uint64_t)A.nslices * kWave), kBlock, 0, st>>>(v, A.sell_val.p);
+        RS_HIP(hipGetLastError());
+        A.sell_valid = true;
+        A.sell32_valid = false;
+    }
+    if (fp32 && !A.sell32_valid) {
+        if ((rc = A.sell_val32.alloc(total))) return rc;
+        k_to_f32<<<vec_blocks(total), kBlock, 0, st>>>(A.sell_val.p, A.sell_val32.p, total);
+        RS_HIP(hipGetLastError());
+        A.sell32_valid = true;
+    }
+    return RS_OK;
+}
+
 // Average device time of `reps` launches of SpMV plan `id`.
 static int time_plan(Matrix &A, int id, bool fp32, uint32_t reps, const void *x, void *y, double &ms)
 {
+    int rc = matrix_sync_plan(A, id, fp32);
+    if (rc) return rc;
     hipStream_t st = runtime().stream;
     uint32_t nb = 0;
     CsrView v = plan_view(A, id, nb);
@@ -974,7 +1080,7 @@ extern "C" int rs_matrix_create(uint32_t n_rows, const uint32_t *row_ptr, const 
         RS_HIP(hipMemcpyAsync(A->val.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
         if ((rc = matrix_frobenius(*A))) return rc;
     }
-    if ((rc = matrix_set_tiles(*A, row_ptr))) return rc;
+    if ((rc = matrix_set_layout(*A, row_ptr))) return rc;
     RS_HIP(hipStreamSynchronize(st));
     *out = A.release();
     return RS_OK;
@@ -989,8 +1095,7 @@ extern "C" int rs_matrix_update_values(rs_matrix *A, const double *val)
         RS_HIP(hipMemcpyAsync(A->val.p, val, sizeof(double) * A->nnz, hipMemcpyHostToDevice, runtime().stream));
         if ((rc = matrix_frobenius(*A))) return rc;
     }
-    A->val32_valid = false;
-    A->version++;
+    matrix_values_changed(*A);
     return RS_OK;
 }
 
@@ -1004,8 +1109,7 @@ extern "C" int rs_matrix_update_values_device(rs_matrix *A, const double *d_val)
             RS_HIP(hipMemcpyAsync(A->val.p, d_val, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, runtime().stream));
         if ((rc = matrix_frobenius(*A))) return rc;
     }
-    A->val32_valid = false;
-    A->version++;
+    matrix_values_changed(*A);
     return RS_OK;
 }
 
@@ -1037,7 +1141,7 @@ extern "C" int rs_matrix_spmv_device(rs_matrix *A, const double *d_x, double *d_
     int rc = require_runtime();
     if (rc) return rc;
     if (A->n == 0) return RS_OK;
-    if ((rc = matrix_tune(*A, false))) return rc;
+    if ((rc = matrix_tune(*A, false)) || (rc = matrix_sync_plan(*A, A->plan[0], false))) return rc;
     uint32_t nb = 0;
     CsrView v = plan_view(*A, A->plan[0], nb);
     RS_SHAPE_SWITCH(A->plan[0], k_spmv, nb, runtime().stream, v, d_x, d_y);
@@ -1396,6 +1500,7 @@ static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, ui
     int rc = matrix_tune(A, false);
     if (rc) return rc;
     const int plan = A.plan[0];
+    if ((rc = matrix_sync_plan(A, plan, false))) return rc;
     uint32_t nb = 0;
     CgArgs a{};
     a.A = plan_view(A, plan, nb);
@@ -1439,6 +1544,7 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
         (rc = matrix_tune(A, false)))
         return rc;
     const int plan32 = A.plan[1], plan64 = A.plan[0];
+    if ((rc = matrix_sync_plan(A, plan32, true)) || (rc = matrix_sync_plan(A, plan64, false))) return rc;
     uint32_t nb32 = 0, nb64 = 0;
     MpArgs a{};
     a.A = plan_view(A, plan32, nb32);
@@ -1529,6 +1635,7 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
     int rc = matrix_tune(A, false);
     if (rc) return rc;
     const int plan = A.plan[0];
+    if ((rc = matrix_sync_plan(A, plan, false))) return rc;
     uint32_t nb = 0;
     CsrView v = plan_view(A, plan, nb);
     uint32_t nvb = vec_blocks(n);
@@ -1710,6 +1817,7 @@ static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d
     if (n > 0) {
         uint32_t nb = 0;
         const int plan = A.plan[0] >= 0 ? A.plan[0] : default_plan(A);
+        if ((rc = matrix_sync_plan(A, plan, false))) return rc;
         CsrView v = plan_view(A, plan, nb);
         RS_SHAPE_SWITCH(plan, k_resid_norm, nb, stream, v, d_b, d_x, S.partials.p);
         fin(S.partials.p, nb, 2, FIN_NORM01, 0, S.st.p, S.hist.p);

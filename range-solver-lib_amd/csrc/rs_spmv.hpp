@@ -1,5 +1,6 @@
-// rs_spmv.hpp -- CSR SpMV building blocks for gfx950, shared by every kernel that multiplies
-// by the matrix (plain SpMV, fused CG kernels, residuals).  Two row mappings:
+// rs_spmv.hpp -- SpMV building blocks for gfx950, shared by every kernel that multiplies by
+// the matrix (plain SpMV, fused CG kernels, residuals).  Three row mappings (the autotuner
+// times all of them per matrix and keeps the fastest):
 //
 //  VecRows<G>    "vector CSR": G lanes per row (G | 64, so 64/G rows per wavefront); each
 //                workgroup owns a contiguous, XCD-aware chunk of rows; lane-group sums by
@@ -10,7 +11,9 @@
 //                then one thread per row sums its segment from LDS.  No dependent
 //                rowptr -> col -> x chain per row, so many more loads are in flight.
 //
-// Both call epi.row(r, s) exactly once per row with the row's dot product s (accumulated in
+//  SellRows      SELL-64 copy of the matrix, one row per lane (below).
+//
+// All call epi.row(r, s) exactly once per row with the row's dot product s (accumulated in
 // AT: double for the FP64 path, float for the FP32 path); the epilogue fuses whatever the
 // caller needs (store, residual, dot products).  Sums are in a fixed order (deterministic).
 #pragma once
@@ -29,7 +32,14 @@ struct CsrView {
     const float *val32;
     const uint32_t *tiles;     // TileRows: tile start rows (ntiles + 1)
     uint32_t ntiles, tpb;      // tiles, tiles per workgroup
+    const uint32_t *sell_ptr;  // SellRows: slice start offsets (nslices + 1)
+    const uint32_t *sell_col;  // column-major within a slice, padding = kSellPad
+    const double *sell_val;
+    const float *sell_val32;
+    uint32_t nslices, spb;     // slices, slices per workgroup
 };
+
+constexpr uint32_t kSellPad = 0xFFFFFFFFu;
 
 template <int G>
 struct VecRows {
@@ -116,14 +126,58 @@ struct TileRows {
     }
 };
 
+// SELL-64 (sliced ELLPACK, Kreutzer et al. 2014, sigma = 1): rows in slices of 64, one row per
+// lane of a wavefront; the entries of a slice are stored column-major, so entry j of the 64
+// rows is one contiguous 64-wide load for columns and values.  On mesh matrices the j-th
+// neighbours of consecutive rows are consecutive too, so the x gather coalesces as well.
+// Padding entries carry column kSellPad and are skipped.
+__device__ __forceinline__ const double *sell_values(const CsrView &A, const double *) { return A.sell_val; }
+__device__ __forceinline__ const float *sell_values(const CsrView &A, const float *) { return A.sell_val32; }
+
+struct SellRows {
+    static constexpr bool kTile = false;
+    static constexpr int kParam = 64;
+    template <typename AT, typename VT, typename XT, class Epi>
+    __device__ __forceinline__ static void run(const CsrView &A, const VT *__restrict__ csr_val,
+                                               const XT *__restrict__ x, Epi &epi)
+    {
+        const VT *__restrict__ val = sell_values(A, csr_val);
+        const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+        const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+        const uint32_t s0 = lb * A.spb, s1 = min(A.nslices, s0 + A.spb);
+        const uint32_t *__restrict__ col = A.sell_col;
+        for (uint32_t s = s0 + wid; s < s1; s += kBlock / kWave) {
+            const uint32_t b = A.sell_ptr[s], e = A.sell_ptr[s + 1];
+            AT acc = 0;
+            uint32_t j = b + lane;
+            for (; j + 3 * kWave < e; j += 4 * kWave) {
+                uint32_t c0 = col[j], c1 = col[j + kWave], c2 = col[j + 2 * kWave], c3 = col[j + 3 * kWave];
+                VT v0 = val[j], v1 = val[j + kWave], v2 = val[j + 2 * kWave], v3 = val[j + 3 * kWave];
+                XT x0 = c0 != kSellPad ? x[c0] : XT(0), x1 = c1 != kSellPad ? x[c1] : XT(0);
+                XT x2 = c2 != kSellPad ? x[c2] : XT(0), x3 = c3 != kSellPad ? x[c3] : XT(0);
+                acc += (AT)v0 * (AT)x0;
+                acc += (AT)v1 * (AT)x1;
+                acc += (AT)v2 * (AT)x2;
+                acc += (AT)v3 * (AT)x3;
+            }
+            for (; j < e; j += kWave) {
+                uint32_t c = col[j];
+                if (c != kSellPad) acc += (AT)val[j] * (AT)x[c];
+            }
+            const uint32_t r = s * kWave + lane;
+            if (r < A.n) epi.row(r, acc);
+        }
+    }
+};
+
 // Launch plan of one matrix: which shape, and its grid.
 struct SpmvPlan {
     int id = -1;           // index into kShapes
     uint32_t nb = 0;       // workgroups
 };
 
-// Shape ids: 0..4 VecRows<4,8,16,32,64>, 5 TileRows<2048>, 6 TileRows<4096>
-constexpr int kNumShapes = 7;
+// Shape ids: 0..4 VecRows<4,8,16,32,64>, 5 TileRows<2048>, 6 TileRows<4096>, 7 SellRows
+constexpr int kNumShapes = 8;
 
 #define RS_SHAPE_SWITCH(id, KERNEL, grid, stream, ...)                                         \
     do {                                                                                       \
@@ -134,7 +188,8 @@ constexpr int kNumShapes = 7;
             case 3: KERNEL<::rs::VecRows<32>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;  \
             case 4: KERNEL<::rs::VecRows<64>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;  \
             case 5: KERNEL<::rs::TileRows<2048>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
-            default: KERNEL<::rs::TileRows<4096>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
+            case 6: KERNEL<::rs::TileRows<4096>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
+            default: KERNEL<::rs::SellRows><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;       \
         }                                                                                      \
     } while (0)
 
