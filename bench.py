@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC_128 = {
     "vector-csr g4": 584.6e6, "vector-csr g8": 507.6e6, "vector-csr g16": 487.7e6,
     "vector-csr g32": 456.4e6, "vector-csr g64": 456.9e6, "tile-csr 2048": 456.3e6,
-    "tile-csr 4096": 441.9e6,
+    "tile-csr 4096": 441.9e6, "sell-64": 439.1e6,
 }
 MEASURED_TRAFFIC_128_SOURCE = "profiles/spmv_heat128_pmc.md"
 
