@@ -95,6 +95,8 @@ typedef struct rs_info {
     double b_norm;                /* ||b|| */
     double true_relres;           /* ||b - A x|| / ||b||, recomputed in FP64 after the solve */
     double solve_ms;              /* device time of the solve, host-visible stream events */
+    uint32_t precond_applications;/* learned preconditioner: network evaluations */
+    uint32_t fallback;            /* learned solve: 1 when it finished with Jacobi-CG */
 } rs_info;
 
 /* ---------------------------------------------------------------- runtime ------------ */
@@ -231,6 +233,50 @@ int rs_assemble_stress(rs_assembly *S, const double *E, const double *nu, const 
 /* Write-back (src/rsolverheat.cpp:473-480): node[i*dpn+c] = x[book] where enabled. */
 int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node);
 int rs_assembly_destroy(rs_assembly *S);
+
+/* ---------------------------------------------------------------- learned preconditioner */
+/* No counterpart in the reference (a new preconditioner type, RS_PRECOND_LEARNED): a two-level
+ * preconditioner z = D^-1 r + P e whose coarse correction e is computed by a 6-layer, 64-wide
+ * message-passing network over the coarse graph of caller-given aggregates (P piecewise
+ * constant, coarse operator P^T A P), applied inside flexible CG in mixed precision with
+ * Jacobi-CG as the fallback.  At most 512 aggregates. */
+typedef struct rs_learned rs_learned;
+/* Aggregates from node coordinates: nbx*nby*nbz bins over the bounding box; agg_of_dof gets
+ * the bin of each enabled position of the node book (ids compacted to 0..n_agg-1). */
+int rs_aggregates_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node,
+                          const uint32_t *book, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                          uint32_t *agg_of_dof, uint32_t *n_agg);
+int rs_learned_create(const rs_matrix *A, const uint32_t *agg_of_row, uint32_t n_agg,
+                      rs_learned **out);
+/* Network parameters, FP32, 49,921 values (layout: W_in[4][64], b_in[64], 6 x {W_s[64][64],
+ * W_n[64][64], b[64]}, w_out[64], b_out). */
+int rs_learned_set_params(rs_learned *L, const float *params, uint64_t count);
+int rs_learned_info(const rs_learned *L, uint32_t *n_coarse, uint64_t *n_params, uint64_t *coarse_nnz);
+/* Coarse operator for the current matrix values (done by the solve as well). */
+int rs_learned_setup(rs_learned *L, rs_matrix *A);
+int rs_learned_coarse_matrix(const rs_learned *L, double *dense);         /* n_coarse^2 */
+int rs_learned_restrict_device(rs_learned *L, const double *d_r, float *rc_host);
+int rs_learned_apply_coarse(rs_learned *L, const float *rc_host, float *e_host);
+/* Training pair of a CG iterate x of A x* = b (device arrays): r_c = P^T (b - A x) and
+ * s = (x* - x) . (b - A x), the squared A-norm of the iterate's error. */
+int rs_learned_pair(rs_learned *L, rs_matrix *A, const double *d_b, const double *d_x,
+                    const double *d_xstar, float *rc_host, double *s_out);
+int rs_learned_destroy(rs_learned *L);
+/* Flexible CG with the learned preconditioner (solver conf: CG; b, x device arrays). */
+int rs_solver_solve_learned_device(rs_solver *S, rs_matrix *A, rs_learned *L, const double *d_b,
+                                   double *d_x, rs_info *info);
+
+/* Training of the network on the GPU (forward, backward and Adam kernels).  A batch holds B
+ * samples on coarse graphs of n <= 64 nodes: dense coarse operators Ac[B][64][64] (zero
+ * padded), coarse residuals rc[B][64] and s[B] = e_k . r_k of the (residual, error) pair. */
+typedef struct rs_trainer rs_trainer;
+int rs_trainer_create(const float *params, uint64_t count, uint32_t max_batch, rs_trainer **out);
+int rs_trainer_gradient(rs_trainer *T, uint32_t n, uint32_t B, const float *Ac, const float *rc,
+                        const float *s, float *loss_out, float *grad_out);
+int rs_trainer_step(rs_trainer *T, uint32_t n, uint32_t B, const float *Ac, const float *rc,
+                    const float *s, float lr, float *mean_loss);
+int rs_trainer_get_params(const rs_trainer *T, float *params, uint64_t count);
+int rs_trainer_destroy(rs_trainer *T);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,151 @@
+"""CPU oracle of the learned coarse-correction network (numpy, FP64).
+
+TEST INFRASTRUCTURE ONLY: the checker for the GPU forward/backward kernels of the learned
+preconditioner.  The learned preconditioner has no counterpart in the reference; this file
+pins its mathematics: the backward pass here is checked against central finite differences of
+the forward pass (tests/test_learned_oracle.py), and the GPU gradients are then checked against
+this backward pass with the same BF16 rounding points (`bf16=True`).
+
+Network (per coarse graph with N nodes, dense coarse operator Ac, coarse residual rc):
+    d_I = Ac[I,I];  u_I = rc_I / d_I;  sigma = sqrt(mean(u^2)) + tiny
+    wn_IJ = -Ac[I,J] / d_I  (J != I), 0 on the diagonal
+    F = [u/sigma, log(d/geomean(d)), 1, 0]                       (N x 4)
+    H0 = relu(F Win + bin)
+    for l in 0..L-1:  M = wn H;  Y = H Ws[l] + M Wn[l] + b[l];  H = H + relu(Y)
+    o = H wo + bo;   e = u + sigma * o
+Loss for a (residual, error) pair with s = e_k.r_k = ||e_k||_A^2:
+    L = (e.Ac.e - 2 e.rc) / s + 1   (the A-norm error fraction left after the correction P e)
+"""
+from __future__ import annotations
+
+import numpy as np
+
+HID = 64
+NIN = 4
+LAYERS = 6
+TINY = 1e-30
+
+
+def param_count(layers: int = LAYERS, hid: int = HID) -> int:
+    return NIN * hid + hid + layers * (2 * hid * hid + hid) + hid + 1
+
+
+def unpack(theta: np.ndarray, layers: int = LAYERS, hid: int = HID) -> dict:
+    t = np.asarray(theta, dtype=np.float64)
+    o = 0
+
+    def take(n, shape):
+        nonlocal o
+        a = t[o:o + n].reshape(shape)
+        o += n
+        return a
+    p = {"Win": take(NIN * hid, (NIN, hid)), "bin": take(hid, (hid,)), "Ws": [], "Wn": [], "b": []}
+    for _ in range(layers):
+        p["Ws"].append(take(hid * hid, (hid, hid)))
+        p["Wn"].append(take(hid * hid, (hid, hid)))
+        p["b"].append(take(hid, (hid,)))
+    p["wo"] = take(hid, (hid,))
+    p["bo"] = take(1, (1,))
+    assert o == t.size
+    return p
+
+
+def pack(g: dict) -> np.ndarray:
+    parts = [g["Win"].ravel(), g["bin"].ravel()]
+    for l in range(len(g["Ws"])):
+        parts += [g["Ws"][l].ravel(), g["Wn"][l].ravel(), g["b"][l].ravel()]
+    parts += [g["wo"].ravel(), g["bo"].ravel()]
+    return np.concatenate(parts)
+
+
+def init_params(seed: int, layers: int = LAYERS, hid: int = HID) -> np.ndarray:
+    """He-style init for the hidden maps, zero output head (the untrained network is the
+    coarse Jacobi estimate e = u)."""
+    rng = np.random.default_rng(seed)
+    g = {"Win": rng.standard_normal((NIN, hid)) * np.sqrt(2.0 / NIN), "bin": np.zeros(hid),
+         "Ws": [], "Wn": [], "b": []}
+    for _ in range(layers):
+        g["Ws"].append(rng.standard_normal((hid, hid)) * np.sqrt(1.0 / hid) * 0.5)
+        g["Wn"].append(rng.standard_normal((hid, hid)) * np.sqrt(1.0 / hid) * 0.5)
+        g["b"].append(np.zeros(hid))
+    g["wo"] = np.zeros(hid)
+    g["bo"] = np.zeros(1)
+    return pack(g)
+
+
+def bf16(a: np.ndarray) -> np.ndarray:
+    """Round to bfloat16 (round to nearest even) and back to float64."""
+    f = np.asarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return (r.astype(np.uint32)).view(np.float32).astype(np.float64)
+
+
+def _inputs(Ac, rc):
+    d = np.diag(Ac).copy()
+    u = rc / d
+    sigma = np.sqrt(np.mean(u * u)) + TINY
+    logd = np.log(d)
+    F = np.zeros((Ac.shape[0], NIN))
+    F[:, 0] = u / sigma
+    F[:, 1] = logd - logd.mean()
+    F[:, 2] = 1.0
+    wn = -Ac / d[:, None]
+    np.fill_diagonal(wn, 0.0)
+    return d, u, sigma, F, wn
+
+
+def forward(theta, Ac, rc, s=None, bf16_points: bool = False, layers: int = LAYERS):
+    """Returns (e, cache) or (e, loss, cache) when s is given."""
+    p = unpack(theta, layers)
+    q = bf16 if bf16_points else (lambda a: np.asarray(a, dtype=np.float64))
+    d, u, sigma, F, wn = _inputs(Ac, rc)
+    Z0 = F @ p["Win"] + p["bin"]
+    H = [q(np.maximum(Z0, 0.0))]
+    Ms, Ys = [], []
+    for l in range(layers):
+        M = q(wn @ H[l])
+        Y = q(H[l]) @ q(p["Ws"][l]) + M @ q(p["Wn"][l]) + p["b"][l]
+        Ms.append(M)
+        Ys.append(Y)
+        H.append(q(H[l] + np.maximum(Y, 0.0)))
+    o = H[-1] @ p["wo"] + p["bo"][0]
+    e = u + sigma * o
+    cache = dict(p=p, d=d, u=u, sigma=sigma, F=F, wn=wn, Z0=Z0, H=H, M=Ms, Y=Ys, o=o, e=e)
+    if s is None:
+        return e, cache
+    loss = (e @ Ac @ e - 2.0 * e @ rc) / s + 1.0
+    return e, loss, cache
+
+
+def backward(theta, Ac, rc, s, bf16_points: bool = False, layers: int = LAYERS):
+    """Gradient of the loss with respect to theta (sigma and the inputs are constants)."""
+    q = bf16 if bf16_points else (lambda a: np.asarray(a, dtype=np.float64))
+    e, loss, c = forward(theta, Ac, rc, s, bf16_points, layers)
+    p = c["p"]
+    ge = 2.0 * (Ac @ e - rc) / s
+    go = ge * c["sigma"]
+    g = {"Ws": [None] * layers, "Wn": [None] * layers, "b": [None] * layers}
+    H = c["H"]
+    g["wo"] = H[-1].T @ go
+    g["bo"] = np.array([go.sum()])
+    dH = np.outer(go, p["wo"])
+    for l in reversed(range(layers)):
+        dY = q(dH * (c["Y"][l] > 0))
+        g["Ws"][l] = q(H[l]).T @ dY
+        g["Wn"][l] = c["M"][l].T @ dY
+        g["b"][l] = dY.sum(axis=0)
+        dM = dY @ q(p["Wn"][l]).T
+        dH = dH + dY @ q(p["Ws"][l]).T + c["wn"].T @ dM
+    dZ0 = dH * (c["Z0"] > 0)
+    g["Win"] = c["F"].T @ dZ0
+    g["bin"] = dZ0.sum(axis=0)
+    return pack(g), loss
+
+
+def galerkin(A, agg, n_agg):
+    """Dense Pt A P for piecewise-constant aggregation (A scipy sparse)."""
+    import scipy.sparse as sp
+    n = A.shape[0]
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, n_agg))
+    return (P.T @ A @ P).toarray()

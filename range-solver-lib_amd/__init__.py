@@ -57,7 +57,8 @@ class Info(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("inner_iterations", C.c_uint32),
                 ("reason", C.c_int32), ("converged", C.c_int32), ("error", C.c_double),
                 ("trend", C.c_double), ("equation_scale", C.c_double), ("a_norm", C.c_double),
-                ("b_norm", C.c_double), ("true_relres", C.c_double), ("solve_ms", C.c_double)]
+                ("b_norm", C.c_double), ("true_relres", C.c_double), ("solve_ms", C.c_double),
+                ("precond_applications", C.c_uint32), ("fallback", C.c_uint32)]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -107,6 +108,19 @@ _SIGS = {
     "rs_assemble_heat_device": [_P, _P, _P, _P, _P, C.c_int32, C.c_double, C.c_double, _P],
     "rs_assemble_stress": [_P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P],
     "rs_assembly_dofs_to_nodes": [_P, _P, _P], "rs_assembly_destroy": [_P],
+    "rs_aggregates_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                              _P, _u32p],
+    "rs_learned_create": [_P, _P, C.c_uint32, C.POINTER(_P)],
+    "rs_learned_set_params": [_P, _P, C.c_uint64],
+    "rs_learned_info": [_P, _u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "rs_learned_setup": [_P, _P], "rs_learned_coarse_matrix": [_P, _P],
+    "rs_learned_restrict_device": [_P, _P, _P], "rs_learned_apply_coarse": [_P, _P, _P],
+    "rs_learned_pair": [_P, _P, _P, _P, _P, _P, _f64p], "rs_learned_destroy": [_P],
+    "rs_solver_solve_learned_device": [_P, _P, _P, _P, _P, C.POINTER(Info)],
+    "rs_trainer_create": [_P, C.c_uint64, C.c_uint32, C.POINTER(_P)],
+    "rs_trainer_gradient": [_P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P],
+    "rs_trainer_step": [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_float, C.POINTER(C.c_float)],
+    "rs_trainer_get_params": [_P, _P, C.c_uint64], "rs_trainer_destroy": [_P],
 }
 
 _lib = None
@@ -381,6 +395,14 @@ class RMatrixSolver:
         self.info = info
         return info
 
+    def solve_learned_device(self, A: SparseMatrix, L: "LearnedPreconditioner", db: DeviceArray,
+                             dx: DeviceArray) -> Info:
+        """Flexible CG with the learned two-level preconditioner (mixed precision)."""
+        info = Info()
+        _check(_lib.rs_solver_solve_learned_device(self.h, A.h, L.h, db.ptr, dx.ptr, C.byref(info)))
+        self.info = info
+        return info
+
     def history(self) -> np.ndarray:
         cnt = C.c_uint32()
         _check(_lib.rs_solver_history(self.h, None, 0, C.byref(cnt)))
@@ -507,4 +529,116 @@ class Assembly:
     def __del__(self):
         if _lib is not None and getattr(self, "h", None):
             _lib.rs_assembly_destroy(self.h)
+            self.h = None
+
+
+# ------------------------------------------------------------------------------ learned preconditioner
+N_PARAMS = 49921
+TRAIN_MAX_NODES = 64
+
+
+def aggregates_by_bins(xyz, book, dofs_per_node: int, bins) -> tuple[np.ndarray, int]:
+    """Coordinate-bin aggregates of the enabled DOFs (bins = (nbx, nby, nbz))."""
+    load()
+    xyz = _f64(xyz).reshape(-1)
+    book = _u32(book)
+    n_dof = int((book != RS_EOD).sum())
+    agg = np.empty(max(1, n_dof), np.uint32)
+    n = C.c_uint32()
+    _check(_lib.rs_aggregates_by_bins(xyz.size // 3, _ptr(xyz), dofs_per_node, _ptr(book), bins[0],
+                                      bins[1], bins[2], _ptr(agg), C.byref(n)))
+    return agg[:n_dof], n.value
+
+
+class LearnedPreconditioner:
+    """Two-level preconditioner z = D^-1 r + P N(P^T r) with a message-passing network N."""
+
+    def __init__(self, A: SparseMatrix, agg, n_agg: int, params=None):
+        load()
+        self.A = A
+        self.agg = _u32(agg)
+        self.h = C.c_void_p()
+        _check(_lib.rs_learned_create(A.h, _ptr(self.agg), n_agg, C.byref(self.h)))
+        nc, npar, cnnz = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(_lib.rs_learned_info(self.h, C.byref(nc), C.byref(npar), C.byref(cnnz)))
+        self.n_coarse, self.n_params, self.coarse_nnz = nc.value, npar.value, cnnz.value
+        if params is not None:
+            self.set_params(params)
+
+    def set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(_lib.rs_learned_set_params(self.h, _ptr(p), p.size))
+
+    def setup(self, A: SparseMatrix | None = None) -> None:
+        _check(_lib.rs_learned_setup(self.h, (A or self.A).h))
+
+    def coarse_matrix(self) -> np.ndarray:
+        out = np.empty(self.n_coarse * self.n_coarse)
+        _check(_lib.rs_learned_coarse_matrix(self.h, _ptr(out)))
+        return out.reshape(self.n_coarse, self.n_coarse)
+
+    def restrict(self, dr: DeviceArray) -> np.ndarray:
+        out = np.empty(self.n_coarse, np.float32)
+        _check(_lib.rs_learned_restrict_device(self.h, dr.ptr, _ptr(out)))
+        return out
+
+    def apply_coarse(self, rc) -> np.ndarray:
+        r = np.ascontiguousarray(rc, dtype=np.float32)
+        out = np.empty(self.n_coarse, np.float32)
+        _check(_lib.rs_learned_apply_coarse(self.h, _ptr(r), _ptr(out)))
+        return out
+
+    def pair(self, A: SparseMatrix, db: DeviceArray, dx: DeviceArray, dxs: DeviceArray):
+        rc = np.empty(self.n_coarse, np.float32)
+        s = C.c_double()
+        _check(_lib.rs_learned_pair(self.h, A.h, db.ptr, dx.ptr, dxs.ptr, _ptr(rc), C.byref(s)))
+        return rc, s.value
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.rs_learned_destroy(self.h)
+            self.h = None
+
+
+class Trainer:
+    """GPU training of the network: forward/backward/Adam kernels, batches of <= max_batch."""
+
+    def __init__(self, params, max_batch: int = 64):
+        load()
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self.max_batch = max_batch
+        self.h = C.c_void_p()
+        _check(_lib.rs_trainer_create(_ptr(p), p.size, max_batch, C.byref(self.h)))
+
+    @staticmethod
+    def _pad(Ac, rc, s):
+        Ac = np.asarray(Ac, dtype=np.float64)
+        B, n = Ac.shape[0], Ac.shape[1]
+        A = np.zeros((B, TRAIN_MAX_NODES, TRAIN_MAX_NODES), np.float32)
+        A[:, :n, :n] = Ac
+        R = np.zeros((B, TRAIN_MAX_NODES), np.float32)
+        R[:, :n] = rc
+        return n, B, A, R, np.ascontiguousarray(s, dtype=np.float32)
+
+    def gradient(self, Ac, rc, s):
+        n, B, A, R, S = self._pad(Ac, rc, s)
+        loss = np.empty(B, np.float32)
+        grad = np.empty(N_PARAMS, np.float32)
+        _check(_lib.rs_trainer_gradient(self.h, n, B, _ptr(A), _ptr(R), _ptr(S), _ptr(loss), _ptr(grad)))
+        return loss, grad
+
+    def step(self, Ac, rc, s, lr: float) -> float:
+        n, B, A, R, S = self._pad(Ac, rc, s)
+        m = C.c_float()
+        _check(_lib.rs_trainer_step(self.h, n, B, _ptr(A), _ptr(R), _ptr(S), lr, C.byref(m)))
+        return m.value
+
+    def params(self) -> np.ndarray:
+        out = np.empty(N_PARAMS, np.float32)
+        _check(_lib.rs_trainer_get_params(self.h, _ptr(out), out.size))
+        return out
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.rs_trainer_destroy(self.h)
             self.h = None

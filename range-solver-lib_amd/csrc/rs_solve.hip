@@ -46,7 +46,7 @@ struct SolveState {
     double cvg, scale, rmax, t0;
     double t1, t2, t3, t4;
     uint32_t it, done, reason, n_outer;
-    uint32_t stop_rule, hist_cap, mode, pad;
+    uint32_t stop_rule, hist_cap, mode, it0;   // it0: first iteration of the current CG segment
 };
 enum { kStopRunning = 0, kStopDone = 1, kStopPause = 2 };
 constexpr double kReliableDelta = 0.1;   // FP64 residual replacement when |r| drops 10x
@@ -76,6 +76,8 @@ enum FinOp {
     FIN_MP_UPDATE,      // mixed: recursive residual, pause for replacement
     FIN_MP_TRUE,        // mixed: FP64 true residual (flag: first)
     FIN_MP_RHO,         // mixed: rho after a replacement (flag: first)
+    FIN_FCG_BETA,       // flexible CG: rho = r.z, beta = -(z.q_prev) / (p.q)_prev
+    FIN_FCG_UPDATE,     // flexible CG: recursive residual, pause for replacement
 };
 
 constexpr int kFinBlock = 1024;
@@ -84,7 +86,9 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ pa
                                                    int nv, int op, int flag, SolveState *st,
                                                    double *__restrict__ hist)
 {
-    if ((op == FIN_DOT || op == FIN_UPDATE || op == FIN_MP_UPDATE) && st->done) return;
+    if ((op == FIN_DOT || op == FIN_UPDATE || op == FIN_MP_UPDATE || op == FIN_FCG_BETA || op == FIN_FCG_UPDATE) &&
+        st->done)
+        return;
     __shared__ double lds[4 * (kFinBlock / kWave)];
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
     // all loads of a thread are independent and issued before the sums (up to 4 partials per
@@ -122,6 +126,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ pa
             else dot = eps;
             st->alpha = st->rho0 / dot;
             st->rho1 = st->rho0;
+            st->t3 = dot;
             break;
         }
         case FIN_UPDATE: {
@@ -202,6 +207,33 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ pa
             if (!flag) st->beta = v[0] / st->rho1;
             st->rho0 = v[0];
             break;
+        case FIN_FCG_BETA:
+            st->rho0 = v[0];
+            st->beta = (st->it == st->it0 || st->t3 == 0.0) ? 0.0 : -v[1] / st->t3;
+            break;
+        case FIN_FCG_UPDATE: {
+            double rn = sqrt(v[0]);
+            double err = rn / fmax(st->bn, eps);
+            double trend = err - st->error;
+            st->trend = trend;
+            st->error = err;
+            uint32_t it = st->it;
+            if (it < st->hist_cap) {
+                hist[2 * it] = err;
+                hist[2 * it + 1] = trend;
+            }
+            st->it = it + 1;
+            if (!dev_isnormal(err)) {
+                st->reason = RS_REASON_NONNORMAL;
+                st->done = kStopDone;
+            } else if (it + 1 >= st->n_outer) {
+                st->reason = RS_REASON_MAXIT;
+                st->done = kStopDone;
+            } else if (rn < kReliableDelta * st->rmax || err < st->cvg) {
+                st->done = kStopPause;
+            }
+            break;
+        }
     }
 }
 
@@ -513,6 +545,17 @@ __global__ void k_state_init(SolveState *st, double an, double cvg, double prev_
     st->stop_rule = stop_rule;
     st->hist_cap = hist_cap;
     st->mode = mode;
+    st->it0 = 0;
+}
+
+// Start a new CG segment on the same solution (the learned solve's Jacobi-CG fallback).
+__global__ void k_state_restart(SolveState *st)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    st->it0 = st->it;
+    st->done = kStopRunning;
+    st->reason = RS_REASON_RUNNING;
+    st->beta = 0.0;
 }
 
 // y = b*s ; x *= s   (src/rmatrixsolver.cpp:181-184)
@@ -568,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs a)
     constexpr int WM = W == 0 ? 16 : W;
     const uint32_t ww = W == 0 ? a.pw : W;
     const uint64_t nb = a.A.n / ww;
-    const bool first = a.st->it == 0;
+    const bool first = a.st->it == a.st->it0;
     const double beta = a.st->beta;
     for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
         double r[WM], z[WM];
@@ -646,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_dir(MpArgs a)
     constexpr int WM = W == 0 ? 16 : W;
     const uint32_t ww = W == 0 ? a.pw : W;
     const uint64_t nb = a.A.n / ww;
-    const bool first = a.st->it == 0;
+    const bool first = a.st->it == a.st->it0;
     const float beta = (float)a.st->beta;
     for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
         float r[WM], z[WM];
@@ -1335,7 +1378,7 @@ struct rs_solver {
     std::vector<double> history;
     // device workspace
     DBuf<double> y, r, p, q, ro, w, V, Z, hbuf;
-    DBuf<float> r32, p32, q32, dx32;
+    DBuf<float> r32, p32, q32, dx32, z32;
     DBuf<double> partials, hist;
     DBuf<SolveState> st;
     // captured iteration graph (or, with RS_NO_GRAPH=1 in the environment, the same S
@@ -1736,15 +1779,23 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
     return RS_OK;
 }
 
-static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d_x, int32_t ptype,
-                        uint32_t block_size, rs_info *info)
+}  // namespace rs
+struct rs_learned;
+namespace rs {
+static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learned &L, double *d_x, uint32_t &done,
+                             uint32_t &refinements);
+
+static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, double *d_x, int32_t ptype,
+                            uint32_t block_size, rs_learned *L, rs_info *info)
 {
     const rs_conf &c = S.conf;
     if (c.solver_type != RS_SOLVER_CG && c.solver_type != RS_SOLVER_GMRES)
         return fail(RS_ERR_UNKNOWN_SOLVER, "Unknown matrix solver type '" + std::to_string(c.solver_type) + "'");
-    if (ptype < RS_PRECOND_NONE || ptype > RS_PRECOND_BLOCK_JACOBI)
+    const bool learned = L != nullptr;
+    if (!learned && (ptype < RS_PRECOND_NONE || ptype > RS_PRECOND_BLOCK_JACOBI))
         return fail(RS_ERR_INVALID_PRECOND, "Invalid matrix preconditioner type '" + std::to_string(ptype) + "'");
-    bool mixed = c.precision == RS_PRECISION_MIXED;
+    if (learned) ptype = RS_PRECOND_JACOBI;   // the smoother part and the fallback
+    bool mixed = c.precision == RS_PRECISION_MIXED || learned;
     bool gmres = c.solver_type == RS_SOLVER_GMRES;
     if (mixed && gmres) return fail(RS_ERR_UNSUPPORTED, "mixed precision is implemented for CG only");
     int rc;
@@ -1770,12 +1821,32 @@ static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d
     fin(S.partials.p, nvb, 1, FIN_SCALE, 0, S.st.p, S.hist.p);
     k_apply_scale<<<nvb, kBlock, 0, stream>>>(d_b, S.y.p, d_x, n, S.st.p);
     RS_HIP(hipGetLastError());
-    uint32_t done = 0, iters = 0, inner = 0, refinements = 0;
+    uint32_t done = 0, iters = 0, inner = 0, refinements = 0, applications = 0, fallback = 0;
     S.history.assign(2ull * std::min(hist_cap, std::max(1u, c.n_outer_iterations)), 0.0);
     if (n > 0) {
-        if (gmres) rc = solve_gmres(S, A, P, d_x, iters, inner);
-        else if (mixed) rc = solve_cg_mixed(S, A, P, d_x, done, refinements);
-        else rc = solve_cg64(S, A, P, d_x, done);
+        if (gmres) {
+            rc = solve_gmres(S, A, P, d_x, iters, inner);
+        } else if (learned) {
+            rc = solve_fcg_learned(S, A, P, *L, d_x, done, refinements);
+            S.body = nullptr;
+            if (rc) return rc;
+            SolveState hs;
+            RS_HIP(hipMemcpyAsync(&hs, S.st.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
+            RS_HIP(hipStreamSynchronize(stream));
+            applications = hs.it;
+            if (hs.reason != RS_REASON_CONVERGED) {
+                // Jacobi-CG takes over from the current solution
+                fallback = 1;
+                k_state_restart<<<1, 64, 0, stream>>>(S.st.p);
+                uint32_t more = 0;
+                rc = solve_cg_mixed(S, A, P, d_x, done, more);
+                refinements += more;
+            }
+        } else if (mixed) {
+            rc = solve_cg_mixed(S, A, P, d_x, done, refinements);
+        } else {
+            rc = solve_cg64(S, A, P, d_x, done);
+        }
         S.body = nullptr;  // the iteration body refers to this call's locals
         if (rc) return rc;
     }
@@ -1840,8 +1911,16 @@ static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d
         info->b_norm = b_norm;
         info->true_relres = true_relres;
         info->solve_ms = ms;
+        info->precond_applications = applications;
+        info->fallback = fallback;
     }
     return RS_OK;
+}
+
+static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d_x, int32_t ptype,
+                        uint32_t block_size, rs_info *info)
+{
+    return solve_device_ext(S, A, d_b, d_x, ptype, block_size, nullptr, info);
 }
 
 }  // namespace rs
@@ -1921,3 +2000,5 @@ extern "C" int rs_solver_destroy(rs_solver *S)
     delete S;
     return RS_OK;
 }
+
+#include "rs_learned.inc.hip"
