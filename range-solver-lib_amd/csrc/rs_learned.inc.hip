@@ -743,7 +743,10 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
             no_gain++;
         }
     }
-    k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
+    // a segment that ended on a non-finite residual leaves x as of its last replacement
+    RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    RS_HIP(hipStreamSynchronize(stream));
+    if (h.reason != RS_REASON_NONNORMAL) k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
     RS_HIP(hipGetLastError());
     return RS_OK;
 }
