@@ -235,45 +235,74 @@ int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node)
 int rs_assembly_destroy(rs_assembly *S);
 
 /* ---------------------------------------------------------------- learned preconditioner */
-/* No counterpart in the reference (a new preconditioner type, RS_PRECOND_LEARNED): a two-level
- * preconditioner z = D^-1 r + P e whose coarse correction e is computed by a 6-layer, 64-wide
- * message-passing network over the coarse graph of caller-given aggregates (P piecewise
- * constant, coarse operator P^T A P), applied inside flexible CG in mixed precision with
- * Jacobi-CG as the fallback.  At most 512 aggregates. */
+/* No counterpart in the reference (a new preconditioner type, RS_PRECOND_LEARNED): one V-cycle
+ * over an aggregation hierarchy (piecewise-constant prolongations, Galerkin operators
+ * P^T A P, Chebyshev smoothing in FP32) whose coarsest level (<= 512 nodes) is solved by a
+ * 6-layer, 64-wide message-passing network, applied inside flexible CG in mixed precision
+ * with Jacobi-CG as the fallback. */
 typedef struct rs_learned rs_learned;
+enum {
+    RS_COARSE_NETWORK = 0,   /* the learned network */
+    RS_COARSE_JACOBI = 1,    /* coarse Jacobi: what the untrained network computes */
+    RS_COARSE_EXACT = 2      /* dense inverse of the coarsest operator */
+};
 /* Aggregates from node coordinates: nbx*nby*nbz bins over the bounding box; agg_of_dof gets
  * the bin of each enabled position of the node book (ids compacted to 0..n_agg-1). */
 int rs_aggregates_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node,
                           const uint32_t *book, uint32_t nbx, uint32_t nby, uint32_t nbz,
                           uint32_t *agg_of_dof, uint32_t *n_agg);
-int rs_learned_create(const rs_matrix *A, const uint32_t *agg_of_row, uint32_t n_agg,
-                      rs_learned **out);
+/* Nested coordinate bins: level 1 uses nbx*nby*nbz bins, every further level halves them,
+ * until a level has at most max_coarse nodes.  agg receives the level maps one after the
+ * other (level k maps the nodes of level k-1, level 0 being the enabled positions, to those
+ * of level k; capacity agg_capacity entries, 2 * n_enabled always suffices), n_agg[k-1] the
+ * node count of level k, *n_levels the number of coarse levels (<= max_levels). */
+int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node,
+                         const uint32_t *book, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                         uint32_t max_coarse, uint32_t max_levels, uint32_t *agg,
+                         uint64_t agg_capacity, uint32_t *n_agg, uint32_t *n_levels);
+/* Hierarchy for the pattern of A: n_levels coarse levels given by their maps (as above). */
+int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const uint32_t *agg,
+                      const uint32_t *n_agg, rs_learned **out);
 /* Network parameters, FP32, 49,921 values (layout: W_in[4][64], b_in[64], 6 x {W_s[64][64],
  * W_n[64][64], b[64]}, w_out[64], b_out). */
 int rs_learned_set_params(rs_learned *L, const float *params, uint64_t count);
-int rs_learned_info(const rs_learned *L, uint32_t *n_coarse, uint64_t *n_params, uint64_t *coarse_nnz);
-/* Coarse operator for the current matrix values (done by the solve as well). */
+/* Smoothing steps before (0..4) and after (1..4) the coarse correction, coarsest solver. */
+int rs_learned_set_cycle(rs_learned *L, uint32_t pre, uint32_t post, int coarse_mode);
+/* Levels including the finest; level_rows / level_nnz hold n_levels entries when given. */
+int rs_learned_info(const rs_learned *L, uint32_t *n_levels, uint32_t *level_rows,
+                    uint64_t *level_nnz, uint64_t *n_params);
+/* Level operators, diagonals and smoother bounds for the current values of A (done by the
+ * solve as well). */
 int rs_learned_setup(rs_learned *L, rs_matrix *A);
+int rs_learned_level_lambda(const rs_learned *L, double *lambda);        /* n_levels bounds */
+int rs_learned_level_matrix(const rs_learned *L, uint32_t level, uint32_t *row_ptr,
+                            uint32_t *col, double *val);
 int rs_learned_coarse_matrix(const rs_learned *L, double *dense);         /* n_coarse^2 */
 int rs_learned_restrict_device(rs_learned *L, const double *d_r, float *rc_host);
 int rs_learned_apply_coarse(rs_learned *L, const float *rc_host, float *e_host);
-/* Training pair of a CG iterate x of A x* = b (device arrays): r_c = P^T (b - A x) and
- * s = (x* - x) . (b - A x), the squared A-norm of the iterate's error. */
-int rs_learned_pair(rs_learned *L, rs_matrix *A, const double *d_b, const double *d_x,
-                    const double *d_xstar, float *rc_host, double *s_out);
+/* One V-cycle z = B r (host vectors of length n). */
+int rs_learned_apply(rs_learned *L, rs_matrix *A, const float *r_host, float *z_host);
+/* Training data: the coarsest right-hand side of each of the first `capacity` iterations of
+ * the following solves is kept on the device; rs_learned_records copies `count` of them. */
+int rs_learned_record(rs_learned *L, uint32_t capacity);
+int rs_learned_records(const rs_learned *L, uint32_t count, float *out);
 int rs_learned_destroy(rs_learned *L);
 /* Flexible CG with the learned preconditioner (solver conf: CG; b, x device arrays). */
 int rs_solver_solve_learned_device(rs_solver *S, rs_matrix *A, rs_learned *L, const double *d_b,
                                    double *d_x, rs_info *info);
 
 /* Training of the network on the GPU (forward, backward and Adam kernels).  A batch holds B
- * samples on coarse graphs of n <= 64 nodes: dense coarse operators Ac[B][64][64] (zero
- * padded), coarse residuals rc[B][64] and s[B] = e_k . r_k of the (residual, error) pair. */
+ * samples on one coarse pattern (cptr, ccol: n <= 512 rows, nnz entries, structurally
+ * symmetric with its diagonal): operator values cval[B][nnz], coarse residuals rc[B][n] and
+ * s[B] = rc . A^-1 . rc. */
 typedef struct rs_trainer rs_trainer;
-int rs_trainer_create(const float *params, uint64_t count, uint32_t max_batch, rs_trainer **out);
-int rs_trainer_gradient(rs_trainer *T, uint32_t n, uint32_t B, const float *Ac, const float *rc,
+int rs_trainer_create(const float *params, uint64_t count, uint32_t max_batch, uint32_t max_nodes,
+                      uint64_t max_nnz, rs_trainer **out);
+int rs_trainer_gradient(rs_trainer *T, uint32_t n, uint64_t nnz, const uint32_t *cptr,
+                        const uint32_t *ccol, uint32_t B, const float *cval, const float *rc,
                         const float *s, float *loss_out, float *grad_out);
-int rs_trainer_step(rs_trainer *T, uint32_t n, uint32_t B, const float *Ac, const float *rc,
+int rs_trainer_step(rs_trainer *T, uint32_t n, uint64_t nnz, const uint32_t *cptr,
+                    const uint32_t *ccol, uint32_t B, const float *cval, const float *rc,
                     const float *s, float lr, float *mean_loss);
 int rs_trainer_get_params(const rs_trainer *T, float *params, uint64_t count);
 int rs_trainer_destroy(rs_trainer *T);

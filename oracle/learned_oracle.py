@@ -15,6 +15,13 @@ Network (per coarse graph with N nodes, dense coarse operator Ac, coarse residua
     o = H wo + bo;   e = u + sigma * o
 Loss for a (residual, error) pair with s = e_k.r_k = ||e_k||_A^2:
     L = (e.Ac.e - 2 e.rc) / s + 1   (the A-norm error fraction left after the correction P e)
+
+Multilevel cycle (the preconditioner the network sits in): piecewise-constant prolongations
+P_l from level maps, Galerkin operators A_{l+1} = P_l^T A_l P_l, Chebyshev smoothing in D^-1 A_l
+over [lambda/30, 1.05 lambda] with lambda the Gershgorin bound of D^-1 A_l (Adams, Brezina, Hu,
+Tuminaro 2003), pre/post steps, a coarsest solver.  Pinned by closed forms in
+tests/test_learned_oracle.py: the smoother on D^-1 A = I against the Chebyshev residual
+polynomial, the two-level projection property, symmetry of the symmetric cycle.
 """
 from __future__ import annotations
 
@@ -149,3 +156,80 @@ def galerkin(A, agg, n_agg):
     n = A.shape[0]
     P = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, n_agg))
     return (P.T @ A @ P).toarray()
+
+
+# ------------------------------------------------------------------------------ multilevel cycle
+def prolongation(m, n_next):
+    import scipy.sparse as sp
+    m = np.asarray(m, dtype=np.int64)
+    return sp.csr_matrix((np.ones(m.size), (np.arange(m.size), m)), shape=(m.size, n_next))
+
+
+def hierarchy(A, maps):
+    """Level operators [A_0, A_1, ...] and prolongations for level maps (scipy sparse A)."""
+    levels, Ps = [A.tocsr()], []
+    for m in maps:
+        P = prolongation(m, int(np.max(m)) + 1)
+        Ps.append(P)
+        levels.append((P.T @ levels[-1] @ P).tocsr())
+    return levels, Ps
+
+
+def gershgorin(A):
+    A = A.tocsr()
+    d = A.diagonal()
+    s = np.asarray(abs(A).sum(axis=1)).ravel()
+    ok = d != 0
+    return float(np.max(s[ok] / np.abs(d[ok])))
+
+
+def cheb_coefs(lam, steps=4):
+    """(c1, c2) per step: d_k = c1 d_{k-1} + c2 D^-1 res_k over [lam/30, 1.05 lam]."""
+    hi, lo = 1.05 * lam, lam / 30.0
+    theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    sig = theta / delta
+    rho = 1.0 / sig
+    out = [(0.0, 1.0 / theta)]
+    for _ in range(1, steps):
+        rn = 1.0 / (2.0 * sig - rho)
+        out.append((rn * rho, 2.0 * rn / delta))
+        rho = rn
+    return out
+
+
+def cheb_smooth(A, dinv, coefs, r, z, steps):
+    """`steps` Chebyshev steps from z (z None: zero start)."""
+    d = None
+    for k in range(steps):
+        c1, c2 = coefs[k]
+        res = r if z is None else r - A @ z
+        dd = c2 * dinv * res
+        if k > 0:
+            dd = dd + c1 * d
+        d = dd
+        z = dd if z is None else z + dd
+    return z
+
+
+def vcycle(levels, Ps, r, pre, post, coarse, lams=None, level=0):
+    """z = B r; coarse(Ac, r) solves the coarsest level."""
+    A = levels[level]
+    if level == len(levels) - 1:
+        return coarse(A, r)
+    lam = gershgorin(A) if lams is None else lams[level]
+    coefs = cheb_coefs(lam)
+    dg = A.diagonal()
+    dinv = np.where(dg != 0, 1.0 / np.where(dg != 0, dg, 1.0), 0.0)
+    z = cheb_smooth(A, dinv, coefs, r, None, pre) if pre > 0 else None
+    res = r if z is None else r - A @ z
+    e = vcycle(levels, Ps, Ps[level].T @ res, pre, post, coarse, lams, level + 1)
+    z = Ps[level] @ e if z is None else z + Ps[level] @ e
+    return cheb_smooth(A, dinv, coefs, r, z, post)
+
+
+def coarse_exact(Ac, r):
+    return np.linalg.solve(Ac.toarray(), r)
+
+
+def coarse_jacobi(Ac, r):
+    return r / Ac.diagonal()

@@ -110,16 +110,24 @@ _SIGS = {
     "rs_assembly_dofs_to_nodes": [_P, _P, _P], "rs_assembly_destroy": [_P],
     "rs_aggregates_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                               _P, _u32p],
-    "rs_learned_create": [_P, _P, C.c_uint32, C.POINTER(_P)],
+    "rs_hierarchy_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                             C.c_uint32, C.c_uint32, _P, C.c_uint64, _P, _u32p],
+    "rs_learned_create": [_P, C.c_uint32, _P, _P, C.POINTER(_P)],
     "rs_learned_set_params": [_P, _P, C.c_uint64],
-    "rs_learned_info": [_P, _u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
-    "rs_learned_setup": [_P, _P], "rs_learned_coarse_matrix": [_P, _P],
+    "rs_learned_set_cycle": [_P, C.c_uint32, C.c_uint32, C.c_int],
+    "rs_learned_info": [_P, _u32p, _P, _P, C.POINTER(C.c_uint64)],
+    "rs_learned_setup": [_P, _P], "rs_learned_level_lambda": [_P, _P],
+    "rs_learned_level_matrix": [_P, C.c_uint32, _P, _P, _P],
+    "rs_learned_coarse_matrix": [_P, _P],
     "rs_learned_restrict_device": [_P, _P, _P], "rs_learned_apply_coarse": [_P, _P, _P],
-    "rs_learned_pair": [_P, _P, _P, _P, _P, _P, _f64p], "rs_learned_destroy": [_P],
+    "rs_learned_apply": [_P, _P, _P, _P],
+    "rs_learned_record": [_P, C.c_uint32], "rs_learned_records": [_P, C.c_uint32, _P],
+    "rs_learned_destroy": [_P],
     "rs_solver_solve_learned_device": [_P, _P, _P, _P, _P, C.POINTER(Info)],
-    "rs_trainer_create": [_P, C.c_uint64, C.c_uint32, C.POINTER(_P)],
-    "rs_trainer_gradient": [_P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P],
-    "rs_trainer_step": [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_float, C.POINTER(C.c_float)],
+    "rs_trainer_create": [_P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(_P)],
+    "rs_trainer_gradient": [_P, C.c_uint32, C.c_uint64, _P, _P, C.c_uint32, _P, _P, _P, _P, _P],
+    "rs_trainer_step": [_P, C.c_uint32, C.c_uint64, _P, _P, C.c_uint32, _P, _P, _P, C.c_float,
+                        C.POINTER(C.c_float)],
     "rs_trainer_get_params": [_P, _P, C.c_uint64], "rs_trainer_destroy": [_P],
 }
 
@@ -534,7 +542,8 @@ class Assembly:
 
 # ------------------------------------------------------------------------------ learned preconditioner
 N_PARAMS = 49921
-TRAIN_MAX_NODES = 64
+MAX_COARSE = 512
+COARSE_NETWORK, COARSE_JACOBI, COARSE_EXACT = 0, 1, 2
 
 
 def aggregates_by_bins(xyz, book, dofs_per_node: int, bins) -> tuple[np.ndarray, int]:
@@ -550,18 +559,57 @@ def aggregates_by_bins(xyz, book, dofs_per_node: int, bins) -> tuple[np.ndarray,
     return agg[:n_dof], n.value
 
 
-class LearnedPreconditioner:
-    """Two-level preconditioner z = D^-1 r + P N(P^T r) with a message-passing network N."""
+def hierarchy_by_bins(xyz, book, dofs_per_node: int, bins, max_coarse: int = MAX_COARSE,
+                      max_levels: int = 16) -> list[np.ndarray]:
+    """Nested coordinate bins, halved per level until a level has <= max_coarse nodes.
+    Returns the level maps: maps[k] sends the nodes of level k (level 0: the DOFs) to level k+1."""
+    load()
+    xyz = _f64(xyz).reshape(-1)
+    book = _u32(book)
+    n_dof = int((book != RS_EOD).sum())
+    cap = 2 * max(n_dof, 1) + 16
+    agg = np.empty(cap, np.uint32)
+    n_agg = np.empty(max_levels, np.uint32)
+    nl = C.c_uint32()
+    _check(_lib.rs_hierarchy_by_bins(xyz.size // 3, _ptr(xyz), dofs_per_node, _ptr(book), bins[0],
+                                     bins[1], bins[2], max_coarse, max_levels, _ptr(agg), cap,
+                                     _ptr(n_agg), C.byref(nl)))
+    maps, off, n = [], 0, n_dof
+    for k in range(nl.value):
+        maps.append(agg[off:off + n].copy())
+        off += n
+        n = int(n_agg[k])
+    return maps
 
-    def __init__(self, A: SparseMatrix, agg, n_agg: int, params=None):
+
+class LearnedPreconditioner:
+    """V-cycle over an aggregation hierarchy with a learned coarsest solver (flexible CG).
+
+    levels: a list of level maps (levels[k] sends the nodes of level k to those of level k+1),
+    or a single map (one coarse level)."""
+
+    def __init__(self, A: SparseMatrix, levels, n_agg=None, params=None):
         load()
+        if isinstance(levels, np.ndarray) and levels.ndim == 1 and n_agg is not None and np.isscalar(n_agg):
+            levels = [levels]
+        maps = [_u32(m) for m in levels]
+        sizes = [int(m.max()) + 1 for m in maps] if n_agg is None else \
+            ([int(n_agg)] if np.isscalar(n_agg) else [int(v) for v in n_agg])
         self.A = A
-        self.agg = _u32(agg)
+        self._agg = np.ascontiguousarray(np.concatenate(maps)) if maps else np.zeros(1, np.uint32)
+        self._nagg = np.asarray(sizes, np.uint32)
         self.h = C.c_void_p()
-        _check(_lib.rs_learned_create(A.h, _ptr(self.agg), n_agg, C.byref(self.h)))
-        nc, npar, cnnz = C.c_uint32(), C.c_uint64(), C.c_uint64()
-        _check(_lib.rs_learned_info(self.h, C.byref(nc), C.byref(npar), C.byref(cnnz)))
-        self.n_coarse, self.n_params, self.coarse_nnz = nc.value, npar.value, cnnz.value
+        _check(_lib.rs_learned_create(A.h, len(maps), _ptr(self._agg), _ptr(self._nagg), C.byref(self.h)))
+        nl = C.c_uint32()
+        rows = np.zeros(32, np.uint32)
+        nnz = np.zeros(32, np.uint64)
+        npar = C.c_uint64()
+        _check(_lib.rs_learned_info(self.h, C.byref(nl), _ptr(rows), _ptr(nnz), C.byref(npar)))
+        self.n_levels = nl.value
+        self.level_rows = rows[:nl.value].copy()
+        self.level_nnz = nnz[:nl.value].copy()
+        self.n_coarse = int(self.level_rows[-1])
+        self.n_params = npar.value
         if params is not None:
             self.set_params(params)
 
@@ -569,8 +617,25 @@ class LearnedPreconditioner:
         p = np.ascontiguousarray(params, dtype=np.float32)
         _check(_lib.rs_learned_set_params(self.h, _ptr(p), p.size))
 
+    def set_cycle(self, pre: int = 1, post: int = 1, coarse: int = COARSE_NETWORK) -> None:
+        _check(_lib.rs_learned_set_cycle(self.h, pre, post, coarse))
+
     def setup(self, A: SparseMatrix | None = None) -> None:
         _check(_lib.rs_learned_setup(self.h, (A or self.A).h))
+
+    def level_lambda(self) -> np.ndarray:
+        out = np.empty(self.n_levels)
+        _check(_lib.rs_learned_level_lambda(self.h, _ptr(out)))
+        return out
+
+    def level_matrix(self, level: int):
+        import scipy.sparse as sp
+        n, nnz = int(self.level_rows[level]), int(self.level_nnz[level])
+        rp = np.empty(n + 1, np.uint32)
+        col = np.empty(max(nnz, 1), np.uint32)
+        val = np.empty(max(nnz, 1))
+        _check(_lib.rs_learned_level_matrix(self.h, level, _ptr(rp), _ptr(col), _ptr(val)))
+        return sp.csr_matrix((val[:nnz], col[:nnz], rp), shape=(n, n))
 
     def coarse_matrix(self) -> np.ndarray:
         out = np.empty(self.n_coarse * self.n_coarse)
@@ -588,11 +653,20 @@ class LearnedPreconditioner:
         _check(_lib.rs_learned_apply_coarse(self.h, _ptr(r), _ptr(out)))
         return out
 
-    def pair(self, A: SparseMatrix, db: DeviceArray, dx: DeviceArray, dxs: DeviceArray):
-        rc = np.empty(self.n_coarse, np.float32)
-        s = C.c_double()
-        _check(_lib.rs_learned_pair(self.h, A.h, db.ptr, dx.ptr, dxs.ptr, _ptr(rc), C.byref(s)))
-        return rc, s.value
+    def apply(self, r, A: SparseMatrix | None = None) -> np.ndarray:
+        """One V-cycle z = B r."""
+        rr = np.ascontiguousarray(r, dtype=np.float32)
+        out = np.empty_like(rr)
+        _check(_lib.rs_learned_apply(self.h, (A or self.A).h, _ptr(rr), _ptr(out)))
+        return out
+
+    def record(self, capacity: int) -> None:
+        _check(_lib.rs_learned_record(self.h, capacity))
+
+    def records(self, count: int) -> np.ndarray:
+        out = np.empty((max(count, 0), self.n_coarse), np.float32)
+        _check(_lib.rs_learned_records(self.h, count, _ptr(out)))
+        return out
 
     def __del__(self):
         if _lib is not None and getattr(self, "h", None):
@@ -601,37 +675,54 @@ class LearnedPreconditioner:
 
 
 class Trainer:
-    """GPU training of the network: forward/backward/Adam kernels, batches of <= max_batch."""
+    """GPU training of the network: forward/backward/Adam kernels on batches of coarse graphs."""
 
-    def __init__(self, params, max_batch: int = 64):
+    def __init__(self, params, max_batch: int = 64, max_nodes: int = MAX_COARSE, max_nnz: int = 65536):
         load()
         p = np.ascontiguousarray(params, dtype=np.float32)
         self.max_batch = max_batch
         self.h = C.c_void_p()
-        _check(_lib.rs_trainer_create(_ptr(p), p.size, max_batch, C.byref(self.h)))
+        _check(_lib.rs_trainer_create(_ptr(p), p.size, max_batch, max_nodes, max_nnz, C.byref(self.h)))
 
     @staticmethod
-    def _pad(Ac, rc, s):
-        Ac = np.asarray(Ac, dtype=np.float64)
-        B, n = Ac.shape[0], Ac.shape[1]
-        A = np.zeros((B, TRAIN_MAX_NODES, TRAIN_MAX_NODES), np.float32)
-        A[:, :n, :n] = Ac
-        R = np.zeros((B, TRAIN_MAX_NODES), np.float32)
-        R[:, :n] = rc
-        return n, B, A, R, np.ascontiguousarray(s, dtype=np.float32)
+    def dense_pattern(n: int):
+        cptr = (np.arange(n + 1) * n).astype(np.uint32)
+        ccol = np.tile(np.arange(n, dtype=np.uint32), n)
+        return cptr, ccol
 
-    def gradient(self, Ac, rc, s):
-        n, B, A, R, S = self._pad(Ac, rc, s)
+    def gradient_csr(self, cptr, ccol, cvals, rc, s):
+        cptr, ccol = _u32(cptr), _u32(ccol)
+        cv = np.ascontiguousarray(cvals, dtype=np.float32)
+        R = np.ascontiguousarray(rc, dtype=np.float32)
+        S = np.ascontiguousarray(s, dtype=np.float32)
+        B, n = R.shape
         loss = np.empty(B, np.float32)
         grad = np.empty(N_PARAMS, np.float32)
-        _check(_lib.rs_trainer_gradient(self.h, n, B, _ptr(A), _ptr(R), _ptr(S), _ptr(loss), _ptr(grad)))
+        _check(_lib.rs_trainer_gradient(self.h, n, ccol.size, _ptr(cptr), _ptr(ccol), B, _ptr(cv), _ptr(R),
+                                        _ptr(S), _ptr(loss), _ptr(grad)))
         return loss, grad
 
-    def step(self, Ac, rc, s, lr: float) -> float:
-        n, B, A, R, S = self._pad(Ac, rc, s)
+    def step_csr(self, cptr, ccol, cvals, rc, s, lr: float) -> float:
+        cptr, ccol = _u32(cptr), _u32(ccol)
+        cv = np.ascontiguousarray(cvals, dtype=np.float32)
+        R = np.ascontiguousarray(rc, dtype=np.float32)
+        S = np.ascontiguousarray(s, dtype=np.float32)
+        B, n = R.shape
         m = C.c_float()
-        _check(_lib.rs_trainer_step(self.h, n, B, _ptr(A), _ptr(R), _ptr(S), lr, C.byref(m)))
+        _check(_lib.rs_trainer_step(self.h, n, ccol.size, _ptr(cptr), _ptr(ccol), B, _ptr(cv), _ptr(R), _ptr(S),
+                                    lr, C.byref(m)))
         return m.value
+
+    def gradient(self, Ac, rc, s):
+        """Dense coarse operators Ac[B][n][n] (every entry of the n x n pattern)."""
+        Ac = np.asarray(Ac, dtype=np.float32)
+        cptr, ccol = self.dense_pattern(Ac.shape[1])
+        return self.gradient_csr(cptr, ccol, Ac.reshape(Ac.shape[0], -1), rc, s)
+
+    def step(self, Ac, rc, s, lr: float) -> float:
+        Ac = np.asarray(Ac, dtype=np.float32)
+        cptr, ccol = self.dense_pattern(Ac.shape[1])
+        return self.step_csr(cptr, ccol, Ac.reshape(Ac.shape[0], -1), rc, s, lr)
 
     def params(self) -> np.ndarray:
         out = np.empty(N_PARAMS, np.float32)

@@ -1,24 +1,29 @@
-// rs_learned.inc.hip -- learned two-level preconditioner (no counterpart in the reference).
+// rs_learned.inc.hip -- learned multilevel preconditioner (no counterpart in the reference).
 // Included at the end of rs_solve.hip: it shares that translation unit's solver state, graph
 // replay, finalize kernel and SpMV kernels.
 //
-// Preconditioner:  z = D^-1 r + P e,   e = N_theta(P^T r; A_c)
-//   P      piecewise-constant prolongation over caller-given aggregates (coordinate bins)
-//   A_c    Galerkin coarse operator P^T A P, accumulated in 64-bit fixed point (integer
-//          atomics are associative, so the result does not depend on scheduling)
-//   N      6-layer, 64-wide message-passing network over the coarse graph: messages are the
-//          neighbour features weighted by -A_c[I,J]/A_c[I,I]; BF16 weights and activations on
-//          MFMA (v_mfma_f32_16x16x16_bf16), FP32 accumulation; output e = u + sigma * o, a
-//          learned correction of the coarse Jacobi estimate u = r_c / diag(A_c).
-// Inference runs in one workgroup with the whole coarse graph resident in LDS (<= 512 coarse
-// nodes: 2 x 64 KB of BF16 features).  The solve is flexible CG (Notay's FCG(1): beta from the
-// A-orthogonalisation against the previous direction) in mixed precision with FP64 residual
-// replacement; Jacobi-CG finishes the solve if the flexible iteration stalls.
-//
-// Training kernels: one workgroup per (residual, error) sample on graphs of <= 64 coarse
-// nodes: forward with the activations kept in LDS, the loss (e.A_c.e - 2 e.r_c)/s + 1 (the
-// A-norm error fraction the correction leaves, s = e_k.r_k), backward, per-sample gradients;
-// then a fixed-order batch mean and Adam.
+// Preconditioner: one V-cycle over an aggregation hierarchy, applied inside flexible CG.
+//   levels    caller-given aggregations (for example nested coordinate bins of 2x2x2 nodes);
+//             piecewise-constant prolongation P_l; Galerkin operators A_{l+1} = P_l^T A_l P_l
+//             summed in FP64 in a fixed order (one gather list per coarse entry), so the
+//             set-up is deterministic
+//   smoother  Chebyshev polynomial in D^-1 A_l over [lambda/30, 1.05 lambda], lambda the
+//             Gershgorin bound of D^-1 A_l (Adams, Brezina, Hu, Tuminaro 2003); FP32 values
+//             and vectors, each step one fused SpMV kernel
+//   coarsest  the learned part: a 6-layer, 64-wide message-passing network over the coarsest
+//             graph (<= 512 nodes) maps the coarsest residual r_c to a correction e.  Other
+//             modes: coarse Jacobi (what the untrained network computes) and the exact dense
+//             inverse (the ceiling a coarse solve can reach).
+// Network: features [u/sigma, log d - mean, 1, 0] with u = r_c / diag(A_c), sigma = rms(u);
+// messages are neighbour features weighted by -A_c[I,J]/A_c[I,I]; Y = H Ws + M Wn + b,
+// H += relu(Y); BF16 weights and activations on MFMA (v_mfma_f32_16x16x16_bf16), FP32
+// accumulation; output e = u + sigma * (H wo + bo).  Inference runs in one workgroup with the
+// coarsest graph resident in LDS.
+// Training (one workgroup per sample, activations in global memory): forward, the loss
+// (e.A_c.e - 2 e.r_c)/s + 1 with s = r_c.A_c^-1.r_c -- the A_c-norm error fraction the
+// correction leaves -- backward, per-sample gradients, a fixed-order batch mean and Adam.
+// The solve is flexible CG (Notay 2000, FCG(1)) in mixed precision with FP64 residual
+// replacement; Jacobi-CG finishes the solve if the flexible iteration stops unconverged.
 
 namespace rs {
 
@@ -26,7 +31,7 @@ constexpr int kHid = 64;
 constexpr int kNin = 4;
 constexpr int kLayers = 6;
 constexpr uint32_t kMaxCoarse = 512;
-constexpr uint32_t kTrainN = 64;
+constexpr uint32_t kMaxDeg = 4;          // Chebyshev steps per smoothing
 constexpr uint32_t kLayerStride = 2 * kHid * kHid + kHid;
 constexpr uint32_t kOffWin = 0, kOffBin = kNin * kHid, kOffL0 = kNin * kHid + kHid;
 constexpr uint32_t kOffWo = kOffL0 + kLayers * kLayerStride, kOffBo = kOffWo + kHid;
@@ -68,6 +73,7 @@ __device__ __forceinline__ f32x4 mfma_tile(FA a, FB b, int K, f32x4 acc)
     return acc;
 }
 
+// Sum over the workgroup, result in every thread.
 __device__ __forceinline__ float block_sum_f(float v, float *red, int nthreads)
 {
     v = group_sum<kWave>(v);
@@ -81,13 +87,10 @@ __device__ __forceinline__ float block_sum_f(float v, float *red, int nthreads)
     return s;
 }
 
-// ---------------------------------------------------------------------------- setup kernels
-__global__ __launch_bounds__(kBlock) void k_maxabs(const double *__restrict__ v, uint64_t n, double *partials)
+// Workgroup maximum of m, stored by thread 0 at partials[blockIdx.x].
+__device__ __forceinline__ void block_max_store(double m, double *partials)
 {
     __shared__ double red[kBlock / kWave];
-    double m = 0.0;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
-        m = fmax(m, fabs(v[i]));
     for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
     if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = m;
     __syncthreads();
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_maxabs(const double *__restrict__ v,
     }
 }
 
-__global__ void k_maxabs_fin(const double *partials, uint32_t nparts, double *out)
+__global__ void k_max_fin(const double *partials, uint32_t nparts, double *out)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double t = 0.0;
@@ -106,80 +109,195 @@ __global__ void k_maxabs_fin(const double *partials, uint32_t nparts, double *ou
     *out = t;
 }
 
-__global__ void k_zero_u64(unsigned long long *__restrict__ a, uint64_t n)
+// ---------------------------------------------------------------------------- hierarchy set-up
+// Coarse values: cval[k] = sum of the finer entries listed for coarse entry k, in list order.
+__global__ void k_galerkin_gather(const double *__restrict__ fval, const uint32_t *__restrict__ gptr,
+                                  const uint32_t *__restrict__ gsrc, uint64_t nnzc, double *__restrict__ cval)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) a[i] = 0ull;
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < nnzc; k += (uint64_t)gridDim.x * kBlock) {
+        double s = 0.0;
+        for (uint32_t j = gptr[k]; j < gptr[k + 1]; j++) s += fval[gsrc[j]];
+        cval[k] = s;
+    }
 }
 
-// Fixed-point scale: |entries| <= 2^44, so 2^19 of headroom for sums of up to half a million
-// terms; resolution 2^-44 * max|A|.
-__device__ __forceinline__ double galerkin_scale(double maxabs) { return ldexp(1.0, 44) / fmax(maxabs, 1e-300); }
-
-// A_c[I][J] += A_ij for I = agg(i), J = agg(j); entries of a row that fall in the same coarse
-// column are combined before the atomics.
-__global__ void k_galerkin(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
-                           const double *__restrict__ val, uint32_t n, const uint32_t *__restrict__ agg,
-                           uint32_t nc, const double *__restrict__ maxabs, unsigned long long *__restrict__ acc)
+// FP32 inverse diagonal (0 where the diagonal is absent or zero) and the block maxima of the
+// Gershgorin bound sum_j |a_ij| / |a_ii| of D^-1 A.
+__global__ __launch_bounds__(kBlock) void k_level_diag(const uint32_t *__restrict__ rowptr,
+                                                       const uint32_t *__restrict__ col,
+                                                       const double *__restrict__ val, uint32_t n,
+                                                       float *__restrict__ dinv, double *__restrict__ partials)
 {
-    const double scale = galerkin_scale(*maxabs);
+    double m = 0.0;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const uint32_t I = agg[i];
-        uint32_t cj[8];
-        long long cv[8];
-        int nk = 0;
+        double d = 0.0, s = 0.0;
         for (uint32_t j = rowptr[i]; j < rowptr[i + 1]; j++) {
-            uint32_t J = agg[col[j]];
-            long long qv = __double2ll_rn(val[j] * scale);
-            int k = 0;
-            while (k < nk && cj[k] != J) k++;
-            if (k < nk) {
-                cv[k] += qv;
-            } else if (nk < 8) {
-                cj[nk] = J;
-                cv[nk] = qv;
-                nk++;
-            } else {
-                atomicAdd(&acc[(uint64_t)I * nc + J], (unsigned long long)qv);
-            }
+            double a = val[j];
+            s += fabs(a);
+            if (col[j] == i) d += a;
         }
-        for (int k = 0; k < nk; k++) atomicAdd(&acc[(uint64_t)I * nc + cj[k]], (unsigned long long)cv[k]);
+        dinv[i] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
+        if (d != 0.0) m = fmax(m, s / fabs(d));
     }
+    block_max_store(m, partials);
 }
 
-// Coarse CSR values, diagonal, message weights -A_c[I,J]/A_c[I,I], and the dense copy.
-__global__ void k_galerkin_finish(const unsigned long long *__restrict__ acc, uint32_t nc,
-                                  const double *__restrict__ maxabs, const uint32_t *__restrict__ cptr,
-                                  const uint32_t *__restrict__ ccol, double *__restrict__ cval,
-                                  float *__restrict__ cw, float *__restrict__ cdiag, double *__restrict__ dense)
+// Coarsest graph for the network: diagonal and message weights -A[I,J]/A[I,I] (0 on the diagonal).
+__global__ void k_coarse_weights(const uint32_t *__restrict__ cptr, const uint32_t *__restrict__ ccol,
+                                 const double *__restrict__ cval, uint32_t nc, float *__restrict__ cw,
+                                 float *__restrict__ cdiag)
 {
-    const double inv = 1.0 / galerkin_scale(*maxabs);
-    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < (uint64_t)nc * nc; k += (uint64_t)gridDim.x * kBlock)
-        dense[k] = (double)(long long)acc[k] * inv;
     for (uint32_t I = blockIdx.x * kBlock + threadIdx.x; I < nc; I += gridDim.x * kBlock) {
-        double d = (double)(long long)acc[(uint64_t)I * nc + I] * inv;
+        double d = 0.0;
+        for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++)
+            if (ccol[p] == I) d += cval[p];
         cdiag[I] = (float)d;
-        for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++) {
-            uint32_t J = ccol[p];
-            double a = (double)(long long)acc[(uint64_t)I * nc + J] * inv;
-            cval[p] = a;
-            cw[p] = (J == I || d == 0.0) ? 0.0f : (float)(-a / d);
-        }
+        for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++)
+            cw[p] = (ccol[p] == I || d == 0.0) ? 0.0f : (float)(-cval[p] / d);
     }
 }
 
-// r_c[I] = sum of r over the rows of aggregate I (fixed order: member list, then block sum).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restrict(const T *__restrict__ r, const uint32_t *__restrict__ mptr,
-                                                     const uint32_t *__restrict__ mrows, float *__restrict__ rc,
-                                                     const SolveState *st)
+// Fine vector restricted straight to the coarsest level (block per coarsest node, FP64 sum).
+__global__ __launch_bounds__(kBlock) void k_restrict_total(const double *__restrict__ r, const uint32_t *__restrict__ mptr,
+                                                           const uint32_t *__restrict__ mrows, float *__restrict__ rc)
 {
-    if (st && st->done) return;
     __shared__ double lds[kBlock / kWave];
     const uint32_t I = blockIdx.x;
     double v[1] = { 0.0 };
-    for (uint32_t k = mptr[I] + threadIdx.x; k < mptr[I + 1]; k += kBlock) v[0] += (double)r[mrows[k]];
+    for (uint32_t k = mptr[I] + threadIdx.x; k < mptr[I + 1]; k += kBlock) v[0] += r[mrows[k]];
     block_sum<1, kBlock>(v, lds);
     if (threadIdx.x == 0) rc[I] = (float)v[0];
+}
+
+// ---------------------------------------------------------------------------- cycle kernels
+struct CycArgs {
+    CsrView A;                 // level matrix (FP32 values)
+    const float *r;            // right-hand side of the level
+    const float *zin;          // iterate the SpMV reads
+    float *zout, *d, *res;
+    const float *dinv;
+    const float *coef;         // (c1, c2) of this Chebyshev step
+    const SolveState *st;      // nullptr outside a solve
+    const float *q;            // finest level, last step: the flexible CG's previous A p
+    double *partials;          // finest level, last step: partial r.z and z.q
+    uint32_t n;
+};
+
+__device__ __forceinline__ bool cyc_idle(const SolveState *st) { return st && st->done; }
+
+// First Chebyshev step from a zero iterate: d = c2 D^-1 r, z = d.
+__global__ void k_cyc_first(CycArgs a)
+{
+    if (cyc_idle(a.st)) return;
+    const float c2 = a.coef[1];
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * kBlock) {
+        float z = c2 * a.dinv[i] * a.r[i];
+        a.zout[i] = z;
+        a.d[i] = z;
+    }
+}
+
+// Chebyshev step: res = r - A z_in, d = c1 d + c2 D^-1 res, z_out = z_in + d (c1 = 0: first
+// step from a non-zero iterate, d not read).
+struct EpiCheb {
+    const float *r, *zin, *dinv, *q;
+    float *zout, *d;
+    float c1, c2;
+    bool dots;
+    double acc[2];
+    __device__ void row(uint32_t i, float s)
+    {
+        const float ri = r[i];
+        float dd = c2 * dinv[i] * (ri - s);
+        if (c1 != 0.0f) dd = fmaf(c1, d[i], dd);
+        d[i] = dd;
+        const float z = zin[i] + dd;
+        zout[i] = z;
+        if (dots) {
+            acc[0] += (double)ri * (double)z;
+            acc[1] += (double)z * (double)q[i];
+        }
+    }
+};
+
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_cyc_step(CycArgs a)
+{
+    if (cyc_idle(a.st)) return;
+    EpiCheb e{ a.r, a.zin, a.dinv, a.q, a.zout, a.d, a.coef[0], a.coef[1], a.partials != nullptr, { 0.0, 0.0 } };
+    Shape::template run<float>(a.A, a.A.val32, a.zin, e);
+    if (a.partials) store_partials<2>(e.acc, a.partials);
+}
+
+struct EpiResid32 {
+    const float *r;
+    float *res;
+    __device__ void row(uint32_t i, float s) { res[i] = r[i] - s; }
+};
+
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_cyc_resid(CycArgs a)
+{
+    if (cyc_idle(a.st)) return;
+    EpiResid32 e{ a.r, a.res };
+    Shape::template run<float>(a.A, a.A.val32, a.zin, e);
+}
+
+// r_next[I] = sum of res over the members of I (G lanes per aggregate, FP64 sum, fixed order).
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_cyc_restrict(const float *__restrict__ res, const uint32_t *__restrict__ mptr,
+                                                         const uint32_t *__restrict__ mrows, uint32_t nnext,
+                                                         float *__restrict__ rnext, const SolveState *st)
+{
+    if (cyc_idle(st)) return;
+    const uint32_t gl = threadIdx.x % G;
+    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / G);
+    for (uint64_t I = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G; I < nnext; I += stride) {
+        double s = 0.0;
+        for (uint32_t k = mptr[I] + gl; k < mptr[I + 1]; k += G) s += (double)res[mrows[k]];
+        s = group_sum<G>(s);
+        if (gl == 0) rnext[I] = (float)s;
+    }
+}
+
+// z_out = z_in + e[agg] (z_in nullptr: zero).
+__global__ void k_cyc_prolong(const float *__restrict__ zin, const float *__restrict__ e, const uint32_t *__restrict__ agg,
+                              float *__restrict__ zout, uint32_t n, const SolveState *st)
+{
+    if (cyc_idle(st)) return;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        zout[i] = (zin ? zin[i] : 0.0f) + e[agg[i]];
+}
+
+__global__ void k_coarse_jacobi(const float *__restrict__ r, const float *__restrict__ dinv, float *__restrict__ z,
+                                uint32_t n, const SolveState *st)
+{
+    if (cyc_idle(st)) return;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) z[i] = dinv[i] * r[i];
+}
+
+// z = C r with the dense FP32 inverse C (row-major n x n), one wavefront per row.
+__global__ __launch_bounds__(kBlock) void k_coarse_exact(const float *__restrict__ cinv, const float *__restrict__ r,
+                                                         float *__restrict__ z, uint32_t n, const SolveState *st)
+{
+    if (cyc_idle(st)) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    for (uint32_t row = (blockIdx.x * kBlock + threadIdx.x) / kWave; row < n; row += gridDim.x * (kBlock / kWave)) {
+        float s = 0.0f;
+        for (uint32_t j = lane; j < n; j += kWave) s += cinv[(uint64_t)row * n + j] * r[j];
+        s = group_sum<kWave>(s);
+        if (lane == 0) z[row] = s;
+    }
+}
+
+// Keep the coarsest right-hand side of iteration it (training data).
+__global__ void k_record(const float *__restrict__ r, float *__restrict__ rec, uint32_t n, uint32_t cap,
+                         const SolveState *st)
+{
+    if (!st || st->done) return;
+    const uint32_t it = st->it;
+    if (it >= cap) return;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) rec[(uint64_t)it * n + i] = r[i];
 }
 
 // ---------------------------------------------------------------------------- inference
@@ -194,7 +312,7 @@ struct GnnArgs {
     const SolveState *st;
 };
 
-constexpr int kInferThreads = 1024;
+constexpr int kInferThreads = 512;
 
 __global__ __launch_bounds__(kInferThreads) void k_gnn_infer(GnnArgs g)
 {
@@ -253,6 +371,7 @@ __global__ __launch_bounds__(kInferThreads) void k_gnn_infer(GnnArgs g)
                 acc[ct] = mfma_tile([&](int r, int k) { return M[(r0 + r) * kHid + k]; },
                                     [&](int k, int c) { return W[kHid * kHid + k * kHid + ct * 16 + c]; }, kHid, acc[ct]);
             }
+            // every MFMA of the strip has read its H rows (this wavefront's only) before the update
 #pragma unroll
             for (int ct = 0; ct < 4; ct++)
 #pragma unroll
@@ -272,54 +391,72 @@ __global__ __launch_bounds__(kInferThreads) void k_gnn_infer(GnnArgs g)
     }
 }
 
+}  // namespace rs
+
+namespace rs {
+
 // ---------------------------------------------------------------------------- training
+// One workgroup per sample; every sample of a launch shares the coarse pattern (structurally
+// symmetric; tpos[p] is the position of entry (J, I) for entry p = (I, J)) and has its own
+// operator values, coarse residual and normaliser s.
 struct TrainArgs {
-    uint32_t n;               // coarse nodes of every sample (<= 64); rows n..63 are padding
-    const float *Ac;          // [B][64][64] dense coarse operators (padding zero)
-    const float *rc;          // [B][64]
-    const float *s;           // [B] e_k . r_k
+    uint32_t n, npad, nnz;
+    const uint32_t *cptr, *ccol, *tpos;
+    const float *cval;                  // [B][nnz]
+    const float *rc;                    // [B][n]
+    const float *s;                     // [B]
     const float *params;
     const unsigned short *wbf;
-    float *grads;             // [B][kNumParams]
-    float *loss;              // [B]
+    float *grads;                       // [B][kNumParams]
+    float *loss;                        // [B]
+    unsigned short *hs;                 // [B][kLayers + 1][npad][kHid] activations
+    unsigned short *mb, *dy;            // [B][npad][kHid]
+    float *dh, *dm;                     // [B][npad][kHid]
 };
 
-__global__ __launch_bounds__(kBlock) void k_gnn_train(TrainArgs t)
+constexpr int kTrainThreads = 512;
+
+__global__ __launch_bounds__(kTrainThreads) void k_gnn_train(TrainArgs t)
 {
-    constexpr uint32_t N = kTrainN;
-    __shared__ unsigned short Hs[(kLayers + 1) * N * kHid];
-    __shared__ float Ac[N * N];
-    __shared__ float dH[N * kHid];
-    __shared__ float dM[N * kHid];
-    __shared__ unsigned short Mb[N * kHid];
-    __shared__ unsigned short dY[N * kHid];
+    constexpr int kWaves = kTrainThreads / kWave;
     __shared__ unsigned short W[kBfPerLayer];
-    __shared__ float u[N], ld[N], ev[N], gov[N];
-    __shared__ float red[kBlock / kWave];
+    __shared__ float u[kMaxCoarse], ld[kMaxCoarse], ev[kMaxCoarse], gov[kMaxCoarse], dg[kMaxCoarse];
+    __shared__ float red[kWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
-    const uint32_t n = t.n, b = blockIdx.x;
+    const uint32_t n = t.n, npad = t.npad, b = blockIdx.x;
+    const uint64_t act = (uint64_t)npad * kHid;
     const float *P = t.params;
-    const float *Ag = t.Ac + (uint64_t)b * N * N;
-    const float *rcg = t.rc + (uint64_t)b * N;
+    const uint32_t *cptr = t.cptr, *ccol = t.ccol;
+    const float *cv = t.cval + (uint64_t)b * t.nnz;
+    const float *rcg = t.rc + (uint64_t)b * n;
     float *G = t.grads + (uint64_t)b * kNumParams;
-    for (uint32_t k = tid; k < N * N; k += kBlock) Ac[k] = Ag[k];
+    unsigned short *Hs = t.hs + (uint64_t)b * (kLayers + 1) * act;
+    unsigned short *Mb = t.mb + (uint64_t)b * act;
+    unsigned short *dY = t.dy + (uint64_t)b * act;
+    float *dH = t.dh + (uint64_t)b * act;
+    float *dM = t.dm + (uint64_t)b * act;
+    for (uint32_t I = tid; I < npad; I += kTrainThreads) {
+        float d = 0.0f;
+        if (I < n)
+            for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++)
+                if (ccol[p] == I) d += cv[p];
+        dg[I] = d;
+    }
     __syncthreads();
-    // inputs
     float su = 0.0f, sl = 0.0f;
-    for (uint32_t I = tid; I < N; I += kBlock) {
+    for (uint32_t I = tid; I < npad; I += kTrainThreads) {
         float uu = 0.0f, l = 0.0f;
         if (I < n) {
-            float d = Ac[I * N + I];
-            uu = rcg[I] / d;
-            l = logf(d);
+            uu = rcg[I] / dg[I];
+            l = logf(dg[I]);
         }
         u[I] = uu;
         ld[I] = l;
         su += uu * uu;
         sl += l;
     }
-    const float sigma = sqrtf(block_sum_f(su, red, kBlock) / (float)n) + 1e-30f;
-    const float meanlog = block_sum_f(sl, red, kBlock) / (float)n;
+    const float sigma = sqrtf(block_sum_f(su, red, kTrainThreads) / (float)n) + 1e-30f;
+    const float meanlog = block_sum_f(sl, red, kTrainThreads) / (float)n;
     auto feat = [&](uint32_t I, int k) -> float {
         if (I >= n) return 0.0f;
         return k == 0 ? u[I] / sigma : k == 1 ? ld[I] - meanlog : k == 2 ? 1.0f : 0.0f;
@@ -329,15 +466,25 @@ __global__ __launch_bounds__(kBlock) void k_gnn_train(TrainArgs t)
         return feat(I, 0) * P[kOffWin + f] + feat(I, 1) * P[kOffWin + kHid + f] + P[kOffWin + 2 * kHid + f] +
                P[kOffBin + f];
     };
-    auto wn = [&](uint32_t I, uint32_t J) -> float {   // -A[I][J]/A[I][I], 0 on the diagonal
-        return (I == J || I >= n || J >= n) ? 0.0f : -Ac[I * N + J] / Ac[I * N + I];
-    };
-    for (uint32_t idx = tid; idx < N * kHid; idx += kBlock)
-        Hs[idx] = f2bf(fmaxf(z0(idx / kHid, idx % kHid), 0.0f));
+    for (uint32_t idx = tid; idx < act; idx += kTrainThreads) Hs[idx] = f2bf(fmaxf(z0(idx / kHid, idx % kHid), 0.0f));
     __syncthreads();
-    // forward; Y is recomputed in the backward pass from the stored H
-    auto layer_y = [&](int l, const unsigned short *Hl, f32x4 (&acc)[4]) {
-        const uint32_t r0 = wave * 16;
+    // M = wn H, wn[I][J] = -A[I][J]/A[I][I] off the diagonal
+    auto aggregate = [&](const unsigned short *Hl) {
+        for (uint32_t idx = tid; idx < act; idx += kTrainThreads) {
+            uint32_t I = idx / kHid, f = idx % kHid;
+            float m = 0.0f;
+            if (I < n) {
+                const float inv = 1.0f / dg[I];
+                for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++) {
+                    uint32_t J = ccol[p];
+                    if (J != I) m += (-cv[p] * inv) * bf2f(Hl[J * kHid + f]);
+                }
+            }
+            Mb[idx] = f2bf(m);
+        }
+    };
+    // pre-activation Y - b of a 16-row strip, four 16-column tiles
+    auto layer_y = [&](const unsigned short *Hl, uint32_t r0, f32x4 (&acc)[4]) {
 #pragma unroll
         for (int ct = 0; ct < 4; ct++) {
             acc[ct] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
@@ -347,109 +494,104 @@ __global__ __launch_bounds__(kBlock) void k_gnn_train(TrainArgs t)
                                 [&](int k, int c) { return W[kHid * kHid + k * kHid + ct * 16 + c]; }, kHid, acc[ct]);
         }
     };
-    auto aggregate = [&](const unsigned short *Hl) {
-        for (uint32_t idx = tid; idx < N * kHid; idx += kBlock) {
-            uint32_t I = idx / kHid, f = idx % kHid;
-            float m = 0.0f;
-            for (uint32_t J = 0; J < n; J++) m += wn(I, J) * bf2f(Hl[J * kHid + f]);
-            Mb[idx] = f2bf(m);
-        }
-    };
     for (int l = 0; l < kLayers; l++) {
-        const unsigned short *Hl = Hs + l * N * kHid;
-        unsigned short *Hn = Hs + (l + 1) * N * kHid;
-        for (uint32_t k = tid; k < kBfPerLayer; k += kBlock) W[k] = t.wbf[l * kBfPerLayer + k];
+        const unsigned short *Hl = Hs + l * act;
+        unsigned short *Hn = Hs + (l + 1) * act;
+        for (uint32_t k = tid; k < kBfPerLayer; k += kTrainThreads) W[k] = t.wbf[l * kBfPerLayer + k];
         aggregate(Hl);
         __syncthreads();
-        f32x4 acc[4];
-        layer_y(l, Hl, acc);
+        for (uint32_t strip = wave; strip < npad / 16; strip += kWaves) {
+            f32x4 acc[4];
+            layer_y(Hl, strip * 16, acc);
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++)
+            for (int ct = 0; ct < 4; ct++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                uint32_t row = wave * 16 + (lane >> 4) * 4 + i, f = ct * 16 + (lane & 15);
-                float y = acc[ct][i] + P[off_b(l) + f];
-                float h = bf2f(Hl[row * kHid + f]) + fmaxf(y, 0.0f);
-                Hn[row * kHid + f] = row < n ? f2bf(h) : (unsigned short)0;
-            }
+                for (int i = 0; i < 4; i++) {
+                    uint32_t row = strip * 16 + (lane >> 4) * 4 + i, f = ct * 16 + (lane & 15);
+                    float y = acc[ct][i] + P[off_b(l) + f];
+                    float h = bf2f(Hl[row * kHid + f]) + fmaxf(y, 0.0f);
+                    Hn[row * kHid + f] = row < n ? f2bf(h) : (unsigned short)0;
+                }
+        }
         __syncthreads();
     }
-    const unsigned short *H6 = Hs + kLayers * N * kHid;
-    for (uint32_t I = tid; I < N; I += kBlock) {
+    const unsigned short *H6 = Hs + kLayers * act;
+    for (uint32_t I = tid; I < npad; I += kTrainThreads) {
         float o = P[kOffBo];
         for (int f = 0; f < kHid; f++) o += bf2f(H6[I * kHid + f]) * P[kOffWo + f];
         ev[I] = I < n ? u[I] + sigma * o : 0.0f;
     }
     __syncthreads();
-    // loss and output-head gradient
+    // loss and the gradient at the output
     float part = 0.0f;
     const float s = t.s[b];
-    for (uint32_t I = tid; I < N; I += kBlock) {
-        float ae = 0.0f;
-        for (uint32_t J = 0; J < n; J++) ae += Ac[I * N + J] * ev[J];
-        float ge = I < n ? 2.0f * (ae - rcg[I]) / s : 0.0f;
-        gov[I] = ge * sigma;
-        part += I < n ? ev[I] * ae - 2.0f * ev[I] * rcg[I] : 0.0f;
+    for (uint32_t I = tid; I < npad; I += kTrainThreads) {
+        float g = 0.0f;
+        if (I < n) {
+            float ae = 0.0f;
+            for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++) ae += cv[p] * ev[ccol[p]];
+            g = 2.0f * (ae - rcg[I]) / s * sigma;
+            part += ev[I] * ae - 2.0f * ev[I] * rcg[I];
+        }
+        gov[I] = g;
     }
-    float L = block_sum_f(part, red, kBlock) / s + 1.0f;
+    const float L = block_sum_f(part, red, kTrainThreads) / s + 1.0f;
     if (tid == 0) t.loss[b] = L;
     __syncthreads();
-    for (uint32_t f = tid; f < (uint32_t)kHid; f += kBlock) {
+    if (tid < kHid) {
         float gw = 0.0f;
-        for (uint32_t I = 0; I < n; I++) gw += gov[I] * bf2f(H6[I * kHid + f]);
-        G[kOffWo + f] = gw;
-    }
-    if (tid == 0) {
+        for (uint32_t I = 0; I < n; I++) gw += gov[I] * bf2f(H6[I * kHid + tid]);
+        G[kOffWo + tid] = gw;
+    } else if (tid == kHid) {
         float gb = 0.0f;
         for (uint32_t I = 0; I < n; I++) gb += gov[I];
         G[kOffBo] = gb;
     }
-    for (uint32_t idx = tid; idx < N * kHid; idx += kBlock) dH[idx] = gov[idx / kHid] * P[kOffWo + idx % kHid];
+    for (uint32_t idx = tid; idx < act; idx += kTrainThreads) dH[idx] = gov[idx / kHid] * P[kOffWo + idx % kHid];
     __syncthreads();
-    // backward through the layers
     for (int l = kLayers - 1; l >= 0; l--) {
-        const unsigned short *Hl = Hs + l * N * kHid;
-        for (uint32_t k = tid; k < kBfPerLayer; k += kBlock) W[k] = t.wbf[l * kBfPerLayer + k];
+        const unsigned short *Hl = Hs + l * act;
+        for (uint32_t k = tid; k < kBfPerLayer; k += kTrainThreads) W[k] = t.wbf[l * kBfPerLayer + k];
         aggregate(Hl);
         __syncthreads();
-        f32x4 acc[4];
-        layer_y(l, Hl, acc);
+        // dY = dH where Y > 0 (rows < n), rounded to BF16
+        for (uint32_t strip = wave; strip < npad / 16; strip += kWaves) {
+            f32x4 acc[4];
+            layer_y(Hl, strip * 16, acc);
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                uint32_t row = wave * 16 + (lane >> 4) * 4 + i, f = ct * 16 + (lane & 15);
-                float y = acc[ct][i] + P[off_b(l) + f];
-                float g = (y > 0.0f && row < n) ? dH[row * kHid + f] : 0.0f;
-                dY[row * kHid + f] = f2bf(g);
-            }
-        __syncthreads();
-        // dWs = H^T dY, dWn = M^T dY (rows of the output: input features, strip per wave)
-        {
-            const uint32_t r0 = wave * 16;
-#pragma unroll
-            for (int ct = 0; ct < 4; ct++) {
-                f32x4 gs = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f }, gn = gs;
-                gs = mfma_tile([&](int r, int k) { return Hl[k * kHid + r0 + r]; },
-                               [&](int k, int c) { return dY[k * kHid + ct * 16 + c]; }, N, gs);
-                gn = mfma_tile([&](int r, int k) { return Mb[k * kHid + r0 + r]; },
-                               [&](int k, int c) { return dY[k * kHid + ct * 16 + c]; }, N, gn);
+            for (int ct = 0; ct < 4; ct++)
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    uint32_t fi = r0 + (lane >> 4) * 4 + i, fo = ct * 16 + (lane & 15);
-                    G[off_ws(l) + fi * kHid + fo] = gs[i];
-                    G[off_wn(l) + fi * kHid + fo] = gn[i];
+                    uint32_t row = strip * 16 + (lane >> 4) * 4 + i, f = ct * 16 + (lane & 15);
+                    float y = acc[ct][i] + P[off_b(l) + f];
+                    float g = (y > 0.0f && row < n) ? dH[row * kHid + f] : 0.0f;
+                    dY[row * kHid + f] = f2bf(g);
                 }
+        }
+        __syncthreads();
+        // dWs = H^T dY and dWn = M^T dY: 16 tiles (input-feature strip x output tile)
+        for (int tile = wave; tile < 16; tile += kWaves) {
+            const uint32_t fi0 = (tile >> 2) * 16, fo0 = (tile & 3) * 16;
+            f32x4 gs = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f }, gn = gs;
+            gs = mfma_tile([&](int r, int k) { return Hl[k * kHid + fi0 + r]; },
+                           [&](int k, int c) { return dY[k * kHid + fo0 + c]; }, (int)npad, gs);
+            gn = mfma_tile([&](int r, int k) { return Mb[k * kHid + fi0 + r]; },
+                           [&](int k, int c) { return dY[k * kHid + fo0 + c]; }, (int)npad, gn);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint32_t fi = fi0 + (lane >> 4) * 4 + i, fo = fo0 + (lane & 15);
+                G[off_ws(l) + fi * kHid + fo] = gs[i];
+                G[off_wn(l) + fi * kHid + fo] = gn[i];
             }
         }
-        for (uint32_t f = tid; f < (uint32_t)kHid; f += kBlock) {
+        if (tid < kHid) {
             float gb = 0.0f;
-            for (uint32_t I = 0; I < N; I++) gb += bf2f(dY[I * kHid + f]);
-            G[off_b(l) + f] = gb;
+            for (uint32_t I = 0; I < n; I++) gb += bf2f(dY[I * kHid + tid]);
+            G[off_b(l) + tid] = gb;
         }
-        // dM = dY Wn^T ; dH += dY Ws^T   (node strip per wave)
-        {
-            const uint32_t r0 = wave * 16;
+        // dM = dY Wn^T ; dH += dY Ws^T
+        for (uint32_t strip = wave; strip < npad / 16; strip += kWaves) {
+            const uint32_t r0 = strip * 16;
 #pragma unroll
             for (int ct = 0; ct < 4; ct++) {
                 f32x4 am = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f }, as = am;
@@ -466,25 +608,29 @@ __global__ __launch_bounds__(kBlock) void k_gnn_train(TrainArgs t)
             }
         }
         __syncthreads();
-        // dH[I] += sum_J wn(J, I) dM[J]
-        for (uint32_t idx = tid; idx < N * kHid; idx += kBlock) {
+        // dH[I] += sum_J wn[J][I] dM[J]
+        for (uint32_t idx = tid; idx < act; idx += kTrainThreads) {
             uint32_t I = idx / kHid, f = idx % kHid;
+            if (I >= n) continue;
             float a = 0.0f;
-            for (uint32_t J = 0; J < n; J++) a += wn(J, I) * dM[J * kHid + f];
+            for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++) {
+                uint32_t J = ccol[p];
+                if (J != I) a += (-cv[t.tpos[p]] / dg[J]) * dM[J * kHid + f];
+            }
             dH[idx] += a;
         }
         __syncthreads();
     }
-    // embedding
-    for (uint32_t f = tid; f < (uint32_t)kHid; f += kBlock) {
-        float gw[kNin] = { 0.0f, 0.0f, 0.0f, 0.0f }, gb = 0.0f;
+    // embedding: rows 0..3 of Win, then bin
+    for (uint32_t task = tid; task < (kNin + 1) * kHid; task += kTrainThreads) {
+        const uint32_t k = task / kHid, f = task % kHid;
+        float g = 0.0f;
         for (uint32_t I = 0; I < n; I++) {
             float dz = z0(I, f) > 0.0f ? dH[I * kHid + f] : 0.0f;
-            for (int k = 0; k < kNin; k++) gw[k] += feat(I, k) * dz;
-            gb += dz;
+            g += (k < (uint32_t)kNin ? feat(I, k) : 1.0f) * dz;
         }
-        for (int k = 0; k < kNin; k++) G[kOffWin + k * kHid + f] = gw[k];
-        G[kOffBin + f] = gb;
+        if (k < (uint32_t)kNin) G[kOffWin + k * kHid + f] = g;
+        else G[kOffBin + f] = g;
     }
 }
 
@@ -498,10 +644,9 @@ __global__ void k_grad_mean(const float *__restrict__ grads, uint32_t B, float *
     }
 }
 
-// Adam (Kingma & Ba 2015) on the FP32 master weights; refreshes the BF16 layer weights.
+// Adam (Kingma & Ba 2015) on the FP32 master weights.
 __global__ void k_adam(float *__restrict__ theta, float *__restrict__ m, float *__restrict__ v,
-                       const float *__restrict__ g, float lr, float b1, float b2, float eps, float c1, float c2,
-                       unsigned short *__restrict__ wbf)
+                       const float *__restrict__ g, float lr, float b1, float b2, float eps, float c1, float c2)
 {
     for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < kNumParams; p += gridDim.x * kBlock) {
         float gp = g[p];
@@ -522,51 +667,19 @@ __global__ void k_weights_bf16(const float *__restrict__ theta, unsigned short *
     }
 }
 
-// r = b - y (y = A x), e = x* - x ; partial e.r  (training pairs)
-__global__ __launch_bounds__(kBlock) void k_pair(const double *__restrict__ b, const double *__restrict__ y,
-                                                 const double *__restrict__ x, const double *__restrict__ xs,
-                                                 double *__restrict__ r, uint32_t n, double *partials)
-{
-    double v[1] = { 0.0 };
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
-        double ri = b[i] - y[i];
-        r[i] = ri;
-        v[0] += (xs[i] - x[i]) * ri;
-    }
-    store_partials<1>(v, partials);
-}
-
 // ---------------------------------------------------------------------------- flexible CG
 struct FcgArgs {
     uint32_t n;
     float *r, *z, *p, *q, *dx;
-    const float *d32;            // Jacobi inverse diagonal (FP32)
-    const uint32_t *agg;
-    const float *ec;             // network output per aggregate
     double *partials;
     SolveState *st;
 };
 
-// z = D^-1 r + P e ; partial r.z and z.q_prev
-__global__ __launch_bounds__(kBlock) void k_fcg_z(FcgArgs a)
-{
-    if (a.st->done) return;
-    double v[2] = { 0.0, 0.0 };
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * kBlock) {
-        float ri = a.r[i];
-        float zi = a.d32[i] * ri + a.ec[a.agg[i]];
-        a.z[i] = zi;
-        v[0] += (double)ri * (double)zi;
-        v[1] += (double)zi * (double)a.q[i];
-    }
-    store_partials<2>(v, a.partials);
-}
-
-// p = z (first iteration) or z + beta p
+// p = z (first iteration of a segment) or z + beta p
 __global__ void k_fcg_p(FcgArgs a)
 {
     if (a.st->done) return;
-    const bool first = a.st->it == 0;
+    const bool first = a.st->it == a.st->it0;
     const float beta = (float)a.st->beta;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * kBlock)
         a.p[i] = first ? a.z[i] : fmaf(beta, a.p[i], a.z[i]);
@@ -589,64 +702,290 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(FcgArgs a)
 
 }  // namespace rs
 
-// ============================================================================ C ABI: learned
+// ============================================================================ hierarchy (host)
+namespace rs {
+
+struct CycLevel {
+    Matrix *M = nullptr;                 // level 0: the solver's matrix; coarser levels: own
+    rs_matrix *own = nullptr;
+    uint32_t n = 0;
+    DBuf<float> dinv, r, za, zb, d, res;
+    // to the next level
+    uint32_t nnext = 0, rgroup = 1;
+    DBuf<uint32_t> agg, mptr, mrows, gptr, gsrc;
+    ~CycLevel()
+    {
+        delete own;
+    }
+};
+
+}  // namespace rs
+
 struct rs_learned {
-    uint32_t n = 0, nc = 0;
-    std::vector<uint32_t> agg_h;
-    DBuf<uint32_t> agg, mptr, mrows, cptr, ccol;
-    DBuf<double> cval, dense, maxabs, mpart;
-    DBuf<float> cw, cdiag, rc, ec;
-    DBuf<unsigned long long> acc;
-    DBuf<float> theta;
-    DBuf<unsigned short> wbf;
-    DBuf<double> tmp_y, tmp_r, tmp_part;
+    uint32_t n = 0, nc = 0, nlev = 0;                // fine rows, coarsest nodes, levels incl. fine
+    const rs_matrix *owner = nullptr;                // the matrix whose pattern the hierarchy follows
+    std::vector<std::unique_ptr<rs::CycLevel>> lv;   // lv[0] fine ... lv[nlev - 1] coarsest
+    rs::DBuf<uint32_t> tot_mptr, tot_mrows;          // fine rows of each coarsest node
+    rs::DBuf<float> cw, cdiag, coef, cinv, rec, tmp_rc, tmp_r;
+    rs::DBuf<double> part, gersh;
+    rs::DBuf<float> theta;
+    rs::DBuf<unsigned short> wbf;
     bool have_params = false;
+    uint32_t pre = 1, post = 1;
+    int coarse = RS_COARSE_NETWORK;
+    uint32_t rec_cap = 0;
     uint64_t setup_version = ~0ull;
-    const rs_matrix *setup_matrix = nullptr;
-    double setup_ms = 0.0;
+    bool inverse_valid = false;
+    std::vector<double> lam;                         // Gershgorin bound per level
+    const float *r0 = nullptr;                       // finest right-hand side of the cycle
 };
 
 namespace rs {
 
+// Chebyshev coefficients (c1, c2) of steps 0..kMaxDeg-1 over [lambda/30, 1.05 lambda].
+static void cheb_coefs(double lambda, float *out)
+{
+    const double hi = 1.05 * lambda, lo = lambda / 30.0;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sig = theta / delta;
+    double rho = 1.0 / sig;
+    out[0] = 0.0f;
+    out[1] = (float)(1.0 / theta);
+    for (uint32_t k = 1; k < kMaxDeg; k++) {
+        double rn = 1.0 / (2.0 * sig - rho);
+        out[2 * k] = (float)(rn * rho);
+        out[2 * k + 1] = (float)(2.0 * rn / delta);
+        rho = rn;
+    }
+}
+
+// In-place inverse of a dense row-major matrix (Gauss-Jordan, partial pivoting).
+static bool dense_inverse(std::vector<double> &a, uint32_t n)
+{
+    std::vector<double> inv((size_t)n * n, 0.0);
+    for (uint32_t i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+    for (uint32_t c = 0; c < n; c++) {
+        uint32_t piv = c;
+        for (uint32_t r = c + 1; r < n; r++)
+            if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)piv * n + c])) piv = r;
+        if (a[(size_t)piv * n + c] == 0.0) return false;
+        if (piv != c)
+            for (uint32_t k = 0; k < n; k++) {
+                std::swap(a[(size_t)piv * n + k], a[(size_t)c * n + k]);
+                std::swap(inv[(size_t)piv * n + k], inv[(size_t)c * n + k]);
+            }
+        const double s = 1.0 / a[(size_t)c * n + c];
+        for (uint32_t k = 0; k < n; k++) {
+            a[(size_t)c * n + k] *= s;
+            inv[(size_t)c * n + k] *= s;
+        }
+        for (uint32_t r = 0; r < n; r++) {
+            if (r == c) continue;
+            const double f = a[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            double *ar = &a[(size_t)r * n], *ir = &inv[(size_t)r * n];
+            const double *ac = &a[(size_t)c * n], *ic = &inv[(size_t)c * n];
+            for (uint32_t k = 0; k < n; k++) {
+                ar[k] -= f * ac[k];
+                ir[k] -= f * ic[k];
+            }
+        }
+    }
+    a.swap(inv);
+    return true;
+}
+
+static int coarsest_dense(const rs_learned &L, std::vector<double> &dense)
+{
+    const Matrix &C = *L.lv.back()->M;
+    std::vector<uint32_t> cp(C.n + 1ull), cc(C.nnz);
+    std::vector<double> cv(C.nnz);
+    hipStream_t st = runtime().stream;
+    RS_HIP(hipMemcpyAsync(cp.data(), C.rowptr.p, sizeof(uint32_t) * cp.size(), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipMemcpyAsync(cc.data(), C.col.p, sizeof(uint32_t) * C.nnz, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipMemcpyAsync(cv.data(), C.val.p, sizeof(double) * C.nnz, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    dense.assign((size_t)C.n * C.n, 0.0);
+    for (uint32_t I = 0; I < C.n; I++)
+        for (uint32_t p = cp[I]; p < cp[I + 1]; p++) dense[(size_t)I * C.n + cc[p]] += cv[p];
+    return RS_OK;
+}
+
+// Values of every level for the current values of A: Galerkin gathers, inverse diagonals,
+// Chebyshev coefficients, the coarsest graph weights (and its inverse in exact mode).
 static int learned_setup(rs_learned &L, rs_matrix &A)
 {
-    if (L.setup_matrix == &A && L.setup_version == A.version) return RS_OK;
-    if (A.n != L.n) return fail(RS_ERR_INVALID_ARG, "learned preconditioner: matrix size differs from its aggregation");
+    if (&A != L.owner) return fail(RS_ERR_INVALID_ARG, "learned preconditioner: built for another matrix");
+    if (L.setup_version == A.version && (L.coarse != RS_COARSE_EXACT || L.inverse_valid)) return RS_OK;
     hipStream_t st = runtime().stream;
     int rc;
-    uint32_t nb = vec_blocks(std::max<uint64_t>(A.nnz, 1));
-    if ((rc = L.mpart.alloc(nb))) return rc;
-    k_maxabs<<<nb, kBlock, 0, st>>>(A.val.p, A.nnz, L.mpart.p);
-    k_maxabs_fin<<<1, 64, 0, st>>>(L.mpart.p, nb, L.maxabs.p);
-    uint64_t nn = (uint64_t)L.nc * L.nc;
-    k_zero_u64<<<vec_blocks(nn), kBlock, 0, st>>>(L.acc.p, nn);
-    k_galerkin<<<vec_blocks(A.n), kBlock, 0, st>>>(A.rowptr.p, A.col.p, A.val.p, A.n, L.agg.p, L.nc, L.maxabs.p, L.acc.p);
-    k_galerkin_finish<<<vec_blocks(nn), kBlock, 0, st>>>(L.acc.p, L.nc, L.maxabs.p, L.cptr.p, L.ccol.p, L.cval.p, L.cw.p,
-                                                          L.cdiag.p, L.dense.p);
+    L.lv[0]->M = &A;
+    for (uint32_t l = 0; l + 1 < L.nlev; l++) {
+        CycLevel &F = *L.lv[l], &C = *L.lv[l + 1];
+        k_galerkin_gather<<<vec_blocks(std::max<uint64_t>(C.M->nnz, 1)), kBlock, 0, st>>>(F.M->val.p, F.gptr.p, F.gsrc.p,
+                                                                                         C.M->nnz, C.M->val.p);
+        matrix_values_changed(*C.M);
+    }
+    const uint32_t kParts = 4096;
+    for (uint32_t l = 0; l < L.nlev; l++) {
+        CycLevel &V = *L.lv[l];
+        uint32_t nb = std::min(vec_blocks(std::max(V.n, 1u)), kParts);
+        k_level_diag<<<nb, kBlock, 0, st>>>(V.M->rowptr.p, V.M->col.p, V.M->val.p, V.n, V.dinv.p, L.part.p + (uint64_t)l * kParts);
+        k_max_fin<<<1, 64, 0, st>>>(L.part.p + (uint64_t)l * kParts, nb, L.gersh.p + l);
+    }
+    const Matrix &C = *L.lv.back()->M;
+    k_coarse_weights<<<vec_blocks(L.nc), kBlock, 0, st>>>(C.rowptr.p, C.col.p, C.val.p, L.nc, L.cw.p, L.cdiag.p);
     RS_HIP(hipGetLastError());
-    L.setup_matrix = &A;
+    L.lam.assign(L.nlev, 0.0);
+    RS_HIP(hipMemcpyAsync(L.lam.data(), L.gersh.p, sizeof(double) * L.nlev, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    std::vector<float> coef((size_t)L.nlev * kMaxDeg * 2);
+    for (uint32_t l = 0; l < L.nlev; l++) cheb_coefs(L.lam[l] > 0.0 ? L.lam[l] : 1.0, &coef[(size_t)l * kMaxDeg * 2]);
+    RS_HIP(hipMemcpyAsync(L.coef.p, coef.data(), sizeof(float) * coef.size(), hipMemcpyHostToDevice, st));
+    L.inverse_valid = false;
+    if (L.coarse == RS_COARSE_EXACT) {
+        std::vector<double> dense;
+        if ((rc = coarsest_dense(L, dense))) return rc;
+        if (!dense_inverse(dense, L.nc)) return fail(RS_ERR_UNSUPPORTED, "learned preconditioner: singular coarsest operator");
+        std::vector<float> f(dense.begin(), dense.end());
+        if ((rc = L.cinv.alloc(f.size()))) return rc;
+        RS_HIP(hipMemcpyAsync(L.cinv.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice, st));
+        L.inverse_valid = true;
+    }
+    RS_HIP(hipStreamSynchronize(st));
     L.setup_version = A.version;
+    return RS_OK;
+}
+
+// FP32 SpMV plans of every level (autotuned once per level).
+static int learned_plans(rs_learned &L)
+{
+    int rc;
+    for (uint32_t l = 0; l < L.nlev; l++) {
+        Matrix &M = *L.lv[l]->M;
+        if ((rc = matrix_refresh_fp32(M)) || (rc = matrix_tune(M, true)) || (rc = matrix_sync_plan(M, M.plan[1], true)))
+            return rc;
+    }
     return RS_OK;
 }
 
 static GnnArgs gnn_args(rs_learned &L, const SolveState *st)
 {
+    CycLevel &V = *L.lv.back();
     GnnArgs g{};
     g.nc = L.nc;
     g.npad = (L.nc + 15) / 16 * 16;
-    g.rc = L.rc.p;
+    g.rc = V.r.p;
     g.diag = L.cdiag.p;
-    g.cptr = L.cptr.p;
-    g.ccol = L.ccol.p;
+    g.cptr = V.M->rowptr.p;
+    g.ccol = V.M->col.p;
     g.cw = L.cw.p;
     g.params = L.theta.p;
     g.wbf = L.wbf.p;
-    g.e = L.ec.p;
+    g.e = V.za.p;
     g.st = st;
     return g;
 }
 
-// Flexible CG with the learned preconditioner (mixed precision, FP64 residual replacement).
+static void launch_restrict(uint32_t G, uint32_t nb, const float *res, const CycLevel &V, float *rnext, const SolveState *st)
+{
+    hipStream_t s = runtime().stream;
+    switch (G) {
+        case 1: k_cyc_restrict<1><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st); break;
+        case 4: k_cyc_restrict<4><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st); break;
+        case 16: k_cyc_restrict<16><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st); break;
+        default: k_cyc_restrict<64><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st); break;
+    }
+}
+
+// Launches the V-cycle from level l; returns the buffer that holds the level's correction.
+// On the finest level with q and partials given, the last step also leaves the partial sums
+// r.z and z.q for the flexible CG.
+static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st, const float *q, double *partials)
+{
+    CycLevel &V = *L.lv[l];
+    hipStream_t s = runtime().stream;
+    const uint32_t nvb = vec_blocks(std::max(V.n, 1u));
+    const float *r = l == 0 ? L.r0 : V.r.p;
+    if (l + 1 == L.nlev) {
+        if (L.coarse == RS_COARSE_NETWORK) k_gnn_infer<<<1, kInferThreads, 0, s>>>(gnn_args(L, st));
+        else if (L.coarse == RS_COARSE_EXACT) k_coarse_exact<<<vec_blocks((uint64_t)V.n * kWave), kBlock, 0, s>>>(L.cinv.p, r, V.za.p, V.n, st);
+        else k_coarse_jacobi<<<nvb, kBlock, 0, s>>>(r, V.dinv.p, V.za.p, V.n, st);
+        return V.za.p;
+    }
+    CycLevel &C = *L.lv[l + 1];
+    const int plan = V.M->plan[1];
+    uint32_t nb = 0;
+    CycArgs a{};
+    a.A = plan_view(*V.M, plan, nb);
+    a.r = r;
+    a.d = V.d.p;
+    a.res = V.res.p;
+    a.dinv = V.dinv.p;
+    a.st = st;
+    a.n = V.n;
+    const float *coef = L.coef.p + (size_t)l * kMaxDeg * 2;
+    float *buf[2] = { V.za.p, V.zb.p };
+    int cur = -1;
+    for (uint32_t k = 0; k < L.pre; k++) {
+        a.coef = coef + 2 * k;
+        if (k == 0) {
+            a.zout = buf[0];
+            k_cyc_first<<<nvb, kBlock, 0, s>>>(a);
+            cur = 0;
+        } else {
+            a.zin = buf[cur];
+            a.zout = buf[cur ^ 1];
+            RS_SHAPE_SWITCH(plan, k_cyc_step, nb, s, a);
+            cur ^= 1;
+        }
+    }
+    const float *rres = r;
+    if (cur >= 0) {
+        a.zin = buf[cur];
+        RS_SHAPE_SWITCH(plan, k_cyc_resid, nb, s, a);
+        rres = V.res.p;
+    }
+    const uint32_t G = V.rgroup;
+    launch_restrict(G, vec_blocks((uint64_t)V.nnext * G), rres, V, C.r.p, st);
+    const float *e = cycle_level(L, l + 1, st, nullptr, nullptr);
+    const int nxt = cur >= 0 ? cur ^ 1 : 0;
+    k_cyc_prolong<<<nvb, kBlock, 0, s>>>(cur >= 0 ? buf[cur] : nullptr, e, V.agg.p, buf[nxt], V.n, st);
+    cur = nxt;
+    for (uint32_t k = 0; k < L.post; k++) {
+        const bool last = k + 1 == L.post;
+        a.coef = coef + 2 * k;
+        a.zin = buf[cur];
+        a.zout = buf[cur ^ 1];
+        a.q = last ? q : nullptr;
+        a.partials = last ? partials : nullptr;
+        RS_SHAPE_SWITCH(plan, k_cyc_step, nb, s, a);
+        cur ^= 1;
+    }
+    return buf[cur];
+}
+
+static std::string learned_key(const rs_learned &L)
+{
+    std::vector<uint64_t> k;
+    for (uint32_t l = 0; l < L.nlev; l++) {
+        const Matrix &M = *L.lv[l]->M;
+        k.push_back((uint64_t)(uintptr_t)&M);
+        k.push_back((uint64_t)M.plan[1]);
+        k.push_back((uint64_t)(uintptr_t)M.val32.p);
+        k.push_back((uint64_t)(uintptr_t)M.sell_val32.p);
+    }
+    k.push_back(L.pre);
+    k.push_back(L.post);
+    k.push_back((uint64_t)L.coarse);
+    k.push_back(L.rec_cap);
+    k.push_back((uint64_t)(uintptr_t)L.rec.p);
+    k.push_back((uint64_t)(uintptr_t)L.cinv.p);
+    k.push_back((uint64_t)(uintptr_t)L.r0);
+    return std::string(reinterpret_cast<const char *>(k.data()), k.size() * sizeof(uint64_t));
+}
+
+// Flexible CG with the learned multilevel preconditioner (mixed precision, FP64 residual
+// replacement).
 static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learned &L, double *d_x, uint32_t &done,
                              uint32_t &refinements)
 {
@@ -656,7 +995,7 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
         return rc;
     const int plan32 = A.plan[1], plan64 = A.plan[0];
     if ((rc = matrix_sync_plan(A, plan32, true)) || (rc = matrix_sync_plan(A, plan64, false))) return rc;
-    if ((rc = learned_setup(L, A))) return rc;
+    if ((rc = learned_setup(L, A)) || (rc = learned_plans(L))) return rc;
     if ((rc = S.z32.alloc(std::max(A.n, 1u)))) return rc;
     uint32_t nb32 = 0, nb64 = 0;
     MpArgs a{};
@@ -678,18 +1017,14 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
     FcgArgs f{};
     f.n = A.n;
     f.r = S.r32.p;
-    f.z = S.z32.p;
     f.p = S.p32.p;
     f.q = S.q32.p;
     f.dx = S.dx32.p;
-    f.d32 = P.data32.p;
-    f.agg = L.agg.p;
-    f.ec = L.ec.p;
     f.partials = S.partials.p;
     f.st = S.st.p;
-    GnnArgs g = gnn_args(L, S.st.p);
     const uint32_t nvb = vec_blocks(A.n);
     hipStream_t stream = runtime().stream;
+    L.r0 = S.r32.p;
     k_fill32<<<nvb, kBlock, 0, stream>>>(S.dx32.p, A.n, 0.0f);
     k_fill32<<<nvb, kBlock, 0, stream>>>(S.q32.p, A.n, 0.0f);
     RS_SHAPE_SWITCH(plan64, k_mp_true_resid, nb64, stream, a64);
@@ -697,17 +1032,21 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
     RS_HIP(hipGetLastError());
     rc = capture(S, [&] {
         for (uint32_t i = 0; i < kGraphIters; i++) {
-            k_restrict<float><<<L.nc, kBlock, 0, stream>>>(S.r32.p, L.mptr.p, L.mrows.p, L.rc.p, S.st.p);
-            k_gnn_infer<<<1, kInferThreads, 0, stream>>>(g);
-            k_fcg_z<<<nvb, kBlock, 0, stream>>>(f);
-            fin(S.partials.p, nvb, 2, FIN_FCG_BETA, 0, S.st.p, S.hist.p);
-            k_fcg_p<<<nvb, kBlock, 0, stream>>>(f);
+            const float *z = cycle_level(L, 0, S.st.p, S.q32.p, S.partials.p);
+            if (L.rec_cap) {
+                CycLevel &Cl = *L.lv.back();
+                k_record<<<vec_blocks(L.nc), kBlock, 0, stream>>>(Cl.r.p, L.rec.p, L.nc, L.rec_cap, S.st.p);
+            }
+            fin(S.partials.p, nb32, 2, FIN_FCG_BETA, 0, S.st.p, S.hist.p);
+            FcgArgs fz = f;
+            fz.z = const_cast<float *>(z);
+            k_fcg_p<<<nvb, kBlock, 0, stream>>>(fz);
             RS_SHAPE_SWITCH(plan32, k_mp_spmv_dot, nb32, stream, a);
             fin(S.partials.p, nb32, 1, FIN_DOT, 0, S.st.p, S.hist.p);
             k_fcg_update<<<nvb, kBlock, 0, stream>>>(f);
             fin(S.partials.p, nvb, 1, FIN_FCG_UPDATE, 0, S.st.p, S.hist.p);
         }
-    }, graph_key_of(3, a, plan32, nb32, L.nc, nvb) + graph_key_of(3, f, plan64, nb64, 0, 0));
+    }, graph_key_of(3, a, plan32, nb32, nb64, nvb) + learned_key(L));
     if (rc) return rc;
     refinements = 0;
     SolveState h;
@@ -786,57 +1125,202 @@ extern "C" int rs_aggregates_by_bins(uint32_t n_nodes, const double *xyz, uint32
     return RS_OK;
 }
 
-extern "C" int rs_learned_create(const rs_matrix *A, const uint32_t *agg_of_row, uint32_t n_agg, rs_learned **out)
+extern "C" int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node, const uint32_t *book,
+                                    uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t max_coarse, uint32_t max_levels,
+                                    uint32_t *agg, uint64_t agg_capacity, uint32_t *n_agg, uint32_t *n_levels)
 {
-    if (!A || !agg_of_row || !out || n_agg == 0) return fail(RS_ERR_INVALID_ARG, "rs_learned_create: bad arguments");
-    if (n_agg > kMaxCoarse) return fail(RS_ERR_UNSUPPORTED, "rs_learned_create: more than 512 aggregates");
-    for (uint32_t i = 0; i < A->n; i++)
-        if (agg_of_row[i] >= n_agg) return fail(RS_ERR_INVALID_ARG, "rs_learned_create: aggregate id out of range");
+    if (!xyz || !book || !agg || !n_agg || !n_levels || !nbx || !nby || !nbz || !dofs_per_node || !max_coarse ||
+        !max_levels)
+        return fail(RS_ERR_INVALID_ARG, "rs_hierarchy_by_bins: bad arguments");
+    // enabled positions and their nodes
+    std::vector<uint32_t> node_of;
+    for (uint64_t i = 0; i < (uint64_t)n_nodes * dofs_per_node; i++)
+        if (book[i] != RS_EOD) {
+            if (book[i] >= node_of.size()) node_of.resize(book[i] + 1ull, RS_EOD);
+            node_of[book[i]] = (uint32_t)(i / dofs_per_node);
+        }
+    const uint32_t n_dof = (uint32_t)node_of.size();
+    for (uint32_t d = 0; d < n_dof; d++)
+        if (node_of[d] == RS_EOD) return fail(RS_ERR_INVALID_ARG, "rs_hierarchy_by_bins: book is not a numbering 0..n-1");
+    double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+    for (uint64_t i = 0; i < n_nodes; i++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], xyz[3 * i + a]);
+            hi[a] = std::max(hi[a], xyz[3 * i + a]);
+        }
+    // rep[J]: a fine position of node J of the current level (its coordinates place J in a bin)
+    std::vector<uint32_t> rep(n_dof);
+    for (uint32_t d = 0; d < n_dof; d++) rep[d] = d;
+    uint32_t cur = n_dof, levels = 0;
+    uint64_t used = 0;
+    for (uint32_t k = 0; cur > max_coarse || levels == 0; k++) {
+        if (levels == max_levels) return fail(RS_ERR_UNSUPPORTED, "rs_hierarchy_by_bins: coarsest level still too large");
+        const uint32_t sh = std::min(k, 31u);
+        const uint32_t nb[3] = { std::max(1u, nbx >> sh), std::max(1u, nby >> sh), std::max(1u, nbz >> sh) };
+        if (used + cur > agg_capacity) return fail(RS_ERR_INVALID_ARG, "rs_hierarchy_by_bins: agg capacity too small");
+        std::vector<uint32_t> id((uint64_t)nb[0] * nb[1] * nb[2], RS_EOD), next_rep;
+        uint32_t count = 0;
+        for (uint32_t J = 0; J < cur; J++) {
+            const double *p = xyz + 3ull * node_of[rep[J]];
+            uint64_t c[3];
+            for (int a = 0; a < 3; a++) {
+                double t = hi[a] > lo[a] ? (p[a] - lo[a]) / (hi[a] - lo[a]) : 0.0;
+                c[a] = std::min<uint64_t>(nb[a] - 1, (uint64_t)(t * nb[a]));
+            }
+            uint64_t b = c[0] + nb[0] * (c[1] + (uint64_t)nb[1] * c[2]);
+            if (id[b] == RS_EOD) {
+                id[b] = count++;
+                next_rep.push_back(rep[J]);
+            }
+            agg[used + J] = id[b];
+        }
+        if (count == cur) {
+            if (nb[0] == 1 && nb[1] == 1 && nb[2] == 1)
+                return fail(RS_ERR_UNSUPPORTED, "rs_hierarchy_by_bins: cannot coarsen further");
+            continue;   // the bins are still as fine as the nodes: halve again
+        }
+        n_agg[levels++] = count;
+        used += cur;
+        cur = count;
+        rep.swap(next_rep);
+    }
+    *n_levels = levels;
+    return RS_OK;
+}
+
+extern "C" int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const uint32_t *agg, const uint32_t *n_agg,
+                                 rs_learned **out)
+{
+    if (!A || !agg || !n_agg || !out || n_levels == 0 || n_levels > 16)
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_create: bad arguments");
+    const uint32_t nc = n_agg[n_levels - 1];
+    if (nc == 0 || nc > kMaxCoarse) return fail(RS_ERR_UNSUPPORTED, "rs_learned_create: coarsest level must have 1..512 nodes");
+    {
+        uint64_t off = 0;
+        uint32_t nprev = A->n;
+        for (uint32_t l = 0; l < n_levels; l++) {
+            if (n_agg[l] == 0 || n_agg[l] > nprev) return fail(RS_ERR_INVALID_ARG, "rs_learned_create: bad level size");
+            std::vector<uint8_t> seen(n_agg[l], 0);
+            for (uint32_t i = 0; i < nprev; i++) {
+                if (agg[off + i] >= n_agg[l]) return fail(RS_ERR_INVALID_ARG, "rs_learned_create: aggregate id out of range");
+                seen[agg[off + i]] = 1;
+            }
+            for (uint32_t I = 0; I < n_agg[l]; I++)
+                if (!seen[I]) return fail(RS_ERR_INVALID_ARG, "rs_learned_create: empty aggregate");
+            off += nprev;
+            nprev = n_agg[l];
+        }
+    }
     int rc = require_runtime();
     if (rc) return rc;
     std::unique_ptr<rs_learned> L(new rs_learned());
     L->n = A->n;
-    L->nc = n_agg;
-    L->agg_h.assign(agg_of_row, agg_of_row + A->n);
-    // member lists (rows of each aggregate in increasing order)
-    std::vector<uint32_t> mptr(n_agg + 1, 0), mrows(A->n);
-    for (uint32_t i = 0; i < A->n; i++) mptr[agg_of_row[i] + 1]++;
-    for (uint32_t I = 0; I < n_agg; I++) mptr[I + 1] += mptr[I];
-    {
-        std::vector<uint32_t> fill(mptr.begin(), mptr.end() - 1);
-        for (uint32_t i = 0; i < A->n; i++) mrows[fill[agg_of_row[i]]++] = i;
-    }
-    for (uint32_t I = 0; I < n_agg; I++)
-        if (mptr[I + 1] == mptr[I]) return fail(RS_ERR_INVALID_ARG, "rs_learned_create: empty aggregate");
-    // coarse pattern from the fine pattern
-    std::vector<uint32_t> rowptr(A->n + 1ull), col(A->nnz);
+    L->nc = nc;
+    L->nlev = n_levels + 1;
+    L->owner = A;
     hipStream_t st = runtime().stream;
-    RS_HIP(hipMemcpyAsync(rowptr.data(), A->rowptr.p, sizeof(uint32_t) * rowptr.size(), hipMemcpyDeviceToHost, st));
-    if (A->nnz) RS_HIP(hipMemcpyAsync(col.data(), A->col.p, sizeof(uint32_t) * A->nnz, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> rp(A->n + 1ull), ci(A->nnz);
+    RS_HIP(hipMemcpyAsync(rp.data(), A->rowptr.p, sizeof(uint32_t) * rp.size(), hipMemcpyDeviceToHost, st));
+    if (A->nnz) RS_HIP(hipMemcpyAsync(ci.data(), A->col.p, sizeof(uint32_t) * A->nnz, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
-    std::vector<std::vector<uint8_t>> adj(n_agg, std::vector<uint8_t>(n_agg, 0));
-    for (uint32_t i = 0; i < A->n; i++)
-        for (uint32_t j = rowptr[i]; j < rowptr[i + 1]; j++) adj[agg_of_row[i]][agg_of_row[col[j]]] = 1;
-    std::vector<uint32_t> cptr(n_agg + 1, 0), ccol;
-    for (uint32_t I = 0; I < n_agg; I++) {
-        adj[I][I] = 1;
-        for (uint32_t J = 0; J < n_agg; J++)
-            if (adj[I][J]) ccol.push_back(J);
-        cptr[I + 1] = (uint32_t)ccol.size();
+    L->lv.emplace_back(new CycLevel());
+    L->lv[0]->M = const_cast<rs_matrix *>(A);
+    L->lv[0]->n = A->n;
+    std::vector<uint32_t> total(A->n);
+    for (uint32_t i = 0; i < A->n; i++) total[i] = i;
+    uint64_t off = 0;
+    for (uint32_t l = 0; l < n_levels; l++) {
+        CycLevel &V = *L->lv[l];
+        const uint32_t nn = n_agg[l];
+        const uint32_t *m = agg + off;
+        off += V.n;
+        V.nnext = nn;
+        std::vector<uint32_t> mptr(nn + 1ull, 0), mrows(V.n);
+        for (uint32_t i = 0; i < V.n; i++) mptr[m[i] + 1]++;
+        for (uint32_t I = 0; I < nn; I++) mptr[I + 1] += mptr[I];
+        {
+            std::vector<uint32_t> fill(mptr.begin(), mptr.end() - 1);
+            for (uint32_t i = 0; i < V.n; i++) mrows[fill[m[i]]++] = i;
+        }
+        // next-level pattern and gather lists: the entries (agg(i), agg(j)) of the member rows,
+        // sorted by (coarse column, entry position)
+        std::vector<std::vector<uint32_t>> ccols(nn), cstart(nn), csrc(nn);
+        parallel_for(nn, [&](uint64_t I0, uint64_t I1) {
+            std::vector<std::pair<uint32_t, uint32_t>> e;
+            for (uint64_t I = I0; I < I1; I++) {
+                e.clear();
+                for (uint32_t k = mptr[I]; k < mptr[I + 1]; k++) {
+                    const uint32_t i = mrows[k];
+                    for (uint32_t p = rp[i]; p < rp[i + 1]; p++) e.emplace_back(m[ci[p]], p);
+                }
+                std::sort(e.begin(), e.end());
+                for (size_t t = 0; t < e.size(); t++) {
+                    if (t == 0 || e[t].first != e[t - 1].first) {
+                        ccols[I].push_back(e[t].first);
+                        cstart[I].push_back((uint32_t)t);
+                    }
+                    csrc[I].push_back(e[t].second);
+                }
+            }
+        });
+        std::vector<uint32_t> cptr(nn + 1ull, 0), ccol, gptr(1, 0), gsrc;
+        gsrc.reserve(rp[V.n]);
+        for (uint32_t I = 0; I < nn; I++) {
+            for (size_t c = 0; c < ccols[I].size(); c++) {
+                ccol.push_back(ccols[I][c]);
+                size_t t1 = c + 1 < ccols[I].size() ? cstart[I][c + 1] : csrc[I].size();
+                for (size_t t = cstart[I][c]; t < t1; t++) gsrc.push_back(csrc[I][t]);
+                gptr.push_back((uint32_t)gsrc.size());
+            }
+            cptr[I + 1] = (uint32_t)ccol.size();
+        }
+        std::vector<double> zeros(std::max<size_t>(ccol.size(), 1), 0.0);
+        std::unique_ptr<CycLevel> C(new CycLevel());
+        if ((rc = rs_matrix_create(nn, cptr.data(), ccol.data(), zeros.data(), &C->own))) return rc;
+        C->M = C->own;
+        C->n = nn;
+        if ((rc = V.agg.alloc(V.n)) || (rc = V.mptr.alloc(mptr.size())) || (rc = V.mrows.alloc(V.n)) ||
+            (rc = V.gptr.alloc(gptr.size())) || (rc = V.gsrc.alloc(std::max<size_t>(gsrc.size(), 1))))
+            return rc;
+        RS_HIP(hipMemcpyAsync(V.agg.p, m, sizeof(uint32_t) * V.n, hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(V.mptr.p, mptr.data(), sizeof(uint32_t) * mptr.size(), hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(V.mrows.p, mrows.data(), sizeof(uint32_t) * V.n, hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(V.gptr.p, gptr.data(), sizeof(uint32_t) * gptr.size(), hipMemcpyHostToDevice, st));
+        if (!gsrc.empty())
+            RS_HIP(hipMemcpyAsync(V.gsrc.p, gsrc.data(), sizeof(uint32_t) * gsrc.size(), hipMemcpyHostToDevice, st));
+        RS_HIP(hipStreamSynchronize(st));
+        const double avg = (double)V.n / nn;
+        V.rgroup = avg <= 6 ? 1 : avg <= 24 ? 4 : avg <= 96 ? 16 : 64;
+        for (uint32_t i = 0; i < A->n; i++) total[i] = m[total[i]];
+        rp.swap(cptr);
+        ci.swap(ccol);
+        L->lv.push_back(std::move(C));
+    }
+    for (auto &lp : L->lv) {
+        CycLevel &V = *lp;
+        const size_t n = std::max(V.n, 1u);
+        if ((rc = V.dinv.alloc(n)) || (rc = V.r.alloc(n)) || (rc = V.za.alloc(n)) || (rc = V.zb.alloc(n)) ||
+            (rc = V.d.alloc(n)) || (rc = V.res.alloc(n)))
+            return rc;
+    }
+    // fine rows of each coarsest node
+    std::vector<uint32_t> tptr(nc + 1ull, 0), trows(A->n);
+    for (uint32_t i = 0; i < A->n; i++) tptr[total[i] + 1]++;
+    for (uint32_t I = 0; I < nc; I++) tptr[I + 1] += tptr[I];
+    {
+        std::vector<uint32_t> fill(tptr.begin(), tptr.end() - 1);
+        for (uint32_t i = 0; i < A->n; i++) trows[fill[total[i]]++] = i;
     }
     rs_learned &R = *L;
-    uint64_t nn = (uint64_t)n_agg * n_agg;
-    if ((rc = R.agg.alloc(std::max(A->n, 1u))) || (rc = R.mptr.alloc(mptr.size())) || (rc = R.mrows.alloc(std::max(A->n, 1u))) ||
-        (rc = R.cptr.alloc(cptr.size())) || (rc = R.ccol.alloc(ccol.size())) || (rc = R.cval.alloc(ccol.size())) ||
-        (rc = R.cw.alloc(ccol.size())) || (rc = R.cdiag.alloc(n_agg)) || (rc = R.dense.alloc(nn)) ||
-        (rc = R.maxabs.alloc(1)) || (rc = R.acc.alloc(nn)) || (rc = R.rc.alloc(n_agg)) || (rc = R.ec.alloc(n_agg)) ||
-        (rc = R.theta.alloc(kNumParams)) || (rc = R.wbf.alloc(kLayers * kBfPerLayer)))
+    const Matrix &Cc = *R.lv.back()->M;
+    if ((rc = R.tot_mptr.alloc(tptr.size())) || (rc = R.tot_mrows.alloc(std::max(A->n, 1u))) ||
+        (rc = R.cw.alloc(std::max<uint64_t>(Cc.nnz, 1))) || (rc = R.cdiag.alloc(nc)) ||
+        (rc = R.coef.alloc((size_t)R.nlev * kMaxDeg * 2)) || (rc = R.tmp_rc.alloc(nc)) ||
+        (rc = R.part.alloc((size_t)R.nlev * 4096)) || (rc = R.gersh.alloc(R.nlev)) || (rc = R.theta.alloc(kNumParams)) ||
+        (rc = R.wbf.alloc(kLayers * kBfPerLayer)))
         return rc;
-    RS_HIP(hipMemcpyAsync(R.agg.p, agg_of_row, sizeof(uint32_t) * A->n, hipMemcpyHostToDevice, st));
-    RS_HIP(hipMemcpyAsync(R.mptr.p, mptr.data(), sizeof(uint32_t) * mptr.size(), hipMemcpyHostToDevice, st));
-    RS_HIP(hipMemcpyAsync(R.mrows.p, mrows.data(), sizeof(uint32_t) * mrows.size(), hipMemcpyHostToDevice, st));
-    RS_HIP(hipMemcpyAsync(R.cptr.p, cptr.data(), sizeof(uint32_t) * cptr.size(), hipMemcpyHostToDevice, st));
-    RS_HIP(hipMemcpyAsync(R.ccol.p, ccol.data(), sizeof(uint32_t) * ccol.size(), hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(R.tot_mptr.p, tptr.data(), sizeof(uint32_t) * tptr.size(), hipMemcpyHostToDevice, st));
+    if (A->n) RS_HIP(hipMemcpyAsync(R.tot_mrows.p, trows.data(), sizeof(uint32_t) * A->n, hipMemcpyHostToDevice, st));
     RS_HIP(hipStreamSynchronize(st));
     *out = L.release();
     return RS_OK;
@@ -857,12 +1341,29 @@ extern "C" int rs_learned_set_params(rs_learned *L, const float *params, uint64_
     return RS_OK;
 }
 
-extern "C" int rs_learned_info(const rs_learned *L, uint32_t *n_coarse, uint64_t *n_params, uint64_t *coarse_nnz)
+extern "C" int rs_learned_set_cycle(rs_learned *L, uint32_t pre, uint32_t post, int coarse_mode)
+{
+    if (!L) return fail(RS_ERR_INVALID_ARG, "rs_learned_set_cycle: null");
+    if (pre > kMaxDeg || post == 0 || post > kMaxDeg)
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_set_cycle: pre in 0..4 and post in 1..4 smoothing steps");
+    if (coarse_mode < RS_COARSE_NETWORK || coarse_mode > RS_COARSE_EXACT)
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_set_cycle: unknown coarse mode");
+    L->pre = pre;
+    L->post = post;
+    L->coarse = coarse_mode;
+    return RS_OK;
+}
+
+extern "C" int rs_learned_info(const rs_learned *L, uint32_t *n_levels, uint32_t *level_rows, uint64_t *level_nnz,
+                               uint64_t *n_params)
 {
     if (!L) return fail(RS_ERR_INVALID_ARG, "rs_learned_info: null");
-    if (n_coarse) *n_coarse = L->nc;
+    if (n_levels) *n_levels = L->nlev;
+    for (uint32_t l = 0; l < L->nlev; l++) {
+        if (level_rows) level_rows[l] = L->lv[l]->n;
+        if (level_nnz) level_nnz[l] = L->lv[l]->M->nnz;
+    }
     if (n_params) *n_params = kNumParams;
-    if (coarse_nnz) *coarse_nnz = L->ccol.n;
     return RS_OK;
 }
 
@@ -876,72 +1377,105 @@ extern "C" int rs_learned_setup(rs_learned *L, rs_matrix *A)
     return RS_OK;
 }
 
+extern "C" int rs_learned_level_lambda(const rs_learned *L, double *lambda)
+{
+    if (!L || !lambda) return fail(RS_ERR_INVALID_ARG, "rs_learned_level_lambda: null");
+    if (L->lam.size() != L->nlev) return fail(RS_ERR_INVALID_ARG, "rs_learned_level_lambda: call rs_learned_setup first");
+    for (uint32_t l = 0; l < L->nlev; l++) lambda[l] = L->lam[l];
+    return RS_OK;
+}
+
+// Download the matrix of level l (CSR, FP64 values of the last set-up).
+extern "C" int rs_learned_level_matrix(const rs_learned *L, uint32_t level, uint32_t *row_ptr, uint32_t *col, double *val)
+{
+    if (!L || level >= L->nlev) return fail(RS_ERR_INVALID_ARG, "rs_learned_level_matrix: bad level");
+    return rs_matrix_download(static_cast<const rs_matrix *>(L->lv[level]->M), row_ptr, col, val);
+}
+
 extern "C" int rs_learned_coarse_matrix(const rs_learned *L, double *dense)
 {
     if (!L || !dense) return fail(RS_ERR_INVALID_ARG, "rs_learned_coarse_matrix: null");
     int rc = require_runtime();
     if (rc) return rc;
-    RS_HIP(hipMemcpyAsync(dense, L->dense.p, sizeof(double) * L->nc * L->nc, hipMemcpyDeviceToHost, runtime().stream));
-    RS_HIP(hipStreamSynchronize(runtime().stream));
+    std::vector<double> d;
+    if ((rc = coarsest_dense(*L, d))) return rc;
+    std::copy(d.begin(), d.end(), dense);
     return RS_OK;
 }
 
+// A fine vector (device, FP64) restricted straight to the coarsest level.
 extern "C" int rs_learned_restrict_device(rs_learned *L, const double *d_r, float *rc_host)
 {
     if (!L || !d_r || !rc_host) return fail(RS_ERR_INVALID_ARG, "rs_learned_restrict_device: null");
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    k_restrict<double><<<L->nc, kBlock, 0, st>>>(d_r, L->mptr.p, L->mrows.p, L->rc.p, nullptr);
+    k_restrict_total<<<L->nc, kBlock, 0, st>>>(d_r, L->tot_mptr.p, L->tot_mrows.p, L->tmp_rc.p);
     RS_HIP(hipGetLastError());
-    RS_HIP(hipMemcpyAsync(rc_host, L->rc.p, sizeof(float) * L->nc, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipMemcpyAsync(rc_host, L->tmp_rc.p, sizeof(float) * L->nc, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
     return RS_OK;
 }
 
-// Network output for a host coarse residual (uses the coarse operator of the last setup).
+// Network output for a host coarsest residual (the coarsest operator of the last set-up).
 extern "C" int rs_learned_apply_coarse(rs_learned *L, const float *rc_host, float *e_host)
 {
     if (!L || !rc_host || !e_host) return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: null");
     if (!L->have_params) return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: parameters not set");
-    if (!L->setup_matrix) return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: call rs_learned_setup first");
+    if (L->lam.empty()) return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: call rs_learned_setup first");
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    RS_HIP(hipMemcpyAsync(L->rc.p, rc_host, sizeof(float) * L->nc, hipMemcpyHostToDevice, st));
-    GnnArgs g = gnn_args(*L, nullptr);
-    k_gnn_infer<<<1, kInferThreads, 0, st>>>(g);
+    CycLevel &V = *L->lv.back();
+    RS_HIP(hipMemcpyAsync(V.r.p, rc_host, sizeof(float) * L->nc, hipMemcpyHostToDevice, st));
+    k_gnn_infer<<<1, kInferThreads, 0, st>>>(gnn_args(*L, nullptr));
     RS_HIP(hipGetLastError());
-    RS_HIP(hipMemcpyAsync(e_host, L->ec.p, sizeof(float) * L->nc, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipMemcpyAsync(e_host, V.za.p, sizeof(float) * L->nc, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
     return RS_OK;
 }
 
-// A (residual, error) pair of a CG iterate x for the system A x* = b: r = b - A x (FP64),
-// r_c = P^T r and s = (x* - x) . r = ||x* - x||_A^2.  All vectors are device arrays.
-extern "C" int rs_learned_pair(rs_learned *L, rs_matrix *A, const double *d_b, const double *d_x,
-                               const double *d_xstar, float *rc_host, double *s_out)
+// One V-cycle z = B r on host vectors of the fine size (set-up and plans as for a solve).
+extern "C" int rs_learned_apply(rs_learned *L, rs_matrix *A, const float *r_host, float *z_host)
 {
-    if (!L || !A || !d_b || !d_x || !d_xstar || !rc_host || !s_out) return fail(RS_ERR_INVALID_ARG, "rs_learned_pair: null");
-    if (A->n != L->n) return fail(RS_ERR_INVALID_ARG, "rs_learned_pair: size mismatch");
+    if (!L || !A || !r_host || !z_host) return fail(RS_ERR_INVALID_ARG, "rs_learned_apply: null");
+    if (L->coarse == RS_COARSE_NETWORK && !L->have_params)
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_apply: parameters not set");
     int rc = require_runtime();
     if (rc) return rc;
+    if ((rc = learned_setup(*L, *A)) || (rc = learned_plans(*L))) return rc;
     hipStream_t st = runtime().stream;
-    uint32_t nb = vec_blocks(std::max(L->n, 1u));
-    if ((rc = L->tmp_y.alloc(std::max(L->n, 1u))) || (rc = L->tmp_r.alloc(std::max(L->n, 1u))) ||
-        (rc = L->tmp_part.alloc(nb)))
-        return rc;
-    if ((rc = rs_matrix_spmv_device(A, d_x, L->tmp_y.p))) return rc;
-    k_pair<<<nb, kBlock, 0, st>>>(d_b, L->tmp_y.p, d_x, d_xstar, L->tmp_r.p, L->n, L->tmp_part.p);
-    k_restrict<double><<<L->nc, kBlock, 0, st>>>(L->tmp_r.p, L->mptr.p, L->mrows.p, L->rc.p, nullptr);
+    CycLevel &V = *L->lv[0];
+    RS_HIP(hipMemcpyAsync(V.r.p, r_host, sizeof(float) * L->n, hipMemcpyHostToDevice, st));
+    L->r0 = V.r.p;
+    const float *z = cycle_level(*L, 0, nullptr, nullptr, nullptr);
     RS_HIP(hipGetLastError());
-    std::vector<double> parts(nb);
-    RS_HIP(hipMemcpyAsync(parts.data(), L->tmp_part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
-    RS_HIP(hipMemcpyAsync(rc_host, L->rc.p, sizeof(float) * L->nc, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipMemcpyAsync(z_host, z, sizeof(float) * L->n, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
-    double sum = 0.0;
-    for (double v : parts) sum += v;
-    *s_out = sum;
+    return RS_OK;
+}
+
+// Keep the coarsest right-hand side of the first `capacity` iterations of the following
+// learned solves (0 switches recording off).
+extern "C" int rs_learned_record(rs_learned *L, uint32_t capacity)
+{
+    if (!L) return fail(RS_ERR_INVALID_ARG, "rs_learned_record: null");
+    int rc = require_runtime();
+    if (rc) return rc;
+    if (capacity && (rc = L->rec.alloc((size_t)capacity * L->nc))) return rc;
+    L->rec_cap = capacity;
+    return RS_OK;
+}
+
+extern "C" int rs_learned_records(const rs_learned *L, uint32_t count, float *out)
+{
+    if (!L || !out || count > L->rec_cap) return fail(RS_ERR_INVALID_ARG, "rs_learned_records: bad arguments");
+    int rc = require_runtime();
+    if (rc) return rc;
+    if (count) {
+        RS_HIP(hipMemcpyAsync(out, L->rec.p, sizeof(float) * count * L->nc, hipMemcpyDeviceToHost, runtime().stream));
+        RS_HIP(hipStreamSynchronize(runtime().stream));
+    }
     return RS_OK;
 }
 
@@ -955,7 +1489,8 @@ extern "C" int rs_solver_solve_learned_device(rs_solver *S, rs_matrix *A, rs_lea
                                               rs_info *info)
 {
     if (!S || !A || !L || (A->n && (!d_b || !d_x))) return fail(RS_ERR_INVALID_ARG, "rs_solver_solve_learned_device: null");
-    if (!L->have_params) return fail(RS_ERR_INVALID_ARG, "learned preconditioner: parameters not set");
+    if (L->coarse == RS_COARSE_NETWORK && !L->have_params)
+        return fail(RS_ERR_INVALID_ARG, "learned preconditioner: parameters not set");
     if (S->conf.solver_type != RS_SOLVER_CG) return fail(RS_ERR_UNSUPPORTED, "learned preconditioner: flexible CG only");
     int rc = require_runtime();
     if (rc) return rc;
@@ -964,30 +1499,39 @@ extern "C" int rs_solver_solve_learned_device(rs_solver *S, rs_matrix *A, rs_lea
 
 // ---------------------------------------------------------------------------- trainer
 struct rs_trainer {
-    uint32_t max_batch = 0;
-    uint64_t step = 0;
-    DBuf<float> theta, m, v, g, grads, loss, Ac, rc, s;
-    DBuf<unsigned short> wbf;
+    uint32_t max_batch = 0, max_nodes = 0;
+    uint64_t max_nnz = 0, step = 0;
+    rs::DBuf<float> theta, m, v, g, grads, loss, cval, rc, s, dh, dm;
+    rs::DBuf<unsigned short> wbf, hs, mb, dy;
+    rs::DBuf<uint32_t> cptr, ccol, tpos;
 };
 
-extern "C" int rs_trainer_create(const float *params, uint64_t count, uint32_t max_batch, rs_trainer **out)
+extern "C" int rs_trainer_create(const float *params, uint64_t count, uint32_t max_batch, uint32_t max_nodes,
+                                 uint64_t max_nnz, rs_trainer **out)
 {
-    if (!params || !out || count != kNumParams || max_batch == 0)
-        return fail(RS_ERR_INVALID_ARG, "rs_trainer_create: bad arguments (expects 49921 parameters)");
+    if (!params || !out || count != kNumParams || max_batch == 0 || max_nodes == 0 || max_nodes > kMaxCoarse ||
+        max_nnz == 0)
+        return fail(RS_ERR_INVALID_ARG, "rs_trainer_create: bad arguments (49921 parameters, <= 512 nodes)");
     int rc = require_runtime();
     if (rc) return rc;
     std::unique_ptr<rs_trainer> T(new rs_trainer());
     T->max_batch = max_batch;
+    T->max_nodes = max_nodes;
+    T->max_nnz = max_nnz;
+    const uint64_t npad = (max_nodes + 15) / 16 * 16, act = npad * kHid;
     if ((rc = T->theta.alloc(kNumParams)) || (rc = T->m.alloc(kNumParams)) || (rc = T->v.alloc(kNumParams)) ||
         (rc = T->g.alloc(kNumParams)) || (rc = T->grads.alloc((uint64_t)max_batch * kNumParams)) ||
-        (rc = T->loss.alloc(max_batch)) || (rc = T->Ac.alloc((uint64_t)max_batch * kTrainN * kTrainN)) ||
-        (rc = T->rc.alloc((uint64_t)max_batch * kTrainN)) || (rc = T->s.alloc(max_batch)) ||
-        (rc = T->wbf.alloc(kLayers * kBfPerLayer)))
+        (rc = T->loss.alloc(max_batch)) || (rc = T->cval.alloc(max_batch * max_nnz)) ||
+        (rc = T->rc.alloc((uint64_t)max_batch * max_nodes)) || (rc = T->s.alloc(max_batch)) ||
+        (rc = T->wbf.alloc(kLayers * kBfPerLayer)) || (rc = T->hs.alloc(max_batch * (kLayers + 1) * act)) ||
+        (rc = T->mb.alloc(max_batch * act)) || (rc = T->dy.alloc(max_batch * act)) ||
+        (rc = T->dh.alloc(max_batch * act)) || (rc = T->dm.alloc(max_batch * act)) ||
+        (rc = T->cptr.alloc(max_nodes + 1ull)) || (rc = T->ccol.alloc(max_nnz)) || (rc = T->tpos.alloc(max_nnz)))
         return rc;
     hipStream_t st = runtime().stream;
     RS_HIP(hipMemcpyAsync(T->theta.p, params, sizeof(float) * kNumParams, hipMemcpyHostToDevice, st));
-    RS_HIP(hipMemsetAsync(T->m.p, 0, sizeof(float) * kNumParams, st));
-    RS_HIP(hipMemsetAsync(T->v.p, 0, sizeof(float) * kNumParams, st));
+    k_fill32<<<vec_blocks(kNumParams), kBlock, 0, st>>>(T->m.p, kNumParams, 0.0f);
+    k_fill32<<<vec_blocks(kNumParams), kBlock, 0, st>>>(T->v.p, kNumParams, 0.0f);
     k_weights_bf16<<<vec_blocks(kLayers * kBfPerLayer), kBlock, 0, st>>>(T->theta.p, T->wbf.p);
     RS_HIP(hipGetLastError());
     RS_HIP(hipStreamSynchronize(st));
@@ -995,21 +1539,61 @@ extern "C" int rs_trainer_create(const float *params, uint64_t count, uint32_t m
     return RS_OK;
 }
 
-// Per-sample losses and the batch-mean gradient (no update).  Ac: [B][64][64] (padding 0),
-// rc: [B][64], s: [B]; every sample has n <= 64 coarse nodes.
-extern "C" int rs_trainer_gradient(rs_trainer *T, uint32_t n, uint32_t B, const float *Ac, const float *rcv,
-                                   const float *s, float *loss_out, float *grad_out)
+// Per-sample losses and the batch-mean gradient (no update).  The B samples share the coarse
+// pattern (cptr, ccol: n rows, nnz entries, structurally symmetric, diagonal present); cval:
+// [B][nnz] operator values, rcv: [B][n] coarse residuals, s: [B] normalisers r.A^-1.r.
+extern "C" int rs_trainer_gradient(rs_trainer *T, uint32_t n, uint64_t nnz, const uint32_t *cptr, const uint32_t *ccol,
+                                   uint32_t B, const float *cval, const float *rcv, const float *s, float *loss_out,
+                                   float *grad_out)
 {
-    if (!T || !Ac || !rcv || !s || n == 0 || n > kTrainN || B == 0 || B > T->max_batch)
+    if (!T || !cptr || !ccol || !cval || !rcv || !s || n == 0 || n > T->max_nodes || nnz == 0 || nnz > T->max_nnz ||
+        B == 0 || B > T->max_batch || cptr[0] != 0 || cptr[n] != nnz)
         return fail(RS_ERR_INVALID_ARG, "rs_trainer_gradient: bad arguments");
+    // transpose positions (the pattern must be structurally symmetric with its diagonal)
+    std::vector<uint32_t> tpos(nnz);
+    for (uint32_t I = 0; I < n; I++) {
+        if (cptr[I + 1] < cptr[I]) return fail(RS_ERR_INVALID_ARG, "rs_trainer_gradient: cptr not monotone");
+        bool diag = false;
+        for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++) {
+            const uint32_t J = ccol[p];
+            if (J >= n) return fail(RS_ERR_INVALID_ARG, "rs_trainer_gradient: column out of range");
+            diag |= J == I;
+            uint32_t q = cptr[J];
+            while (q < cptr[J + 1] && ccol[q] != I) q++;
+            if (q == cptr[J + 1]) return fail(RS_ERR_INVALID_ARG, "rs_trainer_gradient: pattern not symmetric");
+            tpos[p] = q;
+        }
+        if (!diag) return fail(RS_ERR_INVALID_ARG, "rs_trainer_gradient: missing diagonal entry");
+    }
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    RS_HIP(hipMemcpyAsync(T->Ac.p, Ac, sizeof(float) * B * kTrainN * kTrainN, hipMemcpyHostToDevice, st));
-    RS_HIP(hipMemcpyAsync(T->rc.p, rcv, sizeof(float) * B * kTrainN, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(T->cptr.p, cptr, sizeof(uint32_t) * (n + 1ull), hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(T->ccol.p, ccol, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(T->tpos.p, tpos.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(T->cval.p, cval, sizeof(float) * B * nnz, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(T->rc.p, rcv, sizeof(float) * B * n, hipMemcpyHostToDevice, st));
     RS_HIP(hipMemcpyAsync(T->s.p, s, sizeof(float) * B, hipMemcpyHostToDevice, st));
-    TrainArgs t{ n, T->Ac.p, T->rc.p, T->s.p, T->theta.p, T->wbf.p, T->grads.p, T->loss.p };
-    k_gnn_train<<<B, kBlock, 0, st>>>(t);
+    TrainArgs t{};
+    t.n = n;
+    t.npad = (n + 15) / 16 * 16;
+    t.nnz = (uint32_t)nnz;
+    t.cptr = T->cptr.p;
+    t.ccol = T->ccol.p;
+    t.tpos = T->tpos.p;
+    t.cval = T->cval.p;
+    t.rc = T->rc.p;
+    t.s = T->s.p;
+    t.params = T->theta.p;
+    t.wbf = T->wbf.p;
+    t.grads = T->grads.p;
+    t.loss = T->loss.p;
+    t.hs = T->hs.p;
+    t.mb = T->mb.p;
+    t.dy = T->dy.p;
+    t.dh = T->dh.p;
+    t.dm = T->dm.p;
+    k_gnn_train<<<B, kTrainThreads, 0, st>>>(t);
     k_grad_mean<<<vec_blocks(kNumParams), kBlock, 0, st>>>(T->grads.p, B, T->g.p);
     RS_HIP(hipGetLastError());
     if (loss_out) RS_HIP(hipMemcpyAsync(loss_out, T->loss.p, sizeof(float) * B, hipMemcpyDeviceToHost, st));
@@ -1019,23 +1603,23 @@ extern "C" int rs_trainer_gradient(rs_trainer *T, uint32_t n, uint32_t B, const 
 }
 
 // One Adam step on a batch; returns the mean loss.
-extern "C" int rs_trainer_step(rs_trainer *T, uint32_t n, uint32_t B, const float *Ac, const float *rcv, const float *s,
-                               float lr, float *mean_loss)
+extern "C" int rs_trainer_step(rs_trainer *T, uint32_t n, uint64_t nnz, const uint32_t *cptr, const uint32_t *ccol,
+                               uint32_t B, const float *cval, const float *rcv, const float *s, float lr, float *mean_loss)
 {
-    std::vector<float> losses(B);
-    int rc = rs_trainer_gradient(T, n, B, Ac, rcv, s, losses.data(), nullptr);
+    std::vector<float> losses(std::max(B, 1u));
+    int rc = rs_trainer_gradient(T, n, nnz, cptr, ccol, B, cval, rcv, s, losses.data(), nullptr);
     if (rc) return rc;
     T->step++;
     const float b1 = 0.9f, b2 = 0.999f;
-    float c1 = 1.0f - std::pow(b1, (float)T->step), c2 = 1.0f - std::pow(b2, (float)T->step);
+    const float c1 = 1.0f - std::pow(b1, (float)T->step), c2 = 1.0f - std::pow(b2, (float)T->step);
     hipStream_t st = runtime().stream;
-    k_adam<<<vec_blocks(kNumParams), kBlock, 0, st>>>(T->theta.p, T->m.p, T->v.p, T->g.p, lr, b1, b2, 1e-8f, c1, c2, T->wbf.p);
+    k_adam<<<vec_blocks(kNumParams), kBlock, 0, st>>>(T->theta.p, T->m.p, T->v.p, T->g.p, lr, b1, b2, 1e-8f, c1, c2);
     k_weights_bf16<<<vec_blocks(kLayers * kBfPerLayer), kBlock, 0, st>>>(T->theta.p, T->wbf.p);
     RS_HIP(hipGetLastError());
     RS_HIP(hipStreamSynchronize(st));
     if (mean_loss) {
         double m = 0.0;
-        for (float l : losses) m += l;
+        for (uint32_t b = 0; b < B; b++) m += losses[b];
         *mean_loss = (float)(m / B);
     }
     return RS_OK;

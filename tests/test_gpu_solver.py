@@ -270,8 +270,8 @@ def test_cxx_drop_in_replays_reference_test(gpu):
     """The C++ mirror (include/rmatrixsolver_amd.hpp) replays tst_matrix_solver.cpp's cases."""
     import subprocess
     exe = os.path.join(os.path.dirname(__file__), "cxx", "tst_matrix_solver_amd")
-    if not os.path.exists(exe):
-        subprocess.run(["make", "-s", "-C", os.path.dirname(exe)], check=True)
+    # make rebuilds the binary when the C ABI header changed since it was built
+    subprocess.run(["make", "-s", "-C", os.path.dirname(exe)], check=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr

@@ -103,3 +103,24 @@ def test_config_counts(rs):
     fixed = ((np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12)).astype(np.uint8)
     _, n_dof = rs.node_book(xyz.shape[0], tets, 1, fixed)
     assert n_dof == 8379
+
+
+def test_hierarchy_by_bins_nested(rs, orc):
+    """Nested coordinate bins: each level map halves the bins; composing the maps gives the
+    direct binning of the finest positions at the coarser resolution."""
+    xyz, tets = rs.kuhn_box(16, 16, 16)
+    x = xyz[:, 0]
+    fixed = ((np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12)).astype(np.uint8)
+    book, n_dof = orc.node_book(xyz.shape[0], tets, 1, fixed)
+    maps = rs.hierarchy_by_bins(xyz, book, 1, (16, 16, 16), max_coarse=512)
+    assert len(maps) == 2 and maps[0].size == n_dof
+    sizes = [int(m.max()) + 1 for m in maps]
+    assert sizes[1] == 512 and maps[1].size == sizes[0]
+    direct16, n16 = rs.aggregates_by_bins(xyz, book, 1, (16, 16, 16))
+    direct8, n8 = rs.aggregates_by_bins(xyz, book, 1, (8, 8, 8))
+    assert np.array_equal(maps[0], direct16) and sizes[0] == n16
+    assert np.array_equal(maps[1][maps[0]], direct8) and n8 == 512
+    # every aggregate of a 2x2x2 binning holds at most 8 nodes
+    assert np.bincount(maps[1]).max() <= 8
+    with pytest.raises(rs.RSError):
+        rs.hierarchy_by_bins(xyz, book, 1, (16, 16, 16), max_coarse=512, max_levels=1)

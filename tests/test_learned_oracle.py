@@ -71,3 +71,107 @@ def test_galerkin_operator():
     for I in range(7):
         for J in range(7):
             assert Ac[I, J] == pytest.approx(D[np.ix_(agg == I, agg == J)].sum(), abs=1e-12)
+
+
+# ------------------------------------------------------------------------------ multilevel cycle
+def _cheb_residual_poly(lam, k, x):
+    """Chebyshev residual polynomial R_k(x) = T_k((theta - x)/delta) / T_k(theta/delta)."""
+    hi, lo = 1.05 * lam, lam / 30.0
+    theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    T = np.polynomial.chebyshev.Chebyshev.basis(k)
+    return T((theta - x) / delta) / T(theta / delta)
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 4])
+def test_chebyshev_smoother_closed_form(steps):
+    # A = D: D^-1 A = I, so k steps from zero give z = (1 - R_k(1)) D^-1 r
+    rng = np.random.default_rng(steps)
+    d = 0.5 + rng.random(50)
+    A = sp.diags(d).tocsr()
+    r = rng.standard_normal(50)
+    lam = 1.7
+    z = lo.cheb_smooth(A, 1.0 / d, lo.cheb_coefs(lam), r, None, steps)
+    np.testing.assert_allclose(z, (1.0 - _cheb_residual_poly(lam, steps, 1.0)) * r / d, rtol=1e-12)
+    # error propagation on an eigenvector of D^-1 A with eigenvalue mu: e <- R_k(mu) e
+    n = 40
+    L = sp.diags([-np.ones(n - 1), 2.0 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).tocsr()
+    mu_k = 1.0 - np.cos(np.pi * 3 / (n + 1))            # eigenvalue of D^-1 L (D = 2 I)
+    v = np.sin(np.pi * 3 * np.arange(1, n + 1) / (n + 1))
+    x = lo.cheb_smooth(L, 0.5 * np.ones(n), lo.cheb_coefs(2.0), L @ v, None, steps)
+    np.testing.assert_allclose(v - x, _cheb_residual_poly(2.0, steps, mu_k) * v, atol=1e-12)
+
+
+def _heat_like(n=10, seed=3):
+    """2-D five-point operator with random positive edge weights and a Dirichlet shift."""
+    rng = np.random.default_rng(seed)
+    N = n * n
+    rows, cols, vals = [], [], []
+    diag = np.full(N, 0.05)
+    for i in range(n):
+        for j in range(n):
+            a = i * n + j
+            for di, dj in ((1, 0), (0, 1)):
+                if i + di < n and j + dj < n:
+                    b = (i + di) * n + (j + dj)
+                    w = np.exp(rng.standard_normal())
+                    rows += [a, b]
+                    cols += [b, a]
+                    vals += [-w, -w]
+                    diag[a] += w
+                    diag[b] += w
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N)) + sp.diags(diag)
+    m1 = np.array([(i // 2) * (n // 2) + (j // 2) for i in range(n) for j in range(n)])
+    m2 = np.array([(i // 2) * (n // 4 + (n // 2) % 2) + (j // 2) for i in range(n // 2) for j in range(n // 2)])
+    _, m2 = np.unique(m2, return_inverse=True)
+    return A.tocsr(), [m1, m2]
+
+
+def test_two_level_projection_property():
+    A, maps = _heat_like()
+    levels, Ps = lo.hierarchy(A, maps[:1])
+    y = np.random.default_rng(0).standard_normal(levels[1].shape[0])
+    r = A @ (Ps[0] @ y)
+    # no smoothing: z = P A_c^-1 P^T r = P y exactly for r in A range(P)
+    A0 = levels[0]
+    Ac = levels[1]
+    z = Ps[0] @ lo.coarse_exact(Ac, Ps[0].T @ r)
+    np.testing.assert_allclose(z, Ps[0] @ y, rtol=1e-10, atol=1e-12)
+    # and the Galerkin operator is the restriction of A to range(P)
+    np.testing.assert_allclose(Ac.toarray(), (Ps[0].T @ A0 @ Ps[0]).toarray())
+
+
+def test_symmetric_cycle_is_symmetric_positive_definite():
+    A, maps = _heat_like()
+    levels, Ps = lo.hierarchy(A, maps)
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    for pre in (1, 2):
+        B = lambda r: lo.vcycle(levels, Ps, r, pre, pre, lo.coarse_exact)
+        assert abs(x @ B(y) - y @ B(x)) < 1e-10 * abs(x @ B(x))
+        assert x @ B(x) > 0 and y @ B(y) > 0
+    # a plausible mistake (the post-smoother continuing the pre-smoother's step coefficients)
+    # breaks the symmetry
+    lv2, P2 = lo.hierarchy(A, maps[:1])
+    dinv = 1.0 / A.diagonal()
+    coefs = lo.cheb_coefs(lo.gershgorin(A))
+
+    def bad(r):
+        z = lo.cheb_smooth(A, dinv, coefs, r, None, 1)
+        z = z + P2[0] @ lo.coarse_exact(lv2[1], P2[0].T @ (r - A @ z))
+        return lo.cheb_smooth(A, dinv, coefs[1:], r, z, 1)
+    assert abs(x @ bad(y) - y @ bad(x)) > 1e-4 * abs(x @ bad(x))
+
+
+def test_cycle_preconditions_cg():
+    import scipy.sparse.linalg as sla
+    A, maps = _heat_like(16, 5)
+    levels, Ps = lo.hierarchy(A, maps)
+    b = np.random.default_rng(2).standard_normal(A.shape[0])
+
+    def its(M):
+        k = [0]
+        sla.cg(A, b, rtol=1e-8, maxiter=2000, M=M, callback=lambda _: k.__setitem__(0, k[0] + 1))
+        return k[0]
+    jac = its(sla.LinearOperator(A.shape, matvec=lambda r: r / A.diagonal()))
+    mg = its(sla.LinearOperator(A.shape, matvec=lambda r: lo.vcycle(levels, Ps, r, 1, 1, lo.coarse_exact)))
+    assert mg < 0.5 * jac
