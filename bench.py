@@ -5,11 +5,16 @@ Workload (default): steady heat on the unit cube, 128^3 hexahedral cells split i
 (2,146,689 nodes, 12,582,912 tets, 2,113,407 unknowns), conductivity lognormal(0, 2) per tet
 (seed 2), volumetric source U(0,1) per tet (seed 1), T=0 on x=0 and T=1 on x=1.
 
-One step = one system taken from resident device inputs to a solution twice:
+One step = one system taken from resident device inputs to a solution three times:
   (a) GPU assembly + plain FP64 Jacobi-CG          (the drop-in path)
   (b) GPU assembly + mixed-precision Jacobi-CG     (FP32 SpMV/preconditioner, FP64 residuals)
-both to ||b - A x|| / ||b|| <= 1e-8 (checked in FP64 after the solve).  `value` is the time of
-the fastest method per system; both methods' times are in `methods`.
+  (c) GPU assembly + mixed-precision flexible CG with the learned multilevel preconditioner
+      (V-cycle over nested 2x2x2 aggregates, FP32 Chebyshev smoothing, the trained network on
+      the 512-node coarsest level; hierarchy values rebuilt for every system)
+all to ||b - A x|| / ||b|| <= 1e-8 (checked in FP64 after the solve).  `value` is the time of
+(c) per system; every method's time is in `methods`, with the solution error against an FP64
+solution converged to 1e-12.  After the timed steps, (c) runs once more with the coarsest
+level solved by coarse Jacobi (the untrained network) and by the exact inverse: the ablation.
 
 The roofline object is for the dominant kernel (the CSR SpMV), timed live with stream events
 on the library's stream; algorithmic bytes per launch = nnz*(s_val+4) + n*(4+2*s_x) + 4.
@@ -113,6 +118,28 @@ def spmv_only(rs, asm, dk, dq, dT, n, nnz) -> int:
     return 0
 
 
+def learned_only(rs, args, asm, xyz, dk, dq, dT, db, dx, zero_x, conf) -> int:
+    """Profiling mode: the learned method alone (eager launches under RS_NO_GRAPH=1)."""
+    maps = rs.hierarchy_by_bins(xyz, asm.book(), 1, (max(1, args.cells // 2),) * 3, max_coarse=rs.MAX_COARSE)
+    L = rs.LearnedPreconditioner(asm.matrix, maps)
+    params, _ = rs.load_weights()
+    L.set_params(params if params is not None else rs.init_params(0))
+    L.set_cycle(args.pre, args.post, rs.COARSE_NETWORK)
+    sl = rs.RMatrixSolver(conf(rs.PRECISION_MIXED, args.tol))
+    out = []
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        asm.assemble_heat_device(dk, dq, dT, db)
+        zero_x()
+        info = sl.solve_learned_device(asm.matrix, L, db, dx)
+        rs.synchronize()
+        wall = time.perf_counter() - t0
+        log(f"learned run {i}: {wall:.3f}s, {info.iterations} its, relres {info.true_relres:.2e}")
+        out.append({"wall_s": round(wall, 4), "iterations": info.iterations, "solve_ms": round(info.solve_ms, 2)})
+    print(json.dumps({"learned_only": out, "levels": [int(v) for v in L.level_rows]}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +150,10 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--asm-cells", type=int, default=40)
+    ap.add_argument("--learned-only", action="store_true",
+                    help="profiling mode: time only the learned method (no FP64/mixed CG, no ablation)")
+    ap.add_argument("--pre", type=int, default=1, help="pre-smoothing steps of the V-cycle")
+    ap.add_argument("--post", type=int, default=1, help="post-smoothing steps of the V-cycle")
     ap.add_argument("--spmv-only", action="store_true",
                     help="profiling mode: a 256 MiB device copy (counter calibration), then "
                          "20 FP64 and 20 FP32 SpMV launches of the tuned shapes; no solves")
@@ -173,6 +204,9 @@ def main() -> int:
                 "reason": rs.REASON_NAMES.get(info.reason), "refinements": info.inner_iterations}
 
     # reference solution for the error check: FP64 CG to 1e-12 (untimed)
+    x_star = None
+    if args.learned_only:
+        return learned_only(rs, args, asm, xyz, dk, dq, dT, db, dx, zero_x, conf)
     log("reference FP64 solve to 1e-12")
     sref = rs.RMatrixSolver(conf(rs.PRECISION_FP64, 1e-12))
     asm.assemble_heat_device(dk, dq, dT, db)
@@ -181,36 +215,91 @@ def main() -> int:
     x_star = dx.download()
     log(f"reference: its={iref.iterations} relres={iref.true_relres:.2e} {iref.solve_ms:.0f} ms")
 
+    log("aggregation hierarchy (nested coordinate bins) for the learned preconditioner")
+    t0 = time.perf_counter()
+    maps = rs.hierarchy_by_bins(xyz, asm.book(), 1, (max(1, c // 2),) * 3, max_coarse=rs.MAX_COARSE)
+    L = rs.LearnedPreconditioner(asm.matrix, maps)
+    t_hier = time.perf_counter() - t0
+    params, summary = rs.load_weights()
+    trained = params is not None
+    L.set_params(params if trained else rs.init_params(0))
+    L.set_cycle(args.pre, args.post, rs.COARSE_NETWORK)
+    log(f"levels {list(map(int, L.level_rows))} built in {t_hier:.1f}s; network "
+        f"{'trained (' + rs.WEIGHTS_PATH + ')' if trained else 'UNTRAINED (no weights file)'}")
+    sl = rs.RMatrixSolver(conf(rs.PRECISION_MIXED, args.tol))
+
+    def run_learned():
+        t0 = time.perf_counter()
+        asm.assemble_heat_device(dk, dq, dT, db)
+        asm_ms = asm.last_ms
+        zero_x()
+        info = sl.solve_learned_device(asm.matrix, L, db, dx)
+        rs.synchronize()
+        wall = time.perf_counter() - t0
+        return {"wall_s": wall, "assembly_ms": asm_ms, "solve_ms": info.solve_ms,
+                "iterations": info.iterations, "true_relres": info.true_relres,
+                "reason": rs.REASON_NAMES.get(info.reason), "refinements": info.inner_iterations,
+                "fallback": info.fallback, "network_applications": info.precond_applications}
+
     for i in range(args.warmup):
-        r64, rmp = run(s64), run(smp)
+        r64, rmp, rl = run(s64), run(smp), run_learned()
         log(f"warmup {i}: fp64 {r64['wall_s']:.3f}s ({r64['iterations']} its) "
-            f"mixed {rmp['wall_s']:.3f}s ({rmp['iterations']} its)")
+            f"mixed {rmp['wall_s']:.3f}s ({rmp['iterations']} its) "
+            f"learned {rl['wall_s']:.3f}s ({rl['iterations']} its)")
     rs.synchronize()
-    res64, resmp = [], []
+    res64, resmp, resl = [], [], []
+    err_mp = err_l = None
     t_start = time.perf_counter()
     for i in range(args.steps):
         r64 = run(s64)
         rmp = run(smp)
+        x_mp = dx.download() if i == args.steps - 1 else None
+        rl = run_learned()
         res64.append(r64)
         resmp.append(rmp)
+        resl.append(rl)
         log(f"step {i}: fp64 {r64['wall_s']:.3f}s ({r64['iterations']} its, relres "
-            f"{r64['true_relres']:.2e}) mixed {rmp['wall_s']:.3f}s ({rmp['iterations']} its, "
-            f"relres {rmp['true_relres']:.2e})")
+            f"{r64['true_relres']:.2e}) mixed {rmp['wall_s']:.3f}s ({rmp['iterations']} its) "
+            f"learned {rl['wall_s']:.3f}s ({rl['iterations']} its, relres {rl['true_relres']:.2e}, "
+            f"fallback {rl['fallback']})")
     rs.synchronize()
     t_total = time.perf_counter() - t_start
-    x_mp = dx.download()
-    err_mp = float(np.linalg.norm(x_mp - x_star) / np.linalg.norm(x_star))
+    x_l = dx.download()
+    if x_mp is not None:
+        err_mp = float(np.linalg.norm(x_mp - x_star) / np.linalg.norm(x_star))
+    err_l = float(np.linalg.norm(x_l - x_star) / np.linalg.norm(x_star))
 
     def summarize(rr):
         med = lambda key: float(np.median([r[key] for r in rr]))  # noqa: E731
-        return {"time_s": round(med("wall_s"), 4), "assembly_ms": round(med("assembly_ms"), 3),
-                "solve_ms": round(med("solve_ms"), 2), "iterations": int(med("iterations")),
-                "true_relres": max(r["true_relres"] for r in rr),
-                "reason": rr[-1]["reason"], "fp64_residual_replacements": int(med("refinements"))}
+        out = {"time_s": round(med("wall_s"), 4), "assembly_ms": round(med("assembly_ms"), 3),
+               "solve_ms": round(med("solve_ms"), 2), "iterations": int(med("iterations")),
+               "true_relres": max(r["true_relres"] for r in rr),
+               "reason": rr[-1]["reason"], "fp64_residual_replacements": int(med("refinements"))}
+        if "fallback" in rr[0]:
+            out["fallback"] = max(r["fallback"] for r in rr)
+        return out
 
-    m64, mmp = summarize(res64), summarize(resmp)
+    m64, mmp, ml = summarize(res64), summarize(resmp), summarize(resl)
     mmp["solution_error_vs_fp64"] = err_mp
-    best = min(m64["time_s"], mmp["time_s"])
+    ml["solution_error_vs_fp64"] = err_l
+    ml["network"] = "trained" if trained else "untrained"
+    ml["levels"] = [int(v) for v in L.level_rows]
+    ml["cycle"] = {"pre": args.pre, "post": args.post, "smoother": "Chebyshev in D^-1 A, FP32"}
+    ml["hierarchy_build_s"] = round(t_hier, 2)
+    best = ml["time_s"]
+
+    # ablation of the learned part: the same solve with the coarsest level by coarse Jacobi
+    # (what the untrained network computes) and by the exact dense inverse
+    ablation = {"network": {"iterations": ml["iterations"], "time_s": ml["time_s"]}}
+    for name, mode in (("coarse_jacobi", rs.COARSE_JACOBI), ("exact_inverse", rs.COARSE_EXACT)):
+        L.set_cycle(args.pre, args.post, mode)
+        r = run_learned()
+        ablation[name] = {"iterations": r["iterations"], "time_s": round(r["wall_s"], 4),
+                          "true_relres": r["true_relres"]}
+        log(f"ablation {name}: {r['wall_s']:.3f}s ({r['iterations']} its)")
+    L.set_cycle(args.pre, args.post, rs.COARSE_NETWORK)
+    ablation["note"] = ("exact_inverse includes an O(512^3) host inversion of the coarsest operator "
+                        "per system")
 
     # roofline of the dominant kernel (CSR SpMV), stream events on the library stream
     spmv = {}
@@ -242,14 +331,18 @@ def main() -> int:
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(t_total / max(1, args.steps) * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (plain) / f32 SpMV+precond with f64 residuals (mixed)",
+        "dtype": ("f64 (plain) / f32 SpMV+precond, f64 residuals (mixed) / f32 cycle, bf16 network on "
+                  "MFMA, f64 residuals (learned)"),
         "data": "synthetic",
         "config": {"workload": f"steady heat {c}^3 cells x 6 tets, lognormal(0,{args.sigma}) "
-                               f"conductivity, Jacobi-CG to relres {args.tol}",
+                               f"conductivity, to relres {args.tol}: learned+mixed flexible CG vs "
+                               f"plain FP64 Jacobi-CG",
                    "nodes": int(xyz.shape[0]), "tets": int(tets.shape[0]), "unknowns": n,
                    "nnz": int(nnz), "colours": asm.n_colours, "parallelism": "single GPU"},
-        "methods": {"fp64_jacobi_cg": m64, "mixed_jacobi_cg": mmp,
-                    "speedup_mixed_vs_fp64": round(m64["time_s"] / mmp["time_s"], 3)},
+        "methods": {"fp64_jacobi_cg": m64, "mixed_jacobi_cg": mmp, "learned_mixed_fcg": ml,
+                    "speedup_mixed_vs_fp64": round(m64["time_s"] / mmp["time_s"], 3),
+                    "speedup_learned_vs_fp64": round(m64["time_s"] / ml["time_s"], 3)},
+        "learned_ablation": ablation,
         "spmv": spmv, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -546,6 +546,35 @@ MAX_COARSE = 512
 COARSE_NETWORK, COARSE_JACOBI, COARSE_EXACT = 0, 1, 2
 
 
+NET_HID, NET_LAYERS, NET_IN = 64, 6, 4
+WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "heat_coarse512.npz")
+
+
+def init_params(seed: int) -> np.ndarray:
+    """Initial network parameters: He-scaled input and layer maps, zero biases and a zero
+    output head, so that the untrained network computes coarse Jacobi (e = r_c / diag)."""
+    rng = np.random.default_rng(seed)
+    h = NET_HID
+    parts = [rng.standard_normal(NET_IN * h) * np.sqrt(2.0 / NET_IN), np.zeros(h)]
+    for _ in range(NET_LAYERS):
+        parts += [rng.standard_normal(h * h) * 0.5 / np.sqrt(h), rng.standard_normal(h * h) * 0.5 / np.sqrt(h),
+                  np.zeros(h)]
+    parts += [np.zeros(h), np.zeros(1)]
+    theta = np.concatenate(parts).astype(np.float32)
+    assert theta.size == N_PARAMS
+    return theta
+
+
+def load_weights(path: str = WEIGHTS_PATH):
+    """Trained parameters and their training summary (None, None when absent)."""
+    if not os.path.exists(path):
+        return None, None
+    with np.load(path, allow_pickle=False) as z:
+        params = z["params"].astype(np.float32)
+        summary = str(z["summary"]) if "summary" in z.files else None
+    return params, summary
+
+
 def aggregates_by_bins(xyz, book, dofs_per_node: int, bins) -> tuple[np.ndarray, int]:
     """Coordinate-bin aggregates of the enabled DOFs (bins = (nbx, nby, nbz))."""
     load()

@@ -314,14 +314,41 @@ struct GnnArgs {
 
 constexpr int kInferThreads = 512;
 
+// M[I] = sum over the neighbours J of I of w_IJ H[J]: one wavefront per node, lane = feature,
+// the node's neighbour list read through the scalar unit (wave-uniform addresses).
+__device__ __forceinline__ void gnn_aggregate(const unsigned short *H, unsigned short *M, uint32_t nc, uint32_t npad,
+                                              const uint32_t *__restrict__ cptr, const uint32_t *__restrict__ ccol,
+                                              const float *__restrict__ cw, int nwaves)
+{
+    const int lane = threadIdx.x & 63;
+    for (uint32_t I = threadIdx.x / kWave; I < npad; I += nwaves) {
+        const uint32_t Iu = __builtin_amdgcn_readfirstlane(I);
+        float m = 0.0f;
+        if (Iu < nc) {
+            const uint32_t p1 = cptr[Iu + 1];
+            uint32_t p = cptr[Iu];
+            // eight independent neighbour reads in flight
+            for (; p + 8 <= p1; p += 8) {
+                float t[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) t[k] = cw[p + k] * bf2f(H[ccol[p + k] * kHid + lane]);
+                m += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+            }
+            for (; p < p1; p++) m += cw[p] * bf2f(H[ccol[p] * kHid + lane]);
+        }
+        M[Iu * kHid + lane] = f2bf(m);
+    }
+}
+
 __global__ __launch_bounds__(kInferThreads) void k_gnn_infer(GnnArgs g)
 {
     if (g.st && g.st->done) return;
+    constexpr int kWaves = kInferThreads / kWave;
     __shared__ unsigned short H[kMaxCoarse * kHid];
     __shared__ unsigned short M[kMaxCoarse * kHid];
-    __shared__ unsigned short W[kBfPerLayer];
+    __shared__ unsigned short Wt[kBfPerLayer];   // [0:4096) Ws^T, [4096:8192) Wn^T (row = output feature)
     __shared__ float u[kMaxCoarse], ld[kMaxCoarse];
-    __shared__ float red[kInferThreads / kWave];
+    __shared__ float red[kWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
     const uint32_t nc = g.nc, npad = g.npad;
     const float *P = g.params;
@@ -340,36 +367,33 @@ __global__ __launch_bounds__(kInferThreads) void k_gnn_infer(GnnArgs g)
     }
     const float sigma = sqrtf(block_sum_f(su, red, kInferThreads) / (float)nc) + 1e-30f;
     const float meanlog = block_sum_f(sl, red, kInferThreads) / (float)nc;
-    for (uint32_t idx = tid; idx < npad * kHid; idx += kInferThreads) {
-        uint32_t I = idx / kHid, f = idx % kHid;
-        float z = 0.0f;
-        if (I < nc)
-            z = (u[I] / sigma) * P[kOffWin + f] + (ld[I] - meanlog) * P[kOffWin + kHid + f] +
-                P[kOffWin + 2 * kHid + f] + P[kOffBin + f];
-        H[idx] = f2bf(fmaxf(z, 0.0f));
+    {
+        const float w0 = P[kOffWin + lane], w1 = P[kOffWin + kHid + lane], c = P[kOffWin + 2 * kHid + lane] + P[kOffBin + lane];
+        for (uint32_t I = wave; I < npad; I += kWaves) {
+            float z = I < nc ? (u[I] / sigma) * w0 + (ld[I] - meanlog) * w1 + c : 0.0f;
+            H[I * kHid + lane] = f2bf(fmaxf(z, 0.0f));
+        }
     }
-    __syncthreads();
     for (int l = 0; l < kLayers; l++) {
-        for (uint32_t k = tid; k < kBfPerLayer; k += kInferThreads) W[k] = g.wbf[l * kBfPerLayer + k];
-        for (uint32_t idx = tid; idx < npad * kHid; idx += kInferThreads) {
-            uint32_t I = idx / kHid, f = idx % kHid;
-            float m = 0.0f;
-            if (I < nc)
-                for (uint32_t p = g.cptr[I]; p < g.cptr[I + 1]; p++) m += g.cw[p] * bf2f(H[g.ccol[p] * kHid + f]);
-            M[idx] = f2bf(m);
+        // transposed BF16 layer matrices: Wt[h][c][k] = W_h[k][c]
+        for (uint32_t k = tid; k < kBfPerLayer; k += kInferThreads) {
+            const uint32_t h = k >> 12, r = (k >> 6) & 63, c = k & 63;
+            Wt[(h << 12) + c * kHid + r] = g.wbf[l * kBfPerLayer + k];
         }
         __syncthreads();
+        gnn_aggregate(H, M, nc, npad, g.cptr, g.ccol, g.cw, kWaves);
+        __syncthreads();
         const float *b = P + off_b(l);
-        for (uint32_t strip = wave; strip < npad / 16; strip += kInferThreads / kWave) {
+        for (uint32_t strip = wave; strip < npad / 16; strip += kWaves) {
             const uint32_t r0 = strip * 16;
             f32x4 acc[4];
 #pragma unroll
             for (int ct = 0; ct < 4; ct++) {
                 acc[ct] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
                 acc[ct] = mfma_tile([&](int r, int k) { return H[(r0 + r) * kHid + k]; },
-                                    [&](int k, int c) { return W[k * kHid + ct * 16 + c]; }, kHid, acc[ct]);
+                                    [&](int k, int c) { return Wt[(ct * 16 + c) * kHid + k]; }, kHid, acc[ct]);
                 acc[ct] = mfma_tile([&](int r, int k) { return M[(r0 + r) * kHid + k]; },
-                                    [&](int k, int c) { return W[kHid * kHid + k * kHid + ct * 16 + c]; }, kHid, acc[ct]);
+                                    [&](int k, int c) { return Wt[4096 + (ct * 16 + c) * kHid + k]; }, kHid, acc[ct]);
             }
             // every MFMA of the strip has read its H rows (this wavefront's only) before the update
 #pragma unroll
@@ -384,10 +408,10 @@ __global__ __launch_bounds__(kInferThreads) void k_gnn_infer(GnnArgs g)
         }
         __syncthreads();
     }
-    for (uint32_t I = tid; I < nc; I += kInferThreads) {
-        float o = P[kOffBo];
-        for (int f = 0; f < kHid; f++) o += bf2f(H[I * kHid + f]) * P[kOffWo + f];
-        g.e[I] = u[I] + sigma * o;
+    const float wo = P[kOffWo + lane], bo = P[kOffBo];
+    for (uint32_t I = wave; I < nc; I += kWaves) {
+        float o = group_sum<kWave>(bf2f(H[I * kHid + lane]) * wo);
+        if (lane == 0) g.e[I] = u[I] + sigma * (o + bo);
     }
 }
 
@@ -469,18 +493,19 @@ __global__ __launch_bounds__(kTrainThreads) void k_gnn_train(TrainArgs t)
     for (uint32_t idx = tid; idx < act; idx += kTrainThreads) Hs[idx] = f2bf(fmaxf(z0(idx / kHid, idx % kHid), 0.0f));
     __syncthreads();
     // M = wn H, wn[I][J] = -A[I][J]/A[I][I] off the diagonal
+    // one wavefront per node, lane = feature
     auto aggregate = [&](const unsigned short *Hl) {
-        for (uint32_t idx = tid; idx < act; idx += kTrainThreads) {
-            uint32_t I = idx / kHid, f = idx % kHid;
+        for (uint32_t I = wave; I < npad; I += kWaves) {
+            const uint32_t Iu = __builtin_amdgcn_readfirstlane(I);
             float m = 0.0f;
-            if (I < n) {
-                const float inv = 1.0f / dg[I];
-                for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++) {
-                    uint32_t J = ccol[p];
-                    if (J != I) m += (-cv[p] * inv) * bf2f(Hl[J * kHid + f]);
+            if (Iu < n) {
+                const float inv = 1.0f / dg[Iu];
+                for (uint32_t p = cptr[Iu]; p < cptr[Iu + 1]; p++) {
+                    const uint32_t J = ccol[p];
+                    if (J != Iu) m += (-cv[p] * inv) * bf2f(Hl[J * kHid + lane]);
                 }
             }
-            Mb[idx] = f2bf(m);
+            Mb[Iu * kHid + lane] = f2bf(m);
         }
     };
     // pre-activation Y - b of a 16-row strip, four 16-column tiles
@@ -609,15 +634,14 @@ __global__ __launch_bounds__(kTrainThreads) void k_gnn_train(TrainArgs t)
         }
         __syncthreads();
         // dH[I] += sum_J wn[J][I] dM[J]
-        for (uint32_t idx = tid; idx < act; idx += kTrainThreads) {
-            uint32_t I = idx / kHid, f = idx % kHid;
-            if (I >= n) continue;
+        for (uint32_t I = wave; I < n; I += kWaves) {
+            const uint32_t Iu = __builtin_amdgcn_readfirstlane(I);
             float a = 0.0f;
-            for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++) {
-                uint32_t J = ccol[p];
-                if (J != I) a += (-cv[t.tpos[p]] / dg[J]) * dM[J * kHid + f];
+            for (uint32_t p = cptr[Iu]; p < cptr[Iu + 1]; p++) {
+                const uint32_t J = ccol[p];
+                if (J != Iu) a += (-cv[t.tpos[p]] / dg[J]) * dM[J * kHid + lane];
             }
-            dH[idx] += a;
+            dH[Iu * kHid + lane] += a;
         }
         __syncthreads();
     }
