@@ -136,7 +136,10 @@ def learned_only(rs, args, asm, xyz, dk, dq, dT, db, dx, zero_x, conf) -> int:
         wall = time.perf_counter() - t0
         log(f"learned run {i}: {wall:.3f}s, {info.iterations} its, relres {info.true_relres:.2e}")
         out.append({"wall_s": round(wall, 4), "iterations": info.iterations, "solve_ms": round(info.solve_ms, 2)})
-    print(json.dumps({"learned_only": out, "levels": [int(v) for v in L.level_rows]}), flush=True)
+    infer_ms = L.infer_timed(200)
+    log(f"network inference (back-to-back): {infer_ms * 1e3:.1f} us")
+    print(json.dumps({"learned_only": out, "levels": [int(v) for v in L.level_rows],
+                      "infer_us": round(infer_ms * 1e3, 2)}), flush=True)
     return 0
 
 

@@ -280,6 +280,8 @@ int rs_learned_level_matrix(const rs_learned *L, uint32_t level, uint32_t *row_p
 int rs_learned_coarse_matrix(const rs_learned *L, double *dense);         /* n_coarse^2 */
 int rs_learned_restrict_device(rs_learned *L, const double *d_r, float *rc_host);
 int rs_learned_apply_coarse(rs_learned *L, const float *rc_host, float *e_host);
+/* Average duration (ms) of the network inference kernel over reps back-to-back launches. */
+int rs_learned_infer_timed(rs_learned *L, uint32_t reps, double *avg_ms);
 /* One V-cycle z = B r (host vectors of length n). */
 int rs_learned_apply(rs_learned *L, rs_matrix *A, const float *r_host, float *z_host);
 /* Training data: the coarsest right-hand side of each of the first `capacity` iterations of

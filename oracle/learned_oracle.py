@@ -6,7 +6,9 @@ pins its mathematics: the backward pass here is checked against central finite d
 the forward pass (tests/test_learned_oracle.py), and the GPU gradients are then checked against
 this backward pass with the same BF16 rounding points (`bf16=True`).
 
-Network (per coarse graph with N nodes, dense coarse operator Ac, coarse residual rc):
+Network (per coarse graph with N nodes, dense coarse operator Ac, coarse residual rc; with
+`bf16_points` the weights wn, the activations H, M and the back-propagated dY are rounded to
+BF16 where the GPU kernels round them):
     d_I = Ac[I,I];  u_I = rc_I / d_I;  sigma = sqrt(mean(u^2)) + tiny
     wn_IJ = -Ac[I,J] / d_I  (J != I), 0 on the diagonal
     F = [u/sigma, log(d/geomean(d)), 1, 0]                       (N x 4)
@@ -107,6 +109,7 @@ def forward(theta, Ac, rc, s=None, bf16_points: bool = False, layers: int = LAYE
     p = unpack(theta, layers)
     q = bf16 if bf16_points else (lambda a: np.asarray(a, dtype=np.float64))
     d, u, sigma, F, wn = _inputs(Ac, rc)
+    wn = q(wn)      # the message weights are BF16 operands as well
     Z0 = F @ p["Win"] + p["bin"]
     H = [q(np.maximum(Z0, 0.0))]
     Ms, Ys = [], []

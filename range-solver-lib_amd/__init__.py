@@ -120,7 +120,7 @@ _SIGS = {
     "rs_learned_level_matrix": [_P, C.c_uint32, _P, _P, _P],
     "rs_learned_coarse_matrix": [_P, _P],
     "rs_learned_restrict_device": [_P, _P, _P], "rs_learned_apply_coarse": [_P, _P, _P],
-    "rs_learned_apply": [_P, _P, _P, _P],
+    "rs_learned_apply": [_P, _P, _P, _P], "rs_learned_infer_timed": [_P, C.c_uint32, _f64p],
     "rs_learned_record": [_P, C.c_uint32], "rs_learned_records": [_P, C.c_uint32, _P],
     "rs_learned_destroy": [_P],
     "rs_solver_solve_learned_device": [_P, _P, _P, _P, _P, C.POINTER(Info)],
@@ -681,6 +681,12 @@ class LearnedPreconditioner:
         out = np.empty(self.n_coarse, np.float32)
         _check(_lib.rs_learned_apply_coarse(self.h, _ptr(r), _ptr(out)))
         return out
+
+    def infer_timed(self, reps: int = 100) -> float:
+        """Average inference kernel time (ms), back-to-back launches."""
+        ms = C.c_double()
+        _check(_lib.rs_learned_infer_timed(self.h, reps, C.byref(ms)))
+        return ms.value
 
     def apply(self, r, A: SparseMatrix | None = None) -> np.ndarray:
         """One V-cycle z = B r."""

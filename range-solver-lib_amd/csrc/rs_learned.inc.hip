@@ -44,14 +44,8 @@ __host__ __device__ constexpr uint32_t off_b(int l) { return off_ws(l) + 2 * kHi
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16, round to nearest even (NaN kept quiet).
-__device__ __forceinline__ unsigned short f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 
 // One 16x16 output tile: acc += A(16 x K) B(K x 16), K a multiple of 16, operands fetched as
@@ -301,118 +295,175 @@ __global__ void k_record(const float *__restrict__ r, float *__restrict__ rec, u
 }
 
 // ---------------------------------------------------------------------------- inference
+// The coarsest graph for inference as 16 x 16 BF16 blocks of the message weights over strips
+// of 16 nodes: blocks bptr[s]..bptr[s+1] of row strip s, column strip bcol[b], entries
+// bval[b][c][k] = w(16 s + c, 16 bcol[b] + k) (row-major per block, zero where no edge).
+// Activations live in two global ping-pong buffers (L2-resident, 64 KB each), node-major h and
+// feature-major ht (row stride npad), so that every MFMA operand is one 8-byte load.
 struct GnnArgs {
     uint32_t nc, npad;
     const float *rc, *diag;
-    const uint32_t *cptr, *ccol;
-    const float *cw;
+    const uint32_t *bptr, *bcol;
+    const unsigned short *bval;
     const float *params;
-    const unsigned short *wbf;
+    const unsigned short *wbt;       // transposed BF16 layer matrices [layer][Ws^T | Wn^T][64][64]
+    unsigned short *h[2], *ht[2];
+    float *u;                        // u = r_c / diag (npad)
+    float *norm;                     // [sigma, mean log diag]
     float *e;
     const SolveState *st;
 };
 
-constexpr int kInferThreads = 512;
+__device__ __forceinline__ bf16x4 bf16x4_of(uint2 w) { return __builtin_bit_cast(bf16x4, w); }
 
-// M[I] = sum over the neighbours J of I of w_IJ H[J]: one wavefront per node, lane = feature,
-// the node's neighbour list read through the scalar unit (wave-uniform addresses).
-__device__ __forceinline__ void gnn_aggregate(const unsigned short *H, unsigned short *M, uint32_t nc, uint32_t npad,
-                                              const uint32_t *__restrict__ cptr, const uint32_t *__restrict__ ccol,
-                                              const float *__restrict__ cw, int nwaves)
+constexpr int kInferMaxBlk = 9;     // graph blocks per strip prefetched into registers
+
+// Input features and the first activations of one strip of 16 nodes (64 threads, lane = feature).
+__global__ __launch_bounds__(kWave) void k_gnn_embed(GnnArgs g)
 {
-    const int lane = threadIdx.x & 63;
-    for (uint32_t I = threadIdx.x / kWave; I < npad; I += nwaves) {
-        const uint32_t Iu = __builtin_amdgcn_readfirstlane(I);
-        float m = 0.0f;
-        if (Iu < nc) {
-            const uint32_t p1 = cptr[Iu + 1];
-            uint32_t p = cptr[Iu];
-            // eight independent neighbour reads in flight
-            for (; p + 8 <= p1; p += 8) {
-                float t[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++) t[k] = cw[p + k] * bf2f(H[ccol[p + k] * kHid + lane]);
-                m += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
-            }
-            for (; p < p1; p++) m += cw[p] * bf2f(H[ccol[p] * kHid + lane]);
+    if (g.st && g.st->done) return;
+    const int lane = threadIdx.x;
+    const uint32_t nc = g.nc, npad = g.npad, r0 = blockIdx.x * 16;
+    // sigma = rms(u) and the mean of log diag over all nodes (every workgroup, same order)
+    float su = 0.0f, sl = 0.0f;
+    for (uint32_t I = lane; I < nc; I += kWave) {
+        const float d = g.diag[I], uu = g.rc[I] / d;
+        su += uu * uu;
+        sl += logf(d);
+    }
+    su = group_sum<kWave>(su);
+    sl = group_sum<kWave>(sl);
+    const float sigma = sqrtf(su / (float)nc) + 1e-30f, meanlog = sl / (float)nc;
+    if (blockIdx.x == 0 && lane == 0) {
+        g.norm[0] = sigma;
+        g.norm[1] = meanlog;
+    }
+    const float *P = g.params;
+    const float w0 = P[kOffWin + lane], w1 = P[kOffWin + kHid + lane], c = P[kOffWin + 2 * kHid + lane] + P[kOffBin + lane];
+    for (uint32_t I = r0; I < r0 + 16; I++) {
+        float z = 0.0f;
+        if (I < nc) {
+            const float d = g.diag[I], uu = g.rc[I] / d;
+            z = (uu / sigma) * w0 + (logf(d) - meanlog) * w1 + c;
+            if (lane == 0) g.u[I] = uu;
+        } else if (lane == 0) {
+            g.u[I] = 0.0f;
         }
-        M[Iu * kHid + lane] = f2bf(m);
+        const unsigned short hv = f2bf(fmaxf(z, 0.0f));
+        g.h[0][I * kHid + lane] = hv;
+        g.ht[0][lane * npad + I] = hv;
     }
 }
 
-__global__ __launch_bounds__(kInferThreads) void k_gnn_infer(GnnArgs g)
+// Layer l for one strip of 16 nodes, 4 wavefronts.  Wavefront w computes the messages of feature
+// block w (M^T = H^T G^T over the strip's graph blocks, MFMA; the accumulator layout of M^T is
+// the A-operand layout of M) and then the output column tile w of Y = H Ws + M Wn.
+__global__ __launch_bounds__(4 * kWave) void k_gnn_layer(GnnArgs g, int l)
 {
     if (g.st && g.st->done) return;
-    constexpr int kWaves = kInferThreads / kWave;
-    __shared__ unsigned short H[kMaxCoarse * kHid];
-    __shared__ unsigned short M[kMaxCoarse * kHid];
-    __shared__ unsigned short Wt[kBfPerLayer];   // [0:4096) Ws^T, [4096:8192) Wn^T (row = output feature)
-    __shared__ float u[kMaxCoarse], ld[kMaxCoarse];
-    __shared__ float red[kWaves];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
-    const uint32_t nc = g.nc, npad = g.npad;
+    __shared__ __align__(16) unsigned short sM[16 * kHid];
+    __shared__ float sO[4][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x / 64;
+    const int rc = lane & 15, kq = (lane >> 4) * 4;
+    const uint32_t nc = g.nc, npad = g.npad, s = blockIdx.x, r0 = s * 16;
+    const unsigned short *H = g.h[l & 1], *Ht = g.ht[l & 1];
+    unsigned short *Hn = g.h[(l + 1) & 1], *Htn = g.ht[(l + 1) & 1];
     const float *P = g.params;
-    float su = 0.0f, sl = 0.0f;
-    for (uint32_t I = tid; I < npad; I += kInferThreads) {
-        float uu = 0.0f, l = 0.0f;
-        if (I < nc) {
-            float d = g.diag[I];
-            uu = g.rc[I] / d;
-            l = logf(d);
-        }
-        u[I] = uu;
-        ld[I] = l;
-        su += uu * uu;
-        sl += l;
+    // operands of the output tile first: they do not depend on the messages
+    const unsigned short *Wl = g.wbt + (size_t)l * kBfPerLayer;
+    const int ct = w;
+    uint2 ws[4], wn[4], ha[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) {
+        ws[kt] = *reinterpret_cast<const uint2 *>(Wl + (ct * 16 + rc) * kHid + kt * 16 + kq);
+        wn[kt] = *reinterpret_cast<const uint2 *>(Wl + 4096 + (ct * 16 + rc) * kHid + kt * 16 + kq);
+        ha[kt] = *reinterpret_cast<const uint2 *>(H + (r0 + rc) * kHid + kt * 16 + kq);
     }
-    const float sigma = sqrtf(block_sum_f(su, red, kInferThreads) / (float)nc) + 1e-30f;
-    const float meanlog = block_sum_f(sl, red, kInferThreads) / (float)nc;
+    float hold[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) hold[i] = bf2f(H[(r0 + kq + i) * kHid + ct * 16 + rc]);
+    const float bias = P[off_b(l) + ct * 16 + rc];
+    // messages of feature block kt = w
+    const uint32_t b0 = g.bptr[s], nb = g.bptr[s + 1] - b0, last = b0 + nb - 1;
+    const int kt = w;
+    uint2 bv[kInferMaxBlk], av[kInferMaxBlk];
+#pragma unroll
+    for (int j = 0; j < kInferMaxBlk; j++) {
+        const uint32_t b = min(b0 + j, last);
+        bv[j] = *reinterpret_cast<const uint2 *>(g.bval + (uint64_t)b * 256 + rc * 16 + kq);
+        av[j] = *reinterpret_cast<const uint2 *>(Ht + (kt * 16 + rc) * npad + g.bcol[b] * 16 + kq);
+    }
+    f32x4 macc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int j = 0; j < kInferMaxBlk; j++)
+        if ((uint32_t)j < nb) macc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4_of(av[j]), bf16x4_of(bv[j]), macc, 0, 0, 0);
+    for (uint32_t b = b0 + kInferMaxBlk; b < b0 + nb; b++) {
+        const uint2 bb = *reinterpret_cast<const uint2 *>(g.bval + (uint64_t)b * 256 + rc * 16 + kq);
+        const uint2 aa = *reinterpret_cast<const uint2 *>(Ht + (kt * 16 + rc) * npad + g.bcol[b] * 16 + kq);
+        macc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4_of(aa), bf16x4_of(bb), macc, 0, 0, 0);
+    }
+    // lane holds M[r0 + rc][16 kt + kq + i]
     {
-        const float w0 = P[kOffWin + lane], w1 = P[kOffWin + kHid + lane], c = P[kOffWin + 2 * kHid + lane] + P[kOffBin + lane];
-        for (uint32_t I = wave; I < npad; I += kWaves) {
-            float z = I < nc ? (u[I] / sigma) * w0 + (ld[I] - meanlog) * w1 + c : 0.0f;
-            H[I * kHid + lane] = f2bf(fmaxf(z, 0.0f));
-        }
-    }
-    for (int l = 0; l < kLayers; l++) {
-        // transposed BF16 layer matrices: Wt[h][c][k] = W_h[k][c]
-        for (uint32_t k = tid; k < kBfPerLayer; k += kInferThreads) {
-            const uint32_t h = k >> 12, r = (k >> 6) & 63, c = k & 63;
-            Wt[(h << 12) + c * kHid + r] = g.wbf[l * kBfPerLayer + k];
-        }
-        __syncthreads();
-        gnn_aggregate(H, M, nc, npad, g.cptr, g.ccol, g.cw, kWaves);
-        __syncthreads();
-        const float *b = P + off_b(l);
-        for (uint32_t strip = wave; strip < npad / 16; strip += kWaves) {
-            const uint32_t r0 = strip * 16;
-            f32x4 acc[4];
+        bf16x4 mv;
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++) {
-                acc[ct] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
-                acc[ct] = mfma_tile([&](int r, int k) { return H[(r0 + r) * kHid + k]; },
-                                    [&](int k, int c) { return Wt[(ct * 16 + c) * kHid + k]; }, kHid, acc[ct]);
-                acc[ct] = mfma_tile([&](int r, int k) { return M[(r0 + r) * kHid + k]; },
-                                    [&](int k, int c) { return Wt[4096 + (ct * 16 + c) * kHid + k]; }, kHid, acc[ct]);
+        for (int i = 0; i < 4; i++) mv[i] = (short)f2bf(macc[i]);
+        *reinterpret_cast<bf16x4 *>(&sM[rc * kHid + kt * 16 + kq]) = mv;
+    }
+    __syncthreads();
+    f32x4 acc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint2 ma = *reinterpret_cast<const uint2 *>(&sM[rc * kHid + k * 16 + kq]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4_of(ha[k]), bf16x4_of(ws[k]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4_of(ma), bf16x4_of(wn[k]), acc, 0, 0, 0);
+    }
+    const uint32_t f = ct * 16 + rc;
+    float hv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t row = r0 + kq + i;
+        const unsigned short b16 = row < nc ? f2bf(hold[i] + fmaxf(acc[i] + bias, 0.0f)) : (unsigned short)0;
+        hv[i] = bf2f(b16);
+        Hn[row * kHid + f] = b16;
+        Htn[f * npad + row] = b16;
+    }
+    if (l == kLayers - 1) {
+        // e = u + sigma (H wo + bo) for the strip's nodes
+        const float wo = P[kOffWo + f];
+        float part[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) part[i] = group_sum<16>(hv[i] * wo);
+        if (rc == 0)
+#pragma unroll
+            for (int i = 0; i < 4; i++) sO[w][kq + i] = part[i];
+        __syncthreads();
+        if (threadIdx.x < 16) {
+            const uint32_t row = r0 + threadIdx.x;
+            if (row < nc) {
+                const float o = ((sO[0][threadIdx.x] + sO[1][threadIdx.x]) + (sO[2][threadIdx.x] + sO[3][threadIdx.x])) + P[kOffBo];
+                g.e[row] = g.u[row] + g.norm[0] * o;
             }
-            // every MFMA of the strip has read its H rows (this wavefront's only) before the update
-#pragma unroll
-            for (int ct = 0; ct < 4; ct++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    uint32_t row = r0 + (lane >> 4) * 4 + i, f = ct * 16 + (lane & 15);
-                    float y = acc[ct][i] + b[f];
-                    float h = bf2f(H[row * kHid + f]) + fmaxf(y, 0.0f);
-                    H[row * kHid + f] = row < nc ? f2bf(h) : (unsigned short)0;
-                }
         }
-        __syncthreads();
     }
-    const float wo = P[kOffWo + lane], bo = P[kOffBo];
-    for (uint32_t I = wave; I < nc; I += kWaves) {
-        float o = group_sum<kWave>(bf2f(H[I * kHid + lane]) * wo);
-        if (lane == 0) g.e[I] = u[I] + sigma * (o + bo);
-    }
+}
+
+static void launch_gnn(const GnnArgs &g, hipStream_t s)
+{
+    const uint32_t strips = g.npad / 16;
+    k_gnn_embed<<<strips, kWave, 0, s>>>(g);
+    for (int l = 0; l < kLayers; l++) k_gnn_layer<<<strips, 4 * kWave, 0, s>>>(g, l);
+}
+
+// Graph blocks from the coarsest CSR weights: bval[bidx[p]] = bf16(cw[p]) (bval zeroed first).
+__global__ void k_fill_u16(unsigned short *__restrict__ a, uint64_t n)
+{
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (uint64_t)gridDim.x * kBlock) a[k] = 0;
+}
+__global__ void k_gnn_blocks(const float *__restrict__ cw, const uint32_t *__restrict__ bidx, uint64_t nnz,
+                             unsigned short *__restrict__ bval)
+{
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < nnz; p += (uint64_t)gridDim.x * kBlock)
+        bval[bidx[p]] = f2bf(cw[p]);
 }
 
 }  // namespace rs
@@ -499,10 +550,10 @@ __global__ __launch_bounds__(kTrainThreads) void k_gnn_train(TrainArgs t)
             const uint32_t Iu = __builtin_amdgcn_readfirstlane(I);
             float m = 0.0f;
             if (Iu < n) {
-                const float inv = 1.0f / dg[Iu];
+                const float d = dg[Iu];
                 for (uint32_t p = cptr[Iu]; p < cptr[Iu + 1]; p++) {
                     const uint32_t J = ccol[p];
-                    if (J != Iu) m += (-cv[p] * inv) * bf2f(Hl[J * kHid + lane]);
+                    if (J != Iu) m += bf2f(f2bf(-cv[p] / d)) * bf2f(Hl[J * kHid + lane]);
                 }
             }
             Mb[Iu * kHid + lane] = f2bf(m);
@@ -639,7 +690,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_gnn_train(TrainArgs t)
             float a = 0.0f;
             for (uint32_t p = cptr[Iu]; p < cptr[Iu + 1]; p++) {
                 const uint32_t J = ccol[p];
-                if (J != Iu) a += (-cv[t.tpos[p]] / dg[J]) * dM[J * kHid + lane];
+                if (J != Iu) a += bf2f(f2bf(-cv[t.tpos[p]] / dg[J])) * dM[J * kHid + lane];
             }
             dH[Iu * kHid + lane] += a;
         }
@@ -679,6 +730,15 @@ __global__ void k_adam(float *__restrict__ theta, float *__restrict__ m, float *
         m[p] = mp;
         v[p] = vp;
         theta[p] -= lr * (mp / c1) / (sqrtf(vp / c2) + eps);
+    }
+}
+
+// BF16 transposed layer matrices for inference: wbt[l][h][c][k] = W_h[l][k][c] (h = 0: Ws, 1: Wn)
+__global__ void k_weights_bf16_t(const float *__restrict__ theta, unsigned short *__restrict__ wbt)
+{
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < kLayers * kBfPerLayer; k += gridDim.x * kBlock) {
+        const uint32_t l = k / kBfPerLayer, o = k % kBfPerLayer, h = o >> 12, c = (o >> 6) & 63, r = o & 63;
+        wbt[k] = f2bf(theta[off_ws(l) + (h << 12) + r * kHid + c]);
     }
 }
 
@@ -750,6 +810,10 @@ struct rs_learned {
     const rs_matrix *owner = nullptr;                // the matrix whose pattern the hierarchy follows
     std::vector<std::unique_ptr<rs::CycLevel>> lv;   // lv[0] fine ... lv[nlev - 1] coarsest
     rs::DBuf<uint32_t> tot_mptr, tot_mrows;          // fine rows of each coarsest node
+    rs::DBuf<uint32_t> gbptr, gbcol, gbidx;          // coarsest graph as 16 x 16 blocks (inference)
+    rs::DBuf<unsigned short> gbval;
+    rs::DBuf<unsigned short> gh, ght;                // inference activations, two buffers each
+    rs::DBuf<float> gu, gnorm;
     rs::DBuf<float> cw, cdiag, coef, cinv, rec, tmp_rc, tmp_r;
     rs::DBuf<double> part, gersh;
     rs::DBuf<float> theta;
@@ -858,6 +922,8 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     }
     const Matrix &C = *L.lv.back()->M;
     k_coarse_weights<<<vec_blocks(L.nc), kBlock, 0, st>>>(C.rowptr.p, C.col.p, C.val.p, L.nc, L.cw.p, L.cdiag.p);
+    k_fill_u16<<<vec_blocks(L.gbval.n), kBlock, 0, st>>>(L.gbval.p, L.gbval.n);
+    k_gnn_blocks<<<vec_blocks(std::max<uint64_t>(C.nnz, 1)), kBlock, 0, st>>>(L.cw.p, L.gbidx.p, C.nnz, L.gbval.p);
     RS_HIP(hipGetLastError());
     L.lam.assign(L.nlev, 0.0);
     RS_HIP(hipMemcpyAsync(L.lam.data(), L.gersh.p, sizeof(double) * L.nlev, hipMemcpyDeviceToHost, st));
@@ -900,11 +966,18 @@ static GnnArgs gnn_args(rs_learned &L, const SolveState *st)
     g.npad = (L.nc + 15) / 16 * 16;
     g.rc = V.r.p;
     g.diag = L.cdiag.p;
-    g.cptr = V.M->rowptr.p;
-    g.ccol = V.M->col.p;
-    g.cw = L.cw.p;
+    g.bptr = L.gbptr.p;
+    g.bcol = L.gbcol.p;
+    g.bval = L.gbval.p;
     g.params = L.theta.p;
-    g.wbf = L.wbf.p;
+    g.wbt = L.wbf.p;
+    const size_t act = (size_t)g.npad * kHid;
+    g.h[0] = L.gh.p;
+    g.h[1] = L.gh.p + act;
+    g.ht[0] = L.ght.p;
+    g.ht[1] = L.ght.p + act;
+    g.u = L.gu.p;
+    g.norm = L.gnorm.p;
     g.e = V.za.p;
     g.st = st;
     return g;
@@ -931,7 +1004,7 @@ static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st,
     const uint32_t nvb = vec_blocks(std::max(V.n, 1u));
     const float *r = l == 0 ? L.r0 : V.r.p;
     if (l + 1 == L.nlev) {
-        if (L.coarse == RS_COARSE_NETWORK) k_gnn_infer<<<1, kInferThreads, 0, s>>>(gnn_args(L, st));
+        if (L.coarse == RS_COARSE_NETWORK) launch_gnn(gnn_args(L, st), s);
         else if (L.coarse == RS_COARSE_EXACT) k_coarse_exact<<<vec_blocks((uint64_t)V.n * kWave), kBlock, 0, s>>>(L.cinv.p, r, V.za.p, V.n, st);
         else k_coarse_jacobi<<<nvb, kBlock, 0, s>>>(r, V.dinv.p, V.za.p, V.n, st);
         return V.za.p;
@@ -1335,8 +1408,37 @@ extern "C" int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const ui
         std::vector<uint32_t> fill(tptr.begin(), tptr.end() - 1);
         for (uint32_t i = 0; i < A->n; i++) trows[fill[total[i]]++] = i;
     }
+    // coarsest graph blocks: rp/ci hold the coarsest pattern here
+    const uint32_t nstrips = (nc + 15) / 16;
+    std::vector<uint32_t> gbptr(nstrips + 1ull, 0), gbcol, gbidx(ci.size());
+    for (uint32_t s = 0; s < nstrips; s++) {
+        std::vector<uint32_t> ts;
+        for (uint32_t I = s * 16; I < std::min(nc, s * 16 + 16); I++)
+            for (uint32_t p = rp[I]; p < rp[I + 1]; p++) ts.push_back(ci[p] / 16);
+        std::sort(ts.begin(), ts.end());
+        ts.erase(std::unique(ts.begin(), ts.end()), ts.end());
+        const uint32_t b0 = (uint32_t)gbcol.size();
+        gbcol.insert(gbcol.end(), ts.begin(), ts.end());
+        gbptr[s + 1] = (uint32_t)gbcol.size();
+        for (uint32_t I = s * 16; I < std::min(nc, s * 16 + 16); I++)
+            for (uint32_t p = rp[I]; p < rp[I + 1]; p++) {
+                const uint32_t t = ci[p] / 16;
+                const uint32_t b = b0 + (uint32_t)(std::lower_bound(ts.begin(), ts.end(), t) - ts.begin());
+                gbidx[p] = b * 256 + (I % 16) * 16 + ci[p] % 16;
+            }
+    }
     rs_learned &R = *L;
     const Matrix &Cc = *R.lv.back()->M;
+    if ((rc = R.gbptr.alloc(gbptr.size())) || (rc = R.gbcol.alloc(std::max<size_t>(gbcol.size(), 1))) ||
+        (rc = R.gbidx.alloc(std::max<size_t>(gbidx.size(), 1))) || (rc = R.gbval.alloc(std::max<size_t>(gbcol.size(), 1) * 256)) ||
+        (rc = R.gh.alloc(2ull * nstrips * 16 * kHid)) || (rc = R.ght.alloc(2ull * nstrips * 16 * kHid)) ||
+        (rc = R.gu.alloc(nstrips * 16)) || (rc = R.gnorm.alloc(2)))
+        return rc;
+    RS_HIP(hipMemcpyAsync(R.gbptr.p, gbptr.data(), sizeof(uint32_t) * gbptr.size(), hipMemcpyHostToDevice, st));
+    if (!gbcol.empty())
+        RS_HIP(hipMemcpyAsync(R.gbcol.p, gbcol.data(), sizeof(uint32_t) * gbcol.size(), hipMemcpyHostToDevice, st));
+    if (!gbidx.empty())
+        RS_HIP(hipMemcpyAsync(R.gbidx.p, gbidx.data(), sizeof(uint32_t) * gbidx.size(), hipMemcpyHostToDevice, st));
     if ((rc = R.tot_mptr.alloc(tptr.size())) || (rc = R.tot_mrows.alloc(std::max(A->n, 1u))) ||
         (rc = R.cw.alloc(std::max<uint64_t>(Cc.nnz, 1))) || (rc = R.cdiag.alloc(nc)) ||
         (rc = R.coef.alloc((size_t)R.nlev * kMaxDeg * 2)) || (rc = R.tmp_rc.alloc(nc)) ||
@@ -1358,7 +1460,7 @@ extern "C" int rs_learned_set_params(rs_learned *L, const float *params, uint64_
     if (rc) return rc;
     hipStream_t st = runtime().stream;
     RS_HIP(hipMemcpyAsync(L->theta.p, params, sizeof(float) * kNumParams, hipMemcpyHostToDevice, st));
-    k_weights_bf16<<<vec_blocks(kLayers * kBfPerLayer), kBlock, 0, st>>>(L->theta.p, L->wbf.p);
+    k_weights_bf16_t<<<vec_blocks(kLayers * kBfPerLayer), kBlock, 0, st>>>(L->theta.p, L->wbf.p);
     RS_HIP(hipGetLastError());
     RS_HIP(hipStreamSynchronize(st));
     L->have_params = true;
@@ -1452,10 +1554,38 @@ extern "C" int rs_learned_apply_coarse(rs_learned *L, const float *rc_host, floa
     hipStream_t st = runtime().stream;
     CycLevel &V = *L->lv.back();
     RS_HIP(hipMemcpyAsync(V.r.p, rc_host, sizeof(float) * L->nc, hipMemcpyHostToDevice, st));
-    k_gnn_infer<<<1, kInferThreads, 0, st>>>(gnn_args(*L, nullptr));
+    launch_gnn(gnn_args(*L, nullptr), st);
     RS_HIP(hipGetLastError());
     RS_HIP(hipMemcpyAsync(e_host, V.za.p, sizeof(float) * L->nc, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
+
+// Average duration of one network inference (embedding + 6 layer launches) over `reps`
+// back-to-back inferences on the coarsest residual of the last cycle (stream events on the
+// library stream).
+extern "C" int rs_learned_infer_timed(rs_learned *L, uint32_t reps, double *avg_ms)
+{
+    if (!L || !avg_ms || reps == 0) return fail(RS_ERR_INVALID_ARG, "rs_learned_infer_timed: bad arguments");
+    if (!L->have_params || L->lam.empty()) return fail(RS_ERR_INVALID_ARG, "rs_learned_infer_timed: set parameters and set up first");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    const GnnArgs g = gnn_args(*L, nullptr);
+    launch_gnn(g, st);
+    hipEvent_t e0, e1;
+    RS_HIP(hipEventCreate(&e0));
+    RS_HIP(hipEventCreate(&e1));
+    RS_HIP(hipEventRecord(e0, st));
+    for (uint32_t i = 0; i < reps; i++) launch_gnn(g, st);
+    RS_HIP(hipEventRecord(e1, st));
+    RS_HIP(hipEventSynchronize(e1));
+    RS_HIP(hipGetLastError());
+    float ms = 0.0f;
+    RS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
     return RS_OK;
 }
 

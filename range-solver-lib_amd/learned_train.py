@@ -157,6 +157,10 @@ def train(rs, cells=64, sigma=2.5, problems=2000, seed0=10000, holdout=0.05, rec
             held = evaluate(T, cptr, ccol, hold_data, batch)
             curve.append((step, float(m), held))
             say(f"step {step}/{steps}: batch loss {m:.4f}, held-out {held:.4f}, lr {a:.2e}, {time.time() - t0:.0f}s")
+            if out:
+                os.makedirs(os.path.dirname(out), exist_ok=True)
+                np.savez(out, params=T.params().astype(np.float32),
+                         summary=json.dumps({"partial": True, "step": step, "heldout_loss": held}))
     theta = T.params()
     held = evaluate(T, cptr, ccol, hold_data, batch)
     # held-out solves: iterations with the trained network, coarse Jacobi and the exact inverse
