@@ -55,14 +55,14 @@ def spmv_bytes(n: int, nnz: int, s_val: int, s_x: int) -> int:
     return nnz * (s_val + 4) + n * (4 + 2 * s_x) + 4
 
 
-def build_problem(rs, cells: int, sigma: float):
+def build_problem(rs, cells: int, sigma: float, seed_k: int = 2, seed_q: int = 1):
     xyz, tets = rs.kuhn_box(cells, cells, cells)
     ne = tets.shape[0]
     x = xyz[:, 0]
     fixed = ((np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12)).astype(np.uint8)
     t_node = np.where(np.abs(x - 1) < 1e-12, 1.0, 0.0)
-    k = rs.field_lognormal(2, ne, 0.0, sigma) if sigma > 0 else np.ones(ne)
-    q = rs.field_uniform(1, ne)
+    k = rs.field_lognormal(seed_k, ne, 0.0, sigma) if sigma > 0 else np.ones(ne)
+    q = rs.field_uniform(seed_q, ne)
     return xyz, tets, fixed, t_node, k, q
 
 
@@ -151,6 +151,7 @@ def main() -> int:
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--sigma", type=float, default=2.0)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--seed-k", type=int, default=2, help="conductivity field seed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--asm-cells", type=int, default=40)
     ap.add_argument("--learned-only", action="store_true",
@@ -171,7 +172,7 @@ def main() -> int:
     rs.init(0)
     c = args.cells
     log(f"building {c}^3 Kuhn mesh and fields")
-    xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma)
+    xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma, args.seed_k)
     log(f"nodes={xyz.shape[0]} tets={tets.shape[0]}; pattern + colouring + scatter map")
     t0 = time.perf_counter()
     asm = rs.Assembly(xyz, tets, 1, fixed)
@@ -338,8 +339,8 @@ def main() -> int:
                   "MFMA, f64 residuals (learned)"),
         "data": "synthetic",
         "config": {"workload": f"steady heat {c}^3 cells x 6 tets, lognormal(0,{args.sigma}) "
-                               f"conductivity, to relres {args.tol}: learned+mixed flexible CG vs "
-                               f"plain FP64 Jacobi-CG",
+                               f"conductivity (seed {args.seed_k}), to relres {args.tol}: learned+mixed "
+                               f"flexible CG vs plain FP64 Jacobi-CG",
                    "nodes": int(xyz.shape[0]), "tets": int(tets.shape[0]), "unknowns": n,
                    "nnz": int(nnz), "colours": asm.n_colours, "parallelism": "single GPU"},
         "methods": {"fp64_jacobi_cg": m64, "mixed_jacobi_cg": mmp, "learned_mixed_fcg": ml,

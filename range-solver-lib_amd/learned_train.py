@@ -117,9 +117,12 @@ def batches(data, batch: int, rng):
             yield cv, R, S
 
 
-def evaluate(T, cptr, ccol, data, batch: int) -> float:
+def evaluate(T, cptr, ccol, data, batch: int, max_batches: int = 16) -> float:
+    """Mean loss over (up to max_batches batches of) the held-out pairs."""
     losses = []
-    for cv, R, S in batches_once(data, batch):
+    for k, (cv, R, S) in enumerate(batches_once(data, batch)):
+        if k == max_batches:
+            break
         losses.append(T.gradient_csr(cptr, ccol, cv, R, S)[0])
     return float(np.mean(np.concatenate(losses))) if losses else float("nan")
 
