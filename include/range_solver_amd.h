@@ -268,6 +268,10 @@ int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const uint32_t *agg
 int rs_learned_set_params(rs_learned *L, const float *params, uint64_t count);
 /* Smoothing steps before (0..4) and after (1..4) the coarse correction, coarsest solver. */
 int rs_learned_set_cycle(rs_learned *L, uint32_t pre, uint32_t post, int coarse_mode);
+/* Levels with at least min_rows rows smooth with a compressed copy of D^-1 A (FP16 values,
+ * int16 column offsets in the SELL-64 slots; 4 bytes per entry); 0 keeps every level on its
+ * FP32 values.  Default 65536.  Levels whose offsets do not fit in 16 bits stay FP32. */
+int rs_learned_set_compressed(rs_learned *L, uint32_t min_rows);
 /* Levels including the finest; level_rows / level_nnz hold n_levels entries when given. */
 int rs_learned_info(const rs_learned *L, uint32_t *n_levels, uint32_t *level_rows,
                     uint64_t *level_nnz, uint64_t *n_params);

@@ -214,8 +214,21 @@ def cheb_smooth(A, dinv, coefs, r, z, steps):
     return z
 
 
-def vcycle(levels, Ps, r, pre, post, coarse, lams=None, level=0):
-    """z = B r; coarse(Ac, r) solves the coarsest level."""
+def compressed(A):
+    """D fp16(D^-1 A): the operator a level smooths with when it uses the FP16 copy of D^-1 A
+    (values rounded to binary16, round to nearest even)."""
+    import scipy.sparse as sp
+    A = A.tocsr()
+    dg = A.diagonal()
+    S = sp.diags(1.0 / dg) @ A
+    S = S.tocsr()
+    S.data = S.data.astype(np.float16).astype(np.float64)
+    return (sp.diags(dg) @ S).tocsr()
+
+
+def vcycle(levels, Ps, r, pre, post, coarse, lams=None, level=0, fp16_levels=()):
+    """z = B r; coarse(Ac, r) solves the coarsest level; levels listed in fp16_levels smooth
+    (and form the restricted residual) with compressed(A_l)."""
     A = levels[level]
     if level == len(levels) - 1:
         return coarse(A, r)
@@ -223,11 +236,12 @@ def vcycle(levels, Ps, r, pre, post, coarse, lams=None, level=0):
     coefs = cheb_coefs(lam)
     dg = A.diagonal()
     dinv = np.where(dg != 0, 1.0 / np.where(dg != 0, dg, 1.0), 0.0)
-    z = cheb_smooth(A, dinv, coefs, r, None, pre) if pre > 0 else None
-    res = r if z is None else r - A @ z
-    e = vcycle(levels, Ps, Ps[level].T @ res, pre, post, coarse, lams, level + 1)
+    As = compressed(A) if level in fp16_levels else A
+    z = cheb_smooth(As, dinv, coefs, r, None, pre) if pre > 0 else None
+    res = r if z is None else r - As @ z
+    e = vcycle(levels, Ps, Ps[level].T @ res, pre, post, coarse, lams, level + 1, fp16_levels)
     z = Ps[level] @ e if z is None else z + Ps[level] @ e
-    return cheb_smooth(A, dinv, coefs, r, z, post)
+    return cheb_smooth(As, dinv, coefs, r, z, post)
 
 
 def coarse_exact(Ac, r):

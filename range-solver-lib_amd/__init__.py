@@ -115,6 +115,7 @@ _SIGS = {
     "rs_learned_create": [_P, C.c_uint32, _P, _P, C.POINTER(_P)],
     "rs_learned_set_params": [_P, _P, C.c_uint64],
     "rs_learned_set_cycle": [_P, C.c_uint32, C.c_uint32, C.c_int],
+    "rs_learned_set_compressed": [_P, C.c_uint32],
     "rs_learned_info": [_P, _u32p, _P, _P, C.POINTER(C.c_uint64)],
     "rs_learned_setup": [_P, _P], "rs_learned_level_lambda": [_P, _P],
     "rs_learned_level_matrix": [_P, C.c_uint32, _P, _P, _P],
@@ -648,6 +649,10 @@ class LearnedPreconditioner:
 
     def set_cycle(self, pre: int = 1, post: int = 1, coarse: int = COARSE_NETWORK) -> None:
         _check(_lib.rs_learned_set_cycle(self.h, pre, post, coarse))
+
+    def set_compressed(self, min_rows: int) -> None:
+        """Levels with >= min_rows rows smooth with the FP16 copy of D^-1 A (0: none)."""
+        _check(_lib.rs_learned_set_compressed(self.h, min_rows))
 
     def setup(self, A: SparseMatrix | None = None) -> None:
         _check(_lib.rs_learned_setup(self.h, (A or self.A).h))

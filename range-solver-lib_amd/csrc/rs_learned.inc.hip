@@ -115,22 +115,30 @@ __global__ void k_galerkin_gather(const double *__restrict__ fval, const uint32_
     }
 }
 
-// FP32 inverse diagonal (0 where the diagonal is absent or zero) and the block maxima of the
-// Gershgorin bound sum_j |a_ij| / |a_ii| of D^-1 A.
+// FP32 diagonal and inverse diagonal (0 where the diagonal is absent or zero) and the block
+// maxima of the Gershgorin bound sum_j |a_ij| / |a_ii| of D^-1 A; 16 lanes per row.
 __global__ __launch_bounds__(kBlock) void k_level_diag(const uint32_t *__restrict__ rowptr,
                                                        const uint32_t *__restrict__ col,
                                                        const double *__restrict__ val, uint32_t n,
-                                                       float *__restrict__ dinv, double *__restrict__ partials)
+                                                       float *__restrict__ diag, float *__restrict__ dinv,
+                                                       double *__restrict__ partials)
 {
+    constexpr int G = 16;
+    const uint32_t gl = threadIdx.x % G;
     double m = 0.0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (uint64_t i = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G; i < n; i += (uint64_t)gridDim.x * (kBlock / G)) {
         double d = 0.0, s = 0.0;
-        for (uint32_t j = rowptr[i]; j < rowptr[i + 1]; j++) {
-            double a = val[j];
+        for (uint32_t j = rowptr[i] + gl; j < rowptr[i + 1]; j += G) {
+            const double a = val[j];
             s += fabs(a);
             if (col[j] == i) d += a;
         }
-        dinv[i] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
+        d = group_sum<G>(d);
+        s = group_sum<G>(s);
+        if (gl == 0) {
+            diag[i] = (float)d;
+            dinv[i] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
+        }
         if (d != 0.0) m = fmax(m, s / fabs(d));
     }
     block_max_store(m, partials);
@@ -164,6 +172,28 @@ __global__ __launch_bounds__(kBlock) void k_restrict_total(const double *__restr
 }
 
 // ---------------------------------------------------------------------------- cycle kernels
+// Prolongation fused into the kernel that finishes a coarse correction: every finer row i of
+// node I gets zout[i] = zpre[i] + e_I (zpre nullptr: zero); mptr nullptr: nothing to do.
+struct Prolong {
+    const uint32_t *mptr, *mrows;    // finer rows of each node of this level
+    const float *zpre;
+    float *zout;
+    __device__ __forceinline__ void apply(uint32_t I, float e) const
+    {
+        if (!mptr) return;
+        for (uint32_t k = mptr[I]; k < mptr[I + 1]; k++) {
+            const uint32_t i = mrows[k];
+            zout[i] = (zpre ? zpre[i] : 0.0f) + e;
+        }
+    }
+};
+
+// The first Chebyshev step of the next level fused into the restriction: z = d = c2 D^-1 r.
+struct FirstStep {
+    const float *coef, *dinv;        // coef nullptr: nothing to do
+    float *z, *d;
+};
+
 struct CycArgs {
     CsrView A;                 // level matrix (FP32 values)
     const float *r;            // right-hand side of the level
@@ -174,6 +204,7 @@ struct CycArgs {
     const SolveState *st;      // nullptr outside a solve
     const float *q;            // finest level, last step: the flexible CG's previous A p
     double *partials;          // finest level, last step: partial r.z and z.q
+    Prolong up;                // last step of a coarse level: its correction into the finer level
     uint32_t n;
 };
 
@@ -198,6 +229,7 @@ struct EpiCheb {
     float *zout, *d;
     float c1, c2;
     bool dots;
+    Prolong up;
     double acc[2];
     __device__ void row(uint32_t i, float s)
     {
@@ -207,6 +239,7 @@ struct EpiCheb {
         d[i] = dd;
         const float z = zin[i] + dd;
         zout[i] = z;
+        up.apply(i, z);
         if (dots) {
             acc[0] += (double)ri * (double)z;
             acc[1] += (double)z * (double)q[i];
@@ -218,7 +251,7 @@ template <class Shape>
 __global__ __launch_bounds__(kBlock) void k_cyc_step(CycArgs a)
 {
     if (cyc_idle(a.st)) return;
-    EpiCheb e{ a.r, a.zin, a.dinv, a.q, a.zout, a.d, a.coef[0], a.coef[1], a.partials != nullptr, { 0.0, 0.0 } };
+    EpiCheb e{ a.r, a.zin, a.dinv, a.q, a.zout, a.d, a.coef[0], a.coef[1], a.partials != nullptr, a.up, { 0.0, 0.0 } };
     Shape::template run<float>(a.A, a.A.val32, a.zin, e);
     if (a.partials) store_partials<2>(e.acc, a.partials);
 }
@@ -237,42 +270,161 @@ __global__ __launch_bounds__(kBlock) void k_cyc_resid(CycArgs a)
     Shape::template run<float>(a.A, a.A.val32, a.zin, e);
 }
 
+// ---------------------------------------------------------------------------- compressed smoother copy
+// Row-scaled copy S = D^-1 A of a level matrix in its SELL-64 slots, FP16 values and int16
+// column offsets (col - row) packed into one uint32 per slot: 4 bytes per entry instead of 8 for
+// the FP32 SELL copy.  The smoother only needs D^-1 A (and D for the residual); the Krylov
+// operator stays FP32.  *bad is set when an offset does not fit in 16 bits or a diagonal is zero.
+__global__ void k_s16_build(CsrView A, uint32_t *__restrict__ ent, uint32_t *__restrict__ bad)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < (uint64_t)A.nslices * kWave;
+         r += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t s = (uint32_t)(r / kWave), lane = (uint32_t)(r % kWave);
+        const uint32_t base = A.sell_ptr[s], width = (A.sell_ptr[s + 1] - base) / kWave;
+        uint32_t b = 0, len = 0;
+        double d = 1.0;
+        if (r < A.n) {
+            b = A.rowptr[r];
+            len = A.rowptr[r + 1] - b;
+            d = 0.0;
+            for (uint32_t k = 0; k < len; k++)
+                if (A.col[b + k] == r) d += A.val[b + k];
+            if (d == 0.0) atomicOr(bad, 1u);
+        }
+        for (uint32_t j = 0; j < width; j++) {
+            uint32_t e = 0;
+            if (j < len) {
+                const int64_t off = (int64_t)A.col[b + j] - (int64_t)r;
+                if (off < -32768 || off > 32767) atomicOr(bad, 1u);
+                const _Float16 h = (_Float16)(d != 0.0 ? A.val[b + j] / d : 0.0);
+                e = (uint32_t)(uint16_t)(int16_t)off | ((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
+            }
+            ent[base + j * kWave + lane] = e;
+        }
+    }
+}
+
+struct S16View {
+    uint32_t n, nslices, spb;
+    const uint32_t *sell_ptr, *ent;
+};
+
+// s = S x for the lanes' rows (one row per lane, slices of 64), then epi.row(r, s).
+template <class Epi>
+__device__ __forceinline__ void s16_run(const S16View &S, const float *__restrict__ x, Epi &epi)
+{
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const uint32_t s0 = lb * S.spb, s1 = min(S.nslices, s0 + S.spb);
+    const uint32_t *__restrict__ ent = S.ent;
+    for (uint32_t s = s0 + wid; s < s1; s += kBlock / kWave) {
+        const uint32_t b = S.sell_ptr[s], e = S.sell_ptr[s + 1];
+        const uint32_t r = s * kWave + lane, rr = min(r, S.n - 1);   // padding lanes read row n-1
+        float acc = 0.0f;
+        uint32_t j = b + lane;
+        auto term = [&](uint32_t w) {
+            const int32_t off = (int16_t)(w & 0xffffu);
+            const float v = (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+            return v * x[(int32_t)rr + off];
+        };
+        for (; j + 3 * kWave < e; j += 4 * kWave) {
+            const uint32_t w0 = ent[j], w1 = ent[j + kWave], w2 = ent[j + 2 * kWave], w3 = ent[j + 3 * kWave];
+            acc += term(w0);
+            acc += term(w1);
+            acc += term(w2);
+            acc += term(w3);
+        }
+        for (; j < e; j += kWave) acc += term(ent[j]);
+        if (r < S.n) epi.row(r, acc);
+    }
+}
+
+// res = r - D (S z)
+struct EpiResidS16 {
+    const float *r, *diag;
+    float *res;
+    __device__ void row(uint32_t i, float s) { res[i] = r[i] - diag[i] * s; }
+};
+
+// d = c1 d + c2 (D^-1 r - S z_in), z_out = z_in + d; optional partial r.z_out, z_out.q
+struct EpiChebS16 {
+    const float *r, *zin, *dinv, *q;
+    float *zout, *d;
+    float c1, c2;
+    bool dots;
+    Prolong up;
+    double acc[2];
+    __device__ void row(uint32_t i, float s)
+    {
+        const float ri = r[i];
+        float dd = c2 * (dinv[i] * ri - s);
+        if (c1 != 0.0f) dd = fmaf(c1, d[i], dd);
+        d[i] = dd;
+        const float z = zin[i] + dd;
+        zout[i] = z;
+        up.apply(i, z);
+        if (dots) {
+            acc[0] += (double)ri * (double)z;
+            acc[1] += (double)z * (double)q[i];
+        }
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void k_cyc_resid_s16(S16View S, CycArgs a, const float *__restrict__ diag)
+{
+    if (cyc_idle(a.st)) return;
+    EpiResidS16 e{ a.r, diag, a.res };
+    s16_run(S, a.zin, e);
+}
+
+__global__ __launch_bounds__(kBlock) void k_cyc_step_s16(S16View S, CycArgs a)
+{
+    if (cyc_idle(a.st)) return;
+    EpiChebS16 e{ a.r, a.zin, a.dinv, a.q, a.zout, a.d, a.coef[0], a.coef[1], a.partials != nullptr, a.up, { 0.0, 0.0 } };
+    s16_run(S, a.zin, e);
+    if (a.partials) store_partials<2>(e.acc, a.partials);
+}
+
 // r_next[I] = sum of res over the members of I (G lanes per aggregate, FP64 sum, fixed order).
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_cyc_restrict(const float *__restrict__ res, const uint32_t *__restrict__ mptr,
                                                          const uint32_t *__restrict__ mrows, uint32_t nnext,
-                                                         float *__restrict__ rnext, const SolveState *st)
+                                                         float *__restrict__ rnext, const SolveState *st, FirstStep f)
 {
     if (cyc_idle(st)) return;
     const uint32_t gl = threadIdx.x % G;
     const uint64_t stride = (uint64_t)gridDim.x * (kBlock / G);
+    const float c2 = f.coef ? f.coef[1] : 0.0f;
     for (uint64_t I = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G; I < nnext; I += stride) {
         double s = 0.0;
         for (uint32_t k = mptr[I] + gl; k < mptr[I + 1]; k += G) s += (double)res[mrows[k]];
         s = group_sum<G>(s);
-        if (gl == 0) rnext[I] = (float)s;
+        if (gl == 0) {
+            const float rv = (float)s;
+            rnext[I] = rv;
+            if (f.coef) {
+                const float z = c2 * f.dinv[I] * rv;
+                f.z[I] = z;
+                f.d[I] = z;
+            }
+        }
     }
 }
 
-// z_out = z_in + e[agg] (z_in nullptr: zero).
-__global__ void k_cyc_prolong(const float *__restrict__ zin, const float *__restrict__ e, const uint32_t *__restrict__ agg,
-                              float *__restrict__ zout, uint32_t n, const SolveState *st)
-{
-    if (cyc_idle(st)) return;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
-        zout[i] = (zin ? zin[i] : 0.0f) + e[agg[i]];
-}
-
 __global__ void k_coarse_jacobi(const float *__restrict__ r, const float *__restrict__ dinv, float *__restrict__ z,
-                                uint32_t n, const SolveState *st)
+                                uint32_t n, const SolveState *st, Prolong up)
 {
     if (cyc_idle(st)) return;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) z[i] = dinv[i] * r[i];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float v = dinv[i] * r[i];
+        z[i] = v;
+        up.apply(i, v);
+    }
 }
 
 // z = C r with the dense FP32 inverse C (row-major n x n), one wavefront per row.
 __global__ __launch_bounds__(kBlock) void k_coarse_exact(const float *__restrict__ cinv, const float *__restrict__ r,
-                                                         float *__restrict__ z, uint32_t n, const SolveState *st)
+                                                         float *__restrict__ z, uint32_t n, const SolveState *st, Prolong up)
 {
     if (cyc_idle(st)) return;
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -280,7 +432,10 @@ __global__ __launch_bounds__(kBlock) void k_coarse_exact(const float *__restrict
         float s = 0.0f;
         for (uint32_t j = lane; j < n; j += kWave) s += cinv[(uint64_t)row * n + j] * r[j];
         s = group_sum<kWave>(s);
-        if (lane == 0) z[row] = s;
+        if (lane == 0) {
+            z[row] = s;
+            up.apply(row, s);
+        }
     }
 }
 
@@ -312,6 +467,7 @@ struct GnnArgs {
     float *norm;                     // [sigma, mean log diag]
     float *e;
     const SolveState *st;
+    Prolong up;                      // the correction into the finer level (last layer)
 };
 
 __device__ __forceinline__ bf16x4 bf16x4_of(uint2 w) { return __builtin_bit_cast(bf16x4, w); }
@@ -441,7 +597,9 @@ __global__ __launch_bounds__(4 * kWave) void k_gnn_layer(GnnArgs g, int l)
             const uint32_t row = r0 + threadIdx.x;
             if (row < nc) {
                 const float o = ((sO[0][threadIdx.x] + sO[1][threadIdx.x]) + (sO[2][threadIdx.x] + sO[3][threadIdx.x])) + P[kOffBo];
-                g.e[row] = g.u[row] + g.norm[0] * o;
+                const float ev = g.u[row] + g.norm[0] * o;
+                g.e[row] = ev;
+                g.up.apply(row, ev);
             }
         }
     }
@@ -452,6 +610,11 @@ static void launch_gnn(const GnnArgs &g, hipStream_t s)
     const uint32_t strips = g.npad / 16;
     k_gnn_embed<<<strips, kWave, 0, s>>>(g);
     for (int l = 0; l < kLayers; l++) k_gnn_layer<<<strips, 4 * kWave, 0, s>>>(g, l);
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ a, uint64_t n, uint32_t v)
+{
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (uint64_t)gridDim.x * kBlock) a[k] = v;
 }
 
 // Graph blocks from the coarsest CSR weights: bval[bidx[p]] = bf16(cw[p]) (bval zeroed first).
@@ -793,7 +956,9 @@ struct CycLevel {
     Matrix *M = nullptr;                 // level 0: the solver's matrix; coarser levels: own
     rs_matrix *own = nullptr;
     uint32_t n = 0;
-    DBuf<float> dinv, r, za, zb, d, res;
+    DBuf<float> diag, dinv, r, za, zb, d, res;
+    DBuf<uint32_t> s16;                  // compressed smoother copy (SELL-64 slots), when used
+    bool use_s16 = false;
     // to the next level
     uint32_t nnext = 0, rgroup = 1;
     DBuf<uint32_t> agg, mptr, mrows, gptr, gsrc;
@@ -822,6 +987,8 @@ struct rs_learned {
     uint32_t pre = 1, post = 1;
     int coarse = RS_COARSE_NETWORK;
     uint32_t rec_cap = 0;
+    uint32_t s16_min_rows = 65536;                   // levels at least this large smooth with the FP16 copy
+    rs::DBuf<uint32_t> s16bad;
     uint64_t setup_version = ~0ull;
     bool inverse_valid = false;
     std::vector<double> lam;                         // Gershgorin bound per level
@@ -917,7 +1084,8 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     for (uint32_t l = 0; l < L.nlev; l++) {
         CycLevel &V = *L.lv[l];
         uint32_t nb = std::min(vec_blocks(std::max(V.n, 1u)), kParts);
-        k_level_diag<<<nb, kBlock, 0, st>>>(V.M->rowptr.p, V.M->col.p, V.M->val.p, V.n, V.dinv.p, L.part.p + (uint64_t)l * kParts);
+        k_level_diag<<<nb, kBlock, 0, st>>>(V.M->rowptr.p, V.M->col.p, V.M->val.p, V.n, V.diag.p, V.dinv.p,
+                                            L.part.p + (uint64_t)l * kParts);
         k_max_fin<<<1, 64, 0, st>>>(L.part.p + (uint64_t)l * kParts, nb, L.gersh.p + l);
     }
     const Matrix &C = *L.lv.back()->M;
@@ -925,9 +1093,26 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     k_fill_u16<<<vec_blocks(L.gbval.n), kBlock, 0, st>>>(L.gbval.p, L.gbval.n);
     k_gnn_blocks<<<vec_blocks(std::max<uint64_t>(C.nnz, 1)), kBlock, 0, st>>>(L.cw.p, L.gbidx.p, C.nnz, L.gbval.p);
     RS_HIP(hipGetLastError());
+    // compressed smoother copies of the large levels
+    std::vector<uint32_t> bad(L.nlev, 1u);
+    k_fill_u32<<<1, 64, 0, st>>>(L.s16bad.p, L.nlev, 0u);
+    for (uint32_t l = 0; l + 1 < L.nlev; l++) {
+        CycLevel &V = *L.lv[l];
+        if (L.s16_min_rows == 0 || V.n < L.s16_min_rows || V.M->sell_col.n == 0) continue;
+        if ((rc = V.s16.alloc(V.M->sell_col.n))) return rc;
+        uint32_t nb = 0;
+        const CsrView sv = plan_view(*V.M, kSellPlan, nb);
+        k_s16_build<<<vec_blocks((uint64_t)V.M->nslices * kWave), kBlock, 0, st>>>(sv, V.s16.p, L.s16bad.p + l);
+    }
+    RS_HIP(hipGetLastError());
     L.lam.assign(L.nlev, 0.0);
     RS_HIP(hipMemcpyAsync(L.lam.data(), L.gersh.p, sizeof(double) * L.nlev, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipMemcpyAsync(bad.data(), L.s16bad.p, sizeof(uint32_t) * L.nlev, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
+    for (uint32_t l = 0; l < L.nlev; l++) {
+        CycLevel &V = *L.lv[l];
+        V.use_s16 = l + 1 < L.nlev && L.s16_min_rows != 0 && V.n >= L.s16_min_rows && V.s16.p && bad[l] == 0;
+    }
     std::vector<float> coef((size_t)L.nlev * kMaxDeg * 2);
     for (uint32_t l = 0; l < L.nlev; l++) cheb_coefs(L.lam[l] > 0.0 ? L.lam[l] : 1.0, &coef[(size_t)l * kMaxDeg * 2]);
     RS_HIP(hipMemcpyAsync(L.coef.p, coef.data(), sizeof(float) * coef.size(), hipMemcpyHostToDevice, st));
@@ -983,30 +1168,54 @@ static GnnArgs gnn_args(rs_learned &L, const SolveState *st)
     return g;
 }
 
-static void launch_restrict(uint32_t G, uint32_t nb, const float *res, const CycLevel &V, float *rnext, const SolveState *st)
+static void launch_restrict(uint32_t G, uint32_t nb, const float *res, const CycLevel &V, float *rnext, const SolveState *st,
+                            FirstStep f)
 {
     hipStream_t s = runtime().stream;
     switch (G) {
-        case 1: k_cyc_restrict<1><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st); break;
-        case 4: k_cyc_restrict<4><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st); break;
-        case 16: k_cyc_restrict<16><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st); break;
-        default: k_cyc_restrict<64><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st); break;
+        case 1: k_cyc_restrict<1><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st, f); break;
+        case 4: k_cyc_restrict<4><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st, f); break;
+        case 16: k_cyc_restrict<16><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st, f); break;
+        default: k_cyc_restrict<64><<<nb, kBlock, 0, s>>>(res, V.mptr.p, V.mrows.p, V.nnext, rnext, st, f); break;
     }
 }
 
-// Launches the V-cycle from level l; returns the buffer that holds the level's correction.
-// On the finest level with q and partials given, the last step also leaves the partial sums
-// r.z and z.q for the flexible CG.
-static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st, const float *q, double *partials)
+// One Chebyshev step on level V (SpMV in the level's compressed copy or its FP32 plan).
+static void launch_step(const CycLevel &V, const CycArgs &a, hipStream_t s)
+{
+    uint32_t nb = 0;
+    if (V.use_s16) {
+        const CsrView v16 = plan_view(*V.M, kSellPlan, nb);
+        k_cyc_step_s16<<<nb, kBlock, 0, s>>>(S16View{ V.n, v16.nslices, v16.spb, v16.sell_ptr, V.s16.p }, a);
+    } else {
+        const int plan = V.M->plan[1];
+        plan_view(*V.M, plan, nb);
+        RS_SHAPE_SWITCH(plan, k_cyc_step, nb, s, a);
+    }
+}
+
+// Launches the V-cycle from level l.  `up` says where the level's finished correction goes
+// (the finer level's post-smoothing input, prolongation fused); the finest level returns its
+// result buffer.  On the finest level with q and partials given, the last step also leaves the
+// partial sums r.z and z.q for the flexible CG.  Levels below the finest receive their first
+// pre-smoothing step from the restriction kernel of the level above.
+static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st, const float *q, double *partials,
+                                Prolong up)
 {
     CycLevel &V = *L.lv[l];
     hipStream_t s = runtime().stream;
     const uint32_t nvb = vec_blocks(std::max(V.n, 1u));
     const float *r = l == 0 ? L.r0 : V.r.p;
     if (l + 1 == L.nlev) {
-        if (L.coarse == RS_COARSE_NETWORK) launch_gnn(gnn_args(L, st), s);
-        else if (L.coarse == RS_COARSE_EXACT) k_coarse_exact<<<vec_blocks((uint64_t)V.n * kWave), kBlock, 0, s>>>(L.cinv.p, r, V.za.p, V.n, st);
-        else k_coarse_jacobi<<<nvb, kBlock, 0, s>>>(r, V.dinv.p, V.za.p, V.n, st);
+        if (L.coarse == RS_COARSE_NETWORK) {
+            GnnArgs g = gnn_args(L, st);
+            g.up = up;
+            launch_gnn(g, s);
+        } else if (L.coarse == RS_COARSE_EXACT) {
+            k_coarse_exact<<<vec_blocks((uint64_t)V.n * kWave), kBlock, 0, s>>>(L.cinv.p, r, V.za.p, V.n, st, up);
+        } else {
+            k_coarse_jacobi<<<nvb, kBlock, 0, s>>>(r, V.dinv.p, V.za.p, V.n, st, up);
+        }
         return V.za.p;
     }
     CycLevel &C = *L.lv[l + 1];
@@ -1026,27 +1235,37 @@ static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st,
     for (uint32_t k = 0; k < L.pre; k++) {
         a.coef = coef + 2 * k;
         if (k == 0) {
-            a.zout = buf[0];
-            k_cyc_first<<<nvb, kBlock, 0, s>>>(a);
+            if (l == 0) {
+                a.zout = buf[0];
+                k_cyc_first<<<nvb, kBlock, 0, s>>>(a);
+            }
             cur = 0;
         } else {
             a.zin = buf[cur];
             a.zout = buf[cur ^ 1];
-            RS_SHAPE_SWITCH(plan, k_cyc_step, nb, s, a);
+            launch_step(V, a, s);
             cur ^= 1;
         }
     }
     const float *rres = r;
     if (cur >= 0) {
         a.zin = buf[cur];
-        RS_SHAPE_SWITCH(plan, k_cyc_resid, nb, s, a);
+        if (V.use_s16) {
+            uint32_t nbs = 0;
+            const CsrView v16 = plan_view(*V.M, kSellPlan, nbs);
+            k_cyc_resid_s16<<<nbs, kBlock, 0, s>>>(S16View{ V.n, v16.nslices, v16.spb, v16.sell_ptr, V.s16.p }, a, V.diag.p);
+        } else {
+            RS_SHAPE_SWITCH(plan, k_cyc_resid, nb, s, a);
+        }
         rres = V.res.p;
     }
+    FirstStep fs{};
+    if (L.pre >= 1 && l + 2 < L.nlev) fs = FirstStep{ L.coef.p + (size_t)(l + 1) * kMaxDeg * 2, C.dinv.p, C.za.p, C.d.p };
     const uint32_t G = V.rgroup;
-    launch_restrict(G, vec_blocks((uint64_t)V.nnext * G), rres, V, C.r.p, st);
-    const float *e = cycle_level(L, l + 1, st, nullptr, nullptr);
+    launch_restrict(G, vec_blocks((uint64_t)V.nnext * G), rres, V, C.r.p, st, fs);
+    // the coarser level leaves z_pre + P e in buf[nxt]
     const int nxt = cur >= 0 ? cur ^ 1 : 0;
-    k_cyc_prolong<<<nvb, kBlock, 0, s>>>(cur >= 0 ? buf[cur] : nullptr, e, V.agg.p, buf[nxt], V.n, st);
+    cycle_level(L, l + 1, st, nullptr, nullptr, Prolong{ V.mptr.p, V.mrows.p, cur >= 0 ? buf[cur] : nullptr, buf[nxt] });
     cur = nxt;
     for (uint32_t k = 0; k < L.post; k++) {
         const bool last = k + 1 == L.post;
@@ -1055,7 +1274,8 @@ static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st,
         a.zout = buf[cur ^ 1];
         a.q = last ? q : nullptr;
         a.partials = last ? partials : nullptr;
-        RS_SHAPE_SWITCH(plan, k_cyc_step, nb, s, a);
+        a.up = last ? up : Prolong{};
+        launch_step(V, a, s);
         cur ^= 1;
     }
     return buf[cur];
@@ -1070,6 +1290,7 @@ static std::string learned_key(const rs_learned &L)
         k.push_back((uint64_t)M.plan[1]);
         k.push_back((uint64_t)(uintptr_t)M.val32.p);
         k.push_back((uint64_t)(uintptr_t)M.sell_val32.p);
+        k.push_back(L.lv[l]->use_s16 ? (uint64_t)(uintptr_t)L.lv[l]->s16.p : 0ull);
     }
     k.push_back(L.pre);
     k.push_back(L.post);
@@ -1129,7 +1350,7 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
     RS_HIP(hipGetLastError());
     rc = capture(S, [&] {
         for (uint32_t i = 0; i < kGraphIters; i++) {
-            const float *z = cycle_level(L, 0, S.st.p, S.q32.p, S.partials.p);
+            const float *z = cycle_level(L, 0, S.st.p, S.q32.p, S.partials.p, Prolong{});
             if (L.rec_cap) {
                 CycLevel &Cl = *L.lv.back();
                 k_record<<<vec_blocks(L.nc), kBlock, 0, stream>>>(Cl.r.p, L.rec.p, L.nc, L.rec_cap, S.st.p);
@@ -1396,7 +1617,7 @@ extern "C" int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const ui
     for (auto &lp : L->lv) {
         CycLevel &V = *lp;
         const size_t n = std::max(V.n, 1u);
-        if ((rc = V.dinv.alloc(n)) || (rc = V.r.alloc(n)) || (rc = V.za.alloc(n)) || (rc = V.zb.alloc(n)) ||
+        if ((rc = V.diag.alloc(n)) || (rc = V.dinv.alloc(n)) || (rc = V.r.alloc(n)) || (rc = V.za.alloc(n)) || (rc = V.zb.alloc(n)) ||
             (rc = V.d.alloc(n)) || (rc = V.res.alloc(n)))
             return rc;
     }
@@ -1432,7 +1653,7 @@ extern "C" int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const ui
     if ((rc = R.gbptr.alloc(gbptr.size())) || (rc = R.gbcol.alloc(std::max<size_t>(gbcol.size(), 1))) ||
         (rc = R.gbidx.alloc(std::max<size_t>(gbidx.size(), 1))) || (rc = R.gbval.alloc(std::max<size_t>(gbcol.size(), 1) * 256)) ||
         (rc = R.gh.alloc(2ull * nstrips * 16 * kHid)) || (rc = R.ght.alloc(2ull * nstrips * 16 * kHid)) ||
-        (rc = R.gu.alloc(nstrips * 16)) || (rc = R.gnorm.alloc(2)))
+        (rc = R.gu.alloc(nstrips * 16)) || (rc = R.gnorm.alloc(2)) || (rc = R.s16bad.alloc(R.nlev)))
         return rc;
     RS_HIP(hipMemcpyAsync(R.gbptr.p, gbptr.data(), sizeof(uint32_t) * gbptr.size(), hipMemcpyHostToDevice, st));
     if (!gbcol.empty())
@@ -1477,6 +1698,14 @@ extern "C" int rs_learned_set_cycle(rs_learned *L, uint32_t pre, uint32_t post, 
     L->pre = pre;
     L->post = post;
     L->coarse = coarse_mode;
+    return RS_OK;
+}
+
+extern "C" int rs_learned_set_compressed(rs_learned *L, uint32_t min_rows)
+{
+    if (!L) return fail(RS_ERR_INVALID_ARG, "rs_learned_set_compressed: null");
+    L->s16_min_rows = min_rows;
+    L->setup_version = ~0ull;   // rebuilt at the next set-up
     return RS_OK;
 }
 
@@ -1602,7 +1831,7 @@ extern "C" int rs_learned_apply(rs_learned *L, rs_matrix *A, const float *r_host
     CycLevel &V = *L->lv[0];
     RS_HIP(hipMemcpyAsync(V.r.p, r_host, sizeof(float) * L->n, hipMemcpyHostToDevice, st));
     L->r0 = V.r.p;
-    const float *z = cycle_level(*L, 0, nullptr, nullptr, nullptr);
+    const float *z = cycle_level(*L, 0, nullptr, nullptr, nullptr, Prolong{});
     RS_HIP(hipGetLastError());
     RS_HIP(hipMemcpyAsync(z_host, z, sizeof(float) * L->n, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));

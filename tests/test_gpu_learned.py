@@ -128,24 +128,28 @@ def _coarse_fn(mode):
     return lo.coarse_exact if mode == "exact" else lo.coarse_jacobi
 
 
-@pytest.mark.parametrize("pre,post,mode", [(1, 1, "jacobi"), (2, 2, "exact"), (0, 1, "exact"), (3, 2, "jacobi")])
-def test_vcycle_matches_oracle(gpu, pre, post, mode):
+@pytest.mark.parametrize("pre,post,mode,fp16", [(1, 1, "jacobi", False), (2, 2, "exact", False),
+                                                (0, 1, "exact", False), (3, 2, "jacobi", False),
+                                                (1, 1, "jacobi", True), (2, 1, "exact", True)])
+def test_vcycle_matches_oracle(gpu, pre, post, mode, fp16):
     rs = gpu
-    _say(f"vcycle pre={pre} post={post} coarse={mode}")
+    _say(f"vcycle pre={pre} post={post} coarse={mode} fp16-smoother={fp16}")
     asm, _, xyz = _heat(rs, 20, 2.0)
     maps = rs.hierarchy_by_bins(xyz, asm.book(), 1, (10, 10, 10), max_coarse=200)
     L = rs.LearnedPreconditioner(asm.matrix, maps)
     L.set_cycle(pre, post, rs.COARSE_EXACT if mode == "exact" else rs.COARSE_JACOBI)
+    L.set_compressed(1 if fp16 else 0)
     r = np.random.default_rng(pre * 10 + post).standard_normal(asm.n_dof).astype(np.float32)
     z = L.apply(r)
     levels = _levels_from_gpu(L)
     _, Ps = lo.hierarchy(asm.matrix.to_scipy(), maps)
     lams = L.level_lambda()
-    ref = lo.vcycle(levels, Ps, r.astype(np.float64), pre, post, _coarse_fn(mode), lams)
+    fp16_levels = tuple(range(len(levels) - 1)) if fp16 else ()
+    ref = lo.vcycle(levels, Ps, r.astype(np.float64), pre, post, _coarse_fn(mode), lams, fp16_levels=fp16_levels)
     assert np.all(np.isfinite(z))
     assert _rel(z, ref) < CYC_RTOL, _rel(z, ref)
     # a planted mistake (one post-smoothing step more) is far outside the tolerance
-    bad = lo.vcycle(levels, Ps, r.astype(np.float64), pre, post + 1, _coarse_fn(mode), lams)
+    bad = lo.vcycle(levels, Ps, r.astype(np.float64), pre, post + 1, _coarse_fn(mode), lams, fp16_levels=fp16_levels)
     assert _rel(bad, ref) > 100 * CYC_RTOL
 
 
