@@ -108,6 +108,7 @@ _SIGS = {
     "rs_assemble_heat_device": [_P, _P, _P, _P, _P, C.c_int32, C.c_double, C.c_double, _P],
     "rs_assemble_stress": [_P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P],
     "rs_assembly_dofs_to_nodes": [_P, _P, _P], "rs_assembly_destroy": [_P],
+    "rs_assembly_set_method": [_P, C.c_int32],
     "rs_aggregates_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                               _P, _u32p],
     "rs_hierarchy_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -495,6 +496,10 @@ class Assembly:
         mh = C.c_void_p()
         _check(_lib.rs_assembly_matrix(self.h, C.byref(mh)))
         self.matrix = SparseMatrix(0, None, None, None, _handle=mh, _owner=self)
+
+    def set_method(self, row_gather: bool) -> None:
+        """Heat assembly kernel: row gather (default) or one launch per colour."""
+        _check(_lib.rs_assembly_set_method(self.h, 1 if row_gather else 0))
 
     def book(self) -> np.ndarray:
         out = np.empty(self.nn * self.dpn, np.uint32)

@@ -22,6 +22,7 @@
 // vertex coordinates staged in LDS.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -134,6 +135,78 @@ __global__ __launch_bounds__(kBlock) void k_heat_colour(HeatArgs a, uint32_t e0,
         for (int n = 0; n < 4; n++)
             if (pp[n] != RS_EOD) a.val[pp[n]] += Ae[m][n];
     }
+}
+
+// Row-gather variant: one thread per free DOF row sums the contributions of the elements
+// incident to its node, in colour-sorted element order -- the order in which the colour
+// launches add into that row, so the values are bit-identical to k_heat_colour's -- and writes
+// the row once.  Per row ~24 elements are read (each element by its 4 nodes, mostly from L2)
+// instead of a scattered read-modify-write of 16 values per element.
+constexpr int kRowCap = 16;
+
+struct HeatRowArgs {
+    HeatArgs h;
+    const uint32_t *rowptr;
+    const uint32_t *dof_node;   // row -> node
+    const uint32_t *inc_ptr;    // node -> [inc_ptr[v], inc_ptr[v+1]) in inc
+    const uint32_t *inc;        // colour-sorted element index << 2 | local node index
+    uint32_t n_rows;
+};
+
+__global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
+{
+    __shared__ double acc[kRowCap][kBlock];
+    const uint32_t t = threadIdx.x;
+    const uint32_t i = blockIdx.x * kBlock + t;
+    if (i >= a.n_rows) return;
+    const HeatArgs &h = a.h;
+    const uint32_t r0 = a.rowptr[i], len = a.rowptr[i + 1] - r0;
+#pragma unroll
+    for (int j = 0; j < kRowCap; j++) acc[j][t] = 0.0;
+    double bi = 0.0;
+    const uint32_t v = a.dof_node[i];
+    for (uint32_t k = a.inc_ptr[v]; k < a.inc_ptr[v + 1]; k++) {
+        const uint32_t w = a.inc[k], e = w >> 2, m = w & 3u;
+        const uint4 nd4 = h.conn[e];
+        const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+        const uint32_t orig = h.perm[e];
+        double X[4][3], g[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) X[q][c] = h.xyz[3 * (uint64_t)nd[q] + c];
+        const double V = tet_gradients(X, g);
+        const double kV = h.k[orig] * V;
+        const double fq = h.q[orig] * V * 0.25;
+        double Ae[4], be;
+        if (h.transient) {
+            const double mc = h.rhoc[orig] * V / 20.0;
+            be = h.dt * fq;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const double K = kV * (g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]);
+                const double M = mc * ((uint32_t)n == m ? 2.0 : 1.0);
+                Ae[n] = M + h.alpha * h.dt * K;
+                be += (M - (1.0 - h.alpha) * h.dt * K) * h.t_node[nd[n]];
+            }
+        } else {
+            be = fq;
+#pragma unroll
+            for (int n = 0; n < 4; n++) Ae[n] = kV * (g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]);
+        }
+        // Dirichlet lifting (src/rsolverheat.cpp:787-798), in the colour kernel's order
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+            if (h.book[nd[n]] == RS_EOD) be -= Ae[n] * h.t_node[nd[n]];
+        bi += be;
+        const uint4 pos = reinterpret_cast<const uint4 *>(h.map + 16ull * e)[m];
+        const uint32_t pp[4] = { pos.x, pos.y, pos.z, pos.w };
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+            if (pp[n] != RS_EOD) acc[pp[n] - r0][t] += Ae[n];
+    }
+    for (uint32_t j = 0; j < len; j++) h.val[r0 + j] = acc[j][t];
+    h.b[i] = bi;
 }
 
 struct StressArgs {
@@ -267,6 +340,9 @@ struct rs_assembly {
     std::vector<uint32_t> colour_off;     // n_colours+1
     DBuf<double> xyz;
     DBuf<uint32_t> conn, perm, map, dbook;
+    DBuf<uint32_t> dof_node, inc_ptr, inc;  // row-gather heat assembly
+    bool rows = false;                      // every row fits kRowCap entries
+    bool use_rows = true;                   // heat: row-gather kernel (else the colour launches)
     rs_matrix A;                          // borrowed by callers
     DBuf<double> fields, t_node, bvec;    // staging for the host-array entry points
     DBuf<uint32_t> tris;
@@ -354,6 +430,24 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
             }
         }
     });
+    // node -> incident elements (colour-sorted index, local node) for the row-gather heat kernel
+    std::vector<uint32_t> inc_ptr, inc, dof_node;
+    uint32_t max_row = 0;
+    for (uint32_t r = 0; r < S->n_dof; r++) max_row = std::max(max_row, pat.rowptr[r + 1] - pat.rowptr[r]);
+    S->rows = dpn == 1 && max_row <= (uint32_t)kRowCap && n_tets < (1u << 30);
+    if (S->rows) {
+        inc_ptr.assign(n_nodes + 1ull, 0);
+        for (uint64_t s = 0; s < n_tets; s++)
+            for (int m = 0; m < 4; m++) inc_ptr[conn_s[4 * s + m] + 1]++;
+        for (uint32_t v = 0; v < n_nodes; v++) inc_ptr[v + 1] += inc_ptr[v];
+        inc.resize(4ull * n_tets);
+        std::vector<uint32_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (uint64_t s = 0; s < n_tets; s++)
+            for (uint32_t m = 0; m < 4; m++) inc[fill[conn_s[4 * s + m]]++] = (uint32_t)(s << 2) | m;
+        dof_node.assign(std::max(S->n_dof, 1u), 0);
+        for (uint32_t v = 0; v < n_nodes; v++)
+            if (book[v] != RS_EOD) dof_node[book[v]] = v;
+    }
     hipStream_t st = runtime().stream;
     uint64_t nnz = pat.col.size();
     if ((rc = S->xyz.alloc(3ull * n_nodes)) || (rc = S->conn.alloc(4ull * n_tets)) || (rc = S->perm.alloc(n_tets)) ||
@@ -374,9 +468,25 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
         RS_HIP(hipMemcpyAsync(A.col.p, pat.col.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, st));
         k_zero<<<vblocks(nnz), kBlock, 0, st>>>(A.val.p, nnz);
     }
+    if (S->rows) {
+        if ((rc = S->inc_ptr.alloc(inc_ptr.size())) || (rc = S->inc.alloc(std::max<size_t>(inc.size(), 1))) ||
+            (rc = S->dof_node.alloc(dof_node.size())))
+            return rc;
+        RS_HIP(hipMemcpyAsync(S->inc_ptr.p, inc_ptr.data(), sizeof(uint32_t) * inc_ptr.size(), hipMemcpyHostToDevice, st));
+        if (!inc.empty()) RS_HIP(hipMemcpyAsync(S->inc.p, inc.data(), sizeof(uint32_t) * inc.size(), hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(S->dof_node.p, dof_node.data(), sizeof(uint32_t) * dof_node.size(), hipMemcpyHostToDevice, st));
+    }
     if ((rc = matrix_set_layout(A, pat.rowptr.data()))) return rc;
     RS_HIP(hipStreamSynchronize(st));
     *out = S.release();
+    return RS_OK;
+}
+
+// Heat assembly method: 1 = row gather (default; needs rows of <= 16 entries), 0 = colour launches.
+extern "C" int rs_assembly_set_method(rs_assembly *S, int32_t row_gather)
+{
+    if (!S) return fail(RS_ERR_INVALID_ARG, "rs_assembly_set_method: null");
+    S->use_rows = row_gather != 0;
     return RS_OK;
 }
 
@@ -436,11 +546,16 @@ extern "C" int rs_assemble_heat_device(rs_assembly *S, const double *d_k, const 
     a.transient = transient;
     a.alpha = alpha;
     a.dt = dt;
-    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
-    if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
-    for (uint32_t c = 0; c < S->n_colours; c++) {
-        uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
-        if (e1 > e0) k_heat_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+    if (S->rows && S->use_rows) {
+        HeatRowArgs ra{ a, S->A.rowptr.p, S->dof_node.p, S->inc_ptr.p, S->inc.p, S->n_dof };
+        if (S->n_dof) k_heat_rows<<<grid_for(S->n_dof, kBlock), kBlock, 0, st>>>(ra);
+    } else {
+        if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+        if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
+        for (uint32_t c = 0; c < S->n_colours; c++) {
+            uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+            if (e1 > e0) k_heat_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+        }
     }
     RS_HIP(hipGetLastError());
     if ((rc = timed_end(ev, S->last_ms))) return rc;

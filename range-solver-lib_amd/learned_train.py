@@ -85,9 +85,11 @@ def coarse_pairs(L, count: int):
     return C, R, S
 
 
-def generate(rs, fam: HeatFamily, seeds, sigma: float, record: int, log_every: int = 50):
-    L = rs.LearnedPreconditioner(fam.asm.matrix, fam.maps, params=rs.init_params(0))
-    L.set_cycle(1, 1, rs.COARSE_JACOBI)
+def generate(rs, fam: HeatFamily, seeds, sigma: float, record: int, log_every: int = 50, policy=None):
+    """Pairs from the solver's runs with the coarsest level solved by coarse Jacobi, or by the
+    network with parameters `policy` (the trajectories a trained network actually sees)."""
+    L = rs.LearnedPreconditioner(fam.asm.matrix, fam.maps, params=rs.init_params(0) if policy is None else policy)
+    L.set_cycle(1, 1, rs.COARSE_JACOBI if policy is None else rs.COARSE_NETWORK)
     L.record(record)
     data, its = [], []
     t0 = time.time()
@@ -136,18 +138,34 @@ def batches_once(data, batch: int):
 
 
 def train(rs, cells=64, sigma=2.5, problems=2000, seed0=10000, holdout=0.05, record=64, batch=64,
-          steps=4000, lr=1e-3, out=WEIGHTS, init_seed=0, log_every=200):
+          steps=4000, lr=1e-3, out=WEIGHTS, init_seed=0, log_every=200, init=None, rounds=1):
+    """rounds > 1: after each round the pairs are regenerated with the current network solving
+    the coarsest level (on-policy data) and training continues from the current parameters."""
     say(f"family {cells}^3 sigma={sigma}: mesh, assembly set-up, hierarchy")
     fam = HeatFamily(rs, cells)
     say(f"unknowns {fam.n}, levels {[m.size for m in fam.maps]} -> {int(fam.maps[-1].max()) + 1}")
     seeds = list(range(seed0, seed0 + problems))
     n_hold = max(1, int(round(holdout * problems)))
-    L, cptr, ccol, data = generate(rs, fam, seeds, sigma, record)
+    theta = rs.init_params(init_seed) if init is None else np.asarray(init, np.float32)
+    for rnd in range(rounds):
+        policy = None if (init is None and rnd == 0) else theta
+        say(f"round {rnd + 1}/{rounds}: pairs from {'coarse Jacobi' if policy is None else 'the network'} runs")
+        theta, summary = _train_round(rs, fam, seeds, n_hold, sigma, record, batch, steps, lr, out, init_seed,
+                                      log_every, theta, policy)
+        summary.update({"cells": cells, "sigma": sigma, "problems": problems, "seed0": seed0, "round": rnd + 1,
+                        "rounds": rounds})
+        if out:
+            np.savez(out, params=theta.astype(np.float32), summary=json.dumps(summary))
+            say(f"wrote {out}")
+    return theta, summary
+
+
+def _train_round(rs, fam, seeds, n_hold, sigma, record, batch, steps, lr, out, init_seed, log_every, theta0, policy):
+    L, cptr, ccol, data = generate(rs, fam, seeds, sigma, record, policy=policy)
     train_data, hold_data = data[:-n_hold], data[-n_hold:]
-    theta0 = rs.init_params(init_seed)
     T = rs.Trainer(theta0, max_batch=batch, max_nodes=int(cptr.size - 1), max_nnz=int(ccol.size))
     base = evaluate(T, cptr, ccol, hold_data, batch)
-    say(f"held-out loss before training (coarse Jacobi): {base:.4f}")
+    say(f"held-out loss before this round: {base:.4f}")
     rng = np.random.default_rng(init_seed)
     gen = batches(train_data, batch, rng)
     curve = []
@@ -175,15 +193,10 @@ def train(rs, cells=64, sigma=2.5, problems=2000, seed0=10000, holdout=0.05, rec
         for mode, cm in (("network", rs.COARSE_NETWORK), ("jacobi", rs.COARSE_JACOBI), ("exact", rs.COARSE_EXACT)):
             L.set_cycle(1, 1, cm)
             its[mode].append(fam.solve(L).iterations)
-    summary = {"cells": cells, "sigma": sigma, "problems": problems, "seed0": seed0, "holdout": n_hold,
-               "pairs": int(sum(len(d[1]) for d in train_data)), "batch": batch, "steps": steps, "lr": lr,
-               "heldout_loss_untrained": base, "heldout_loss_trained": held,
+    summary = {"holdout": n_hold, "pairs": int(sum(len(d[1]) for d in train_data)), "batch": batch,
+               "steps": steps, "lr": lr, "heldout_loss_start": base, "heldout_loss_trained": held,
                "heldout_iterations": {k: float(np.mean(v)) for k, v in its.items()}, "curve": curve}
     say(f"held-out loss {base:.4f} -> {held:.4f}; iterations {summary['heldout_iterations']}")
-    if out:
-        os.makedirs(os.path.dirname(out), exist_ok=True)
-        np.savez(out, params=theta.astype(np.float32), summary=json.dumps(summary))
-        say(f"wrote {out}")
     return theta, summary
 
 
@@ -198,11 +211,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--init", default=None, help="start from these weights (.npz)")
+    ap.add_argument("--rounds", type=int, default=1, help="training rounds; later rounds use on-policy pairs")
     args = ap.parse_args()
     rs = _package()
     rs.init(0)
+    init = rs.load_weights(args.init)[0] if args.init else None
     train(rs, args.cells, args.sigma, args.problems, args.seed0, record=args.record, batch=args.batch,
-          steps=args.steps, lr=args.lr, out=args.out)
+          steps=args.steps, lr=args.lr, out=args.out, init=init, rounds=args.rounds)
     return 0
 
 

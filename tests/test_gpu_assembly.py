@@ -45,6 +45,25 @@ def test_heat_assembly_matches_oracle(gpu, orc, n, sigma):
     assert asm.n_colours <= 64
 
 
+@pytest.mark.parametrize("transient", [False, True])
+def test_heat_row_gather_equals_colour_launches(gpu, transient):
+    """Both heat kernels add every entry's contributions in colour-sorted element order."""
+    rs = gpu
+    xyz, tets, fixed, t_node, k, q = heat_problem(14, 2.0, rs=rs)
+    rng = np.random.default_rng(5)
+    rhoc = 1.0 + rng.random(tets.shape[0])
+    asm = rs.Assembly(xyz, tets, 1, fixed)
+    kw = dict(rhoc=rhoc, transient=True, alpha=0.7, dt=1e-3) if transient else {}
+    b_rows = asm.assemble_heat(k, q, t_node, **kw)
+    v_rows = asm.matrix.download()[2]
+    asm.set_method(False)
+    b_col = asm.assemble_heat(k, q, t_node, **kw)
+    v_col = asm.matrix.download()[2]
+    scale = abs(v_col).max()
+    assert abs(v_rows - v_col).max() <= 1e-14 * scale
+    assert abs(b_rows - b_col).max() <= 1e-14 * abs(b_col).max()
+
+
 def test_heat_transient_assembly_matches_oracle(gpu, orc):
     rs = gpu
     xyz, tets, fixed, t_node, k, q = heat_problem(6, 1.5, rs=rs)

@@ -25,11 +25,11 @@ def test_training_pipeline_small(gpu):
     lt = _train_module()
     theta, summary = lt.train(rs, cells=16, sigma=2.5, problems=16, seed0=500, holdout=0.25, record=32,
                               batch=16, steps=80, lr=2e-3, out=None, log_every=40)
-    print(summary["heldout_loss_untrained"], summary["heldout_loss_trained"], summary["heldout_iterations"],
+    print(summary["heldout_loss_start"], summary["heldout_loss_trained"], summary["heldout_iterations"],
           flush=True)
     assert theta.shape == (rs.N_PARAMS,) and np.all(np.isfinite(theta))
     assert summary["pairs"] > 100
-    assert summary["heldout_loss_trained"] < 0.9 * summary["heldout_loss_untrained"]
+    assert summary["heldout_loss_trained"] < 0.9 * summary["heldout_loss_start"]
     its = summary["heldout_iterations"]
     assert its["exact"] <= its["jacobi"]
     assert its["network"] < 2000          # converged (iteration cap of the held-out solves)
