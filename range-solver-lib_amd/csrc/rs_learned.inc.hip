@@ -1363,46 +1363,22 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
             fin(S.partials.p, nb32, 1, FIN_DOT, 0, S.st.p, S.hist.p);
             k_fcg_update<<<nvb, kBlock, 0, stream>>>(f);
             fin(S.partials.p, nvb, 1, FIN_FCG_UPDATE, 0, S.st.p, S.hist.p);
+            // FP64 residual replacement when requested, inside the graph (no host round trip)
+            k_mp_fold_cond<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n, S.st.p);
+            RS_SHAPE_SWITCH(plan64, k_mp_true_resid_cond, nb64, stream, a64);
+            fin(S.partials.p, nb64, 2, FIN_MP_REPLACE, 0, S.st.p, S.hist.p);
         }
     }, graph_key_of(3, a, plan32, nb32, nb64, nvb) + learned_key(L));
     if (rc) return rc;
-    refinements = 0;
     SolveState h;
     RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     RS_HIP(hipStreamSynchronize(stream));
     done = h.done;
-    double best = h.t2;
-    uint32_t no_gain = 0;
-    while (done == kStopRunning) {
-        if ((rc = run_graph_loop(S, done))) return rc;
-        if (done != kStopPause) break;
-        refinements++;
-        k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
-        if (no_gain >= kMaxReplacementsWithoutGain) {
-            RS_HIP(hipGetLastError());
-            h.reason = RS_REASON_STAGNATED;
-            h.done = kStopDone;
-            RS_HIP(hipMemcpyAsync(&S.st.p->done, &h.done, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-            RS_HIP(hipStreamSynchronize(stream));
-            done = kStopDone;
-            return RS_OK;
-        }
-        RS_SHAPE_SWITCH(plan64, k_mp_true_resid, nb64, stream, a64);
-        fin(S.partials.p, nb64, 2, FIN_MP_TRUE, 0, S.st.p, S.hist.p);
-        RS_HIP(hipGetLastError());
-        RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
-        RS_HIP(hipStreamSynchronize(stream));
-        done = h.done;
-        if (h.t2 < 0.5 * best) {
-            best = h.t2;
-            no_gain = 0;
-        } else {
-            no_gain++;
-        }
-    }
-    // a segment that ended on a non-finite residual leaves x as of its last replacement
+    if (done == kStopRunning && (rc = run_graph_loop(S, done))) return rc;
     RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     RS_HIP(hipStreamSynchronize(stream));
+    refinements = h.nrepl;
+    // a segment that ended on a non-finite residual leaves x as of its last replacement
     if (h.reason != RS_REASON_NONNORMAL) k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
     RS_HIP(hipGetLastError());
     return RS_OK;

@@ -47,6 +47,8 @@ struct SolveState {
     double t1, t2, t3, t4;
     uint32_t it, done, reason, n_outer;
     uint32_t stop_rule, hist_cap, mode, it0;   // it0: first iteration of the current CG segment
+    uint32_t replace, no_gain, nrepl, pad;     // device-side residual replacement (flexible CG)
+    double best;                               // lowest FP64 residual so far (stagnation test)
 };
 enum { kStopRunning = 0, kStopDone = 1, kStopPause = 2 };
 constexpr double kReliableDelta = 0.1;   // FP64 residual replacement when |r| drops 10x
@@ -77,7 +79,8 @@ enum FinOp {
     FIN_MP_TRUE,        // mixed: FP64 true residual (flag: first)
     FIN_MP_RHO,         // mixed: rho after a replacement (flag: first)
     FIN_FCG_BETA,       // flexible CG: rho = r.z, beta = -(z.q_prev) / (p.q)_prev
-    FIN_FCG_UPDATE,     // flexible CG: recursive residual, pause for replacement
+    FIN_FCG_UPDATE,     // flexible CG: recursive residual, request a replacement
+    FIN_MP_REPLACE,     // flexible CG: FP64 true residual of a requested replacement
 };
 
 constexpr int kFinBlock = 1024;
@@ -89,6 +92,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ pa
     if ((op == FIN_DOT || op == FIN_UPDATE || op == FIN_MP_UPDATE || op == FIN_FCG_BETA || op == FIN_FCG_UPDATE) &&
         st->done)
         return;
+    if (op == FIN_MP_REPLACE && (st->done || !st->replace)) return;
     __shared__ double lds[4 * (kFinBlock / kWave)];
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
     // all loads of a thread are independent and issued before the sums (up to 4 partials per
@@ -187,6 +191,11 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ pa
             double err = rn / fmax(st->bn, eps);
             st->rmax = rn;
             st->t2 = err;
+            if (flag) {
+                st->best = err;
+                st->no_gain = 0;
+                st->replace = 0;
+            }
             if (!flag) {
                 st->error = err;
                 uint32_t it = st->it;
@@ -230,7 +239,35 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ pa
                 st->reason = RS_REASON_MAXIT;
                 st->done = kStopDone;
             } else if (rn < kReliableDelta * st->rmax || err < st->cvg) {
-                st->done = kStopPause;
+                st->replace = 1;
+            }
+            break;
+        }
+        case FIN_MP_REPLACE: {
+            // the FP64 true residual replaces the recursive one; convergence is decided here;
+            // kMaxReplacementsWithoutGain replacements without halving the FP64 residual stop
+            // the solve as stagnated
+            double rn = sqrt(v[0]);
+            double err = rn / fmax(st->bn, eps);
+            st->rmax = rn;
+            st->t2 = err;
+            st->error = err;
+            uint32_t it = st->it;
+            if (it > 0 && it - 1 < st->hist_cap) hist[2 * (it - 1)] = err;
+            st->replace = 0;
+            st->nrepl++;
+            if (!dev_isnormal(err) && err != 0.0) {
+                st->reason = RS_REASON_NONNORMAL;
+                st->done = kStopDone;
+            } else if (err < st->cvg) {
+                st->reason = RS_REASON_CONVERGED;
+                st->done = kStopDone;
+            } else if (err < 0.5 * st->best) {
+                st->best = err;
+                st->no_gain = 0;
+            } else if (++st->no_gain >= kMaxReplacementsWithoutGain) {
+                st->reason = RS_REASON_STAGNATED;
+                st->done = kStopDone;
             }
             break;
         }
@@ -546,12 +583,17 @@ __global__ void k_state_init(SolveState *st, double an, double cvg, double prev_
     st->hist_cap = hist_cap;
     st->mode = mode;
     st->it0 = 0;
+    st->replace = 0;
+    st->no_gain = 0;
+    st->nrepl = 0;
+    st->best = 0.0;
 }
 
 // Start a new CG segment on the same solution (the learned solve's Jacobi-CG fallback).
 __global__ void k_state_restart(SolveState *st)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    st->replace = 0;
     st->it0 = st->it;
     st->done = kStopRunning;
     st->reason = RS_REASON_RUNNING;
@@ -750,6 +792,26 @@ __global__ void k_mp_fold(double *__restrict__ x, float *__restrict__ dx, uint64
 template <class Shape>
 __global__ __launch_bounds__(kBlock) void k_mp_true_resid(MpArgs a)
 {
+    EpiTrueResid32 e{ a.b, a.r, { 0.0, 0.0 } };
+    Shape::template run<double>(a.A, a.A.val, a.x, e);
+    store_partials<2>(e.acc, a.partials);
+}
+
+// The same two steps of a replacement, executed only when the finalize kernel requested one
+// (flexible CG keeps them inside its captured iterations instead of pausing for the host).
+__global__ void k_mp_fold_cond(double *__restrict__ x, float *__restrict__ dx, uint64_t n, const SolveState *st)
+{
+    if (st->done || !st->replace) return;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        x[i] += (double)dx[i];
+        dx[i] = 0.0f;
+    }
+}
+
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_mp_true_resid_cond(MpArgs a)
+{
+    if (a.st->done || !a.st->replace) return;
     EpiTrueResid32 e{ a.b, a.r, { 0.0, 0.0 } };
     Shape::template run<double>(a.A, a.A.val, a.x, e);
     store_partials<2>(e.acc, a.partials);
@@ -1387,9 +1449,15 @@ struct rs_solver {
     std::string graph_key;   // raw bytes of every kernel argument and launch shape captured
     std::function<void()> body;
     bool eager = false;
+    // replay loop: pinned copies of the stop flags and their events, kept for the solver's life
+    void *hflags = nullptr;
+    hipEvent_t ev[2] = { nullptr, nullptr };
     ~rs_solver()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
+        for (hipEvent_t &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (hflags) (void)hipHostFree(hflags);
     }
 };
 
@@ -1460,11 +1528,13 @@ struct StopFlags {
 static int run_graph_loop(rs_solver &S, uint32_t &done_out)
 {
     hipStream_t st = runtime().stream;
-    StopFlags *hflags = nullptr;
-    RS_HIP(hipHostMalloc(reinterpret_cast<void **>(&hflags), 2 * sizeof(StopFlags), hipHostMallocDefault));
-    hipEvent_t ev[2];
-    RS_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    RS_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    if (!S.hflags) {
+        RS_HIP(hipHostMalloc(&S.hflags, 2 * sizeof(StopFlags), hipHostMallocDefault));
+        RS_HIP(hipEventCreateWithFlags(&S.ev[0], hipEventDisableTiming));
+        RS_HIP(hipEventCreateWithFlags(&S.ev[1], hipEventDisableTiming));
+    }
+    StopFlags *hflags = static_cast<StopFlags *>(S.hflags);
+    hipEvent_t *ev = S.ev;
     int rc = RS_OK;
     auto enqueue = [&](int slot) -> int {
         if (S.eager) {
@@ -1496,9 +1566,6 @@ static int run_graph_loop(rs_solver &S, uint32_t &done_out)
         }
     }
     (void)hipStreamSynchronize(st);
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-    (void)hipHostFree(hflags);
     return rc;
 }
 
