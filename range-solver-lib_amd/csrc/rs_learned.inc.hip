@@ -328,7 +328,8 @@ __device__ __forceinline__ void s16_run(const S16View &S, const float *__restric
             return v * x[(int32_t)rr + off];
         };
         for (; j + 3 * kWave < e; j += 4 * kWave) {
-            const uint32_t w0 = ent[j], w1 = ent[j + kWave], w2 = ent[j + 2 * kWave], w3 = ent[j + 3 * kWave];
+            const uint32_t w0 = __builtin_nontemporal_load(&ent[j]), w1 = __builtin_nontemporal_load(&ent[j + kWave]);
+            const uint32_t w2 = __builtin_nontemporal_load(&ent[j + 2 * kWave]), w3 = __builtin_nontemporal_load(&ent[j + 3 * kWave]);
             acc += term(w0);
             acc += term(w1);
             acc += term(w2);
