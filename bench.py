@@ -205,6 +205,137 @@ def elasticity(rs, args) -> int:
     return 0
 
 
+def batch_transient(rs, args) -> int:
+    """Batched transient heat: args.batch_count systems, system i the unit cube in k_i^3 cells
+    (k_i uniform on {8..48} from seed 4) x 6 tets, conductivity lognormal(0, 1.5), source
+    U(0, 1), rho*c = 1, T = 0/1 on x = 0/1, T0 = 0; implicit Euler dt = 1e-3, 10 steps each, to
+    relres 1e-8 per step.  Timed: the whole batch (every system's 10 steps) in one batched
+    loop, mixed precision (FP32 SpMV/preconditioner, FP64 residual replacement) and FP64;
+    beside it, the same systems one at a time through the single-system API (GPU assembly +
+    mixed Jacobi-CG per step, the reference's per-step flow).  Prints one JSON line."""
+    import __graft_entry__ as ge
+    tr = ge.load_module("transient")
+    count, steps, dt, sigma = args.batch_count, args.batch_steps, 1e-3, 1.5
+    sizes = tr.system_sizes(rs, count, 8, 48, 4)
+    log(f"batch: {count} systems, k in [{sizes.min()}, {sizes.max()}], {steps} steps; assembling")
+    B, meshes, t_build = tr.build_batch(rs, sizes, sigma=sigma, dt=dt, seed=4, log=log)
+    n_rows = np.array([meshes[k].n for k in sizes])
+    nnz = np.array([meshes[k].nnz for k in sizes])
+    log(f"built in {t_build:.1f}s: unknowns {n_rows.sum()}, nnz {nnz.sum()}")
+
+    def conf(precision, cvg):
+        return rs.MatrixSolverConf(n_outer_iterations=100000, solver_cvg_value=cvg, stop_rule=rs.STOP_RELRES,
+                                   precision=precision)
+
+    # FP64 reference solutions (tight) for the error of every result
+    ref = B.solve(conf(rs.PRECISION_FP64, 1e-12), steps)
+    x_ref = [B.solution(i) for i in range(count)]
+    log(f"FP64 batch to 1e-12: {ref.solve_ms:.1f} ms, rounds {ref.rounds}")
+    methods = {}
+    for name, prec in (("batched_fp64_cg", rs.PRECISION_FP64), ("batched_mixed_cg", rs.PRECISION_MIXED)):
+        B.solve(conf(prec, args.tol), steps)   # warm-up: tuning, graph capture
+        times, info = [], None
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            info = B.solve(conf(prec, args.tol), steps)
+            times.append(time.perf_counter() - t0)
+        its, rr, reasons = B.stats()
+        err = max(float(np.linalg.norm(B.solution(i) - x_ref[i]) / max(np.linalg.norm(x_ref[i]), 1e-300))
+                  for i in range(count))
+        shape, shape_ms = B.shape(prec)
+        methods[name] = {"time_s": round(float(np.median(times)), 4), "device_ms": round(info.solve_ms, 2),
+                         "rounds": info.rounds, "total_iterations": int(info.total_iterations),
+                         "max_iterations_per_system": info.max_iterations,
+                         "replacements": info.replacements, "systems_converged": info.systems_converged,
+                         "max_step_relres": info.max_relres, "max_solution_error_vs_fp64": err,
+                         "spmv_shape": rs.BATCH_SHAPE_NAMES[shape],
+                         "spmv_tuning_us": {rs.BATCH_SHAPE_NAMES[j]: round(v * 1e3, 1) for j, v in enumerate(shape_ms)}}
+        log(f"{name}: {methods[name]['time_s'] * 1e3:.1f} ms, rounds {info.rounds}, "
+            f"iterations {info.total_iterations}, max relres {info.max_relres:.2e}, error {err:.2e}")
+    # dominant kernel: the batched FP32 SpMV over every system (tuning time of the chosen shape)
+    lay = B.layout()
+    shape, shape_ms = B.shape(rs.PRECISION_MIXED)
+    ms = float(shape_ms[shape])
+    byts = spmv_bytes(lay["padded_rows"], lay["nnz"], 4, 4)
+    ach = byts / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"rs::k_bt_spmv FP32 batched SELL SpMV ({rs.BATCH_SHAPE_NAMES[shape]})",
+                "algorithmic_bytes": byts, "avg_us": round(ms * 1e3, 1)}
+    # the same systems one at a time through the single-system API
+    unb = None
+    if args.unbatched_budget > 0:
+        log("unbatched: one system at a time (assembly + mixed Jacobi-CG per step)")
+        solver = rs.RMatrixSolver(conf(rs.PRECISION_MIXED, args.tol))
+        done_nnz, t_unb, done = 0, 0.0, 0
+        for i, k in enumerate(sizes):
+            m = meshes[int(k)]
+            kf, qf = tr.fields(rs, m, 4, i, sigma)
+            dk, dq = rs.DeviceArray.from_host(kf), rs.DeviceArray.from_host(qf)
+            dT = rs.DeviceArray.from_host(m.t0)
+            db, dx, dz = rs.DeviceArray(m.n), rs.DeviceArray(m.n), rs.DeviceArray.from_host(np.zeros(m.n))
+            rs.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                m.asm.assemble_heat_device(dk, dq, dT, db, drhoc=m.drhoc, transient=True, alpha=1.0, dt=dt)
+                rs._check(rs.load().rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * m.n))
+                solver.solve_device(m.asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+                m.asm.dofs_to_nodes(dx, dT)
+            rs.synchronize()
+            t_unb += time.perf_counter() - t0
+            done_nnz += int(nnz[i])
+            done += 1
+            if t_unb > args.unbatched_budget:
+                break
+        est = t_unb * float(nnz.sum()) / done_nnz
+        unb = {"time_s": round(est, 3), "measured_systems": done, "measured_s": round(t_unb, 3),
+               "note": ("all systems" if done == count else
+                        f"first {done} systems timed, scaled by total nnz / their nnz")}
+        log(f"unbatched: {t_unb:.1f}s for {done} systems -> {est:.2f}s for the batch")
+    # CPU baseline: the oracle's per-step loop (assembly + Jacobi-CG) on a bounded sample
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        orc.load()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t_cpu, done_nnz, done = 0.0, 0, 0
+        for i, k in enumerate(sizes):
+            m = meshes[int(k)]
+            kf, qf = tr.fields(rs, m, 4, i, sigma)
+            book, n_dof = orc.node_book(m.xyz.shape[0], m.tets, 1, m.fixed)
+            t0 = time.perf_counter()
+            orc.transient_heat(m.xyz, m.tets, kf, qf, np.ones(m.tets.shape[0]), m.t0, book, n_dof, dt, steps,
+                               cvg=args.tol)
+            t_cpu += time.perf_counter() - t0
+            done_nnz += int(nnz[i])
+            done += 1
+            if t_cpu > args.cpu_budget:
+                break
+        cpu = {"value": round(t_cpu * float(nnz.sum()) / done_nnz, 2), "unit": "s", "cores": threads,
+               "kind": "port",
+               "sample": (f"oracle time loop (assembly + Jacobi-CG per step, {threads} threads) on the first "
+                          f"{done} systems ({t_cpu:.1f}s), scaled by total nnz / their nnz")}
+        log(f"cpu baseline: {cpu['value']} s per batch")
+    best = methods["batched_mixed_cg"]["time_s"]
+    out = {
+        "metric": f"batched transient heat: time for {count} systems x {steps} implicit-Euler steps to relres {args.tol}",
+        "value": best, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(best * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 SpMV/precond, f64 residual replacement and solution (batched_mixed_cg)",
+        "data": "synthetic",
+        "config": {"workload": (f"{count} cube systems, k~U{{8..48}} cells/edge (seed 4) x 6 tets, "
+                                f"lognormal(0,{sigma}) conductivity, implicit Euler dt {dt}, {steps} steps"),
+                   "unknowns": int(n_rows.sum()), "nnz": int(nnz.sum()), "padded_rows": lay["padded_rows"],
+                   "sell_entries": lay["sell_entries"], "parallelism": "single GPU"},
+        "system_steps_per_s": round(count * steps / best, 1),
+        "methods": methods, "unbatched_mixed_cg": unb,
+        "speedup_batched_vs_unbatched": round(unb["time_s"] / best, 2) if unb else None,
+        "roofline": roofline, "cpu_baseline": cpu, "build_s": round(t_build, 2),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,8 +347,14 @@ def main() -> int:
     ap.add_argument("--seed-k", type=int, default=2, help="conductivity field seed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--asm-cells", type=int, default=40)
-    ap.add_argument("--workload", choices=["heat", "elasticity"], default="heat",
-                    help="heat (default, the headline) or the elasticity cantilever")
+    ap.add_argument("--workload", choices=["heat", "elasticity", "batch"], default="heat",
+                    help="heat (default, the headline), the elasticity cantilever, or the batched "
+                         "transient heat systems")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--unbatched-budget", type=float, default=40.0,
+                    help="seconds of one-system-at-a-time solves timed beside the batch (0: skip)")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--elast-cells", type=int, nargs=3, default=[128, 32, 32])
     ap.add_argument("--learned-only", action="store_true",
                     help="profiling mode: time only the learned method (no FP64/mixed CG, no ablation)")
@@ -237,6 +374,8 @@ def main() -> int:
     rs.init(0)
     if args.workload == "elasticity":
         return elasticity(rs, args)
+    if args.workload == "batch":
+        return batch_transient(rs, args)
     c = args.cells
     log(f"building {c}^3 Kuhn mesh and fields")
     xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma, args.seed_k)

@@ -239,6 +239,45 @@ int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node)
 int rs_assembly_destroy(rs_assembly *S);
 
 /* ---------------------------------------------------------------- learned preconditioner */
+/* ---------------------------------------------------------------- batched systems
+ * Many independent systems solved together (no counterpart in the reference, which solves
+ * one system per RMatrixSolver::solve call, src/rmatrixsolver.cpp:158-201).  Each system runs
+ * `steps` implicit-Euler steps of Jacobi-CG as RSolverHeat does per time step
+ * (src/rsolverheat.cpp:83-88 clears x, :757-800 forms the step's system, :458-480 solves and
+ * writes back): step k solves A x_k = c + M x_{k-1} (x_{-1} = the initial solution), where A
+ * is the step matrix (Me + dt Ke with alpha = 1), M the free-free mass matrix (omit it for a
+ * steady system: b = c every step) and c the right-hand side assembled with the free
+ * temperatures at zero.  Every step starts from x = 0 (warm_start = 0, as the reference) or
+ * from the previous step's solution.  A step ends when the FP64 ||b - A x|| / ||b|| is below
+ * solver_cvg_value (conf->stop_rule must be RS_STOP_RELRES, solver_type RS_SOLVER_CG).
+ * MIXED: FP32 matrix, vectors and preconditioner inside, FP64 solution, dot products and
+ * residual replacement.  All systems share each launch; the SpMV shape is autotuned per batch
+ * (shape ids: 0..2 one lane per row with 4/8/16 waves per workgroup, 3 two lanes per row,
+ * 4 four lanes per row). */
+typedef struct rs_batch rs_batch;
+typedef struct rs_batch_info {
+    uint32_t rounds;              /* rounds of the batched loop (one CG iteration of every iterating system) */
+    uint32_t max_iterations;      /* largest sum over one system's steps */
+    uint64_t total_iterations;    /* sum over systems and steps */
+    uint32_t systems_converged;   /* systems whose every step converged */
+    uint32_t replacements;        /* FP64 residual replacements, all systems */
+    double max_relres;            /* largest true relative residual at the end of a step */
+    double solve_ms;              /* device time of the batched loop */
+} rs_batch_info;
+int rs_batch_create(uint32_t count, const uint32_t *n_rows, const uint64_t *nnz, rs_batch **out);
+int rs_batch_set_matrix(rs_batch *B, uint32_t i, const rs_matrix *A);      /* copies A (device) */
+int rs_batch_set_mass(rs_batch *B, uint32_t i, const rs_matrix *M);        /* same pattern as A */
+int rs_batch_set_rhs(rs_batch *B, uint32_t i, const double *d_c, const double *d_x0); /* d_x0 NULL: 0 */
+int rs_batch_solve(rs_batch *B, const rs_conf *conf, uint32_t steps, int32_t warm_start, rs_batch_info *info);
+/* per system and step (count*steps, row-major): CG iterations and the final true relative
+ * residual; reasons: rs_reason per system (count) */
+int rs_batch_stats(const rs_batch *B, uint32_t *iterations, double *relres, int32_t *reasons);
+int rs_batch_solution(const rs_batch *B, uint32_t i, double *d_x);       /* last step's x */
+int rs_batch_shape(rs_batch *B, int32_t precision, int32_t *shape, double *shape_ms);
+int rs_batch_force_shape(rs_batch *B, int32_t precision, int32_t shape);  /* -1: autotune */
+int rs_batch_info_layout(const rs_batch *B, uint32_t *padded_rows, uint64_t *nnz, uint64_t *sell_entries);
+int rs_batch_destroy(rs_batch *B);
+
 /* No counterpart in the reference (a new preconditioner type, RS_PRECOND_LEARNED): one V-cycle
  * over an aggregation hierarchy (piecewise-constant prolongations, Galerkin operators
  * P^T A P, Chebyshev smoothing in FP32) whose coarsest level (<= 512 nodes) is solved by a

@@ -208,3 +208,24 @@ def stress_assemble(xyz, tets, E, nu, rho, gravity, u_node, tris, traction, book
     lib.or_stress_assemble(xyz.size // 3, _p(xyz), tets.size // 4, _p(tets), _p(E), _p(nu), _p(rho),
                            _p(g), _p(u_node), ntri, _p(tr), _p(tt), _p(book), n_dof, A, _p(b))
     return _export(A, n_dof), b[:n_dof]
+
+
+def transient_heat(xyz, tets, k, q, rhoc, t_node0, book, n_dof, dt, steps, cvg=1e-13,
+                   n_outer=100000, stop_rule=STOP_RELRES):
+    """The RSolverHeat time loop restated, one system: every step assembles A and b with the
+    previous step's nodeTemperature (src/rsolverheat.cpp:757-800, alpha = 1), clears x
+    (prepare, :83-88), runs Jacobi-CG (RSolverHeat::solve, :458-465) and writes x back into
+    the free nodes (:473-480).  Returns (per-step solutions, per-step CG iterations)."""
+    t = np.array(t_node0, np.float64, copy=True)
+    book = np.asarray(book, np.uint32)
+    free = book != 0xFFFFFFFF
+    pos = book[free].astype(np.int64)
+    xs, its = [], []
+    for _ in range(steps):
+        A, b = heat_assemble(xyz, tets, k, q, t, book, n_dof, rhoc=rhoc, transient=True, alpha=1.0, dt=dt)
+        x, info = solve(A, b, None, PRECOND_JACOBI, 1, SOLVER_CG, n_outer, cvg=cvg, stop_rule=stop_rule)
+        t[free] = x[pos]
+        xs.append(x)
+        its.append(int(info["iterations"]))
+    return xs, its
+

@@ -66,6 +66,15 @@ class Info(C.Structure):
         return d
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("rounds", C.c_uint32), ("max_iterations", C.c_uint32),
+                ("total_iterations", C.c_uint64), ("systems_converged", C.c_uint32),
+                ("replacements", C.c_uint32), ("max_relres", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _P = C.c_void_p
 _u32p = C.POINTER(C.c_uint32)
 _f64p = C.POINTER(C.c_double)
@@ -131,6 +140,15 @@ _SIGS = {
     "rs_trainer_step": [_P, C.c_uint32, C.c_uint64, _P, _P, C.c_uint32, _P, _P, _P, C.c_float,
                         C.POINTER(C.c_float)],
     "rs_trainer_get_params": [_P, _P, C.c_uint64], "rs_trainer_destroy": [_P],
+    "rs_batch_create": [C.c_uint32, _P, _P, C.POINTER(_P)],
+    "rs_batch_set_matrix": [_P, C.c_uint32, _P], "rs_batch_set_mass": [_P, C.c_uint32, _P],
+    "rs_batch_set_rhs": [_P, C.c_uint32, _P, _P],
+    "rs_batch_solve": [_P, C.POINTER(Conf), C.c_uint32, C.c_int32, C.POINTER(BatchInfo)],
+    "rs_batch_stats": [_P, _P, _P, _P], "rs_batch_solution": [_P, C.c_uint32, _P],
+    "rs_batch_shape": [_P, C.c_int32, C.POINTER(C.c_int32), _P],
+    "rs_batch_force_shape": [_P, C.c_int32, C.c_int32],
+    "rs_batch_info_layout": [_P, _u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "rs_batch_destroy": [_P],
 }
 
 _lib = None
@@ -543,6 +561,77 @@ class Assembly:
     def __del__(self):
         if _lib is not None and getattr(self, "h", None):
             _lib.rs_assembly_destroy(self.h)
+            self.h = None
+
+
+# ------------------------------------------------------------------------------ batched systems
+BATCH_SHAPES = 5
+BATCH_SHAPE_NAMES = ["sell-64 1 lane/row 4 waves", "sell-64 1 lane/row 8 waves",
+                     "sell-64 1 lane/row 16 waves", "sell-64 2 lanes/row", "sell-64 4 lanes/row"]
+
+
+class Batch:
+    """Many independent systems solved together (rs_batch_*): each runs `steps` implicit-Euler
+    steps of Jacobi-CG, A x_k = c + M x_{k-1}, as RSolverHeat does once per time step
+    (src/rsolverheat.cpp:83-88, :757-800, :458-480); every launch covers all systems."""
+
+    def __init__(self, n_rows, nnz):
+        load()
+        n_rows = np.ascontiguousarray(n_rows, np.uint32)
+        nnz = np.ascontiguousarray(nnz, np.uint64)
+        assert n_rows.size == nnz.size
+        self.count = int(n_rows.size)
+        self.n_rows = n_rows
+        self.h = C.c_void_p()
+        _check(_lib.rs_batch_create(self.count, _ptr(n_rows), _ptr(nnz), C.byref(self.h)))
+        self.steps = 0
+
+    def set_matrix(self, i: int, A: SparseMatrix) -> None:
+        _check(_lib.rs_batch_set_matrix(self.h, i, A.h))
+
+    def set_mass(self, i: int, M: SparseMatrix) -> None:
+        _check(_lib.rs_batch_set_mass(self.h, i, M.h))
+
+    def set_rhs(self, i: int, dc: DeviceArray, dx0: DeviceArray | None = None) -> None:
+        _check(_lib.rs_batch_set_rhs(self.h, i, dc.ptr, dx0.ptr if dx0 is not None else None))
+
+    def solve(self, conf: MatrixSolverConf, steps: int, warm_start: bool = False) -> BatchInfo:
+        c = conf.to_c()
+        info = BatchInfo()
+        _check(_lib.rs_batch_solve(self.h, C.byref(c), steps, int(bool(warm_start)), C.byref(info)))
+        self.steps = steps
+        return info
+
+    def stats(self):
+        """(iterations[count, steps], relres[count, steps], reasons[count])."""
+        its = np.empty(self.count * self.steps, np.uint32)
+        rr = np.empty(self.count * self.steps, np.float64)
+        reasons = np.empty(self.count, np.int32)
+        _check(_lib.rs_batch_stats(self.h, _ptr(its), _ptr(rr), _ptr(reasons)))
+        return its.reshape(self.count, self.steps), rr.reshape(self.count, self.steps), reasons
+
+    def solution(self, i: int, dx: DeviceArray | None = None):
+        out = dx if dx is not None else DeviceArray(int(self.n_rows[i]))
+        _check(_lib.rs_batch_solution(self.h, i, out.ptr))
+        return out if dx is not None else out.download()
+
+    def shape(self, precision: int = PRECISION_MIXED):
+        sid = C.c_int32()
+        ms = np.zeros(BATCH_SHAPES, np.float64)
+        _check(_lib.rs_batch_shape(self.h, precision, C.byref(sid), _ptr(ms)))
+        return sid.value, ms
+
+    def force_shape(self, precision: int, shape: int) -> None:
+        _check(_lib.rs_batch_force_shape(self.h, precision, shape))
+
+    def layout(self):
+        rows, nnz, ent = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(_lib.rs_batch_info_layout(self.h, C.byref(rows), C.byref(nnz), C.byref(ent)))
+        return {"padded_rows": rows.value, "nnz": nnz.value, "sell_entries": ent.value}
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.rs_batch_destroy(self.h)
             self.h = None
 
 
