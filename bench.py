@@ -51,8 +51,8 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n: int, nnz: int, s_val: int, s_x: int) -> int:
-    return nnz * (s_val + 4) + n * (4 + 2 * s_x) + 4
+def spmv_bytes(n: int, nnz: int, s_val: int, s_x: int, s_col: int = 4) -> int:
+    return nnz * (s_val + s_col) + n * (4 + 2 * s_x) + 4
 
 
 def build_problem(rs, cells: int, sigma: float, seed_k: int = 2, seed_q: int = 1):
@@ -219,6 +219,7 @@ def batch_transient(rs, args) -> int:
     sizes = tr.system_sizes(rs, count, 8, 48, 4)
     log(f"batch: {count} systems, k in [{sizes.min()}, {sizes.max()}], {steps} steps; assembling")
     B, meshes, t_build = tr.build_batch(rs, sizes, sigma=sigma, dt=dt, seed=4, log=log)
+    B.set_replacement(args.batch_replacement)
     n_rows = np.array([meshes[k].n for k in sizes])
     nnz = np.array([meshes[k].nnz for k in sizes])
     log(f"built in {t_build:.1f}s: unknowns {n_rows.sum()}, nnz {nnz.sum()}")
@@ -256,7 +257,7 @@ def batch_transient(rs, args) -> int:
     lay = B.layout()
     shape, shape_ms = B.shape(rs.PRECISION_MIXED)
     ms = float(shape_ms[shape])
-    byts = spmv_bytes(lay["padded_rows"], lay["nnz"], 4, 4)
+    byts = spmv_bytes(lay["padded_rows"], lay["nnz"], 4, 4, lay["column_bytes"])
     ach = byts / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
@@ -326,7 +327,8 @@ def batch_transient(rs, args) -> int:
         "config": {"workload": (f"{count} cube systems, k~U{{8..48}} cells/edge (seed 4) x 6 tets, "
                                 f"lognormal(0,{sigma}) conductivity, implicit Euler dt {dt}, {steps} steps"),
                    "unknowns": int(n_rows.sum()), "nnz": int(nnz.sum()), "padded_rows": lay["padded_rows"],
-                   "sell_entries": lay["sell_entries"], "parallelism": "single GPU"},
+                   "sell_entries": lay["sell_entries"], "column_bytes": lay["column_bytes"],
+                   "replacement_ratio": args.batch_replacement, "parallelism": "single GPU"},
         "system_steps_per_s": round(count * steps / best, 1),
         "methods": methods, "unbatched_mixed_cg": unb,
         "speedup_batched_vs_unbatched": round(unb["time_s"] / best, 2) if unb else None,
@@ -355,6 +357,8 @@ def main() -> int:
     ap.add_argument("--unbatched-budget", type=float, default=40.0,
                     help="seconds of one-system-at-a-time solves timed beside the batch (0: skip)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--batch-replacement", type=float, default=0.01,
+                    help="mixed batch: FP64 residual replacement when the residual fell by this ratio")
     ap.add_argument("--elast-cells", type=int, nargs=3, default=[128, 32, 32])
     ap.add_argument("--learned-only", action="store_true",
                     help="profiling mode: time only the learned method (no FP64/mixed CG, no ablation)")

@@ -275,7 +275,13 @@ int rs_batch_stats(const rs_batch *B, uint32_t *iterations, double *relres, int3
 int rs_batch_solution(const rs_batch *B, uint32_t i, double *d_x);       /* last step's x */
 int rs_batch_shape(rs_batch *B, int32_t precision, int32_t *shape, double *shape_ms);
 int rs_batch_force_shape(rs_batch *B, int32_t precision, int32_t shape);  /* -1: autotune */
-int rs_batch_info_layout(const rs_batch *B, uint32_t *padded_rows, uint64_t *nnz, uint64_t *sell_entries);
+/* column_bytes: 2 when every column fits an int16 offset from its 64-row slice (mesh
+ * matrices), else 4 */
+int rs_batch_info_layout(const rs_batch *B, uint32_t *padded_rows, uint64_t *nnz, uint64_t *sell_entries,
+                         int32_t *column_bytes);
+/* MIXED: an FP64 residual replacement whenever the recursive residual fell below ratio times
+ * the last replaced one (default 0.01; 0 = only when it claims convergence) */
+int rs_batch_set_replacement(rs_batch *B, double ratio);
 int rs_batch_destroy(rs_batch *B);
 
 /* No counterpart in the reference (a new preconditioner type, RS_PRECOND_LEARNED): one V-cycle

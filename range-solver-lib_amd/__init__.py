@@ -147,7 +147,8 @@ _SIGS = {
     "rs_batch_stats": [_P, _P, _P, _P], "rs_batch_solution": [_P, C.c_uint32, _P],
     "rs_batch_shape": [_P, C.c_int32, C.POINTER(C.c_int32), _P],
     "rs_batch_force_shape": [_P, C.c_int32, C.c_int32],
-    "rs_batch_info_layout": [_P, _u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "rs_batch_info_layout": [_P, _u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int32)],
+    "rs_batch_set_replacement": [_P, C.c_double],
     "rs_batch_destroy": [_P],
 }
 
@@ -625,9 +626,13 @@ class Batch:
         _check(_lib.rs_batch_force_shape(self.h, precision, shape))
 
     def layout(self):
-        rows, nnz, ent = C.c_uint32(), C.c_uint64(), C.c_uint64()
-        _check(_lib.rs_batch_info_layout(self.h, C.byref(rows), C.byref(nnz), C.byref(ent)))
-        return {"padded_rows": rows.value, "nnz": nnz.value, "sell_entries": ent.value}
+        rows, nnz, ent, cb = C.c_uint32(), C.c_uint64(), C.c_uint64(), C.c_int32()
+        _check(_lib.rs_batch_info_layout(self.h, C.byref(rows), C.byref(nnz), C.byref(ent), C.byref(cb)))
+        return {"padded_rows": rows.value, "nnz": nnz.value, "sell_entries": ent.value, "column_bytes": cb.value}
+
+    def set_replacement(self, ratio: float) -> None:
+        """MIXED: FP64 residual replacement when the residual fell by `ratio` (default 0.01)."""
+        _check(_lib.rs_batch_set_replacement(self.h, ratio))
 
     def __del__(self):
         if _lib is not None and getattr(self, "h", None):

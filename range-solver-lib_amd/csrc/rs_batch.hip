@@ -55,7 +55,7 @@ constexpr uint32_t kFlagFirst = 2u;          // next direction is p = z
 constexpr uint32_t kFlagReplace = 4u;        // FP64 residual replacement pending
 constexpr uint32_t kFlagAdvance = 8u;        // step start pending
 constexpr uint32_t kEvent = kFlagReplace | kFlagAdvance;
-constexpr double kBatchDelta = 0.1;          // mixed: replace when |r| fell by this since the last replacement
+constexpr double kBatchDelta = 0.01;         // mixed: replace when |r| fell by this since the last replacement
 constexpr uint32_t kBatchNoGain = 20;        // replacements without halving the true residual
 constexpr uint32_t kBatchRounds = 16;        // rounds per captured graph
 constexpr int kBatchShapes = 5;             // see RS_BT_SHAPE
@@ -78,6 +78,7 @@ struct BtArgs {
     const uint32_t *slice_sys, *sys_s0, *sys_ns;
     const double *an;            // ||A||_F per system
     const uint32_t *sell_ptr, *sell_col;
+    const int16_t *sell_off;     // columns as offsets from the slice's first row (when they fit)
     const double *sell_val;
     const float *sell_val32;
     const double *sell_mval;     // null: no mass matrix (b = c every step)
@@ -116,21 +117,41 @@ __device__ __forceinline__ double *bt_bank(const BtArgs &a, uint32_t step) { ret
 template <typename P>
 __device__ __forceinline__ P bt_nt(const P *p) { return __builtin_nontemporal_load(p); }
 
+// Column of SELL entry k in slice s: the full uint32 column, or (when every column of the batch
+// lies within +-32767 of its slice's first row, as on mesh matrices) an int16 offset from the
+// slice's first row -- 2 bytes per entry instead of 4.
+struct ColFull {
+    const uint32_t *c;
+    __device__ __forceinline__ uint32_t operator()(uint32_t k, uint32_t) const { return bt_nt(c + k); }
+};
+struct ColOff {
+    const int16_t *c;
+    __device__ __forceinline__ uint32_t operator()(uint32_t k, uint32_t sbase) const
+    {
+        return (uint32_t)((int32_t)sbase + (int32_t)bt_nt(c + k));
+    }
+};
+template <bool OFF>
+__device__ __forceinline__ auto bt_cols(const BtArgs &a)
+{
+    if constexpr (OFF) return ColOff{ a.sell_off };
+    else return ColFull{ a.sell_col };
+}
+
 // Row `row` of a slice whose entries start at `base` (width entries per row), G lanes per
 // row: lane sub takes entries sub, sub + G, ...; the G partial sums are combined by shuffles
 // in a fixed order (result in every lane of the row).
-template <int G, typename AT, typename VT, typename XT>
-__device__ __forceinline__ AT bt_row(const uint32_t *__restrict__ col, const VT *__restrict__ val,
-                                     const XT *__restrict__ x, uint32_t base, uint32_t width, uint32_t row,
-                                     uint32_t sub)
+template <int G, typename AT, class CR, typename VT, typename XT>
+__device__ __forceinline__ AT bt_row(CR col, const VT *__restrict__ val, const XT *__restrict__ x, uint32_t base,
+                                     uint32_t width, uint32_t row, uint32_t sub, uint32_t sbase)
 {
-    const uint32_t *__restrict__ cp = col + base + row;
-    const VT *__restrict__ vp = val + base + row;
+    const uint32_t k0 = base + row;
+    const VT *__restrict__ vp = val + k0;
     AT s = 0;
     uint32_t j = sub;
     for (; j + 3 * G < width; j += 4 * G) {
-        uint32_t c0 = bt_nt(cp + j * kSlice), c1 = bt_nt(cp + (j + G) * kSlice);
-        uint32_t c2 = bt_nt(cp + (j + 2 * G) * kSlice), c3 = bt_nt(cp + (j + 3 * G) * kSlice);
+        uint32_t c0 = col(k0 + j * kSlice, sbase), c1 = col(k0 + (j + G) * kSlice, sbase);
+        uint32_t c2 = col(k0 + (j + 2 * G) * kSlice, sbase), c3 = col(k0 + (j + 3 * G) * kSlice, sbase);
         VT v0 = bt_nt(vp + j * kSlice), v1 = bt_nt(vp + (j + G) * kSlice);
         VT v2 = bt_nt(vp + (j + 2 * G) * kSlice), v3 = bt_nt(vp + (j + 3 * G) * kSlice);
         XT y0 = x[c0], y1 = x[c1], y2 = x[c2], y3 = x[c3];
@@ -139,24 +160,24 @@ __device__ __forceinline__ AT bt_row(const uint32_t *__restrict__ col, const VT 
         s += (AT)v2 * (AT)y2;
         s += (AT)v3 * (AT)y3;
     }
-    for (; j < width; j += G) s += (AT)bt_nt(vp + j * kSlice) * (AT)x[bt_nt(cp + j * kSlice)];
+    for (; j < width; j += G) s += (AT)bt_nt(vp + j * kSlice) * (AT)x[col(k0 + j * kSlice, sbase)];
 #pragma unroll
     for (int off = kWave / 2; off >= kWave / G; off >>= 1) s += __shfl_xor(s, off, kWave);
     return s;
 }
 
 // Both products of one row with the same vector: (A y)_row and (M y)_row, FP64.
-template <int G>
-__device__ __forceinline__ void bt_row2(const BtArgs &a, const double *__restrict__ y, uint32_t base, uint32_t width,
-                                        uint32_t row, uint32_t sub, double &ay, double &my)
+template <int G, class CR>
+__device__ __forceinline__ void bt_row2(const BtArgs &a, CR col, const double *__restrict__ y, uint32_t base,
+                                        uint32_t width, uint32_t row, uint32_t sub, uint32_t sbase, double &ay,
+                                        double &my)
 {
-    const uint32_t *__restrict__ cp = a.sell_col + base + row;
-    const double *__restrict__ va = a.sell_val + base + row;
-    const double *__restrict__ vm = a.sell_mval + base + row;
+    const uint32_t k0 = base + row;
+    const double *__restrict__ va = a.sell_val + k0;
+    const double *__restrict__ vm = a.sell_mval + k0;
     double s = 0.0, m = 0.0;
     for (uint32_t j = sub; j < width; j += G) {
-        uint32_t c = bt_nt(cp + j * kSlice);
-        double yc = y[c];
+        const double yc = y[col(k0 + j * kSlice, sbase)];
         s += bt_nt(va + j * kSlice) * yc;
         m += bt_nt(vm + j * kSlice) * yc;
     }
@@ -214,7 +235,8 @@ __global__ void k_bt_sell_fill(const uint32_t *__restrict__ rpc, const uint32_t 
                                const double *__restrict__ val, const double *__restrict__ mval,
                                const uint32_t *__restrict__ sell_ptr, uint32_t nslices, uint32_t *__restrict__ scol,
                                double *__restrict__ sval, float *__restrict__ sval32, double *__restrict__ smval,
-                               double *__restrict__ dinv, float *__restrict__ dinv32)
+                               double *__restrict__ dinv, float *__restrict__ dinv32, int16_t *__restrict__ soff,
+                               uint32_t *__restrict__ far)
 {
     const uint64_t rows = (uint64_t)nslices * kSlice;
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (uint64_t)gridDim.x * blockDim.x) {
@@ -238,6 +260,12 @@ __global__ void k_bt_sell_fill(const uint32_t *__restrict__ rpc, const uint32_t 
                 sval32[k] = 0.0f;
                 if (smval) smval[k] = 0.0;
             }
+        }
+        for (uint32_t j = 0; j < width; j++) {
+            const uint32_t k = base + j * kSlice + lane;
+            const int64_t o = (int64_t)scol[k] - (int64_t)s * kSlice;
+            if (o < -32767 || o > 32767) atomicAdd(far, 1u);
+            else soff[k] = (int16_t)o;
         }
         const double inv = d != 0.0 ? 1.0 / d : 0.0;
         dinv[r] = inv;
@@ -312,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_dir(BtArgs a)
     }
 }
 
-template <int G, int W, typename T>
+template <int G, int W, typename T, bool OFF>
 __global__ __launch_bounds__(W * kWave) void k_bt_spmv(BtArgs a)
 {
     if (a.g->done) return;
@@ -322,16 +350,18 @@ __global__ __launch_bounds__(W * kWave) void k_bt_spmv(BtArgs a)
     const T *__restrict__ p = (const T *)a.p;
     T *__restrict__ q = (T *)a.q;
     const T *__restrict__ val = bt_vals<T>(a);
+    const auto cols = bt_cols<OFF>(a);
     constexpr uint32_t R = kWave / G;
     const uint32_t sub = lane / R;
     for (uint32_t s = lw; s < a.nslices; s += nw) {
         if (!bt_iterating(a.sys[a.slice_sys[s]].flags)) continue;
+        const uint32_t sbase = s * kSlice;
         const uint32_t base = a.sell_ptr[s], width = (a.sell_ptr[s + 1] - base) / kSlice;
         double acc = 0.0;
 #pragma unroll
         for (int pass = 0; pass < G; pass++) {
             const uint32_t row = pass * R + lane % R;
-            const T sum = bt_row<G, T>(a.sell_col, val, p, base, width, row, sub);
+            const T sum = bt_row<G, T>(cols, val, p, base, width, row, sub, sbase);
             if (sub == 0) {
                 const uint64_t i = (uint64_t)s * kSlice + row;
                 q[i] = sum;
@@ -375,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_update(BtArgs a)
 
 // Events: a replacement (r = b - A x in FP64) or a step start (b = c + M x_prev, x = 0 as the
 // reference's prepare does, or x = x_prev with warm starts; r = b - A x).
-template <int G, int W, typename T>
+template <int G, int W, typename T, bool OFF>
 __global__ __launch_bounds__(W * kWave) void k_bt_event(BtArgs a)
 {
     if (a.g->done) return;
@@ -384,9 +414,11 @@ __global__ __launch_bounds__(W * kWave) void k_bt_event(BtArgs a)
     const uint32_t lw = xcd_remap(blockIdx.x, gridDim.x) * W + threadIdx.x / kWave;
     T *__restrict__ r = (T *)a.r;
     const T *__restrict__ dinv = bt_dinv<T>(a);
+    const auto cols = bt_cols<OFF>(a);
     constexpr uint32_t R = kWave / G;
     const uint32_t sub = lane / R;
     for (uint32_t s = lw; s < a.nslices; s += nw) {
+        const uint32_t sbase = s * kSlice;
         const BtSys &S = a.sys[a.slice_sys[s]];
         const uint32_t fl = S.flags;
         if (!(fl & kFlagActive) || !(fl & kEvent)) continue;
@@ -402,8 +434,8 @@ __global__ __launch_bounds__(W * kWave) void k_bt_event(BtArgs a)
             double bi, ax = 0.0;
             if (adv) {
                 double my = 0.0, ay = 0.0;
-                if (a.sell_mval) bt_row2<G>(a, xp, base, width, row, sub, ay, my);
-                else if (a.warm) ay = bt_row<G, double>(a.sell_col, a.sell_val, xp, base, width, row, sub);
+                if (a.sell_mval) bt_row2<G>(a, cols, xp, base, width, row, sub, sbase, ay, my);
+                else if (a.warm) ay = bt_row<G, double>(cols, a.sell_val, xp, base, width, row, sub, sbase);
                 bi = a.c[i] + my;
                 if (a.warm) ax = ay;
                 if (sub == 0) {
@@ -411,7 +443,7 @@ __global__ __launch_bounds__(W * kWave) void k_bt_event(BtArgs a)
                     x[i] = a.warm ? xp[i] : 0.0;
                 }
             } else {
-                ax = bt_row<G, double>(a.sell_col, a.sell_val, x, base, width, row, sub);
+                ax = bt_row<G, double>(cols, a.sell_val, x, base, width, row, sub, sbase);
                 bi = a.b[i];
             }
             if (sub == 0) {
@@ -570,15 +602,15 @@ __global__ __launch_bounds__(kBlock) void k_bt_fin(BtArgs a)
     S->flags = nf;
 }
 
-#define RS_BT_SHAPE(id, KERNEL, T, nslices, stream, args)                                               \
-    do {                                                                                             \
-        switch (id) {                                                                                \
-            case 0: KERNEL<1, 4, T><<<bt_grid(nslices, 4), 4 * kWave, 0, stream>>>(args); break;     \
-            case 1: KERNEL<1, 8, T><<<bt_grid(nslices, 8), 8 * kWave, 0, stream>>>(args); break;     \
-            case 2: KERNEL<1, 16, T><<<bt_grid(nslices, 16), 16 * kWave, 0, stream>>>(args); break;  \
-            case 3: KERNEL<2, 4, T><<<bt_grid(nslices, 4), 4 * kWave, 0, stream>>>(args); break;     \
-            default: KERNEL<4, 4, T><<<bt_grid(nslices, 4), 4 * kWave, 0, stream>>>(args); break;    \
-        }                                                                                            \
+#define RS_BT_SHAPE(id, KERNEL, T, OFF, nslices, stream, args)                                               \
+    do {                                                                                                  \
+        switch (id) {                                                                                     \
+            case 0: KERNEL<1, 4, T, OFF><<<bt_grid(nslices, 4), 4 * kWave, 0, stream>>>(args); break;     \
+            case 1: KERNEL<1, 8, T, OFF><<<bt_grid(nslices, 8), 8 * kWave, 0, stream>>>(args); break;     \
+            case 2: KERNEL<1, 16, T, OFF><<<bt_grid(nslices, 16), 16 * kWave, 0, stream>>>(args); break;  \
+            case 3: KERNEL<2, 4, T, OFF><<<bt_grid(nslices, 4), 4 * kWave, 0, stream>>>(args); break;     \
+            default: KERNEL<4, 4, T, OFF><<<bt_grid(nslices, 4), 4 * kWave, 0, stream>>>(args); break;    \
+        }                                                                                                 \
     } while (0)
 
 uint32_t round8(uint32_t a) { return (a + kXcd - 1) / kXcd * kXcd; }
@@ -613,7 +645,9 @@ struct rs_batch {
     DBuf<uint32_t> sell_ptr, sell_col, slice_sys, sys_s0, sys_ns, widths, bad;
     DBuf<double> sell_val, sell_mval, dinv, an;
     DBuf<float> sell_val32, dinv32;
-    bool built = false;
+    DBuf<int16_t> sell_off;
+    bool built = false, compact = false;
+    double delta = kBatchDelta;
     // vectors and state
     DBuf<double> c, b, x0, x1, xinit, r, p, q, partials, relres;
     DBuf<uint32_t> its;
@@ -666,21 +700,26 @@ int bt_build(rs_batch &B)
     B.sell_entries = acc;
     size_t E = std::max<uint64_t>(1, acc);
     if ((rc = B.sell_col.alloc(E)) || (rc = B.sell_val.alloc(E)) || (rc = B.sell_val32.alloc(E)) ||
+        (rc = B.sell_off.alloc(E)) ||
         (rc = B.sell_mval.alloc(any_mass ? E : 1)) || (rc = B.dinv.alloc(std::max(1u, B.N))) ||
         (rc = B.dinv32.alloc(std::max(1u, B.N))))
         return rc;
     RS_HIP(hipMemcpyAsync(B.sell_ptr.p, sp.data(), 4ull * sp.size(), hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemsetAsync(B.bad.p, 0, sizeof(uint32_t), st));
     if (B.nslices)
         k_bt_sell_fill<<<bt_vec_grid(B.nslices), kBlock, 0, st>>>(B.rowptr.p, B.col.p, B.val.p,
                                                                   any_mass ? B.mval.p : nullptr, B.sell_ptr.p,
                                                                   B.nslices, B.sell_col.p, B.sell_val.p,
                                                                   B.sell_val32.p, any_mass ? B.sell_mval.p : nullptr,
-                                                                  B.dinv.p, B.dinv32.p);
+                                                                  B.dinv.p, B.dinv32.p, B.sell_off.p, B.bad.p);
     RS_HIP(hipGetLastError());
     if ((rc = B.an.alloc(B.count))) return rc;
     k_bt_frob<<<B.count, kBlock, 0, st>>>(B.val.p, B.rowptr.p, B.sys_s0.p, B.sys_ns.p, B.an.p);
     RS_HIP(hipGetLastError());
+    uint32_t far = 0;
+    RS_HIP(hipMemcpyAsync(&far, B.bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
+    B.compact = far == 0 && std::getenv("RS_BATCH_FULL_COLUMNS") == nullptr;
     B.built = true;
     if (B.graph) {
         (void)hipGraphExecDestroy(B.graph);
@@ -702,6 +741,7 @@ BtArgs bt_args(rs_batch &B, bool mixed, uint32_t steps, uint32_t n_outer, double
     a.an = B.an.p;
     a.sell_ptr = B.sell_ptr.p;
     a.sell_col = B.sell_col.p;
+    a.sell_off = B.compact ? B.sell_off.p : nullptr;
     a.sell_val = B.sell_val.p;
     a.sell_val32 = B.sell_val32.p;
     a.sell_mval = B.have_mass.size() && std::any_of(B.have_mass.begin(), B.have_mass.end(), [](uint8_t v) { return v; })
@@ -723,15 +763,27 @@ BtArgs bt_args(rs_batch &B, bool mixed, uint32_t steps, uint32_t n_outer, double
     a.its = B.its.p;
     a.relres = B.relres.p;
     a.cvg = cvg;
-    a.delta = mixed ? kBatchDelta : 0.0;
+    a.delta = mixed ? B.delta : 0.0;
     a.warm = warm;
     return a;
 }
 
 void bt_spmv_launch(int id, bool mixed, uint32_t nslices, hipStream_t st, const BtArgs &a)
 {
-    if (mixed) RS_BT_SHAPE(id, k_bt_spmv, float, nslices, st, a);
-    else RS_BT_SHAPE(id, k_bt_spmv, double, nslices, st, a);
+    const bool off = a.sell_off != nullptr;
+    if (mixed && off) RS_BT_SHAPE(id, k_bt_spmv, float, true, nslices, st, a);
+    else if (mixed) RS_BT_SHAPE(id, k_bt_spmv, float, false, nslices, st, a);
+    else if (off) RS_BT_SHAPE(id, k_bt_spmv, double, true, nslices, st, a);
+    else RS_BT_SHAPE(id, k_bt_spmv, double, false, nslices, st, a);
+}
+
+void bt_event_launch(int id, bool mixed, uint32_t nslices, hipStream_t st, const BtArgs &a)
+{
+    const bool off = a.sell_off != nullptr;
+    if (mixed && off) RS_BT_SHAPE(id, k_bt_event, float, true, nslices, st, a);
+    else if (mixed) RS_BT_SHAPE(id, k_bt_event, float, false, nslices, st, a);
+    else if (off) RS_BT_SHAPE(id, k_bt_event, double, true, nslices, st, a);
+    else RS_BT_SHAPE(id, k_bt_event, double, false, nslices, st, a);
 }
 
 // The batched SpMV autotuner: times every SELL shape (lanes per row x waves per workgroup)
@@ -788,8 +840,7 @@ void bt_round(int id, bool mixed, uint32_t nslices, uint32_t count, hipStream_t 
     if (mixed) k_bt_update<float><<<vg, kBlock, 0, st>>>(a);
     else k_bt_update<double><<<vg, kBlock, 0, st>>>(a);
     k_bt_fin<BT_UPDATE><<<count, kBlock, 0, st>>>(a);
-    if (mixed) RS_BT_SHAPE(id, k_bt_event, float, nslices, st, a);
-    else RS_BT_SHAPE(id, k_bt_event, double, nslices, st, a);
+    bt_event_launch(id, mixed, nslices, st, a);
     k_bt_fin<BT_EVENT><<<count, kBlock, 0, st>>>(a);
 }
 
@@ -1129,12 +1180,21 @@ extern "C" int rs_batch_force_shape(rs_batch *B, int32_t precision, int32_t shap
     return RS_OK;
 }
 
-extern "C" int rs_batch_info_layout(const rs_batch *B, uint32_t *padded_rows, uint64_t *nnz, uint64_t *sell_entries)
+extern "C" int rs_batch_info_layout(const rs_batch *B, uint32_t *padded_rows, uint64_t *nnz, uint64_t *sell_entries,
+                                    int32_t *column_bytes)
 {
     if (!B) return fail(RS_ERR_INVALID_ARG, "rs_batch_info_layout: null batch");
+    if (column_bytes) *column_bytes = B->built ? (B->compact ? 2 : 4) : 0;
     if (padded_rows) *padded_rows = B->N;
     if (nnz) *nnz = B->NNZ;
     if (sell_entries) *sell_entries = B->built ? B->sell_entries : 0;
+    return RS_OK;
+}
+
+extern "C" int rs_batch_set_replacement(rs_batch *B, double ratio)
+{
+    if (!B || !(ratio >= 0.0 && ratio < 1.0)) return fail(RS_ERR_INVALID_ARG, "rs_batch_set_replacement: ratio must be in [0, 1)");
+    B->delta = ratio;
     return RS_OK;
 }
 

@@ -205,3 +205,43 @@ def test_batch_invalid_arguments(gpu):
     B.set_mass(1, dA)
     info = B.solve(_conf(rs, rs.PRECISION_FP64, 1e-10), 2)
     assert info.systems_converged == 2
+
+
+def test_batch_column_offsets_match_full_columns(gpu, tr, monkeypatch):
+    """int16 column offsets (the default on mesh matrices) and full uint32 columns read the same
+    entries in the same order: bit-identical solves."""
+    rs = gpu
+    sizes = [5, 3, 6, 4]
+    out = {}
+    for full in (False, True):
+        if full:
+            monkeypatch.setenv("RS_BATCH_FULL_COLUMNS", "1")
+        B, _, _ = tr.build_batch(rs, sizes, sigma=1.5, dt=1e-2, seed=33)
+        for prec in (rs.PRECISION_FP64, rs.PRECISION_MIXED):
+            B.solve(_conf(rs, prec, 1e-9), 3)
+            out[(full, prec)] = (B.stats()[0].copy(), [B.solution(i) for i in range(len(sizes))])
+        assert B.layout()["column_bytes"] == (4 if full else 2)
+    for prec in (rs.PRECISION_FP64, rs.PRECISION_MIXED):
+        a, b = out[(False, prec)], out[(True, prec)]
+        assert np.array_equal(a[0], b[0])
+        for x, y in zip(a[1], b[1]):
+            assert np.array_equal(x, y)
+
+
+def test_batch_replacement_ratio(gpu, tr):
+    """Fewer FP64 replacements with a smaller ratio; the steps still end below the tolerance."""
+    rs = gpu
+    B, _, _ = tr.build_batch(rs, [6, 4, 7], sigma=1.5, dt=1e-2, seed=41)
+    conf = _conf(rs, rs.PRECISION_MIXED, 1e-8)
+    reps = []
+    for ratio in (0.1, 1e-2, 1e-3):
+        B.set_replacement(ratio)
+        info = B.solve(conf, 3)
+        its, rr, reasons = B.stats()
+        assert (reasons == 1).all() and (rr <= 1e-8).all(), ratio
+        reps.append(info.replacements)
+    assert reps[0] > reps[1] >= reps[2] >= 9      # at least one per step and system
+    B.set_replacement(0.01)
+    with pytest.raises(rs.RSError):
+        B.set_replacement(1.0)
+
