@@ -111,7 +111,7 @@ def spmv_only(rs, asm, dk, dq, dT, n, nnz) -> int:
     for name, prec, sv, sx in (("fp64", rs.PRECISION_FP64, 8, 8), ("fp32", rs.PRECISION_MIXED, 4, 4)):
         plan, _ = asm.matrix.spmv_plan(prec)
         ms = asm.matrix.spmv_timed(prec, reps=20)
-        byts = spmv_bytes(n, nnz, sv, sx)
+        byts = spmv_bytes(n, nnz, sv, sx, rs.SPMV_COLUMN_BYTES[plan])
         out[name] = {"shape": rs.SPMV_SHAPE_NAMES[plan], "avg_us": round(ms * 1e3, 3),
                      "algorithmic_bytes": byts, "gbs": round(byts / (ms * 1e-3) / 1e9, 1)}
     print(json.dumps(out), flush=True)
@@ -520,7 +520,7 @@ def main() -> int:
     for name, prec, sv, sx in (("fp64", rs.PRECISION_FP64, 8, 8), ("fp32", rs.PRECISION_MIXED, 4, 4)):
         plan, tune_ms = asm.matrix.spmv_plan(prec)
         ms = asm.matrix.spmv_timed(prec, reps=200)
-        byts = spmv_bytes(n, nnz, sv, sx)
+        byts = spmv_bytes(n, nnz, sv, sx, rs.SPMV_COLUMN_BYTES[plan])
         spmv[name] = {"avg_us": round(ms * 1e3, 3), "bytes": byts,
                       "gbs": round(byts / (ms * 1e-3) / 1e9, 1),
                       "shape": rs.SPMV_SHAPE_NAMES[plan],

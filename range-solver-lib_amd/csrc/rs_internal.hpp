@@ -109,12 +109,14 @@ struct Matrix {
     uint32_t ntiles[2] = { 0, 0 };
     // SELL-64 copy: slice offsets, column-major columns (padding RS_EOD) and values
     DBuf<uint32_t> sell_ptr, sell_col;
+    DBuf<int16_t> sell_off;     // int16 column offsets from each slice's first row
+    bool sell_off_ok = false;   // every column fits such an offset
     DBuf<double> sell_val;
     DBuf<float> sell_val32;
     uint32_t nslices = 0;
     bool sell_valid = false, sell32_valid = false;
     int plan[2] = { -1, -1 };   // autotuned SpMV shape for FP64 / FP32
-    double plan_ms[2][8] = {};  // tuning times per shape
+    double plan_ms[2][9] = {};  // tuning times per shape
 };
 
 // Every change of the FP64 values goes through here: derived copies become stale.

@@ -429,6 +429,26 @@ __global__ void k_sell_cols(CsrView A, uint32_t *__restrict__ sell_col)
     }
 }
 
+// int16 offsets of the SELL-64 columns from each slice's first row; padding points at the
+// lane's own row.  `far` counts columns that do not fit.
+__global__ void k_sell_offs(CsrView A, const uint32_t *__restrict__ sell_col, int16_t *__restrict__ sell_off,
+                            uint32_t *__restrict__ far)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < (uint64_t)A.nslices * kWave;
+         r += (uint64_t)gridDim.x * kBlock) {
+        uint32_t s = (uint32_t)(r / kWave), lane = (uint32_t)(r % kWave);
+        uint32_t base = A.sell_ptr[s], width = (A.sell_ptr[s + 1] - base) / kWave;
+        uint32_t miss = 0;
+        for (uint32_t j = 0; j < width; j++) {
+            const uint32_t k = base + j * kWave + lane, c = sell_col[k];
+            const int64_t o = c == kSellPad ? (int64_t)lane : (int64_t)c - (int64_t)s * kWave;
+            if (o < -32767 || o > 32767) miss++;
+            else sell_off[k] = (int16_t)o;
+        }
+        if (miss) atomicAdd(far, miss);
+    }
+}
+
 // SELL-64 values from the CSR values (padding 0); FP32 copy from the FP64 SELL values.
 __global__ void k_sell_vals(CsrView A, double *__restrict__ sell_val)
 {
@@ -925,7 +945,10 @@ static uint32_t vec_blocks(uint64_t items)
 
 static const int kVecLanes[5] = { 4, 8, 16, 32, 64 };
 static const uint32_t kTileCap[2] = { 2048, 4096 };
-constexpr int kSellPlan = 7;
+constexpr int kSellPlan = 7;      // SellRows; kSellPlan + 1 = SellRowsOff (same slices)
+static_assert(kNumShapes == 9, "plan_ms holds 9 shapes");
+
+static bool is_sell_plan(int id) { return id >= kSellPlan; }
 
 // Workgroups of plan `id` for matrix A, and the view its kernels read.
 static CsrView plan_view(const Matrix &A, int id, uint32_t &nb)
@@ -951,6 +974,7 @@ static CsrView plan_view(const Matrix &A, int id, uint32_t &nb)
     } else {
         v.sell_ptr = A.sell_ptr.p;
         v.sell_col = A.sell_col.p;
+        v.sell_off = A.sell_off_ok ? A.sell_off.p : nullptr;
         v.sell_val = A.sell_val.p;
         v.sell_val32 = A.sell_val32.p;
         v.nslices = A.nslices;
@@ -1051,6 +1075,19 @@ int matrix_set_layout(Matrix &A, const uint32_t *host_rowptr)
         CsrView v = plan_view(A, kSellPlan, nb);
         if (A.nslices) k_sell_cols<<<vec_blocks((uint64_t)A.nslices * kWave), kBlock, 0, runtime().stream>>>(v, A.sell_col.p);
         RS_HIP(hipGetLastError());
+        // int16 offsets for SellRowsOff when every column fits
+        A.sell_off_ok = false;
+        DBuf<uint32_t> far;
+        if ((rc = A.sell_off.alloc(std::max<uint32_t>(sp.back(), 1))) || (rc = far.alloc(1))) return rc;
+        RS_HIP(hipMemsetAsync(far.p, 0, sizeof(uint32_t), runtime().stream));
+        if (A.nslices)
+            k_sell_offs<<<vec_blocks((uint64_t)A.nslices * kWave), kBlock, 0, runtime().stream>>>(v, A.sell_col.p,
+                                                                                              A.sell_off.p, far.p);
+        RS_HIP(hipGetLastError());
+        uint32_t nfar = 0;
+        RS_HIP(hipMemcpyAsync(&nfar, far.p, sizeof(uint32_t), hipMemcpyDeviceToHost, runtime().stream));
+        RS_HIP(hipStreamSynchronize(runtime().stream));
+        A.sell_off_ok = nfar == 0;
     }
     A.sell_valid = A.sell32_valid = false;
     for (int t = 0; t < 2; t++) {
@@ -1070,8 +1107,8 @@ int matrix_set_layout(Matrix &A, const uint32_t *host_rowptr)
 int matrix_sync_plan(Matrix &A, int plan, bool fp32)
 {
     int rc;
-    if (fp32 && plan != kSellPlan && (rc = matrix_refresh_fp32(A))) return rc;
-    if (plan != kSellPlan) return RS_OK;
+    if (fp32 && !is_sell_plan(plan) && (rc = matrix_refresh_fp32(A))) return rc;
+    if (!is_sell_plan(plan)) return RS_OK;
     hipStream_t st = runtime().stream;
     uint32_t nb = 0;
     CsrView v = plan_view(A, kSellPlan, nb);

@@ -12,6 +12,10 @@
 //                rowptr -> col -> x chain per row, so many more loads are in flight.
 //
 //  SellRows      SELL-64 copy of the matrix, one row per lane (below).
+//  SellRowsOff   the same slices with int16 column offsets from the slice's first row (when
+//                every column is within +-32767 of it, as on mesh matrices): 2 bytes per
+//                entry instead of 4, and padding points at the lane's own row (value 0), so the
+//                product loop has no branch.
 //
 // All call epi.row(r, s) exactly once per row with the row's dot product s (accumulated in
 // AT: double for the FP64 path, float for the FP32 path); the epilogue fuses whatever the
@@ -34,6 +38,7 @@ struct CsrView {
     uint32_t ntiles, tpb;      // tiles, tiles per workgroup
     const uint32_t *sell_ptr;  // SellRows: slice start offsets (nslices + 1)
     const uint32_t *sell_col;  // column-major within a slice, padding = kSellPad
+    const int16_t *sell_off;   // the same as offsets from the slice's first row (or null)
     const double *sell_val;
     const float *sell_val32;
     uint32_t nslices, spb;     // slices, slices per workgroup
@@ -174,14 +179,55 @@ struct SellRows {
     }
 };
 
+struct SellRowsOff {
+    static constexpr bool kTile = false;
+    static constexpr int kParam = 64;
+    template <typename AT, typename VT, typename XT, class Epi>
+    __device__ __forceinline__ static void run(const CsrView &A, const VT *__restrict__ csr_val,
+                                               const XT *__restrict__ x, Epi &epi)
+    {
+        if (A.sell_off == nullptr) {
+            // columns do not fit 16-bit offsets: the full-column slices
+            SellRows::run<AT, VT, XT, Epi>(A, csr_val, x, epi);
+            return;
+        }
+        const VT *__restrict__ val = sell_values(A, csr_val);
+        const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+        const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+        const uint32_t s0 = lb * A.spb, s1 = min(A.nslices, s0 + A.spb);
+        const int16_t *__restrict__ off = A.sell_off;
+        auto ld = [](const auto *p) { return __builtin_nontemporal_load(p); };
+        for (uint32_t s = s0 + wid; s < s1; s += kBlock / kWave) {
+            const uint32_t b = A.sell_ptr[s], e = A.sell_ptr[s + 1];
+            const int32_t sb = (int32_t)(s * kWave);
+            AT acc = 0;
+            uint32_t j = b + lane;
+            for (; j + 3 * kWave < e; j += 4 * kWave) {
+                int32_t c0 = sb + ld(&off[j]), c1 = sb + ld(&off[j + kWave]);
+                int32_t c2 = sb + ld(&off[j + 2 * kWave]), c3 = sb + ld(&off[j + 3 * kWave]);
+                VT v0 = ld(&val[j]), v1 = ld(&val[j + kWave]), v2 = ld(&val[j + 2 * kWave]), v3 = ld(&val[j + 3 * kWave]);
+                XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+                acc += (AT)v0 * (AT)x0;
+                acc += (AT)v1 * (AT)x1;
+                acc += (AT)v2 * (AT)x2;
+                acc += (AT)v3 * (AT)x3;
+            }
+            for (; j < e; j += kWave) acc += (AT)ld(&val[j]) * (AT)x[sb + ld(&off[j])];
+            const uint32_t r = s * kWave + lane;
+            if (r < A.n) epi.row(r, acc);
+        }
+    }
+};
+
 // Launch plan of one matrix: which shape, and its grid.
 struct SpmvPlan {
     int id = -1;           // index into kShapes
     uint32_t nb = 0;       // workgroups
 };
 
-// Shape ids: 0..4 VecRows<4,8,16,32,64>, 5 TileRows<2048>, 6 TileRows<4096>, 7 SellRows
-constexpr int kNumShapes = 8;
+// Shape ids: 0..4 VecRows<4,8,16,32,64>, 5 TileRows<2048>, 6 TileRows<4096>, 7 SellRows,
+// 8 SellRowsOff
+constexpr int kNumShapes = 9;
 
 #define RS_SHAPE_SWITCH(id, KERNEL, grid, stream, ...)                                         \
     do {                                                                                       \
@@ -193,7 +239,8 @@ constexpr int kNumShapes = 8;
             case 4: KERNEL<::rs::VecRows<64>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;  \
             case 5: KERNEL<::rs::TileRows<2048>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
             case 6: KERNEL<::rs::TileRows<4096>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
-            default: KERNEL<::rs::SellRows><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;       \
+            case 7: KERNEL<::rs::SellRows><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;       \
+            default: KERNEL<::rs::SellRowsOff><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;    \
         }                                                                                      \
     } while (0)
 
