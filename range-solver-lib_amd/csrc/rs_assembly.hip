@@ -140,18 +140,79 @@ __global__ __launch_bounds__(kBlock) void k_heat_colour(HeatArgs a, uint32_t e0,
 // Row-gather variant: one thread per free DOF row sums the contributions of the elements
 // incident to its node, in colour-sorted element order -- the order in which the colour
 // launches add into that row, so the values are bit-identical to k_heat_colour's -- and writes
-// the row once.  Per row ~24 elements are read (each element by its 4 nodes, mostly from L2)
-// instead of a scattered read-modify-write of 16 values per element.
+// the row once.
+//
+// What does not change between assemblies of one mesh is computed once, at create, into one
+// record per (row, incident element) ("row records"): the four gradient products
+// grad N_m . grad N_n of the row's local node m, the volume, the element's original index (and
+// m in its top two bits), the row slots of the element's four columns (one byte each, 0xFF for
+// a fixed column) and the four node ids.  Records are laid out like SELL-64: the records of the
+// 64 rows of a slice are interleaved (record j of lane l at base + 64 j + l), so every field is
+// one coalesced load per wave.  An assembly then streams the records and gathers only the
+// coefficients k, q, rho*c and the prescribed temperatures.
 constexpr int kRowCap = 16;
+constexpr uint32_t kSlotFixed = 0xFFu;
+
+struct RowRecords {
+    const double *grad;         // 4 per record: grad N_m . grad N_n, n = 0..3
+    const double *vol;
+    const uint32_t *elem;       // original element index | m << 30
+    const uint32_t *slots;      // byte n: slot of column n in the row, kSlotFixed if fixed
+    const uint4 *nodes;
+    const uint64_t *base;       // per slice: first record
+    const uint32_t *count;      // per row: records
+};
 
 struct HeatRowArgs {
     HeatArgs h;
     const uint32_t *rowptr;
-    const uint32_t *dof_node;   // row -> node
-    const uint32_t *inc_ptr;    // node -> [inc_ptr[v], inc_ptr[v+1]) in inc
-    const uint32_t *inc;        // colour-sorted element index << 2 | local node index
+    RowRecords rec;
     uint32_t n_rows;
 };
+
+// Records of every free row (once per mesh): the same tet_gradients and products as the
+// assembly kernels, so the assembled values do not change.
+__global__ __launch_bounds__(kBlock) void k_heat_records(HeatArgs h, const uint32_t *__restrict__ rowptr,
+                                                         const uint32_t *__restrict__ dof_node,
+                                                         const uint32_t *__restrict__ inc_ptr,
+                                                         const uint32_t *__restrict__ inc, uint32_t n_rows,
+                                                         double *__restrict__ grad, double *__restrict__ vol,
+                                                         uint32_t *__restrict__ elem, uint32_t *__restrict__ slots,
+                                                         uint4 *__restrict__ nodes, const uint64_t *__restrict__ base,
+                                                         uint32_t *__restrict__ count)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_rows) return;
+    const uint32_t r0 = rowptr[i];
+    const uint32_t v = dof_node[i];
+    const uint32_t k0 = inc_ptr[v], k1 = inc_ptr[v + 1];
+    const uint64_t b = base[i / kWave] + (i % kWave);
+    count[i] = k1 - k0;
+    for (uint32_t k = k0; k < k1; k++) {
+        const uint32_t w = inc[k], e = w >> 2, m = w & 3u;
+        const uint4 nd4 = h.conn[e];
+        const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+        double X[4][3], g[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) X[q][c] = h.xyz[3 * (uint64_t)nd[q] + c];
+        const double V = tet_gradients(X, g);
+        const uint64_t r = b + (uint64_t)(k - k0) * kWave;
+        const uint4 pos = reinterpret_cast<const uint4 *>(h.map + 16ull * e)[m];
+        const uint32_t pp[4] = { pos.x, pos.y, pos.z, pos.w };
+        uint32_t sl = 0;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            grad[4 * r + n] = g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2];
+            sl |= (pp[n] == RS_EOD ? kSlotFixed : (pp[n] - r0)) << (8 * n);
+        }
+        vol[r] = V;
+        elem[r] = h.perm[e] | (m << 30);
+        slots[r] = sl;
+        nodes[r] = nd4;
+    }
+}
 
 __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
 {
@@ -160,31 +221,35 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
     const uint32_t i = blockIdx.x * kBlock + t;
     if (i >= a.n_rows) return;
     const HeatArgs &h = a.h;
+    const RowRecords &R = a.rec;
     const uint32_t r0 = a.rowptr[i], len = a.rowptr[i + 1] - r0;
 #pragma unroll
     for (int j = 0; j < kRowCap; j++) acc[j][t] = 0.0;
     double bi = 0.0;
-    const uint32_t v = a.dof_node[i];
-    for (uint32_t k = a.inc_ptr[v]; k < a.inc_ptr[v + 1]; k++) {
-        const uint32_t w = a.inc[k], e = w >> 2, m = w & 3u;
-        const uint4 nd4 = h.conn[e];
-        const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
-        const uint32_t orig = h.perm[e];
-        double X[4][3], g[4][3];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) X[q][c] = h.xyz[3 * (uint64_t)nd[q] + c];
-        const double V = tet_gradients(X, g);
+    const uint64_t b = R.base[i / kWave] + (i % kWave);
+    const uint32_t cnt = R.count[i];
+    for (uint32_t k = 0; k < cnt; k++) {
+        const uint64_t r = b + (uint64_t)k * kWave;
+        const double2 g01 = reinterpret_cast<const double2 *>(R.grad)[2 * r];
+        const double2 g23 = reinterpret_cast<const double2 *>(R.grad)[2 * r + 1];
+        const double gg[4] = { g01.x, g01.y, g23.x, g23.y };
+        const double V = R.vol[r];
+        const uint32_t em = R.elem[r], orig = em & 0x3FFFFFFFu, m = em >> 30;
+        const uint32_t sl = R.slots[r];
         const double kV = h.k[orig] * V;
         const double fq = h.q[orig] * V * 0.25;
         double Ae[4], be;
+        const bool lift = ((sl & 0xFFu) == kSlotFixed) | (((sl >> 8) & 0xFFu) == kSlotFixed) |
+                          (((sl >> 16) & 0xFFu) == kSlotFixed) | ((sl >> 24) == kSlotFixed);
+        uint4 nd4 = make_uint4(0, 0, 0, 0);
+        if (h.transient || lift) nd4 = R.nodes[r];
+        const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
         if (h.transient) {
             const double mc = h.rhoc[orig] * V / 20.0;
             be = h.dt * fq;
 #pragma unroll
             for (int n = 0; n < 4; n++) {
-                const double K = kV * (g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]);
+                const double K = kV * gg[n];
                 const double M = mc * ((uint32_t)n == m ? 2.0 : 1.0);
                 Ae[n] = M + h.alpha * h.dt * K;
                 be += (M - (1.0 - h.alpha) * h.dt * K) * h.t_node[nd[n]];
@@ -192,18 +257,18 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
         } else {
             be = fq;
 #pragma unroll
-            for (int n = 0; n < 4; n++) Ae[n] = kV * (g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]);
+            for (int n = 0; n < 4; n++) Ae[n] = kV * gg[n];
         }
         // Dirichlet lifting (src/rsolverheat.cpp:787-798), in the colour kernel's order
 #pragma unroll
         for (int n = 0; n < 4; n++)
-            if (h.book[nd[n]] == RS_EOD) be -= Ae[n] * h.t_node[nd[n]];
+            if (((sl >> (8 * n)) & 0xFFu) == kSlotFixed) be -= Ae[n] * h.t_node[nd[n]];
         bi += be;
-        const uint4 pos = reinterpret_cast<const uint4 *>(h.map + 16ull * e)[m];
-        const uint32_t pp[4] = { pos.x, pos.y, pos.z, pos.w };
 #pragma unroll
-        for (int n = 0; n < 4; n++)
-            if (pp[n] != RS_EOD) acc[pp[n] - r0][t] += Ae[n];
+        for (int n = 0; n < 4; n++) {
+            const uint32_t slot = (sl >> (8 * n)) & 0xFFu;
+            if (slot != kSlotFixed) acc[slot][t] += Ae[n];
+        }
     }
     for (uint32_t j = 0; j < len; j++) h.val[r0 + j] = acc[j][t];
     h.b[i] = bi;
@@ -341,6 +406,10 @@ struct rs_assembly {
     DBuf<double> xyz;
     DBuf<uint32_t> conn, perm, map, dbook;
     DBuf<uint32_t> dof_node, inc_ptr, inc;  // row-gather heat assembly
+    DBuf<double> rec_grad, rec_vol;         // row records (built once per mesh)
+    DBuf<uint32_t> rec_elem, rec_slots, rec_count;
+    DBuf<uint4> rec_nodes;
+    DBuf<uint64_t> rec_base;
     bool rows = false;                      // every row fits kRowCap entries
     bool use_rows = true;                   // heat: row-gather kernel (else the colour launches)
     rs_matrix A;                          // borrowed by callers
@@ -475,6 +544,33 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
         RS_HIP(hipMemcpyAsync(S->inc_ptr.p, inc_ptr.data(), sizeof(uint32_t) * inc_ptr.size(), hipMemcpyHostToDevice, st));
         if (!inc.empty()) RS_HIP(hipMemcpyAsync(S->inc.p, inc.data(), sizeof(uint32_t) * inc.size(), hipMemcpyHostToDevice, st));
         RS_HIP(hipMemcpyAsync(S->dof_node.p, dof_node.data(), sizeof(uint32_t) * dof_node.size(), hipMemcpyHostToDevice, st));
+        // row records: slices of 64 rows, padded to the slice's longest incidence list
+        const uint32_t nsl = (S->n_dof + kWave - 1) / kWave;
+        std::vector<uint64_t> rb(nsl + 1ull, 0);
+        for (uint32_t sidx = 0; sidx < nsl; sidx++) {
+            uint32_t w = 0;
+            for (uint32_t r = sidx * kWave; r < std::min(S->n_dof, (sidx + 1) * kWave); r++) {
+                const uint32_t v = dof_node[r];
+                w = std::max(w, inc_ptr[v + 1] - inc_ptr[v]);
+            }
+            rb[sidx + 1] = rb[sidx] + (uint64_t)w * kWave;
+        }
+        const uint64_t nrec = std::max<uint64_t>(rb[nsl], 1);
+        if ((rc = S->rec_grad.alloc(4 * nrec)) || (rc = S->rec_vol.alloc(nrec)) || (rc = S->rec_elem.alloc(nrec)) ||
+            (rc = S->rec_slots.alloc(nrec)) || (rc = S->rec_nodes.alloc(nrec)) || (rc = S->rec_base.alloc(rb.size())) ||
+            (rc = S->rec_count.alloc(std::max(S->n_dof, 1u))))
+            return rc;
+        RS_HIP(hipMemcpyAsync(S->rec_base.p, rb.data(), sizeof(uint64_t) * rb.size(), hipMemcpyHostToDevice, st));
+        HeatArgs ha{};
+        ha.xyz = S->xyz.p;
+        ha.conn = reinterpret_cast<const uint4 *>(S->conn.p);
+        ha.perm = S->perm.p;
+        ha.map = S->map.p;
+        if (S->n_dof)
+            k_heat_records<<<grid_for(S->n_dof, kBlock), kBlock, 0, st>>>(
+                ha, A.rowptr.p, S->dof_node.p, S->inc_ptr.p, S->inc.p, S->n_dof, S->rec_grad.p, S->rec_vol.p,
+                S->rec_elem.p, S->rec_slots.p, S->rec_nodes.p, S->rec_base.p, S->rec_count.p);
+        RS_HIP(hipGetLastError());
     }
     if ((rc = matrix_set_layout(A, pat.rowptr.data()))) return rc;
     RS_HIP(hipStreamSynchronize(st));
@@ -547,7 +643,9 @@ extern "C" int rs_assemble_heat_device(rs_assembly *S, const double *d_k, const 
     a.alpha = alpha;
     a.dt = dt;
     if (S->rows && S->use_rows) {
-        HeatRowArgs ra{ a, S->A.rowptr.p, S->dof_node.p, S->inc_ptr.p, S->inc.p, S->n_dof };
+        RowRecords rec{ S->rec_grad.p, S->rec_vol.p, S->rec_elem.p, S->rec_slots.p, S->rec_nodes.p, S->rec_base.p,
+                        S->rec_count.p };
+        HeatRowArgs ra{ a, S->A.rowptr.p, rec, S->n_dof };
         if (S->n_dof) k_heat_rows<<<grid_for(S->n_dof, kBlock), kBlock, 0, st>>>(ra);
     } else {
         if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
