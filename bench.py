@@ -40,9 +40,10 @@ HBM_PEAK_GBS = 8000.0
 # --pmc runs (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB copy); the
 # runs and their commands are summarised in profiles/spmv_heat128_pmc.md.
 MEASURED_TRAFFIC_128 = {
-    "vector-csr g4": 588.2e6, "vector-csr g8": 507.8e6, "vector-csr g16": 487.6e6,
-    "vector-csr g32": 456.5e6, "vector-csr g64": 456.9e6, "tile-csr 2048": 456.4e6,
-    "tile-csr 4096": 442.3e6, "sell-64": 422.4e6, "sell-64 int16": 352.7e6,
+    "vector-csr g4": 588.9e6, "vector-csr g8": 507.8e6, "vector-csr g16": 487.7e6,
+    "vector-csr g32": 456.5e6, "vector-csr g64": 456.9e6, "tile-csr 2048": 456.6e6,
+    "tile-csr 4096": 442.2e6, "sell-64": 422.4e6, "sell-64 int16": 352.7e6,
+    "sell-64 u8": 426.8e6, "sell-64 int16 u8": 353.4e6,
 }
 MEASURED_TRAFFIC_128_SOURCE = "profiles/spmv_heat128_pmc.md"
 

@@ -28,10 +28,11 @@ PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2, 3
 SOLVER_CG, SOLVER_GMRES = 0, 1
 STOP_REFERENCE, STOP_RELRES = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-SPMV_SHAPES = 9
+SPMV_SHAPES = 11
 SPMV_SHAPE_NAMES = ["vector-csr g4", "vector-csr g8", "vector-csr g16", "vector-csr g32",
-                    "vector-csr g64", "tile-csr 2048", "tile-csr 4096", "sell-64", "sell-64 int16"]
-SPMV_COLUMN_BYTES = [4, 4, 4, 4, 4, 4, 4, 4, 2]
+                    "vector-csr g64", "tile-csr 2048", "tile-csr 4096", "sell-64", "sell-64 int16",
+                    "sell-64 u8", "sell-64 int16 u8"]
+SPMV_COLUMN_BYTES = [4, 4, 4, 4, 4, 4, 4, 4, 2, 4, 2]
 REASON_NAMES = {0: "running", 1: "converged", 2: "stagnated", 3: "nonnormal", 4: "maxit"}
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "UNKNOWN_SOLVER", 3: "INVALID_PRECOND",

@@ -116,7 +116,7 @@ struct Matrix {
     uint32_t nslices = 0;
     bool sell_valid = false, sell32_valid = false;
     int plan[2] = { -1, -1 };   // autotuned SpMV shape for FP64 / FP32
-    double plan_ms[2][9] = {};  // tuning times per shape
+    double plan_ms[2][11] = {}; // tuning times per shape
 };
 
 // Every change of the FP64 values goes through here: derived copies become stale.

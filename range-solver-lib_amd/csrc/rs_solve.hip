@@ -945,8 +945,8 @@ static uint32_t vec_blocks(uint64_t items)
 
 static const int kVecLanes[5] = { 4, 8, 16, 32, 64 };
 static const uint32_t kTileCap[2] = { 2048, 4096 };
-constexpr int kSellPlan = 7;      // SellRows; kSellPlan + 1 = SellRowsOff (same slices)
-static_assert(kNumShapes == 9, "plan_ms holds 9 shapes");
+constexpr int kSellPlan = 7;      // SellRows; the shapes after it read the same slices
+static_assert(kNumShapes == 11, "plan_ms holds 11 shapes");
 
 static bool is_sell_plan(int id) { return id >= kSellPlan; }
 

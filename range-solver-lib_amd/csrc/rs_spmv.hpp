@@ -139,7 +139,10 @@ struct TileRows {
 __device__ __forceinline__ const double *sell_values(const CsrView &A, const double *) { return A.sell_val; }
 __device__ __forceinline__ const float *sell_values(const CsrView &A, const float *) { return A.sell_val32; }
 
-struct SellRows {
+// U entries of a row per loop trip, all loads of a trip independent (U = 4 or 8: the autotuner
+// keeps the faster; deeper unrolling keeps more loads in flight per wave).
+template <int U>
+struct SellRowsU {
     static constexpr bool kTile = false;
     static constexpr int kParam = 64;
     template <typename AT, typename VT, typename XT, class Epi>
@@ -159,27 +162,32 @@ struct SellRows {
             // the vectors (measured: FP64 SpMV 93.6 -> 78.7 us at 128^3, and the FP32 products
             // inside the solvers faster although a lone FP32 SpMV is ~5% slower)
             auto ld = [](const auto *p) { return __builtin_nontemporal_load(p); };
-            for (; j + 3 * kWave < e; j += 4 * kWave) {
-                uint32_t c0 = ld(&col[j]), c1 = ld(&col[j + kWave]), c2 = ld(&col[j + 2 * kWave]), c3 = ld(&col[j + 3 * kWave]);
-                VT v0 = ld(&val[j]), v1 = ld(&val[j + kWave]), v2 = ld(&val[j + 2 * kWave]), v3 = ld(&val[j + 3 * kWave]);
-                XT x0 = c0 != kSellPad ? x[c0] : XT(0), x1 = c1 != kSellPad ? x[c1] : XT(0);
-                XT x2 = c2 != kSellPad ? x[c2] : XT(0), x3 = c3 != kSellPad ? x[c3] : XT(0);
-                acc += (AT)v0 * (AT)x0;
-                acc += (AT)v1 * (AT)x1;
-                acc += (AT)v2 * (AT)x2;
-                acc += (AT)v3 * (AT)x3;
+            for (; j + (U - 1) * kWave < e; j += U * kWave) {
+                uint32_t c[U];
+                VT v[U];
+                XT y[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) c[u] = ld(&col[j + u * kWave]);
+#pragma unroll
+                for (int u = 0; u < U; u++) v[u] = ld(&val[j + u * kWave]);
+#pragma unroll
+                for (int u = 0; u < U; u++) y[u] = c[u] != kSellPad ? x[c[u]] : XT(0);
+#pragma unroll
+                for (int u = 0; u < U; u++) acc += (AT)v[u] * (AT)y[u];
             }
             for (; j < e; j += kWave) {
-                uint32_t c = col[j];
-                if (c != kSellPad) acc += (AT)val[j] * (AT)x[c];
+                uint32_t cc = col[j];
+                if (cc != kSellPad) acc += (AT)val[j] * (AT)x[cc];
             }
             const uint32_t r = s * kWave + lane;
             if (r < A.n) epi.row(r, acc);
         }
     }
 };
+using SellRows = SellRowsU<4>;
 
-struct SellRowsOff {
+template <int U>
+struct SellRowsOffU {
     static constexpr bool kTile = false;
     static constexpr int kParam = 64;
     template <typename AT, typename VT, typename XT, class Epi>
@@ -188,7 +196,7 @@ struct SellRowsOff {
     {
         if (A.sell_off == nullptr) {
             // columns do not fit 16-bit offsets: the full-column slices
-            SellRows::run<AT, VT, XT, Epi>(A, csr_val, x, epi);
+            SellRowsU<U>::template run<AT, VT, XT, Epi>(A, csr_val, x, epi);
             return;
         }
         const VT *__restrict__ val = sell_values(A, csr_val);
@@ -202,15 +210,18 @@ struct SellRowsOff {
             const int32_t sb = (int32_t)(s * kWave);
             AT acc = 0;
             uint32_t j = b + lane;
-            for (; j + 3 * kWave < e; j += 4 * kWave) {
-                int32_t c0 = sb + ld(&off[j]), c1 = sb + ld(&off[j + kWave]);
-                int32_t c2 = sb + ld(&off[j + 2 * kWave]), c3 = sb + ld(&off[j + 3 * kWave]);
-                VT v0 = ld(&val[j]), v1 = ld(&val[j + kWave]), v2 = ld(&val[j + 2 * kWave]), v3 = ld(&val[j + 3 * kWave]);
-                XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-                acc += (AT)v0 * (AT)x0;
-                acc += (AT)v1 * (AT)x1;
-                acc += (AT)v2 * (AT)x2;
-                acc += (AT)v3 * (AT)x3;
+            for (; j + (U - 1) * kWave < e; j += U * kWave) {
+                int32_t c[U];
+                VT v[U];
+                XT y[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) c[u] = sb + ld(&off[j + u * kWave]);
+#pragma unroll
+                for (int u = 0; u < U; u++) v[u] = ld(&val[j + u * kWave]);
+#pragma unroll
+                for (int u = 0; u < U; u++) y[u] = x[c[u]];
+#pragma unroll
+                for (int u = 0; u < U; u++) acc += (AT)v[u] * (AT)y[u];
             }
             for (; j < e; j += kWave) acc += (AT)ld(&val[j]) * (AT)x[sb + ld(&off[j])];
             const uint32_t r = s * kWave + lane;
@@ -218,6 +229,7 @@ struct SellRowsOff {
         }
     }
 };
+using SellRowsOff = SellRowsOffU<4>;
 
 // Launch plan of one matrix: which shape, and its grid.
 struct SpmvPlan {
@@ -226,8 +238,8 @@ struct SpmvPlan {
 };
 
 // Shape ids: 0..4 VecRows<4,8,16,32,64>, 5 TileRows<2048>, 6 TileRows<4096>, 7 SellRows,
-// 8 SellRowsOff
-constexpr int kNumShapes = 9;
+// 8 SellRowsOff, 9 SellRowsU<8>, 10 SellRowsOffU<8>
+constexpr int kNumShapes = 11;
 
 #define RS_SHAPE_SWITCH(id, KERNEL, grid, stream, ...)                                         \
     do {                                                                                       \
@@ -240,7 +252,9 @@ constexpr int kNumShapes = 9;
             case 5: KERNEL<::rs::TileRows<2048>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
             case 6: KERNEL<::rs::TileRows<4096>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
             case 7: KERNEL<::rs::SellRows><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;       \
-            default: KERNEL<::rs::SellRowsOff><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;    \
+            case 8: KERNEL<::rs::SellRowsOff><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;    \
+            case 9: KERNEL<::rs::SellRowsU<8>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break;   \
+            default: KERNEL<::rs::SellRowsOffU<8>><<<grid, ::rs::kBlock, 0, stream>>>(__VA_ARGS__); break; \
         }                                                                                      \
     } while (0)
 
