@@ -339,6 +339,66 @@ def batch_transient(rs, args) -> int:
     return 0
 
 
+def eigen(rs, args) -> int:
+    """Modal analysis of the heat operator (REigenValueSolver, Arnoldi): K x = lambda M x with K
+    the 128^3 x 6-tet heat stiffness (lognormal(0,1) conductivity, T fixed on x = 0 and x = 1),
+    M the consistent mass; args.eigen_modes eigenvalues, inner Jacobi-CG to the reference's
+    stopping rule at 1e-12.  GPU (rs_eigen_solve) and the oracle start from the same
+    srand(1) vector.  Prints one JSON line."""
+    import __graft_entry__ as ge
+    c, ne = args.eigen_cells, args.eigen_modes
+    xyz, tets = rs.kuhn_box(c, c, c)
+    x = xyz[:, 0]
+    fixed = ((np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12)).astype(np.uint8)
+    nt = tets.shape[0]
+    k = rs.field_lognormal(5, nt, 0.0, 1.0)
+    asm = rs.Assembly(xyz, tets, 1, fixed)
+    asm.assemble_heat(k, np.zeros(nt), np.zeros(xyz.shape[0]), rhoc=np.ones(nt), transient=True, alpha=1.0, dt=0.0)
+    Ms = asm.matrix.to_scipy()
+    asm.assemble_heat(k, np.zeros(nt), np.zeros(xyz.shape[0]))
+    Ks = asm.matrix.to_scipy()
+    M, K = rs.SparseMatrix.from_scipy(Ms), rs.SparseMatrix.from_scipy(Ks)
+    log(f"eigen: {c}^3, n={Ks.shape[0]}, nnz={Ks.nnz}, {ne} modes (Arnoldi)")
+    econf = rs.EigenValueSolverConf(method=rs.EIGEN_ARNOLDI, n_eigen_values=ne, n_iterations=500,
+                                    solver_cvg_value=1e-10)
+    mconf = rs.MatrixSolverConf(n_outer_iterations=100000, solver_cvg_value=1e-12)
+    times, d, info = [], None, None
+    for rep in range(1 + max(1, args.steps)):
+        rs.seed_c_rand(1)
+        S = rs.REigenValueSolver(econf, mconf)
+        t0 = time.perf_counter()
+        d, ev = S.solve(M, K)
+        dt_ = time.perf_counter() - t0
+        info = S.info
+        if rep > 0:
+            times.append(dt_)
+        log(f"gpu arnoldi: {dt_ * 1e3:.1f} ms, {info.cg_iterations} CG iterations")
+    gpu_s = float(np.median(times))
+    cpu = None
+    agree = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        orc.load()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        rs.seed_c_rand(1)
+        t0 = time.perf_counter()
+        d_ref, _, its_ref = orc.eigen_solve(Ms, Ks, 1, n_eigen=ne, n_iter=500, eig_cvg=1e-10, cvg=1e-12)
+        t_cpu = time.perf_counter() - t0
+        agree = float(np.max(np.abs(d - d_ref) / np.abs(d_ref)))
+        cpu = {"value": round(t_cpu, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": f"the oracle's Arnoldi on the same {c}^3 problem ({its_ref} CG iterations, {threads} threads)"}
+        log(f"cpu arnoldi: {t_cpu:.2f}s; eigenvalues agree to {agree:.2e}")
+    out = {"metric": f"modal analysis: time for {ne} eigenvalues (Arnoldi, REigenValueSolver)",
+           "value": round(gpu_s, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"heat operator {c}^3 cells x 6 tets, lognormal(0,1) conductivity, consistent mass",
+                      "unknowns": int(Ks.shape[0]), "nnz": int(Ks.nnz), "modes": ne},
+           "eigenvalues": [round(float(v), 6) for v in d], "cg_iterations": int(info.cg_iterations),
+           "max_relative_difference_vs_oracle": agree, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,9 +410,11 @@ def main() -> int:
     ap.add_argument("--seed-k", type=int, default=2, help="conductivity field seed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--asm-cells", type=int, default=40)
-    ap.add_argument("--workload", choices=["heat", "elasticity", "batch"], default="heat",
-                    help="heat (default, the headline), the elasticity cantilever, or the batched "
-                         "transient heat systems")
+    ap.add_argument("--eigen-cells", type=int, default=128)
+    ap.add_argument("--eigen-modes", type=int, default=10)
+    ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen"], default="heat",
+                    help="heat (default, the headline), the elasticity cantilever, the batched "
+                         "transient heat systems, or the modal analysis (Arnoldi)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--unbatched-budget", type=float, default=40.0,
@@ -381,6 +443,8 @@ def main() -> int:
         return elasticity(rs, args)
     if args.workload == "batch":
         return batch_transient(rs, args)
+    if args.workload == "eigen":
+        return eigen(rs, args)
     c = args.cells
     log(f"building {c}^3 Kuhn mesh and fields")
     xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma, args.seed_k)

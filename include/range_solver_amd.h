@@ -239,6 +239,34 @@ int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node)
 int rs_assembly_destroy(rs_assembly *S);
 
 /* ---------------------------------------------------------------- learned preconditioner */
+/* ---------------------------------------------------------------- eigenvalues
+ * REigenValueSolver (include/reigenvaluesolver.h:9-63, src/reigenvaluesolver.cpp:41-592): the
+ * generalized problem K x = lambda M x through K^-1 M (every inner solve is CG with Jacobi on the
+ * device, configured by the rs_conf as RMatrixSolverConf), methods as
+ * REigenValueSolverConf::Method.  LANCZOS returns ne values and no vectors, ARNOLDI ne values
+ * and ne vector rows of n, RAYLEIGH one value (1/lambda of the dominant mode of K^-1 M, as the
+ * reference returns it) and one vector.  With more than one value they are post-processed as
+ * the reference does: d = |1/d|, d[0] and d[1] swapped, sorted ascending, vector rows
+ * reordered by its swap loop.  The start vector (and Rayleigh's start shift) come from
+ * std::rand() / RAND_MAX like the reference's, so srand() by the caller fixes them.  The
+ * reference's QL step returns the Lanczos diagonal unrotated (its split search exits at once,
+ * src/reigenvaluesolver.cpp:440-452); that is reproduced. */
+typedef enum rs_eigen_method { RS_EIGEN_LANCZOS = 0, RS_EIGEN_ARNOLDI = 1, RS_EIGEN_RAYLEIGH = 2 } rs_eigen_method;
+typedef struct rs_eigen_conf {
+    int32_t method;               /* rs_eigen_method */
+    uint32_t n_eigen_values;      /* REigenValueSolverConf::nEigenValues */
+    uint32_t n_iterations;        /* QR iterations (Arnoldi) / Rayleigh iterations */
+    double solver_cvg_value;      /* QR subdiagonal tolerance / Rayleigh |mu - mu_old| */
+} rs_eigen_conf;
+typedef struct rs_eigen_info {
+    uint32_t outer_iterations;    /* Lanczos/Arnoldi steps or Rayleigh iterations */
+    uint64_t cg_iterations;       /* inner CG iterations in total */
+    double solve_ms;              /* device time of the iterations */
+} rs_eigen_info;
+int rs_eigen_solve(const rs_eigen_conf *econf, const rs_conf *mconf, rs_matrix *M, rs_matrix *K, double *d,
+                   uint32_t d_capacity, uint32_t *n_d, double *ev, uint64_t ev_capacity, uint32_t *n_ev_rows,
+                   rs_eigen_info *info);
+
 /* ---------------------------------------------------------------- batched systems
  * Many independent systems solved together (no counterpart in the reference, which solves
  * one system per RMatrixSolver::solve call, src/rmatrixsolver.cpp:158-201).  Each system runs

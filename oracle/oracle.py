@@ -59,6 +59,9 @@ def load() -> C.CDLL:
     lib.or_solve.argtypes = [C.c_uint, _P, _P, _P, _P, _P, C.c_int, C.c_uint, C.POINTER(OrConf),
                              C.c_double, C.POINTER(OrInfo), _P, C.c_uint]
     lib.or_spmv.argtypes = [C.c_uint, _P, _P, _P, _P, _P]
+    lib.or_eigen_solve.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_double, C.POINTER(OrConf), C.c_uint, _P, _P,
+                                   _P, _P, _P, C.POINTER(C.c_uint), _P, C.POINTER(C.c_uint),
+                                   C.POINTER(C.c_ulonglong)]
     lib.or_spmv.restype = None
     lib.or_node_book.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint, _P, _P]
     lib.or_node_book.restype = C.c_uint32
@@ -228,4 +231,41 @@ def transient_heat(xyz, tets, k, q, rhoc, t_node0, book, n_dof, dt, steps, cvg=1
         xs.append(x)
         its.append(int(info["iterations"]))
     return xs, its
+
+
+def eigen_solve(M, K, method, n_eigen=1, n_iter=100, eig_cvg=1e-10, cvg=1e-13, n_outer=100000,
+                stop_rule=STOP_REFERENCE):
+    """REigenValueSolver::solve restated (src/reigenvaluesolver.cpp:41-592).  M and K are put
+    on one (union) pattern.  Returns (d, ev rows or None, inner CG iterations)."""
+    import scipy.sparse as sp
+    M, K = M.tocsr(), K.tocsr()
+    M.sort_indices()
+    K.sort_indices()
+    if np.array_equal(M.indptr, K.indptr) and np.array_equal(M.indices, K.indices):
+        P, mv, kv = M, M.data.astype(np.float64), K.data.astype(np.float64)
+    else:
+        # the union pattern, both matrices' values on it (explicit zeros elsewhere)
+        P = (abs(M) + abs(K)).tocsr()
+        P.sort_indices()
+        coo = P.tocoo()
+
+        def on(A):
+            A = A.tocoo()
+            pos = {(int(r), int(c)): v for r, c, v in zip(A.row, A.col, A.data)}
+            return np.array([pos.get((int(r), int(c)), 0.0) for r, c in zip(coo.row, coo.col)], np.float64)
+        mv, kv = on(M), on(K)
+    rp, col = P.indptr.astype(np.uint32), P.indices.astype(np.uint32)
+    n = P.shape[0]
+    ne = max(1, min(n_eigen, n))
+    d = np.zeros(ne)
+    ev = np.zeros(ne * n)
+    nd, nr = C.c_uint(), C.c_uint()
+    its = C.c_ulonglong()
+    conf = OrConf(SOLVER_CG, n_outer, 1000, cvg, stop_rule)
+    rc = load().or_eigen_solve(method, n_eigen, n_iter, eig_cvg, C.byref(conf), n, _p(rp), _p(col), _p(mv),
+                               _p(kv), _p(d), C.byref(nd), _p(ev), C.byref(nr), C.byref(its))
+    if rc != 0:
+        raise ValueError(f"oracle eigen solve failed ({rc})")
+    rows = ev[: nr.value * n].reshape(nr.value, n) if nr.value else None
+    return d[: nd.value], rows, its.value
 

@@ -773,3 +773,247 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ REigenValueSolver
+ * src/reigenvaluesolver.cpp:41-592: K x = lambda M x through K^-1 M.  Start vectors from
+ * rand() / RAND_MAX as the reference; inner solves by or_solve (CG, Jacobi) with the conf,
+ * warm-started from the vector's previous contents, RIterationInfo's error carried from one
+ * solve to the next (one RMatrixSolver object). */
+static double vdot(unsigned n, const double *a, const double *b)
+{
+    double s = 0.0;
+    for (unsigned i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+static int eig_inner(unsigned n, const uint32_t *rp, const uint32_t *col, const double *val, const double *b,
+                     double *x, const or_conf *conf, double *prev_error, unsigned long long *its)
+{
+    or_info info;
+    int rc = or_solve(n, rp, col, val, b, x, OR_PRECOND_JACOBI, 1, conf, *prev_error, &info, NULL, 0);
+    *prev_error = info.error;
+    *its += info.iterations;
+    return rc;
+}
+
+static void random_unit(unsigned n, double *v)
+{
+    for (unsigned i = 0; i < n; i++) v[i] = (double)rand() / (double)RAND_MAX;
+    double nrm = sqrt(vdot(n, v, v));
+    for (unsigned i = 0; i < n; i++) v[i] /= nrm;
+}
+
+/* qlDecomposition, :420-496: the split search starts at m = l and the loop exits on
+ * "m >= l" at once, so only the renumbering of e takes effect. */
+static void ql_reference(unsigned n, double *d, double *e)
+{
+    (void)d;
+    for (unsigned i = 1; i < n; i++) e[i - 1] = e[i];
+    if (n) e[n - 1] = 0.0;
+}
+
+/* qrDecomposition, :498-592 */
+static int qr_reference(unsigned n, const double *H, unsigned n_iter, double cvg, double *d, double *V)
+{
+    size_t nn = (size_t)n * n;
+    double *A = malloc(sizeof(double) * nn), *Q = calloc(nn, sizeof(double)), *R = calloc(nn, sizeof(double));
+    double *a = malloc(sizeof(double) * n), *e = malloc(sizeof(double) * n), *u = malloc(sizeof(double) * n);
+    double *T = malloc(sizeof(double) * nn);
+    memcpy(A, H, sizeof(double) * nn);
+    memset(V, 0, sizeof(double) * nn);
+    int rc = 0;
+    for (unsigned it = 0; it < n_iter; it++) {
+        for (unsigned i = 0; i < n; i++) {
+            for (unsigned j = 0; j < n; j++) a[j] = A[(size_t)j * n + i];
+            memcpy(u, a, sizeof(double) * n);
+            for (unsigned j = 0; j < i; j++) {
+                for (unsigned k = 0; k < n; k++) e[k] = Q[(size_t)k * n + j];
+                double dot = vdot(n, a, e);
+                for (unsigned k = 0; k < n; k++) u[k] -= dot * e[k];
+            }
+            double norm = sqrt(vdot(n, u, u));
+            if (norm == 0.0) { rc = -1; goto done; }
+            for (unsigned k = 0; k < n; k++) e[k] = u[k] * (1.0 / norm);
+            for (unsigned j = 0; j < n; j++) Q[(size_t)j * n + i] = e[j];
+        }
+        for (unsigned i = 0; i < n; i++) {
+            for (unsigned j = 0; j < n; j++) e[j] = Q[(size_t)j * n + i];
+            for (unsigned j = i; j < n; j++) {
+                for (unsigned k = 0; k < n; k++) a[k] = A[(size_t)k * n + j];
+                R[(size_t)i * n + j] = vdot(n, a, e);
+            }
+        }
+        for (unsigned i = 0; i < n; i++)
+            for (unsigned j = 0; j < n; j++) {
+                double sum = 0.0;
+                for (unsigned k = 0; k < n; k++) sum += R[(size_t)i * n + k] * Q[(size_t)k * n + j];
+                A[(size_t)i * n + j] = sum;
+            }
+        if (it == 0) {
+            memcpy(V, Q, sizeof(double) * nn);
+        } else {
+            memcpy(T, V, sizeof(double) * nn);
+            for (unsigned i = 0; i < n; i++)
+                for (unsigned j = 0; j < n; j++) {
+                    double sum = 0.0;
+                    for (unsigned k = 0; k < n; k++) sum += T[(size_t)i * n + k] * Q[(size_t)k * n + j];
+                    V[(size_t)i * n + j] = sum;
+                }
+        }
+        int converged = 1;
+        for (unsigned i = 1; i < n; i++)
+            if (fabs(A[(size_t)i * n + i - 1]) > cvg) converged = 0;
+        if (converged) break;
+    }
+    for (unsigned i = 0; i < n; i++) d[i] = A[(size_t)i * n + i];
+done:
+    free(A); free(Q); free(R); free(a); free(e); free(u); free(T);
+    return rc;
+}
+
+int or_eigen_solve(int method, unsigned n_eigen, unsigned n_iter, double eig_cvg, const or_conf *conf, unsigned n,
+                   const uint32_t *rp, const uint32_t *col, const double *mval, const double *kval, double *d,
+                   unsigned *n_d, double *ev, unsigned *n_ev_rows, unsigned long long *cg_its)
+{
+    double prev = 0.0;
+    unsigned long long its = 0;
+    unsigned nrows = 0, nd = 0;
+    int rc = 0;
+    if (n == 0) return -1;
+    if (method == 0) {                                   /* solveLanczos :119-205 */
+        unsigned ne = n_eigen < n ? n_eigen : n;
+        double *v = calloc(n, sizeof(double)), *vo = calloc(n, sizeof(double)), *w = calloc(n, sizeof(double));
+        double *b = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n), *e = calloc(ne ? ne : 1, sizeof(double));
+        random_unit(n, v);
+        for (unsigned i = 0; i < ne; i++) {
+            or_spmv(n, rp, col, mval, v, b);
+            if (i > 0) {
+                or_spmv(n, rp, col, kval, vo, t);
+                for (unsigned j = 0; j < n; j++) t[j] *= e[i];
+                for (unsigned j = 0; j < n; j++) b[j] -= t[j];
+            }
+            if ((rc = eig_inner(n, rp, col, kval, b, w, conf, &prev, &its))) break;
+            d[i] = vdot(n, w, v);
+            if (i + 1 < ne) {
+                for (unsigned j = 0; j < n; j++) w[j] -= d[i] * v[j];
+                e[i + 1] = sqrt(vdot(n, w, w));
+                if (e[i + 1] == 0.0) { rc = -4; break; }
+                memcpy(vo, v, sizeof(double) * n);
+                for (unsigned j = 0; j < n; j++) v[j] = w[j] / e[i + 1];
+            }
+        }
+        if (!rc) ql_reference(ne, d, e);
+        nd = ne;
+        free(v); free(vo); free(w); free(b); free(t); free(e);
+    } else if (method == 1) {                            /* solveArnoldi :207-287 */
+        unsigned ne = n_eigen < n ? n_eigen : n;
+        double *Qa = calloc((size_t)n * ne, sizeof(double)), *q = calloc(n, sizeof(double)), *f = malloc(sizeof(double) * n);
+        double *H = calloc((size_t)ne * ne, sizeof(double)), *Qq = malloc(sizeof(double) * ne * ne);
+        random_unit(n, q);
+        memcpy(Qa, q, sizeof(double) * n);               /* column 0 (column-major here) */
+        for (unsigned i = 1; i < ne + 1 && !rc; i++) {
+            or_spmv(n, rp, col, mval, Qa + (size_t)(i - 1) * n, f);
+            if ((rc = eig_inner(n, rp, col, kval, f, q, conf, &prev, &its))) break;
+            for (unsigned j = 0; j < i; j++) {
+                const double *qj = Qa + (size_t)j * n;
+                double h = vdot(n, qj, q);
+                H[(size_t)j * ne + i - 1] = h;
+                for (unsigned k = 0; k < n; k++) q[k] -= h * qj[k];
+            }
+            if (i < ne) {
+                double h = sqrt(vdot(n, q, q));
+                if (h == 0.0) { rc = -4; break; }
+                H[(size_t)i * ne + i - 1] = h;
+                for (unsigned k = 0; k < n; k++) Qa[(size_t)i * n + k] = q[k] / h;
+            }
+        }
+        if (!rc) rc = qr_reference(ne, H, n_iter, eig_cvg, d, Qq);
+        if (!rc && ev)
+            for (unsigned i = 0; i < ne; i++)
+                for (unsigned k = 0; k < n; k++) {
+                    double sum = 0.0;
+                    for (unsigned j = 0; j < ne; j++) sum += Qa[(size_t)j * n + k] * Qq[(size_t)j * ne + i];
+                    ev[(size_t)i * n + k] = sum;
+                }
+        nd = ne;
+        nrows = ne;
+        free(Qa); free(q); free(f); free(H); free(Qq);
+    } else if (method == 2) {                            /* solveRayleigh :289-418 (K, M share rp/col) */
+        size_t nnz = rp[n];
+        double *m2 = malloc(sizeof(double) * (nnz ? nnz : 1));
+        double *b = malloc(sizeof(double) * n), *f = malloc(sizeof(double) * n), *c = calloc(n, sizeof(double));
+        double *dd = calloc(n, sizeof(double));
+        memcpy(m2, mval, sizeof(double) * nnz);
+        random_unit(n, b);
+        double mu = 1.0e9 * (double)rand() / (double)RAND_MAX;
+        for (unsigned it = 0; it < n_iter; it++) {
+            double muo = mu;
+            for (unsigned i = 0; i < n; i++) {
+                f[i] = 0.0;
+                for (uint32_t j = rp[i]; j < rp[i + 1]; j++) {
+                    double value = kval[j] * mu;
+                    if (value != 0.0) m2[j] += value;
+                    f[i] += value * b[col[j]];
+                }
+            }
+            if ((rc = eig_inner(n, rp, col, m2, f, c, conf, &prev, &its))) break;
+            double norm = sqrt(vdot(n, c, c));
+            if (norm == 0.0) { rc = -4; break; }
+            for (unsigned i = 0; i < n; i++) f[i] *= 1.0 / norm;
+            if ((rc = eig_inner(n, rp, col, mval, f, b, conf, &prev, &its))) break;
+            for (unsigned i = 0; i < n; i++)
+                for (uint32_t j = rp[i]; j < rp[i + 1]; j++) {
+                    double value = -1.0 * kval[j] * mu;
+                    if (value != 0.0) m2[j] += value;
+                }
+            for (unsigned i = 0; i < n; i++) {
+                f[i] = 0.0;
+                for (uint32_t j = rp[i]; j < rp[i + 1]; j++) f[i] += m2[j] * b[col[j]];
+            }
+            if ((rc = eig_inner(n, rp, col, kval, f, dd, conf, &prev, &its))) break;
+            double bb = vdot(n, b, b);
+            if (bb == 0.0) { rc = -4; break; }
+            mu = vdot(n, dd, b) / bb;
+            if (it > 0 && fabs(muo - mu) < eig_cvg) break;
+        }
+        d[0] = mu;
+        if (ev) memcpy(ev, b, sizeof(double) * n);
+        nd = 1;
+        nrows = 1;
+        free(m2); free(b); free(f); free(c); free(dd);
+    } else {
+        return -2;
+    }
+    if (rc) return rc;
+    /* REigenValueSolver::solve post-processing :89-116 (RUtil::qSort: ascending, stable) */
+    if (nd > 1) {
+        for (unsigned i = 0; i < nd; i++) d[i] = fabs(1.0 / d[i]);
+        double tmp = d[0]; d[0] = d[1]; d[1] = tmp;
+        unsigned *idx = malloc(sizeof(unsigned) * nd);
+        double *sorted = malloc(sizeof(double) * nd);
+        for (unsigned i = 0; i < nd; i++) idx[i] = i;
+        for (unsigned i = 1; i < nd; i++) {              /* insertion sort: stable */
+            unsigned v = idx[i], j = i;
+            while (j > 0 && d[idx[j - 1]] > d[v]) { idx[j] = idx[j - 1]; j--; }
+            idx[j] = v;
+        }
+        for (unsigned i = 0; i < nd; i++) sorted[i] = d[idx[i]];
+        memcpy(d, sorted, sizeof(double) * nd);
+        if (ev)
+            for (unsigned i = 0; i < nrows; i++) {
+                unsigned n1 = i, n2 = idx[i];
+                for (unsigned j = 0; j < n; j++) {
+                    double t = ev[(size_t)n1 * n + j];
+                    ev[(size_t)n1 * n + j] = ev[(size_t)n2 * n + j];
+                    ev[(size_t)n2 * n + j] = t;
+                }
+                idx[n2] = n2;
+            }
+        free(idx); free(sorted);
+    }
+    *n_d = nd;
+    *n_ev_rows = nrows;
+    if (cg_its) *cg_its = its;
+    return 0;
+}
+

@@ -112,6 +112,13 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
                        uint32_t ntri, const uint32_t *tris, const double *traction,
                        const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b);
 
+/* REigenValueSolver::solve, src/reigenvaluesolver.cpp:41-592.  method 0 Lanczos, 1 Arnoldi,
+ * 2 Rayleigh (K and M share rp/col).  d holds min(n_eigen, n) values (1 for Rayleigh); ev
+ * (optional) n_ev_rows rows of n.  Start vectors from rand() / RAND_MAX. */
+int or_eigen_solve(int method, unsigned n_eigen, unsigned n_iter, double eig_cvg, const or_conf *conf, unsigned n,
+                   const uint32_t *rp, const uint32_t *col, const double *mval, const double *kval, double *d,
+                   unsigned *n_d, double *ev, unsigned *n_ev_rows, unsigned long long *cg_its);
+
 #ifdef __cplusplus
 }
 #endif

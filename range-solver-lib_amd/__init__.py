@@ -68,6 +68,15 @@ class Info(C.Structure):
         return d
 
 
+class EigenConf(C.Structure):
+    _fields_ = [("method", C.c_int32), ("n_eigen_values", C.c_uint32), ("n_iterations", C.c_uint32),
+                ("solver_cvg_value", C.c_double)]
+
+
+class EigenInfo(C.Structure):
+    _fields_ = [("outer_iterations", C.c_uint32), ("cg_iterations", C.c_uint64), ("solve_ms", C.c_double)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("rounds", C.c_uint32), ("max_iterations", C.c_uint32),
                 ("total_iterations", C.c_uint64), ("systems_converged", C.c_uint32),
@@ -152,6 +161,8 @@ _SIGS = {
     "rs_batch_info_layout": [_P, _u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_int32)],
     "rs_batch_set_replacement": [_P, C.c_double],
     "rs_batch_destroy": [_P],
+    "rs_eigen_solve": [C.POINTER(EigenConf), C.POINTER(Conf), _P, _P, _P, C.c_uint32, _u32p, _P, C.c_uint64,
+                       _u32p, C.POINTER(EigenInfo)],
 }
 
 _lib = None
@@ -565,6 +576,53 @@ class Assembly:
         if _lib is not None and getattr(self, "h", None):
             _lib.rs_assembly_destroy(self.h)
             self.h = None
+
+
+# ------------------------------------------------------------------------------ eigenvalues
+EIGEN_LANCZOS, EIGEN_ARNOLDI, EIGEN_RAYLEIGH = 0, 1, 2
+
+
+@dataclass
+class EigenValueSolverConf:
+    """REigenValueSolverConf as the reference uses it (method, nEigenValues, nIterations,
+    solverCvgValue)."""
+    method: int = EIGEN_ARNOLDI
+    n_eigen_values: int = 1
+    n_iterations: int = 100
+    solver_cvg_value: float = 1e-10
+
+
+class REigenValueSolver:
+    """REigenValueSolver(conf, matrixSolverConf).solve(M, K, d, ev) -- include/reigenvaluesolver.h:9-63,
+    on the GPU (rs_eigen_solve).  Returns (d, ev rows or None); info in self.info."""
+
+    def __init__(self, conf: EigenValueSolverConf, matrix_conf: MatrixSolverConf):
+        load()
+        self.conf = conf
+        self.matrix_conf = matrix_conf
+        self.info: EigenInfo | None = None
+
+    def solve(self, M: SparseMatrix, K: SparseMatrix):
+        n = M.n
+        ne = max(1, min(self.conf.n_eigen_values, n))
+        d = np.zeros(ne)
+        ev = np.zeros(ne * n)
+        nd, nr = C.c_uint32(), C.c_uint32()
+        ec = EigenConf(self.conf.method, self.conf.n_eigen_values, self.conf.n_iterations,
+                       self.conf.solver_cvg_value)
+        mc = self.matrix_conf.to_c()
+        info = EigenInfo()
+        _check(_lib.rs_eigen_solve(C.byref(ec), C.byref(mc), M.h, K.h, _ptr(d), ne, C.byref(nd), _ptr(ev),
+                                   ev.size, C.byref(nr), C.byref(info)))
+        self.info = info
+        rows = ev[: nr.value * n].reshape(nr.value, n) if nr.value else None
+        return d[: nd.value], rows
+
+
+def seed_c_rand(seed: int) -> None:
+    """srand() of the process's C library: fixes the start vectors of REigenValueSolver, which
+    come from rand() / RAND_MAX as in the reference."""
+    C.CDLL(None).srand(C.c_uint(seed))
 
 
 # ------------------------------------------------------------------------------ batched systems
