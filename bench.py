@@ -195,13 +195,26 @@ def elasticity(rs, args) -> int:
                      "true_relres": info.true_relres, "reason": rs.REASON_NAMES.get(info.reason),
                      "solution_error_vs_fp64": float(np.linalg.norm(x - x_star) / np.linalg.norm(x_star))}
         log(f"{name}: {res[name]['time_s'] * 1e3:.1f} ms, {info.iterations} its, relres {info.true_relres:.2e}")
+    # post-processing (RSolverStress::process): stresses and node forces from the solution
+    du = rs.DeviceArray.from_host(np.zeros(3 * nn))
+    asm.dofs_to_nodes(dx, du)
+    dE, dnu = rs.DeviceArray.from_host(E), rs.DeviceArray.from_host(nu)
+    dN, dS, dV, dF = rs.DeviceArray(ne), rs.DeviceArray(ne), rs.DeviceArray(ne), rs.DeviceArray(3 * nn)
+    sp_ms = []
+    for _ in range(6):
+        asm.stress_process_device(dE, dnu, du, dN, dS, dV, dF)
+        sp_ms.append(asm.last_ms)
+    spm = float(np.median(sp_ms[1:]))
+    vmax = float(dV.download().max())
+    log(f"stress processing: {spm:.3f} ms, max von Mises {vmax:.4g}")
     out = {"metric": "time-to-1e-8-residual per system (elasticity, block-Jacobi-3 CG)",
            "value": res["mixed_blockjacobi3_cg"]["time_s"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
            "dtype": "f64 / f32 SpMV+precond with f64 residuals", "data": "synthetic",
            "config": {"workload": f"cantilever {nx}x{ny}x{nz} cells x 6 tets, E lognormal(0,1), nu 0.3",
                       "nodes": int(nn), "tets": int(ne), "dofs": int(n), "nnz": int(asm.nnz)},
            "assembly_ms": round(asm.last_ms, 3), "methods": res,
-           "speedup_mixed_vs_fp64": round(res["fp64_blockjacobi3_cg"]["time_s"] / res["mixed_blockjacobi3_cg"]["time_s"], 3)}
+           "speedup_mixed_vs_fp64": round(res["fp64_blockjacobi3_cg"]["time_s"] / res["mixed_blockjacobi3_cg"]["time_s"], 3),
+           "post_processing": {"stress_process_ms": round(spm, 4), "max_von_mises": vmax}}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -598,6 +611,20 @@ def main() -> int:
                 "kernel": f"rs::k_spmv FP64 CSR SpMV ({spmv['fp64']['shape']})",
                 "algorithmic_bytes": spmv["fp64"]["bytes"]}
 
+    # post-processing (RSolverHeat::process): heat flux per tet from the solved temperatures
+    dTn = rs.DeviceArray.from_host(t_node)
+    asm.dofs_to_nodes(dx, dTn)
+    dflux = rs.DeviceArray(3 * tets.shape[0])
+    fl_ms = []
+    for _ in range(6):
+        asm.heat_flux_device(dk, dTn, dflux)
+        fl_ms.append(asm.last_ms)
+    fl = float(np.median(fl_ms[1:]))
+    fl_bytes = tets.shape[0] * (16 + 4 + 8 + 24) + xyz.shape[0] * (24 + 8)
+    post = {"heat_flux_ms": round(fl, 4), "algorithmic_bytes": fl_bytes,
+            "gbs": round(fl_bytes / (fl * 1e-3) / 1e9, 1)}
+    log(f"heat flux: {fl:.3f} ms ({post['gbs']} GB/s algorithmic)")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle sample)")
@@ -622,7 +649,7 @@ def main() -> int:
                     "speedup_mixed_vs_fp64": round(m64["time_s"] / mmp["time_s"], 3),
                     "speedup_learned_vs_fp64": round(m64["time_s"] / ml["time_s"], 3)},
         "learned_ablation": ablation,
-        "spmv": spmv, "roofline": roofline, "cpu_baseline": cpu,
+        "spmv": spmv, "roofline": roofline, "cpu_baseline": cpu, "post_processing": post,
     }
     print(json.dumps(out), flush=True)
     return 0

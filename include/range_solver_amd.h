@@ -236,6 +236,16 @@ int rs_assemble_stress(rs_assembly *S, const double *E, const double *nu, const 
 int rs_assembly_set_method(rs_assembly *S, int32_t row_gather);
 /* Write-back (src/rsolverheat.cpp:473-480): node[i*dpn+c] = x[book] where enabled. */
 int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node);
+/* Post-processing of the volume elements (device arrays, element outputs in the caller's element
+ * order):
+ * RSolverHeat::process, src/rsolverheat.cpp:646-692 -- 3 flux values per tet, q = -sum_m B_m k T_m
+ * with B_m = grad N_m * detJ (the reference's integration of grad N detJ / nInp). */
+int rs_heat_flux_device(rs_assembly *S, const double *d_k, const double *d_t_node, double *d_flux);
+/* RSolverStress::process, src/rsolverstress.cpp:1238-1408, static case without thermal load --
+ * per tet the normal and shear stress measures and their sum (the reference's von Mises), and the
+ * node forces sum_e Ke u_e (3 per node; reactions on the fixed nodes). */
+int rs_stress_process_device(rs_assembly *S, const double *d_E, const double *d_nu, const double *d_u_node,
+                             double *d_normal, double *d_shear, double *d_von_mises, double *d_node_force);
 int rs_assembly_destroy(rs_assembly *S);
 
 /* ---------------------------------------------------------------- learned preconditioner */

@@ -59,6 +59,8 @@ def load() -> C.CDLL:
     lib.or_solve.argtypes = [C.c_uint, _P, _P, _P, _P, _P, C.c_int, C.c_uint, C.POINTER(OrConf),
                              C.c_double, C.POINTER(OrInfo), _P, C.c_uint]
     lib.or_spmv.argtypes = [C.c_uint, _P, _P, _P, _P, _P]
+    lib.or_heat_flux.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P]
+    lib.or_stress_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.or_eigen_solve.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_double, C.POINTER(OrConf), C.c_uint, _P, _P,
                                    _P, _P, _P, C.POINTER(C.c_uint), _P, C.POINTER(C.c_uint),
                                    C.POINTER(C.c_ulonglong)]
@@ -268,4 +270,27 @@ def eigen_solve(M, K, method, n_eigen=1, n_iter=100, eig_cvg=1e-10, cvg=1e-13, n
         raise ValueError(f"oracle eigen solve failed ({rc})")
     rows = ev[: nr.value * n].reshape(nr.value, n) if nr.value else None
     return d[: nd.value], rows, its.value
+
+
+def heat_flux(xyz, tets, k, t_node):
+    """RSolverHeat::process, volume elements (src/rsolverheat.cpp:646-692): (ne, 3) fluxes."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    k, t_node = np.ascontiguousarray(k, np.float64), np.ascontiguousarray(t_node, np.float64)
+    ne = tets.size // 4
+    out = np.empty(3 * ne)
+    load().or_heat_flux(xyz.size // 3, _p(xyz), ne, _p(tets), _p(k), _p(t_node), _p(out))
+    return out.reshape(ne, 3)
+
+
+def stress_process(xyz, tets, E, nu, u_node):
+    """RSolverStress::process, volume elements, static (src/rsolverstress.cpp:1238-1408):
+    (normal, shear, von Mises as the reference sums them, node forces (nn, 3))."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    E, nu, u = (np.ascontiguousarray(a, np.float64).reshape(-1) for a in (E, nu, u_node))
+    nn, ne = xyz.size // 3, tets.size // 4
+    nrm, sh, vm, f = np.empty(ne), np.empty(ne), np.empty(ne), np.empty(3 * nn)
+    load().or_stress_process(nn, _p(xyz), ne, _p(tets), _p(E), _p(nu), _p(u), _p(nrm), _p(sh), _p(vm), _p(f))
+    return nrm, sh, vm, f.reshape(nn, 3)
 

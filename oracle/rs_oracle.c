@@ -774,6 +774,108 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
     return 0;
 }
 
+/* ------------------------------------------------------------------ post-processing
+ * RSolverHeat::process, volume elements (src/rsolverheat.cpp:646-692): per tet
+ * B[m] = sum over the integration points of grad N_m * detJ / nInp; q = -sum_m B[m] k T_m. */
+int or_heat_flux(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *k,
+                 const double *t_node, double *flux)
+{
+    (void)nn;
+#pragma omp parallel for
+    for (int64_t e = 0; e < (int64_t)ne; e++) {
+        const uint32_t *nd = tets + 4 * e;
+        double X[4][3], J[3][3], B[4][3] = { { 0 } };
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) X[m][a] = xyz[3 * (size_t)nd[m] + a];
+        for (int ip = 0; ip < 4; ip++) {
+            double detJ = tet_jacobian(X, J);
+            for (int m = 0; m < 4; m++)
+                for (int a = 0; a < 3; a++)
+                    B[m][a] += (TET_DN[m][0] * J[a][0] + TET_DN[m][1] * J[a][1] + TET_DN[m][2] * J[a][2]) * detJ / 4.0;
+        }
+        double Q[3] = { 0.0, 0.0, 0.0 };
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) Q[a] -= B[m][a] * k[e] * t_node[nd[m]];
+        for (int a = 0; a < 3; a++) flux[3 * (size_t)e + a] = Q[a];
+    }
+    return 0;
+}
+
+/* RSolverStress::process, volume elements, static case (src/rsolverstress.cpp:1238-1408):
+ * Ke and the element stresses Qe = sum_ip sum_m BeD^T u_m / nInp (no thermal load), normal
+ * sqrt(sx^2 + sy^2 + sz^2 - (sx sy + sy sz + sz sx)), shear sqrt(3 (tyz^2 + txz^2 + txy^2)),
+ * "von Mises" = normal + shear as the reference sums them, node forces += Ke u_e. */
+int or_stress_process(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *E,
+                      const double *nu, const double *u_node, double *normal, double *shear, double *vm,
+                      double *node_force)
+{
+    memset(node_force, 0, sizeof(double) * 3 * (size_t)nn);
+    for (int64_t e = 0; e < (int64_t)ne; e++) {
+        const uint32_t *nd = tets + 4 * e;
+        double X[4][3], J[3][3], B[4][3];
+        double De[6][6] = { { 0 } }, Be[12][6], BeD[12][6];
+        double Ke[144] = { 0 }, Qe[6] = { 0 }, xe[12];
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) {
+                X[m][a] = xyz[3 * (size_t)nd[m] + a];
+                xe[3 * m + a] = u_node[3 * (size_t)nd[m] + a];
+            }
+        double Ee = E[e], v = nu[e];
+        De[0][0] = 1.0 - v; De[0][1] = v;       De[0][2] = v;
+        De[1][0] = v;       De[1][1] = 1.0 - v; De[1][2] = v;
+        De[2][0] = v;       De[2][1] = v;       De[2][2] = 1.0 - v;
+        De[3][3] = De[4][4] = De[5][5] = (1.0 - 2.0 * v) / 2.0;
+        double f = Ee / ((1.0 + v) * (1.0 - 2.0 * v));
+        for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) De[i][j] *= f;
+        for (int ip = 0; ip < 4; ip++) {
+            double detJ = tet_jacobian(X, J);
+            for (int m = 0; m < 4; m++)
+                for (int a = 0; a < 3; a++)
+                    B[m][a] = TET_DN[m][0] * J[a][0] + TET_DN[m][1] * J[a][1] + TET_DN[m][2] * J[a][2];
+            memset(Be, 0, sizeof(Be));
+            for (int m = 0; m < 4; m++) {
+                Be[3 * m + 0][0] = B[m][0];
+                Be[3 * m + 1][1] = B[m][1];
+                Be[3 * m + 2][2] = B[m][2];
+                Be[3 * m + 1][3] = B[m][2]; Be[3 * m + 2][3] = B[m][1];
+                Be[3 * m + 0][4] = B[m][2]; Be[3 * m + 2][4] = B[m][0];
+                Be[3 * m + 0][5] = B[m][1]; Be[3 * m + 1][5] = B[m][0];
+            }
+            for (int i = 0; i < 12; i++)
+                for (int j = 0; j < 6; j++) {
+                    double sum = 0.0;
+                    for (int l = 0; l < 6; l++) sum += Be[i][l] * De[l][j];
+                    BeD[i][j] = sum;
+                }
+            for (int i = 0; i < 12; i++)
+                for (int j = 0; j < 12; j++) {
+                    double sum = 0.0;
+                    for (int l = 0; l < 6; l++) sum += BeD[i][l] * Be[j][l];
+                    Ke[i * 12 + j] += sum * detJ * TET_W;
+                }
+            double iv = 1.0 / 4.0;
+            for (int m = 0; m < 4; m++)
+                for (int n = 0; n < 6; n++)
+                    Qe[n] += BeD[3 * m + 0][n] * xe[3 * m + 0] * iv + BeD[3 * m + 1][n] * xe[3 * m + 1] * iv +
+                             BeD[3 * m + 2][n] * xe[3 * m + 2] * iv;
+        }
+        double fe[12];
+        for (int i = 0; i < 12; i++) {
+            double sum = 0.0;
+            for (int j = 0; j < 12; j++) sum += Ke[i * 12 + j] * xe[j];
+            fe[i] = sum;
+        }
+        double QeN = sqrt(Qe[0] * Qe[0] + Qe[1] * Qe[1] + Qe[2] * Qe[2] - (Qe[0] * Qe[1] + Qe[1] * Qe[2] + Qe[2] * Qe[0]));
+        double QeS = sqrt(3.0 * (Qe[3] * Qe[3] + Qe[4] * Qe[4] + Qe[5] * Qe[5]));
+        normal[e] = QeN;
+        shear[e] = QeS;
+        vm[e] = QeN + QeS;
+        for (int m = 0; m < 4; m++)
+            for (int c = 0; c < 3; c++) node_force[3 * (size_t)nd[m] + c] += fe[3 * m + c];
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ REigenValueSolver
  * src/reigenvaluesolver.cpp:41-592: K x = lambda M x through K^-1 M.  Start vectors from
  * rand() / RAND_MAX as the reference; inner solves by or_solve (CG, Jacobi) with the conf,

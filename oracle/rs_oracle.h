@@ -112,6 +112,14 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
                        uint32_t ntri, const uint32_t *tris, const double *traction,
                        const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b);
 
+/* RSolverHeat::process (volume), src/rsolverheat.cpp:646-692: 3 flux values per tet. */
+int or_heat_flux(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *k,
+                 const double *t_node, double *flux);
+/* RSolverStress::process (volume, static), src/rsolverstress.cpp:1238-1408. */
+int or_stress_process(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *E,
+                      const double *nu, const double *u_node, double *normal, double *shear, double *vm,
+                      double *node_force);
+
 /* REigenValueSolver::solve, src/reigenvaluesolver.cpp:41-592.  method 0 Lanczos, 1 Arnoldi,
  * 2 Rayleigh (K and M share rp/col).  d holds min(n_eigen, n) values (1 for Rayleigh); ev
  * (optional) n_ev_rows rows of n.  Start vectors from rand() / RAND_MAX. */

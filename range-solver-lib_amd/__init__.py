@@ -129,6 +129,8 @@ _SIGS = {
     "rs_assemble_stress": [_P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P],
     "rs_assembly_dofs_to_nodes": [_P, _P, _P], "rs_assembly_destroy": [_P],
     "rs_assembly_set_method": [_P, C.c_int32],
+    "rs_heat_flux_device": [_P, _P, _P, _P],
+    "rs_stress_process_device": [_P, _P, _P, _P, _P, _P, _P, _P],
     "rs_aggregates_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                               _P, _u32p],
     "rs_hierarchy_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -568,6 +570,15 @@ class Assembly:
         _check(_lib.rs_assemble_stress(self.h, _ptr(E), _ptr(nu), _ptr(rho), _ptr(g),
                                        _ptr(u_node), ntri, _ptr(tr), _ptr(tt), _ptr(b)))
         return b
+
+    def heat_flux_device(self, dk: DeviceArray, dt_node: DeviceArray, dflux: DeviceArray) -> None:
+        """RSolverHeat::process (volume elements): 3 flux values per element."""
+        _check(_lib.rs_heat_flux_device(self.h, dk.ptr, dt_node.ptr, dflux.ptr))
+
+    def stress_process_device(self, dE, dnu, du_node, dnormal, dshear, dvm, dforce) -> None:
+        """RSolverStress::process (volume elements, static): normal, shear, their sum, node forces."""
+        _check(_lib.rs_stress_process_device(self.h, dE.ptr, dnu.ptr, du_node.ptr, dnormal.ptr, dshear.ptr,
+                                             dvm.ptr, dforce.ptr))
 
     def dofs_to_nodes(self, dx: DeviceArray, dnode: DeviceArray) -> None:
         _check(_lib.rs_assembly_dofs_to_nodes(self.h, dx.ptr, dnode.ptr))

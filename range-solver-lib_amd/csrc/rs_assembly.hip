@@ -274,6 +274,106 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
     h.b[i] = bi;
 }
 
+// ------------------------------------------------------------------ post-processing
+// The caller's element order of the connectivity (the assembly keeps it colour-sorted), for
+// the element-local post-processing: conn0[perm[s]] = conn[s].
+__global__ void k_unsort_conn(const uint4 *__restrict__ conn, const uint32_t *__restrict__ perm, uint32_t ne,
+                              uint4 *__restrict__ conn0)
+{
+    for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < ne; s += gridDim.x * kBlock) conn0[perm[s]] = conn[s];
+}
+
+// RSolverHeat::process, volume elements (src/rsolverheat.cpp:646-692): per tet the reference
+// integrates grad N_m * detJ / nInp over its points, so q = -detJ k grad T (detJ = 6 V, the
+// signed determinant of the reference-tet map).  One thread per element in the caller's order:
+// connectivity, conductivity and flux stream coalesced.
+__global__ __launch_bounds__(kBlock) void k_heat_flux(const double *__restrict__ xyz, const uint4 *__restrict__ conn0,
+                                                      uint32_t ne, const double *__restrict__ k,
+                                                      const double *__restrict__ t_node, double *__restrict__ flux)
+{
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= ne) return;
+    const uint4 nd4 = conn0[e];
+    const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+    double X[4][3], g[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) X[m][c] = xyz[3 * (uint64_t)nd[m] + c];
+    const double detJ = 6.0 * tet_gradients(X, g);
+    const uint32_t orig = e;
+    const double ke = k[orig];
+    double Q[3] = { 0.0, 0.0, 0.0 };
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const double T = t_node[nd[m]];
+#pragma unroll
+        for (int c = 0; c < 3; c++) Q[c] -= g[m][c] * detJ * ke * T;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) flux[3 * (uint64_t)orig + c] = Q[c];
+}
+
+// RSolverStress::process, volume elements, static case (src/rsolverstress.cpp:1238-1408):
+// element stresses sigma = De eps (Voigt xx, yy, zz, yz, xz, xy, engineering shear), the normal
+// and shear measures and their sum (the reference's "von Mises"), and node forces Ke u_e =
+// V Be sigma added to the element's nodes (one launch per colour: no shared nodes, a fixed
+// order).
+struct StressPostArgs {
+    const double *xyz;
+    const uint4 *conn;
+    const uint32_t *perm;
+    const double *E, *nu, *u;
+    double *normal, *shear, *vm, *force;
+};
+
+__global__ __launch_bounds__(kBlock) void k_stress_post(StressPostArgs a, uint32_t e0, uint32_t e1)
+{
+    const uint32_t e = e0 + blockIdx.x * kBlock + threadIdx.x;
+    if (e >= e1) return;
+    const uint4 nd4 = a.conn[e];
+    const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+    double X[4][3], g[4][3], u[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            X[m][c] = a.xyz[3 * (uint64_t)nd[m] + c];
+            u[m][c] = a.u[3 * (uint64_t)nd[m] + c];
+        }
+    const double V = tet_gradients(X, g);
+    const uint32_t orig = a.perm[e];
+    const double E = a.E[orig], v = a.nu[orig];
+    const double f = E / ((1.0 + v) * (1.0 - 2.0 * v));
+    double ex = 0.0, ey = 0.0, ez = 0.0, gyz = 0.0, gxz = 0.0, gxy = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        ex += g[m][0] * u[m][0];
+        ey += g[m][1] * u[m][1];
+        ez += g[m][2] * u[m][2];
+        gyz += g[m][2] * u[m][1] + g[m][1] * u[m][2];
+        gxz += g[m][2] * u[m][0] + g[m][0] * u[m][2];
+        gxy += g[m][1] * u[m][0] + g[m][0] * u[m][1];
+    }
+    const double sx = f * ((1.0 - v) * ex + v * ey + v * ez);
+    const double sy = f * (v * ex + (1.0 - v) * ey + v * ez);
+    const double sz = f * (v * ex + v * ey + (1.0 - v) * ez);
+    const double sh = f * (1.0 - 2.0 * v) / 2.0;
+    const double tyz = sh * gyz, txz = sh * gxz, txy = sh * gxy;
+    const double QeN = sqrt(sx * sx + sy * sy + sz * sz - (sx * sy + sy * sz + sz * sx));
+    const double QeS = sqrt(3.0 * (tyz * tyz + txz * txz + txy * txy));
+    a.normal[orig] = QeN;
+    a.shear[orig] = QeS;
+    a.vm[orig] = QeN + QeS;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        double *F = a.force + 3 * (uint64_t)nd[m];
+        F[0] += V * (g[m][0] * sx + g[m][2] * txz + g[m][1] * txy);
+        F[1] += V * (g[m][1] * sy + g[m][2] * tyz + g[m][0] * txy);
+        F[2] += V * (g[m][2] * sz + g[m][1] * tyz + g[m][0] * txz);
+    }
+}
+
 struct StressArgs {
     const double *xyz;
     const uint4 *conn;
@@ -405,6 +505,7 @@ struct rs_assembly {
     std::vector<uint32_t> colour_off;     // n_colours+1
     DBuf<double> xyz;
     DBuf<uint32_t> conn, perm, map, dbook;
+    DBuf<uint32_t> conn0;                   // connectivity in the caller's order (post-processing)
     DBuf<uint32_t> dof_node, inc_ptr, inc;  // row-gather heat assembly
     DBuf<double> rec_grad, rec_vol;         // row records (built once per mesh)
     DBuf<uint32_t> rec_elem, rec_slots, rec_count;
@@ -783,3 +884,44 @@ extern "C" int rs_assembly_destroy(rs_assembly *S)
     delete S;
     return RS_OK;
 }
+
+extern "C" int rs_heat_flux_device(rs_assembly *S, const double *d_k, const double *d_t_node, double *d_flux)
+{
+    if (!S || !d_k || !d_t_node || !d_flux) return fail(RS_ERR_INVALID_ARG, "rs_heat_flux_device: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipEvent_t ev[2];
+    if ((rc = timed_begin(ev))) return rc;
+    if (S->n_elems && !S->conn0.p) {
+        if ((rc = S->conn0.alloc(4ull * S->n_elems))) return rc;
+        k_unsort_conn<<<vblocks(S->n_elems), kBlock, 0, runtime().stream>>>(
+            reinterpret_cast<const uint4 *>(S->conn.p), S->perm.p, S->n_elems, reinterpret_cast<uint4 *>(S->conn0.p));
+    }
+    if (S->n_elems)
+        k_heat_flux<<<grid_for(S->n_elems, kBlock), kBlock, 0, runtime().stream>>>(
+            S->xyz.p, reinterpret_cast<const uint4 *>(S->conn0.p), S->n_elems, d_k, d_t_node, d_flux);
+    RS_HIP(hipGetLastError());
+    return timed_end(ev, S->last_ms);
+}
+
+extern "C" int rs_stress_process_device(rs_assembly *S, const double *d_E, const double *d_nu, const double *d_u_node,
+                                        double *d_normal, double *d_shear, double *d_von_mises, double *d_node_force)
+{
+    if (!S || !d_E || !d_nu || !d_u_node || !d_normal || !d_shear || !d_von_mises || !d_node_force)
+        return fail(RS_ERR_INVALID_ARG, "rs_stress_process_device: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    hipEvent_t ev[2];
+    if ((rc = timed_begin(ev))) return rc;
+    k_zero<<<vblocks(3ull * S->n_nodes), kBlock, 0, st>>>(d_node_force, 3ull * S->n_nodes);
+    StressPostArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p), S->perm.p, d_E, d_nu, d_u_node,
+                      d_normal, d_shear, d_von_mises, d_node_force };
+    for (uint32_t c = 0; c < S->n_colours; c++) {
+        const uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+        if (e1 > e0) k_stress_post<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+    }
+    RS_HIP(hipGetLastError());
+    return timed_end(ev, S->last_ms);
+}
+
