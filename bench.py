@@ -412,6 +412,89 @@ def eigen(rs, args) -> int:
     return 0
 
 
+def electrostatics(rs, args) -> int:
+    """Electrostatics (RSolverElectrostatics) on the 128^3 x 6-tet unit cube: relative
+    permittivity lognormal(0,1), charge density U(-0.5, 0.5) 1e-9, conductivity lognormal(2,1),
+    plates at 0 V (x = 0) and 5 V (x = 1).  GPU assembly, plain FP64 Jacobi-CG to relres 1e-8 and
+    the processing of field, current, energy, resistivity and Joule heat; the oracle times the
+    same steps on a bounded sample.  Prints one JSON line."""
+    import __graft_entry__ as ge
+    c = args.cells
+    xyz, tets = rs.kuhn_box(c, c, c)
+    x = xyz[:, 0]
+    fixed = ((np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12)).astype(np.uint8)
+    phi = np.where(np.abs(x - 1) < 1e-12, 5.0, 0.0)
+    ne, nn = tets.shape[0], xyz.shape[0]
+    eps_r = rs.field_lognormal(7, ne, 0.0, 1.0)
+    rho = (rs.field_uniform(8, ne) - 0.5) * 1e-9
+    sigma = rs.field_lognormal(9, ne, 2.0, 1.0)
+    asm = rs.Assembly(xyz, tets, 1, fixed)
+    n = asm.n_dof
+    deps, drho, dsig, dphi = (rs.DeviceArray.from_host(a) for a in (eps_r, rho, sigma, phi))
+    db, dx, dz = rs.DeviceArray(n), rs.DeviceArray(n), rs.DeviceArray.from_host(np.zeros(n))
+    outs = [rs.DeviceArray(3 * ne), rs.DeviceArray(3 * ne), rs.DeviceArray(ne), rs.DeviceArray(ne), rs.DeviceArray(ne)]
+    conf = rs.MatrixSolverConf(n_outer_iterations=400000, solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES)
+    lib = rs.load()
+    res = []
+    for rep in range(1 + max(1, args.steps)):
+        rs._check(lib.rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * n))
+        rs.synchronize()
+        t0 = time.perf_counter()
+        asm.assemble_electrostatics_device(deps, drho, dphi, db)
+        a_ms = asm.last_ms
+        info = rs.RMatrixSolver(conf).solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+        asm.dofs_to_nodes(dx, dphi)
+        asm.electrostatics_process_device(deps, dsig, dphi, *outs)
+        p_ms = asm.last_ms
+        rs.synchronize()
+        t = time.perf_counter() - t0
+        if rep > 0:
+            res.append((t, a_ms, info.solve_ms, p_ms, info.iterations, info.true_relres))
+        log(f"electrostatics: {t * 1e3:.1f} ms (assembly {a_ms:.2f} ms, solve {info.solve_ms:.1f} ms "
+            f"{info.iterations} its, processing {p_ms:.3f} ms)")
+    best = min(res)
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        orc.load()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        sub = args.asm_cells
+        sx, st_ = rs.kuhn_box(sub, sub, sub)
+        sxx = sx[:, 0]
+        sfix = ((np.abs(sxx) < 1e-12) | (np.abs(sxx - 1) < 1e-12)).astype(np.uint8)
+        sphi = np.where(np.abs(sxx - 1) < 1e-12, 5.0, 0.0)
+        sne = st_.shape[0]
+        book, ndof = orc.node_book(sx.shape[0], st_, 1, sfix)
+        t0 = time.perf_counter()
+        A_s, b_s = orc.electrostatics_assemble(sx, st_, eps_r[:sne], rho[:sne], sphi, book, ndof)
+        t_asm = (time.perf_counter() - t0) * ne / sne
+        t0 = time.perf_counter()
+        orc.electrostatics_process(sx, st_, eps_r[:sne], sigma[:sne], sphi)
+        t_proc = (time.perf_counter() - t0) * ne / sne
+        A_full = asm.matrix.to_scipy()
+        tt = []
+        for its in (4, 24):
+            t0 = time.perf_counter()
+            orc.solve(A_full, np.ones(n), precond=orc.PRECOND_JACOBI, n_outer=its, cvg=1e-300, stop_rule=1)
+            tt.append(time.perf_counter() - t0)
+        t_iter = (tt[1] - tt[0]) / 20.0
+        total = t_asm + t_proc + best[4] * t_iter
+        cpu = {"value": round(total, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": (f"oracle on {threads} threads: assembly and processing of a {sub}^3 sub-box scaled per "
+                          f"tet ({t_asm:.2f} s, {t_proc:.2f} s), Jacobi-CG per iteration from 4- and 24-iteration "
+                          f"runs on the full system ({t_iter * 1e3:.1f} ms x {best[4]})")}
+        log(f"cpu baseline: {total:.2f} s")
+    out = {"metric": "electrostatics: assembly + FP64 Jacobi-CG to relres 1e-8 + processing, per system",
+           "value": round(best[0], 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"electrostatics {c}^3 cells x 6 tets, eps_r lognormal(0,1), plates 0/5 V",
+                      "unknowns": n, "nnz": int(asm.nnz), "tets": int(ne)},
+           "assembly_ms": round(best[1], 3), "solve_ms": round(best[2], 2), "processing_ms": round(best[3], 4),
+           "iterations": best[4], "true_relres": best[5], "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -425,9 +508,9 @@ def main() -> int:
     ap.add_argument("--asm-cells", type=int, default=40)
     ap.add_argument("--eigen-cells", type=int, default=128)
     ap.add_argument("--eigen-modes", type=int, default=10)
-    ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen"], default="heat",
+    ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen", "electrostatics"], default="heat",
                     help="heat (default, the headline), the elasticity cantilever, the batched "
-                         "transient heat systems, or the modal analysis (Arnoldi)")
+                         "transient heat systems, the modal analysis (Arnoldi), or electrostatics")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--unbatched-budget", type=float, default=40.0,
@@ -458,6 +541,8 @@ def main() -> int:
         return batch_transient(rs, args)
     if args.workload == "eigen":
         return eigen(rs, args)
+    if args.workload == "electrostatics":
+        return electrostatics(rs, args)
     c = args.cells
     log(f"building {c}^3 Kuhn mesh and fields")
     xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma, args.seed_k)

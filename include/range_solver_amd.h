@@ -241,6 +241,14 @@ int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node)
  * RSolverHeat::process, src/rsolverheat.cpp:646-692 -- 3 flux values per tet, q = -sum_m B_m k T_m
  * with B_m = grad N_m * detJ (the reference's integration of grad N detJ / nInp). */
 int rs_heat_flux_device(rs_assembly *S, const double *d_k, const double *d_t_node, double *d_flux);
+/* RSolverElectrostatics (src/rsolverelectrostatics.cpp): prepare's volume assembly (:300-345) --
+ * Ke = B.B eps_r e0 V, fe = -rho V / 4, potential lifting -- and process (:578-657) -- per tet
+ * the electric field (3), current density (3), energy, resistivity and Joule heat. */
+int rs_assemble_electrostatics_device(rs_assembly *S, const double *d_eps_r, const double *d_rho,
+                                      const double *d_potential, double *d_b);
+int rs_electrostatics_process_device(rs_assembly *S, const double *d_eps_r, const double *d_sigma,
+                                     const double *d_potential, double *d_field, double *d_current,
+                                     double *d_energy, double *d_resistivity, double *d_joule);
 /* RSolverStress::process, src/rsolverstress.cpp:1238-1408, static case without thermal load --
  * per tet the normal and shear stress measures and their sum (the reference's von Mises), and the
  * node forces sum_e Ke u_e (3 per node; reactions on the fixed nodes). */

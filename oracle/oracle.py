@@ -59,6 +59,7 @@ def load() -> C.CDLL:
     lib.or_solve.argtypes = [C.c_uint, _P, _P, _P, _P, _P, C.c_int, C.c_uint, C.POINTER(OrConf),
                              C.c_double, C.POINTER(OrInfo), _P, C.c_uint]
     lib.or_spmv.argtypes = [C.c_uint, _P, _P, _P, _P, _P]
+    lib.or_electrostatics_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.or_heat_flux.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P]
     lib.or_stress_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.or_eigen_solve.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_double, C.POINTER(OrConf), C.c_uint, _P, _P,
@@ -293,4 +294,29 @@ def stress_process(xyz, tets, E, nu, u_node):
     nrm, sh, vm, f = np.empty(ne), np.empty(ne), np.empty(ne), np.empty(3 * nn)
     load().or_stress_process(nn, _p(xyz), ne, _p(tets), _p(E), _p(nu), _p(u), _p(nrm), _p(sh), _p(vm), _p(f))
     return nrm, sh, vm, f.reshape(nn, 3)
+
+
+E0 = 8.854187817e-12   # RSolverGeneric::e0, src/rsolvergeneric.cpp:13
+
+
+def electrostatics_assemble(xyz, tets, eps_r, rho, phi_node, book, n_dof):
+    """RSolverElectrostatics::prepare, volume elements (src/rsolverelectrostatics.cpp:300-345):
+    Ke = B.B eps_r e0 detJ w, fe = -rho N detJ w, the potential lifting of assemblyMatrix
+    (:793-833) -- the heat form with k = eps_r e0 and Q = -rho."""
+    eps_r, rho = np.asarray(eps_r, np.float64), np.asarray(rho, np.float64)
+    return heat_assemble(xyz, tets, eps_r * E0, -rho, phi_node, book, n_dof)
+
+
+def electrostatics_process(xyz, tets, eps_r, sigma, phi_node):
+    """RSolverElectrostatics::process, volume elements: (field (ne,3), current (ne,3), energy,
+    resistivity, Joule heat)."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    eps_r, sigma, phi = (np.ascontiguousarray(a, np.float64) for a in (eps_r, sigma, phi_node))
+    nn, ne = xyz.size // 3, tets.size // 4
+    f, j = np.empty(3 * ne), np.empty(3 * ne)
+    en, rs_, jh = np.empty(ne), np.empty(ne), np.empty(ne)
+    load().or_electrostatics_process(nn, _p(xyz), ne, _p(tets), _p(eps_r), _p(sigma), _p(phi), _p(f), _p(j),
+                                     _p(en), _p(rs_), _p(jh))
+    return f.reshape(ne, 3), j.reshape(ne, 3), en, rs_, jh
 

@@ -801,6 +801,48 @@ int or_heat_flux(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *te
     return 0;
 }
 
+/* RSolverElectrostatics::process, volume elements (src/rsolverelectrostatics.cpp:578-657). */
+int or_electrostatics_process(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *eps_r,
+                              const double *sigma, const double *phi, double *field, double *current, double *energy,
+                              double *resistivity, double *joule)
+{
+    (void)nn;
+    const double e0 = 8.854187817e-12;                    /* RSolverGeneric::e0, src/rsolvergeneric.cpp:13 */
+#pragma omp parallel for
+    for (int64_t e = 0; e < (int64_t)ne; e++) {
+        const uint32_t *nd = tets + 4 * e;
+        double X[4][3], J[3][3], B[4][3] = { { 0 } };
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) X[m][a] = xyz[3 * (size_t)nd[m] + a];
+        for (int ip = 0; ip < 4; ip++) {
+            double detJ = tet_jacobian(X, J);
+            for (int m = 0; m < 4; m++)
+                for (int a = 0; a < 3; a++)
+                    B[m][a] += (TET_DN[m][0] * J[a][0] + TET_DN[m][1] * J[a][1] + TET_DN[m][2] * J[a][2]) * detJ / 4.0;
+        }
+        double E[3] = { 0.0, 0.0, 0.0 }, Jc[3];
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) E[a] -= B[m][a] * phi[nd[m]];
+        double dE = E[0] * E[0] + E[1] * E[1] + E[2] * E[2];
+        double ere = sqrt(dE);
+        for (int a = 0; a < 3; a++) Jc[a] = E[a] * sigma[e];
+        double jre = sqrt(Jc[0] * Jc[0] + Jc[1] * Jc[1] + Jc[2] * Jc[2]);
+        double sv[3] = { 0.0, 0.0, 0.0 }, L = 0.0;
+        if (ere > OR_EPS)
+            for (int a = 0; a < 3; a++) sv[a] = E[a] / ere;
+        for (int m = 0; m < 4; m++) L += fabs(sv[0] * B[m][0] + sv[1] * B[m][1] + sv[2] * B[m][2]);
+        if (L > OR_EPS) L = 2.0 / L;
+        for (int a = 0; a < 3; a++) {
+            field[3 * (size_t)e + a] = E[a];
+            current[3 * (size_t)e + a] = Jc[a];
+        }
+        energy[e] = eps_r[e] * e0 * dE / 2.0;
+        resistivity[e] = jre > OR_EPS ? ere / jre : 0.0;
+        joule[e] = sigma[e] * dE * L;
+    }
+    return 0;
+}
+
 /* RSolverStress::process, volume elements, static case (src/rsolverstress.cpp:1238-1408):
  * Ke and the element stresses Qe = sum_ip sum_m BeD^T u_m / nInp (no thermal load), normal
  * sqrt(sx^2 + sy^2 + sz^2 - (sx sy + sy sz + sz sx)), shear sqrt(3 (tyz^2 + txz^2 + txy^2)),

@@ -115,6 +115,10 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
 /* RSolverHeat::process (volume), src/rsolverheat.cpp:646-692: 3 flux values per tet. */
 int or_heat_flux(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *k,
                  const double *t_node, double *flux);
+/* RSolverElectrostatics::process (volume), src/rsolverelectrostatics.cpp:578-657. */
+int or_electrostatics_process(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *eps_r,
+                              const double *sigma, const double *phi, double *field, double *current, double *energy,
+                              double *resistivity, double *joule);
 /* RSolverStress::process (volume, static), src/rsolverstress.cpp:1238-1408. */
 int or_stress_process(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *E,
                       const double *nu, const double *u_node, double *normal, double *shear, double *vm,

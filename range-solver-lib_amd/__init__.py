@@ -130,6 +130,8 @@ _SIGS = {
     "rs_assembly_dofs_to_nodes": [_P, _P, _P], "rs_assembly_destroy": [_P],
     "rs_assembly_set_method": [_P, C.c_int32],
     "rs_heat_flux_device": [_P, _P, _P, _P],
+    "rs_assemble_electrostatics_device": [_P, _P, _P, _P, _P],
+    "rs_electrostatics_process_device": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
     "rs_stress_process_device": [_P, _P, _P, _P, _P, _P, _P, _P],
     "rs_aggregates_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                               _P, _u32p],
@@ -574,6 +576,16 @@ class Assembly:
     def heat_flux_device(self, dk: DeviceArray, dt_node: DeviceArray, dflux: DeviceArray) -> None:
         """RSolverHeat::process (volume elements): 3 flux values per element."""
         _check(_lib.rs_heat_flux_device(self.h, dk.ptr, dt_node.ptr, dflux.ptr))
+
+    def assemble_electrostatics_device(self, deps_r, drho, dphi_node, db) -> None:
+        """RSolverElectrostatics prepare (volume): the heat form with eps_r e0 and source -rho."""
+        _check(_lib.rs_assemble_electrostatics_device(self.h, deps_r.ptr, drho.ptr, dphi_node.ptr, db.ptr))
+
+    def electrostatics_process_device(self, deps_r, dsigma, dphi_node, dfield, dcurrent, denergy, dres,
+                                      djoule) -> None:
+        """RSolverElectrostatics::process (volume): field, current density, energy, resistivity, Joule."""
+        _check(_lib.rs_electrostatics_process_device(self.h, deps_r.ptr, dsigma.ptr, dphi_node.ptr, dfield.ptr,
+                                                     dcurrent.ptr, denergy.ptr, dres.ptr, djoule.ptr))
 
     def stress_process_device(self, dE, dnu, du_node, dnormal, dshear, dvm, dforce) -> None:
         """RSolverStress::process (volume elements, static): normal, shear, their sum, node forces."""

@@ -314,6 +314,77 @@ __global__ __launch_bounds__(kBlock) void k_heat_flux(const double *__restrict__
     for (int c = 0; c < 3; c++) flux[3 * (uint64_t)orig + c] = Q[c];
 }
 
+// RSolverElectrostatics (src/rsolverelectrostatics.cpp): the volume assembly (:300-345) is
+// the heat form with coefficient eps_r * e0 and source -rho (fe -= rho N detJ w); e0 as
+// RSolverGeneric::e0 (src/rsolvergeneric.cpp:13).
+constexpr double kVacuumPermittivity = 8.854187817e-12;
+
+__global__ void k_electro_coeffs(const double *__restrict__ eps_r, const double *__restrict__ rho, uint32_t ne,
+                                 double *__restrict__ k, double *__restrict__ q)
+{
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock) {
+        k[e] = eps_r[e] * kVacuumPermittivity;
+        q[e] = -rho[e];
+    }
+}
+
+// RSolverElectrostatics::process, volume elements (:578-657): B_m = grad N_m * detJ (integrated
+// grad N detJ / nInp), E = -sum B_m phi_m, J = sigma E, energy eps_r e0 |E|^2 / 2, resistivity
+// |E| / |J|, Joule heat sigma |E|^2 L with L = 2 / sum_m |s . B_m|, s = E / |E|.
+struct ElectroPostArgs {
+    const double *xyz;
+    const uint4 *conn0;
+    uint32_t ne;
+    const double *eps_r, *sigma, *phi;
+    double *field, *current, *energy, *resistivity, *joule;
+};
+
+__global__ __launch_bounds__(kBlock) void k_electro_post(ElectroPostArgs a)
+{
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= a.ne) return;
+    const uint4 nd4 = a.conn0[e];
+    const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+    double X[4][3], g[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) X[m][c] = a.xyz[3 * (uint64_t)nd[m] + c];
+    const double detJ = 6.0 * tet_gradients(X, g);
+    double B[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) B[m][c] = g[m][c] * detJ;
+    double E[3] = { 0.0, 0.0, 0.0 };
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const double ph = a.phi[nd[m]];
+#pragma unroll
+        for (int c = 0; c < 3; c++) E[c] -= B[m][c] * ph;
+    }
+    const double sig = a.sigma[e];
+    const double dE = E[0] * E[0] + E[1] * E[1] + E[2] * E[2];
+    const double ere = sqrt(dE);
+    const double J[3] = { E[0] * sig, E[1] * sig, E[2] * sig };
+    const double jre = sqrt(J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+    double s[3] = { 0.0, 0.0, 0.0 };
+    if (ere > kEps)
+        for (int c = 0; c < 3; c++) s[c] = E[c] / ere;
+    double L = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) L += fabs(s[0] * B[m][0] + s[1] * B[m][1] + s[2] * B[m][2]);
+    if (L > kEps) L = 2.0 / L;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        a.field[3 * (uint64_t)e + c] = E[c];
+        a.current[3 * (uint64_t)e + c] = J[c];
+    }
+    a.energy[e] = a.eps_r[e] * kVacuumPermittivity * dE / 2.0;
+    a.resistivity[e] = jre > kEps ? ere / jre : 0.0;
+    a.joule[e] = sig * dE * L;
+}
+
 // RSolverStress::process, volume elements, static case (src/rsolverstress.cpp:1238-1408):
 // element stresses sigma = De eps (Voigt xx, yy, zz, yz, xz, xy, engineering shear), the normal
 // and shear measures and their sum (the reference's "von Mises"), and node forces Ke u_e =
@@ -506,6 +577,7 @@ struct rs_assembly {
     DBuf<double> xyz;
     DBuf<uint32_t> conn, perm, map, dbook;
     DBuf<uint32_t> conn0;                   // connectivity in the caller's order (post-processing)
+    DBuf<double> coeff;                     // mapped element coefficients (electrostatics)
     DBuf<uint32_t> dof_node, inc_ptr, inc;  // row-gather heat assembly
     DBuf<double> rec_grad, rec_vol;         // row records (built once per mesh)
     DBuf<uint32_t> rec_elem, rec_slots, rec_count;
@@ -885,6 +957,8 @@ extern "C" int rs_assembly_destroy(rs_assembly *S)
     return RS_OK;
 }
 
+static int ensure_conn0(rs_assembly *S);
+
 extern "C" int rs_heat_flux_device(rs_assembly *S, const double *d_k, const double *d_t_node, double *d_flux)
 {
     if (!S || !d_k || !d_t_node || !d_flux) return fail(RS_ERR_INVALID_ARG, "rs_heat_flux_device: null argument");
@@ -892,11 +966,7 @@ extern "C" int rs_heat_flux_device(rs_assembly *S, const double *d_k, const doub
     if (rc) return rc;
     hipEvent_t ev[2];
     if ((rc = timed_begin(ev))) return rc;
-    if (S->n_elems && !S->conn0.p) {
-        if ((rc = S->conn0.alloc(4ull * S->n_elems))) return rc;
-        k_unsort_conn<<<vblocks(S->n_elems), kBlock, 0, runtime().stream>>>(
-            reinterpret_cast<const uint4 *>(S->conn.p), S->perm.p, S->n_elems, reinterpret_cast<uint4 *>(S->conn0.p));
-    }
+    if ((rc = ensure_conn0(S))) return rc;
     if (S->n_elems)
         k_heat_flux<<<grid_for(S->n_elems, kBlock), kBlock, 0, runtime().stream>>>(
             S->xyz.p, reinterpret_cast<const uint4 *>(S->conn0.p), S->n_elems, d_k, d_t_node, d_flux);
@@ -925,3 +995,48 @@ extern "C" int rs_stress_process_device(rs_assembly *S, const double *d_E, const
     return timed_end(ev, S->last_ms);
 }
 
+// The caller's element order of the connectivity, built on first use.
+static int ensure_conn0(rs_assembly *S)
+{
+    if (!S->n_elems || S->conn0.p) return RS_OK;
+    int rc = S->conn0.alloc(4ull * S->n_elems);
+    if (rc) return rc;
+    k_unsort_conn<<<vblocks(S->n_elems), kBlock, 0, runtime().stream>>>(
+        reinterpret_cast<const uint4 *>(S->conn.p), S->perm.p, S->n_elems, reinterpret_cast<uint4 *>(S->conn0.p));
+    RS_HIP(hipGetLastError());
+    return RS_OK;
+}
+
+extern "C" int rs_assemble_electrostatics_device(rs_assembly *S, const double *d_eps_r, const double *d_rho,
+                                                 const double *d_potential, double *d_b)
+{
+    if (!S || !d_eps_r || !d_rho || !d_potential || !d_b)
+        return fail(RS_ERR_INVALID_ARG, "rs_assemble_electrostatics_device: null argument");
+    if (S->dpn != 1) return fail(RS_ERR_INVALID_ARG, "rs_assemble_electrostatics_device: assembly has 3 dofs per node");
+    int rc = require_runtime();
+    if (rc) return rc;
+    const uint64_t ne = S->n_elems;
+    if ((rc = S->coeff.alloc(2 * ne + 1))) return rc;
+    if (ne) k_electro_coeffs<<<vblocks(ne), kBlock, 0, runtime().stream>>>(d_eps_r, d_rho, (uint32_t)ne, S->coeff.p,
+                                                                            S->coeff.p + ne);
+    RS_HIP(hipGetLastError());
+    return rs_assemble_heat_device(S, S->coeff.p, S->coeff.p + ne, nullptr, d_potential, 0, 1.0, 1.0, d_b);
+}
+
+extern "C" int rs_electrostatics_process_device(rs_assembly *S, const double *d_eps_r, const double *d_sigma,
+                                                const double *d_potential, double *d_field, double *d_current,
+                                                double *d_energy, double *d_resistivity, double *d_joule)
+{
+    if (!S || !d_eps_r || !d_sigma || !d_potential || !d_field || !d_current || !d_energy || !d_resistivity || !d_joule)
+        return fail(RS_ERR_INVALID_ARG, "rs_electrostatics_process_device: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipEvent_t ev[2];
+    if ((rc = timed_begin(ev))) return rc;
+    if ((rc = ensure_conn0(S))) return rc;
+    ElectroPostArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn0.p), S->n_elems, d_eps_r, d_sigma,
+                       d_potential, d_field, d_current, d_energy, d_resistivity, d_joule };
+    if (S->n_elems) k_electro_post<<<grid_for(S->n_elems, kBlock), kBlock, 0, runtime().stream>>>(a);
+    RS_HIP(hipGetLastError());
+    return timed_end(ev, S->last_ms);
+}
