@@ -61,6 +61,28 @@ public:
     void fill(double v) { std::fill(begin(), end(), v); }
 };
 
+// RRMatrix stand-in: a dense row-major matrix (rows x columns).
+class RRMatrix {
+public:
+    RRMatrix() = default;
+    RRMatrix(unsigned r, unsigned c, double v = 0.0) : r_(r), c_(c), a_((size_t)r * c, v) {}
+    unsigned getNRows() const { return r_; }
+    unsigned getNColumns() const { return c_; }
+    void resize(unsigned r, unsigned c, double v = 0.0)
+    {
+        r_ = r;
+        c_ = c;
+        a_.assign((size_t)r * c, v);
+    }
+    double *operator[](unsigned i) { return a_.data() + (size_t)i * c_; }
+    const double *operator[](unsigned i) const { return a_.data() + (size_t)i * c_; }
+    double *data() { return a_.data(); }
+
+private:
+    unsigned r_ = 0, c_ = 0;
+    std::vector<double> a_;
+};
+
 // RSparseMatrix stand-in: rows of (column, value) kept sorted; addValue accumulates.
 class RSparseMatrix {
 public:
@@ -220,6 +242,64 @@ private:
     RMatrixSolverConf conf_;
     rs_solver *h_ = nullptr;
     rs_info info_{};
+};
+
+// REigenValueSolverConf stand-in (range-model-lib): method, eigenvalues, iterations, tolerance.
+class REigenValueSolverConf {
+public:
+    enum Method { Lanczos = RS_EIGEN_LANCZOS, Arnoldi = RS_EIGEN_ARNOLDI, Rayleigh = RS_EIGEN_RAYLEIGH };
+    void setMethod(Method m) { method_ = m; }
+    void setNEigenValues(unsigned n) { nEigen_ = n; }
+    void setNIterations(unsigned n) { nIter_ = n; }
+    void setSolverCvgValue(double v) { cvg_ = v; }
+    Method getMethod() const { return method_; }
+    unsigned getNEigenValues() const { return nEigen_; }
+    unsigned getNIterations() const { return nIter_; }
+    double getSolverCvgValue() const { return cvg_; }
+
+private:
+    Method method_ = Arnoldi;
+    unsigned nEigen_ = 1, nIter_ = 100;
+    double cvg_ = 1e-10;
+};
+
+// REigenValueSolver (include/reigenvaluesolver.h:9-63): solve(M, K, d, ev) on the GPU.
+class REigenValueSolver {
+public:
+    REigenValueSolver(const REigenValueSolverConf &conf, const RMatrixSolverConf &matrixSolverConf)
+        : conf_(conf), mconf_(matrixSolverConf)
+    {
+    }
+    void solve(const RSparseMatrix &M, const RSparseMatrix &K, RRVector &d, RRMatrix &ev)
+    {
+        rs_matrix *Mh = detail::device_matrix(M);
+        rs_matrix *Kh = detail::device_matrix(K);
+        const unsigned n = K.getNRows();
+        const unsigned cap = std::max(1u, std::min(conf_.getNEigenValues(), n));
+        rs_eigen_conf ec{ (int32_t)conf_.getMethod(), conf_.getNEigenValues(), conf_.getNIterations(),
+                          conf_.getSolverCvgValue() };
+        rs_conf c{};
+        c.solver_type = RS_SOLVER_CG;
+        c.n_outer_iterations = mconf_.getNOuterIterations();
+        c.n_inner_iterations = mconf_.getNInnerIterations();
+        c.solver_cvg_value = mconf_.getSolverCvgValue();
+        c.stop_rule = RS_STOP_REFERENCE;
+        c.precision = RS_PRECISION_FP64;
+        std::vector<double> dv(cap), evv((size_t)cap * n);
+        uint32_t nd = 0, nr = 0;
+        check(rs_eigen_solve(&ec, &c, Mh, Kh, dv.data(), cap, &nd, evv.data(), evv.size(), &nr, &info_));
+        d.resize(nd);
+        for (unsigned i = 0; i < nd; i++) d[i] = dv[i];
+        ev.resize(nr, nr ? n : 0);
+        for (unsigned i = 0; i < nr; i++)
+            for (unsigned j = 0; j < n; j++) ev[i][j] = evv[(size_t)i * n + j];
+    }
+    const rs_eigen_info &lastInfo() const { return info_; }
+
+private:
+    REigenValueSolverConf conf_;
+    RMatrixSolverConf mconf_;
+    rs_eigen_info info_{};
 };
 
 }  // namespace rsamd

@@ -1,7 +1,8 @@
 // Drop-in check of the C++ call surface: the known-answer cases of the reference's
 // tests/tst_matrix_solver.cpp:33-77 (CG, preconditioner NONE, cvg 1e-13, 1000 iterations,
 // tolerance 1e-6) through rsamd::RMatrixSolver, plus the error behaviour of an unknown
-// preconditioner type.  Exit status 0 when every check passes.
+// preconditioner type, and an eigenvalue solve through rsamd::REigenValueSolver.  Exit status 0
+// when every check passes.
 #include <cmath>
 #include <cstdio>
 
@@ -86,6 +87,31 @@ int main()
             threw = e.getType() == RError::Type::Application;
         }
         EXPECT(threw);
+    }
+    {
+        // REigenValueSolver (Arnoldi) on K = diag of three distinct values (10 each), M = I: the
+        // 3-vector Krylov space is exact, so d = {2, 5, 11} after the reference's post-processing
+        RSparseMatrix K, M;
+        const unsigned n = 30;
+        K.setNRows(n);
+        M.setNRows(n);
+        for (unsigned i = 0; i < n; i++) {
+            K.addValue(i, i, i < 10 ? 2.0 : (i < 20 ? 5.0 : 11.0));
+            M.addValue(i, i, 1.0);
+        }
+        REigenValueSolverConf econf;
+        econf.setMethod(REigenValueSolverConf::Arnoldi);
+        econf.setNEigenValues(3);
+        econf.setNIterations(2000);
+        econf.setSolverCvgValue(1e-14);
+        RMatrixSolverConf mconf = cg_conf();
+        REigenValueSolver es(econf, mconf);
+        RRVector d;
+        RRMatrix ev;
+        es.solve(M, K, d, ev);
+        EXPECT(d.getNRows() == 3u && ev.getNRows() == 3u && ev.getNColumns() == n);
+        EXPECT(std::fabs(d[0] - 2.0) < 1e-8 && std::fabs(d[1] - 5.0) < 1e-8 && std::fabs(d[2] - 11.0) < 1e-8);
+        std::printf("arnoldi: d = [%.10f, %.10f, %.10f]\n", d[0], d[1], d[2]);
     }
     std::printf("%s (%d failures)\n", failures ? "FAIL" : "PASS", failures);
     return failures ? 1 : 0;
