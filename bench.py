@@ -495,6 +495,78 @@ def electrostatics(rs, args) -> int:
     return 0
 
 
+def magnetostatics(rs, args) -> int:
+    """Magnetostatics (RSolverMagnetostatics) on the args.mag_cells^3 x 6-tet unit cube, three
+    unknowns per node: node current densities N(0,1) 1e6 (seeded), field fixed to zero on x = 0
+    and x = 1.  GPU assembly and FP64 Jacobi-CG to relres 1e-8; the oracle times the assembly on
+    a sub-box (scaled per tet) and CG iterations on the full system.  Prints one JSON line."""
+    import __graft_entry__ as ge
+    c = args.mag_cells
+    xyz, tets = rs.kuhn_box(c, c, c)
+    nn, ne = xyz.shape[0], tets.shape[0]
+    x = xyz[:, 0]
+    fixed_node = (np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12)
+    fixed = np.repeat(fixed_node, 3).astype(np.uint8)
+    J = np.random.default_rng(11).standard_normal(3 * nn) * 1e6
+    H = np.zeros(3 * nn)
+    asm = rs.Assembly(xyz, tets, 3, fixed)
+    n = asm.n_dof
+    dJ, dH = rs.DeviceArray.from_host(J), rs.DeviceArray.from_host(H)
+    db, dx, dz = rs.DeviceArray(n), rs.DeviceArray(n), rs.DeviceArray.from_host(np.zeros(n))
+    conf = rs.MatrixSolverConf(n_outer_iterations=400000, solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES)
+    lib = rs.load()
+    res = []
+    for rep in range(1 + max(1, args.steps)):
+        rs._check(lib.rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * n))
+        rs.synchronize()
+        t0 = time.perf_counter()
+        asm.assemble_magnetostatics_device(dJ, dH, db)
+        a_ms = asm.last_ms
+        info = rs.RMatrixSolver(conf).solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+        rs.synchronize()
+        t = time.perf_counter() - t0
+        if rep > 0:
+            res.append((t, a_ms, info.solve_ms, info.iterations, info.true_relres))
+        log(f"magnetostatics: {t * 1e3:.1f} ms (assembly {a_ms:.2f} ms, solve {info.solve_ms:.1f} ms "
+            f"{info.iterations} its, relres {info.true_relres:.2e})")
+    best = min(res)
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        orc.load()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        sub = 24
+        sx, st_ = rs.kuhn_box(sub, sub, sub)
+        sfix_node = (np.abs(sx[:, 0]) < 1e-12) | (np.abs(sx[:, 0] - 1) < 1e-12)
+        sfix = np.repeat(sfix_node, 3).astype(np.uint8)
+        book, ndof = orc.node_book(sx.shape[0], st_, 3, sfix)
+        t0 = time.perf_counter()
+        orc.magnetostatics_assemble(sx, st_, J[:3 * sx.shape[0]], np.zeros(3 * sx.shape[0]), book, ndof)
+        t_asm = (time.perf_counter() - t0) * ne / st_.shape[0]
+        A_full = asm.matrix.to_scipy()
+        tt = []
+        for its in (4, 14):
+            t0 = time.perf_counter()
+            orc.solve(A_full, np.ones(n), precond=orc.PRECOND_JACOBI, n_outer=its, cvg=1e-300, stop_rule=1)
+            tt.append(time.perf_counter() - t0)
+        t_iter = (tt[1] - tt[0]) / 10.0
+        total = t_asm + best[3] * t_iter
+        cpu = {"value": round(total, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": (f"oracle on {threads} threads: assembly of a {sub}^3 sub-box scaled per tet "
+                          f"({t_asm:.2f} s), Jacobi-CG per iteration from 4- and 14-iteration runs on the full "
+                          f"system ({t_iter * 1e3:.1f} ms x {best[3]})")}
+        log(f"cpu baseline: {total:.2f} s")
+    out = {"metric": "magnetostatics: assembly + FP64 Jacobi-CG to relres 1e-8, per system",
+           "value": round(best[0], 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"magnetostatics {c}^3 cells x 6 tets, 3 unknowns per node, random current density",
+                      "unknowns": n, "nnz": int(asm.nnz), "tets": int(ne)},
+           "assembly_ms": round(best[1], 3), "solve_ms": round(best[2], 2), "iterations": best[3],
+           "true_relres": best[4], "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -508,9 +580,12 @@ def main() -> int:
     ap.add_argument("--asm-cells", type=int, default=40)
     ap.add_argument("--eigen-cells", type=int, default=128)
     ap.add_argument("--eigen-modes", type=int, default=10)
-    ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen", "electrostatics"], default="heat",
+    ap.add_argument("--mag-cells", type=int, default=96)
+    ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen", "electrostatics", "magnetostatics"],
+                    default="heat",
                     help="heat (default, the headline), the elasticity cantilever, the batched "
-                         "transient heat systems, the modal analysis (Arnoldi), or electrostatics")
+                         "transient heat systems, the modal analysis (Arnoldi), electrostatics or "
+                         "magnetostatics")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--unbatched-budget", type=float, default=40.0,
@@ -543,6 +618,8 @@ def main() -> int:
         return eigen(rs, args)
     if args.workload == "electrostatics":
         return electrostatics(rs, args)
+    if args.workload == "magnetostatics":
+        return magnetostatics(rs, args)
     c = args.cells
     log(f"building {c}^3 Kuhn mesh and fields")
     xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma, args.seed_k)

@@ -249,6 +249,12 @@ int rs_assemble_electrostatics_device(rs_assembly *S, const double *d_eps_r, con
 int rs_electrostatics_process_device(rs_assembly *S, const double *d_eps_r, const double *d_sigma,
                                      const double *d_potential, double *d_field, double *d_current,
                                      double *d_energy, double *d_resistivity, double *d_joule);
+/* RSolverMagnetostatics::prepare + assemblyMatrix (src/rsolvermagnetostatics.cpp:44-145, 234-287)
+ * on a 3-DOF assembly whose nodes are fixed with all three components: per component the
+ * negative Laplacian, source e0 V/4 (grad N_m x J_m) from the node current density (3 per node),
+ * lifting of the prescribed field (3 per node). */
+int rs_assemble_magnetostatics_device(rs_assembly *S, const double *d_current_node, const double *d_field_node,
+                                      double *d_b);
 /* RSolverStress::process, src/rsolverstress.cpp:1238-1408, static case without thermal load --
  * per tet the normal and shear stress measures and their sum (the reference's von Mises), and the
  * node forces sum_e Ke u_e (3 per node; reactions on the fixed nodes). */

@@ -60,6 +60,7 @@ def load() -> C.CDLL:
                              C.c_double, C.POINTER(OrInfo), _P, C.c_uint]
     lib.or_spmv.argtypes = [C.c_uint, _P, _P, _P, _P, _P]
     lib.or_electrostatics_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    lib.or_magnetostatics_assemble.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, C.c_uint32, _P, _P]
     lib.or_heat_flux.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P]
     lib.or_stress_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.or_eigen_solve.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_double, C.POINTER(OrConf), C.c_uint, _P, _P,
@@ -319,4 +320,19 @@ def electrostatics_process(xyz, tets, eps_r, sigma, phi_node):
     load().or_electrostatics_process(nn, _p(xyz), ne, _p(tets), _p(eps_r), _p(sigma), _p(phi), _p(f), _p(j),
                                      _p(en), _p(rs_), _p(jh))
     return f.reshape(ne, 3), j.reshape(ne, 3), en, rs_, jh
+
+
+def magnetostatics_assemble(xyz, tets, current_node, field_node, book, n_dof):
+    """RSolverMagnetostatics::prepare + assemblyMatrix restated; returns (A as scipy CSR, b)."""
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    cur = np.ascontiguousarray(current_node, np.float64).reshape(-1)
+    fld = np.ascontiguousarray(field_node, np.float64).reshape(-1)
+    book = np.ascontiguousarray(book, np.uint32)
+    A = lib.or_sparse_create(n_dof)
+    b = np.zeros(max(1, n_dof))
+    lib.or_magnetostatics_assemble(xyz.size // 3, _p(xyz), tets.size // 4, _p(tets), _p(cur), _p(fld), _p(book),
+                                   n_dof, A, _p(b))
+    return _export(A, n_dof), b[:n_dof]
 

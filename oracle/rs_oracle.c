@@ -774,6 +774,54 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
     return 0;
 }
 
+/* RSolverMagnetostatics::prepare + assemblyMatrix (src/rsolvermagnetostatics.cpp:44-145,
+ * 234-287): per tet and integration point Ke[3m+c][3n+c] -= (B_m . B_n) detJ w for each
+ * component c, fe[3m+c] += N_m detJ w e0 (B_m x J_m)_c with J_m the node current density;
+ * lifting of the prescribed field H on fixed nodes; unknowns 3 * position + c. */
+int or_magnetostatics_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                               const double *current, const double *field, const uint32_t *book, uint32_t n_dof,
+                               or_sparse *A, double *b)
+{
+    (void)nn; (void)n_dof;
+    const double e0 = 8.854187817e-12;
+    for (int64_t e = 0; e < (int64_t)ne; e++) {
+        const uint32_t *nd = tets + 4 * e;
+        double X[4][3], J[3][3], B[4][3], N[4];
+        double Ke[144] = { 0 }, fe[12] = { 0 };
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) X[m][a] = xyz[3 * (size_t)nd[m] + a];
+        for (int ip = 0; ip < 4; ip++) {
+            tet_shape(TET_IP[ip], N);
+            double detJ = tet_jacobian(X, J);
+            for (int m = 0; m < 4; m++)
+                for (int a = 0; a < 3; a++)
+                    B[m][a] = TET_DN[m][0] * J[a][0] + TET_DN[m][1] * J[a][1] + TET_DN[m][2] * J[a][2];
+            for (int m = 0; m < 4; m++) {
+                const double *Jm = current + 3 * (size_t)nd[m];
+                for (int n = 0; n < 4; n++) {
+                    double kv = (B[m][0] * B[n][0] + B[m][1] * B[n][1] + B[m][2] * B[n][2]) * detJ * TET_W;
+                    for (int c = 0; c < 3; c++) Ke[(3 * m + c) * 12 + 3 * n + c] -= kv;
+                }
+                double fv = N[m] * detJ * TET_W * e0;
+                double jsx = -B[m][2] * Jm[1] + B[m][1] * Jm[2];
+                double jsy = B[m][2] * Jm[0] - B[m][0] * Jm[2];
+                double jsz = -B[m][1] * Jm[0] + B[m][0] * Jm[1];
+                fe[3 * m + 0] += fv * jsx;
+                fe[3 * m + 1] += fv * jsy;
+                fe[3 * m + 2] += fv * jsz;
+            }
+        }
+        for (int m = 0; m < 4; m++) {
+            if (book[3 * (size_t)nd[m]] != OR_EOD) continue;
+            const double *H = field + 3 * (size_t)nd[m];
+            for (int n = 0; n < 12; n++)
+                for (int c = 0; c < 3; c++) fe[n] -= Ke[n * 12 + 3 * m + c] * H[c];
+        }
+        stress_scatter(A, b, book, nd, 4, Ke, fe, 1);
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ post-processing
  * RSolverHeat::process, volume elements (src/rsolverheat.cpp:646-692): per tet
  * B[m] = sum over the integration points of grad N_m * detJ / nInp; q = -sum_m B[m] k T_m. */

@@ -112,6 +112,11 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
                        uint32_t ntri, const uint32_t *tris, const double *traction,
                        const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b);
 
+/* RSolverMagnetostatics prepare + assemblyMatrix (volume), src/rsolvermagnetostatics.cpp:44-145,
+ * 234-287; book has 3 entries per node (all three fixed together). */
+int or_magnetostatics_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                               const double *current, const double *field, const uint32_t *book, uint32_t n_dof,
+                               or_sparse *A, double *b);
 /* RSolverHeat::process (volume), src/rsolverheat.cpp:646-692: 3 flux values per tet. */
 int or_heat_flux(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *k,
                  const double *t_node, double *flux);

@@ -131,6 +131,7 @@ _SIGS = {
     "rs_assembly_set_method": [_P, C.c_int32],
     "rs_heat_flux_device": [_P, _P, _P, _P],
     "rs_assemble_electrostatics_device": [_P, _P, _P, _P, _P],
+    "rs_assemble_magnetostatics_device": [_P, _P, _P, _P],
     "rs_electrostatics_process_device": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
     "rs_stress_process_device": [_P, _P, _P, _P, _P, _P, _P, _P],
     "rs_aggregates_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -580,6 +581,10 @@ class Assembly:
     def assemble_electrostatics_device(self, deps_r, drho, dphi_node, db) -> None:
         """RSolverElectrostatics prepare (volume): the heat form with eps_r e0 and source -rho."""
         _check(_lib.rs_assemble_electrostatics_device(self.h, deps_r.ptr, drho.ptr, dphi_node.ptr, db.ptr))
+
+    def assemble_magnetostatics_device(self, dcurrent_node, dfield_node, db) -> None:
+        """RSolverMagnetostatics prepare (volume): three negative Laplacians, curl source."""
+        _check(_lib.rs_assemble_magnetostatics_device(self.h, dcurrent_node.ptr, dfield_node.ptr, db.ptr))
 
     def electrostatics_process_device(self, deps_r, dsigma, dphi_node, dfield, dcurrent, denergy, dres,
                                       djoule) -> None:

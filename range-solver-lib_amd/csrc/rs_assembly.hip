@@ -314,6 +314,70 @@ __global__ __launch_bounds__(kBlock) void k_heat_flux(const double *__restrict__
     for (int c = 0; c < 3; c++) flux[3 * (uint64_t)orig + c] = Q[c];
 }
 
+// RSolverMagnetostatics::prepare + assemblyMatrix (src/rsolvermagnetostatics.cpp:44-145,
+// 234-287): three copies of the negative Laplacian, Ke[3m+c][3n+c] = -(g_m . g_n) V, source
+// fe[3m+c] = e0 V / 4 (g_m x J_m)_c with the node current density J, lifting of the prescribed
+// field on fixed nodes.  One launch per colour, added into the 3-DOF CSR through the scatter map.
+struct MagnetoArgs {
+    const double *xyz;
+    const uint4 *conn;
+    const uint32_t *map;        // 144 positions per element, [(4m+n)*9 + 3i+j]
+    const uint32_t *book;       // 3*node+c -> dof
+    const double *current, *field;
+    double *val, *b;
+};
+
+__global__ __launch_bounds__(kBlock) void k_magneto_colour(MagnetoArgs a, uint32_t e0, uint32_t e1)
+{
+    const uint32_t e = e0 + blockIdx.x * kBlock + threadIdx.x;
+    if (e >= e1) return;
+    const uint4 nd4 = a.conn[e];
+    const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+    double X[4][3], g[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) X[m][c] = a.xyz[3 * (uint64_t)nd[m] + c];
+    const double V = tet_gradients(X, g);
+    double K[4][4], fe[12];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+#pragma unroll
+        for (int n = 0; n < 4; n++) K[m][n] = -((g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]) * V);
+        const double *Jm = a.current + 3 * (uint64_t)nd[m];
+        const double fv = V * 0.25 * 8.854187817e-12;
+        fe[3 * m + 0] = fv * (-g[m][2] * Jm[1] + g[m][1] * Jm[2]);
+        fe[3 * m + 1] = fv * (g[m][2] * Jm[0] - g[m][0] * Jm[2]);
+        fe[3 * m + 2] = fv * (-g[m][1] * Jm[0] + g[m][0] * Jm[1]);
+    }
+    bool free_[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) free_[m] = a.book[3 * (uint64_t)nd[m]] != RS_EOD;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (free_[m]) continue;
+        const double *H = a.field + 3 * (uint64_t)nd[m];
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) fe[3 * n + c] -= K[n][m] * H[c];
+    }
+    const uint32_t *mp = a.map + 144ull * e;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (!free_[m]) continue;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a.b[a.book[3 * (uint64_t)nd[m] + c]] += fe[3 * m + c];
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const uint32_t pos = mp[(4 * m + n) * 9 + 4 * c];
+                if (pos != RS_EOD) a.val[pos] += K[m][n];
+            }
+        }
+    }
+}
+
 // RSolverElectrostatics (src/rsolverelectrostatics.cpp): the volume assembly (:300-345) is
 // the heat form with coefficient eps_r * e0 and source -rho (fe -= rho N detJ w); e0 as
 // RSolverGeneric::e0 (src/rsolvergeneric.cpp:13).
@@ -1040,3 +1104,30 @@ extern "C" int rs_electrostatics_process_device(rs_assembly *S, const double *d_
     RS_HIP(hipGetLastError());
     return timed_end(ev, S->last_ms);
 }
+
+extern "C" int rs_assemble_magnetostatics_device(rs_assembly *S, const double *d_current_node,
+                                                 const double *d_field_node, double *d_b)
+{
+    if (!S || !d_current_node || !d_field_node || !d_b)
+        return fail(RS_ERR_INVALID_ARG, "rs_assemble_magnetostatics_device: null argument");
+    if (S->dpn != 3) return fail(RS_ERR_INVALID_ARG, "rs_assemble_magnetostatics_device: assembly needs 3 dofs per node");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    hipEvent_t ev[2];
+    if ((rc = timed_begin(ev))) return rc;
+    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+    if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
+    MagnetoArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p), S->map.p, S->dbook.p, d_current_node,
+                   d_field_node, S->A.val.p, d_b };
+    for (uint32_t c = 0; c < S->n_colours; c++) {
+        const uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+        if (e1 > e0) k_magneto_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+    }
+    RS_HIP(hipGetLastError());
+    if ((rc = timed_end(ev, S->last_ms))) return rc;
+    if ((rc = matrix_frobenius(S->A))) return rc;
+    matrix_values_changed(S->A);
+    return RS_OK;
+}
+
