@@ -61,6 +61,10 @@ def load() -> C.CDLL:
     lib.or_spmv.argtypes = [C.c_uint, _P, _P, _P, _P, _P]
     lib.or_electrostatics_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.or_magnetostatics_assemble.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, C.c_uint32, _P, _P]
+    lib.or_fluid_assemble.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, _P, C.c_uint32, _P, _P]
+    lib.or_fluid_update.argtypes = [C.c_uint32, _P, _P, C.c_int, C.c_double, _P, _P, _P, _P]
+    lib.or_fluid_update.restype = None
     lib.or_heat_flux.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P]
     lib.or_stress_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.or_eigen_solve.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_double, C.POINTER(OrConf), C.c_uint, _P, _P,
@@ -320,6 +324,43 @@ def electrostatics_process(xyz, tets, eps_r, sigma, phi_node):
     load().or_electrostatics_process(nn, _p(xyz), ne, _p(tets), _p(eps_r), _p(sigma), _p(phi), _p(f), _p(j),
                                      _p(en), _p(rs_), _p(jh))
     return f.reshape(ne, 3), j.reshape(ne, 3), en, rs_, jh
+
+
+def fluid_assemble(xyz, tets, rho, mu, gravity, state, v_old, acc, stream_velocity, alpha, dt, unsteady, book,
+                   n_dof):
+    """RSolverFluid::prepare (volume) + computeElementConstantDerivative + assemblyMatrix restated;
+    state is (vx, vy, vz, p) per node, v_old and acc 3 per node, gravity 3 per element.
+    Returns (A as scipy CSR, b)."""
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    ne = tets.size // 4
+    args = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), shp)).reshape(-1)
+            for v, shp in ((rho, (ne,)), (mu, (ne,)), (gravity, (ne, 3)))]
+    st = np.ascontiguousarray(state, np.float64).reshape(-1)
+    vo = np.ascontiguousarray(v_old, np.float64).reshape(-1)
+    ac = np.ascontiguousarray(acc, np.float64).reshape(-1)
+    book = np.ascontiguousarray(book, np.uint32)
+    A = lib.or_sparse_create(n_dof)
+    b = np.zeros(max(1, n_dof))
+    lib.or_fluid_assemble(xyz.size // 3, _p(xyz), ne, _p(tets), _p(args[0]), _p(args[1]), _p(args[2]), _p(st),
+                          _p(vo), _p(ac), float(stream_velocity), float(alpha), float(dt), int(bool(unsteady)),
+                          _p(book), n_dof, A, _p(b))
+    return _export(A, n_dof), b[:n_dof]
+
+
+def fluid_update(book, x, unsteady, dt, state, v_old, acc):
+    """RSolverFluid::solve write-back restated; updates state, v_old, acc in place (float64,
+    contiguous) and returns the norms (|v| before, |v| after, |p| before, |p| after)."""
+    lib = load()
+    book = np.ascontiguousarray(book, np.uint32)
+    x = np.ascontiguousarray(x, np.float64)
+    for a in (state, v_old, acc):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    norms = np.zeros(4)
+    lib.or_fluid_update(state.size // 4, _p(book), _p(x), int(bool(unsteady)), float(dt), _p(state), _p(v_old),
+                        _p(acc), _p(norms))
+    return norms
 
 
 def magnetostatics_assemble(xyz, tets, current_node, field_node, book, n_dof):

@@ -822,6 +822,216 @@ int or_magnetostatics_assemble(uint32_t nn, const double *xyz, uint32_t ne, cons
     return 0;
 }
 
+/* ------------------------------------------------------------------ fluid
+ * RSolverFluid::prepare for the volume elements (src/rsolverfluid.cpp:306-331 element
+ * averages of velocity and pressure, :425-437 element loop), computeElementConstantDerivative
+ * (:1845-2201, the stabilised tetrahedron: SUPG, PSPG and LSIC parameters :1947-1987),
+ * computeElementScales (:2235-2268, tetrahedron h_n = cbrt(6 V / pi)) and assemblyMatrix
+ * (:2399-2419).  Four unknowns per node (vx, vy, vz, p), book[4 node + c]; the system is for
+ * the increments, so disabled positions are simply left out (no lifting).  The element
+ * constants iN, iNiN and wt (RElement::getMassVector, getMassMatrix, getTotalWeightFactor)
+ * are the sums over the integration points of w N_m, w N_m N_n and w. */
+int or_fluid_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                      const double *rho, const double *mu, const double *grav, const double *state,
+                      const double *v_old, const double *acc, double stream_velocity, double alpha,
+                      double dt, int unsteady, const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b)
+{
+    (void)nn;
+    memset(b, 0, sizeof(double) * n_dof);
+    double iN[4] = { 0 }, iNiN[4][4] = { { 0 } }, wt = 0.0;
+    for (int ip = 0; ip < 4; ip++) {
+        double N[4];
+        tet_shape(TET_IP[ip], N);
+        wt += TET_W;
+        for (int m = 0; m < 4; m++) {
+            iN[m] += TET_W * N[m];
+            for (int n = 0; n < 4; n++) iNiN[m][n] += TET_W * N[m] * N[n];
+        }
+    }
+    const double inv6 = 1.0 / 6.0;
+    const double mVh = stream_velocity, invmVh = 1.0 / stream_velocity;
+    const double alphaDt = alpha * dt;
+    for (int64_t e = 0; e < (int64_t)ne; e++) {
+        const uint32_t *nd = tets + 4 * e;
+        double X[4][3], J[3][3], B[4][3];
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) X[m][a] = xyz[3 * (size_t)nd[m] + a];
+        const double detJ = tet_jacobian(X, J);
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++)
+                B[m][a] = TET_DN[m][0] * J[a][0] + TET_DN[m][1] * J[a][1] + TET_DN[m][2] * J[a][2];
+        const double ro = rho[e], invro = 1.0 / ro, u = mu[e];
+        const double hn = cbrt(6.0 * (fabs(detJ) / 6.0) / 3.14159265358979323846); /* RConstants::pi */
+        double ve[3] = { 0, 0, 0 }, veo[3] = { 0, 0, 0 }, p = 0.0, ax[4], ay[4], az[4];
+        for (int m = 0; m < 4; m++) {
+            const double *st = state + 4 * (size_t)nd[m];
+            for (int c = 0; c < 3; c++) { ve[c] += st[c]; veo[c] += v_old[3 * (size_t)nd[m] + c]; }
+            p += st[3];
+            ax[m] = acc[3 * (size_t)nd[m] + 0];
+            ay[m] = acc[3 * (size_t)nd[m] + 1];
+            az[m] = acc[3 * (size_t)nd[m] + 2];
+        }
+        for (int c = 0; c < 3; c++) { ve[c] *= 0.25; veo[c] *= 0.25; }
+        p *= 0.25;
+        const double *g = grav + 3 * e;
+        const double mvh = sqrt(veo[0] * veo[0] + veo[1] * veo[1] + veo[2] * veo[2]);
+        const double invmvh = 1.0 / mvh;
+        double s[3] = { 0, 0, 0 };
+        if (mvh > 0.0) for (int c = 0; c < 3; c++) s[c] = veo[c] / mvh;
+        double vdiv[4], vex[3] = { 0, 0, 0 }, vey[3] = { 0, 0, 0 }, vez[3] = { 0, 0, 0 };
+        double px = 0, py = 0, pz = 0, h = 0;
+        for (int m = 0; m < 4; m++) {                                          /* :1913-1940 */
+            const double *st = state + 4 * (size_t)nd[m];
+            vdiv[m] = ve[0] * B[m][0] + ve[1] * B[m][1] + ve[2] * B[m][2];
+            for (int c = 0; c < 3; c++) {
+                vex[c] += B[m][0] * st[c];
+                vey[c] += B[m][1] * st[c];
+                vez[c] += B[m][2] * st[c];
+            }
+            px += B[m][0] * st[3];
+            py += B[m][1] * st[3];
+            pz += B[m][2] * st[3];
+            h += fabs(s[0] * B[m][0] + s[1] * B[m][1] + s[2] * B[m][2]);
+        }
+        if (h != 0.0) h = 2.0 / h;
+        const double roD2u = 0.5 * ro / u;
+        const double Re = roD2u * mvh * h, Ren = roD2u * mVh * hn;
+        double Tsupg = 0.0, Tpspg = 0.0;
+        if (mvh > 0.0) Tsupg = (Re > 0.0 && Re <= 3.0) ? h * Re * invmvh * inv6 : h * invmvh * 0.5;
+        if (mVh > 0.0) Tpspg = (Ren > 0.0 && Ren <= 3.0) ? hn * Ren * invmVh * inv6 : hn * invmVh * 0.5;
+        const double Tlsic = mvh * h * 0.5;
+        const double vv[3] = { ve[0] * vex[0] + ve[1] * vey[0] + ve[2] * vez[0],
+                               ve[0] * vex[1] + ve[1] * vey[1] + ve[2] * vez[1],
+                               ve[0] * vex[2] + ve[1] * vey[2] + ve[2] * vez[2] };
+        const double divV = vex[0] + vey[1] + vez[2];
+        /* gradient of velocity component c along axis a: G[a][c] */
+        const double G[3][3] = { { vex[0], vex[1], vex[2] }, { vey[0], vey[1], vey[2] }, { vez[0], vez[1], vez[2] } };
+        const double gp[3] = { px, py, pz };
+        const double acc_n[4][3] = { { ax[0], ay[0], az[0] }, { ax[1], ay[1], az[1] },
+                                     { ax[2], ay[2], az[2] }, { ax[3], ay[3], az[3] } };
+        double Ae[16][16], be[16];
+        memset(Ae, 0, sizeof(Ae));
+        memset(be, 0, sizeof(be));
+        for (int m = 0; m < 4; m++) {
+            for (int n = 0; n < 4; n++) {
+                const double BB = B[m][0] * B[n][0] + B[m][1] * B[n][1] + B[m][2] * B[n][2];
+                const double kScale = ro * u * wt, cpeScale = ro * iNiN[m][n];
+                const double kte = Tsupg * ro * vdiv[m] * vdiv[n] * wt;
+                const double ktpe = Tsupg * ro * vdiv[m] * iN[n], ktppe = Tsupg * ro * iN[n];
+                const double ytpe = Tsupg * iN[n], epe = Tlsic * ro * wt;
+                const double ce = ro * iN[m] * vdiv[n];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        /* row (m, i), column (n, j); Bij = B[m][i] B[n][j] as in :2010-2018 */
+                        const double Bmi_nj = B[m][i] * B[n][j];
+                        const double Bmj_ni = B[m][j] * B[n][i];
+                        const double k = kScale * ((i == j ? BB : 0.0) + Bmj_ni);
+                        const double cpe = cpeScale * G[j][i];
+                        double steady = cpe + k + (i == j ? kte : 0.0) + ktpe * G[j][i]
+                                      + ktppe * B[m][j] * vv[i] + ytpe * B[m][j] * gp[i] + Bmi_nj * epe;
+                        double a = 0.0;
+                        if (unsteady) {
+                            const double mass = ro * iNiN[m][n];
+                            const double ctilde = Tsupg * ro * vdiv[m] * iN[n];
+                            const double ctpe = Tsupg * ro * iN[n];
+                            if (i == j) a += mass + ctilde;
+                            a += alphaDt * (ctpe * B[m][j] * acc_n[n][i]);
+                        }
+                        if (i == j) a += unsteady ? alphaDt * ce : ce;
+                        if (unsteady) a += alphaDt * (steady - Bmi_nj * epe) + dt * Bmi_nj * epe;
+                        else a += steady;
+                        Ae[4 * m + i][4 * n + j] += a;
+                    }
+                const double yte = Tsupg * vdiv[m] * wt, s12 = unsteady ? -dt : -1.0;
+                for (int i = 0; i < 3; i++) Ae[4 * m + i][4 * n + 3] += s12 * (B[m][i] * iN[n] + yte * B[n][i]);
+                const double bte = Tpspg * iN[n], ye = Tpspg * vdiv[n] * wt, ype = Tpspg * iN[n];
+                for (int j = 0; j < 3; j++) {
+                    const double ypej = ype * (B[m][0] * G[j][0] + B[m][1] * G[j][1] + B[m][2] * G[j][2]);
+                    if (unsteady)
+                        Ae[4 * m + 3][4 * n + j] += bte * B[m][j] + dt * (B[n][j] * iN[m]) + alphaDt * (ye * B[m][j] + ypej);
+                    else
+                        Ae[4 * m + 3][4 * n + j] += B[n][j] * iN[m] + ye * B[m][j] + ypej;
+                }
+                const double theta = Tpspg * (BB * invro) * wt;
+                Ae[4 * m + 3][4 * n + 3] += unsteady ? dt * theta : theta;
+            }
+            /* right-hand side :2150-2196 */
+            const double iNm = iN[m], vdivm = vdiv[m];
+            const double am[3] = { ax[m], ay[m], az[m] };
+            for (int i = 0; i < 3; i++) {
+                const double cv = iNm * ro * vv[i];
+                const double kv = u * (B[m][0] * G[0][i] + B[m][1] * G[1][i] + B[m][2] * G[2][i]
+                                      + B[m][0] * G[i][0] + B[m][1] * G[i][1] + B[m][2] * G[i][2]) * wt;
+                const double gv = B[m][i] * p * wt;
+                const double ktv = Tsupg * ro * vdivm * wt * vv[i];
+                const double ytv = Tsupg * vdivm * wt * gp[i];
+                const double ev = Tlsic * ro * divV * wt * B[m][i];
+                const double fv = ro * iNm * g[i] + Tsupg * ro * vdivm * wt * g[i];
+                const double res = cv + kv - gv + ktv - ytv + ev;
+                if (unsteady) be[4 * m + i] += dt * (fv - (iNm * ro * am[i] + Tsupg * ro * vdivm * wt * am[i] + res));
+                else be[4 * m + i] += fv - res;
+            }
+            const double Bg = B[m][0] * g[0] + B[m][1] * g[1] + B[m][2] * g[2];
+            const double Ba = B[m][0] * am[0] + B[m][1] * am[1] + B[m][2] * am[2];
+            const double Bv = B[m][0] * vv[0] + B[m][1] * vv[1] + B[m][2] * vv[2];
+            const double Bp = B[m][0] * px + B[m][1] * py + B[m][2] * pz;
+            if (unsteady)
+                be[4 * m + 3] += dt * (Tpspg * Bg * wt - (Tpspg * Ba * wt + iNm * divV + Tpspg * Bv * wt
+                                                          + Tpspg * (Bp * invro) * wt));
+            else
+                be[4 * m + 3] += Tpspg * Bg * wt - (iNm * divV + Tpspg * Bv * wt + Tpspg * (Bp * invro) * wt);
+        }
+        for (int r = 0; r < 16; r++) {
+            be[r] *= detJ;
+            for (int c = 0; c < 16; c++) Ae[r][c] *= detJ;
+        }
+        for (int r = 0; r < 16; r++) {                                          /* :2399-2419 */
+            const uint32_t mp = book[4 * (size_t)nd[r / 4] + r % 4];
+            if (mp == OR_EOD) continue;
+            b[mp] += be[r];
+            for (int c = 0; c < 16; c++) {
+                const uint32_t np = book[4 * (size_t)nd[c / 4] + c % 4];
+                if (np != OR_EOD) add_value(A, mp, np, Ae[r][c]);
+            }
+        }
+    }
+    return 0;
+}
+
+/* RSolverFluid::solve after the linear solve (src/rsolverfluid.cpp:519-624): the increments
+ * are added to the node velocities and pressures; in a steady run the old velocities are the
+ * ones before the update, in an unsteady run the acceleration is (v - v_old) / dt.  norms
+ * receives |v| before and after, |p| before and after (Euclidean over all nodes). */
+void or_fluid_update(uint32_t nn, const uint32_t *book, const double *x, int unsteady, double dt,
+                     double *state, double *v_old, double *acc, double norms[4])
+{
+    double u0 = 0, p0 = 0, u1 = 0, p1 = 0;
+    for (uint32_t i = 0; i < nn; i++) {
+        for (int c = 0; c < 3; c++) u0 += state[4 * (size_t)i + c] * state[4 * (size_t)i + c];
+        p0 += state[4 * (size_t)i + 3] * state[4 * (size_t)i + 3];
+    }
+    if (!unsteady)
+        for (uint32_t i = 0; i < nn; i++)
+            for (int c = 0; c < 3; c++) v_old[3 * (size_t)i + c] = state[4 * (size_t)i + c];
+    for (uint32_t i = 0; i < nn; i++)
+        for (int c = 0; c < 4; c++) {
+            const uint32_t d = book[4 * (size_t)i + c];
+            if (d != OR_EOD) state[4 * (size_t)i + c] += x[d];
+        }
+    if (unsteady)
+        for (uint32_t i = 0; i < nn; i++)
+            for (int c = 0; c < 3; c++)
+                acc[3 * (size_t)i + c] = (state[4 * (size_t)i + c] - v_old[3 * (size_t)i + c]) * (1.0 / dt);
+    for (uint32_t i = 0; i < nn; i++) {
+        for (int c = 0; c < 3; c++) u1 += state[4 * (size_t)i + c] * state[4 * (size_t)i + c];
+        p1 += state[4 * (size_t)i + 3] * state[4 * (size_t)i + 3];
+    }
+    norms[0] = sqrt(u0);
+    norms[1] = sqrt(u1);
+    norms[2] = sqrt(p0);
+    norms[3] = sqrt(p1);
+}
+
 /* ------------------------------------------------------------------ post-processing
  * RSolverHeat::process, volume elements (src/rsolverheat.cpp:646-692): per tet
  * B[m] = sum over the integration points of grad N_m * detJ / nInp; q = -sum_m B[m] k T_m. */

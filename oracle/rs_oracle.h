@@ -117,6 +117,16 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
 int or_magnetostatics_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
                                const double *current, const double *field, const uint32_t *book, uint32_t n_dof,
                                or_sparse *A, double *b);
+/* RSolverFluid prepare (volume) + computeElementConstantDerivative + assemblyMatrix,
+ * src/rsolverfluid.cpp:289-497, 1845-2201, 2399-2419; state = (vx, vy, vz, p) per node, v_old and
+ * acc 3 per node, gravity 3 per element, book 4 per node. */
+int or_fluid_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                      const double *rho, const double *mu, const double *grav, const double *state,
+                      const double *v_old, const double *acc, double stream_velocity, double alpha,
+                      double dt, int unsteady, const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b);
+/* RSolverFluid::solve write-back (src/rsolverfluid.cpp:519-624). */
+void or_fluid_update(uint32_t nn, const uint32_t *book, const double *x, int unsteady, double dt,
+                     double *state, double *v_old, double *acc, double norms[4]);
 /* RSolverHeat::process (volume), src/rsolverheat.cpp:646-692: 3 flux values per tet. */
 int or_heat_flux(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *k,
                  const double *t_node, double *flux);
