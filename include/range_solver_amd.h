@@ -190,7 +190,8 @@ int rs_mesh_plane_faces(uint32_t n_nodes, const double *xyz, uint32_t n_tets,
 int rs_field_uniform(uint64_t seed, uint64_t n, double *out);
 int rs_field_lognormal(uint64_t seed, uint64_t n, double mu, double sigma, double *out);
 /* RSolverGeneric::generateNodeBook/rebuildNodeBook (dofs_per_node 1) and
- * RSolverStress::generateNodeBook (dofs_per_node 3): fixed[n_nodes*dofs_per_node] marks
+ * RSolverStress::generateNodeBook (dofs_per_node 3) and RSolverFluid::generateNodeBook
+ * (dofs_per_node 4, src/rsolverfluid.cpp:1086-1207): fixed[n_nodes*dofs_per_node] marks
  * prescribed positions; positions of nodes in no element are disabled too.  book receives
  * 0..n-1 in increasing position order or RS_EOD.  *n_enabled receives n. */
 int rs_node_book(uint32_t n_nodes, uint32_t n_elems, uint32_t nodes_per_elem,
@@ -255,6 +256,28 @@ int rs_electrostatics_process_device(rs_assembly *S, const double *d_eps_r, cons
  * lifting of the prescribed field (3 per node). */
 int rs_assemble_magnetostatics_device(rs_assembly *S, const double *d_current_node, const double *d_field_node,
                                       double *d_b);
+/* RSolverFluid time-step parameters: streamVelocity (computeStreamVelocity, src/rsolverfluid.cpp:
+ * 2453-2536, > 0), the time-march coefficient alpha and step dt, and whether the time solver is
+ * enabled. */
+typedef struct rs_fluid_conf {
+    double stream_velocity;
+    double alpha;
+    double dt;
+    int32_t unsteady;
+} rs_fluid_conf;
+/* RSolverFluid::prepare, volume elements (src/rsolverfluid.cpp:289-497, 1845-2201, 2399-2419) on a
+ * 4-DOF assembly (positions 4 node + (vx, vy, vz, p)): the stabilised (SUPG/PSPG/LSIC) linear
+ * tetrahedron for the increments, enabled positions only.  rho, mu: per element; gravity: 3 per
+ * element; state: 4 per node (vx, vy, vz, p); v_old, acc: 3 per node (acc read only when
+ * unsteady).  Surface free-pressure elements (:392-428) and local rotations are not applied. */
+int rs_assemble_fluid_device(rs_assembly *S, const rs_fluid_conf *conf, const double *d_rho, const double *d_mu,
+                             const double *d_gravity, const double *d_state, const double *d_v_old,
+                             const double *d_acc, double *d_b);
+/* RSolverFluid::solve after the linear solve (src/rsolverfluid.cpp:519-624): state += x on the
+ * enabled positions; steady: v_old = the velocities before the update; unsteady: acc = (v - v_old)
+ * / dt.  norms (host, 4, may be NULL): |v| before, |v| after, |p| before, |p| after. */
+int rs_fluid_update_device(rs_assembly *S, const double *d_x, int32_t unsteady, double dt, double *d_state,
+                           double *d_v_old, double *d_acc, double *norms);
 /* RSolverStress::process, src/rsolverstress.cpp:1238-1408, static case without thermal load --
  * per tet the normal and shear stress measures and their sum (the reference's von Mises), and the
  * node forces sum_e Ke u_e (3 per node; reactions on the fixed nodes). */

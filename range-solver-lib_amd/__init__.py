@@ -77,6 +77,12 @@ class EigenInfo(C.Structure):
     _fields_ = [("outer_iterations", C.c_uint32), ("cg_iterations", C.c_uint64), ("solve_ms", C.c_double)]
 
 
+class FluidConf(C.Structure):
+    """RSolverFluid time-step parameters: stream velocity (> 0), time-march alpha, step dt, unsteady."""
+    _fields_ = [("stream_velocity", C.c_double), ("alpha", C.c_double), ("dt", C.c_double),
+                ("unsteady", C.c_int32)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("rounds", C.c_uint32), ("max_iterations", C.c_uint32),
                 ("total_iterations", C.c_uint64), ("systems_converged", C.c_uint32),
@@ -132,6 +138,8 @@ _SIGS = {
     "rs_heat_flux_device": [_P, _P, _P, _P],
     "rs_assemble_electrostatics_device": [_P, _P, _P, _P, _P],
     "rs_assemble_magnetostatics_device": [_P, _P, _P, _P],
+    "rs_assemble_fluid_device": [_P, C.POINTER(FluidConf), _P, _P, _P, _P, _P, _P, _P],
+    "rs_fluid_update_device": [_P, _P, C.c_int32, C.c_double, _P, _P, _P, _f64p],
     "rs_electrostatics_process_device": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
     "rs_stress_process_device": [_P, _P, _P, _P, _P, _P, _P, _P],
     "rs_aggregates_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -585,6 +593,20 @@ class Assembly:
     def assemble_magnetostatics_device(self, dcurrent_node, dfield_node, db) -> None:
         """RSolverMagnetostatics prepare (volume): three negative Laplacians, curl source."""
         _check(_lib.rs_assemble_magnetostatics_device(self.h, dcurrent_node.ptr, dfield_node.ptr, db.ptr))
+
+    def assemble_fluid_device(self, conf: "FluidConf", drho, dmu, dgravity, dstate, dv_old, dacc, db) -> None:
+        """RSolverFluid prepare (volume): the stabilised tetrahedron for the increments of
+        (vx, vy, vz, p); state 4 per node, v_old and acc 3 per node, gravity 3 per element."""
+        _check(_lib.rs_assemble_fluid_device(self.h, C.byref(conf), drho.ptr, dmu.ptr, dgravity.ptr, dstate.ptr,
+                                             dv_old.ptr, dacc.ptr if dacc is not None else None, db.ptr))
+
+    def fluid_update_device(self, dx, unsteady: bool, dt: float, dstate, dv_old, dacc) -> np.ndarray:
+        """RSolverFluid::solve write-back; returns (|v| before, |v| after, |p| before, |p| after)."""
+        norms = np.zeros(4)
+        _check(_lib.rs_fluid_update_device(self.h, dx.ptr, int(bool(unsteady)), dt, dstate.ptr, dv_old.ptr,
+                                           dacc.ptr if dacc is not None else None,
+                                           norms.ctypes.data_as(_f64p)))
+        return norms
 
     def electrostatics_process_device(self, deps_r, dsigma, dphi_node, dfield, dcurrent, denergy, dres,
                                       djoule) -> None:

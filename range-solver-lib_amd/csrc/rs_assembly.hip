@@ -378,6 +378,248 @@ __global__ __launch_bounds__(kBlock) void k_magneto_colour(MagnetoArgs a, uint32
     }
 }
 
+// RSolverFluid (src/rsolverfluid.cpp): the stabilised tetrahedron of
+// computeElementConstantDerivative (:1845-2201) -- SUPG / PSPG parameters switching at element
+// Reynolds number 3 (:1947-1983), LSIC |v| h / 2 (:1986) -- with the element averages of
+// prepare (:306-331) and h_n = cbrt(6 V / pi) (computeElementScales :2235-2268), scattered
+// into the (vx, vy, vz, p) CSR as assemblyMatrix does (:2399-2419: enabled positions only, no
+// lifting, the system is for increments).
+//
+// Sixteen lanes per element, one per node pair (m, n).  Each lane rebuilds the element
+// quantities (the 16 lanes of an element read the same addresses, served as broadcasts) and
+// owns the 4x4 block (m, n) of Ae; the lanes with n = 0 own node m's four right-hand-side
+// rows.  The linear tetrahedron's integration constants in closed form: sum w N_m = 1/24,
+// sum w N_m N_n = (1 + d_mn) / 120, sum w = 1/6, detJ = 6 V.
+struct FluidArgs {
+    const double *xyz;
+    const uint4 *conn;          // colour-sorted elements
+    const uint32_t *perm;       // colour-sorted index -> original element id
+    const uint32_t *map;        // 64 per element: [(4m+n)*4 + i] = CSR position of node n's first
+                                // enabled column in row (m, i), RS_EOD if none or the row is disabled
+    const uint4 *book;          // (vx, vy, vz, p) positions of a node -> dof
+    const double *rho, *mu, *grav;   // per original element (grav: 3 per element)
+    const double *state;        // 4 per node: vx vy vz p
+    const double *v_old, *acc;  // 3 per node
+    double *val, *b;
+    double stream_velocity, alpha, dt;
+    int unsteady;
+};
+
+constexpr uint32_t kFluidLanes = 16;
+constexpr double kPi = 3.14159265358979323846;
+
+__global__ __launch_bounds__(kBlock) void k_fluid_colour(FluidArgs a, uint32_t e0, uint32_t e1)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t e = e0 + t / kFluidLanes;
+    if (e >= e1) return;
+    const uint32_t lane = t % kFluidLanes, m = lane >> 2, n = lane & 3u;
+    const uint4 nd4 = a.conn[e];
+    const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+    const uint32_t orig = a.perm[e];
+    double X[4][3], B[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) X[q][c] = a.xyz[3 * (uint64_t)nd[q] + c];
+    const double V = tet_gradients(X, B);
+    const double detJ = 6.0 * V;
+    const double wt = 1.0 / 6.0, iN = 1.0 / 24.0, inv6 = 1.0 / 6.0;
+    const double ro = a.rho[orig], invro = 1.0 / ro, u = a.mu[orig];
+    const double hn = cbrt(6.0 * fabs(V) / kPi);
+    double st[4][4], ve[3] = { 0, 0, 0 }, veo[3] = { 0, 0, 0 }, p = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) st[q][c] = a.state[4 * (uint64_t)nd[q] + c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            ve[c] += st[q][c];
+            veo[c] += a.v_old[3 * (uint64_t)nd[q] + c];
+        }
+        p += st[q][3];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) { ve[c] *= 0.25; veo[c] *= 0.25; }
+    p *= 0.25;
+    const double mvh = sqrt(veo[0] * veo[0] + veo[1] * veo[1] + veo[2] * veo[2]);
+    double sdir[3] = { 0, 0, 0 };
+    if (mvh > 0.0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) sdir[c] = veo[c] / mvh;
+    }
+    // G[a][c] = d v_c / d x_a, gp = grad p, vdiv[q] = ve . grad N_q
+    double vdiv[4], G[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } }, gp[3] = { 0, 0, 0 }, h = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        vdiv[q] = ve[0] * B[q][0] + ve[1] * B[q][1] + ve[2] * B[q][2];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) G[d][c] += B[q][d] * st[q][c];
+            gp[d] += B[q][d] * st[q][3];
+        }
+        h += fabs(sdir[0] * B[q][0] + sdir[1] * B[q][1] + sdir[2] * B[q][2]);
+    }
+    if (h != 0.0) h = 2.0 / h;
+    const double roD2u = 0.5 * ro / u;
+    const double Re = roD2u * mvh * h, Ren = roD2u * a.stream_velocity * hn;
+    double Tsupg = 0.0, Tpspg = 0.0;
+    if (mvh > 0.0) Tsupg = (Re > 0.0 && Re <= 3.0) ? h * Re * (1.0 / mvh) * inv6 : h * (1.0 / mvh) * 0.5;
+    if (a.stream_velocity > 0.0) {
+        const double inv = 1.0 / a.stream_velocity;
+        Tpspg = (Ren > 0.0 && Ren <= 3.0) ? hn * Ren * inv * inv6 : hn * inv * 0.5;
+    }
+    const double Tlsic = mvh * h * 0.5;
+    double vv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) vv[c] = ve[0] * G[0][c] + ve[1] * G[1][c] + ve[2] * G[2][c];
+    const double divV = G[0][0] + G[1][1] + G[2][2];
+    const bool unsteady = a.unsteady != 0;
+    const double dt = a.dt, alphaDt = a.alpha * a.dt;
+
+    // ---- block (m, n) of Ae
+    const double iNiN = (m == n ? 2.0 : 1.0) / 120.0;
+    const double BB = B[m][0] * B[n][0] + B[m][1] * B[n][1] + B[m][2] * B[n][2];
+    const double kScale = ro * u * wt, cpeScale = ro * iNiN;
+    const double kte = Tsupg * ro * vdiv[m] * vdiv[n] * wt;
+    const double ktpe = Tsupg * ro * vdiv[m] * iN, ktppe = Tsupg * ro * iN;
+    const double ytpe = Tsupg * iN, epe = Tlsic * ro * wt;
+    const double ce = ro * iN * vdiv[n];
+    double an[3] = { 0, 0, 0 };
+    if (unsteady) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) an[c] = a.acc[3 * (uint64_t)nd[n] + c];
+    }
+    double blk[4][4];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double Bmi_nj = B[m][i] * B[n][j];
+            const double Bmj_ni = B[m][j] * B[n][i];
+            const double k = kScale * ((i == j ? BB : 0.0) + Bmj_ni);
+            const double cpe = cpeScale * G[j][i];
+            const double steady = cpe + k + (i == j ? kte : 0.0) + ktpe * G[j][i] + ktppe * B[m][j] * vv[i]
+                                + ytpe * B[m][j] * gp[i] + Bmi_nj * epe;
+            double v = 0.0;
+            if (unsteady) {
+                if (i == j) v += ro * iNiN + Tsupg * ro * vdiv[m] * iN;
+                v += alphaDt * (Tsupg * ro * iN * B[m][j] * an[i]);
+                if (i == j) v += alphaDt * ce;
+                v += alphaDt * (steady - Bmi_nj * epe) + dt * Bmi_nj * epe;
+            } else {
+                if (i == j) v += ce;
+                v += steady;
+            }
+            blk[i][j] = v * detJ;
+        }
+    const double yte = Tsupg * vdiv[m] * wt, s12 = unsteady ? -dt : -1.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) blk[i][3] = s12 * (B[m][i] * iN + yte * B[n][i]) * detJ;
+    const double bte = Tpspg * iN, ye = Tpspg * vdiv[n] * wt, ype = Tpspg * iN;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const double ypej = ype * (B[m][0] * G[j][0] + B[m][1] * G[j][1] + B[m][2] * G[j][2]);
+        const double v = unsteady ? bte * B[m][j] + dt * (B[n][j] * iN) + alphaDt * (ye * B[m][j] + ypej)
+                                  : B[n][j] * iN + ye * B[m][j] + ypej;
+        blk[3][j] = v * detJ;
+    }
+    const double theta = Tpspg * (BB * invro) * wt;
+    blk[3][3] = (unsteady ? dt * theta : theta) * detJ;
+
+    const uint4 bk4 = a.book[nd[n]];
+    const uint32_t bn[4] = { bk4.x, bk4.y, bk4.z, bk4.w };
+    const uint32_t *mp = a.map + 64ull * e + (4 * m + n) * 4;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t pos = mp[i];
+        if (pos == RS_EOD) continue;
+        uint32_t k = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (bn[j] != RS_EOD) a.val[pos + k++] += blk[i][j];
+    }
+
+    // ---- right-hand side of node m (:2150-2196)
+    if (n != 0) return;
+    double am[3] = { 0, 0, 0 };
+    if (unsteady) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) am[c] = a.acc[3 * (uint64_t)nd[m] + c];
+    }
+    const double *g = a.grav + 3 * (uint64_t)orig;
+    const double gr[3] = { g[0], g[1], g[2] };
+    const double vdivm = vdiv[m];
+    double be[4];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double cv = iN * ro * vv[i];
+        const double kv = u * (B[m][0] * G[0][i] + B[m][1] * G[1][i] + B[m][2] * G[2][i]
+                              + B[m][0] * G[i][0] + B[m][1] * G[i][1] + B[m][2] * G[i][2]) * wt;
+        const double gv = B[m][i] * p * wt;
+        const double ktv = Tsupg * ro * vdivm * wt * vv[i];
+        const double ytv = Tsupg * vdivm * wt * gp[i];
+        const double ev = Tlsic * ro * divV * wt * B[m][i];
+        const double fv = ro * iN * gr[i] + Tsupg * ro * vdivm * wt * gr[i];
+        const double res = cv + kv - gv + ktv - ytv + ev;
+        be[i] = unsteady ? dt * (fv - (iN * ro * am[i] + Tsupg * ro * vdivm * wt * am[i] + res)) : fv - res;
+    }
+    const double Bg = B[m][0] * gr[0] + B[m][1] * gr[1] + B[m][2] * gr[2];
+    const double Ba = B[m][0] * am[0] + B[m][1] * am[1] + B[m][2] * am[2];
+    const double Bv = B[m][0] * vv[0] + B[m][1] * vv[1] + B[m][2] * vv[2];
+    const double Bp = B[m][0] * gp[0] + B[m][1] * gp[1] + B[m][2] * gp[2];
+    be[3] = unsteady ? dt * (Tpspg * Bg * wt - (Tpspg * Ba * wt + iN * divV + Tpspg * Bv * wt + Tpspg * (Bp * invro) * wt))
+                     : Tpspg * Bg * wt - (iN * divV + Tpspg * Bv * wt + Tpspg * (Bp * invro) * wt);
+    const uint4 bm4 = a.book[nd[m]];
+    const uint32_t bm[4] = { bm4.x, bm4.y, bm4.z, bm4.w };
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        if (bm[i] != RS_EOD) a.b[bm[i]] += be[i] * detJ;
+}
+
+// RSolverFluid::solve write-back (src/rsolverfluid.cpp:519-624): state += x through the node
+// book; a steady run keeps the pre-update velocities as v_old, an unsteady one sets the
+// acceleration (v - v_old) / dt.  Per block: sums of |v|^2 and p^2 before and after, in a fixed
+// order (the host adds the block partials in block order).
+__global__ __launch_bounds__(kBlock) void k_fluid_update(const uint4 *__restrict__ book, uint32_t nn,
+                                                         const double *__restrict__ x, int unsteady, double inv_dt,
+                                                         double *__restrict__ state, double *__restrict__ v_old,
+                                                         double *__restrict__ acc, double *__restrict__ part)
+{
+    __shared__ double lds[4 * kBlock / kWave];
+    double s[4] = { 0, 0, 0, 0 };
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nn; i += gridDim.x * kBlock) {
+        const uint4 bk4 = book[i];
+        const uint32_t bk[4] = { bk4.x, bk4.y, bk4.z, bk4.w };
+        double v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) v[c] = state[4 * (uint64_t)i + c];
+        s[0] += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        s[2] += v[3] * v[3];
+        if (!unsteady) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) v_old[3 * (uint64_t)i + c] = v[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (bk[c] != RS_EOD) v[c] += x[bk[c]];
+#pragma unroll
+        for (int c = 0; c < 4; c++) state[4 * (uint64_t)i + c] = v[c];
+        if (unsteady) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[3 * (uint64_t)i + c] = (v[c] - v_old[3 * (uint64_t)i + c]) * inv_dt;
+        }
+        s[1] += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        s[3] += v[3] * v[3];
+    }
+    block_sum<4, kBlock>(s, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) part[4 * blockIdx.x + k] = s[k];
+    }
+}
+
 // RSolverElectrostatics (src/rsolverelectrostatics.cpp): the volume assembly (:300-345) is
 // the heat form with coefficient eps_r * e0 and source -rho (fe -= rho N detJ w); e0 as
 // RSolverGeneric::e0 (src/rsolvergeneric.cpp:13).
@@ -642,6 +884,7 @@ struct rs_assembly {
     DBuf<uint32_t> conn, perm, map, dbook;
     DBuf<uint32_t> conn0;                   // connectivity in the caller's order (post-processing)
     DBuf<double> coeff;                     // mapped element coefficients (electrostatics)
+    DBuf<double> part;                      // block partials of the fluid write-back norms
     DBuf<uint32_t> dof_node, inc_ptr, inc;  // row-gather heat assembly
     DBuf<double> rec_grad, rec_vol;         // row records (built once per mesh)
     DBuf<uint32_t> rec_elem, rec_slots, rec_count;
@@ -679,8 +922,8 @@ static int timed_end(hipEvent_t (&ev)[2], double &ms)
 extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t n_tets, const uint32_t *tets,
                                   uint32_t dofs_per_node, const uint8_t *fixed, rs_assembly **out)
 {
-    if (!xyz || !tets || !out || (dofs_per_node != 1 && dofs_per_node != 3))
-        return fail(RS_ERR_INVALID_ARG, "rs_assembly_create: bad arguments (dofs_per_node must be 1 or 3)");
+    if (!xyz || !tets || !out || (dofs_per_node != 1 && dofs_per_node != 3 && dofs_per_node != 4))
+        return fail(RS_ERR_INVALID_ARG, "rs_assembly_create: bad arguments (dofs_per_node must be 1, 3 or 4)");
     for (uint64_t i = 0; i < 4ull * n_tets; i++)
         if (tets[i] >= n_nodes) return fail(RS_ERR_INVALID_ARG, "rs_assembly_create: node id out of range");
     int rc = require_runtime();
@@ -705,7 +948,7 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
         std::vector<uint32_t> fill(S->colour_off.begin(), S->colour_off.end() - 1);
         for (uint32_t e = 0; e < n_tets; e++) perm[fill[colour[e]]++] = e;
     }
-    const uint32_t per = dpn == 1 ? 16 : 144;
+    const uint32_t per = dpn == 1 ? 16 : dpn == 3 ? 144 : 64;
     std::vector<uint32_t> map((uint64_t)per * n_tets);
     const uint32_t *rp = pat.rowptr.data(), *cl = pat.col.data();
     const std::vector<uint32_t> &book = S->book;
@@ -723,6 +966,18 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
                     for (int n = 0; n < 4; n++) {
                         uint32_t r = book[nd[m]], c = book[nd[n]];
                         map[16 * s + 4 * m + n] = (r == RS_EOD || c == RS_EOD) ? RS_EOD : find(r, c);
+                    }
+            } else if (dpn == 4) {
+                // the enabled columns of one node are adjacent in a row (the node book is monotone
+                // in (node, component)), so one position per (m, n, row component) locates them
+                for (int m = 0; m < 4; m++)
+                    for (int n = 0; n < 4; n++) {
+                        uint32_t c0 = RS_EOD;
+                        for (int j = 0; j < 4 && c0 == RS_EOD; j++) c0 = book[4ull * nd[n] + j];
+                        for (int i = 0; i < 4; i++) {
+                            uint32_t r = book[4ull * nd[m] + i];
+                            map[64 * s + (4 * m + n) * 4 + i] = (r == RS_EOD || c0 == RS_EOD) ? RS_EOD : find(r, c0);
+                        }
                     }
             } else {
                 for (int m = 0; m < 4; m++)
@@ -1131,3 +1386,67 @@ extern "C" int rs_assemble_magnetostatics_device(rs_assembly *S, const double *d
     return RS_OK;
 }
 
+// RSolverFluid::prepare, volume elements (src/rsolverfluid.cpp:289-497 with
+// computeElementConstantDerivative :1845-2201 and assemblyMatrix :2399-2419).
+extern "C" int rs_assemble_fluid_device(rs_assembly *S, const rs_fluid_conf *conf, const double *d_rho,
+                                        const double *d_mu, const double *d_gravity, const double *d_state,
+                                        const double *d_v_old, const double *d_acc, double *d_b)
+{
+    if (!S || !conf || !d_rho || !d_mu || !d_gravity || !d_state || !d_v_old || !d_b || (conf->unsteady && !d_acc))
+        return fail(RS_ERR_INVALID_ARG, "rs_assemble_fluid_device: null argument");
+    if (S->dpn != 4) return fail(RS_ERR_INVALID_ARG, "rs_assemble_fluid_device: assembly needs 4 dofs per node");
+    if (!(conf->stream_velocity > 0.0))
+        return fail(RS_ERR_INVALID_ARG, "rs_assemble_fluid_device: stream velocity must be positive");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    hipEvent_t ev[2];
+    if ((rc = timed_begin(ev))) return rc;
+    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+    if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
+    FluidArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p), S->perm.p, S->map.p,
+                 reinterpret_cast<const uint4 *>(S->dbook.p), d_rho, d_mu, d_gravity, d_state, d_v_old,
+                 conf->unsteady ? d_acc : d_v_old, S->A.val.p, d_b, conf->stream_velocity, conf->alpha, conf->dt,
+                 conf->unsteady ? 1 : 0 };
+    for (uint32_t c = 0; c < S->n_colours; c++) {
+        const uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+        if (e1 > e0) k_fluid_colour<<<grid_for((uint64_t)(e1 - e0) * kFluidLanes, kBlock), kBlock, 0, st>>>(a, e0, e1);
+    }
+    RS_HIP(hipGetLastError());
+    if ((rc = timed_end(ev, S->last_ms))) return rc;
+    if ((rc = matrix_frobenius(S->A))) return rc;
+    matrix_values_changed(S->A);
+    return RS_OK;
+}
+
+// RSolverFluid::solve after the linear solve (src/rsolverfluid.cpp:519-624).
+extern "C" int rs_fluid_update_device(rs_assembly *S, const double *d_x, int32_t unsteady, double dt,
+                                      double *d_state, double *d_v_old, double *d_acc, double *norms)
+{
+    if (!S || !d_x || !d_state || !d_v_old || (unsteady && !d_acc))
+        return fail(RS_ERR_INVALID_ARG, "rs_fluid_update_device: null argument");
+    if (S->dpn != 4) return fail(RS_ERR_INVALID_ARG, "rs_fluid_update_device: assembly needs 4 dofs per node");
+    if (unsteady && !(dt > 0.0)) return fail(RS_ERR_INVALID_ARG, "rs_fluid_update_device: dt must be positive");
+    int rc = require_runtime();
+    if (rc) return rc;
+    const uint32_t nb = vblocks(S->n_nodes);
+    if (S->part.n < 4ull * nb && (rc = S->part.alloc(4ull * nb))) return rc;
+    hipStream_t st = runtime().stream;
+    k_fluid_update<<<nb, kBlock, 0, st>>>(reinterpret_cast<const uint4 *>(S->dbook.p), S->n_nodes, d_x,
+                                          unsteady ? 1 : 0, unsteady ? 1.0 / dt : 0.0, d_state, d_v_old,
+                                          unsteady ? d_acc : d_v_old, S->part.p);
+    RS_HIP(hipGetLastError());
+    std::vector<double> part(4ull * nb);
+    RS_HIP(hipMemcpyAsync(part.data(), S->part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    double sum[4] = { 0, 0, 0, 0 };
+    for (uint32_t blk = 0; blk < nb; blk++)
+        for (int k = 0; k < 4; k++) sum[k] += part[4ull * blk + k];
+    if (norms) {
+        norms[0] = std::sqrt(sum[0]);
+        norms[1] = std::sqrt(sum[1]);
+        norms[2] = std::sqrt(sum[2]);
+        norms[3] = std::sqrt(sum[3]);
+    }
+    return RS_OK;
+}
