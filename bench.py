@@ -567,6 +567,123 @@ def magnetostatics(rs, args) -> int:
     return 0
 
 
+def fluid(rs, args) -> int:
+    """Steady duct flow (RSolverFluid) on (4c x c x c) cells x 6 tets, c = args.fluid_cells, length
+    4: parabolic inflow at x = 0, no-slip walls y, z = 0, 1, pressure 0 at x = 4, rho = 1,
+    mu = 0.02 (Reynolds number 50 on the duct width).  A step is one Picard iteration as the
+    reference runs it: GPU assembly of the stabilised system, restarted GMRES + Jacobi to relres
+    1e-8 (restart args.fluid_restart: at 128 x 32 x 32 a restart of 100 stagnates near 3e-2),
+    write-back.  The oracle times its assembly on a sub-duct (scaled per tet) and partial GMRES
+    cycles on the first full-size system.  Prints one JSON line."""
+    import __graft_entry__ as ge
+    c = args.fluid_cells
+    lx = 4.0
+    xyz, tets = rs.kuhn_box(4 * c, c, c, lx, 1.0, 1.0)
+    nn, ne = xyz.shape[0], tets.shape[0]
+
+    def duct(xyz):
+        x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+        near = lambda a, v: np.abs(a - v) < 1e-12
+        wall = near(y, 0) | near(y, 1) | near(z, 0) | near(z, 1)
+        fixed = np.zeros((xyz.shape[0], 4), np.uint8)
+        fixed[wall | near(x, 0), :3] = 1
+        fixed[near(x, lx), 3] = 1
+        inflow = near(x, 0) & ~wall
+        st = np.zeros((xyz.shape[0], 4))
+        st[inflow, 0] = 16 * y[inflow] * (1 - y[inflow]) * z[inflow] * (1 - z[inflow])
+        return fixed.reshape(-1), st
+
+    fixed, state = duct(xyz)
+    mu = 0.02
+    t0 = time.perf_counter()
+    asm = rs.Assembly(xyz, tets, 4, fixed)
+    log(f"fluid: {nn} nodes, {ne} tets, {asm.n_dof} unknowns, {asm.nnz} nonzeros, {asm.n_colours} colours "
+        f"(set-up {time.perf_counter() - t0:.1f} s)")
+    conf = rs.FluidConf(1.0, 1.0, 1.0, 0)
+    drho, dmu, dg = (rs.DeviceArray.from_host(a) for a in (np.ones(ne), np.full(ne, mu), np.zeros(3 * ne)))
+    ds = rs.DeviceArray.from_host(state.reshape(-1).copy())
+    dvo = rs.DeviceArray.from_host(state[:, :3].reshape(-1).copy())
+    dacc = rs.DeviceArray.from_host(np.zeros(3 * nn))
+    n = asm.n_dof
+    db, dx, dz = rs.DeviceArray(n), rs.DeviceArray(n), rs.DeviceArray.from_host(np.zeros(n))
+    restart = args.fluid_restart
+    sconf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=60, n_inner_iterations=restart,
+                                solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES)
+    solver = rs.RMatrixSolver(sconf)
+    lib = rs.load()
+    steps = []
+    for it in range(args.warmup + args.steps):
+        rs.synchronize()
+        t0 = time.perf_counter()
+        asm.assemble_fluid_device(conf, drho, dmu, dg, ds, dvo, dacc, db)
+        a_ms = asm.last_ms
+        rs._check(lib.rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * n))
+        info = solver.solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+        norms = asm.fluid_update_device(dx, False, 1.0, ds, dvo, dacc)
+        rs.synchronize()
+        t = time.perf_counter() - t0
+        if it >= args.warmup:
+            steps.append((t, a_ms, info.solve_ms, info.inner_iterations, info.true_relres, info.iterations,
+                          info.converged))
+        log(f"fluid step {it}: {t * 1e3:.1f} ms (assembly {a_ms:.2f} ms, GMRES {info.solve_ms:.1f} ms, "
+            f"{info.inner_iterations} inner its, relres {info.true_relres:.2e}, converged {info.converged}), "
+            f"|v| {norms[1]:.4f} |p| {norms[3]:.4f}")
+    ts = np.array([x[0] for x in steps])
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        orc.load()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        sub = max(2, c // 4)
+        sxyz, stets = rs.kuhn_box(4 * sub, sub, sub, lx, 1.0, 1.0)
+        sfixed, sstate = duct(sxyz)
+        sbook, sndof = orc.node_book(sxyz.shape[0], stets, 4, sfixed)
+        sne = stets.shape[0]
+        t0 = time.perf_counter()
+        orc.fluid_assemble(sxyz, stets, np.ones(sne), np.full(sne, mu), np.zeros((sne, 3)), sstate,
+                           sstate[:, :3].copy(), np.zeros((sxyz.shape[0], 3)), 1.0, 1.0, 1.0, 0, sbook, sndof)
+        t_asm = (time.perf_counter() - t0) * ne / sne
+        # partial GMRES cycles of 100 and 200 inner iterations on the full first system: cost
+        # t(k) = a k + c k^2 (SpMV + preconditioner linear, Gram-Schmidt quadratic in the cycle
+        # length), applied to the GPU's cycles (each taken at the mean cycle length)
+        book = asm.book()
+        A_full, b_full = orc.fluid_assemble(xyz, tets, np.ones(ne), np.full(ne, mu), np.zeros((ne, 3)), state,
+                                            state[:, :3].copy(), np.zeros((nn, 3)), 1.0, 1.0, 1.0, 0, book, n)
+        tk = []
+        for k in (100, 200):
+            t0 = time.perf_counter()
+            orc.solve(A_full, b_full, precond=orc.PRECOND_JACOBI, solver=orc.SOLVER_GMRES, n_outer=1, n_inner=k,
+                      cvg=1e-300, stop_rule=1)
+            tk.append(time.perf_counter() - t0)
+        cq = max(0.0, (tk[1] - 2 * tk[0]) / (200.0 ** 2 - 2 * 100.0 ** 2))
+        ca = (tk[0] - cq * 100.0 ** 2) / 100.0
+        cycles = float(np.mean([x[5] for x in steps]))
+        klen = float(np.mean([x[3] for x in steps])) / max(cycles, 1.0)
+        t_solve = cycles * (ca * klen + cq * klen ** 2)
+        total = t_asm + t_solve
+        cpu = {"value": round(total, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": (f"oracle on {threads} threads: assembly of a {4 * sub}x{sub}x{sub} sub-duct scaled per tet "
+                          f"({t_asm:.2f} s); GMRES+Jacobi cycles of 100 and 200 inner iterations on the full first "
+                          f"system ({tk[0]:.2f} s, {tk[1]:.2f} s) fitted as a k + c k^2 and applied to "
+                          f"{cycles:.1f} cycles of {klen:.0f} ({t_solve:.1f} s)")}
+        log(f"cpu baseline: {total:.2f} s per Picard step")
+    out = {"metric": "steady duct flow: Picard step (assembly + restarted GMRES-Jacobi to relres 1e-8 + write-back)",
+           "value": round(float(np.median(ts)), 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(float(np.median(ts)) * 1e3, 2), "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"duct flow {4 * c}x{c}x{c} cells x 6 tets, 4 unknowns per node, Re 50, "
+                                  f"GMRES({restart})",
+                      "unknowns": n, "nnz": int(asm.nnz), "tets": int(ne)},
+           "assembly_ms": round(float(np.median([x[1] for x in steps])), 3),
+           "gmres_ms": round(float(np.median([x[2] for x in steps])), 2),
+           "gmres_restart": restart, "inner_iterations": [int(x[3]) for x in steps],
+           "cycles": [int(x[5]) for x in steps], "all_converged": all(x[6] == 1 for x in steps),
+           "true_relres": max(x[4] for x in steps),
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -581,11 +698,14 @@ def main() -> int:
     ap.add_argument("--eigen-cells", type=int, default=128)
     ap.add_argument("--eigen-modes", type=int, default=10)
     ap.add_argument("--mag-cells", type=int, default=96)
-    ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen", "electrostatics", "magnetostatics"],
+    ap.add_argument("--fluid-cells", type=int, default=32)
+    ap.add_argument("--fluid-restart", type=int, default=400)
+    ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen", "electrostatics", "magnetostatics",
+                                                 "fluid"],
                     default="heat",
                     help="heat (default, the headline), the elasticity cantilever, the batched "
-                         "transient heat systems, the modal analysis (Arnoldi), electrostatics or "
-                         "magnetostatics")
+                         "transient heat systems, the modal analysis (Arnoldi), electrostatics, "
+                         "magnetostatics or the steady duct flow (Picard step)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--unbatched-budget", type=float, default=40.0,
@@ -620,6 +740,8 @@ def main() -> int:
         return electrostatics(rs, args)
     if args.workload == "magnetostatics":
         return magnetostatics(rs, args)
+    if args.workload == "fluid":
+        return fluid(rs, args)
     c = args.cells
     log(f"building {c}^3 Kuhn mesh and fields")
     xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma, args.seed_k)
