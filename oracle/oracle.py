@@ -65,6 +65,10 @@ def load() -> C.CDLL:
                                       C.c_double, C.c_int, _P, C.c_uint32, _P, _P]
     lib.or_fluid_update.argtypes = [C.c_uint32, _P, _P, C.c_int, C.c_double, _P, _P, _P, _P]
     lib.or_fluid_update.restype = None
+    lib.or_radiosity_assemble.argtypes = [C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]
+    lib.or_radiosity_assemble.restype = None
+    lib.or_radiosity_process.argtypes = [C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P]
+    lib.or_radiosity_process.restype = None
     lib.or_heat_flux.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P]
     lib.or_stress_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.or_eigen_solve.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_double, C.POINTER(OrConf), C.c_uint, _P, _P,
@@ -361,6 +365,36 @@ def fluid_update(book, x, unsteady, dt, state, v_old, acc):
     lib.or_fluid_update(state.size // 4, _p(book), _p(x), int(bool(unsteady)), float(dt), _p(state), _p(v_old),
                         _p(acc), _p(norms))
     return norms
+
+
+SIGMA = 5.670367e-8
+
+
+def radiosity_assemble(F, patch_ptr, patch_elems, emissivity, temperature, ambient):
+    """RSolverRadiativeHeat::prepare restated (dense F, n_patches x n_patches); returns the dense
+    matrix A and b."""
+    lib = load()
+    F = np.ascontiguousarray(F, np.float64)
+    npch = F.shape[0]
+    pp = np.ascontiguousarray(patch_ptr, np.uint32)
+    pe = np.ascontiguousarray(patch_elems, np.uint32)
+    em, te, am = (np.ascontiguousarray(a, np.float64) for a in (emissivity, temperature, ambient))
+    A = np.empty((npch, npch))
+    b = np.empty(npch)
+    lib.or_radiosity_assemble(npch, _p(F), _p(pp), _p(pe), _p(em), _p(te), _p(am), _p(A), _p(b))
+    return A, b
+
+
+def radiosity_process(patch_ptr, patch_elems, elem_area, patch_heat):
+    """RSolverRadiativeHeat::process restated; returns the element heats."""
+    lib = load()
+    pp = np.ascontiguousarray(patch_ptr, np.uint32)
+    pe = np.ascontiguousarray(patch_elems, np.uint32)
+    ar = np.ascontiguousarray(elem_area, np.float64)
+    ph = np.ascontiguousarray(patch_heat, np.float64)
+    out = np.empty(ar.size)
+    lib.or_radiosity_process(ph.size, ar.size, _p(pp), _p(pe), _p(ar), _p(ph), _p(out))
+    return out
 
 
 def magnetostatics_assemble(xyz, tets, current_node, field_node, book, n_dof):

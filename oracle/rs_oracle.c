@@ -1032,6 +1032,66 @@ void or_fluid_update(uint32_t nn, const uint32_t *book, const double *x, int uns
     norms[3] = sqrt(p1);
 }
 
+/* ------------------------------------------------------------------ radiative heat
+ * RSolverRadiativeHeat::prepare with the view-factor matrix loaded (src/rsolverradiativeheat.cpp:
+ * 145-241): patch means of the element emissivity, temperature and ambient temperature over the
+ * patch's elements (:175-205), then every (i, j) of the dense system
+ *   A_ij = d_ij / E_j - F_ij (1 - E_j) / E_j   (0 when E_j == 0)
+ *   b_i  = -sum_j (d_ij - F_ij) sigma T_j^4 + (1 - sum_j F_ij) sigma Tamb_i^4     (:211-238)
+ * F is dense row-major (RViewFactorRow::getViewFactors().getValues(nPatches)); A is dense
+ * row-major n_patches^2.  sigma = RSolverGeneric::sigma (src/rsolvergeneric.cpp:12). */
+void or_radiosity_assemble(uint32_t np, const double *F, const uint32_t *patch_ptr, const uint32_t *patch_elems,
+                           const double *emissivity, const double *temperature, const double *ambient,
+                           double *A, double *b)
+{
+    const double sigma = 5.670367e-8;
+    double *E = calloc(np ? np : 1, sizeof(double));
+    double *T = calloc(np ? np : 1, sizeof(double));
+    double *Ta = calloc(np ? np : 1, sizeof(double));
+    for (uint32_t p = 0; p < np; p++) {
+        for (uint32_t k = patch_ptr[p]; k < patch_ptr[p + 1]; k++) {
+            uint32_t e = patch_elems[k];
+            E[p] += emissivity[e];
+            T[p] += temperature[e];
+            Ta[p] += ambient[e];
+        }
+        uint32_t cnt = patch_ptr[p + 1] - patch_ptr[p];
+        if (cnt > 0) { E[p] /= (double)cnt; T[p] /= (double)cnt; Ta[p] /= (double)cnt; }
+    }
+#pragma omp parallel for
+    for (int64_t i = 0; i < (int64_t)np; i++) {
+        double Fsum = 0.0, bi = 0.0;
+        for (uint32_t j = 0; j < np; j++) {
+            const double dij = (i == (int64_t)j) ? 1.0 : 0.0;
+            const double Fij = F[(size_t)i * np + j], Ej = E[j];
+            Fsum += Fij;
+            double Aij = 0.0;
+            if (Ej != 0.0) Aij = dij / Ej - Fij * (1.0 - Ej) / Ej;
+            bi -= (dij - Fij) * sigma * pow(T[j], 4);
+            A[(size_t)i * np + j] = Aij;
+        }
+        bi += (1.0 - Fsum) * sigma * pow(Ta[i], 4);
+        b[i] = bi;
+    }
+    free(E); free(T); free(Ta);
+}
+
+/* RSolverRadiativeHeat::process (src/rsolverradiativeheat.cpp:276-303): each element of a patch
+ * receives the patch heat times its share of the patch area; elements in no patch get 0. */
+void or_radiosity_process(uint32_t np, uint32_t ne, const uint32_t *patch_ptr, const uint32_t *patch_elems,
+                          const double *elem_area, const double *patch_heat, double *elem_heat)
+{
+    memset(elem_heat, 0, sizeof(double) * ne);
+    for (uint32_t p = 0; p < np; p++) {
+        double area = 0.0;
+        for (uint32_t k = patch_ptr[p]; k < patch_ptr[p + 1]; k++) area += elem_area[patch_elems[k]];
+        for (uint32_t k = patch_ptr[p]; k < patch_ptr[p + 1]; k++) {
+            uint32_t e = patch_elems[k];
+            elem_heat[e] = patch_heat[p] * (elem_area[e] / area);
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ post-processing
  * RSolverHeat::process, volume elements (src/rsolverheat.cpp:646-692): per tet
  * B[m] = sum over the integration points of grad N_m * detJ / nInp; q = -sum_m B[m] k T_m. */

@@ -127,6 +127,13 @@ int or_fluid_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_
 /* RSolverFluid::solve write-back (src/rsolverfluid.cpp:519-624). */
 void or_fluid_update(uint32_t nn, const uint32_t *book, const double *x, int unsteady, double dt,
                      double *state, double *v_old, double *acc, double norms[4]);
+/* RSolverRadiativeHeat::prepare / process with a loaded view-factor matrix,
+ * src/rsolverradiativeheat.cpp:145-241, 276-303 (dense row-major F and A). */
+void or_radiosity_assemble(uint32_t np, const double *F, const uint32_t *patch_ptr, const uint32_t *patch_elems,
+                           const double *emissivity, const double *temperature, const double *ambient,
+                           double *A, double *b);
+void or_radiosity_process(uint32_t np, uint32_t ne, const uint32_t *patch_ptr, const uint32_t *patch_elems,
+                          const double *elem_area, const double *patch_heat, double *elem_heat);
 /* RSolverHeat::process (volume), src/rsolverheat.cpp:646-692: 3 flux values per tet. */
 int or_heat_flux(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *k,
                  const double *t_node, double *flux);
