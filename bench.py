@@ -684,6 +684,82 @@ def fluid(rs, args) -> int:
     return 0
 
 
+def radiosity(rs, args) -> int:
+    """Radiative heat (RSolverRadiativeHeat with a loaded view-factor matrix) in the closed unit
+    box with k x k patches per face (k = args.rad_patches): emissivity U(0.2, 0.95) and
+    temperatures 400 + 100 face + U(0, 50) K per element, ambient 300 K.  A step is the GPU
+    assembly of the dense patch system, CG + Jacobi to relres 1e-12 and the split of the patch
+    heats onto the elements.  The oracle runs the same steps on the host (all of them, the system
+    is small).  Prints one JSON line."""
+    import __graft_entry__ as ge
+    import scipy.sparse as sp
+    rad = ge.load_module("radiation")
+    k = args.rad_patches
+    t0 = time.perf_counter()
+    xyz, tris, pp, pe, F, area, face = rad.box_enclosure(k)
+    npch, ne = F.shape[0], tris.shape[0]
+    rng = np.random.default_rng(8)
+    em = rng.uniform(0.2, 0.95, ne)
+    te = 400.0 + 100.0 * face.repeat(2) + rng.uniform(0, 50, ne)
+    am = np.full(ne, 300.0)
+    R = rs.Radiosity(xyz, tris, pp, pe)
+    dF, dem, dte, dam = (rs.DeviceArray.from_host(a.reshape(-1)) for a in (F, em, te, am))
+    log(f"radiosity: {npch} patches, {ne} elements, dense system {npch}^2 (set-up {time.perf_counter() - t0:.1f} s)")
+    db, dq, dz, dh = rs.DeviceArray(npch), rs.DeviceArray(npch), rs.DeviceArray.from_host(np.zeros(npch)), \
+        rs.DeviceArray(ne)
+    conf = rs.MatrixSolverConf(n_outer_iterations=1000, solver_cvg_value=1e-12, stop_rule=rs.STOP_RELRES)
+    solver = rs.RMatrixSolver(conf)
+    lib = rs.load()
+    res = []
+    for it in range(args.warmup + args.steps):
+        rs.synchronize()
+        t0 = time.perf_counter()
+        R.assemble_device(dF, dem, dte, dam, db)
+        a_ms = R.last_ms
+        rs._check(lib.rs_memcpy_dtod(dq.ptr, dz.ptr, 8 * npch))
+        info = solver.solve_device(R.matrix, db, dq, rs.PRECOND_JACOBI, 1)
+        R.process_device(dq, dh)
+        rs.synchronize()
+        t = time.perf_counter() - t0
+        if it >= args.warmup:
+            res.append((t, a_ms, info.solve_ms, info.iterations, info.true_relres))
+        log(f"radiosity step {it}: {t * 1e3:.2f} ms (assembly {a_ms:.3f} ms, CG {info.solve_ms:.2f} ms / "
+            f"{info.iterations} its, relres {info.true_relres:.2e})")
+    q = dq.download()
+    balance = abs(float(area @ q)) / float(area @ np.abs(q))
+    ts = np.array([x[0] for x in res])
+    a_med = float(np.median([x[1] for x in res]))
+    alg = 16.0 * npch * npch   # read F, write A
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        orc.load()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        A_ref, b_ref = orc.radiosity_assemble(F, pp, pe, em, te, am)
+        A_csr = sp.csr_matrix(A_ref)
+        t1 = time.perf_counter()
+        q_ref, _ = orc.solve(A_csr, b_ref, precond=orc.PRECOND_JACOBI, n_outer=1000, cvg=1e-12, stop_rule=1)
+        orc.radiosity_process(pp, pe, np.full(ne, 0.5 / (k * k)), q_ref)
+        t2 = time.perf_counter()
+        cpu = {"value": round(t2 - t0, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": f"the whole step on the oracle, {threads} threads (assembly {t1 - t0:.2f} s incl. CSR "
+                         f"conversion, CG + process {t2 - t1:.2f} s)"}
+        log(f"cpu baseline: {t2 - t0:.2f} s; max |q_gpu - q_oracle| / max |q| = "
+            f"{np.abs(q - q_ref).max() / np.abs(q_ref).max():.2e}")
+    out = {"metric": "radiative heat: dense patch-system assembly + CG-Jacobi to relres 1e-12 + element split",
+           "value": round(float(np.median(ts)), 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(float(np.median(ts)) * 1e3, 3), "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"closed box, {k}x{k} patches per face ({npch} patches, {ne} elements), "
+                                  f"dense view factors", "patches": npch},
+           "assembly_ms": round(a_med, 4), "assembly_GBps": round(alg / (a_med * 1e-3) / 1e9, 1),
+           "cg_ms": round(float(np.median([x[2] for x in res])), 3), "cg_iterations": int(res[-1][3]),
+           "true_relres": max(x[4] for x in res), "energy_balance": balance, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -700,12 +776,13 @@ def main() -> int:
     ap.add_argument("--mag-cells", type=int, default=96)
     ap.add_argument("--fluid-cells", type=int, default=32)
     ap.add_argument("--fluid-restart", type=int, default=400)
+    ap.add_argument("--rad-patches", type=int, default=32)
     ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen", "electrostatics", "magnetostatics",
-                                                 "fluid"],
+                                                 "fluid", "radiosity"],
                     default="heat",
                     help="heat (default, the headline), the elasticity cantilever, the batched "
                          "transient heat systems, the modal analysis (Arnoldi), electrostatics, "
-                         "magnetostatics or the steady duct flow (Picard step)")
+                         "magnetostatics, the steady duct flow (Picard step) or radiative heat")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--unbatched-budget", type=float, default=40.0,
@@ -742,6 +819,8 @@ def main() -> int:
         return magnetostatics(rs, args)
     if args.workload == "fluid":
         return fluid(rs, args)
+    if args.workload == "radiosity":
+        return radiosity(rs, args)
     c = args.cells
     log(f"building {c}^3 Kuhn mesh and fields")
     xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma, args.seed_k)

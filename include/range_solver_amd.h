@@ -278,6 +278,30 @@ int rs_assemble_fluid_device(rs_assembly *S, const rs_fluid_conf *conf, const do
  * / dt.  norms (host, 4, may be NULL): |v| before, |v| after, |p| before, |p| after. */
 int rs_fluid_update_device(rs_assembly *S, const double *d_x, int32_t unsteady, double dt, double *d_state,
                            double *d_v_old, double *d_acc, double *norms);
+/* ---------------------------------------------------------------- radiative heat
+ * RSolverRadiativeHeat (src/rsolverradiativeheat.cpp) with the view-factor matrix loaded (from its
+ * file or computed by RHemiCube).  A radiosity handle holds the patch book (patch p owns the
+ * surface elements patch_elems[patch_ptr[p] .. patch_ptr[p+1]), RPatchBook), the element areas
+ * (tris: 3 nodes per surface element) and the dense n_patches x n_patches system matrix. */
+typedef struct rs_radiosity rs_radiosity;
+int rs_radiosity_create(uint32_t n_patches, uint32_t n_nodes, const double *xyz, uint32_t n_elems,
+                        const uint32_t *tris, const uint32_t *patch_ptr, const uint32_t *patch_elems,
+                        rs_radiosity **out);
+/* The system matrix (owned by the handle; do not destroy). */
+int rs_radiosity_matrix(rs_radiosity *R, rs_matrix **A);
+/* RSolverRadiativeHeat::prepare (:145-241): patch means of the element emissivity, temperature and
+ * ambient temperature (n_elems each), then A_ij = d_ij/E_j - F_ij (1 - E_j)/E_j (0 if E_j = 0) and
+ * b_i = -sum_j (d_ij - F_ij) sigma T_j^4 + (1 - sum_j F_ij) sigma Tamb_i^4 from the dense row-major
+ * view factors (n_patches^2).  The solve is rs_solver_solve_device with CG + Jacobi (:243-266). */
+int rs_assemble_radiosity_device(rs_radiosity *R, const double *d_view_factors, const double *d_emissivity,
+                                 const double *d_temperature, const double *d_ambient, double *d_b);
+/* RSolverRadiativeHeat::process (:268-303): element heat = patch heat * element area / patch area
+ * (n_elems outputs; elements in no patch get 0). */
+int rs_radiosity_process_device(rs_radiosity *R, const double *d_patch_heat, double *d_elem_heat);
+/* Device time (ms) of the last rs_assemble_radiosity_device. */
+int rs_radiosity_last_ms(const rs_radiosity *R, double *ms);
+int rs_radiosity_destroy(rs_radiosity *R);
+
 /* RSolverStress::process, src/rsolverstress.cpp:1238-1408, static case without thermal load --
  * per tet the normal and shear stress measures and their sum (the reference's von Mises), and the
  * node forces sum_e Ke u_e (3 per node; reactions on the fixed nodes). */

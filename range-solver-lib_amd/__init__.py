@@ -140,6 +140,10 @@ _SIGS = {
     "rs_assemble_magnetostatics_device": [_P, _P, _P, _P],
     "rs_assemble_fluid_device": [_P, C.POINTER(FluidConf), _P, _P, _P, _P, _P, _P, _P],
     "rs_fluid_update_device": [_P, _P, C.c_int32, C.c_double, _P, _P, _P, _f64p],
+    "rs_radiosity_create": [C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P, _P, C.POINTER(_P)],
+    "rs_radiosity_matrix": [_P, C.POINTER(_P)], "rs_radiosity_last_ms": [_P, _f64p],
+    "rs_assemble_radiosity_device": [_P, _P, _P, _P, _P, _P],
+    "rs_radiosity_process_device": [_P, _P, _P], "rs_radiosity_destroy": [_P],
     "rs_electrostatics_process_device": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
     "rs_stress_process_device": [_P, _P, _P, _P, _P, _P, _P, _P],
     "rs_aggregates_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -517,6 +521,44 @@ def node_book(n_nodes: int, conn, dofs_per_node: int, fixed) -> tuple[np.ndarray
     _check(_lib.rs_node_book(n_nodes, conn.size // npe, npe, _ptr(conn), dofs_per_node,
                              _ptr(fixed), _ptr(book), C.byref(n)))
     return book, n.value
+
+
+# ------------------------------------------------------------------------------ radiative heat
+class Radiosity:
+    """RSolverRadiativeHeat with a loaded view-factor matrix: patch book, element areas and the
+    dense patch system (rs_radiosity_*)."""
+
+    def __init__(self, xyz, tris, patch_ptr, patch_elems):
+        load()
+        self.xyz = _f64(xyz).reshape(-1)
+        self.tris = _u32(tris).reshape(-1)
+        pp, pe = _u32(patch_ptr).reshape(-1), _u32(patch_elems).reshape(-1)
+        self.n_patches = pp.size - 1
+        self.n_elems = self.tris.size // 3
+        self.h = C.c_void_p()
+        _check(_lib.rs_radiosity_create(self.n_patches, self.xyz.size // 3, _ptr(self.xyz), self.n_elems,
+                                        _ptr(self.tris), _ptr(pp), _ptr(pe), C.byref(self.h)))
+        mh = C.c_void_p()
+        _check(_lib.rs_radiosity_matrix(self.h, C.byref(mh)))
+        self.matrix = SparseMatrix(0, None, None, None, _handle=mh, _owner=self)
+
+    def assemble_device(self, dview_factors, demissivity, dtemperature, dambient, db) -> None:
+        _check(_lib.rs_assemble_radiosity_device(self.h, dview_factors.ptr, demissivity.ptr, dtemperature.ptr,
+                                                 dambient.ptr, db.ptr))
+
+    def process_device(self, dpatch_heat, delem_heat) -> None:
+        _check(_lib.rs_radiosity_process_device(self.h, dpatch_heat.ptr, delem_heat.ptr))
+
+    @property
+    def last_ms(self) -> float:
+        v = C.c_double()
+        _check(_lib.rs_radiosity_last_ms(self.h, C.byref(v)))
+        return v.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.rs_radiosity_destroy(self.h)
+            self.h = None
 
 
 # ------------------------------------------------------------------------------ assembly
