@@ -69,6 +69,10 @@ def load() -> C.CDLL:
     lib.or_radiosity_assemble.restype = None
     lib.or_radiosity_process.argtypes = [C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P]
     lib.or_radiosity_process.restype = None
+    lib.or_transport_assemble.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, C.c_double, C.c_double,
+                                          C.c_int, _P, C.c_uint32, _P, _P]
+    lib.or_transport_update.argtypes = [C.c_uint32, _P, _P, _P, _P]
+    lib.or_transport_update.restype = None
     lib.or_heat_flux.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P]
     lib.or_stress_process.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.or_eigen_solve.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_double, C.POINTER(OrConf), C.c_uint, _P, _P,
@@ -364,6 +368,37 @@ def fluid_update(book, x, unsteady, dt, state, v_old, acc):
     norms = np.zeros(4)
     lib.or_fluid_update(state.size // 4, _p(book), _p(x), int(bool(unsteady)), float(dt), _p(state), _p(v_old),
                         _p(acc), _p(norms))
+    return norms
+
+
+def transport_assemble(xyz, tets, ca, k, velocity, q_node, field, alpha, dt, unsteady, book, n_dof):
+    """RSolverFluidHeat / RSolverFluidParticle element + assemblyMatrix restated; ca, k per element,
+    velocity 3 per element, q_node and field per node.  Returns (A as scipy CSR, b)."""
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    ne = tets.size // 4
+    ca, k = (np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (ne,))) for a in (ca, k))
+    vel = np.ascontiguousarray(np.broadcast_to(np.asarray(velocity, np.float64), (ne, 3))).reshape(-1)
+    q = np.ascontiguousarray(q_node, np.float64)
+    f = np.ascontiguousarray(field, np.float64)
+    book = np.ascontiguousarray(book, np.uint32)
+    A = lib.or_sparse_create(n_dof)
+    b = np.zeros(max(1, n_dof))
+    lib.or_transport_assemble(xyz.size // 3, _p(xyz), ne, _p(tets), _p(ca), _p(k), _p(vel), _p(q), _p(f),
+                              float(alpha), float(dt), int(bool(unsteady)), _p(book), n_dof, A, _p(b))
+    return _export(A, n_dof), b[:n_dof]
+
+
+def transport_update(book, x, field):
+    """Write-back restated: field (float64, contiguous, updated in place) = max(x, 0) on the
+    enabled nodes; returns (|field| before, |field| after)."""
+    lib = load()
+    book = np.ascontiguousarray(book, np.uint32)
+    x = np.ascontiguousarray(x, np.float64)
+    assert field.dtype == np.float64 and field.flags.c_contiguous
+    norms = np.zeros(2)
+    lib.or_transport_update(field.size, _p(book), _p(x), _p(field), _p(norms))
     return norms
 
 

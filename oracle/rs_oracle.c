@@ -1032,6 +1032,106 @@ void or_fluid_update(uint32_t nn, const uint32_t *book, const double *x, int uns
     norms[3] = sqrt(p1);
 }
 
+/* ------------------------------------------------------------------ scalar transport
+ * RSolverFluidHeat and RSolverFluidParticle share one element (src/rsolverfluidheat.cpp:669-803,
+ * src/rsolverfluidparticle.cpp:552-686): SUPG advection-diffusion of a node field (temperature /
+ * concentration) by the element velocity, with capacity ca (rho c for heat, 1 for particles),
+ * diffusivity k (conduction / diffusion), node source q (nodeHeat / nodeRate), and the heat-style
+ * lifting of fixed nodes (assemblyMatrix :845-880 / :728-763).  cte stays zero (the container is
+ * cleared and never set).  book has one entry per node. */
+int or_transport_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *ca_e,
+                          const double *k_e, const double *vel_e, const double *q_node, const double *field,
+                          double alpha, double dt, int unsteady, const uint32_t *book, uint32_t n_dof,
+                          or_sparse *A, double *b)
+{
+    (void)nn;
+    memset(b, 0, sizeof(double) * n_dof);
+    double iN[4] = { 0 }, iNiN[4][4] = { { 0 } }, wt = 0.0;
+    for (int ip = 0; ip < 4; ip++) {
+        double N[4];
+        tet_shape(TET_IP[ip], N);
+        wt += TET_W;
+        for (int m = 0; m < 4; m++) {
+            iN[m] += TET_W * N[m];
+            for (int n = 0; n < 4; n++) iNiN[m][n] += TET_W * N[m] * N[n];
+        }
+    }
+    for (int64_t e = 0; e < (int64_t)ne; e++) {
+        const uint32_t *nd = tets + 4 * e;
+        double X[4][3], J[3][3], B[4][3];
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++) X[m][a] = xyz[3 * (size_t)nd[m] + a];
+        const double detJ = tet_jacobian(X, J);
+        for (int m = 0; m < 4; m++)
+            for (int a = 0; a < 3; a++)
+                B[m][a] = TET_DN[m][0] * J[a][0] + TET_DN[m][1] * J[a][1] + TET_DN[m][2] * J[a][2];
+        const double ca = ca_e[e], k = k_e[e];
+        const double *ve = vel_e + 3 * e;
+        const double mvh = sqrt(ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+        double s[3] = { 0, 0, 0 };
+        if (mvh > 0.0) for (int c = 0; c < 3; c++) s[c] = ve[c] / mvh;
+        double vdiv[4], h = 0.0;
+        for (int m = 0; m < 4; m++) {
+            vdiv[m] = ve[0] * B[m][0] + ve[1] * B[m][1] + ve[2] * B[m][2];
+            h += fabs(s[0] * B[m][0] + s[1] * B[m][1] + s[2] * B[m][2]);
+        }
+        if (h != 0.0) h = 2.0 / h;
+        const double Re = k == 0.0 ? 0.0 : mvh * h / (2.0 * k);
+        double Tsupg = 0.0;
+        if (mvh > 0.0) {
+            Tsupg = h / (2.0 * mvh);
+            if (Re > 0.0 && Re <= 3.0) Tsupg *= Re / 3.0;
+        }
+        double Ae[4][4], be[4];
+        for (int m = 0; m < 4; m++) {
+            const double fv = q_node[nd[m]] * iN[m];
+            double sum = 0.0;
+            for (int n = 0; n < 4; n++) {
+                const double me = unsteady ? ca * iNiN[m][n] : 0.0;
+                const double ce = ca * iN[m] * vdiv[n];
+                const double ke = -k * wt * (B[m][0] * B[n][0] + B[m][1] * B[n][1] + B[m][2] * B[n][2]);
+                const double kte = Tsupg * ca * vdiv[m] * vdiv[n] * wt;
+                const double yte = Tsupg * vdiv[m] * wt;
+                const double cte = 0.0;
+                const double ss = ce + ke + kte + yte;
+                Ae[m][n] = unsteady ? me + cte + alpha * dt * ss : ss;
+                if (unsteady) sum += (me + cte - (1.0 - alpha) * dt * ss) * field[nd[n]];
+            }
+            be[m] = unsteady ? dt * fv + sum : fv;
+        }
+        for (int m = 0; m < 4; m++) {
+            be[m] *= detJ;
+            for (int n = 0; n < 4; n++) Ae[m][n] *= detJ;
+        }
+        for (int m = 0; m < 4; m++)                                              /* lifting */
+            if (book[nd[m]] == OR_EOD)
+                for (int n = 0; n < 4; n++) be[n] -= Ae[n][m] * field[nd[m]];
+        for (int m = 0; m < 4; m++) {
+            const uint32_t mp = book[nd[m]];
+            if (mp == OR_EOD) continue;
+            b[mp] += be[m];
+            for (int n = 0; n < 4; n++) {
+                const uint32_t np = book[nd[n]];
+                if (np != OR_EOD) add_value(A, mp, np, Ae[m][n]);
+            }
+        }
+    }
+    return 0;
+}
+
+/* RSolverFluidHeat::solve / RSolverFluidParticle::solve write-back (src/rsolverfluidheat.cpp:
+ * 320-344): field = max(x, 0) on the enabled nodes; norms: |field| before and after. */
+void or_transport_update(uint32_t nn, const uint32_t *book, const double *x, double *field, double norms[2])
+{
+    double a = 0.0, c = 0.0;
+    for (uint32_t i = 0; i < nn; i++) a += field[i] * field[i];
+    for (uint32_t i = 0; i < nn; i++)
+        if (book[i] != OR_EOD) field[i] = fmax(x[book[i]], 0.0);
+    for (uint32_t i = 0; i < nn; i++) c += field[i] * field[i];
+    norms[0] = sqrt(a);
+    norms[1] = sqrt(c);
+}
+
 /* ------------------------------------------------------------------ radiative heat
  * RSolverRadiativeHeat::prepare with the view-factor matrix loaded (src/rsolverradiativeheat.cpp:
  * 145-241): patch means of the element emissivity, temperature and ambient temperature over the

@@ -127,6 +127,13 @@ int or_fluid_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_
 /* RSolverFluid::solve write-back (src/rsolverfluid.cpp:519-624). */
 void or_fluid_update(uint32_t nn, const uint32_t *book, const double *x, int unsteady, double dt,
                      double *state, double *v_old, double *acc, double norms[4]);
+/* RSolverFluidHeat / RSolverFluidParticle element + assemblyMatrix (src/rsolverfluidheat.cpp:167-299,
+ * 669-880; src/rsolverfluidparticle.cpp:552-763); vel_e 3 per element; book 1 per node. */
+int or_transport_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets, const double *ca_e,
+                          const double *k_e, const double *vel_e, const double *q_node, const double *field,
+                          double alpha, double dt, int unsteady, const uint32_t *book, uint32_t n_dof,
+                          or_sparse *A, double *b);
+void or_transport_update(uint32_t nn, const uint32_t *book, const double *x, double *field, double norms[2]);
 /* RSolverRadiativeHeat::prepare / process with a loaded view-factor matrix,
  * src/rsolverradiativeheat.cpp:145-241, 276-303 (dense row-major F and A). */
 void or_radiosity_assemble(uint32_t np, const double *F, const uint32_t *patch_ptr, const uint32_t *patch_elems,
