@@ -278,6 +278,25 @@ int rs_assemble_fluid_device(rs_assembly *S, const rs_fluid_conf *conf, const do
  * / dt.  norms (host, 4, may be NULL): |v| before, |v| after, |p| before, |p| after. */
 int rs_fluid_update_device(rs_assembly *S, const double *d_x, int32_t unsteady, double dt, double *d_state,
                            double *d_v_old, double *d_acc, double *norms);
+/* RSolverFluidHeat / RSolverFluidParticle time-step parameters (time-march alpha, dt, time solver on). */
+typedef struct rs_transport_conf {
+    double alpha;
+    double dt;
+    int32_t unsteady;
+} rs_transport_conf;
+/* RSolverFluidHeat::prepare / RSolverFluidParticle::prepare, volume elements
+ * (src/rsolverfluidheat.cpp:167-299, 669-880; src/rsolverfluidparticle.cpp:121-253, 552-763) on a
+ * 1-DOF assembly: SUPG advection-diffusion of the node field (temperature / concentration) with
+ * per-element capacity ca (rho c; 1 for particles), diffusivity k (conduction / diffusion) and
+ * velocity (3 per element, the element vector of the node velocities), node source (nodeHeat /
+ * nodeRate), lifting of the fixed nodes with their field values.  The solve is GMRES + Jacobi. */
+int rs_assemble_transport_device(rs_assembly *S, const rs_transport_conf *conf, const double *d_ca,
+                                 const double *d_k, const double *d_velocity, const double *d_node_source,
+                                 const double *d_node_field, double *d_b);
+/* Write-back (src/rsolverfluidheat.cpp:320-344, src/rsolverfluidparticle.cpp:274-298): field =
+ * max(x, 0) on the enabled nodes; norms (host, 2, may be NULL): |field| before and after. */
+int rs_transport_update_device(rs_assembly *S, const double *d_x, double *d_node_field, double *norms);
+
 /* ---------------------------------------------------------------- radiative heat
  * RSolverRadiativeHeat (src/rsolverradiativeheat.cpp) with the view-factor matrix loaded (from its
  * file or computed by RHemiCube).  A radiosity handle holds the patch book (patch p owns the

@@ -83,6 +83,11 @@ class FluidConf(C.Structure):
                 ("unsteady", C.c_int32)]
 
 
+class TransportConf(C.Structure):
+    """RSolverFluidHeat / RSolverFluidParticle time-step parameters."""
+    _fields_ = [("alpha", C.c_double), ("dt", C.c_double), ("unsteady", C.c_int32)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("rounds", C.c_uint32), ("max_iterations", C.c_uint32),
                 ("total_iterations", C.c_uint64), ("systems_converged", C.c_uint32),
@@ -140,6 +145,8 @@ _SIGS = {
     "rs_assemble_magnetostatics_device": [_P, _P, _P, _P],
     "rs_assemble_fluid_device": [_P, C.POINTER(FluidConf), _P, _P, _P, _P, _P, _P, _P],
     "rs_fluid_update_device": [_P, _P, C.c_int32, C.c_double, _P, _P, _P, _f64p],
+    "rs_assemble_transport_device": [_P, C.POINTER(TransportConf), _P, _P, _P, _P, _P, _P],
+    "rs_transport_update_device": [_P, _P, _P, _f64p],
     "rs_radiosity_create": [C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P, _P, C.POINTER(_P)],
     "rs_radiosity_matrix": [_P, C.POINTER(_P)], "rs_radiosity_last_ms": [_P, _f64p],
     "rs_assemble_radiosity_device": [_P, _P, _P, _P, _P, _P],
@@ -648,6 +655,17 @@ class Assembly:
         _check(_lib.rs_fluid_update_device(self.h, dx.ptr, int(bool(unsteady)), dt, dstate.ptr, dv_old.ptr,
                                            dacc.ptr if dacc is not None else None,
                                            norms.ctypes.data_as(_f64p)))
+        return norms
+
+    def assemble_transport_device(self, conf: "TransportConf", dca, dk, dvelocity, dsource, dfield, db) -> None:
+        """RSolverFluidHeat / RSolverFluidParticle prepare (volume): SUPG advection-diffusion."""
+        _check(_lib.rs_assemble_transport_device(self.h, C.byref(conf), dca.ptr, dk.ptr, dvelocity.ptr, dsource.ptr,
+                                                 dfield.ptr, db.ptr))
+
+    def transport_update_device(self, dx, dfield) -> np.ndarray:
+        """Write-back field = max(x, 0); returns (|field| before, |field| after)."""
+        norms = np.zeros(2)
+        _check(_lib.rs_transport_update_device(self.h, dx.ptr, dfield.ptr, norms.ctypes.data_as(_f64p)))
         return norms
 
     def electrostatics_process_device(self, deps_r, dsigma, dphi_node, dfield, dcurrent, denergy, dres,

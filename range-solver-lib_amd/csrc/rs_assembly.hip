@@ -620,6 +620,126 @@ __global__ __launch_bounds__(kBlock) void k_fluid_update(const uint4 *__restrict
     }
 }
 
+// RSolverFluidHeat / RSolverFluidParticle (src/rsolverfluidheat.cpp:669-880,
+// src/rsolverfluidparticle.cpp:552-763): SUPG advection-diffusion of a node field by the element
+// velocity -- Galerkin advection ca iN_m (v . grad N_n), the reference's negated diffusion
+// -k (grad N_m . grad N_n), streamline diffusion tau ca (v . grad N_m)(v . grad N_n) and the
+// SUPG term tau (v . grad N_m) that does not depend on n; tau = h / (2 |v|) min(Pe / 3, 1) --
+// with the heat-style lifting of fixed nodes.  Same layout as the heat colour kernel.
+struct TransportArgs {
+    const double *xyz;
+    const uint4 *conn;
+    const uint32_t *perm;
+    const uint32_t *map;
+    const uint32_t *book;
+    const double *ca, *k, *vel;     // per original element (vel: 3 per element)
+    const double *q, *field;        // per node
+    double *val, *b;
+    double alpha, dt;
+    int unsteady;
+};
+
+__global__ __launch_bounds__(kBlock) void k_transport_colour(TransportArgs a, uint32_t e0, uint32_t e1)
+{
+    const uint32_t e = e0 + blockIdx.x * kBlock + threadIdx.x;
+    if (e >= e1) return;
+    const uint4 nd4 = a.conn[e];
+    const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+    const uint32_t orig = a.perm[e];
+    double X[4][3], B[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) X[q][c] = a.xyz[3 * (uint64_t)nd[q] + c];
+    const double V = tet_gradients(X, B);
+    const double detJ = 6.0 * V, wt = 1.0 / 6.0, iN = 1.0 / 24.0;
+    const double ca = a.ca[orig], k = a.k[orig];
+    const double ve[3] = { a.vel[3 * (uint64_t)orig], a.vel[3 * (uint64_t)orig + 1], a.vel[3 * (uint64_t)orig + 2] };
+    const double mvh = sqrt(ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+    double s[3] = { 0, 0, 0 };
+    if (mvh > 0.0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) s[c] = ve[c] / mvh;
+    }
+    double vdiv[4], h = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        vdiv[m] = ve[0] * B[m][0] + ve[1] * B[m][1] + ve[2] * B[m][2];
+        h += fabs(s[0] * B[m][0] + s[1] * B[m][1] + s[2] * B[m][2]);
+    }
+    if (h != 0.0) h = 2.0 / h;
+    const double Re = k == 0.0 ? 0.0 : mvh * h / (2.0 * k);
+    double Tsupg = 0.0;
+    if (mvh > 0.0) {
+        Tsupg = h / (2.0 * mvh);
+        if (Re > 0.0 && Re <= 3.0) Tsupg *= Re / 3.0;
+    }
+    const bool unsteady = a.unsteady != 0;
+    double T[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) T[n] = a.field[nd[n]];
+    double Ae[4][4], be[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const double fv = a.q[nd[m]] * iN;
+        double sum = 0.0;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const double me = unsteady ? ca * ((m == n ? 2.0 : 1.0) / 120.0) : 0.0;
+            const double ss = ca * iN * vdiv[n] - k * wt * (B[m][0] * B[n][0] + B[m][1] * B[n][1] + B[m][2] * B[n][2])
+                            + Tsupg * ca * vdiv[m] * vdiv[n] * wt + Tsupg * vdiv[m] * wt;
+            Ae[m][n] = (unsteady ? me + a.alpha * a.dt * ss : ss) * detJ;
+            if (unsteady) sum += (me - (1.0 - a.alpha) * a.dt * ss) * T[n];
+        }
+        be[m] = (unsteady ? a.dt * fv + sum : fv) * detJ;
+    }
+    uint32_t dof[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) dof[m] = a.book[nd[m]];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (dof[m] != RS_EOD) continue;
+#pragma unroll
+        for (int n = 0; n < 4; n++) be[n] -= Ae[n][m] * T[m];
+    }
+    const uint4 *mp = reinterpret_cast<const uint4 *>(a.map + 16ull * e);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (dof[m] == RS_EOD) continue;
+        a.b[dof[m]] += be[m];
+        const uint4 pos = mp[m];
+        const uint32_t pp[4] = { pos.x, pos.y, pos.z, pos.w };
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+            if (pp[n] != RS_EOD) a.val[pp[n]] += Ae[m][n];
+    }
+}
+
+// RSolverFluidHeat::solve / RSolverFluidParticle::solve write-back: field = max(x, 0) on the
+// enabled nodes; block partials of |field|^2 before and after.
+__global__ __launch_bounds__(kBlock) void k_transport_update(const uint32_t *__restrict__ book, uint32_t nn,
+                                                             const double *__restrict__ x,
+                                                             double *__restrict__ field, double *__restrict__ part)
+{
+    __shared__ double lds[2 * kBlock / kWave];
+    double s[2] = { 0, 0 };
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nn; i += gridDim.x * kBlock) {
+        double f = field[i];
+        s[0] += f * f;
+        const uint32_t d = book[i];
+        if (d != RS_EOD) {
+            f = fmax(x[d], 0.0);
+            field[i] = f;
+        }
+        s[1] += f * f;
+    }
+    block_sum<2, kBlock>(s, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s[0];
+        part[2 * blockIdx.x + 1] = s[1];
+    }
+}
+
 // RSolverElectrostatics (src/rsolverelectrostatics.cpp): the volume assembly (:300-345) is
 // the heat form with coefficient eps_r * e0 and source -rho (fe -= rho N detJ w); e0 as
 // RSolverGeneric::e0 (src/rsolvergeneric.cpp:13).
@@ -1447,6 +1567,61 @@ extern "C" int rs_fluid_update_device(rs_assembly *S, const double *d_x, int32_t
         norms[1] = std::sqrt(sum[1]);
         norms[2] = std::sqrt(sum[2]);
         norms[3] = std::sqrt(sum[3]);
+    }
+    return RS_OK;
+}
+
+// RSolverFluidHeat / RSolverFluidParticle prepare (volume elements) on a 1-DOF assembly.
+extern "C" int rs_assemble_transport_device(rs_assembly *S, const rs_transport_conf *conf, const double *d_ca,
+                                            const double *d_k, const double *d_velocity, const double *d_node_source,
+                                            const double *d_node_field, double *d_b)
+{
+    if (!S || !conf || !d_ca || !d_k || !d_velocity || !d_node_source || !d_node_field || !d_b)
+        return fail(RS_ERR_INVALID_ARG, "rs_assemble_transport_device: null argument");
+    if (S->dpn != 1) return fail(RS_ERR_INVALID_ARG, "rs_assemble_transport_device: assembly needs 1 dof per node");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    hipEvent_t ev[2];
+    if ((rc = timed_begin(ev))) return rc;
+    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+    if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
+    TransportArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p), S->perm.p, S->map.p, S->dbook.p, d_ca, d_k,
+                     d_velocity, d_node_source, d_node_field, S->A.val.p, d_b, conf->alpha, conf->dt,
+                     conf->unsteady ? 1 : 0 };
+    for (uint32_t c = 0; c < S->n_colours; c++) {
+        const uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+        if (e1 > e0) k_transport_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+    }
+    RS_HIP(hipGetLastError());
+    if ((rc = timed_end(ev, S->last_ms))) return rc;
+    if ((rc = matrix_frobenius(S->A))) return rc;
+    matrix_values_changed(S->A);
+    return RS_OK;
+}
+
+extern "C" int rs_transport_update_device(rs_assembly *S, const double *d_x, double *d_node_field, double *norms)
+{
+    if (!S || !d_x || !d_node_field) return fail(RS_ERR_INVALID_ARG, "rs_transport_update_device: null argument");
+    if (S->dpn != 1) return fail(RS_ERR_INVALID_ARG, "rs_transport_update_device: assembly needs 1 dof per node");
+    int rc = require_runtime();
+    if (rc) return rc;
+    const uint32_t nb = vblocks(S->n_nodes);
+    if (S->part.n < 2ull * nb && (rc = S->part.alloc(2ull * nb))) return rc;
+    hipStream_t st = runtime().stream;
+    k_transport_update<<<nb, kBlock, 0, st>>>(S->dbook.p, S->n_nodes, d_x, d_node_field, S->part.p);
+    RS_HIP(hipGetLastError());
+    std::vector<double> part(2ull * nb);
+    RS_HIP(hipMemcpyAsync(part.data(), S->part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    double sum[2] = { 0, 0 };
+    for (uint32_t blk = 0; blk < nb; blk++) {
+        sum[0] += part[2ull * blk];
+        sum[1] += part[2ull * blk + 1];
+    }
+    if (norms) {
+        norms[0] = std::sqrt(sum[0]);
+        norms[1] = std::sqrt(sum[1]);
     }
     return RS_OK;
 }
