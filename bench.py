@@ -684,6 +684,107 @@ def fluid(rs, args) -> int:
     return 0
 
 
+def transport(rs, args) -> int:
+    """Fluid heat (RSolverFluidHeat; RSolverFluidParticle runs the same element) on the c^3 x 6-tet
+    unit cube, c = args.cells: a rotating flow (-(y - 1/2), x - 1/2, 0.2) per element centroid,
+    rho c = 1, conduction 5e-5, implicit time steps dt = 1e-2 (alpha = 1), temperature 1 on z = 0
+    and 0 on z = 1.  The reference's diffusion block enters with a negative sign, so the step
+    matrix is M - dt k K + ...: with dt k |grad N|^2 above the mass term (k = 1e-3 at 128^3) its
+    diagonal degenerates and GMRES + Jacobi makes no progress, on the oracle as on the GPU; the
+    conduction is chosen below that.  A step is one time step as the reference runs it: GPU assembly of the SUPG
+    system, GMRES(100) + Jacobi to relres 1e-8, the clamped write-back.  The oracle times its
+    assembly on a sub-cube (scaled per tet) and partial GMRES cycles on the full system (fitted
+    a k + c k^2 and applied to the GPU's cycles).  Prints one JSON line."""
+    import __graft_entry__ as ge
+    c = args.cells
+    xyz, tets = rs.kuhn_box(c, c, c)
+    nn, ne = xyz.shape[0], tets.shape[0]
+
+    def setup(xyz, tets):
+        cen = xyz[tets].mean(1) - 0.5
+        vel = np.stack([-cen[:, 1], cen[:, 0], 0.2 * np.ones(tets.shape[0])], 1)
+        z = xyz[:, 2]
+        fixed = ((np.abs(z) < 1e-12) | (np.abs(z - 1) < 1e-12)).astype(np.uint8)
+        return vel, fixed, np.where(np.abs(z) < 1e-12, 1.0, 0.0)
+
+    vel, fixed, field = setup(xyz, tets)
+    kc, dt = 5e-5, 1e-2
+    t0 = time.perf_counter()
+    asm = rs.Assembly(xyz, tets, 1, fixed)
+    n = asm.n_dof
+    log(f"transport: {nn} nodes, {ne} tets, {n} unknowns, {asm.nnz} nonzeros (set-up {time.perf_counter() - t0:.1f} s)")
+    conf = rs.TransportConf(1.0, dt, 1)
+    dca, dk, dv, dq = (rs.DeviceArray.from_host(a.reshape(-1)) for a in (np.ones(ne), np.full(ne, kc), vel,
+                                                                            np.zeros(nn)))
+    dfield = rs.DeviceArray.from_host(field.copy())
+    db, dx, dz = rs.DeviceArray(n), rs.DeviceArray(n), rs.DeviceArray.from_host(np.zeros(n))
+    sconf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=100, n_inner_iterations=100,
+                                solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES)
+    solver = rs.RMatrixSolver(sconf)
+    lib = rs.load()
+    steps = []
+    for it in range(args.warmup + args.steps):
+        rs.synchronize()
+        t0 = time.perf_counter()
+        asm.assemble_transport_device(conf, dca, dk, dv, dq, dfield, db)
+        a_ms = asm.last_ms
+        rs._check(lib.rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * n))       # prepare clears x every step
+        info = solver.solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+        norms = asm.transport_update_device(dx, dfield)
+        rs.synchronize()
+        t = time.perf_counter() - t0
+        if it >= args.warmup:
+            steps.append((t, a_ms, info.solve_ms, info.inner_iterations, info.true_relres, info.iterations,
+                          info.converged))
+        log(f"transport step {it}: {t * 1e3:.1f} ms (assembly {a_ms:.2f} ms, GMRES {info.solve_ms:.1f} ms, "
+            f"{info.inner_iterations} inner its, relres {info.true_relres:.2e}, converged {info.converged}), "
+            f"|T| {norms[1]:.3f}")
+    ts = np.array([x[0] for x in steps])
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        orc.load()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        sub = max(2, c // 4)
+        sxyz, stets = rs.kuhn_box(sub, sub, sub)
+        svel, sfixed, sfield = setup(sxyz, stets)
+        sbook, sndof = orc.node_book(sxyz.shape[0], stets, 1, sfixed)
+        t0 = time.perf_counter()
+        orc.transport_assemble(sxyz, stets, 1.0, kc, svel, np.zeros(sxyz.shape[0]), sfield, 1.0, dt, 1, sbook, sndof)
+        t_asm = (time.perf_counter() - t0) * ne / stets.shape[0]
+        A_full = asm.matrix.to_scipy()
+        b_full = db.download()
+        tk = []
+        for kk in (20, 40):
+            t0 = time.perf_counter()
+            orc.solve(A_full, b_full, precond=orc.PRECOND_JACOBI, solver=orc.SOLVER_GMRES, n_outer=1, n_inner=kk,
+                      cvg=1e-300, stop_rule=1)
+            tk.append(time.perf_counter() - t0)
+        cq = max(0.0, (tk[1] - 2 * tk[0]) / (40.0 ** 2 - 2 * 20.0 ** 2))
+        ca_ = (tk[0] - cq * 20.0 ** 2) / 20.0
+        cycles = float(np.mean([x[5] for x in steps]))
+        klen = float(np.mean([x[3] for x in steps])) / max(cycles, 1.0)
+        t_solve = cycles * (ca_ * klen + cq * klen ** 2)
+        total = t_asm + t_solve
+        cpu = {"value": round(total, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": (f"oracle on {threads} threads: assembly of a {sub}^3 sub-cube scaled per tet ({t_asm:.2f} s); "
+                          f"GMRES+Jacobi cycles of 20 and 40 inner iterations on the full system ({tk[0]:.2f} s, "
+                          f"{tk[1]:.2f} s) fitted as a k + c k^2 and applied to {cycles:.1f} cycles of {klen:.0f}")}
+        log(f"cpu baseline: {total:.2f} s per time step")
+    out = {"metric": "fluid heat: implicit time step (SUPG assembly + GMRES(100)-Jacobi to relres 1e-8 + write-back)",
+           "value": round(float(np.median(ts)), 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(float(np.median(ts)) * 1e3, 3), "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"fluid heat {c}^3 cells x 6 tets, rotating flow, k 5e-5, dt 1e-2",
+                      "unknowns": n, "nnz": int(asm.nnz), "tets": int(ne)},
+           "assembly_ms": round(float(np.median([x[1] for x in steps])), 3),
+           "gmres_ms": round(float(np.median([x[2] for x in steps])), 3),
+           "inner_iterations": [int(x[3]) for x in steps], "all_converged": all(x[6] == 1 for x in steps),
+           "true_relres": max(x[4] for x in steps), "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def radiosity(rs, args) -> int:
     """Radiative heat (RSolverRadiativeHeat with a loaded view-factor matrix) in the closed unit
     box with k x k patches per face (k = args.rad_patches): emissivity U(0.2, 0.95) and
@@ -778,11 +879,12 @@ def main() -> int:
     ap.add_argument("--fluid-restart", type=int, default=400)
     ap.add_argument("--rad-patches", type=int, default=32)
     ap.add_argument("--workload", choices=["heat", "elasticity", "batch", "eigen", "electrostatics", "magnetostatics",
-                                                 "fluid", "radiosity"],
+                                                 "fluid", "radiosity", "transport"],
                     default="heat",
                     help="heat (default, the headline), the elasticity cantilever, the batched "
                          "transient heat systems, the modal analysis (Arnoldi), electrostatics, "
-                         "magnetostatics, the steady duct flow (Picard step) or radiative heat")
+                         "magnetostatics, the steady duct flow (Picard step), radiative heat or fluid "
+                         "heat transport")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--unbatched-budget", type=float, default=40.0,
@@ -821,6 +923,8 @@ def main() -> int:
         return fluid(rs, args)
     if args.workload == "radiosity":
         return radiosity(rs, args)
+    if args.workload == "transport":
+        return transport(rs, args)
     c = args.cells
     log(f"building {c}^3 Kuhn mesh and fields")
     xyz, tets, fixed, t_node, k, q = build_problem(rs, c, args.sigma, args.seed_k)
