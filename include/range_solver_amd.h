@@ -128,9 +128,11 @@ int rs_matrix_download(const rs_matrix *A, uint32_t *row_ptr, uint32_t *col, dou
 int rs_matrix_spmv(rs_matrix *A, const double *x, double *y);               /* host vectors */
 int rs_matrix_spmv_device(rs_matrix *A, const double *d_x, double *d_y);
 /* SpMV shapes: 0..4 = vector CSR with 4/8/16/32/64 lanes per row; 5, 6 = LDS-staged tile CSR
- * with <= 2048 / <= 4096 nonzeros per tile; 7 = SELL-64 copy (one row per lane).  The library
- * times every shape once per matrix and precision (autotuner) and uses the fastest. */
-#define RS_SPMV_SHAPES 8
+ * with <= 2048 / <= 4096 nonzeros per tile; 7 = SELL-64 copy (one row per lane); 8 = SELL-64 with
+ * int16 column offsets from the slice's first row; 9, 10 = 7, 8 with 8 entries of a row per loop
+ * trip.  The library times every shape once per matrix and precision (autotuner) and uses the
+ * fastest. */
+#define RS_SPMV_SHAPES 11
 /* Measurement: average device time (ms, stream events) of `reps` back-to-back SpMV launches
  * (precision RS_PRECISION_FP64: FP64 values/vectors; RS_PRECISION_MIXED: FP32) of shape
  * `plan` (-1 = the autotuned shape). */

@@ -947,6 +947,7 @@ static const int kVecLanes[5] = { 4, 8, 16, 32, 64 };
 static const uint32_t kTileCap[2] = { 2048, 4096 };
 constexpr int kSellPlan = 7;      // SellRows; the shapes after it read the same slices
 static_assert(kNumShapes == 11, "plan_ms holds 11 shapes");
+static_assert(RS_SPMV_SHAPES == kNumShapes, "the header's shape count matches the library's");
 
 static bool is_sell_plan(int id) { return id >= kSellPlan; }
 

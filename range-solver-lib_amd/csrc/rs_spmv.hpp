@@ -238,7 +238,8 @@ struct SpmvPlan {
 };
 
 // Shape ids: 0..4 VecRows<4,8,16,32,64>, 5 TileRows<2048>, 6 TileRows<4096>, 7 SellRows,
-// 8 SellRowsOff, 9 SellRowsU<8>, 10 SellRowsOffU<8>
+// 8 SellRowsOff, 9 SellRowsU<8>, 10 SellRowsOffU<8>.  (16 entries per trip were measured too and
+// lost on every matrix tried: more VGPRs, lower occupancy.)
 constexpr int kNumShapes = 11;
 
 #define RS_SHAPE_SWITCH(id, KERNEL, grid, stream, ...)                                         \

@@ -27,6 +27,15 @@ def test_library_exports_every_declared_symbol(rs):
     assert set(rs.exported_symbols()) == set(names)
 
 
+def test_spmv_shape_count_matches_header(rs):
+    """rs_matrix_spmv_plan writes RS_SPMV_SHAPES tuning times: the header, the binding's shape
+    tables and the library (a static_assert in the library) must agree."""
+    with open(os.path.join(ROOT, "include", "range_solver_amd.h")) as f:
+        m = re.search(r"#define\s+RS_SPMV_SHAPES\s+(\d+)", f.read())
+    assert m and int(m.group(1)) == rs.SPMV_SHAPES
+    assert len(rs.SPMV_SHAPE_NAMES) == rs.SPMV_SHAPES == len(rs.SPMV_COLUMN_BYTES)
+
+
 def test_no_device_is_reported_cleanly(rs):
     rs.load()
     if rs.device_count() > 0:
