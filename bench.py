@@ -164,6 +164,7 @@ def elasticity(rs, args) -> int:
     t0 = time.perf_counter()
     b = asm.assemble_stress(E, nu, rho, [0.0, 0.0, 0.0], np.zeros(3 * nn), tris, trac)
     t_asm = time.perf_counter() - t0
+    asm_kernel_ms = asm.last_ms
     log(f"dofs={n} nnz={asm.nnz} colours={asm.n_colours} assembly {asm.last_ms:.1f} ms (kernels) "
         f"{t_asm * 1e3:.0f} ms (with host transfers)")
     db, dx = rs.DeviceArray.from_host(b), rs.DeviceArray(n)
@@ -212,7 +213,7 @@ def elasticity(rs, args) -> int:
            "dtype": "f64 / f32 SpMV+precond with f64 residuals", "data": "synthetic",
            "config": {"workload": f"cantilever {nx}x{ny}x{nz} cells x 6 tets, E lognormal(0,1), nu 0.3",
                       "nodes": int(nn), "tets": int(ne), "dofs": int(n), "nnz": int(asm.nnz)},
-           "assembly_ms": round(asm.last_ms, 3), "methods": res,
+           "assembly_ms": round(asm_kernel_ms, 3), "methods": res,
            "speedup_mixed_vs_fp64": round(res["fp64_blockjacobi3_cg"]["time_s"] / res["mixed_blockjacobi3_cg"]["time_s"], 3),
            "post_processing": {"stress_process_ms": round(spm, 4), "max_von_mises": vmax}}
     print(json.dumps(out), flush=True)
