@@ -252,6 +252,27 @@ int rs_assemble_electrostatics_device(rs_assembly *S, const double *d_eps_r, con
 int rs_electrostatics_process_device(rs_assembly *S, const double *d_eps_r, const double *d_sigma,
                                      const double *d_potential, double *d_field, double *d_current,
                                      double *d_energy, double *d_resistivity, double *d_joule);
+/* RSolverStress problem kinds: static (Ae = Ke), transient (time solver on: Ae = Me + alpha dt Ke,
+ * be = dt fe + (Me - (1 - alpha) dt Ke) u) and modal (R_PROBLEM_STRESS_MODAL: Ae = Ke, M = Me). */
+enum { RS_STRESS_STATIC = 0, RS_STRESS_TRANSIENT = 1, RS_STRESS_MODAL = 2 };
+typedef struct rs_stress_conf {
+    int32_t problem;
+    double alpha;               /* time-march approximation coefficient (transient) */
+    double dt;                  /* current time-step size (transient) */
+    double gravity[3];
+} rs_stress_conf;
+/* The full RSolverStress::prepare for tetrahedra (src/rsolverstress.cpp:54-724, 1654-1772): host
+ * arrays as rs_assemble_stress, plus the thermal-expansion load (thermal_expansion and dT = element
+ * temperature - environment temperature, per element; both NULL for none, :688-693), the consistent
+ * mass (:665-680) for the transient and modal kinds, and the lifting with Ae + Be (:1719-1739).
+ * u_node holds nodeDisplacement: the prescribed values and, when transient, the previous step. */
+int rs_assemble_stress_ex(rs_assembly *S, const rs_stress_conf *conf, const double *E, const double *nu,
+                          const double *rho, const double *thermal_expansion, const double *dT,
+                          const double *u_node, uint32_t n_tri, const uint32_t *tris, const double *traction,
+                          double *b);
+/* The mass matrix of the last modal rs_assemble_stress_ex (owned by the assembly; same pattern as
+ * rs_assembly_matrix); RSolverStress::solveEigenValue passes (M, A) to REigenValueSolver. */
+int rs_assembly_mass_matrix(rs_assembly *S, rs_matrix **M);
 /* RSolverMagnetostatics::prepare + assemblyMatrix (src/rsolvermagnetostatics.cpp:44-145, 234-287)
  * on a 3-DOF assembly whose nodes are fixed with all three components: per component the
  * negative Laplacian, source e0 V/4 (grad N_m x J_m) from the node current density (3 per node),

@@ -93,6 +93,8 @@ def load() -> C.CDLL:
                                      C.c_int, C.c_double, C.c_double, _P, _P]
     lib.or_stress_assemble.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P,
                                        C.c_uint32, _P, _P, _P, C.c_uint32, _P, _P]
+    lib.or_stress_assemble_ex.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint32, _P,
+                                          _P, C.c_int, C.c_double, C.c_double, _P, C.c_uint32, _P, _P, _P]
     _lib = lib
     return lib
 
@@ -227,6 +229,33 @@ def stress_assemble(xyz, tets, E, nu, rho, gravity, u_node, tris, traction, book
     lib.or_stress_assemble(xyz.size // 3, _p(xyz), tets.size // 4, _p(tets), _p(E), _p(nu), _p(rho),
                            _p(g), _p(u_node), ntri, _p(tr), _p(tt), _p(book), n_dof, A, _p(b))
     return _export(A, n_dof), b[:n_dof]
+
+
+STRESS_STATIC, STRESS_TRANSIENT, STRESS_MODAL = 0, 1, 2
+
+
+def stress_assemble_ex(xyz, tets, E, nu, rho, gravity, u_node, book, n_dof, alpha_t=None, dT=None, tris=None,
+                       traction=None, problem=STRESS_STATIC, alpha=1.0, dt=1.0):
+    """The full RSolverStress volume loop + assemblyMatrix restated (thermal load, consistent
+    mass, transient and modal branches); returns (A, M or None, b)."""
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1)
+    E, nu, rho, u_node = (np.ascontiguousarray(a, np.float64) for a in (E, nu, rho, u_node))
+    at = np.ascontiguousarray(alpha_t, np.float64) if alpha_t is not None else None
+    dt_ = np.ascontiguousarray(dT, np.float64) if dT is not None else None
+    g = np.ascontiguousarray(gravity, np.float64)
+    ntri = 0 if tris is None else int(np.asarray(tris).size // 3)
+    tr = np.ascontiguousarray(tris, np.uint32).reshape(-1) if ntri else None
+    tt = np.ascontiguousarray(traction, np.float64).reshape(-1) if ntri else None
+    book = np.ascontiguousarray(book, np.uint32)
+    A = lib.or_sparse_create(n_dof)
+    M = lib.or_sparse_create(n_dof) if problem == STRESS_MODAL else None
+    b = np.zeros(max(1, n_dof))
+    lib.or_stress_assemble_ex(xyz.size // 3, _p(xyz), tets.size // 4, _p(tets), _p(E), _p(nu), _p(rho), _p(at),
+                              _p(dt_), _p(g), _p(u_node), ntri, _p(tr), _p(tt), int(problem), float(alpha),
+                              float(dt), _p(book), n_dof, A, M, _p(b))
+    return _export(A, n_dof), (_export(M, n_dof) if M is not None else None), b[:n_dof]
 
 
 def transient_heat(xyz, tets, k, q, rhoc, t_node0, book, n_dof, dt, steps, cvg=1e-13,

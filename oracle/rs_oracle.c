@@ -690,7 +690,28 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
                        uint32_t ntri, const uint32_t *tris, const double *traction,
                        const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b)
 {
+    return or_stress_assemble_ex(nn, xyz, ne, tets, E, nu, rho, NULL, NULL, gravity, u_node, ntri, tris, traction,
+                                 OR_STRESS_STATIC, 1.0, 1.0, book, n_dof, A, NULL, b);
+}
+
+/* The whole volume loop of RSolverStress::prepare (src/rsolverstress.cpp:573-715: Ke, the
+ * consistent mass Me when the time solver is on or the analysis is modal (:665-680, needsMass
+ * :56), gravity and thermal-expansion loads :683-693) and assemblyMatrix (:1654-1772): transient
+ * Ae = Me + alpha dt Ke, be = dt fe + (Me - (1 - alpha) dt Ke) u (:1669-1703); modal Ae = Ke,
+ * Be = Me (:1704-1709), M receives Me; static Ae = Ke; lifting subtracts (Ae + Be) u of every
+ * prescribed position (:1719-1739).  alpha_t and dT (thermal expansion, element temperature
+ * minus environment temperature) may be NULL (no thermal load).  Surface tractions enter as fe,
+ * times dt in the transient case. */
+int or_stress_assemble_ex(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                          const double *E, const double *nu, const double *rho, const double *alpha_t,
+                          const double *dT, const double *gravity, const double *u_node,
+                          uint32_t ntri, const uint32_t *tris, const double *traction,
+                          int problem, double alpha, double dt,
+                          const uint32_t *book, uint32_t n_dof, or_sparse *A, or_sparse *M, double *b)
+{
     (void)nn;
+    const int transient = problem == OR_STRESS_TRANSIENT, modal = problem == OR_STRESS_MODAL;
+    const int needs_mass = transient || modal;
     memset(b, 0, sizeof(double) * n_dof);
     /* Surface elements: force per unit area, :445-515 with surfaceThickness == 0, so the
      * element matrices are zero and only fe is scattered. */
@@ -713,6 +734,7 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
                 for (int c = 0; c < 3; c++) fe[3 * m + c] += traction[3 * (size_t)t + c] * integ;
             }
         }
+        if (transient) for (int k = 0; k < 9; k++) fe[k] *= dt;
         stress_scatter(A, b, book, nd, 3, NULL, fe, 0);
     }
 #pragma omp parallel for
@@ -720,7 +742,7 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
         const uint32_t *nd = tets + 4 * e;
         double X[4][3], J[3][3], B[4][3], N[4];
         double De[6][6] = { { 0 } }, Be[12][6], BeD[12][6];
-        double Ke[144] = { 0 }, fe[12] = { 0 };
+        double Ke[144] = { 0 }, Me[144] = { 0 }, fe[12] = { 0 };
         for (int m = 0; m < 4; m++)
             for (int a = 0; a < 3; a++) X[m][a] = xyz[3 * (size_t)nd[m] + a];
         double Ee = E[e], v = nu[e];
@@ -730,6 +752,7 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
         De[3][3] = De[4][4] = De[5][5] = (1.0 - 2.0 * v) / 2.0;
         double f = Ee / ((1.0 + v) * (1.0 - 2.0 * v));
         for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) De[i][j] *= f;
+        const double th = (alpha_t && dT) ? alpha_t[e] * dT[e] : 0.0;
         for (int ip = 0; ip < 4; ip++) {
             tet_shape(TET_IP[ip], N);
             double detJ = tet_jacobian(X, J);
@@ -757,19 +780,46 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
                     for (int l = 0; l < 6; l++) s += BeD[i][l] * Be[j][l];
                     Ke[i * 12 + j] += s * detJ * TET_W;
                 }
-            for (int m = 0; m < 4; m++)                                         /* :683-685 */
-                for (int c = 0; c < 3; c++) fe[3 * m + c] += gravity[c] * rho[e] * N[m] * detJ * TET_W;
+            for (int m = 0; m < 4; m++) {
+                if (needs_mass)                                                 /* :665-680 */
+                    for (int n = 0; n < 4; n++) {
+                        const double val = N[m] * N[n] * rho[e] * detJ * TET_W;
+                        for (int c = 0; c < 3; c++) Me[(3 * m + c) * 12 + 3 * n + c] += val;
+                    }
+                for (int c = 0; c < 3; c++) fe[3 * m + c] += gravity[c] * rho[e] * N[m] * detJ * TET_W;  /* :683-685 */
+                for (int n = 0; n < 3; n++)                                     /* :688-693 */
+                    for (int c = 0; c < 3; c++) fe[3 * m + c] += th * BeD[3 * m + c][n] * detJ * TET_W;
+            }
         }
-        /* assemblyMatrix, static branch :1710-1714, lifting :1720-1739 */
-        for (int m = 0; m < 4; m++)
+        /* assemblyMatrix */
+        double Ae[144], Bm[144], be[12];
+        for (int i = 0; i < 144; i++) {
+            Ae[i] = transient ? Me[i] + alpha * dt * Ke[i] : Ke[i];
+            Bm[i] = modal ? Me[i] : 0.0;
+        }
+        for (int i = 0; i < 12; i++) {
+            be[i] = transient ? dt * fe[i] : fe[i];
+            if (transient)
+                for (int n = 0; n < 4; n++)
+                    for (int c = 0; c < 3; c++)
+                        be[i] += (Me[i * 12 + 3 * n + c] - (1.0 - alpha) * dt * Ke[i * 12 + 3 * n + c])
+                               * u_node[3 * (size_t)nd[n] + c];
+        }
+        for (int m = 0; m < 4; m++)                                             /* :1719-1739 */
             for (int c = 0; c < 3; c++) {
                 uint32_t dof = 3 * nd[m] + c;
                 if (book[dof] != OR_EOD) continue;
                 double u = u_node[dof];
-                for (int n = 0; n < 12; n++) fe[n] -= Ke[n * 12 + 3 * m + c] * u;
+                for (int n = 0; n < 12; n++) be[n] -= (Ae[n * 12 + 3 * m + c] + Bm[n * 12 + 3 * m + c]) * u;
             }
 #pragma omp critical
-        stress_scatter(A, b, book, nd, 4, Ke, fe, 1);
+        {
+            stress_scatter(A, b, book, nd, 4, Ae, be, 1);
+            if (modal && M) {
+                double zero[12] = { 0 };
+                stress_scatter(M, zero, book, nd, 4, Me, zero, 1);
+            }
+        }
     }
     return 0;
 }

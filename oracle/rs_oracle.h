@@ -111,6 +111,16 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
                        const double *gravity, const double *u_node,
                        uint32_t ntri, const uint32_t *tris, const double *traction,
                        const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b);
+enum { OR_STRESS_STATIC = 0, OR_STRESS_TRANSIENT = 1, OR_STRESS_MODAL = 2 };
+/* The full RSolverStress volume loop + assemblyMatrix (src/rsolverstress.cpp:573-715, 1654-1772):
+ * thermal load (alpha_t, dT per element, may be NULL), consistent mass, transient theta-scheme,
+ * modal M (may be NULL unless modal). */
+int or_stress_assemble_ex(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
+                          const double *E, const double *nu, const double *rho, const double *alpha_t,
+                          const double *dT, const double *gravity, const double *u_node,
+                          uint32_t ntri, const uint32_t *tris, const double *traction,
+                          int problem, double alpha, double dt,
+                          const uint32_t *book, uint32_t n_dof, or_sparse *A, or_sparse *M, double *b);
 
 /* RSolverMagnetostatics prepare + assemblyMatrix (volume), src/rsolvermagnetostatics.cpp:44-145,
  * 234-287; book has 3 entries per node (all three fixed together). */

@@ -83,6 +83,14 @@ class FluidConf(C.Structure):
                 ("unsteady", C.c_int32)]
 
 
+STRESS_STATIC, STRESS_TRANSIENT, STRESS_MODAL = 0, 1, 2
+
+
+class StressConf(C.Structure):
+    """RSolverStress problem kind (static / transient / modal), time-march alpha, dt, gravity."""
+    _fields_ = [("problem", C.c_int32), ("alpha", C.c_double), ("dt", C.c_double), ("gravity", C.c_double * 3)]
+
+
 class TransportConf(C.Structure):
     """RSolverFluidHeat / RSolverFluidParticle time-step parameters."""
     _fields_ = [("alpha", C.c_double), ("dt", C.c_double), ("unsteady", C.c_int32)]
@@ -138,6 +146,8 @@ _SIGS = {
     "rs_assemble_heat": [_P, _P, _P, _P, _P, C.c_int32, C.c_double, C.c_double, _P],
     "rs_assemble_heat_device": [_P, _P, _P, _P, _P, C.c_int32, C.c_double, C.c_double, _P],
     "rs_assemble_stress": [_P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P],
+    "rs_assemble_stress_ex": [_P, C.POINTER(StressConf), _P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P],
+    "rs_assembly_mass_matrix": [_P, C.POINTER(_P)],
     "rs_assembly_dofs_to_nodes": [_P, _P, _P], "rs_assembly_destroy": [_P],
     "rs_assembly_set_method": [_P, C.c_int32],
     "rs_heat_flux_device": [_P, _P, _P, _P],
@@ -630,6 +640,26 @@ class Assembly:
         _check(_lib.rs_assemble_stress(self.h, _ptr(E), _ptr(nu), _ptr(rho), _ptr(g),
                                        _ptr(u_node), ntri, _ptr(tr), _ptr(tt), _ptr(b)))
         return b
+
+    def assemble_stress_ex(self, conf: "StressConf", E, nu, rho, u_node, thermal_expansion=None, dT=None, tris=None,
+                           traction=None):
+        """The full RSolverStress prepare (thermal load, consistent mass, transient / modal);
+        host arrays; returns b.  After a modal call mass_matrix() is the assembled M."""
+        E, nu, rho, u_node = _f64(E), _f64(nu), _f64(rho), _f64(u_node)
+        at = _f64(thermal_expansion) if thermal_expansion is not None else None
+        dt_ = _f64(dT) if dT is not None else None
+        ntri = 0 if tris is None else int(np.asarray(tris).size // 3)
+        tr = _u32(tris) if ntri else None
+        tt = _f64(traction) if ntri else None
+        b = np.empty(self.n_dof, np.float64)
+        _check(_lib.rs_assemble_stress_ex(self.h, C.byref(conf), _ptr(E), _ptr(nu), _ptr(rho), _ptr(at), _ptr(dt_),
+                                          _ptr(u_node), ntri, _ptr(tr), _ptr(tt), _ptr(b)))
+        return b
+
+    def mass_matrix(self) -> "SparseMatrix":
+        mh = C.c_void_p()
+        _check(_lib.rs_assembly_mass_matrix(self.h, C.byref(mh)))
+        return SparseMatrix(0, None, None, None, _handle=mh, _owner=self)
 
     def heat_flux_device(self, dk: DeviceArray, dt_node: DeviceArray, dflux: DeviceArray) -> None:
         """RSolverHeat::process (volume elements): 3 flux values per element."""

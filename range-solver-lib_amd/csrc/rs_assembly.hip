@@ -878,9 +878,13 @@ struct StressArgs {
     const uint32_t *map;        // 144 positions per element, [(4m+n)*9 + 3i+j]
     const uint32_t *book;       // 3*node+c -> dof
     const double *E, *nu, *rho;
+    const double *alpha_t, *dT; // thermal expansion, element minus environment temperature (or null)
     double gx, gy, gz;
-    const double *u_node;       // prescribed displacements (3 per node)
+    const double *u_node;       // nodeDisplacement: prescribed values, and the previous step (transient)
     double *val, *b;
+    double *mval;               // modal: values of M (same pattern as A)
+    int problem;                // RS_STRESS_STATIC / _TRANSIENT / _MODAL
+    double alpha, dt;
 };
 
 // 16 lanes per element: lane (m, n) assembles the 3x3 block coupling node m to node n.
@@ -914,29 +918,55 @@ __global__ __launch_bounds__(kBlock) void k_stress_colour(StressArgs a, uint32_t
 #pragma unroll
         for (int j = 0; j < 3; j++)
             K[i][j] = V * (lam * g[m][i] * g[n][j] + mu * g[m][j] * g[n][i] + (i == j ? mu * gmn : 0.0));
-    // Dirichlet lifting of node n's prescribed components into rows of node m
-    double lift[3] = { 0.0, 0.0, 0.0 };
+    // consistent mass rho V (1 + d_mn) / 20 on the component diagonal (src/rsolverstress.cpp:665-680)
+    const bool transient = a.problem == RS_STRESS_TRANSIENT, modal = a.problem == RS_STRESS_MODAL;
+    const double Mmn = (transient || modal) ? a.rho[orig] * V * (m == n ? 2.0 : 1.0) / 20.0 : 0.0;
+    // assemblyMatrix (:1654-1772): transient Ae = Me + alpha dt Ke, modal Ae = Ke and Be = Me
+    double Ae[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Ae[i][j] = transient ? (i == j ? Mmn : 0.0) + a.alpha * a.dt * K[i][j] : K[i][j];
+    // node n's displacements into the rows of node m: the transient history term (all components)
+    // and the lifting of the prescribed ones with Ae + Be
+    double rhs[3] = { 0.0, 0.0, 0.0 };
+    const double *un = a.u_node + 3 * (uint64_t)nd[n];
+    if (transient) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) rhs[i] += ((i == j ? Mmn : 0.0) - (1.0 - a.alpha) * a.dt * K[i][j]) * un[j];
+    }
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         if (a.book[3 * (uint64_t)nd[n] + j] != RS_EOD) continue;
-        double u = a.u_node[3 * (uint64_t)nd[n] + j];
+        const double u = un[j];
 #pragma unroll
-        for (int i = 0; i < 3; i++) lift[i] += K[i][j] * u;
+        for (int i = 0; i < 3; i++) rhs[i] -= (Ae[i][j] + (modal && i == j ? Mmn : 0.0)) * u;
     }
 #pragma unroll
-    for (int i = 0; i < 3; i++) lift[i] = group_sum<4>(lift[i]);
+    for (int i = 0; i < 3; i++) rhs[i] = group_sum<4>(rhs[i]);
     const uint32_t *mp = a.map + 144ull * e + lane * 9;
+    // node m's own loads: gravity (:683-685) and thermal expansion (:688-693), whose sum over the
+    // three normal components of Be De is alpha dT E / (1 - 2 nu) grad N_m for the isotropic De
     double w = a.rho[orig] * V * 0.25;
-    double grav[3] = { a.gx * w, a.gy * w, a.gz * w };
+    double fe[3] = { a.gx * w, a.gy * w, a.gz * w };
+    if (a.alpha_t) {
+        const double th = a.alpha_t[orig] * a.dT[orig] * E / (1.0 - 2.0 * v) * V;
+#pragma unroll
+        for (int i = 0; i < 3; i++) fe[i] += th * g[m][i];
+    }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         uint32_t row = a.book[3 * (uint64_t)nd[m] + i];
         if (row == RS_EOD) continue;
-        if (n == 0) a.b[row] += grav[i] - lift[i];
+        if (n == 0) a.b[row] += (transient ? a.dt * fe[i] : fe[i]) + rhs[i];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             uint32_t pos = mp[3 * i + j];
-            if (pos != RS_EOD) a.val[pos] += K[i][j];
+            if (pos == RS_EOD) continue;
+            a.val[pos] += Ae[i][j];
+            if (modal && i == j) a.mval[pos] += Mmn;
         }
     }
 }
@@ -945,7 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_stress_colour(StressArgs a, uint32_t
 // (src/rsolverstress.cpp:498-507 with N integrated exactly).
 __global__ void k_traction_colour(const double *__restrict__ xyz, const uint32_t *__restrict__ tris,
                                   const double *__restrict__ t, const uint32_t *__restrict__ book,
-                                  double *__restrict__ b, uint32_t f0, uint32_t f1)
+                                  double *__restrict__ b, uint32_t f0, uint32_t f1, double scale)
 {
     uint32_t f = f0 + blockIdx.x * kBlock + threadIdx.x;
     if (f >= f1) return;
@@ -956,7 +986,7 @@ __global__ void k_traction_colour(const double *__restrict__ xyz, const uint32_t
         e2[i] = xyz[3 * (uint64_t)nd[2] + i] - xyz[3 * (uint64_t)nd[0] + i];
     }
     double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
-    double area3 = 0.5 * sqrt(cx * cx + cy * cy + cz * cz) / 3.0;
+    double area3 = 0.5 * sqrt(cx * cx + cy * cy + cz * cz) / 3.0 * scale;
     for (int m = 0; m < 3; m++)
         for (int c = 0; c < 3; c++) {
             uint32_t row = book[3 * (uint64_t)nd[m] + c];
@@ -1013,6 +1043,8 @@ struct rs_assembly {
     bool rows = false;                      // every row fits kRowCap entries
     bool use_rows = true;                   // heat: row-gather kernel (else the colour launches)
     rs_matrix A;                          // borrowed by callers
+    rs_matrix M;                          // modal stress: the mass matrix (A's pattern)
+    bool has_mass = false;
     DBuf<double> fields, t_node, bvec;    // staging for the host-array entry points
     DBuf<uint32_t> tris;
     DBuf<double> trac;
@@ -1297,25 +1329,58 @@ extern "C" int rs_assemble_heat(rs_assembly *S, const double *k, const double *q
     return RS_OK;
 }
 
-extern "C" int rs_assemble_stress(rs_assembly *S, const double *E, const double *nu, const double *rho,
-                                  const double *gravity, const double *u_node, uint32_t n_tri, const uint32_t *tris,
-                                  const double *traction, double *b)
+// Modal analysis: the mass matrix M shares A's pattern (its own copy of rowptr/col and layouts).
+static int ensure_mass(rs_assembly *S)
 {
-    if (!S || !E || !nu || !rho || !gravity || !u_node || !b || (n_tri && (!tris || !traction)))
-        return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress: null argument");
-    if (S->dpn != 3) return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress: assembly has 1 dof per node");
+    if (S->has_mass) return RS_OK;
+    rs_matrix &M = S->M;
+    const rs_matrix &A = S->A;
+    hipStream_t st = runtime().stream;
+    M.n = A.n;
+    M.nnz = A.nnz;
+    M.borrowed = true;
+    int rc;
+    if ((rc = M.rowptr.alloc(A.n + 1ull)) || (rc = M.col.alloc(A.nnz)) || (rc = M.val.alloc(A.nnz))) return rc;
+    std::vector<uint32_t> rowptr(A.n + 1ull);
+    RS_HIP(hipMemcpyAsync(M.rowptr.p, A.rowptr.p, sizeof(uint32_t) * rowptr.size(), hipMemcpyDeviceToDevice, st));
+    if (A.nnz) RS_HIP(hipMemcpyAsync(M.col.p, A.col.p, sizeof(uint32_t) * A.nnz, hipMemcpyDeviceToDevice, st));
+    RS_HIP(hipMemcpyAsync(rowptr.data(), A.rowptr.p, sizeof(uint32_t) * rowptr.size(), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    if ((rc = matrix_set_layout(M, rowptr.data()))) return rc;
+    S->has_mass = true;
+    return RS_OK;
+}
+
+// RSolverStress::prepare (src/rsolverstress.cpp:54-724: traction surfaces :384-571, volume
+// :573-715) + assemblyMatrix (:1654-1772) for a static, transient or modal problem.
+static int stress_assemble(rs_assembly *S, const rs_stress_conf &conf, const double *E, const double *nu,
+                           const double *rho, const double *alpha_t, const double *dT, const double *u_node,
+                           uint32_t n_tri, const uint32_t *tris, const double *traction, double *b, const char *who)
+{
+    if (!S || !E || !nu || !rho || !u_node || !b || (n_tri && (!tris || !traction)) || (!alpha_t != !dT))
+        return fail(RS_ERR_INVALID_ARG, std::string(who) + ": null argument");
+    if (S->dpn != 3) return fail(RS_ERR_INVALID_ARG, std::string(who) + ": assembly needs 3 dofs per node");
+    if (conf.problem < RS_STRESS_STATIC || conf.problem > RS_STRESS_MODAL)
+        return fail(RS_ERR_INVALID_ARG, std::string(who) + ": unknown problem type");
     for (uint64_t i = 0; i < 3ull * n_tri; i++)
-        if (tris[i] >= S->n_nodes) return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress: triangle node out of range");
+        if (tris[i] >= S->n_nodes) return fail(RS_ERR_INVALID_ARG, std::string(who) + ": triangle node out of range");
     int rc = require_runtime();
     if (rc) return rc;
+    const bool transient = conf.problem == RS_STRESS_TRANSIENT, modal = conf.problem == RS_STRESS_MODAL;
+    if (modal && (rc = ensure_mass(S))) return rc;
     hipStream_t st = runtime().stream;
     uint64_t ne = S->n_elems;
-    if ((rc = S->fields.alloc(3 * ne + 1)) || (rc = S->t_node.alloc(3ull * S->n_nodes + 1)) ||
+    const bool thermal = alpha_t != nullptr;
+    if ((rc = S->fields.alloc((thermal ? 5 : 3) * ne + 1)) || (rc = S->t_node.alloc(3ull * S->n_nodes + 1)) ||
         (rc = S->bvec.alloc(S->n_dof + 1ull)))
         return rc;
     RS_HIP(hipMemcpyAsync(S->fields.p, E, sizeof(double) * ne, hipMemcpyHostToDevice, st));
     RS_HIP(hipMemcpyAsync(S->fields.p + ne, nu, sizeof(double) * ne, hipMemcpyHostToDevice, st));
     RS_HIP(hipMemcpyAsync(S->fields.p + 2 * ne, rho, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    if (thermal) {
+        RS_HIP(hipMemcpyAsync(S->fields.p + 3 * ne, alpha_t, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(S->fields.p + 4 * ne, dT, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+    }
     RS_HIP(hipMemcpyAsync(S->t_node.p, u_node, sizeof(double) * 3 * S->n_nodes, hipMemcpyHostToDevice, st));
     // colour the traction triangles (node-disjoint) on the host
     std::vector<uint16_t> fcol;
@@ -1350,20 +1415,29 @@ extern "C" int rs_assemble_stress(rs_assembly *S, const double *E, const double 
     a.E = S->fields.p;
     a.nu = S->fields.p + ne;
     a.rho = S->fields.p + 2 * ne;
-    a.gx = gravity[0];
-    a.gy = gravity[1];
-    a.gz = gravity[2];
+    a.alpha_t = thermal ? S->fields.p + 3 * ne : nullptr;
+    a.dT = thermal ? S->fields.p + 4 * ne : nullptr;
+    a.gx = conf.gravity[0];
+    a.gy = conf.gravity[1];
+    a.gz = conf.gravity[2];
     a.u_node = S->t_node.p;
     a.val = S->A.val.p;
     a.b = S->bvec.p;
+    a.mval = modal ? S->M.val.p : nullptr;
+    a.problem = conf.problem;
+    a.alpha = conf.alpha;
+    a.dt = conf.dt;
     if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+    if (modal && S->M.nnz) k_zero<<<vblocks(S->M.nnz), kBlock, 0, st>>>(S->M.val.p, S->M.nnz);
     if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(S->bvec.p, S->n_dof);
-    // surfaces first, then volumes, as the reference's prepare() (src/rsolverstress.cpp:384, 573)
+    // surfaces first, then volumes, as the reference's prepare() (src/rsolverstress.cpp:384, 573);
+    // a transient step scales the surface loads by dt like every fe (:1662-1668)
     for (uint32_t c = 0; c < nfc; c++) {
         uint32_t f0 = foff[c], f1 = foff[c + 1];
         if (f1 > f0)
             k_traction_colour<<<grid_for(f1 - f0, kBlock), kBlock, 0, st>>>(S->xyz.p, S->tris.p, S->trac.p, S->dbook.p,
-                                                                            S->bvec.p, f0, f1);
+                                                                            S->bvec.p, f0, f1,
+                                                                            transient ? conf.dt : 1.0);
     }
     constexpr uint32_t EPB = kBlock / 16;
     for (uint32_t c = 0; c < S->n_colours; c++) {
@@ -1374,8 +1448,40 @@ extern "C" int rs_assemble_stress(rs_assembly *S, const double *E, const double 
     if ((rc = timed_end(ev, S->last_ms))) return rc;
     if ((rc = matrix_frobenius(S->A))) return rc;
     matrix_values_changed(S->A);
+    if (modal) {
+        if ((rc = matrix_frobenius(S->M))) return rc;
+        matrix_values_changed(S->M);
+    }
     RS_HIP(hipMemcpyAsync(b, S->bvec.p, sizeof(double) * S->n_dof, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
+
+extern "C" int rs_assemble_stress(rs_assembly *S, const double *E, const double *nu, const double *rho,
+                                  const double *gravity, const double *u_node, uint32_t n_tri, const uint32_t *tris,
+                                  const double *traction, double *b)
+{
+    if (!gravity) return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress: null argument");
+    rs_stress_conf conf{ RS_STRESS_STATIC, 1.0, 1.0, { gravity[0], gravity[1], gravity[2] } };
+    return stress_assemble(S, conf, E, nu, rho, nullptr, nullptr, u_node, n_tri, tris, traction, b,
+                           "rs_assemble_stress");
+}
+
+extern "C" int rs_assemble_stress_ex(rs_assembly *S, const rs_stress_conf *conf, const double *E, const double *nu,
+                                     const double *rho, const double *thermal_expansion, const double *dT,
+                                     const double *u_node, uint32_t n_tri, const uint32_t *tris,
+                                     const double *traction, double *b)
+{
+    if (!conf) return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress_ex: null argument");
+    return stress_assemble(S, *conf, E, nu, rho, thermal_expansion, dT, u_node, n_tri, tris, traction, b,
+                           "rs_assemble_stress_ex");
+}
+
+extern "C" int rs_assembly_mass_matrix(rs_assembly *S, rs_matrix **M)
+{
+    if (!S || !M) return fail(RS_ERR_INVALID_ARG, "rs_assembly_mass_matrix: null");
+    if (!S->has_mass) return fail(RS_ERR_INVALID_ARG, "rs_assembly_mass_matrix: no modal assembly yet");
+    *M = &S->M;
     return RS_OK;
 }
 
