@@ -273,6 +273,16 @@ int rs_assemble_stress_ex(rs_assembly *S, const rs_stress_conf *conf, const doub
 /* The mass matrix of the last modal rs_assemble_stress_ex (owned by the assembly; same pattern as
  * rs_assembly_matrix); RSolverStress::solveEigenValue passes (M, A) to REigenValueSolver. */
 int rs_assembly_mass_matrix(rs_assembly *S, rs_matrix **M);
+/* Local rotations of a 3-DOF assembly (RSolverGeneric, src/rsolvergeneric.cpp:353-365;
+ * RLocalRotation): R (9 doubles per node, row-major) for the nodes with active[node] != 0; NULL R
+ * removes them.  rs_assemble_stress_ex then applies applyLocalRotations
+ * (src/rsolverstress.cpp:1774-1833): Ae -> T^T Ae T, Be -> T^T Be T and be -> T be (as the
+ * reference does) before the lifting; the modal M keeps the unrotated Me. */
+int rs_assembly_set_rotations(rs_assembly *S, const double *R, const uint8_t *active);
+/* RSolverStress::setDisplacement (src/rsolverstress.cpp:800-836): d_u_node (3 per node) = x on the
+ * enabled components, 0 on the disabled ones, rotated back with iR^T (iR = R^-1) where a rotation is
+ * active. */
+int rs_stress_set_displacement_device(rs_assembly *S, const double *d_x, double *d_u_node);
 /* RSolverMagnetostatics::prepare + assemblyMatrix (src/rsolvermagnetostatics.cpp:44-145, 234-287)
  * on a 3-DOF assembly whose nodes are fixed with all three components: per component the
  * negative Laplacian, source e0 V/4 (grad N_m x J_m) from the node current density (3 per node),

@@ -94,7 +94,9 @@ def load() -> C.CDLL:
     lib.or_stress_assemble.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P,
                                        C.c_uint32, _P, _P, _P, C.c_uint32, _P, _P]
     lib.or_stress_assemble_ex.argtypes = [C.c_uint32, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint32, _P,
-                                          _P, C.c_int, C.c_double, C.c_double, _P, C.c_uint32, _P, _P, _P]
+                                          _P, C.c_int, C.c_double, C.c_double, _P, _P, _P, C.c_uint32, _P, _P, _P]
+    lib.or_stress_set_displacement.argtypes = [C.c_uint32, _P, _P, _P, _P, _P]
+    lib.or_stress_set_displacement.restype = None
     _lib = lib
     return lib
 
@@ -235,7 +237,7 @@ STRESS_STATIC, STRESS_TRANSIENT, STRESS_MODAL = 0, 1, 2
 
 
 def stress_assemble_ex(xyz, tets, E, nu, rho, gravity, u_node, book, n_dof, alpha_t=None, dT=None, tris=None,
-                       traction=None, problem=STRESS_STATIC, alpha=1.0, dt=1.0):
+                       traction=None, problem=STRESS_STATIC, alpha=1.0, dt=1.0, rot=None, rot_active=None):
     """The full RSolverStress volume loop + assemblyMatrix restated (thermal load, consistent
     mass, transient and modal branches); returns (A, M or None, b)."""
     lib = load()
@@ -252,10 +254,25 @@ def stress_assemble_ex(xyz, tets, E, nu, rho, gravity, u_node, book, n_dof, alph
     A = lib.or_sparse_create(n_dof)
     M = lib.or_sparse_create(n_dof) if problem == STRESS_MODAL else None
     b = np.zeros(max(1, n_dof))
+    R = np.ascontiguousarray(rot, np.float64).reshape(-1) if rot is not None else None
+    ra = np.ascontiguousarray(rot_active, np.uint8) if rot is not None else None
     lib.or_stress_assemble_ex(xyz.size // 3, _p(xyz), tets.size // 4, _p(tets), _p(E), _p(nu), _p(rho), _p(at),
                               _p(dt_), _p(g), _p(u_node), ntri, _p(tr), _p(tt), int(problem), float(alpha),
-                              float(dt), _p(book), n_dof, A, M, _p(b))
+                              float(dt), _p(R), _p(ra), _p(book), n_dof, A, M, _p(b))
     return _export(A, n_dof), (_export(M, n_dof) if M is not None else None), b[:n_dof]
+
+
+def stress_set_displacement(book, x, rot=None, rot_active=None):
+    """RSolverStress::setDisplacement restated: node displacements (3 per node)."""
+    lib = load()
+    book = np.ascontiguousarray(book, np.uint32)
+    x = np.ascontiguousarray(x, np.float64)
+    nn = book.size // 3
+    R = np.ascontiguousarray(rot, np.float64).reshape(-1) if rot is not None else None
+    ra = np.ascontiguousarray(rot_active, np.uint8) if rot is not None else None
+    u = np.empty(3 * nn)
+    lib.or_stress_set_displacement(nn, _p(book), _p(x), _p(R), _p(ra), _p(u))
+    return u
 
 
 def transient_heat(xyz, tets, k, q, rhoc, t_node0, book, n_dof, dt, steps, cvg=1e-13,

@@ -691,7 +691,7 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
                        const uint32_t *book, uint32_t n_dof, or_sparse *A, double *b)
 {
     return or_stress_assemble_ex(nn, xyz, ne, tets, E, nu, rho, NULL, NULL, gravity, u_node, ntri, tris, traction,
-                                 OR_STRESS_STATIC, 1.0, 1.0, book, n_dof, A, NULL, b);
+                                 OR_STRESS_STATIC, 1.0, 1.0, NULL, NULL, book, n_dof, A, NULL, b);
 }
 
 /* The whole volume loop of RSolverStress::prepare (src/rsolverstress.cpp:573-715: Ke, the
@@ -701,12 +701,15 @@ int or_stress_assemble(uint32_t nn, const double *xyz, uint32_t ne, const uint32
  * Be = Me (:1704-1709), M receives Me; static Ae = Ke; lifting subtracts (Ae + Be) u of every
  * prescribed position (:1719-1739).  alpha_t and dT (thermal expansion, element temperature
  * minus environment temperature) may be NULL (no thermal load).  Surface tractions enter as fe,
- * times dt in the transient case. */
+ * times dt in the transient case.  Local rotations (rot: 3x3 row-major per node, rot_active per
+ * node; NULL for none) act as applyLocalRotations (:1774-1833): Ae -> T^T Ae T and Be -> T^T Be T,
+ * but be -> T be, with T = diag(R of the active nodes, I otherwise), before the lifting; M receives
+ * the unrotated Me. */
 int or_stress_assemble_ex(uint32_t nn, const double *xyz, uint32_t ne, const uint32_t *tets,
                           const double *E, const double *nu, const double *rho, const double *alpha_t,
                           const double *dT, const double *gravity, const double *u_node,
                           uint32_t ntri, const uint32_t *tris, const double *traction,
-                          int problem, double alpha, double dt,
+                          int problem, double alpha, double dt, const double *rot, const uint8_t *rot_active,
                           const uint32_t *book, uint32_t n_dof, or_sparse *A, or_sparse *M, double *b)
 {
     (void)nn;
@@ -805,6 +808,45 @@ int or_stress_assemble_ex(uint32_t nn, const double *xyz, uint32_t ne, const uin
                         be[i] += (Me[i * 12 + 3 * n + c] - (1.0 - alpha) * dt * Ke[i * 12 + 3 * n + c])
                                * u_node[3 * (size_t)nd[n] + c];
         }
+        if (rot) {                                                              /* :1713-1716 */
+            double T[144] = { 0 };
+            int any = 0;
+            for (int i = 0; i < 12; i++) T[i * 12 + i] = 1.0;
+            for (int m = 0; m < 4; m++) {
+                if (!rot_active[nd[m]]) continue;
+                any = 1;
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) T[(3 * m + i) * 12 + 3 * m + j] = rot[9 * (size_t)nd[m] + 3 * i + j];
+            }
+            if (any) {
+                double tmp[144], out[144], bt[12];
+                double *mats[2] = { Ae, Bm };
+                for (int k = 0; k < 2; k++) {
+                    double *X = mats[k];
+                    for (int i = 0; i < 12; i++)                                /* tmp = T^T X */
+                        for (int j = 0; j < 12; j++) {
+                            double sum = 0.0;
+                            for (int l = 0; l < 12; l++) sum += T[l * 12 + i] * X[l * 12 + j];
+                            tmp[i * 12 + j] = sum;
+                        }
+                    for (int i = 0; i < 12; i++)                                /* out = tmp T */
+                        for (int j = 0; j < 12; j++) {
+                            double sum = 0.0;
+                            for (int l = 0; l < 12; l++) sum += tmp[i * 12 + l] * T[l * 12 + j];
+                            out[i * 12 + j] = sum;
+                        }
+                    memcpy(X, out, sizeof(out));
+                }
+                for (int i = 0; i < 12; i++) {                                  /* be = T be */
+                    double sum = 0.0;
+                    for (int l = 0; l < 12; l++) sum += T[i * 12 + l] * be[l];
+                    bt[i] = sum;
+                }
+                memcpy(be, bt, sizeof(bt));
+                /* M keeps the unrotated Me: assemblyMatrix scatters its Me argument, not the
+                 * rotated copy Be (:1762-1765) */
+            }
+        }
         for (int m = 0; m < 4; m++)                                             /* :1719-1739 */
             for (int c = 0; c < 3; c++) {
                 uint32_t dof = 3 * nd[m] + c;
@@ -822,6 +864,40 @@ int or_stress_assemble_ex(uint32_t nn, const double *xyz, uint32_t ne, const uin
         }
     }
     return 0;
+}
+
+/* RSolverStress::setDisplacement (src/rsolverstress.cpp:800-836): du = x on the enabled
+ * components and 0 on the disabled ones; an active local rotation writes iR^T du (iR = R^-1,
+ * RLocalRotation::rotateResultsVector, src/rlocalrotation.cpp:60-77). */
+void or_stress_set_displacement(uint32_t nn, const uint32_t *book, const double *x, const double *rot,
+                                const uint8_t *rot_active, double *u)
+{
+    for (uint32_t i = 0; i < nn; i++) {
+        double du[3];
+        for (int c = 0; c < 3; c++) {
+            const uint32_t p = book[3 * (size_t)i + c];
+            du[c] = p != OR_EOD ? x[p] : 0.0;
+        }
+        if (rot && rot_active[i]) {
+            const double *R = rot + 9 * (size_t)i;
+            double iR[9];
+            const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6])
+                             + R[2] * (R[3] * R[7] - R[4] * R[6]);
+            iR[0] = (R[4] * R[8] - R[5] * R[7]) / det;
+            iR[1] = (R[2] * R[7] - R[1] * R[8]) / det;
+            iR[2] = (R[1] * R[5] - R[2] * R[4]) / det;
+            iR[3] = (R[5] * R[6] - R[3] * R[8]) / det;
+            iR[4] = (R[0] * R[8] - R[2] * R[6]) / det;
+            iR[5] = (R[2] * R[3] - R[0] * R[5]) / det;
+            iR[6] = (R[3] * R[7] - R[4] * R[6]) / det;
+            iR[7] = (R[1] * R[6] - R[0] * R[7]) / det;
+            iR[8] = (R[0] * R[4] - R[1] * R[3]) / det;
+            for (int c = 0; c < 3; c++)
+                u[3 * (size_t)i + c] = iR[0 * 3 + c] * du[0] + iR[1 * 3 + c] * du[1] + iR[2 * 3 + c] * du[2];
+        } else {
+            for (int c = 0; c < 3; c++) u[3 * (size_t)i + c] = du[c];
+        }
+    }
 }
 
 /* RSolverMagnetostatics::prepare + assemblyMatrix (src/rsolvermagnetostatics.cpp:44-145,

@@ -119,8 +119,11 @@ int or_stress_assemble_ex(uint32_t nn, const double *xyz, uint32_t ne, const uin
                           const double *E, const double *nu, const double *rho, const double *alpha_t,
                           const double *dT, const double *gravity, const double *u_node,
                           uint32_t ntri, const uint32_t *tris, const double *traction,
-                          int problem, double alpha, double dt,
+                          int problem, double alpha, double dt, const double *rot, const uint8_t *rot_active,
                           const uint32_t *book, uint32_t n_dof, or_sparse *A, or_sparse *M, double *b);
+/* RSolverStress::setDisplacement with local rotations (src/rsolverstress.cpp:800-836). */
+void or_stress_set_displacement(uint32_t nn, const uint32_t *book, const double *x, const double *rot,
+                                const uint8_t *rot_active, double *u);
 
 /* RSolverMagnetostatics prepare + assemblyMatrix (volume), src/rsolvermagnetostatics.cpp:44-145,
  * 234-287; book has 3 entries per node (all three fixed together). */

@@ -148,6 +148,7 @@ _SIGS = {
     "rs_assemble_stress": [_P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P],
     "rs_assemble_stress_ex": [_P, C.POINTER(StressConf), _P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P, _P],
     "rs_assembly_mass_matrix": [_P, C.POINTER(_P)],
+    "rs_assembly_set_rotations": [_P, _P, _P], "rs_stress_set_displacement_device": [_P, _P, _P],
     "rs_assembly_dofs_to_nodes": [_P, _P, _P], "rs_assembly_destroy": [_P],
     "rs_assembly_set_method": [_P, C.c_int32],
     "rs_heat_flux_device": [_P, _P, _P, _P],
@@ -655,6 +656,19 @@ class Assembly:
         _check(_lib.rs_assemble_stress_ex(self.h, C.byref(conf), _ptr(E), _ptr(nu), _ptr(rho), _ptr(at), _ptr(dt_),
                                           _ptr(u_node), ntri, _ptr(tr), _ptr(tt), _ptr(b)))
         return b
+
+    def set_rotations(self, R, active) -> None:
+        """Local rotations (3x3 per node, row-major) for the nodes with active != 0; R None clears."""
+        if R is None:
+            _check(_lib.rs_assembly_set_rotations(self.h, None, None))
+            return
+        self._rot = _f64(R).reshape(-1)
+        self._rot_active = np.ascontiguousarray(active, dtype=np.uint8).reshape(-1)
+        _check(_lib.rs_assembly_set_rotations(self.h, _ptr(self._rot), _ptr(self._rot_active)))
+
+    def stress_set_displacement_device(self, dx, du_node) -> None:
+        """RSolverStress::setDisplacement: x -> node displacements (3 per node), rotated back."""
+        _check(_lib.rs_stress_set_displacement_device(self.h, dx.ptr, du_node.ptr))
 
     def mass_matrix(self) -> "SparseMatrix":
         mh = C.c_void_p()

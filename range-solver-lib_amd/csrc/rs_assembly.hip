@@ -885,7 +885,18 @@ struct StressArgs {
     double *mval;               // modal: values of M (same pattern as A)
     int problem;                // RS_STRESS_STATIC / _TRANSIENT / _MODAL
     double alpha, dt;
+    const double *rot;          // local rotations: 3x3 row-major per node (null: none)
+    const uint8_t *rot_active;  // per node
 };
+
+__device__ __forceinline__ void node_rotation(const StressArgs &a, uint32_t node, double (&R)[3][3])
+{
+    const bool on = a.rot_active[node] != 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) R[i][j] = on ? a.rot[9 * (uint64_t)node + 3 * i + j] : (i == j ? 1.0 : 0.0);
+}
 
 // 16 lanes per element: lane (m, n) assembles the 3x3 block coupling node m to node n.
 __global__ __launch_bounds__(kBlock) void k_stress_colour(StressArgs a, uint32_t e0, uint32_t e1)
@@ -927,25 +938,60 @@ __global__ __launch_bounds__(kBlock) void k_stress_colour(StressArgs a, uint32_t
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) Ae[i][j] = transient ? (i == j ? Mmn : 0.0) + a.alpha * a.dt * K[i][j] : K[i][j];
-    // node n's displacements into the rows of node m: the transient history term (all components)
-    // and the lifting of the prescribed ones with Ae + Be
-    double rhs[3] = { 0.0, 0.0, 0.0 };
+    // node n's displacements into the rows of node m: the transient history term (all components,
+    // unrotated blocks)
+    double hist[3] = { 0.0, 0.0, 0.0 };
     const double *un = a.u_node + 3 * (uint64_t)nd[n];
     if (transient) {
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) rhs[i] += ((i == j ? Mmn : 0.0) - (1.0 - a.alpha) * a.dt * K[i][j]) * un[j];
+            for (int j = 0; j < 3; j++) hist[i] += ((i == j ? Mmn : 0.0) - (1.0 - a.alpha) * a.dt * K[i][j]) * un[j];
     }
+    // applyLocalRotations (:1774-1833): Ae -> Rm^T Ae Rn, Be -> Rm^T Be Rn; be -> Rm be (below)
+    double Be[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Be[i][j] = (modal && i == j) ? Mmn : 0.0;
+    double Rm[3][3], Rn[3][3];
+    if (a.rot) {
+        node_rotation(a, nd[m], Rm);
+        node_rotation(a, nd[n], Rn);
+        double t[3][3], t2[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) t[i][j] = Rm[0][i] * Ae[0][j] + Rm[1][i] * Ae[1][j] + Rm[2][i] * Ae[2][j];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) Ae[i][j] = t[i][0] * Rn[0][j] + t[i][1] * Rn[1][j] + t[i][2] * Rn[2][j];
+        if (modal) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) t2[i][j] = Rm[0][i] * Be[0][j] + Rm[1][i] * Be[1][j] + Rm[2][i] * Be[2][j];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) Be[i][j] = t2[i][0] * Rn[0][j] + t2[i][1] * Rn[1][j] + t2[i][2] * Rn[2][j];
+        }
+    }
+    // lifting of the prescribed components of node n with the (rotated) Ae + Be (:1719-1739)
+    double lift[3] = { 0.0, 0.0, 0.0 };
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         if (a.book[3 * (uint64_t)nd[n] + j] != RS_EOD) continue;
         const double u = un[j];
 #pragma unroll
-        for (int i = 0; i < 3; i++) rhs[i] -= (Ae[i][j] + (modal && i == j ? Mmn : 0.0)) * u;
+        for (int i = 0; i < 3; i++) lift[i] += (Ae[i][j] + Be[i][j]) * u;
     }
 #pragma unroll
-    for (int i = 0; i < 3; i++) rhs[i] = group_sum<4>(rhs[i]);
+    for (int i = 0; i < 3; i++) {
+        hist[i] = group_sum<4>(hist[i]);
+        lift[i] = group_sum<4>(lift[i]);
+    }
     const uint32_t *mp = a.map + 144ull * e + lane * 9;
     // node m's own loads: gravity (:683-685) and thermal expansion (:688-693), whose sum over the
     // three normal components of Be De is alpha dT E / (1 - 2 nu) grad N_m for the isotropic De
@@ -956,17 +1002,54 @@ __global__ __launch_bounds__(kBlock) void k_stress_colour(StressArgs a, uint32_t
 #pragma unroll
         for (int i = 0; i < 3; i++) fe[i] += th * g[m][i];
     }
+    double be[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) be[i] = (transient ? a.dt * fe[i] : fe[i]) + hist[i];
+    if (a.rot) {
+        const double r0 = Rm[0][0] * be[0] + Rm[0][1] * be[1] + Rm[0][2] * be[2];
+        const double r1 = Rm[1][0] * be[0] + Rm[1][1] * be[1] + Rm[1][2] * be[2];
+        const double r2 = Rm[2][0] * be[0] + Rm[2][1] * be[1] + Rm[2][2] * be[2];
+        be[0] = r0;
+        be[1] = r1;
+        be[2] = r2;
+    }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         uint32_t row = a.book[3 * (uint64_t)nd[m] + i];
         if (row == RS_EOD) continue;
-        if (n == 0) a.b[row] += (transient ? a.dt * fe[i] : fe[i]) + rhs[i];
+        if (n == 0) a.b[row] += be[i] - lift[i];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             uint32_t pos = mp[3 * i + j];
             if (pos == RS_EOD) continue;
             a.val[pos] += Ae[i][j];
+            // M receives the unrotated Me: assemblyMatrix scatters its Me argument (:1762-1765)
             if (modal && i == j) a.mval[pos] += Mmn;
+        }
+    }
+}
+
+// RSolverStress::setDisplacement (src/rsolverstress.cpp:800-836): du = x on the enabled
+// components, 0 on the disabled ones; an active local rotation writes iR^T du
+// (RLocalRotation::rotateResultsVector, src/rlocalrotation.cpp:60-77).
+__global__ void k_stress_set_displacement(const uint32_t *__restrict__ book, uint32_t nn, const double *__restrict__ x,
+                                          const double *__restrict__ irot, const uint8_t *__restrict__ active,
+                                          double *__restrict__ u)
+{
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nn; i += gridDim.x * kBlock) {
+        double du[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const uint32_t p = book[3 * (uint64_t)i + c];
+            du[c] = p != RS_EOD ? x[p] : 0.0;
+        }
+        if (irot && active[i]) {
+            const double *iR = irot + 9 * (uint64_t)i;
+#pragma unroll
+            for (int c = 0; c < 3; c++) u[3 * (uint64_t)i + c] = iR[c] * du[0] + iR[3 + c] * du[1] + iR[6 + c] * du[2];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) u[3 * (uint64_t)i + c] = du[c];
         }
     }
 }
@@ -1045,6 +1128,9 @@ struct rs_assembly {
     rs_matrix A;                          // borrowed by callers
     rs_matrix M;                          // modal stress: the mass matrix (A's pattern)
     bool has_mass = false;
+    DBuf<double> rot, irot;               // stress local rotations: R and R^-1 per node (row-major)
+    DBuf<uint8_t> rot_active;
+    bool has_rot = false;
     DBuf<double> fields, t_node, bvec;    // staging for the host-array entry points
     DBuf<uint32_t> tris;
     DBuf<double> trac;
@@ -1427,6 +1513,8 @@ static int stress_assemble(rs_assembly *S, const rs_stress_conf &conf, const dou
     a.problem = conf.problem;
     a.alpha = conf.alpha;
     a.dt = conf.dt;
+    a.rot = S->has_rot ? S->rot.p : nullptr;
+    a.rot_active = S->has_rot ? S->rot_active.p : nullptr;
     if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
     if (modal && S->M.nnz) k_zero<<<vblocks(S->M.nnz), kBlock, 0, st>>>(S->M.val.p, S->M.nnz);
     if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(S->bvec.p, S->n_dof);
@@ -1475,6 +1563,63 @@ extern "C" int rs_assemble_stress_ex(rs_assembly *S, const rs_stress_conf *conf,
     if (!conf) return fail(RS_ERR_INVALID_ARG, "rs_assemble_stress_ex: null argument");
     return stress_assemble(S, *conf, E, nu, rho, thermal_expansion, dT, u_node, n_tri, tris, traction, b,
                            "rs_assemble_stress_ex");
+}
+
+// RSolverGeneric local rotations (src/rsolvergeneric.cpp:353-365, RLocalRotation::activate): R per
+// node (row-major), its inverse for the write-back; NULL R clears them.
+extern "C" int rs_assembly_set_rotations(rs_assembly *S, const double *R, const uint8_t *active)
+{
+    if (!S) return fail(RS_ERR_INVALID_ARG, "rs_assembly_set_rotations: null");
+    if (S->dpn != 3) return fail(RS_ERR_INVALID_ARG, "rs_assembly_set_rotations: assembly needs 3 dofs per node");
+    if (!R) {
+        S->has_rot = false;
+        return RS_OK;
+    }
+    if (!active) return fail(RS_ERR_INVALID_ARG, "rs_assembly_set_rotations: null active flags");
+    const uint64_t nn = S->n_nodes;
+    std::vector<double> inv(9 * nn, 0.0);
+    for (uint64_t i = 0; i < nn; i++) {
+        if (!active[i]) continue;
+        const double *r = R + 9 * i;
+        const double det = r[0] * (r[4] * r[8] - r[5] * r[7]) - r[1] * (r[3] * r[8] - r[5] * r[6]) +
+                           r[2] * (r[3] * r[7] - r[4] * r[6]);
+        if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+            return fail(RS_ERR_INVALID_ARG, "rs_assembly_set_rotations: singular rotation matrix");
+        double *o = inv.data() + 9 * i;
+        o[0] = (r[4] * r[8] - r[5] * r[7]) / det;
+        o[1] = (r[2] * r[7] - r[1] * r[8]) / det;
+        o[2] = (r[1] * r[5] - r[2] * r[4]) / det;
+        o[3] = (r[5] * r[6] - r[3] * r[8]) / det;
+        o[4] = (r[0] * r[8] - r[2] * r[6]) / det;
+        o[5] = (r[2] * r[3] - r[0] * r[5]) / det;
+        o[6] = (r[3] * r[7] - r[4] * r[6]) / det;
+        o[7] = (r[1] * r[6] - r[0] * r[7]) / det;
+        o[8] = (r[0] * r[4] - r[1] * r[3]) / det;
+    }
+    int rc = require_runtime();
+    if (rc) return rc;
+    if ((rc = S->rot.alloc(9 * nn)) || (rc = S->irot.alloc(9 * nn)) || (rc = S->rot_active.alloc(nn))) return rc;
+    hipStream_t st = runtime().stream;
+    RS_HIP(hipMemcpyAsync(S->rot.p, R, sizeof(double) * 9 * nn, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->irot.p, inv.data(), sizeof(double) * 9 * nn, hipMemcpyHostToDevice, st));
+    RS_HIP(hipMemcpyAsync(S->rot_active.p, active, nn, hipMemcpyHostToDevice, st));
+    RS_HIP(hipStreamSynchronize(st));
+    S->has_rot = true;
+    return RS_OK;
+}
+
+extern "C" int rs_stress_set_displacement_device(rs_assembly *S, const double *d_x, double *d_u_node)
+{
+    if (!S || !d_x || !d_u_node) return fail(RS_ERR_INVALID_ARG, "rs_stress_set_displacement_device: null");
+    if (S->dpn != 3)
+        return fail(RS_ERR_INVALID_ARG, "rs_stress_set_displacement_device: assembly needs 3 dofs per node");
+    int rc = require_runtime();
+    if (rc) return rc;
+    k_stress_set_displacement<<<vblocks(S->n_nodes), kBlock, 0, runtime().stream>>>(
+        S->dbook.p, S->n_nodes, d_x, S->has_rot ? S->irot.p : nullptr, S->has_rot ? S->rot_active.p : nullptr,
+        d_u_node);
+    RS_HIP(hipGetLastError());
+    return RS_OK;
 }
 
 extern "C" int rs_assembly_mass_matrix(rs_assembly *S, rs_matrix **M)

@@ -97,3 +97,98 @@ def test_modal_cantilever_matches_dense(gpu, orc):
     rs.seed_c_rand(3)
     d_ref, _, _ = orc.eigen_solve(Ms, Ks, 1, n_eigen=24, n_iter=500, eig_cvg=1e-12, cvg=1e-13)
     np.testing.assert_allclose(d, d_ref, rtol=1e-7)
+
+
+def _rotations(rng, nn, frac=0.4):
+    rot = np.tile(np.eye(3), (nn, 1, 1))
+    active = (rng.random(nn) < frac).astype(np.uint8)
+    for i in np.nonzero(active)[0]:
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        rot[i] = Q
+    return rot, active
+
+
+@pytest.mark.parametrize("problem", [0, 1, 2])
+def test_stress_rotations_match_oracle(gpu, orc, problem):
+    rs = gpu
+    xyz, tets, E, nu, rho, fixed, tris, rng = cantilever(rs, 4, 2, 3, seed=30 + problem)
+    nn, ne = xyz.shape[0], tets.shape[0]
+    rot, active = _rotations(rng, nn)
+    u = rng.standard_normal(3 * nn) * 1e-3
+    trac = np.tile([0.1, -0.2, -1.0], (tris.shape[0], 1))
+    g = (0.0, -0.5, -9.81)
+    asm = rs.Assembly(xyz, tets, 3, fixed)
+    asm.set_rotations(rot, active)
+    b = asm.assemble_stress_ex(rs.StressConf(problem, 0.6, 0.02, g), E, nu, rho, u, tris=tris, traction=trac)
+    book, n_dof = orc.node_book(nn, tets, 3, fixed)
+    A_ref, M_ref, b_ref = orc.stress_assemble_ex(xyz, tets, E, nu, rho, g, u, book, n_dof, tris=tris, traction=trac,
+                                                 problem=problem, alpha=0.6, dt=0.02, rot=rot, rot_active=active)
+    D = (asm.matrix.to_scipy() - A_ref).toarray()
+    assert np.abs(D).max() <= 1e-12 * np.abs(A_ref.data).max()
+    assert np.abs(b - b_ref).max() <= 1e-12 * np.abs(b_ref).max()
+    if problem == 2:
+        DM = (asm.mass_matrix().to_scipy() - M_ref).toarray()
+        assert np.abs(DM).max() <= 1e-12 * np.abs(M_ref.data).max()
+    x = rng.standard_normal(n_dof)
+    du = rs.DeviceArray(3 * nn)
+    asm.stress_set_displacement_device(rs.DeviceArray.from_host(x), du)
+    np.testing.assert_allclose(du.download(), orc.stress_set_displacement(book, x, rot=rot, rot_active=active),
+                               rtol=1e-13, atol=1e-15)
+
+
+def test_skew_roller_gpu(gpu):
+    """Nodes on x = L slide on the plane with normal n (local first axis n from a Householder
+    reflection, first local component fixed), x = 0 clamped, gravity: the GPU solution equals a
+    dense null-space solve of the constrained problem."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from helpers import bary_gradients, dense_stress_K
+    rs = gpu
+    L = 2.0
+    xyz, tets = rs.kuhn_box(4, 2, 2, L, 1.0, 1.0)
+    nn, ne = xyz.shape[0], tets.shape[0]
+    rng = np.random.default_rng(44)
+    E, nu, rho = rng.uniform(1.0, 3.0, ne), rng.uniform(0.2, 0.35, ne), np.ones(ne)
+    g = np.array([0.2, -0.3, -1.0])
+    clamp = np.abs(xyz[:, 0]) < 1e-12
+    roller = np.abs(xyz[:, 0] - L) < 1e-12
+    n = np.array([1.0, 0.6, -0.4])
+    n /= np.linalg.norm(n)
+    v = np.array([1.0, 0.0, 0.0]) - n
+    v /= np.linalg.norm(v)
+    H = np.eye(3) - 2.0 * np.outer(v, v)
+    rot = np.tile(np.eye(3), (nn, 1, 1))
+    rot[roller] = H
+    fixed = np.zeros((nn, 3), np.uint8)
+    fixed[clamp, :] = 1
+    fixed[roller, 0] = 1
+    asm = rs.Assembly(xyz, tets, 3, fixed.reshape(-1))
+    asm.set_rotations(rot, roller.astype(np.uint8))
+    b = asm.assemble_stress_ex(rs.StressConf(0, 1.0, 1.0, tuple(g)), E, nu, rho, np.zeros(3 * nn))
+    conf = rs.MatrixSolverConf(n_outer_iterations=100000, solver_cvg_value=1e-13, stop_rule=rs.STOP_RELRES)
+    db, dx = rs.DeviceArray.from_host(b), rs.DeviceArray.from_host(np.zeros(asm.n_dof))
+    info = rs.RMatrixSolver(conf).solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+    assert info.converged == 1
+    du = rs.DeviceArray(3 * nn)
+    asm.stress_set_displacement_device(dx, du)
+    u = du.download().reshape(nn, 3)
+    K = dense_stress_K(xyz, tets, E, nu)
+    f = np.zeros(3 * nn)
+    for e, t in enumerate(tets):
+        _, V = bary_gradients(xyz[t])
+        for c in range(3):
+            f[3 * t + c] += rho[e] * g[c] * V / 4.0
+    t1 = np.cross(n, [0.0, 0.0, 1.0])
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(n, t1)
+    cols = []
+    for i in range(nn):
+        if clamp[i]:
+            continue
+        for d in ((t1, t2) if roller[i] else np.eye(3)):
+            z = np.zeros(3 * nn)
+            z[3 * i:3 * i + 3] = d
+            cols.append(z)
+    Z = np.array(cols).T
+    u_ref = (Z @ np.linalg.solve(Z.T @ K @ Z, Z.T @ f)).reshape(nn, 3)
+    np.testing.assert_allclose(u, u_ref, rtol=0, atol=1e-9 * np.abs(u_ref).max())
