@@ -85,3 +85,26 @@ def test_wrong_assembly_kind_is_rejected(gpu):
     a4 = rs.Assembly(xyz, tets, 4, np.zeros(4 * nn, np.uint8))
     with pytest.raises(rs.RSError, match="stream velocity"):
         a4.assemble_fluid_device(rs.FluidConf(0.0, 1.0, 1.0, 0), d, d, d, d, d, d, d)
+
+
+def test_stress_ex_and_rotation_errors(gpu):
+    rs = gpu
+    xyz, tets = rs.kuhn_box(2, 2, 2)
+    nn, ne = xyz.shape[0], tets.shape[0]
+    a3 = rs.Assembly(xyz, tets, 3, np.zeros(3 * nn, np.uint8))
+    ones = np.ones(ne)
+    with pytest.raises(rs.RSError, match="unknown problem"):
+        a3.assemble_stress_ex(rs.StressConf(7, 1.0, 1.0, (0.0, 0.0, 0.0)), ones, np.full(ne, 0.3), ones,
+                              np.zeros(3 * nn))
+    with pytest.raises(rs.RSError, match="null"):
+        a3.assemble_stress_ex(rs.StressConf(0, 1.0, 1.0, (0.0, 0.0, 0.0)), ones, np.full(ne, 0.3), ones,
+                              np.zeros(3 * nn), thermal_expansion=ones)
+    with pytest.raises(rs.RSError, match="no modal assembly"):
+        a3.mass_matrix()
+    R = np.tile(np.eye(3), (nn, 1, 1))
+    R[1] = 0.0
+    with pytest.raises(rs.RSError, match="singular"):
+        a3.set_rotations(R, np.ones(nn, np.uint8))
+    a1 = rs.Assembly(xyz, tets, 1, np.zeros(nn, np.uint8))
+    with pytest.raises(rs.RSError, match="3 dofs per node"):
+        a1.set_rotations(np.tile(np.eye(3), (nn, 1, 1)), np.ones(nn, np.uint8))
