@@ -430,7 +430,8 @@ __global__ void k_sell_cols(CsrView A, uint32_t *__restrict__ sell_col)
 }
 
 // int16 offsets of the SELL-64 columns from each slice's first row; padding points at the
-// lane's own row.  `far` counts columns that do not fit.
+// lane's own row, or at row n-1 for the lanes of the last slice past the end (the offset shape
+// reads x there unconditionally, and 0 * x must stay finite).  `far` counts columns that do not fit.
 __global__ void k_sell_offs(CsrView A, const uint32_t *__restrict__ sell_col, int16_t *__restrict__ sell_off,
                             uint32_t *__restrict__ far)
 {
@@ -441,7 +442,8 @@ __global__ void k_sell_offs(CsrView A, const uint32_t *__restrict__ sell_col, in
         uint32_t miss = 0;
         for (uint32_t j = 0; j < width; j++) {
             const uint32_t k = base + j * kWave + lane, c = sell_col[k];
-            const int64_t o = c == kSellPad ? (int64_t)lane : (int64_t)c - (int64_t)s * kWave;
+            const int64_t o = c == kSellPad ? (int64_t)min(r, (uint64_t)A.n - 1) - (int64_t)s * kWave
+                                            : (int64_t)c - (int64_t)s * kWave;
             if (o < -32767 || o > 32767) miss++;
             else sell_off[k] = (int16_t)o;
         }

@@ -276,3 +276,33 @@ def test_cxx_drop_in_replays_reference_test(gpu):
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "PASS" in r.stdout
+
+
+def test_spmv_shapes_read_no_entry_past_the_vector(gpu):
+    """Padding lanes of the last SELL-64 slice (n not a multiple of 64) must not read x past its
+    end: x is placed directly in front of NaNs, and every shape must return finite, exact rows."""
+    _say("spmv shapes past-the-end")
+    rs = gpu
+    rng = np.random.default_rng(22)
+    n = 1000
+    A = (sp.random(n, n, density=5e-3, random_state=rng, format="csr") + sp.eye(n)).tocsr()
+    A.sort_indices()
+    M = rs.SparseMatrix.from_scipy(A)
+    x = rng.standard_normal(n)
+    buf = rs.DeviceArray.from_host(np.concatenate([x, np.full(256, np.nan)]))
+    dx = object.__new__(rs.DeviceArray)
+    dx.dtype, dx.n, dx.ptr = np.dtype(np.float64), n, buf.ptr
+    dy = rs.DeviceArray(n)
+    lib = rs.load()
+    ref = A @ x
+    try:
+        for plan in range(rs.SPMV_SHAPES):
+            rs._check(lib.rs_matrix_force_plan(M.h, rs.PRECISION_FP64, plan))
+            M.spmv_device(dx, dy)
+            y = dy.download()
+            assert np.isfinite(y).all(), rs.SPMV_SHAPE_NAMES[plan]
+            np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12 * abs(ref).max(),
+                                       err_msg=rs.SPMV_SHAPE_NAMES[plan])
+        rs._check(lib.rs_matrix_force_plan(M.h, rs.PRECISION_FP64, -1))
+    finally:
+        dx.ptr = None      # the alias does not own the allocation
