@@ -1503,7 +1503,7 @@ struct rs_solver {
 
 namespace rs {
 
-static constexpr uint32_t kGraphIters = 16;
+static constexpr uint32_t kGraphIters = 8;
 static constexpr uint32_t kHistCap = 1u << 20;
 
 static int ensure_ws(rs_solver &S, uint32_t n, bool mixed, bool gmres)
@@ -1559,23 +1559,22 @@ static void emit_log(const rs_solver &S, const char *title, uint32_t iters, uint
     std::printf("-------------------------------------------------------------------------------\n");
 }
 
-struct StopFlags {
-    uint32_t it, done, reason, pad;
-};
-
 // Replay the captured iteration graph until the device-side stop flag is raised.  Two
-// replays are kept in flight so the host's flag check overlaps device work.
+// replays are kept in flight so that the host's flag check overlaps device work, except when
+// the error history projects convergence within the replay already in flight: then the next
+// replay waits until that one's state is read.  A replay enqueued after convergence runs all
+// of its iterations idle (each kernel still dispatches and returns), so this only changes how
+// many idle iterations follow the stop, never the iterations or the results.
 static int run_graph_loop(rs_solver &S, uint32_t &done_out)
 {
     hipStream_t st = runtime().stream;
     if (!S.hflags) {
-        RS_HIP(hipHostMalloc(&S.hflags, 2 * sizeof(StopFlags), hipHostMallocDefault));
+        RS_HIP(hipHostMalloc(&S.hflags, 2 * sizeof(SolveState), hipHostMallocDefault));
         RS_HIP(hipEventCreateWithFlags(&S.ev[0], hipEventDisableTiming));
         RS_HIP(hipEventCreateWithFlags(&S.ev[1], hipEventDisableTiming));
     }
-    StopFlags *hflags = static_cast<StopFlags *>(S.hflags);
+    SolveState *hs = static_cast<SolveState *>(S.hflags);
     hipEvent_t *ev = S.ev;
-    int rc = RS_OK;
     auto enqueue = [&](int slot) -> int {
         if (S.eager) {
             S.body();
@@ -1583,27 +1582,43 @@ static int run_graph_loop(rs_solver &S, uint32_t &done_out)
         } else {
             RS_HIP(hipGraphLaunch(S.graph, st));
         }
-        RS_HIP(hipMemcpyAsync(&hflags[slot], &S.st.p->it, sizeof(StopFlags), hipMemcpyDeviceToHost, st));
+        RS_HIP(hipMemcpyAsync(&hs[slot], S.st.p, sizeof(SolveState), hipMemcpyDeviceToHost, st));
         RS_HIP(hipEventRecord(ev[slot], st));
         return RS_OK;
     };
-    uint64_t launched = 0;
-    rc = enqueue(0);
+    uint64_t launched = 0, seen = 0;   // replays enqueued; replays whose state was read
+    bool ahead = true;                 // keep a second replay in flight
+    bool have_prev = false;
+    double e_prev = 0.0;
+    uint32_t it_prev = 0;
+    int rc = enqueue(0);
     launched = 1;
     while (rc == RS_OK) {
-        int slot = (int)(launched % 2);
-        if ((rc = enqueue(slot))) break;
-        launched++;
-        int prev = (int)((launched - 2) % 2);
-        hipError_t e = hipEventSynchronize(ev[prev]);
+        if (launched == seen || (ahead && launched == seen + 1)) {
+            if ((rc = enqueue((int)(launched % 2)))) break;
+            launched++;
+        }
+        const int slot = (int)(seen % 2);
+        hipError_t e = hipEventSynchronize(ev[slot]);
         if (e != hipSuccess) {
             rc = hip_fail(e, "hipEventSynchronize");
             break;
         }
-        if (hflags[prev].done != kStopRunning) {
-            done_out = hflags[prev].done;
+        seen++;
+        const SolveState h = hs[slot];
+        if (h.done != kStopRunning) {
+            done_out = h.done;
             break;
         }
+        // iterations still needed at the error's mean log rate since the previous read
+        ahead = true;
+        if (have_prev && h.it > it_prev && e_prev > 0.0 && h.error > 0.0 && h.cvg > 0.0 && std::isfinite(h.error)) {
+            const double rate = std::log(h.error / e_prev) / (double)(h.it - it_prev);
+            if (rate < 0.0 && std::log(h.cvg / h.error) / rate <= (double)kGraphIters) ahead = false;
+        }
+        have_prev = true;
+        e_prev = h.error;
+        it_prev = h.it;
     }
     (void)hipStreamSynchronize(st);
     return rc;
