@@ -277,29 +277,32 @@ __global__ __launch_bounds__(kBlock) void k_cyc_resid(CycArgs a)
 // operator stays FP32.  *bad is set when an offset does not fit in 16 bits or a diagonal is zero.
 __global__ void k_s16_build(CsrView A, uint32_t *__restrict__ ent, uint32_t *__restrict__ bad)
 {
+    // built from the FP64 SELL slots, so that the 64 lanes of a wavefront (64 consecutive rows)
+    // read each entry index coalesced; the diagonal sums the row's entries with c == r in entry
+    // order, as the CSR row does
     for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < (uint64_t)A.nslices * kWave;
          r += (uint64_t)gridDim.x * kBlock) {
         const uint32_t s = (uint32_t)(r / kWave), lane = (uint32_t)(r % kWave);
         const uint32_t base = A.sell_ptr[s], width = (A.sell_ptr[s + 1] - base) / kWave;
-        uint32_t b = 0, len = 0;
         double d = 1.0;
         if (r < A.n) {
-            b = A.rowptr[r];
-            len = A.rowptr[r + 1] - b;
             d = 0.0;
-            for (uint32_t k = 0; k < len; k++)
-                if (A.col[b + k] == r) d += A.val[b + k];
+            for (uint32_t j = 0; j < width; j++) {
+                const uint32_t k = base + j * kWave + lane;
+                if (A.sell_col[k] == r) d += A.sell_val[k];
+            }
             if (d == 0.0) atomicOr(bad, 1u);
         }
         for (uint32_t j = 0; j < width; j++) {
+            const uint32_t k = base + j * kWave + lane, c = A.sell_col[k];
             uint32_t e = 0;
-            if (j < len) {
-                const int64_t off = (int64_t)A.col[b + j] - (int64_t)r;
+            if (c != kSellPad) {
+                const int64_t off = (int64_t)c - (int64_t)r;
                 if (off < -32768 || off > 32767) atomicOr(bad, 1u);
-                const _Float16 h = (_Float16)(d != 0.0 ? A.val[b + j] / d : 0.0);
+                const _Float16 h = (_Float16)(d != 0.0 ? A.sell_val[k] / d : 0.0);
                 e = (uint32_t)(uint16_t)(int16_t)off | ((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
             }
-            ent[base + j * kWave + lane] = e;
+            ent[k] = e;
         }
     }
 }
@@ -1100,7 +1103,7 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     for (uint32_t l = 0; l + 1 < L.nlev; l++) {
         CycLevel &V = *L.lv[l];
         if (L.s16_min_rows == 0 || V.n < L.s16_min_rows || V.M->sell_col.n == 0) continue;
-        if ((rc = V.s16.alloc(V.M->sell_col.n))) return rc;
+        if ((rc = V.s16.alloc(V.M->sell_col.n)) || (rc = matrix_sync_plan(*V.M, kSellPlan, false))) return rc;
         uint32_t nb = 0;
         const CsrView sv = plan_view(*V.M, kSellPlan, nb);
         k_s16_build<<<vec_blocks((uint64_t)V.M->nslices * kWave), kBlock, 0, st>>>(sv, V.s16.p, L.s16bad.p + l);
