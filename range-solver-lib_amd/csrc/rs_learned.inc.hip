@@ -95,12 +95,22 @@ __device__ __forceinline__ void block_max_store(double m, double *partials)
     }
 }
 
-__global__ void k_max_fin(const double *partials, uint32_t nparts, double *out)
+constexpr int kMaxFinBlock = 1024;
+__global__ __launch_bounds__(kMaxFinBlock) void k_max_fin(const double *partials, uint32_t nparts, double *out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double t = 0.0;
-    for (uint32_t i = 0; i < nparts; i++) t = fmax(t, partials[i]);
-    *out = t;
+    // the maximum does not depend on the order: 1024 threads with independent loads, then the waves
+    __shared__ double w[kMaxFinBlock / kWave];
+    double m = 0.0;
+#pragma unroll 4
+    for (uint32_t i = threadIdx.x; i < nparts; i += kMaxFinBlock) m = fmax(m, partials[i]);
+    for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) w[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kMaxFinBlock / kWave; k++) t = fmax(t, w[k]);
+        *out = t;
+    }
 }
 
 // ---------------------------------------------------------------------------- hierarchy set-up
@@ -109,8 +119,19 @@ __global__ void k_galerkin_gather(const double *__restrict__ fval, const uint32_
                                   const uint32_t *__restrict__ gsrc, uint64_t nnzc, double *__restrict__ cval)
 {
     for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < nnzc; k += (uint64_t)gridDim.x * kBlock) {
+        uint32_t j = gptr[k];
+        const uint32_t e = gptr[k + 1];
         double s = 0.0;
-        for (uint32_t j = gptr[k]; j < gptr[k + 1]; j++) s += fval[gsrc[j]];
+        // four list entries' loads in flight at once, summed in list order
+        for (; j + 4 <= e; j += 4) {
+            const uint32_t i0 = gsrc[j], i1 = gsrc[j + 1], i2 = gsrc[j + 2], i3 = gsrc[j + 3];
+            const double v0 = fval[i0], v1 = fval[i1], v2 = fval[i2], v3 = fval[i3];
+            s += v0;
+            s += v1;
+            s += v2;
+            s += v3;
+        }
+        for (; j < e; j++) s += fval[gsrc[j]];
         cval[k] = s;
     }
 }
@@ -1090,7 +1111,7 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
         uint32_t nb = std::min(vec_blocks(std::max(V.n, 1u)), kParts);
         k_level_diag<<<nb, kBlock, 0, st>>>(V.M->rowptr.p, V.M->col.p, V.M->val.p, V.n, V.diag.p, V.dinv.p,
                                             L.part.p + (uint64_t)l * kParts);
-        k_max_fin<<<1, 64, 0, st>>>(L.part.p + (uint64_t)l * kParts, nb, L.gersh.p + l);
+        k_max_fin<<<1, kMaxFinBlock, 0, st>>>(L.part.p + (uint64_t)l * kParts, nb, L.gersh.p + l);
     }
     const Matrix &C = *L.lv.back()->M;
     k_coarse_weights<<<vec_blocks(L.nc), kBlock, 0, st>>>(C.rowptr.p, C.col.p, C.val.p, L.nc, L.cw.p, L.cdiag.p);

@@ -1932,7 +1932,8 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
     RS_HIP(hipEventCreate(&e0));
     RS_HIP(hipEventCreate(&e1));
     RS_HIP(hipEventRecord(e0, stream));
-    if ((rc = precond_build(A, ptype, block_size, P))) return rc;
+    // the learned path's iterations do not read P: it is built only if Jacobi-CG takes over
+    if (!learned && (rc = precond_build(A, ptype, block_size, P))) return rc;
     const uint32_t n = A.n;
     uint32_t nvb = vec_blocks(std::max(n, 1u));
     uint32_t hist_cap = (uint32_t)(S.hist.n / 2);
@@ -1959,6 +1960,7 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
             if (hs.reason != RS_REASON_CONVERGED) {
                 // Jacobi-CG takes over from the current solution
                 fallback = 1;
+                if ((rc = precond_build(A, ptype, block_size, P))) return rc;
                 k_state_restart<<<1, 64, 0, stream>>>(S.st.p);
                 uint32_t more = 0;
                 rc = solve_cg_mixed(S, A, P, d_x, done, more);
