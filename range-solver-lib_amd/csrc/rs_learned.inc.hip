@@ -292,9 +292,9 @@ __global__ __launch_bounds__(kBlock) void k_cyc_resid(CycArgs a)
 }
 
 // ---------------------------------------------------------------------------- compressed smoother copy
-// Row-scaled copy S = D^-1 A of a level matrix in its SELL-64 slots, FP16 values and int16
-// column offsets (col - row) packed into one uint32 per slot: 4 bytes per entry instead of 8 for
-// the FP32 SELL copy.  The smoother only needs D^-1 A (and D for the residual); the Krylov
+// Row-scaled copy S = D^-1 A of a level matrix in its SELL-64 pair slots, FP16 values and int16
+// column offsets (col - row) packed into one uint32 per slot: 4 bytes per entry instead of 6 for
+// the FP32 SELL copy with offsets.  The smoother only needs D^-1 A (and D for the residual); the Krylov
 // operator stays FP32.  *bad is set when an offset does not fit in 16 bits or a diagonal is zero.
 __global__ void k_s16_build(CsrView A, uint32_t *__restrict__ ent, uint32_t *__restrict__ bad)
 {
@@ -323,7 +323,7 @@ __global__ void k_s16_build(CsrView A, uint32_t *__restrict__ ent, uint32_t *__r
                 const _Float16 h = (_Float16)(d != 0.0 ? A.sell_val[k] / d : 0.0);
                 e = (uint32_t)(uint16_t)(int16_t)off | ((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
             }
-            ent[k] = e;
+            ent[sell_pair_slot(base, width, j, lane)] = e;
         }
     }
 }
@@ -342,24 +342,32 @@ __device__ __forceinline__ void s16_run(const S16View &S, const float *__restric
     const uint32_t s0 = lb * S.spb, s1 = min(S.nslices, s0 + S.spb);
     const uint32_t *__restrict__ ent = S.ent;
     for (uint32_t s = s0 + wid; s < s1; s += kBlock / kWave) {
-        const uint32_t b = S.sell_ptr[s], e = S.sell_ptr[s + 1];
+        const uint32_t b = S.sell_ptr[s], width = (S.sell_ptr[s + 1] - b) / kWave, np = width >> 1;
         const uint32_t r = s * kWave + lane, rr = min(r, S.n - 1);   // padding lanes read row n-1
         float acc = 0.0f;
-        uint32_t j = b + lane;
         auto term = [&](uint32_t w) {
             const int32_t off = (int16_t)(w & 0xffffu);
             const float v = (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
             return v * x[(int32_t)rr + off];
         };
-        for (; j + 3 * kWave < e; j += 4 * kWave) {
-            const uint32_t w0 = __builtin_nontemporal_load(&ent[j]), w1 = __builtin_nontemporal_load(&ent[j + kWave]);
-            const uint32_t w2 = __builtin_nontemporal_load(&ent[j + 2 * kWave]), w3 = __builtin_nontemporal_load(&ent[j + 3 * kWave]);
-            acc += term(w0);
-            acc += term(w1);
-            acc += term(w2);
-            acc += term(w3);
+        // entries 2p and 2p+1 as one 8-byte load, two pairs per trip, summed in entry order
+        auto pair = [&](uint32_t p) {
+            return __builtin_nontemporal_load(reinterpret_cast<const sell_u2 *>(&ent[b + p * 2 * kWave + 2 * lane]));
+        };
+        uint32_t p = 0;
+        for (; p + 2 <= np; p += 2) {
+            const sell_u2 w0 = pair(p), w1 = pair(p + 1);
+            acc += term(w0.x);
+            acc += term(w0.y);
+            acc += term(w1.x);
+            acc += term(w1.y);
         }
-        for (; j < e; j += kWave) acc += term(ent[j]);
+        for (; p < np; p++) {
+            const sell_u2 w0 = pair(p);
+            acc += term(w0.x);
+            acc += term(w0.y);
+        }
+        if (width & 1u) acc += term(ent[b + (width - 1) * kWave + lane]);
         if (r < S.n) epi.row(r, acc);
     }
 }

@@ -429,9 +429,10 @@ __global__ void k_sell_cols(CsrView A, uint32_t *__restrict__ sell_col)
     }
 }
 
-// int16 offsets of the SELL-64 columns from each slice's first row; padding points at the
-// lane's own row, or at row n-1 for the lanes of the last slice past the end (the offset shape
-// reads x there unconditionally, and 0 * x must stay finite).  `far` counts columns that do not fit.
+// int16 offsets of the SELL-64 columns from each slice's first row, in the pair slots;
+// padding points at the lane's own row, or at row n-1 for the lanes of the last slice past the
+// end (the offset shape reads x there unconditionally, and 0 * x must stay finite).  `far`
+// counts columns that do not fit.
 __global__ void k_sell_offs(CsrView A, const uint32_t *__restrict__ sell_col, int16_t *__restrict__ sell_off,
                             uint32_t *__restrict__ far)
 {
@@ -441,17 +442,29 @@ __global__ void k_sell_offs(CsrView A, const uint32_t *__restrict__ sell_col, in
         uint32_t base = A.sell_ptr[s], width = (A.sell_ptr[s + 1] - base) / kWave;
         uint32_t miss = 0;
         for (uint32_t j = 0; j < width; j++) {
-            const uint32_t k = base + j * kWave + lane, c = sell_col[k];
+            const uint32_t c = sell_col[base + j * kWave + lane];
             const int64_t o = c == kSellPad ? (int64_t)min(r, (uint64_t)A.n - 1) - (int64_t)s * kWave
                                             : (int64_t)c - (int64_t)s * kWave;
             if (o < -32767 || o > 32767) miss++;
-            else sell_off[k] = (int16_t)o;
+            else sell_off[sell_pair_slot(base, width, j, lane)] = (int16_t)o;
         }
         if (miss) atomicAdd(far, miss);
     }
 }
 
-// SELL-64 values from the CSR values (padding 0); FP32 copy from the FP64 SELL values.
+// FP32 SELL values in the pair slots from the FP64 SELL values in the plain slots.
+__global__ void k_sell_to_f32(CsrView A, const double *__restrict__ sell_val, float *__restrict__ sell_val32)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < (uint64_t)A.nslices * kWave;
+         r += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t s = (uint32_t)(r / kWave), lane = (uint32_t)(r % kWave);
+        const uint32_t base = A.sell_ptr[s], width = (A.sell_ptr[s + 1] - base) / kWave;
+        for (uint32_t j = 0; j < width; j++)
+            sell_val32[sell_pair_slot(base, width, j, lane)] = (float)sell_val[base + j * kWave + lane];
+    }
+}
+
+// SELL-64 values from the CSR values (padding 0).
 __global__ void k_sell_vals(CsrView A, double *__restrict__ sell_val)
 {
     for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < (uint64_t)A.nslices * kWave;
@@ -1126,7 +1139,8 @@ int matrix_sync_plan(Matrix &A, int plan, bool fp32)
     }
     if (fp32 && !A.sell32_valid) {
         if ((rc = A.sell_val32.alloc(total))) return rc;
-        k_to_f32<<<vec_blocks(total), kBlock, 0, st>>>(A.sell_val.p, A.sell_val32.p, total);
+        if (A.nslices)
+            k_sell_to_f32<<<vec_blocks((uint64_t)A.nslices * kWave), kBlock, 0, st>>>(v, A.sell_val.p, A.sell_val32.p);
         RS_HIP(hipGetLastError());
         A.sell32_valid = true;
     }

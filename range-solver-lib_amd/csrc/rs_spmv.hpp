@@ -46,6 +46,35 @@ struct CsrView {
 
 constexpr uint32_t kSellPad = 0xFFFFFFFFu;
 
+// The compact SELL arrays (FP32 values, int16 column offsets, and the learned smoother's packed
+// FP16 copy) store the entries of a slice in pairs: entries 2p and 2p+1 of a lane side by side at
+// base + 128 p + 2 lane, an odd last entry in a plain 64-wide row after the pairs.  A lane then
+// loads two entries of its row with one 8-byte (FP32 values, packed FP16) or 4-byte (offsets)
+// load instead of two 4- or 2-byte loads.  FP64 values and 32-bit columns keep the plain slots
+// base + 64 j + lane, where a lane's loads are already 8 or 4 bytes wide.
+__host__ __device__ __forceinline__ uint32_t sell_pair_slot(uint32_t base, uint32_t width, uint32_t j, uint32_t lane)
+{
+    return j < (width & ~1u) ? base + (j >> 1) * 2 * kWave + 2 * lane + (j & 1u) : base + j * kWave + lane;
+}
+
+typedef float sell_f2 __attribute__((ext_vector_type(2)));
+typedef uint32_t sell_u2 __attribute__((ext_vector_type(2)));
+
+// Values of entries 2p and 2p+1 of a lane's row in a slice starting at b.
+__device__ __forceinline__ void sell_two(const double *__restrict__ v, uint32_t b, uint32_t p, uint32_t lane, double &a0,
+                                         double &a1)
+{
+    a0 = __builtin_nontemporal_load(&v[b + 2 * p * kWave + lane]);
+    a1 = __builtin_nontemporal_load(&v[b + (2 * p + 1) * kWave + lane]);
+}
+__device__ __forceinline__ void sell_two(const float *__restrict__ v, uint32_t b, uint32_t p, uint32_t lane, float &a0,
+                                         float &a1)
+{
+    const sell_f2 t = __builtin_nontemporal_load(reinterpret_cast<const sell_f2 *>(&v[b + p * 2 * kWave + 2 * lane]));
+    a0 = t.x;
+    a1 = t.y;
+}
+
 template <int G>
 struct VecRows {
     static constexpr bool kTile = false;
@@ -139,12 +168,14 @@ struct TileRows {
 __device__ __forceinline__ const double *sell_values(const CsrView &A, const double *) { return A.sell_val; }
 __device__ __forceinline__ const float *sell_values(const CsrView &A, const float *) { return A.sell_val32; }
 
-// U entries of a row per loop trip, all loads of a trip independent (U = 4 or 8: the autotuner
-// keeps the faster; deeper unrolling keeps more loads in flight per wave).
+// U entries of a row per loop trip (U / 2 pairs), all loads of a trip independent (U = 4 or 8:
+// the autotuner keeps the faster; deeper unrolling keeps more loads in flight per wave).  The
+// entries are summed in ascending order, as the CSR row is.
 template <int U>
 struct SellRowsU {
     static constexpr bool kTile = false;
     static constexpr int kParam = 64;
+    static_assert(U % 2 == 0, "entries come in pairs");
     template <typename AT, typename VT, typename XT, class Epi>
     __device__ __forceinline__ static void run(const CsrView &A, const VT *__restrict__ csr_val,
                                                const XT *__restrict__ x, Epi &epi)
@@ -154,30 +185,62 @@ struct SellRowsU {
         const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
         const uint32_t s0 = lb * A.spb, s1 = min(A.nslices, s0 + A.spb);
         const uint32_t *__restrict__ col = A.sell_col;
+        // the matrix streams through once per product: non-temporal loads keep the L2 for
+        // the vectors (measured: FP64 SpMV 93.6 -> 78.7 us at 128^3, and the FP32 products
+        // inside the solvers faster although a lone FP32 SpMV is ~5% slower)
+        auto ld = [](const auto *p) { return __builtin_nontemporal_load(p); };
         for (uint32_t s = s0 + wid; s < s1; s += kBlock / kWave) {
-            const uint32_t b = A.sell_ptr[s], e = A.sell_ptr[s + 1];
+            const uint32_t b = A.sell_ptr[s], e = A.sell_ptr[s + 1], width = (e - b) / kWave, np = width >> 1;
             AT acc = 0;
-            uint32_t j = b + lane;
-            // the matrix streams through once per product: non-temporal loads keep the L2 for
-            // the vectors (measured: FP64 SpMV 93.6 -> 78.7 us at 128^3, and the FP32 products
-            // inside the solvers faster although a lone FP32 SpMV is ~5% slower)
-            auto ld = [](const auto *p) { return __builtin_nontemporal_load(p); };
-            for (; j + (U - 1) * kWave < e; j += U * kWave) {
+            if constexpr (sizeof(VT) == 8) {
+                // FP64 values in the plain slots: U entries per trip by one running index (this
+                // form measured 71.7 us at 128^3 against 80.0 us for the pair-indexed loop below)
+                uint32_t j = b + lane;
+                for (; j + (U - 1) * kWave < e; j += U * kWave) {
+                    uint32_t c[U];
+                    VT v[U];
+                    XT y[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) c[u] = ld(&col[j + u * kWave]);
+#pragma unroll
+                    for (int u = 0; u < U; u++) v[u] = ld(&val[j + u * kWave]);
+#pragma unroll
+                    for (int u = 0; u < U; u++) y[u] = c[u] != kSellPad ? x[c[u]] : XT(0);
+#pragma unroll
+                    for (int u = 0; u < U; u++) acc += (AT)v[u] * (AT)y[u];
+                }
+                for (; j < e; j += kWave) {
+                    const uint32_t cc = col[j];
+                    if (cc != kSellPad) acc += (AT)val[j] * (AT)x[cc];
+                }
+                const uint32_t r = s * kWave + lane;
+                if (r < A.n) epi.row(r, acc);
+                continue;
+            }
+            uint32_t p = 0;
+            for (; p + U / 2 <= np; p += U / 2) {
                 uint32_t c[U];
                 VT v[U];
                 XT y[U];
 #pragma unroll
-                for (int u = 0; u < U; u++) c[u] = ld(&col[j + u * kWave]);
+                for (int u = 0; u < U; u++) c[u] = ld(&col[b + (2 * p + u) * kWave + lane]);
 #pragma unroll
-                for (int u = 0; u < U; u++) v[u] = ld(&val[j + u * kWave]);
+                for (int q = 0; q < U / 2; q++) sell_two(val, b, p + q, lane, v[2 * q], v[2 * q + 1]);
 #pragma unroll
                 for (int u = 0; u < U; u++) y[u] = c[u] != kSellPad ? x[c[u]] : XT(0);
 #pragma unroll
                 for (int u = 0; u < U; u++) acc += (AT)v[u] * (AT)y[u];
             }
-            for (; j < e; j += kWave) {
-                uint32_t cc = col[j];
-                if (cc != kSellPad) acc += (AT)val[j] * (AT)x[cc];
+            for (; p < np; p++) {
+                const uint32_t c0 = col[b + 2 * p * kWave + lane], c1 = col[b + (2 * p + 1) * kWave + lane];
+                VT v0, v1;
+                sell_two(val, b, p, lane, v0, v1);
+                if (c0 != kSellPad) acc += (AT)v0 * (AT)x[c0];
+                if (c1 != kSellPad) acc += (AT)v1 * (AT)x[c1];
+            }
+            if (width & 1u) {
+                const uint32_t k = b + (width - 1) * kWave + lane, cc = col[k];
+                if (cc != kSellPad) acc += (AT)val[k] * (AT)x[cc];
             }
             const uint32_t r = s * kWave + lane;
             if (r < A.n) epi.row(r, acc);
@@ -190,6 +253,7 @@ template <int U>
 struct SellRowsOffU {
     static constexpr bool kTile = false;
     static constexpr int kParam = 64;
+    static_assert(U % 2 == 0, "entries come in pairs");
     template <typename AT, typename VT, typename XT, class Epi>
     __device__ __forceinline__ static void run(const CsrView &A, const VT *__restrict__ csr_val,
                                                const XT *__restrict__ x, Epi &epi)
@@ -204,26 +268,42 @@ struct SellRowsOffU {
         const uint32_t lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
         const uint32_t s0 = lb * A.spb, s1 = min(A.nslices, s0 + A.spb);
         const int16_t *__restrict__ off = A.sell_off;
-        auto ld = [](const auto *p) { return __builtin_nontemporal_load(p); };
+        // the offsets of entries 2p and 2p+1 as one 32-bit load (low half: entry 2p)
+        auto ld2 = [&](uint32_t b, uint32_t p) {
+            return __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(&off[b + p * 2 * kWave + 2 * lane]));
+        };
         for (uint32_t s = s0 + wid; s < s1; s += kBlock / kWave) {
-            const uint32_t b = A.sell_ptr[s], e = A.sell_ptr[s + 1];
+            const uint32_t b = A.sell_ptr[s], width = (A.sell_ptr[s + 1] - b) / kWave, np = width >> 1;
             const int32_t sb = (int32_t)(s * kWave);
             AT acc = 0;
-            uint32_t j = b + lane;
-            for (; j + (U - 1) * kWave < e; j += U * kWave) {
-                int32_t c[U];
+            uint32_t p = 0;
+            for (; p + U / 2 <= np; p += U / 2) {
+                uint32_t o[U / 2];
                 VT v[U];
                 XT y[U];
 #pragma unroll
-                for (int u = 0; u < U; u++) c[u] = sb + ld(&off[j + u * kWave]);
+                for (int q = 0; q < U / 2; q++) o[q] = ld2(b, p + q);
 #pragma unroll
-                for (int u = 0; u < U; u++) v[u] = ld(&val[j + u * kWave]);
+                for (int q = 0; q < U / 2; q++) sell_two(val, b, p + q, lane, v[2 * q], v[2 * q + 1]);
 #pragma unroll
-                for (int u = 0; u < U; u++) y[u] = x[c[u]];
+                for (int q = 0; q < U / 2; q++) {
+                    y[2 * q] = x[sb + (int32_t)(int16_t)(o[q] & 0xffffu)];
+                    y[2 * q + 1] = x[sb + (int32_t)(int16_t)(o[q] >> 16)];
+                }
 #pragma unroll
                 for (int u = 0; u < U; u++) acc += (AT)v[u] * (AT)y[u];
             }
-            for (; j < e; j += kWave) acc += (AT)ld(&val[j]) * (AT)x[sb + ld(&off[j])];
+            for (; p < np; p++) {
+                const uint32_t o = ld2(b, p);
+                VT v0, v1;
+                sell_two(val, b, p, lane, v0, v1);
+                acc += (AT)v0 * (AT)x[sb + (int32_t)(int16_t)(o & 0xffffu)];
+                acc += (AT)v1 * (AT)x[sb + (int32_t)(int16_t)(o >> 16)];
+            }
+            if (width & 1u) {
+                const uint32_t k = b + (width - 1) * kWave + lane;
+                acc += (AT)val[k] * (AT)x[sb + off[k]];
+            }
             const uint32_t r = s * kWave + lane;
             if (r < A.n) epi.row(r, acc);
         }
