@@ -292,10 +292,18 @@ __global__ __launch_bounds__(kBlock) void k_cyc_resid(CycArgs a)
 }
 
 // ---------------------------------------------------------------------------- compressed smoother copy
-// Row-scaled copy S = D^-1 A of a level matrix in its SELL-64 pair slots, FP16 values and int16
-// column offsets (col - row) packed into one uint32 per slot: 4 bytes per entry instead of 6 for
-// the FP32 SELL copy with offsets.  The smoother only needs D^-1 A (and D for the residual); the Krylov
-// operator stays FP32.  *bad is set when an offset does not fit in 16 bits or a diagonal is zero.
+// Row-scaled copy S = D^-1 A of a level matrix in the SELL-64 slices, FP16 values and int16
+// column offsets (col - row) packed into one uint32 per entry: 4 bytes per entry instead of 6 for
+// the FP32 SELL copy with offsets.  The entries are stored in quads: entries 4q .. 4q+3 of a lane
+// side by side at base + 256 q + 4 lane, the last width % 4 in plain 64-wide rows after the
+// quads, so that a lane loads four entries of its row with one 16-byte load.  The smoother only
+// needs D^-1 A (and D for the residual); the Krylov operator stays FP32.
+__host__ __device__ __forceinline__ uint32_t s16_slot(uint32_t base, uint32_t width, uint32_t j, uint32_t lane)
+{
+    return j < (width & ~3u) ? base + (j >> 2) * 4 * kWave + 4 * lane + (j & 3u) : base + j * kWave + lane;
+}
+
+// *bad is set when an offset does not fit in 16 bits or a diagonal is zero.
 __global__ void k_s16_build(CsrView A, uint32_t *__restrict__ ent, uint32_t *__restrict__ bad)
 {
     // built from the FP64 SELL slots, so that the 64 lanes of a wavefront (64 consecutive rows)
@@ -323,10 +331,12 @@ __global__ void k_s16_build(CsrView A, uint32_t *__restrict__ ent, uint32_t *__r
                 const _Float16 h = (_Float16)(d != 0.0 ? A.sell_val[k] / d : 0.0);
                 e = (uint32_t)(uint16_t)(int16_t)off | ((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
             }
-            ent[sell_pair_slot(base, width, j, lane)] = e;
+            ent[s16_slot(base, width, j, lane)] = e;
         }
     }
 }
+
+typedef uint32_t s16_u4 __attribute__((ext_vector_type(4)));
 
 struct S16View {
     uint32_t n, nslices, spb;
@@ -342,7 +352,7 @@ __device__ __forceinline__ void s16_run(const S16View &S, const float *__restric
     const uint32_t s0 = lb * S.spb, s1 = min(S.nslices, s0 + S.spb);
     const uint32_t *__restrict__ ent = S.ent;
     for (uint32_t s = s0 + wid; s < s1; s += kBlock / kWave) {
-        const uint32_t b = S.sell_ptr[s], width = (S.sell_ptr[s + 1] - b) / kWave, np = width >> 1;
+        const uint32_t b = S.sell_ptr[s], width = (S.sell_ptr[s + 1] - b) / kWave, nq = width >> 2;
         const uint32_t r = s * kWave + lane, rr = min(r, S.n - 1);   // padding lanes read row n-1
         float acc = 0.0f;
         auto term = [&](uint32_t w) {
@@ -350,24 +360,30 @@ __device__ __forceinline__ void s16_run(const S16View &S, const float *__restric
             const float v = (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
             return v * x[(int32_t)rr + off];
         };
-        // entries 2p and 2p+1 as one 8-byte load, two pairs per trip, summed in entry order
-        auto pair = [&](uint32_t p) {
-            return __builtin_nontemporal_load(reinterpret_cast<const sell_u2 *>(&ent[b + p * 2 * kWave + 2 * lane]));
+        // entries 4q .. 4q+3 as one 16-byte load, two quads per trip, summed in entry order
+        auto quad = [&](uint32_t q) {
+            return __builtin_nontemporal_load(reinterpret_cast<const s16_u4 *>(&ent[b + q * 4 * kWave + 4 * lane]));
         };
-        uint32_t p = 0;
-        for (; p + 2 <= np; p += 2) {
-            const sell_u2 w0 = pair(p), w1 = pair(p + 1);
+        uint32_t q = 0;
+        for (; q + 2 <= nq; q += 2) {
+            const s16_u4 w0 = quad(q), w1 = quad(q + 1);
             acc += term(w0.x);
             acc += term(w0.y);
+            acc += term(w0.z);
+            acc += term(w0.w);
             acc += term(w1.x);
             acc += term(w1.y);
+            acc += term(w1.z);
+            acc += term(w1.w);
         }
-        for (; p < np; p++) {
-            const sell_u2 w0 = pair(p);
+        for (; q < nq; q++) {
+            const s16_u4 w0 = quad(q);
             acc += term(w0.x);
             acc += term(w0.y);
+            acc += term(w0.z);
+            acc += term(w0.w);
         }
-        if (width & 1u) acc += term(ent[b + (width - 1) * kWave + lane]);
+        for (uint32_t j = nq * 4; j < width; j++) acc += term(ent[b + j * kWave + lane]);
         if (r < S.n) epi.row(r, acc);
     }
 }
