@@ -306,3 +306,42 @@ def test_spmv_shapes_read_no_entry_past_the_vector(gpu):
         rs._check(lib.rs_matrix_force_plan(M.h, rs.PRECISION_FP64, -1))
     finally:
         dx.ptr = None      # the alias does not own the allocation
+
+
+@pytest.mark.parametrize("width", [1, 2, 3, 5])
+def test_spmv_shapes_short_rows(gpu, width):
+    """Banded matrices whose SELL slices are 1, 2, 3 or 5 entries wide: the paired and
+    quadrupled slot layouts of the compact arrays and their odd tails, on every shape."""
+    _say(f"spmv shapes width {width}")
+    rs = gpu
+    rng = np.random.default_rng(30 + width)
+    n = 64 * 7 + 13
+    offs = list(range(-(width // 2), width - width // 2))
+    A = sp.diags([rng.uniform(0.5, 1.5, n - abs(o)) * (2.0 * width if o == 0 else -1.0) for o in offs], offs,
+                 shape=(n, n), format="csr")
+    if width % 2:
+        A = ((A + A.T) / 2).tocsr()        # symmetric and diagonally dominant: SPD for the CG part
+    A.sort_indices()
+    M = rs.SparseMatrix.from_scipy(A)
+    x = rng.standard_normal(n)
+    ref = A @ x
+    dx, dy = rs.DeviceArray.from_host(x), rs.DeviceArray(n)
+    lib = rs.load()
+    for plan in range(rs.SPMV_SHAPES):
+        rs._check(lib.rs_matrix_force_plan(M.h, rs.PRECISION_FP64, plan))
+        M.spmv_device(dx, dy)
+        np.testing.assert_allclose(dy.download(), ref, rtol=1e-12, atol=1e-12 * abs(ref).max(),
+                                   err_msg=rs.SPMV_SHAPE_NAMES[plan])
+    rs._check(lib.rs_matrix_force_plan(M.h, rs.PRECISION_FP64, -1))
+    if width % 2 == 0:
+        return
+    # every FP32 shape inside the mixed-precision CG
+    b = A @ np.ones(n)
+    for plan in range(rs.SPMV_SHAPES):
+        rs._check(lib.rs_matrix_force_plan(M.h, rs.PRECISION_MIXED, plan))
+        S = rs.RMatrixSolver(rs.MatrixSolverConf(n_outer_iterations=5000, solver_cvg_value=1e-10,
+                                                 stop_rule=rs.STOP_RELRES, precision=rs.PRECISION_MIXED))
+        xs = S.solve(M, b, None, rs.PRECOND_JACOBI, 1)
+        assert S.info.converged == 1, (rs.SPMV_SHAPE_NAMES[plan], S.info.as_dict())
+        np.testing.assert_allclose(xs, np.ones(n), rtol=0, atol=1e-7, err_msg=rs.SPMV_SHAPE_NAMES[plan])
+    rs._check(lib.rs_matrix_force_plan(M.h, rs.PRECISION_MIXED, -1))
