@@ -364,6 +364,29 @@ __device__ __forceinline__ void s16_run(const S16View &S, const float *__restric
         auto quad = [&](uint32_t q) {
             return __builtin_nontemporal_load(reinterpret_cast<const s16_u4 *>(&ent[b + q * 4 * kWave + 4 * lane]));
         };
+        if (width <= 16) {
+            // the whole row in one round of loads (mesh rows: ~15 entries), summed in entry order
+            s16_u4 w[4];
+            uint32_t t[3];
+            const uint32_t tail = width - 4 * nq;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) w[k] = k < nq ? quad(k) : s16_u4{ 0u, 0u, 0u, 0u };
+#pragma unroll
+            for (uint32_t k = 0; k < 3; k++) t[k] = k < tail ? ent[b + (4 * nq + k) * kWave + lane] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++)
+                if (k < nq) {
+                    acc += term(w[k].x);
+                    acc += term(w[k].y);
+                    acc += term(w[k].z);
+                    acc += term(w[k].w);
+                }
+#pragma unroll
+            for (uint32_t k = 0; k < 3; k++)
+                if (k < tail) acc += term(t[k]);
+            if (r < S.n) epi.row(r, acc);
+            continue;
+        }
         uint32_t q = 0;
         for (; q + 2 <= nq; q += 2) {
             const s16_u4 w0 = quad(q), w1 = quad(q + 1);
