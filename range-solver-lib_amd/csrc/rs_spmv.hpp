@@ -46,12 +46,12 @@ struct CsrView {
 
 constexpr uint32_t kSellPad = 0xFFFFFFFFu;
 
-// The compact SELL arrays (FP32 values, int16 column offsets, and the learned smoother's packed
-// FP16 copy) store the entries of a slice in pairs: entries 2p and 2p+1 of a lane side by side at
-// base + 128 p + 2 lane, an odd last entry in a plain 64-wide row after the pairs.  A lane then
-// loads two entries of its row with one 8-byte (FP32 values, packed FP16) or 4-byte (offsets)
-// load instead of two 4- or 2-byte loads.  FP64 values and 32-bit columns keep the plain slots
-// base + 64 j + lane, where a lane's loads are already 8 or 4 bytes wide.
+// The compact SELL arrays (FP32 values and int16 column offsets) store the entries of a slice in
+// pairs: entries 2p and 2p+1 of a lane side by side at base + 128 p + 2 lane, an odd last entry
+// in a plain 64-wide row after the pairs.  A lane then loads two entries of its row with one
+// 8-byte (FP32 values) or 4-byte (offsets) load instead of two 4- or 2-byte loads.  FP64 values
+// and 32-bit columns keep the plain slots base + 64 j + lane, where a lane's loads are already 8
+// or 4 bytes wide.  (The learned smoother's packed FP16 copy has its own quad layout.)
 __host__ __device__ __forceinline__ uint32_t sell_pair_slot(uint32_t base, uint32_t width, uint32_t j, uint32_t lane)
 {
     return j < (width & ~1u) ? base + (j >> 1) * 2 * kWave + 2 * lane + (j & 1u) : base + j * kWave + lane;
