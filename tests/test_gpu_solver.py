@@ -345,3 +345,30 @@ def test_spmv_shapes_short_rows(gpu, width):
         assert S.info.converged == 1, (rs.SPMV_SHAPE_NAMES[plan], S.info.as_dict())
         np.testing.assert_allclose(xs, np.ones(n), rtol=0, atol=1e-7, err_msg=rs.SPMV_SHAPE_NAMES[plan])
     rs._check(lib.rs_matrix_force_plan(M.h, rs.PRECISION_MIXED, -1))
+
+
+@pytest.mark.parametrize("m", [7, 8, 9, 15, 16, 17, 24])
+def test_cg_stops_on_replay_boundaries(gpu, orc, m):
+    """CG without a preconditioner on a diagonal matrix with m distinct eigenvalues terminates
+    after m iterations (the Krylov space holds the solution): m on and around the 8-iteration
+    replays of the captured loop.  The GPU count equals the oracle's, in FP64 and in mixed
+    precision, so the replay scheduling neither adds nor drops iterations."""
+    _say(f"cg replay boundary m={m}")
+    rs = gpu
+    rng = np.random.default_rng(40 + m)
+    n = 64 * m + 5
+    d = (1.0 + np.arange(n) % m).astype(np.float64)
+    A = sp.diags(d).tocsr()
+    M = rs.SparseMatrix.from_scipy(A)
+    b = rng.uniform(0.5, 1.5, n)
+    x_ref, info_ref = orc.solve(A, b, precond=orc.PRECOND_NONE, cvg=1e-13, n_outer=1000)
+    assert abs(info_ref["iterations"] - m) <= 1
+    S = rs.RMatrixSolver(rs.MatrixSolverConf(n_outer_iterations=1000, solver_cvg_value=1e-13))
+    x = S.solve(M, b, None, rs.PRECOND_NONE, 1)
+    assert S.info.iterations == info_ref["iterations"], (S.info.as_dict(), info_ref)
+    np.testing.assert_allclose(x, b / d, rtol=1e-11)
+    Sm = rs.RMatrixSolver(rs.MatrixSolverConf(n_outer_iterations=1000, solver_cvg_value=1e-10,
+                                              stop_rule=rs.STOP_RELRES, precision=rs.PRECISION_MIXED))
+    xm = Sm.solve(M, b, None, rs.PRECOND_NONE, 1)
+    assert Sm.info.converged == 1 and Sm.info.true_relres <= 1e-10, Sm.info.as_dict()
+    np.testing.assert_allclose(xm, b / d, rtol=1e-9)
