@@ -153,6 +153,10 @@ int rs_precond_create(const rs_matrix *A, int32_t type, uint32_t block_size, rs_
 int rs_precond_compute(const rs_precond *P, const double *x, double *y);
 int rs_precond_width(const rs_precond *P, uint32_t *width);
 int rs_precond_download(const rs_precond *P, double *data);                 /* n*width values */
+/* RMatrixPreconditioner(const RMatrixPreconditioner&) and operator=, include/rmatrixpreconditioner.h:39,45
+ * (src/rmatrixpreconditioner.cpp:6-14,36-50): a new handle holding its own copy of the type, the
+ * block width and the device data; later changes to either handle do not reach the other. */
+int rs_precond_copy(const rs_precond *src, rs_precond **out);
 int rs_precond_destroy(rs_precond *P);
 
 /* ---------------------------------------------------------------- solver ------------- */

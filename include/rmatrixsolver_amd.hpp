@@ -9,7 +9,8 @@
 // namespace rsamd.  Semantics kept: solve() resizes x to n with zeros and uses its content as
 // the initial guess; unknown solver / preconditioner types throw RError; non-convergence does
 // not throw; the flat CSR of a matrix is cached per matrix object and refreshed when the
-// pattern is unchanged (src/rmatrixsolver.cpp:56-109).
+// pattern is unchanged (src/rmatrixsolver.cpp:56-109), up to 16 objects (least recently used
+// freed first; releaseDeviceMatrix frees one on request).
 #pragma once
 
 #include <algorithm>
@@ -141,18 +142,44 @@ namespace detail {
 struct CsrCache {
     rs_matrix *h = nullptr;
     std::vector<uint32_t> rowptr, col;
+    uint64_t last_use = 0;
     ~CsrCache()
     {
         if (h) rs_matrix_destroy(h);
     }
 };
 
+// The reference keeps one flat CSR per matrix object for the life of the process
+// (src/rmatrixsolver.cpp:99-108).  Here the device copies are bounded: past kMaxCachedMatrices
+// objects the least recently used copy is freed (a later solve with that object uploads it
+// again), and releaseDeviceMatrix / releaseDeviceMatrices free them on request.
+constexpr size_t kMaxCachedMatrices = 16;
+
+struct CsrCaches {
+    std::map<const RSparseMatrix *, std::unique_ptr<CsrCache>> map;
+    uint64_t clock = 0;
+};
+
+inline CsrCaches &csr_caches()
+{
+    static CsrCaches c;
+    return c;
+}
+
 inline rs_matrix *device_matrix(const RSparseMatrix &A)
 {
-    static std::map<const RSparseMatrix *, std::unique_ptr<CsrCache>> caches;
-    auto &slot = caches[&A];
+    CsrCaches &all = csr_caches();
+    auto &slot = all.map[&A];
     if (!slot) slot.reset(new CsrCache());
     CsrCache &c = *slot;
+    c.last_use = ++all.clock;
+    if (all.map.size() > kMaxCachedMatrices) {
+        auto victim = all.map.end();
+        for (auto it = all.map.begin(); it != all.map.end(); ++it)
+            if (it->first != &A && (victim == all.map.end() || it->second->last_use < victim->second->last_use))
+                victim = it;
+        if (victim != all.map.end()) all.map.erase(victim);
+    }
     unsigned n = A.getNRows();
     std::vector<uint32_t> rowptr(n + 1, 0), col;
     std::vector<double> val;
@@ -174,7 +201,14 @@ inline rs_matrix *device_matrix(const RSparseMatrix &A)
     c.col.swap(col);
     return c.h;
 }
+
+inline size_t cached_matrices() { return csr_caches().map.size(); }
 }  // namespace detail
+
+// Free the device copy kept for matrix object A (for example before A is destroyed).
+inline void releaseDeviceMatrix(const RSparseMatrix &A) { detail::csr_caches().map.erase(&A); }
+// Free every device copy kept by the solvers of this header.
+inline void releaseDeviceMatrices() { detail::csr_caches().map.clear(); }
 
 // RMatrixPreconditioner (include/rmatrixpreconditioner.h:36-48)
 class RMatrixPreconditioner {
@@ -185,8 +219,20 @@ public:
         check(rs_precond_create(detail::device_matrix(matrix), (int32_t)type, blockSize, &h_));
         n_ = matrix.getNRows();
     }
-    RMatrixPreconditioner(const RMatrixPreconditioner &) = delete;
-    RMatrixPreconditioner &operator=(const RMatrixPreconditioner &) = delete;
+    // copy and assignment copy the type and the preconditioner values, as the reference's
+    // _init does (src/rmatrixpreconditioner.cpp:6-14)
+    RMatrixPreconditioner(const RMatrixPreconditioner &o) : n_(o.n_) { check(rs_precond_copy(o.h_, &h_)); }
+    RMatrixPreconditioner &operator=(const RMatrixPreconditioner &o)
+    {
+        if (this != &o) {
+            rs_precond *h = nullptr;
+            check(rs_precond_copy(o.h_, &h));
+            rs_precond_destroy(h_);
+            h_ = h;
+            n_ = o.n_;
+        }
+        return *this;
+    }
     ~RMatrixPreconditioner() { rs_precond_destroy(h_); }
     void compute(const RRVector &x, RRVector &y) const
     {

@@ -127,7 +127,7 @@ _SIGS = {
     "rs_matrix_force_plan": [_P, C.c_int32, C.c_int32],
     "rs_precond_create": [_P, C.c_int32, C.c_uint32, C.POINTER(_P)],
     "rs_precond_compute": [_P, _P, _P], "rs_precond_width": [_P, _u32p],
-    "rs_precond_download": [_P, _P], "rs_precond_destroy": [_P],
+    "rs_precond_download": [_P, _P], "rs_precond_copy": [_P, C.POINTER(_P)], "rs_precond_destroy": [_P],
     "rs_solver_create": [C.POINTER(Conf), C.POINTER(_P)], "rs_solver_copy": [_P, C.POINTER(_P)],
     "rs_solver_disable_convergence_log": [_P],
     "rs_solver_solve": [_P, _P, _P, _P, C.c_int32, C.c_uint32, C.POINTER(Info)],
@@ -247,6 +247,18 @@ def _u32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
+def _need(name: str, size: int, want: int) -> None:
+    """Arrays cross the C ABI as bare pointers: a wrong length is refused here, before the
+    library reads or writes past the end of the caller's buffer."""
+    if size != want:
+        raise ValueError(f"{name} has {size} entries, expected {want}")
+
+
+def _need_dev(name: str, d: "DeviceArray", want: int, dtype=np.float64) -> None:
+    if d.dtype != np.dtype(dtype) or d.n < want:
+        raise ValueError(f"{name} holds {d.n} {d.dtype} values, expected at least {want} {np.dtype(dtype)}")
+
+
 # ------------------------------------------------------------------------------ runtime
 def init(device: int = 0) -> None:
     load()
@@ -310,6 +322,10 @@ class SparseMatrix:
         else:
             self._borrowed = False
             self.rowptr, self.col, self.val = _u32(rowptr), _u32(col), _f64(val)
+            _need("rowptr", self.rowptr.size, int(n) + 1)
+            nnz = int(self.rowptr[-1])
+            _need("col", self.col.size, nnz)
+            _need("val", self.val.size, nnz)
             self.h = C.c_void_p()
             _check(_lib.rs_matrix_create(n, _ptr(self.rowptr), _ptr(self.col), _ptr(self.val),
                                          C.byref(self.h)))
@@ -334,6 +350,7 @@ class SparseMatrix:
 
     def update_values(self, val) -> None:
         v = _f64(val)
+        _need("val", v.size, self.info()[1])
         _check(_lib.rs_matrix_update_values(self.h, _ptr(v)))
 
     def download(self):
@@ -352,11 +369,14 @@ class SparseMatrix:
 
     def spmv(self, x) -> np.ndarray:
         x = _f64(x)
+        _need("x", x.size, self.n)
         y = np.empty_like(x)
         _check(_lib.rs_matrix_spmv(self.h, _ptr(x), _ptr(y)))
         return y
 
     def spmv_device(self, dx: DeviceArray, dy: DeviceArray) -> None:
+        _need_dev("dx", dx, self.n)
+        _need_dev("dy", dy, self.n)
         _check(_lib.rs_matrix_spmv_device(self.h, dx.ptr, dy.ptr))
 
     def spmv_timed(self, precision: int = PRECISION_FP64, reps: int = 50, plan: int = -1) -> float:
@@ -364,6 +384,10 @@ class SparseMatrix:
         ms = C.c_double()
         _check(_lib.rs_matrix_spmv_timed(self.h, precision, reps, plan, C.byref(ms)))
         return ms.value
+
+    def force_plan(self, precision: int, plan: int) -> None:
+        """Override the autotuner's SpMV shape for `precision` (plan -1: tune again)."""
+        _check(_lib.rs_matrix_force_plan(self.h, precision, plan))
 
     def spmv_plan(self, precision: int = PRECISION_FP64):
         """(autotuned shape id, tuning ms per shape)."""
@@ -397,8 +421,17 @@ class RMatrixPreconditioner:
         _check(_lib.rs_precond_download(self.h, _ptr(out)))
         return out
 
+    def copy(self) -> "RMatrixPreconditioner":
+        """RMatrixPreconditioner(const RMatrixPreconditioner&): its own copy of type and values."""
+        other = RMatrixPreconditioner.__new__(RMatrixPreconditioner)
+        other.A = self.A
+        other.h = C.c_void_p()
+        _check(_lib.rs_precond_copy(self.h, C.byref(other.h)))
+        return other
+
     def compute(self, x) -> np.ndarray:
         x = _f64(x)
+        _need("x", x.size, self.A.n)
         y = np.empty_like(x)
         _check(_lib.rs_precond_compute(self.h, _ptr(x), _ptr(y)))
         return y
@@ -453,6 +486,7 @@ class RMatrixSolver:
               block_size: int = 1) -> np.ndarray:
         b = _f64(b)
         n = A.n
+        _need("b", b.size, n)
         # the reference resizes x to n, keeping the leading entries as the initial guess
         x0 = np.zeros(n) if x is None else _f64(x)
         if x0.size != n:
@@ -466,6 +500,9 @@ class RMatrixSolver:
 
     def solve_device(self, A: SparseMatrix, db: DeviceArray, dx: DeviceArray,
                      precond: int = PRECOND_NONE, block_size: int = 1) -> Info:
+        n = A.n
+        _need_dev("db", db, n)
+        _need_dev("dx", dx, n)
         info = Info()
         _check(_lib.rs_solver_solve_device(self.h, A.h, db.ptr, dx.ptr, precond, block_size,
                                            C.byref(info)))
@@ -475,6 +512,9 @@ class RMatrixSolver:
     def solve_learned_device(self, A: SparseMatrix, L: "LearnedPreconditioner", db: DeviceArray,
                              dx: DeviceArray) -> Info:
         """Flexible CG with the learned two-level preconditioner (mixed precision)."""
+        n = A.n
+        _need_dev("db", db, n)
+        _need_dev("dx", dx, n)
         info = Info()
         _check(_lib.rs_solver_solve_learned_device(self.h, A.h, L.h, db.ptr, dx.ptr, C.byref(info)))
         self.info = info

@@ -1137,23 +1137,20 @@ struct rs_assembly {
     double last_ms = 0.0;
 };
 
-static int timed_begin(hipEvent_t (&ev)[2])
+static int timed_begin(EventPair &ev)
 {
-    RS_HIP(hipEventCreate(&ev[0]));
-    RS_HIP(hipEventCreate(&ev[1]));
-    RS_HIP(hipEventRecord(ev[0], runtime().stream));
+    RS_HIP(ev.create());
+    RS_HIP(hipEventRecord(ev.e[0], runtime().stream));
     return RS_OK;
 }
 
-static int timed_end(hipEvent_t (&ev)[2], double &ms)
+static int timed_end(EventPair &ev, double &ms)
 {
-    RS_HIP(hipEventRecord(ev[1], runtime().stream));
-    RS_HIP(hipEventSynchronize(ev[1]));
+    RS_HIP(hipEventRecord(ev.e[1], runtime().stream));
+    RS_HIP(hipEventSynchronize(ev.e[1]));
     float f = 0.0f;
-    RS_HIP(hipEventElapsedTime(&f, ev[0], ev[1]));
+    RS_HIP(hipEventElapsedTime(&f, ev.e[0], ev.e[1]));
     ms = f;
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
     return RS_OK;
 }
 
@@ -1355,7 +1352,7 @@ extern "C" int rs_assemble_heat_device(rs_assembly *S, const double *d_k, const 
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    hipEvent_t ev[2];
+    EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
     HeatArgs a;
     a.xyz = S->xyz.p;
@@ -1490,7 +1487,7 @@ static int stress_assemble(rs_assembly *S, const rs_stress_conf &conf, const dou
         RS_HIP(hipMemcpyAsync(S->tris.p, fsorted.data(), sizeof(uint32_t) * 3 * n_tri, hipMemcpyHostToDevice, st));
         RS_HIP(hipMemcpyAsync(S->trac.p, tsorted.data(), sizeof(double) * 3 * n_tri, hipMemcpyHostToDevice, st));
     }
-    hipEvent_t ev[2];
+    EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
     StressArgs a;
     a.xyz = S->xyz.p;
@@ -1654,7 +1651,7 @@ extern "C" int rs_heat_flux_device(rs_assembly *S, const double *d_k, const doub
     if (!S || !d_k || !d_t_node || !d_flux) return fail(RS_ERR_INVALID_ARG, "rs_heat_flux_device: null argument");
     int rc = require_runtime();
     if (rc) return rc;
-    hipEvent_t ev[2];
+    EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
     if ((rc = ensure_conn0(S))) return rc;
     if (S->n_elems)
@@ -1672,7 +1669,7 @@ extern "C" int rs_stress_process_device(rs_assembly *S, const double *d_E, const
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    hipEvent_t ev[2];
+    EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
     k_zero<<<vblocks(3ull * S->n_nodes), kBlock, 0, st>>>(d_node_force, 3ull * S->n_nodes);
     StressPostArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p), S->perm.p, d_E, d_nu, d_u_node,
@@ -1721,7 +1718,7 @@ extern "C" int rs_electrostatics_process_device(rs_assembly *S, const double *d_
         return fail(RS_ERR_INVALID_ARG, "rs_electrostatics_process_device: null argument");
     int rc = require_runtime();
     if (rc) return rc;
-    hipEvent_t ev[2];
+    EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
     if ((rc = ensure_conn0(S))) return rc;
     ElectroPostArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn0.p), S->n_elems, d_eps_r, d_sigma,
@@ -1740,7 +1737,7 @@ extern "C" int rs_assemble_magnetostatics_device(rs_assembly *S, const double *d
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    hipEvent_t ev[2];
+    EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
     if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
     if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
@@ -1771,7 +1768,7 @@ extern "C" int rs_assemble_fluid_device(rs_assembly *S, const rs_fluid_conf *con
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    hipEvent_t ev[2];
+    EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
     if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
     if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
@@ -1833,7 +1830,7 @@ extern "C" int rs_assemble_transport_device(rs_assembly *S, const rs_transport_c
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    hipEvent_t ev[2];
+    EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
     if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
     if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);

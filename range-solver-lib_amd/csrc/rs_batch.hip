@@ -804,9 +804,9 @@ int bt_tune(rs_batch &B, bool mixed)
     else k_bt_fill<<<bt_vec_grid(B.nslices), kBlock, 0, st>>>(B.p.p, rows, 1.0);
     RS_HIP(hipGetLastError());
     const uint32_t reps = B.sell_entries > 64000000ull ? 3 : 10;
-    hipEvent_t e0, e1;
-    RS_HIP(hipEventCreate(&e0));
-    RS_HIP(hipEventCreate(&e1));
+    EventPair e0e1_;
+    RS_HIP(e0e1_.create());
+    hipEvent_t e0 = e0e1_.e[0], e1 = e0e1_.e[1];
     double best = 1e30;
     int best_id = 0;
     for (int id = 0; id < kBatchShapes; id++) {
@@ -824,8 +824,6 @@ int bt_tune(rs_batch &B, bool mixed)
             best_id = id;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     B.shape[slot] = best_id;
     return RS_OK;
 }
@@ -1053,9 +1051,9 @@ extern "C" int rs_batch_solve(rs_batch *B, const rs_conf *conf, uint32_t steps, 
         RS_HIP(hipEventCreateWithFlags(&B->ev[0], hipEventDisableTiming));
         RS_HIP(hipEventCreateWithFlags(&B->ev[1], hipEventDisableTiming));
     }
-    hipEvent_t t0, t1;
-    RS_HIP(hipEventCreate(&t0));
-    RS_HIP(hipEventCreate(&t1));
+    EventPair t0t1_;
+    RS_HIP(t0t1_.create());
+    hipEvent_t t0 = t0t1_.e[0], t1 = t0t1_.e[1];
     RS_HIP(hipEventRecord(t0, st));
     k_bt_init<<<(B->count + kBlock - 1) / kBlock, kBlock, 0, st>>>(a, 0);
     RS_HIP(hipGetLastError());
@@ -1097,8 +1095,6 @@ extern "C" int rs_batch_solve(rs_batch *B, const rs_conf *conf, uint32_t steps, 
     if (rc) return rc;
     float ms = 0.0f;
     RS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     // summary
     std::vector<BtSys> S(B->count);
     std::vector<uint32_t> its((uint64_t)B->count * steps);

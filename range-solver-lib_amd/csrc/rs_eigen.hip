@@ -299,9 +299,9 @@ extern "C" int rs_eigen_solve(const rs_eigen_conf *ec, const rs_conf *mc, rs_mat
     if ((rc = solver.create(mc))) return rc;
     std::vector<double> h(n);
     const uint32_t grid = ev_blocks(n);
-    hipEvent_t t0, t1;
-    RS_HIP(hipEventCreate(&t0));
-    RS_HIP(hipEventCreate(&t1));
+    EventPair t0t1_;
+    RS_HIP(t0t1_.create());
+    hipEvent_t t0 = t0t1_.e[0], t1 = t0t1_.e[1];
     RS_HIP(hipEventRecord(t0, st));
     std::vector<double> dv;             // eigenvalues before the post-processing
     std::vector<double> evh;            // rows of n
@@ -446,8 +446,6 @@ extern "C" int rs_eigen_solve(const rs_eigen_conf *ec, const rs_conf *mc, rs_mat
     RS_HIP(hipEventSynchronize(t1));
     float ms = 0.0f;
     RS_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     // REigenValueSolver::solve post-processing (src/reigenvaluesolver.cpp:89-116)
     if (dv.size() > 1) {
         for (double &x : dv) x = std::fabs(1.0 / x);

@@ -70,6 +70,24 @@ void DBuf<T>::release()
     n = 0;
 }
 
+// Two timing events, destroyed on every return path of the function that owns them.
+struct EventPair {
+    hipEvent_t e[2] = { nullptr, nullptr };
+    hipError_t create()
+    {
+        hipError_t r = hipEventCreate(&e[0]);
+        return r == hipSuccess ? hipEventCreate(&e[1]) : r;
+    }
+    ~EventPair()
+    {
+        for (hipEvent_t &x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    EventPair() = default;
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+};
+
 // ----------------------------------------------------------------- host preparation
 // Parallel loop over [0, n) on host threads.
 void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)> &fn);

@@ -1252,6 +1252,15 @@ static void launch_restrict(uint32_t G, uint32_t nb, const float *res, const Cyc
     }
 }
 
+// Workgroups of a Chebyshev step on level V: its partial-sum slot count when it is the finest
+// level's last step (the SELL grid for the compressed copy, else the level's FP32 plan grid).
+static uint32_t step_blocks(const CycLevel &V)
+{
+    uint32_t nb = 0;
+    plan_view(*V.M, V.use_s16 ? kSellPlan : V.M->plan[1], nb);
+    return nb;
+}
+
 // One Chebyshev step on level V (SpMV in the level's compressed copy or its FP32 plan).
 static void launch_step(const CycLevel &V, const CycArgs &a, hipStream_t s)
 {
@@ -1415,19 +1424,22 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
     const uint32_t nvb = vec_blocks(A.n);
     hipStream_t stream = runtime().stream;
     L.r0 = S.r32.p;
+    // r.z and z.q come from the finest level's last post-smoothing step, whose grid is not the
+    // FP32 SpMV plan's when that level smooths with its compressed copy
+    const uint32_t nbz = step_blocks(*L.lv[0]);
     k_fill32<<<nvb, kBlock, 0, stream>>>(S.dx32.p, A.n, 0.0f);
     k_fill32<<<nvb, kBlock, 0, stream>>>(S.q32.p, A.n, 0.0f);
     RS_SHAPE_SWITCH(plan64, k_mp_true_resid, nb64, stream, a64);
     fin(S.partials.p, nb64, 2, FIN_MP_TRUE, 1, S.st.p, S.hist.p);
     RS_HIP(hipGetLastError());
-    rc = capture(S, [&] {
+    const std::function<void()> body = [&] {
         for (uint32_t i = 0; i < kGraphIters; i++) {
             const float *z = cycle_level(L, 0, S.st.p, S.q32.p, S.partials.p, Prolong{});
             if (L.rec_cap) {
                 CycLevel &Cl = *L.lv.back();
                 k_record<<<vec_blocks(L.nc), kBlock, 0, stream>>>(Cl.r.p, L.rec.p, L.nc, L.rec_cap, S.st.p);
             }
-            fin(S.partials.p, nb32, 2, FIN_FCG_BETA, 0, S.st.p, S.hist.p);
+            fin(S.partials.p, nbz, 2, FIN_FCG_BETA, 0, S.st.p, S.hist.p);
             FcgArgs fz = f;
             fz.z = const_cast<float *>(z);
             k_fcg_p<<<nvb, kBlock, 0, stream>>>(fz);
@@ -1440,13 +1452,14 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
             RS_SHAPE_SWITCH(plan64, k_mp_true_resid_cond, nb64, stream, a64);
             fin(S.partials.p, nb64, 2, FIN_MP_REPLACE, 0, S.st.p, S.hist.p);
         }
-    }, graph_key_of(3, a, plan32, nb32, nb64, nvb) + learned_key(L));
+    };
+    rc = capture(S, body, graph_key_of(3, a, plan32, nb32, nb64, nvb) + learned_key(L) + std::to_string(nbz));
     if (rc) return rc;
     SolveState h;
     RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     RS_HIP(hipStreamSynchronize(stream));
     done = h.done;
-    if (done == kStopRunning && (rc = run_graph_loop(S, done))) return rc;
+    if (done == kStopRunning && (rc = run_graph_loop(S, done, body))) return rc;
     RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     RS_HIP(hipStreamSynchronize(stream));
     refinements = h.nrepl;
@@ -1850,9 +1863,9 @@ extern "C" int rs_learned_infer_timed(rs_learned *L, uint32_t reps, double *avg_
     hipStream_t st = runtime().stream;
     const GnnArgs g = gnn_args(*L, nullptr);
     launch_gnn(g, st);
-    hipEvent_t e0, e1;
-    RS_HIP(hipEventCreate(&e0));
-    RS_HIP(hipEventCreate(&e1));
+    EventPair e0e1_;
+    RS_HIP(e0e1_.create());
+    hipEvent_t e0 = e0e1_.e[0], e1 = e0e1_.e[1];
     RS_HIP(hipEventRecord(e0, st));
     for (uint32_t i = 0; i < reps; i++) launch_gnn(g, st);
     RS_HIP(hipEventRecord(e1, st));
@@ -1860,8 +1873,6 @@ extern "C" int rs_learned_infer_timed(rs_learned *L, uint32_t reps, double *avg_
     RS_HIP(hipGetLastError());
     float ms = 0.0f;
     RS_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     *avg_ms = ms / reps;
     return RS_OK;
 }

@@ -205,9 +205,9 @@ extern "C" int rs_assemble_radiosity_device(rs_radiosity *R, const double *d_vie
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
-    hipEvent_t ev[2];
-    RS_HIP(hipEventCreate(&ev[0]));
-    RS_HIP(hipEventCreate(&ev[1]));
+    EventPair evp;
+    RS_HIP(evp.create());
+    hipEvent_t *ev = evp.e;
     RS_HIP(hipEventRecord(ev[0], st));
     k_rad_means<<<(R->np + kBlock - 1) / kBlock, kBlock, 0, st>>>(R->ptr.p, R->elems.p, R->np, d_emissivity,
                                                                   d_temperature, d_ambient, R->E.p, R->T.p, R->Ta.p);
@@ -218,8 +218,6 @@ extern "C" int rs_assemble_radiosity_device(rs_radiosity *R, const double *d_vie
     float f = 0.0f;
     RS_HIP(hipEventElapsedTime(&f, ev[0], ev[1]));
     R->last_ms = f;
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
     if ((rc = matrix_frobenius(R->A))) return rc;
     matrix_values_changed(R->A);
     return RS_OK;

@@ -1160,9 +1160,9 @@ static int time_plan(Matrix &A, int id, bool fp32, uint32_t reps, const void *x,
         else RS_SHAPE_SWITCH(id, k_spmv, nb, st, v, (const double *)x, (double *)y);
     };
     launch();
-    hipEvent_t e0, e1;
-    RS_HIP(hipEventCreate(&e0));
-    RS_HIP(hipEventCreate(&e1));
+    EventPair e0e1_;
+    RS_HIP(e0e1_.create());
+    hipEvent_t e0 = e0e1_.e[0], e1 = e0e1_.e[1];
     RS_HIP(hipEventRecord(e0, st));
     for (uint32_t i = 0; i < reps; i++) launch();
     RS_HIP(hipEventRecord(e1, st));
@@ -1170,8 +1170,6 @@ static int time_plan(Matrix &A, int id, bool fp32, uint32_t reps, const void *x,
     RS_HIP(hipGetLastError());
     float f = 0.0f;
     RS_HIP(hipEventElapsedTime(&f, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     ms = f / reps;
     return RS_OK;
 }
@@ -1478,6 +1476,30 @@ extern "C" int rs_precond_compute(const rs_precond *P, const double *x, double *
     return RS_OK;
 }
 
+extern "C" int rs_precond_copy(const rs_precond *src, rs_precond **out)
+{
+    if (!src || !out) return fail(RS_ERR_INVALID_ARG, "rs_precond_copy: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    std::unique_ptr<rs_precond> P(new rs_precond());
+    P->type = src->type;
+    P->n = src->n;
+    P->width = src->width;
+    hipStream_t st = runtime().stream;
+    if (src->data.n) {
+        if ((rc = P->data.alloc(src->data.n))) return rc;
+        RS_HIP(hipMemcpyAsync(P->data.p, src->data.p, sizeof(double) * src->data.n, hipMemcpyDeviceToDevice, st));
+    }
+    if (src->data32_valid && src->data32.n) {
+        if ((rc = P->data32.alloc(src->data32.n))) return rc;
+        RS_HIP(hipMemcpyAsync(P->data32.p, src->data32.p, sizeof(float) * src->data32.n, hipMemcpyDeviceToDevice, st));
+        P->data32_valid = true;
+    }
+    RS_HIP(hipStreamSynchronize(st));
+    *out = P.release();
+    return RS_OK;
+}
+
 extern "C" int rs_precond_destroy(rs_precond *P)
 {
     delete P;
@@ -1501,7 +1523,6 @@ struct rs_solver {
     // iterations launched one kernel at a time)
     hipGraphExec_t graph = nullptr;
     std::string graph_key;   // raw bytes of every kernel argument and launch shape captured
-    std::function<void()> body;
     bool eager = false;
     // replay loop: pinned copies of the stop flags and their events, kept for the solver's life
     void *hflags = nullptr;
@@ -1573,13 +1594,15 @@ static void emit_log(const rs_solver &S, const char *title, uint32_t iters, uint
     std::printf("-------------------------------------------------------------------------------\n");
 }
 
-// Replay the captured iteration graph until the device-side stop flag is raised.  Two
+// Replay the captured iteration graph until the device-side stop flag is raised.  `body` is the
+// function the graph was captured from; it is called directly only in the eager mode, and the
+// caller keeps it (and the locals it refers to) alive for the whole loop.  Two
 // replays are kept in flight so that the host's flag check overlaps device work, except when
 // the error history projects convergence within the replay already in flight: then the next
 // replay waits until that one's state is read.  A replay enqueued after convergence runs all
 // of its iterations idle (each kernel still dispatches and returns), so this only changes how
 // many idle iterations follow the stop, never the iterations or the results.
-static int run_graph_loop(rs_solver &S, uint32_t &done_out)
+static int run_graph_loop(rs_solver &S, uint32_t &done_out, const std::function<void()> &body)
 {
     hipStream_t st = runtime().stream;
     if (!S.hflags) {
@@ -1591,7 +1614,7 @@ static int run_graph_loop(rs_solver &S, uint32_t &done_out)
     hipEvent_t *ev = S.ev;
     auto enqueue = [&](int slot) -> int {
         if (S.eager) {
-            S.body();
+            body();
             RS_HIP(hipGetLastError());
         } else {
             RS_HIP(hipGraphLaunch(S.graph, st));
@@ -1649,7 +1672,6 @@ static std::string graph_key_of(int kind, const Args &a, int plan, uint32_t nb, 
 
 static int capture(rs_solver &S, const std::function<void()> &body, const std::string &key)
 {
-    S.body = body;
     const char *ng = std::getenv("RS_NO_GRAPH");
     S.eager = ng && ng[0] == '1';
     if (S.eager) return RS_OK;
@@ -1702,7 +1724,7 @@ static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, ui
     RS_DISPATCH_W(w, k_cg_rho, vb, a);
     fin(S.partials.p, vb, 1, FIN_RHO, 0, S.st.p, S.hist.p);
     RS_HIP(hipGetLastError());
-    rc = capture(S, [&] {
+    const std::function<void()> body = [&] {
         for (uint32_t i = 0; i < kGraphIters; i++) {
             RS_DISPATCH_W(w, k_cg_dir, vb, a);
             RS_SHAPE_SWITCH(plan, k_cg_spmv_dot, nb, stream, a);
@@ -1710,9 +1732,10 @@ static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, ui
             RS_DISPATCH_W(w, k_cg_update, vb, a);
             fin(S.partials.p, vb, 3, FIN_UPDATE, 0, S.st.p, S.hist.p);
         }
-    }, graph_key_of(1, a, plan, nb, w, vb));
+    };
+    rc = capture(S, body, graph_key_of(1, a, plan, nb, w, vb));
     if (rc) return rc;
-    return run_graph_loop(S, done);
+    return run_graph_loop(S, done, body);
 }
 
 static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, uint32_t &done,
@@ -1751,7 +1774,7 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
     RS_DISPATCH_W(w, k_mp_rho, vb, a);
     fin(S.partials.p, vb, 1, FIN_MP_RHO, 1, S.st.p, S.hist.p);
     RS_HIP(hipGetLastError());
-    rc = capture(S, [&] {
+    const std::function<void()> body = [&] {
         for (uint32_t i = 0; i < kGraphIters; i++) {
             RS_DISPATCH_W(w, k_mp_dir, vb, a);
             RS_SHAPE_SWITCH(plan32, k_mp_spmv_dot, nb32, stream, a);
@@ -1759,7 +1782,8 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
             RS_DISPATCH_W(w, k_mp_update, vb, a);
             fin(S.partials.p, vb, 2, FIN_MP_UPDATE, 0, S.st.p, S.hist.p);
         }
-    }, graph_key_of(2, a, plan32, nb32, w, vb));
+    };
+    rc = capture(S, body, graph_key_of(2, a, plan32, nb32, w, vb));
     if (rc) return rc;
     refinements = 0;
     SolveState h;
@@ -1769,7 +1793,7 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
     double best = h.t2;
     uint32_t no_gain = 0;
     while (done == kStopRunning) {
-        if ((rc = run_graph_loop(S, done))) return rc;
+        if ((rc = run_graph_loop(S, done, body))) return rc;
         if (done != kStopPause) break;
         refinements++;
         // replacements that stop lowering the FP64 residual mean the FP32 recursion has lost
@@ -1942,9 +1966,9 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
     rs_precond P;
     if ((rc = ensure_ws(S, A.n, mixed, gmres))) return rc;
     hipStream_t stream = runtime().stream;
-    hipEvent_t e0, e1;
-    RS_HIP(hipEventCreate(&e0));
-    RS_HIP(hipEventCreate(&e1));
+    EventPair e0e1_;
+    RS_HIP(e0e1_.create());
+    hipEvent_t e0 = e0e1_.e[0], e1 = e0e1_.e[1];
     RS_HIP(hipEventRecord(e0, stream));
     // the learned path's iterations do not read P: it is built only if Jacobi-CG takes over
     if (!learned && (rc = precond_build(A, ptype, block_size, P))) return rc;
@@ -1965,7 +1989,6 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
             rc = solve_gmres(S, A, P, d_x, iters, inner);
         } else if (learned) {
             rc = solve_fcg_learned(S, A, P, *L, d_x, done, refinements);
-            S.body = nullptr;
             if (rc) return rc;
             SolveState hs;
             RS_HIP(hipMemcpyAsync(&hs, S.st.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
@@ -1985,7 +2008,6 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
         } else {
             rc = solve_cg64(S, A, P, d_x, done);
         }
-        S.body = nullptr;  // the iteration body refers to this call's locals
         if (rc) return rc;
     }
     k_unscale<<<nvb, kBlock, 0, stream>>>(d_x, n, S.st.p);
@@ -1994,8 +2016,6 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
     RS_HIP(hipEventSynchronize(e1));
     float ms = 0.0f;
     RS_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     SolveState h;
     RS_HIP(hipMemcpyAsync(&h, S.st.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     RS_HIP(hipStreamSynchronize(stream));

@@ -1,10 +1,12 @@
 // Drop-in check of the C++ call surface: the known-answer cases of the reference's
 // tests/tst_matrix_solver.cpp:33-77 (CG, preconditioner NONE, cvg 1e-13, 1000 iterations,
 // tolerance 1e-6) through rsamd::RMatrixSolver, plus the error behaviour of an unknown
-// preconditioner type, and an eigenvalue solve through rsamd::REigenValueSolver.  Exit status 0
+// preconditioner type, the copy semantics of RMatrixPreconditioner, the bounded device-matrix
+// cache, and an eigenvalue solve through rsamd::REigenValueSolver.  Exit status 0
 // when every check passes.
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 #include "rmatrixsolver_amd.hpp"
 
@@ -80,6 +82,27 @@ int main()
         RRVector pz;
         P.compute(b, pz);
         EXPECT(std::fabs(pz[1] - 2.0) < 1e-15);
+        // copy constructor and assignment carry the type and the values
+        // (include/rmatrixpreconditioner.h:39,45); the copies stay independent
+        RMatrixPreconditioner Pc(P);
+        RRVector pc;
+        Pc.compute(b, pc);
+        EXPECT(pc.getNRows() == 3u && pc[0] == pz[0] && pc[1] == pz[1] && pc[2] == pz[2]);
+        RMatrixPreconditioner Pa(a, R_MATRIX_PRECONDITIONER_NONE, 1);
+        RRVector pa;
+        Pa.compute(b, pa);
+        EXPECT(pa[0] == b[0]);
+        Pa = P;
+        Pa.compute(b, pa);
+        EXPECT(pa[0] == pz[0] && pa[2] == pz[2]);
+        {
+            RMatrixPreconditioner Pb(a, R_MATRIX_PRECONDITIONER_BLOCK_JACOBI, 3);
+            Pa = Pb;
+        }  // Pb destroyed: Pa keeps its own copy
+        Pa.compute(b, pa);
+        EXPECT(std::fabs(pa[0] - 1.5) < 1e-15 && std::fabs(pa[1] - 2.0) < 1e-15 && std::fabs(pa[2] - 2.0) < 1e-15);
+        P.compute(b, pz);
+        EXPECT(pz[0] == pc[0]);
         bool threw = false;
         try {
             solver.solve(a, b, y, (RMatrixPreconditionerType)7, 1);
@@ -112,6 +135,31 @@ int main()
         EXPECT(d.getNRows() == 3u && ev.getNRows() == 3u && ev.getNColumns() == n);
         EXPECT(std::fabs(d[0] - 2.0) < 1e-8 && std::fabs(d[1] - 5.0) < 1e-8 && std::fabs(d[2] - 11.0) < 1e-8);
         std::printf("arnoldi: d = [%.10f, %.10f, %.10f]\n", d[0], d[1], d[2]);
+    }
+    {
+        // the device copies of solved matrices are bounded: 40 matrix objects solved in turn
+        // leave at most 16 copies, and each solve is still right after its copy was freed
+        std::vector<RSparseMatrix> mats(40);
+        RMatrixSolver solver(cg_conf());
+        bool ok = true;
+        for (int round = 0; round < 2; round++)
+            for (unsigned k = 0; k < mats.size(); k++) {
+                RSparseMatrix &m = mats[k];
+                if (round == 0) {
+                    m.setNRows(2);
+                    m.addValue(0, 0, 1.0 + k);
+                    m.addValue(1, 1, 2.0);
+                }
+                RRVector b(2, 2.0), x;
+                solver.solve(m, b, x);
+                ok = ok && std::fabs(x[0] - 2.0 / (1.0 + k)) < 1e-12 && std::fabs(x[1] - 1.0) < 1e-12;
+            }
+        EXPECT(ok);
+        EXPECT(detail::cached_matrices() <= detail::kMaxCachedMatrices);
+        releaseDeviceMatrix(mats[39]);
+        EXPECT(detail::cached_matrices() <= detail::kMaxCachedMatrices - 1);
+        releaseDeviceMatrices();
+        EXPECT(detail::cached_matrices() == 0u);
     }
     std::printf("%s (%d failures)\n", failures ? "FAIL" : "PASS", failures);
     return failures ? 1 : 0;
