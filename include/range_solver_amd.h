@@ -175,6 +175,19 @@ int rs_solver_solve(rs_solver *S, rs_matrix *A, const double *b, double *x,
 /* Same, with b and x already resident in device memory. */
 int rs_solver_solve_device(rs_solver *S, rs_matrix *A, const double *d_b, double *d_x,
                            int32_t precond_type, uint32_t block_size, rs_info *info);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): rs_solver_solve_device for
+ * FP64 CG (preconditioner NONE, JACOBI or BLOCK_JACOBI) that calls no hipMalloc/hipFree.  Its
+ * workspace is one library-held device array of 134,217,728 bytes per GPU (held until the process
+ * ends): 256-byte-aligned state, partial sums and history, plus 4 n + n*w doubles (w = the block
+ * width, 1 for Jacobi, 0 for NONE), so up to ~3.3 M unknowns with Jacobi; larger systems are
+ * refused with RS_ERR_INVALID_ARG.  Calls on one GPU share that array: calls that overlap on a
+ * GPU race.  Same results as rs_solver_solve_device on the same SpMV shape (the matrix's tuned
+ * shape if its layout is already built, else vector CSR; no tuning here).  Launches are eager (no
+ * graph), no convergence log or history is written (rs_solver_history reports none), and GMRES
+ * or mixed precision return RS_ERR_UNSUPPORTED.  Call rs_init first: it looks the array up. */
+int rs_solver_solve_device_noalloc(rs_solver *S, rs_matrix *A, const double *d_b, double *d_x,
+                                   int32_t precond_type, uint32_t block_size, rs_info *info);
 /* (error, trend) per iteration of the last solve, 2*count doubles. */
 int rs_solver_history(const rs_solver *S, double *out, uint32_t cap, uint32_t *count);
 int rs_solver_destroy(rs_solver *S);

@@ -38,6 +38,8 @@ REASON_NAMES = {0: "running", 1: "converged", 2: "stagnated", 3: "nonnormal", 4:
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "UNKNOWN_SOLVER", 3: "INVALID_PRECOND",
                 4: "DEVICE", 5: "NO_DEVICE", 6: "OUT_OF_MEMORY", 7: "NOT_INITIALIZED",
                 8: "UNSUPPORTED"}
+(RS_OK, RS_ERR_INVALID_ARG, RS_ERR_UNKNOWN_SOLVER, RS_ERR_INVALID_PRECOND, RS_ERR_DEVICE, RS_ERR_NO_DEVICE,
+ RS_ERR_OUT_OF_MEMORY, RS_ERR_NOT_INITIALIZED, RS_ERR_UNSUPPORTED) = range(9)
 
 
 class RSError(RuntimeError):
@@ -132,6 +134,7 @@ _SIGS = {
     "rs_solver_disable_convergence_log": [_P],
     "rs_solver_solve": [_P, _P, _P, _P, C.c_int32, C.c_uint32, C.POINTER(Info)],
     "rs_solver_solve_device": [_P, _P, _P, _P, C.c_int32, C.c_uint32, C.POINTER(Info)],
+    "rs_solver_solve_device_noalloc": [_P, _P, _P, _P, C.c_int32, C.c_uint32, C.POINTER(Info)],
     "rs_solver_history": [_P, _P, C.c_uint32, _u32p], "rs_solver_destroy": [_P],
     "rs_mesh_kuhn_box": [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, _P, _P],
     "rs_mesh_plane_faces": [C.c_uint32, _P, C.c_uint32, _P, C.c_int32, C.c_double, C.c_double, _P,
@@ -506,6 +509,18 @@ class RMatrixSolver:
         info = Info()
         _check(_lib.rs_solver_solve_device(self.h, A.h, db.ptr, dx.ptr, precond, block_size,
                                            C.byref(info)))
+        self.info = info
+        return info
+
+    def solve_device_noalloc(self, A: SparseMatrix, db: DeviceArray, dx: DeviceArray,
+                             precond: int = PRECOND_NONE, block_size: int = 1) -> Info:
+        """rs_solver_solve_device_noalloc: FP64 CG in the library's fixed device workspace."""
+        n = A.n
+        _need_dev("db", db, n)
+        _need_dev("dx", dx, n)
+        info = Info()
+        _check(_lib.rs_solver_solve_device_noalloc(self.h, A.h, db.ptr, dx.ptr, precond, block_size,
+                                                   C.byref(info)))
         self.info = info
         return info
 

@@ -30,6 +30,9 @@ struct Runtime {
 };
 Runtime &runtime();
 int require_runtime();
+// Looks up, on `device` (the current device), the address of the allocation-free solve's
+// workspace array; called by rs_init once per device.
+int scratch_setup(int device);
 
 // RConstants::eps lives in range-base-lib (absent); value chosen here.
 constexpr double kEps = 1.0e-14;

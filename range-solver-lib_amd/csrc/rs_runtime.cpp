@@ -84,6 +84,9 @@ extern "C" int rs_init(int device)
     rt.device = device;
     rt.n_cu = prop.multiProcessorCount;
     rt.ready = true;
+    // the allocation-free solve's workspace address on this device; a failure leaves it unset
+    // and rs_solver_solve_device_noalloc tries again and reports the error
+    if (scratch_setup(device) != RS_OK) set_error("");
     return RS_OK;
 }
 

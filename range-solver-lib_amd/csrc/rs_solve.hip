@@ -31,7 +31,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <memory>
+#include <mutex>
 
 #include "rs_device.hpp"
 #include "rs_internal.hpp"
@@ -1023,19 +1025,19 @@ static void fin(const double *partials, uint32_t nparts, int nv, int op, int fla
     k_fin<<<1, kFinBlock, 0, runtime().stream>>>(partials, nparts, nv, op, flag, st, hist);
 }
 
-// Scratch for standalone reductions (matrix norm).
-struct Scratch {
+// Workspace of standalone reductions (matrix norm).
+struct NormWork {
     DBuf<double> partials;
     DBuf<SolveState> st;
 };
-static Scratch &scratch()
+static NormWork &norm_work()
 {
-    static Scratch s;
+    static NormWork s;
     return s;
 }
-static int ensure_scratch()
+static int ensure_norm_work()
 {
-    Scratch &s = scratch();
+    NormWork &s = norm_work();
     int rc;
     if (!s.partials.p && (rc = s.partials.alloc(8 * 4096))) return rc;
     if (!s.st.p && (rc = s.st.alloc(1))) return rc;
@@ -1044,9 +1046,9 @@ static int ensure_scratch()
 
 int matrix_frobenius(Matrix &A)
 {
-    int rc = ensure_scratch();
+    int rc = ensure_norm_work();
     if (rc) return rc;
-    Scratch &s = scratch();
+    NormWork &s = norm_work();
     uint32_t nb = vec_blocks(A.nnz);
     hipStream_t st = runtime().stream;
     k_norm2<<<nb, kBlock, 0, st>>>(A.val.p, A.nnz, s.partials.p);
@@ -1378,14 +1380,34 @@ struct rs_precond {
 };
 
 namespace rs {
+// Block width the reference uses for block size w: decremented until it divides n
+// (src/rmatrixpreconditioner.cpp:98-105).
+static uint32_t block_width(uint32_t n, uint32_t w)
+{
+    while (w > 1 && n % w != 0) w--;
+    return w;
+}
+
+// Launch the Jacobi / block-Jacobi build of width w into `data` (n or n*w doubles).
+static int precond_build_at(const Matrix &A, int32_t type, uint32_t w, double *data)
+{
+    if (A.n == 0 || type == RS_PRECOND_NONE) return RS_OK;
+    uint32_t nb_unused = 0;
+    CsrView v = plan_view(A, 0, nb_unused);
+    hipStream_t st = runtime().stream;
+    if (type == RS_PRECOND_JACOBI) k_jacobi_build<<<vec_blocks(A.n), kBlock, 0, st>>>(v, data);
+    else if (w == 3) k_blockjacobi_build<3><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, data);
+    else if (w <= 4) k_blockjacobi_build<4><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, data);
+    else k_blockjacobi_build<16><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, data);
+    RS_HIP(hipGetLastError());
+    return RS_OK;
+}
+
 static int precond_build(const Matrix &A, int32_t type, uint32_t block_size, rs_precond &P)
 {
     P.type = type;
     P.n = A.n;
     P.data32_valid = false;
-    uint32_t nb_unused = 0;
-    CsrView v = plan_view(A, 0, nb_unused);
-    hipStream_t st = runtime().stream;
     int rc;
     if (type == RS_PRECOND_NONE) {
         P.width = 1;
@@ -1394,22 +1416,15 @@ static int precond_build(const Matrix &A, int32_t type, uint32_t block_size, rs_
     if (type == RS_PRECOND_JACOBI) {
         P.width = 1;
         if ((rc = P.data.alloc(std::max(A.n, 1u)))) return rc;
-        if (A.n) k_jacobi_build<<<vec_blocks(A.n), kBlock, 0, st>>>(v, P.data.p);
-        RS_HIP(hipGetLastError());
-        return RS_OK;
+        return precond_build_at(A, type, 1, P.data.p);
     }
     if (type == RS_PRECOND_BLOCK_JACOBI) {
-        uint32_t w = block_size;
-        if (w == 0) return fail(RS_ERR_INVALID_ARG, "block-Jacobi: block size 0");
-        if (w > 16) return fail(RS_ERR_UNSUPPORTED, "block-Jacobi: block size above 16");
-        while (A.n % w != 0) w--;  // src/rmatrixpreconditioner.cpp:98-105
+        if (block_size == 0) return fail(RS_ERR_INVALID_ARG, "block-Jacobi: block size 0");
+        if (block_size > 16) return fail(RS_ERR_UNSUPPORTED, "block-Jacobi: block size above 16");
+        const uint32_t w = block_width(A.n, block_size);
         P.width = w;
         if ((rc = P.data.alloc(std::max<uint64_t>((uint64_t)A.n * w, 1)))) return rc;
-        if (A.n && w == 3) k_blockjacobi_build<3><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, P.data.p);
-        else if (A.n && w <= 4) k_blockjacobi_build<4><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, P.data.p);
-        else if (A.n) k_blockjacobi_build<16><<<vec_blocks(A.n / w), kBlock, 0, st>>>(v, w, P.data.p);
-        RS_HIP(hipGetLastError());
-        return RS_OK;
+        return precond_build_at(A, type, w, P.data.p);
     }
     return fail(RS_ERR_INVALID_PRECOND, "Invalid matrix preconditioner type '" + std::to_string(type) + "'");
 }
@@ -1696,6 +1711,19 @@ static int capture(rs_solver &S, const std::function<void()> &body, const std::s
     return RS_OK;
 }
 
+// kGraphIters FP64 CG iterations (src/rmatrixsolver.cpp:248-353), 5 launches each:
+//   dir, spmv_dot, fin DOT, update, fin UPDATE.
+static void cg64_iterations(const CgArgs &a, int plan, uint32_t nb, uint32_t w, uint32_t vb, hipStream_t stream)
+{
+    for (uint32_t i = 0; i < kGraphIters; i++) {
+        RS_DISPATCH_W(w, k_cg_dir, vb, a);
+        RS_SHAPE_SWITCH(plan, k_cg_spmv_dot, nb, stream, a);
+        fin(a.partials, nb, 1, FIN_DOT, 0, a.st, a.hist);
+        RS_DISPATCH_W(w, k_cg_update, vb, a);
+        fin(a.partials, vb, 3, FIN_UPDATE, 0, a.st, a.hist);
+    }
+}
+
 static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, uint32_t &done)
 {
     int rc = matrix_tune(A, false);
@@ -1724,15 +1752,7 @@ static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, ui
     RS_DISPATCH_W(w, k_cg_rho, vb, a);
     fin(S.partials.p, vb, 1, FIN_RHO, 0, S.st.p, S.hist.p);
     RS_HIP(hipGetLastError());
-    const std::function<void()> body = [&] {
-        for (uint32_t i = 0; i < kGraphIters; i++) {
-            RS_DISPATCH_W(w, k_cg_dir, vb, a);
-            RS_SHAPE_SWITCH(plan, k_cg_spmv_dot, nb, stream, a);
-            fin(S.partials.p, nb, 1, FIN_DOT, 0, S.st.p, S.hist.p);
-            RS_DISPATCH_W(w, k_cg_update, vb, a);
-            fin(S.partials.p, vb, 3, FIN_UPDATE, 0, S.st.p, S.hist.p);
-        }
-    };
+    const std::function<void()> body = [&] { cg64_iterations(a, plan, nb, w, vb, stream); };
     rc = capture(S, body, graph_key_of(1, a, plan, nb, w, vb));
     if (rc) return rc;
     return run_graph_loop(S, done, body);
@@ -2083,6 +2103,201 @@ static int solve_device(rs_solver &S, rs_matrix &A, const double *d_b, double *d
 
 }  // namespace rs
 
+// ============================================================================ allocation-free solve
+// Workspace of rs_solver_solve_device_noalloc: one file-scope device array, allocated with the
+// code object on every GPU the library runs on and held until the process ends.  Host code
+// never passes `scratch` itself to a kernel (on the host it is only a stand-in for the
+// symbol): rs_init looks up its device address once per device (scratch_setup) and every
+// launch uses that address.  Calls on one GPU share the array: overlapping calls race, and
+// each call initialises every part it reads (state, partial sums, vectors, preconditioner).
+static constexpr uint64_t kNoAllocScratchBytes = 128ull << 20;   // 134,217,728 bytes
+static constexpr uint32_t kNoAllocHistCap = 65536;                // error history entries kept
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace rs {
+
+static constexpr int kMaxScratchDevices = 64;
+static void *g_scratch_ptr[kMaxScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on `device`, which must be the current device.
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return fail(RS_ERR_UNSUPPORTED, "allocation-free workspace: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return RS_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(RS_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[device] = p;
+    return RS_OK;
+}
+
+static int scratch_address(void *&out)
+{
+    int dev = -1;
+    RS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices || !g_scratch_ptr[dev]) {
+        int rc = scratch_setup(dev);   // a device the library was not initialised on
+        if (rc) return rc;
+    }
+    out = g_scratch_ptr[dev];
+    return RS_OK;
+}
+
+// Bytes of `scratch` a solve of n unknowns needs (state, partial sums, history, y r p q, the
+// preconditioner's n * w values), laid out at 256-byte boundaries.
+struct NoAllocLayout {
+    uint64_t st = 0, partials = 0, hist = 0, y = 0, r = 0, p = 0, q = 0, pd = 0, total = 0;
+    uint32_t hist_cap = 0;
+};
+
+static NoAllocLayout noalloc_layout(uint64_t n, uint32_t w, bool precond, uint32_t n_outer)
+{
+    NoAllocLayout L;
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = off;
+        off += (bytes + 255) & ~255ull;
+        return o;
+    };
+    const uint64_t N = std::max<uint64_t>(n, 1);
+    L.hist_cap = std::min(kNoAllocHistCap, std::max(1u, n_outer));
+    L.st = take(sizeof(SolveState));
+    L.partials = take(sizeof(double) * 4 * 4096);
+    L.hist = take(sizeof(double) * 2 * L.hist_cap);
+    L.y = take(sizeof(double) * N);
+    L.r = take(sizeof(double) * N);
+    L.p = take(sizeof(double) * N);
+    L.q = take(sizeof(double) * N);
+    L.pd = precond ? take(sizeof(double) * N * w) : 0;
+    L.total = off;
+    return L;
+}
+
+// RMatrixSolver::solve (src/rmatrixsolver.cpp:158-201) with solveCG (:208-355), FP64, on the
+// caller's device vectors, using only `scratch`: the same kernels in the same order as
+// solve_device_ext's FP64 CG, launched eagerly (a graph instantiation allocates), with the
+// stop flag read back every kGraphIters iterations.  The SpMV shape is the matrix's tuned one
+// when its layout needs nothing more, else vector CSR (no tuning: the tuner allocates).
+static int solve_device_noalloc(rs_solver &S, rs_matrix &A, const double *d_b, double *d_x, int32_t ptype,
+                                uint32_t block_size, rs_info *info)
+{
+    const rs_conf &c = S.conf;
+    if (c.solver_type != RS_SOLVER_CG && c.solver_type != RS_SOLVER_GMRES)
+        return fail(RS_ERR_UNKNOWN_SOLVER, "Unknown matrix solver type '" + std::to_string(c.solver_type) + "'");
+    if (ptype < RS_PRECOND_NONE || ptype > RS_PRECOND_BLOCK_JACOBI)
+        return fail(RS_ERR_INVALID_PRECOND, "Invalid matrix preconditioner type '" + std::to_string(ptype) + "'");
+    if (c.solver_type != RS_SOLVER_CG || c.precision != RS_PRECISION_FP64)
+        return fail(RS_ERR_UNSUPPORTED, "rs_solver_solve_device_noalloc: FP64 CG only");
+    uint32_t w = 1;
+    if (ptype == RS_PRECOND_BLOCK_JACOBI) {
+        if (block_size == 0) return fail(RS_ERR_INVALID_ARG, "block-Jacobi: block size 0");
+        if (block_size > 16) return fail(RS_ERR_UNSUPPORTED, "block-Jacobi: block size above 16");
+        w = block_width(A.n, block_size);
+    }
+    const NoAllocLayout lay = noalloc_layout(A.n, w, ptype != RS_PRECOND_NONE, c.n_outer_iterations);
+    if (lay.total > kNoAllocScratchBytes)
+        return fail(RS_ERR_INVALID_ARG, "rs_solver_solve_device_noalloc: " + std::to_string(A.n) + " unknowns need " +
+                                            std::to_string(lay.total) + " bytes of workspace, above its " +
+                                            std::to_string(kNoAllocScratchBytes) + "-byte maximum");
+    void *base_v = nullptr;
+    int rc = scratch_address(base_v);
+    if (rc) return rc;
+    unsigned char *base = static_cast<unsigned char *>(base_v);
+    SolveState *st = reinterpret_cast<SolveState *>(base + lay.st);
+    double *partials = reinterpret_cast<double *>(base + lay.partials);
+    double *hist = reinterpret_cast<double *>(base + lay.hist);
+    double *y = reinterpret_cast<double *>(base + lay.y);
+    double *pd = ptype != RS_PRECOND_NONE ? reinterpret_cast<double *>(base + lay.pd) : nullptr;
+    // an SpMV shape whose layout is already there (a SELL shape needs its value copy allocated)
+    int plan = A.plan[0];
+    if (plan < 0 || (is_sell_plan(plan) && (!A.sell_val.p || A.sell_val.n != A.sell_col.n))) plan = default_plan(A);
+    if ((rc = matrix_sync_plan(A, plan, false))) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t stream = runtime().stream;
+    if ((rc = precond_build_at(A, ptype, w, pd))) return rc;
+    const uint32_t n = A.n;
+    const uint32_t nvb = vec_blocks(std::max(n, 1u));
+    k_state_init<<<1, 64, 0, stream>>>(st, A.frob, c.solver_cvg_value, S.error, c.n_outer_iterations,
+                                       (uint32_t)c.stop_rule, lay.hist_cap, 0);
+    k_norm2<<<nvb, kBlock, 0, stream>>>(d_b, n, partials);
+    fin(partials, nvb, 1, FIN_SCALE, 0, st, hist);
+    k_apply_scale<<<nvb, kBlock, 0, stream>>>(d_b, y, d_x, n, st);
+    RS_HIP(hipGetLastError());
+    uint32_t nb = 0;
+    CgArgs a{};
+    a.A = plan_view(A, plan, nb);
+    a.b = y;
+    a.x = d_x;
+    a.r = reinterpret_cast<double *>(base + lay.r);
+    a.p = reinterpret_cast<double *>(base + lay.p);
+    a.q = reinterpret_cast<double *>(base + lay.q);
+    a.ptype = ptype;
+    a.pw = w;
+    a.pd = pd;
+    a.partials = partials;
+    a.st = st;
+    a.hist = hist;
+    const uint32_t vb = vec_blocks(n / w);
+    SolveState h{};
+    if (n > 0) {
+        RS_SHAPE_SWITCH(plan, k_cg_init, nb, stream, a);
+        fin(partials, nb, 1, FIN_CG_INIT, 0, st, hist);
+        RS_DISPATCH_W(w, k_cg_rho, vb, a);
+        fin(partials, vb, 1, FIN_RHO, 0, st, hist);
+        RS_HIP(hipGetLastError());
+        for (;;) {
+            cg64_iterations(a, plan, nb, w, vb, stream);
+            RS_HIP(hipGetLastError());
+            RS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, stream));
+            RS_HIP(hipStreamSynchronize(stream));
+            if (h.done != kStopRunning) break;
+        }
+    }
+    k_unscale<<<nvb, kBlock, 0, stream>>>(d_x, n, st);
+    RS_HIP(hipGetLastError());
+    RS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, stream));
+    RS_HIP(hipStreamSynchronize(stream));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const uint32_t iters = n > 0 ? h.it : 0;
+    const double equation_scale = h.scale;
+    S.error = h.error;
+    S.trend = h.trend;
+    S.last_iterations = 0;   // no host history is kept
+    double true_relres = 0.0, b_norm = 0.0;
+    if (n > 0) {
+        RS_SHAPE_SWITCH(plan, k_resid_norm, nb, stream, a.A, d_b, d_x, partials);
+        fin(partials, nb, 2, FIN_NORM01, 0, st, hist);
+        RS_HIP(hipGetLastError());
+        SolveState r{};
+        RS_HIP(hipMemcpyAsync(&r, st, sizeof(r), hipMemcpyDeviceToHost, stream));
+        RS_HIP(hipStreamSynchronize(stream));
+        true_relres = r.t1 > 0.0 ? r.t0 / r.t1 : r.t0;
+        b_norm = r.t1;
+    }
+    if (info) {
+        info->iterations = iters;
+        info->inner_iterations = 0;
+        info->reason = (int)h.reason;
+        info->converged = h.reason == RS_REASON_CONVERGED;
+        info->error = h.error;
+        info->trend = h.trend;
+        info->equation_scale = equation_scale;
+        info->a_norm = A.frob;
+        info->b_norm = b_norm;
+        info->true_relres = true_relres;
+        info->solve_ms = ms;
+        info->precond_applications = 0;
+        info->fallback = 0;
+    }
+    return RS_OK;
+}
+
+}  // namespace rs
+
 extern "C" int rs_solver_create(const rs_conf *conf, rs_solver **out)
 {
     if (!conf || !out) return fail(RS_ERR_INVALID_ARG, "rs_solver_create: null argument");
@@ -2120,6 +2335,16 @@ extern "C" int rs_solver_solve_device(rs_solver *S, rs_matrix *A, const double *
     int rc = require_runtime();
     if (rc) return rc;
     return solve_device(*S, *A, d_b, d_x, precond_type, block_size, info);
+}
+
+extern "C" int rs_solver_solve_device_noalloc(rs_solver *S, rs_matrix *A, const double *d_b, double *d_x,
+                                              int32_t precond_type, uint32_t block_size, rs_info *info)
+{
+    if (!S || !A || (A->n && (!d_b || !d_x)))
+        return fail(RS_ERR_INVALID_ARG, "rs_solver_solve_device_noalloc: null argument");
+    int rc = require_runtime();
+    if (rc) return rc;
+    return solve_device_noalloc(*S, *A, d_b, d_x, precond_type, block_size, info);
 }
 
 extern "C" int rs_solver_solve(rs_solver *S, rs_matrix *A, const double *b, double *x,
