@@ -500,6 +500,7 @@ int rs_learned_level_matrix(const rs_learned *L, uint32_t level, uint32_t *row_p
                             uint32_t *col, double *val);
 int rs_learned_coarse_matrix(const rs_learned *L, double *dense);         /* n_coarse^2 */
 int rs_learned_restrict_device(rs_learned *L, const double *d_r, float *rc_host);
+/* The configured coarsest solver (network, coarse Jacobi or exact inverse) on rc (n_coarse values). */
 int rs_learned_apply_coarse(rs_learned *L, const float *rc_host, float *e_host);
 /* Average duration (ms) of the network inference kernel over reps back-to-back launches. */
 int rs_learned_infer_timed(rs_learned *L, uint32_t reps, double *avg_ms);

@@ -511,6 +511,156 @@ __global__ __launch_bounds__(kBlock) void k_coarse_exact(const float *__restrict
     }
 }
 
+// ---------------------------------------------------------------- exact coarsest inverse
+// The coarsest operator (SPD, n <= 512) is inverted on the device by the blocked symmetric
+// sweep operator over 64 x 64 pivot blocks (Goodnight 1979, "A tutorial on the SWEEP
+// operator"): sweeping pivot block k maps A_kk -> -D, A_ik -> A_ik D, A_kj -> D A_kj and
+// A_ij -> A_ij - A_ik D A_kj with D = A_kk^-1; after every block has been swept the matrix holds
+// -A^-1.  FP64 throughout, the result rounded to FP32 for the cycle.  The matrix is padded to a
+// multiple of 64 with an identity block (decoupled, so its inverse is the identity there).
+constexpr uint32_t kSweepB = 64;
+constexpr uint32_t kSweepLd = kSweepB + 1;   // LDS row stride (one pad double)
+
+__global__ void k_dense_pad(double *__restrict__ m, uint32_t N, uint32_t n)
+{
+    for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < (uint64_t)N * N; e += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t i = (uint32_t)(e / N), j = (uint32_t)(e % N);
+        m[e] = (i == j && i >= n) ? 1.0 : 0.0;
+    }
+}
+
+__global__ void k_dense_scatter(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
+                                const double *__restrict__ val, uint32_t n, uint32_t N, double *__restrict__ m)
+{
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        for (uint32_t p = rowptr[i]; p < rowptr[i + 1]; p++) m[(uint64_t)i * N + col[p]] = val[p];
+}
+
+// D = A_kk^-1 by sweeping the 64 scalar pivots of the block in LDS (one workgroup).  A pivot that
+// is not positive (not SPD) raises *bad.
+__global__ __launch_bounds__(kBlock) void k_sweep_pivot(const double *__restrict__ m, uint32_t N, uint32_t k,
+                                                        double *__restrict__ d, uint32_t *bad)
+{
+    __shared__ double a[kSweepB * kSweepLd];
+    __shared__ double rowc[kSweepB], colc[kSweepB];
+    const uint64_t o = (uint64_t)k * kSweepB;
+    for (uint32_t e = threadIdx.x; e < kSweepB * kSweepB; e += kBlock)
+        a[(e / kSweepB) * kSweepLd + e % kSweepB] = m[(o + e / kSweepB) * N + o + e % kSweepB];
+    __syncthreads();
+    for (uint32_t c = 0; c < kSweepB; c++) {
+        const double p = a[c * kSweepLd + c];
+        if (threadIdx.x < kSweepB) {
+            rowc[threadIdx.x] = a[c * kSweepLd + threadIdx.x];
+            colc[threadIdx.x] = a[threadIdx.x * kSweepLd + c];
+        }
+        if (threadIdx.x == 0 && !(p > 0.0)) *bad = 1u;
+        __syncthreads();
+        const double ip = p > 0.0 ? 1.0 / p : 0.0;
+        for (uint32_t e = threadIdx.x; e < kSweepB * kSweepB; e += kBlock) {
+            const uint32_t i = e / kSweepB, j = e % kSweepB;
+            double v;
+            if (i == c && j == c) v = -ip;
+            else if (i == c) v = rowc[j] * ip;
+            else if (j == c) v = colc[i] * ip;
+            else v = a[i * kSweepLd + j] - colc[i] * rowc[j] * ip;
+            a[i * kSweepLd + j] = v;
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x; e < kSweepB * kSweepB; e += kBlock)
+        d[e] = -a[(e / kSweepB) * kSweepLd + e % kSweepB];
+}
+
+// C = X Y for two 64 x 64 LDS tiles; each thread owns a 4 x 4 block of C.
+__device__ __forceinline__ void tile_mul(const double *x, const double *y, double (&c)[4][4])
+{
+    const uint32_t rb = threadIdx.x / 16, cb = threadIdx.x % 16;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) c[r][q] = 0.0;
+    for (uint32_t kk = 0; kk < kSweepB; kk++) {
+        double xa[4], yb[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) xa[r] = x[(4 * rb + r) * kSweepLd + kk];
+#pragma unroll
+        for (int q = 0; q < 4; q++) yb[q] = y[kk * kSweepLd + 4 * cb + q];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) c[r][q] = fma(xa[r], yb[q], c[r][q]);
+    }
+}
+
+__device__ __forceinline__ void tile_load(const double *__restrict__ m, uint32_t N, uint32_t ti, uint32_t tj, double *t)
+{
+    for (uint32_t e = threadIdx.x; e < kSweepB * kSweepB; e += kBlock)
+        t[(e / kSweepB) * kSweepLd + e % kSweepB] = m[((uint64_t)ti * kSweepB + e / kSweepB) * N + (uint64_t)tj * kSweepB + e % kSweepB];
+}
+
+// One sweep of pivot block k over every 64 x 64 tile (one workgroup per tile), reading `in` and
+// writing `out` (every tile reads the pre-sweep row and column k).
+__global__ __launch_bounds__(kBlock) void k_sweep_tiles(const double *__restrict__ in, double *__restrict__ out, uint32_t N,
+                                                        uint32_t k, const double *__restrict__ d)
+{
+    __shared__ double X[kSweepB * kSweepLd], Y[kSweepB * kSweepLd], T[kSweepB * kSweepLd];
+    const uint32_t nb = N / kSweepB;
+    const uint32_t ti = blockIdx.x / nb, tj = blockIdx.x % nb;
+    const uint32_t rb = threadIdx.x / 16, cb = threadIdx.x % 16;
+    double c[4][4];
+    auto store = [&](double sign, const double *add) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t i = 4 * rb + r, j = 4 * cb + q;
+                const double base = add ? add[i * kSweepLd + j] : 0.0;
+                out[((uint64_t)ti * kSweepB + i) * N + (uint64_t)tj * kSweepB + j] = base + sign * c[r][q];
+            }
+    };
+    for (uint32_t e = threadIdx.x; e < kSweepB * kSweepB; e += kBlock)
+        Y[(e / kSweepB) * kSweepLd + e % kSweepB] = d[e];              // D
+    if (ti == k && tj == k) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) c[r][q] = Y[(4 * rb + r) * kSweepLd + 4 * cb + q];
+        store(-1.0, nullptr);
+    } else if (ti == k) {                                               // D A_kj
+        tile_load(in, N, k, tj, X);
+        __syncthreads();
+        tile_mul(Y, X, c);
+        store(1.0, nullptr);
+    } else if (tj == k) {                                               // A_ik D
+        tile_load(in, N, ti, k, X);
+        __syncthreads();
+        tile_mul(X, Y, c);
+        store(1.0, nullptr);
+    } else {                                                            // A_ij - (A_ik D) A_kj
+        tile_load(in, N, ti, k, X);
+        __syncthreads();
+        tile_mul(X, Y, c);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) T[(4 * rb + r) * kSweepLd + 4 * cb + q] = c[r][q];
+        tile_load(in, N, k, tj, X);
+        tile_load(in, N, ti, tj, Y);
+        __syncthreads();
+        tile_mul(T, X, c);
+        store(-1.0, Y);
+    }
+}
+
+// cinv = -m (the swept matrix) in FP32, n x n row-major.
+__global__ void k_sweep_result(const double *__restrict__ m, uint32_t N, uint32_t n, float *__restrict__ cinv)
+{
+    for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < (uint64_t)n * n; e += (uint64_t)gridDim.x * kBlock)
+        cinv[e] = (float)(-m[(e / n) * N + e % n]);
+}
+
 // Keep the coarsest right-hand side of iteration it (training data).
 __global__ void k_record(const float *__restrict__ r, float *__restrict__ rec, uint32_t n, uint32_t cap,
                          const SolveState *st)
@@ -1053,6 +1203,8 @@ struct rs_learned {
     rs::DBuf<float> gu, gnorm;
     rs::DBuf<float> cw, cdiag, coef, cinv, rec, tmp_rc, tmp_r;
     rs::DBuf<double> part, gersh;
+    rs::DBuf<double> sweep[2], sweep_d;              // exact coarsest inverse: swept matrix, pivot inverse
+    rs::DBuf<uint32_t> sweep_bad;
     rs::DBuf<float> theta;
     rs::DBuf<unsigned short> wbf;
     bool have_params = false;
@@ -1083,42 +1235,6 @@ static void cheb_coefs(double lambda, float *out)
         out[2 * k + 1] = (float)(2.0 * rn / delta);
         rho = rn;
     }
-}
-
-// In-place inverse of a dense row-major matrix (Gauss-Jordan, partial pivoting).
-static bool dense_inverse(std::vector<double> &a, uint32_t n)
-{
-    std::vector<double> inv((size_t)n * n, 0.0);
-    for (uint32_t i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
-    for (uint32_t c = 0; c < n; c++) {
-        uint32_t piv = c;
-        for (uint32_t r = c + 1; r < n; r++)
-            if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)piv * n + c])) piv = r;
-        if (a[(size_t)piv * n + c] == 0.0) return false;
-        if (piv != c)
-            for (uint32_t k = 0; k < n; k++) {
-                std::swap(a[(size_t)piv * n + k], a[(size_t)c * n + k]);
-                std::swap(inv[(size_t)piv * n + k], inv[(size_t)c * n + k]);
-            }
-        const double s = 1.0 / a[(size_t)c * n + c];
-        for (uint32_t k = 0; k < n; k++) {
-            a[(size_t)c * n + k] *= s;
-            inv[(size_t)c * n + k] *= s;
-        }
-        for (uint32_t r = 0; r < n; r++) {
-            if (r == c) continue;
-            const double f = a[(size_t)r * n + c];
-            if (f == 0.0) continue;
-            double *ar = &a[(size_t)r * n], *ir = &inv[(size_t)r * n];
-            const double *ac = &a[(size_t)c * n], *ic = &inv[(size_t)c * n];
-            for (uint32_t k = 0; k < n; k++) {
-                ar[k] -= f * ac[k];
-                ir[k] -= f * ic[k];
-            }
-        }
-    }
-    a.swap(inv);
-    return true;
 }
 
 static int coarsest_dense(const rs_learned &L, std::vector<double> &dense)
@@ -1190,12 +1306,25 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     RS_HIP(hipMemcpyAsync(L.coef.p, coef.data(), sizeof(float) * coef.size(), hipMemcpyHostToDevice, st));
     L.inverse_valid = false;
     if (L.coarse == RS_COARSE_EXACT) {
-        std::vector<double> dense;
-        if ((rc = coarsest_dense(L, dense))) return rc;
-        if (!dense_inverse(dense, L.nc)) return fail(RS_ERR_UNSUPPORTED, "learned preconditioner: singular coarsest operator");
-        std::vector<float> f(dense.begin(), dense.end());
-        if ((rc = L.cinv.alloc(f.size()))) return rc;
-        RS_HIP(hipMemcpyAsync(L.cinv.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice, st));
+        // the coarsest operator's inverse, on the device (blocked sweep, 2 launches per 64-block)
+        const uint32_t n = L.nc, N = (n + kSweepB - 1) / kSweepB * kSweepB, nbk = N / kSweepB;
+        if ((rc = L.sweep[0].alloc((size_t)N * N)) || (rc = L.sweep[1].alloc((size_t)N * N)) ||
+            (rc = L.sweep_d.alloc(kSweepB * kSweepB)) || (rc = L.sweep_bad.alloc(1)) ||
+            (rc = L.cinv.alloc((size_t)n * n)))
+            return rc;
+        k_fill_u32<<<1, 64, 0, st>>>(L.sweep_bad.p, 1, 0u);
+        k_dense_pad<<<vec_blocks((uint64_t)N * N), kBlock, 0, st>>>(L.sweep[0].p, N, n);
+        k_dense_scatter<<<vec_blocks(n), kBlock, 0, st>>>(C.rowptr.p, C.col.p, C.val.p, n, N, L.sweep[0].p);
+        for (uint32_t k = 0; k < nbk; k++) {
+            k_sweep_pivot<<<1, kBlock, 0, st>>>(L.sweep[k & 1].p, N, k, L.sweep_d.p, L.sweep_bad.p);
+            k_sweep_tiles<<<nbk * nbk, kBlock, 0, st>>>(L.sweep[k & 1].p, L.sweep[(k + 1) & 1].p, N, k, L.sweep_d.p);
+        }
+        k_sweep_result<<<vec_blocks((uint64_t)n * n), kBlock, 0, st>>>(L.sweep[nbk & 1].p, N, n, L.cinv.p);
+        RS_HIP(hipGetLastError());
+        uint32_t bad_pivot = 0;
+        RS_HIP(hipMemcpyAsync(&bad_pivot, L.sweep_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        RS_HIP(hipStreamSynchronize(st));
+        if (bad_pivot) return fail(RS_ERR_UNSUPPORTED, "learned preconditioner: the coarsest operator is not positive definite");
         L.inverse_valid = true;
     }
     RS_HIP(hipStreamSynchronize(st));
@@ -1837,14 +1966,21 @@ extern "C" int rs_learned_restrict_device(rs_learned *L, const double *d_r, floa
 extern "C" int rs_learned_apply_coarse(rs_learned *L, const float *rc_host, float *e_host)
 {
     if (!L || !rc_host || !e_host) return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: null");
-    if (!L->have_params) return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: parameters not set");
-    if (L->lam.empty()) return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: call rs_learned_setup first");
+    if (L->coarse == RS_COARSE_NETWORK && !L->have_params)
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: parameters not set");
+    if (L->lam.empty() || (L->coarse == RS_COARSE_EXACT && !L->inverse_valid))
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_apply_coarse: call rs_learned_setup first");
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
     CycLevel &V = *L->lv.back();
     RS_HIP(hipMemcpyAsync(V.r.p, rc_host, sizeof(float) * L->nc, hipMemcpyHostToDevice, st));
-    launch_gnn(gnn_args(*L, nullptr), st);
+    // the configured coarsest solver: the network, coarse Jacobi or the exact inverse
+    if (L->coarse == RS_COARSE_NETWORK) launch_gnn(gnn_args(*L, nullptr), st);
+    else if (L->coarse == RS_COARSE_EXACT)
+        k_coarse_exact<<<vec_blocks((uint64_t)L->nc * kWave), kBlock, 0, st>>>(L->cinv.p, V.r.p, V.za.p, L->nc, nullptr,
+                                                                              Prolong{});
+    else k_coarse_jacobi<<<vec_blocks(L->nc), kBlock, 0, st>>>(V.r.p, V.dinv.p, V.za.p, L->nc, nullptr, Prolong{});
     RS_HIP(hipGetLastError());
     RS_HIP(hipMemcpyAsync(e_host, V.za.p, sizeof(float) * L->nc, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));

@@ -327,6 +327,30 @@ def test_exact_coarse_not_worse_than_coarse_jacobi(gpu, heat24):
     assert its[rs.COARSE_EXACT] <= its[rs.COARSE_JACOBI]
 
 
+@pytest.mark.parametrize("cells,bins", [(24, (12, 12, 12)), (16, (8, 8, 8))], ids=["216_padded", "512"])
+def test_exact_coarse_inverse_on_device(gpu, cells, bins):
+    """The coarsest operator's inverse computed on the GPU (blocked sweep in FP64, stored in
+    FP32) against numpy's dense solve, on a size that is padded to a multiple of 64 (216) and on
+    512.  Coarse Jacobi, a planted stand-in for a wrong inverse, lands far outside the bar."""
+    rs = gpu
+    asm, _, xyz = _heat(rs, cells, 1.5)
+    maps = rs.hierarchy_by_bins(xyz, asm.book(), 1, bins, max_coarse=512)
+    L = rs.LearnedPreconditioner(asm.matrix, maps)
+    L.set_cycle(1, 1, rs.COARSE_EXACT)
+    L.setup()
+    Ac = L.coarse_matrix()
+    n = L.n_coarse
+    _say(f"exact coarse inverse on the device: n_coarse={n}")
+    rng = np.random.default_rng(11)
+    rc = rng.standard_normal(n).astype(np.float32)
+    e = L.apply_coarse(rc).astype(np.float64)
+    e_ref = np.linalg.solve(Ac, rc.astype(np.float64))
+    assert _rel(e, e_ref) < 1e-5
+    assert np.linalg.norm(Ac @ e - rc) / np.linalg.norm(rc) < 1e-4
+    L.set_cycle(1, 1, rs.COARSE_JACOBI)
+    assert _rel(L.apply_coarse(rc), e_ref) > 1e-2
+
+
 def test_records_are_the_cycle_coarsest_rhs(gpu, heat24):
     rs = gpu
     asm, b, maps, _, _ = heat24
