@@ -1029,6 +1029,209 @@ __global__ __launch_bounds__(kBlock) void k_stress_colour(StressArgs a, uint32_t
     }
 }
 
+// ------------------------------------------------------------------ stress: row gather
+// The static, unrotated stress assembly as a gather over rows instead of colour launches that
+// read-modify-write the CSR values.  One wavefront per node with a free component, one lane per
+// incident element: every lane loads its element (geometry record, Lame constants, loads) and
+// computes the 3 x 3 blocks K_mn coupling the node (local m) to the element's four nodes, with
+// the colour kernel's expressions, into LDS; then lane (b, i) sums row i's entries of column
+// block b over the elements in incidence order, which is colour order, so every entry receives
+// its terms in the colour kernel's order; the right-hand side is added the same way (load minus
+// lifting, the lifting summed over the four nodes as the colour kernel's lane sum does).  The
+// rows are written whole: no zero fill of the matrix, no scatter.  Per mesh: the element
+// geometry records and, per node, its column blocks (free neighbours and the offsets of their
+// first free columns).  b is added to (the traction kernels run first).
+constexpr int kStressRowCap = 48;     // entries of a row: up to 16 node blocks of 3 columns
+constexpr int kStressMaxInc = 32;     // incident elements of a node (more: the colour launches)
+constexpr int kStressMaxNbr = kStressRowCap / 3;
+
+struct StressRowArgs {
+    const uint32_t *nodes;      // nodes with a free component, ascending
+    uint32_t n_nodes;
+    const uint32_t *inc_ptr, *inc;   // per node: incident elements (colour-sorted index << 2 | local node)
+    const uint32_t *nbr_ptr;    // per node with a free component: its column blocks
+    const uint2 *nbr;           // (node, offset of its first free column | free mask << 8)
+    const double *geo;          // per colour-sorted element: kGeoWords values (geometry record)
+    const double *coef;         // per colour-sorted element: lambda, mu, rho, thermal factor / V
+    const uint32_t *book;
+    const uint32_t *rowptr;
+    double gx, gy, gz;
+    int thermal;
+    const double *u_node;
+    double *val, *b;
+};
+
+// Geometry record of an element (built once per mesh): the 12 gradient values g[q][d] at 3q + d,
+// the volume at 12, the free-component masks of its four nodes in the bytes of word 13, and the
+// four node ids in the last 16 bytes: one 128-byte line per element.
+constexpr uint32_t kGeoWords = 16;
+
+__global__ void k_stress_geometry(const double *__restrict__ xyz, const uint4 *__restrict__ conn,
+                                  const uint8_t *__restrict__ fmask, uint32_t ne, double *__restrict__ geo)
+{
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock) {
+        const uint4 nd4 = conn[e];
+        const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+        double X[4][3], g[4][3];
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) X[p][i] = xyz[3 * (uint64_t)nd[p] + i];
+        double *r = geo + (uint64_t)kGeoWords * e;
+        r[12] = tet_gradients(X, g);
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) r[3 * p + i] = g[p][i];
+        const uint32_t fm = fmask[nd[0]] | (fmask[nd[1]] << 8) | (fmask[nd[2]] << 16) | ((uint32_t)fmask[nd[3]] << 24);
+        reinterpret_cast<uint2 *>(r)[13] = make_uint2(fm, 0u);
+        reinterpret_cast<uint4 *>(r)[7] = nd4;
+    }
+}
+
+// Per assembly: each element's Lame constants, density and thermal factor (before the volume),
+// as the colour kernel computes them.
+__global__ void k_stress_coef(const uint32_t *__restrict__ perm, const double *__restrict__ E,
+                              const double *__restrict__ nu, const double *__restrict__ rho,
+                              const double *__restrict__ alpha_t, const double *__restrict__ dT, uint32_t ne,
+                              double *__restrict__ coef)
+{
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock) {
+        const uint32_t orig = perm[e];
+        const double Ee = E[orig], v = nu[orig];
+        double4 c;
+        c.x = Ee * v / ((1.0 + v) * (1.0 - 2.0 * v));
+        c.y = Ee / (2.0 * (1.0 + v));
+        c.z = rho[orig];
+        c.w = alpha_t ? alpha_t[orig] * dT[orig] * Ee / (1.0 - 2.0 * v) : 0.0;
+        reinterpret_cast<double4 *>(coef)[e] = c;
+    }
+}
+
+// One wavefront per node, four nodes per workgroup.  Phase 1: lane q < cnt loads incident
+// element q into LDS (gradients, volume, Lame constants, nodes, the node's local index) and
+// computes its right-hand-side terms.  Phase 2: lane (b, i) owns row i's entries of column block
+// b and sums, over the elements in incidence order, K_mn[i][j] of the elements that contain block
+// b's node (local n), computed from LDS with the colour kernel's expressions, in registers; lanes
+// 0-2 add the right-hand-side terms to b.
+constexpr int kStressWaves = kBlock / kWave;
+constexpr int kElemWords = 16;        // per element in LDS: g[12], V, lambda, mu, (unused)
+
+__device__ __forceinline__ double stress_k(const double *x, uint32_t m, uint32_t n, uint32_t i, uint32_t j)
+{
+    // K_mn[i][j] = V (lambda g_mi g_nj + mu g_mj g_ni + delta_ij mu g_m.g_n), as k_stress_colour
+    const double gmn = x[3 * m] * x[3 * n] + x[3 * m + 1] * x[3 * n + 1] + x[3 * m + 2] * x[3 * n + 2];
+    const double V = x[12], lam = x[13], mu = x[14];
+    return V * (lam * x[3 * m + i] * x[3 * n + j] + mu * x[3 * m + j] * x[3 * n + i] + (i == j ? mu * gmn : 0.0));
+}
+
+__global__ __launch_bounds__(kBlock) void k_stress_rows(StressRowArgs a)
+{
+    __shared__ double el[kStressWaves][kStressMaxInc][kElemWords];
+    __shared__ double bcon[kStressWaves][3][kStressMaxInc];   // load - lifting of rows i, element q
+    __shared__ uint4 enode[kStressWaves][kStressMaxInc];
+    __shared__ uint32_t elocal[kStressWaves][kStressMaxInc];
+    const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const uint32_t k = blockIdx.x * kStressWaves + w;
+    const bool live = k < a.n_nodes;                  // whole wavefronts
+    const uint32_t node = live ? a.nodes[k] : 0;
+    const uint32_t p0 = live ? a.inc_ptr[node] : 0, cnt = live ? a.inc_ptr[node + 1] - p0 : 0;
+    if (lane < cnt) {
+        const uint32_t wd = a.inc[p0 + lane], e = wd >> 2, m = wd & 3u;
+        const double2 *r = reinterpret_cast<const double2 *>(a.geo + (uint64_t)kGeoWords * e);
+        double2 t[7];
+#pragma unroll
+        for (int q = 0; q < 7; q++) t[q] = r[q];
+        const uint4 nd4 = reinterpret_cast<const uint4 *>(r)[7];
+        const double4 cf = reinterpret_cast<const double4 *>(a.coef)[e];
+        const uint32_t fm = reinterpret_cast<const uint2 *>(r)[13].x;
+        double *x = el[w][lane];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            x[2 * q] = t[q].x;
+            x[2 * q + 1] = t[q].y;
+        }
+        x[12] = t[6].x;
+        x[13] = cf.x;
+        x[14] = cf.y;
+        enode[w][lane] = nd4;
+        elocal[w][lane] = m;
+        double g[4][3];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            g[(2 * q) / 3][(2 * q) % 3] = t[q].x;
+            g[(2 * q + 1) / 3][(2 * q + 1) % 3] = t[q].y;
+        }
+        const double V = t[6].x, lam = cf.x, mu = cf.y;
+        double gm[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) gm[d] = m == 0 ? g[0][d] : m == 1 ? g[1][d] : m == 2 ? g[2][d] : g[3][d];
+        const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+        // the colour kernel's right-hand side of node m: load minus the lifting of the prescribed
+        // components, the lifting summed over j per node and then as (n0 + n2) + (n1 + n3)
+        double lift[4][3];
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const double gmn = gm[0] * g[n][0] + gm[1] * g[n][1] + gm[2] * g[n][2];
+            const uint32_t fmn = (fm >> (8 * n)) & 0xFFu;
+#pragma unroll
+            for (int i = 0; i < 3; i++) lift[n][i] = 0.0;
+            if (fmn == 7u) continue;                  // every component free: nothing to lift
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (!(fmn & (1u << j))) {
+                        const double K = V * (lam * gm[i] * g[n][j] + mu * gm[j] * g[n][i] + (i == j ? mu * gmn : 0.0));
+                        lift[n][i] += K * a.u_node[3 * (uint64_t)nd[n] + j];
+                    }
+        }
+        const double wq = cf.z * V * 0.25;
+        const double gv[3] = { a.gx, a.gy, a.gz };
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double fe = gv[i] * wq;
+            if (a.thermal) fe += cf.w * V * gm[i];
+            bcon[w][i][lane] = fe - ((lift[0][i] + lift[2][i]) + (lift[1][i] + lift[3][i]));
+        }
+    }
+    __syncthreads();
+    const uint32_t b0 = live ? a.nbr_ptr[k] : 0, nb = live ? a.nbr_ptr[k + 1] - b0 : 0;
+    if (lane < 3 * nb) {
+        const uint32_t bk = lane / 3, i = lane % 3;
+        const uint32_t row = a.book[3 * (uint64_t)node + i];
+        if (row != RS_EOD) {
+            const uint2 nb2 = a.nbr[b0 + bk];
+            const uint32_t nn = nb2.x, off = nb2.y & 0xFFu, fm = nb2.y >> 8;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (uint32_t q = 0; q < cnt; q++) {
+                const uint4 e4 = enode[w][q];
+                const uint32_t n = e4.x == nn ? 0u : e4.y == nn ? 1u : e4.z == nn ? 2u : e4.w == nn ? 3u : 4u;
+                if (n < 4u) {
+                    const double *x = el[w][q];
+                    const uint32_t m = elocal[w][q];
+                    s0 += stress_k(x, m, n, i, 0);
+                    s1 += stress_k(x, m, n, i, 1);
+                    s2 += stress_k(x, m, n, i, 2);
+                }
+            }
+            double *v = a.val + a.rowptr[row] + off;
+            uint32_t c = 0;
+            if (fm & 1u) v[c++] = s0;
+            if (fm & 2u) v[c++] = s1;
+            if (fm & 4u) v[c] = s2;
+        }
+    }
+    if (lane < 3 && live) {
+        const uint32_t row = a.book[3 * (uint64_t)node + lane];
+        if (row != RS_EOD) {
+            double bi = a.b[row];
+            for (uint32_t q = 0; q < cnt; q++) bi += bcon[w][lane][q];
+            a.b[row] = bi;
+        }
+    }
+}
+
 // RSolverStress::setDisplacement (src/rsolverstress.cpp:800-836): du = x on the enabled
 // components, 0 on the disabled ones; an active local rotation writes iR^T du
 // (RLocalRotation::rotateResultsVector, src/rlocalrotation.cpp:60-77).
@@ -1124,7 +1327,14 @@ struct rs_assembly {
     DBuf<uint4> rec_nodes;
     DBuf<uint64_t> rec_base;
     bool rows = false;                      // every row fits kRowCap entries
-    bool use_rows = true;                   // heat: row-gather kernel (else the colour launches)
+    bool use_rows = true;                   // row-gather kernels (else the colour launches)
+    // static stress row gather: nodes with a free component, their column blocks, per node its
+    // free-component mask, element geometry records and per-assembly coefficients
+    DBuf<uint32_t> s_nodes, s_nbr_ptr, s_nbr;
+    DBuf<uint8_t> s_fmask;
+    DBuf<double> s_geo, s_coef;
+    uint32_t s_count = 0;
+    bool srows = false;
     rs_matrix A;                          // borrowed by callers
     rs_matrix M;                          // modal stress: the mass matrix (A's pattern)
     bool has_mass = false;
@@ -1244,6 +1454,59 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
         for (uint32_t v = 0; v < n_nodes; v++)
             if (book[v] != RS_EOD) dof_node[book[v]] = v;
     }
+    // static stress row gather: node incidence (colour order), free-component masks, and each
+    // node's column blocks (the free neighbours and the offsets of their first free columns)
+    std::vector<uint32_t> s_nodes, s_nbr_ptr, s_nbr;
+    std::vector<uint8_t> s_fmask;
+    S->srows = dpn == 3 && max_row <= (uint32_t)kStressRowCap && n_tets < (1u << 30);
+    if (S->srows) {
+        inc_ptr.assign(n_nodes + 1ull, 0);
+        for (uint64_t s = 0; s < n_tets; s++)
+            for (int m = 0; m < 4; m++) inc_ptr[conn_s[4 * s + m] + 1]++;
+        for (uint32_t v = 0; v < n_nodes; v++) inc_ptr[v + 1] += inc_ptr[v];
+        inc.resize(4ull * n_tets);
+        std::vector<uint32_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (uint64_t s = 0; s < n_tets; s++)
+            for (uint32_t m = 0; m < 4; m++) inc[fill[conn_s[4 * s + m]]++] = (uint32_t)(s << 2) | m;
+        s_fmask.assign(n_nodes, 0);
+        std::vector<uint32_t> first_dof(n_nodes, RS_EOD);
+        for (uint32_t v = 0; v < n_nodes; v++)
+            for (int c = 2; c >= 0; c--)
+                if (book[3ull * v + c] != RS_EOD) {
+                    s_fmask[v] |= (uint8_t)(1u << c);
+                    first_dof[v] = book[3ull * v + c];
+                }
+        for (uint32_t v = 0; v < n_nodes; v++)
+            if (s_fmask[v]) s_nodes.push_back(v);
+        bool ok = true;
+        // column blocks of each node's rows: the nodes with a free component that share an
+        // element with it, ascending (the order of their columns in the rows)
+        s_nbr_ptr.assign(s_nodes.size() + 1, 0);
+        for (size_t k = 0; k < s_nodes.size() && ok; k++) {
+            const uint32_t v = s_nodes[k];
+            if (inc_ptr[v + 1] - inc_ptr[v] > (uint32_t)kStressMaxInc) ok = false;
+            std::vector<uint32_t> nb;
+            for (uint32_t q = inc_ptr[v]; q < inc_ptr[v + 1]; q++)
+                for (int n = 0; n < 4; n++) {
+                    const uint32_t u = conn_s[4ull * (inc[q] >> 2) + n];
+                    if (s_fmask[u]) nb.push_back(u);
+                }
+            std::sort(nb.begin(), nb.end());
+            nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+            if (nb.size() > (size_t)kStressMaxNbr) ok = false;
+            const uint32_t r = first_dof[v];
+            const uint32_t *lo = cl + rp[r], *hi = cl + rp[r + 1];
+            for (uint32_t u : nb) {
+                const uint32_t *it = std::lower_bound(lo, hi, first_dof[u]);
+                if (it == hi || *it != first_dof[u]) ok = false;
+                s_nbr.push_back(u);
+                s_nbr.push_back((uint32_t)(it - lo) | ((uint32_t)s_fmask[u] << 8));
+            }
+            s_nbr_ptr[k + 1] = (uint32_t)(s_nbr.size() / 2);
+        }
+        S->srows = ok;
+        S->s_count = (uint32_t)s_nodes.size();
+    }
     hipStream_t st = runtime().stream;
     uint64_t nnz = pat.col.size();
     if ((rc = S->xyz.alloc(3ull * n_nodes)) || (rc = S->conn.alloc(4ull * n_tets)) || (rc = S->perm.alloc(n_tets)) ||
@@ -1299,13 +1562,33 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
                 S->rec_elem.p, S->rec_slots.p, S->rec_nodes.p, S->rec_base.p, S->rec_count.p);
         RS_HIP(hipGetLastError());
     }
+    if (S->srows) {
+        if ((rc = S->inc_ptr.alloc(inc_ptr.size())) || (rc = S->inc.alloc(std::max<size_t>(inc.size(), 1))) ||
+            (rc = S->s_nodes.alloc(std::max<size_t>(s_nodes.size(), 1))) || (rc = S->s_nbr_ptr.alloc(s_nbr_ptr.size())) ||
+            (rc = S->s_nbr.alloc(std::max<size_t>(s_nbr.size(), 2))) ||
+            (rc = S->s_fmask.alloc(std::max(n_nodes, 1u))) || (rc = S->s_geo.alloc((uint64_t)kGeoWords * std::max(n_tets, 1u))) ||
+            (rc = S->s_coef.alloc(4ull * std::max(n_tets, 1u))))
+            return rc;
+        RS_HIP(hipMemcpyAsync(S->inc_ptr.p, inc_ptr.data(), sizeof(uint32_t) * inc_ptr.size(), hipMemcpyHostToDevice, st));
+        if (!inc.empty()) RS_HIP(hipMemcpyAsync(S->inc.p, inc.data(), sizeof(uint32_t) * inc.size(), hipMemcpyHostToDevice, st));
+        if (!s_nodes.empty())
+            RS_HIP(hipMemcpyAsync(S->s_nodes.p, s_nodes.data(), sizeof(uint32_t) * s_nodes.size(), hipMemcpyHostToDevice, st));
+        RS_HIP(hipMemcpyAsync(S->s_nbr_ptr.p, s_nbr_ptr.data(), sizeof(uint32_t) * s_nbr_ptr.size(), hipMemcpyHostToDevice, st));
+        if (!s_nbr.empty())
+            RS_HIP(hipMemcpyAsync(S->s_nbr.p, s_nbr.data(), sizeof(uint32_t) * s_nbr.size(), hipMemcpyHostToDevice, st));
+        if (n_nodes) RS_HIP(hipMemcpyAsync(S->s_fmask.p, s_fmask.data(), n_nodes, hipMemcpyHostToDevice, st));
+        if (n_tets)
+            k_stress_geometry<<<vblocks(n_tets), kBlock, 0, st>>>(S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p),
+                                                                 S->s_fmask.p, n_tets, S->s_geo.p);
+        RS_HIP(hipGetLastError());
+    }
     if ((rc = matrix_set_layout(A, pat.rowptr.data()))) return rc;
     RS_HIP(hipStreamSynchronize(st));
     *out = S.release();
     return RS_OK;
 }
 
-// Heat assembly method: 1 = row gather (default; needs rows of <= 16 entries), 0 = colour launches.
+// Assembly method: 1 = row gather (heat rows of <= 16 entries, static stress rows of <= 48), 0 = colour launches.
 extern "C" int rs_assembly_set_method(rs_assembly *S, int32_t row_gather)
 {
     if (!S) return fail(RS_ERR_INVALID_ARG, "rs_assembly_set_method: null");
@@ -1512,7 +1795,10 @@ static int stress_assemble(rs_assembly *S, const rs_stress_conf &conf, const dou
     a.dt = conf.dt;
     a.rot = S->has_rot ? S->rot.p : nullptr;
     a.rot_active = S->has_rot ? S->rot_active.p : nullptr;
-    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+    // the static, unrotated problem gathers rows (every value written once); the others scatter
+    // per colour into zeroed values
+    const bool gather = S->srows && S->use_rows && conf.problem == RS_STRESS_STATIC && !S->has_rot;
+    if (S->A.nnz && !gather) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
     if (modal && S->M.nnz) k_zero<<<vblocks(S->M.nnz), kBlock, 0, st>>>(S->M.val.p, S->M.nnz);
     if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(S->bvec.p, S->n_dof);
     // surfaces first, then volumes, as the reference's prepare() (src/rsolverstress.cpp:384, 573);
@@ -1524,10 +1810,34 @@ static int stress_assemble(rs_assembly *S, const rs_stress_conf &conf, const dou
                                                                             S->bvec.p, f0, f1,
                                                                             transient ? conf.dt : 1.0);
     }
-    constexpr uint32_t EPB = kBlock / 16;
-    for (uint32_t c = 0; c < S->n_colours; c++) {
-        uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
-        if (e1 > e0) k_stress_colour<<<grid_for(e1 - e0, EPB), kBlock, 0, st>>>(a, e0, e1);
+    if (gather) {
+        StressRowArgs r{};
+        r.nodes = S->s_nodes.p;
+        r.n_nodes = S->s_count;
+        r.inc_ptr = S->inc_ptr.p;
+        r.inc = S->inc.p;
+        r.nbr_ptr = S->s_nbr_ptr.p;
+        r.nbr = reinterpret_cast<const uint2 *>(S->s_nbr.p);
+        r.geo = S->s_geo.p;
+        r.coef = S->s_coef.p;
+        r.book = a.book;
+        r.rowptr = S->A.rowptr.p;
+        r.thermal = thermal ? 1 : 0;
+        r.gx = a.gx;
+        r.gy = a.gy;
+        r.gz = a.gz;
+        r.u_node = a.u_node;
+        r.val = a.val;
+        r.b = a.b;
+        if (ne) k_stress_coef<<<vblocks(ne), kBlock, 0, st>>>(a.perm, a.E, a.nu, a.rho, a.alpha_t, a.dT, (uint32_t)ne,
+                                                            S->s_coef.p);
+        if (S->s_count) k_stress_rows<<<grid_for(S->s_count, kStressWaves), kBlock, 0, st>>>(r);
+    } else {
+        constexpr uint32_t EPB = kBlock / 16;
+        for (uint32_t c = 0; c < S->n_colours; c++) {
+            uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+            if (e1 > e0) k_stress_colour<<<grid_for(e1 - e0, EPB), kBlock, 0, st>>>(a, e0, e1);
+        }
     }
     RS_HIP(hipGetLastError());
     if ((rc = timed_end(ev, S->last_ms))) return rc;
