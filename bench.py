@@ -1239,6 +1239,12 @@ def main() -> int:
                     "gbs": round(fused / (it_us * 1e-6) / 1e9, 1),
                     "frac": round(fused / (it_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
                     "note": "solve_ms / iterations of the FP64 Jacobi-CG, set-up included"}
+    # the solver's fused FP64 product (q = A p with the partial p.q sums) launched on its own: the
+    # same kernel that runs inside the CG iterations, outside their cache context
+    ms = asm.matrix.spmv_timed(rs.TIMED_CG_PRODUCT, reps=200)
+    spmv["fp64_cg_product"] = {"avg_us": round(ms * 1e3, 3), "bytes": spmv["fp64"]["bytes"],
+                               "gbs": round(spmv["fp64"]["bytes"] / (ms * 1e-3) / 1e9, 1),
+                               "shape": spmv["fp64"]["shape"]}
     ach = spmv["fp64"]["gbs"]
     traffic = MEASURED_TRAFFIC_128.get(spmv["fp64"]["shape"]) if (c == 128 and args.sigma == 2.0) else None
     roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -134,8 +134,10 @@ int rs_matrix_spmv_device(rs_matrix *A, const double *d_x, double *d_y);
  * fastest. */
 #define RS_SPMV_SHAPES 11
 /* Measurement: average device time (ms, stream events) of `reps` back-to-back SpMV launches
- * (precision RS_PRECISION_FP64: FP64 values/vectors; RS_PRECISION_MIXED: FP32) of shape
- * `plan` (-1 = the autotuned shape). */
+ * (precision RS_PRECISION_FP64: FP64 values/vectors; RS_PRECISION_MIXED: FP32;
+ * RS_TIMED_CG_PRODUCT: the FP64 CG iteration's fused product q = A p with its partial p.q sums,
+ * as the solver launches it) of shape `plan` (-1 = the autotuned FP64 / FP32 shape). */
+#define RS_TIMED_CG_PRODUCT 2
 int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t reps, int32_t plan,
                          double *avg_ms);
 /* The autotuned shape for `precision` and (optional) the tuning time of every shape

@@ -28,6 +28,7 @@ PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2, 3
 SOLVER_CG, SOLVER_GMRES = 0, 1
 STOP_REFERENCE, STOP_RELRES = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+TIMED_CG_PRODUCT = 2     # spmv_timed: the FP64 CG iteration's fused product (q = A p, partial p.q)
 SPMV_SHAPES = 11
 SPMV_SHAPE_NAMES = ["vector-csr g4", "vector-csr g8", "vector-csr g16", "vector-csr g32",
                     "vector-csr g64", "tile-csr 2048", "tile-csr 4096", "sell-64", "sell-64 int16",

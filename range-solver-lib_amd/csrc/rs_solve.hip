@@ -1332,14 +1332,41 @@ extern "C" int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t re
         *avg_ms = 0.0;
         return RS_OK;
     }
+    const bool fused = precision == RS_TIMED_CG_PRODUCT;
     bool fp32 = precision == RS_PRECISION_MIXED;
     if ((rc = matrix_tune(*A, fp32))) return rc;
     int id = plan < 0 ? A->plan[fp32 ? 1 : 0] : plan;
     DBuf<double> x, y;
     if ((rc = x.alloc(A->n)) || (rc = y.alloc(A->n))) return rc;
-    k_fill<<<vec_blocks(A->n), kBlock, 0, runtime().stream>>>(x.p, A->n, 1.0);
-    if (fp32) k_fill32<<<vec_blocks(A->n), kBlock, 0, runtime().stream>>>((float *)x.p, A->n, 1.0f);
-    return time_plan(*A, id, fp32, reps, x.p, y.p, *avg_ms);
+    hipStream_t st = runtime().stream;
+    k_fill<<<vec_blocks(A->n), kBlock, 0, st>>>(x.p, A->n, 1.0);
+    if (fp32) k_fill32<<<vec_blocks(A->n), kBlock, 0, st>>>((float *)x.p, A->n, 1.0f);
+    if (!fused) return time_plan(*A, id, fp32, reps, x.p, y.p, *avg_ms);
+    // the FP64 CG iteration's product as the solver launches it: q = A p with the per-workgroup
+    // partial sums of p.q, on a running solver state
+    DBuf<double> partials;
+    DBuf<SolveState> state;
+    if ((rc = partials.alloc(4 * 4096)) || (rc = state.alloc(1)) || (rc = matrix_sync_plan(*A, id, false))) return rc;
+    k_state_init<<<1, 64, 0, st>>>(state.p, A->frob, 0.0, 0.0, 1, RS_STOP_RELRES, 0, 0);
+    uint32_t nb = 0;
+    CgArgs a{};
+    a.A = plan_view(*A, id, nb);
+    a.p = x.p;
+    a.q = y.p;
+    a.partials = partials.p;
+    a.st = state.p;
+    RS_SHAPE_SWITCH(id, k_cg_spmv_dot, nb, st, a);
+    EventPair ev;
+    RS_HIP(ev.create());
+    RS_HIP(hipEventRecord(ev.e[0], st));
+    for (uint32_t i = 0; i < reps; i++) RS_SHAPE_SWITCH(id, k_cg_spmv_dot, nb, st, a);
+    RS_HIP(hipEventRecord(ev.e[1], st));
+    RS_HIP(hipEventSynchronize(ev.e[1]));
+    RS_HIP(hipGetLastError());
+    float f = 0.0f;
+    RS_HIP(hipEventElapsedTime(&f, ev.e[0], ev.e[1]));
+    *avg_ms = f / reps;
+    return RS_OK;
 }
 
 extern "C" int rs_matrix_spmv_plan(rs_matrix *A, int32_t precision, int32_t *plan, double *plan_ms)
