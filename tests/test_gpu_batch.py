@@ -49,7 +49,7 @@ def test_batch_transient_matches_oracle(gpu, orc, tr, precision):
     prec = rs.PRECISION_FP64 if precision == "fp64" else rs.PRECISION_MIXED
     cvg = 1e-10 if precision == "fp64" else 1e-8
     _say(f"batch transient {precision}: sizes {SIZES}, {steps} steps")
-    B, meshes, _ = tr.build_batch(rs, SIZES, sigma=sigma, dt=dt, seed=seed)
+    B, meshes, _, _ = tr.build_batch(rs, SIZES, sigma=sigma, dt=dt, seed=seed)
     info = B.solve(_conf(rs, prec, cvg), steps)
     its, rr, reasons = B.stats()
     assert info.systems_converged == len(SIZES), reasons
@@ -142,7 +142,7 @@ def test_batch_zero_rhs_and_initial_solution(gpu):
 def test_batch_warm_start_shapes_and_determinism(gpu, tr):
     rs = gpu
     sizes = [4, 7, 3, 6, 5, 8, 2, 7]
-    B, _, _ = tr.build_batch(rs, sizes, sigma=1.5, dt=1e-2, seed=21)
+    B, _, _, _ = tr.build_batch(rs, sizes, sigma=1.5, dt=1e-2, seed=21)
     conf = _conf(rs, rs.PRECISION_MIXED, 1e-8)
     info = B.solve(conf, 5)
     its0, _, _ = B.stats()
@@ -216,7 +216,7 @@ def test_batch_column_offsets_match_full_columns(gpu, tr, monkeypatch):
     for full in (False, True):
         if full:
             monkeypatch.setenv("RS_BATCH_FULL_COLUMNS", "1")
-        B, _, _ = tr.build_batch(rs, sizes, sigma=1.5, dt=1e-2, seed=33)
+        B, _, _, _ = tr.build_batch(rs, sizes, sigma=1.5, dt=1e-2, seed=33)
         for prec in (rs.PRECISION_FP64, rs.PRECISION_MIXED):
             B.solve(_conf(rs, prec, 1e-9), 3)
             out[(full, prec)] = (B.stats()[0].copy(), [B.solution(i) for i in range(len(sizes))])
@@ -231,7 +231,7 @@ def test_batch_column_offsets_match_full_columns(gpu, tr, monkeypatch):
 def test_batch_replacement_ratio(gpu, tr):
     """Fewer FP64 replacements with a smaller ratio; the steps still end below the tolerance."""
     rs = gpu
-    B, _, _ = tr.build_batch(rs, [6, 4, 7], sigma=1.5, dt=1e-2, seed=41)
+    B, _, _, _ = tr.build_batch(rs, [6, 4, 7], sigma=1.5, dt=1e-2, seed=41)
     conf = _conf(rs, rs.PRECISION_MIXED, 1e-8)
     reps = []
     for ratio in (0.1, 1e-2, 1e-3):
