@@ -136,11 +136,14 @@ def oracle_leg(rs, xyz, tets, fixed, t_node, k, q, tol: float, reps: int):
 def spmv_only(rs, asm, dk, dq, dT, n, nnz) -> int:
     lib = rs.load()
     db = rs.DeviceArray(n)
-    asm.assemble_heat_device(dk, dq, dT, db)
+    asm_ms = []
+    for _ in range(5):
+        asm.assemble_heat_device(dk, dq, dT, db)
+        asm_ms.append(asm.last_ms)
     copy_bytes = 256 << 20
     src, dst = rs.DeviceArray(copy_bytes // 8), rs.DeviceArray(copy_bytes // 8)
     rs._check(lib.rs_memcpy_dtod(dst.ptr, src.ptr, copy_bytes))
-    out = {"copy_bytes": copy_bytes}
+    out = {"copy_bytes": copy_bytes, "heat_assembly_ms": round(float(np.median(asm_ms)), 4)}
     for name, prec, sv, sx in (("fp64", rs.PRECISION_FP64, 8, 8), ("fp32", rs.PRECISION_MIXED, 4, 4)):
         plan, _ = asm.matrix.spmv_plan(prec)
         ms = asm.matrix.spmv_timed(prec, reps=20)
@@ -199,6 +202,24 @@ def elasticity(rs, args) -> int:
     asm_kernel_ms = asm.last_ms
     log(f"dofs={n} nnz={asm.nnz} colours={asm.n_colours} assembly {asm.last_ms:.1f} ms (kernels) "
         f"{t_asm * 1e3:.0f} ms (with host transfers)")
+    if args.assembly_only:
+        # profiling mode: a 256 MiB device copy (counter calibration), then 5 assemblies by the row
+        # gather and 5 by the colour launches
+        lib = rs.load()
+        copy_bytes = 256 << 20
+        src, dst = rs.DeviceArray(copy_bytes // 8), rs.DeviceArray(copy_bytes // 8)
+        rs._check(lib.rs_memcpy_dtod(dst.ptr, src.ptr, copy_bytes))
+        out = {"copy_bytes": copy_bytes,
+               "algorithmic_bytes": int(ne * (16 + 8) + nn * 24 + asm.nnz * 8 + n * 8)}
+        for name, rows in (("row_gather", True), ("colour_launches", False)):
+            asm.set_method(rows)
+            ms = []
+            for _ in range(5):
+                asm.assemble_stress(E, nu, rho, [0.0, 0.0, 0.0], np.zeros(3 * nn), tris, trac)
+                ms.append(asm.last_ms)
+            out[name] = {"assembly_ms": round(float(np.median(ms)), 4)}
+        print(json.dumps(out), flush=True)
+        return 0
     db, dx = rs.DeviceArray.from_host(b), rs.DeviceArray(n)
     dzero = rs.DeviceArray.from_host(np.zeros(n))
     lib = rs.load()
@@ -1090,6 +1111,9 @@ def main() -> int:
                     help="profiling mode: time only the learned method (no FP64/mixed CG, no ablation)")
     ap.add_argument("--pre", type=int, default=1, help="pre-smoothing steps of the V-cycle")
     ap.add_argument("--post", type=int, default=1, help="post-smoothing steps of the V-cycle")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="elasticity profiling mode: a 256 MiB device copy (counter calibration), then 5 "
+                         "stress assemblies by the row gather and 5 by the colour launches; no solves")
     ap.add_argument("--spmv-only", action="store_true",
                     help="profiling mode: a 256 MiB device copy (counter calibration), then "
                          "20 FP64 and 20 FP32 SpMV launches of the tuned shapes; no solves")

@@ -163,6 +163,8 @@ struct RowRecords {
     const uint32_t *count;      // per row: records
 };
 
+typedef double rec_d2 __attribute__((ext_vector_type(2)));
+
 struct HeatRowArgs {
     HeatArgs h;
     const uint32_t *rowptr;
@@ -218,7 +220,9 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
 {
     __shared__ double acc[kRowCap][kBlock];
     const uint32_t t = threadIdx.x;
-    const uint32_t i = blockIdx.x * kBlock + t;
+    // neighbouring row ranges on one XCD: the k, q entries that rows of adjacent node lines
+    // gather stay in that XCD's L2
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + t;
     if (i >= a.n_rows) return;
     const HeatArgs &h = a.h;
     const RowRecords &R = a.rec;
@@ -228,16 +232,31 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
     double bi = 0.0;
     const uint64_t b = R.base[i / kWave] + (i % kWave);
     const uint32_t cnt = R.count[i];
-    for (uint32_t k = 0; k < cnt; k++) {
+    // one record: its fields, loaded before any of them is used
+    struct Rec {
+        double gg[4], V, kk, qq;
+        uint32_t orig, m, sl;
+    };
+    auto load = [&](uint32_t k, Rec &c) {
         const uint64_t r = b + (uint64_t)k * kWave;
-        const double2 g01 = reinterpret_cast<const double2 *>(R.grad)[2 * r];
-        const double2 g23 = reinterpret_cast<const double2 *>(R.grad)[2 * r + 1];
-        const double gg[4] = { g01.x, g01.y, g23.x, g23.y };
-        const double V = R.vol[r];
-        const uint32_t em = R.elem[r], orig = em & 0x3FFFFFFFu, m = em >> 30;
-        const uint32_t sl = R.slots[r];
-        const double kV = h.k[orig] * V;
-        const double fq = h.q[orig] * V * 0.25;
+        // the records stream once per assembly (non-temporal), so the gathered k and q lines,
+        // each shared by the rows of an element's four nodes, keep their place in L2
+        const rec_d2 g01 = __builtin_nontemporal_load(reinterpret_cast<const rec_d2 *>(R.grad) + 2 * r);
+        const rec_d2 g23 = __builtin_nontemporal_load(reinterpret_cast<const rec_d2 *>(R.grad) + 2 * r + 1);
+        c.gg[0] = g01.x;
+        c.gg[1] = g01.y;
+        c.gg[2] = g23.x;
+        c.gg[3] = g23.y;
+        c.V = __builtin_nontemporal_load(R.vol + r);
+        const uint32_t em = __builtin_nontemporal_load(R.elem + r);
+        c.orig = em & 0x3FFFFFFFu;
+        c.m = em >> 30;
+        c.sl = __builtin_nontemporal_load(R.slots + r);
+    };
+    auto process = [&](uint32_t k, const Rec &c) {
+        const uint64_t r = b + (uint64_t)k * kWave;
+        const double V = c.V, kV = c.kk * V, fq = c.qq * V * 0.25;
+        const uint32_t sl = c.sl, m = c.m;
         double Ae[4], be;
         const bool lift = ((sl & 0xFFu) == kSlotFixed) | (((sl >> 8) & 0xFFu) == kSlotFixed) |
                           (((sl >> 16) & 0xFFu) == kSlotFixed) | ((sl >> 24) == kSlotFixed);
@@ -245,11 +264,11 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
         if (h.transient || lift) nd4 = R.nodes[r];
         const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
         if (h.transient) {
-            const double mc = h.rhoc[orig] * V / 20.0;
+            const double mc = h.rhoc[c.orig] * V / 20.0;
             be = h.dt * fq;
 #pragma unroll
             for (int n = 0; n < 4; n++) {
-                const double K = kV * gg[n];
+                const double K = kV * c.gg[n];
                 const double M = mc * ((uint32_t)n == m ? 2.0 : 1.0);
                 Ae[n] = M + h.alpha * h.dt * K;
                 be += (M - (1.0 - h.alpha) * h.dt * K) * h.t_node[nd[n]];
@@ -257,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
         } else {
             be = fq;
 #pragma unroll
-            for (int n = 0; n < 4; n++) Ae[n] = kV * gg[n];
+            for (int n = 0; n < 4; n++) Ae[n] = kV * c.gg[n];
         }
         // Dirichlet lifting (src/rsolverheat.cpp:787-798), in the colour kernel's order
 #pragma unroll
@@ -269,6 +288,20 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(HeatRowArgs a)
             const uint32_t slot = (sl >> (8 * n)) & 0xFFu;
             if (slot != kSlotFixed) acc[slot][t] += Ae[n];
         }
+    };
+    // two records per trip: both records and both coefficient gathers are in flight before the
+    // first is processed; the sums still run in record order
+    for (uint32_t k = 0; k < cnt; k += 2) {
+        const bool two = k + 1 < cnt;
+        Rec c0, c1;
+        load(k, c0);
+        load(two ? k + 1 : k, c1);
+        c0.kk = h.k[c0.orig];
+        c0.qq = h.q[c0.orig];
+        c1.kk = h.k[c1.orig];
+        c1.qq = h.q[c1.orig];
+        process(k, c0);
+        if (two) process(k + 1, c1);
     }
     for (uint32_t j = 0; j < len; j++) h.val[r0 + j] = acc[j][t];
     h.b[i] = bi;
@@ -1052,7 +1085,7 @@ struct StressRowArgs {
     const uint32_t *nbr_ptr;    // per node with a free component: its column blocks
     const uint2 *nbr;           // (node, offset of its first free column | free mask << 8)
     const double *geo;          // per colour-sorted element: kGeoWords values (geometry record)
-    const double *coef;         // per colour-sorted element: lambda, mu, rho, thermal factor / V
+    const double *coef;         // per original element: lambda, mu, rho, thermal factor / V
     const uint32_t *book;
     const uint32_t *rowptr;
     double gx, gy, gz;
@@ -1062,12 +1095,14 @@ struct StressRowArgs {
 };
 
 // Geometry record of an element (built once per mesh): the 12 gradient values g[q][d] at 3q + d,
-// the volume at 12, the free-component masks of its four nodes in the bytes of word 13, and the
-// four node ids in the last 16 bytes: one 128-byte line per element.
+// the volume at 12, the free-component masks of its four nodes in the low 4 bytes of word 13 and
+// its original element id in the high 4, and the four node ids in the last 16 bytes: one 128-byte
+// line per element.
 constexpr uint32_t kGeoWords = 16;
 
 __global__ void k_stress_geometry(const double *__restrict__ xyz, const uint4 *__restrict__ conn,
-                                  const uint8_t *__restrict__ fmask, uint32_t ne, double *__restrict__ geo)
+                                  const uint32_t *__restrict__ perm, const uint8_t *__restrict__ fmask, uint32_t ne,
+                                  double *__restrict__ geo)
 {
     for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock) {
         const uint4 nd4 = conn[e];
@@ -1084,26 +1119,25 @@ __global__ void k_stress_geometry(const double *__restrict__ xyz, const uint4 *_
 #pragma unroll
             for (int i = 0; i < 3; i++) r[3 * p + i] = g[p][i];
         const uint32_t fm = fmask[nd[0]] | (fmask[nd[1]] << 8) | (fmask[nd[2]] << 16) | ((uint32_t)fmask[nd[3]] << 24);
-        reinterpret_cast<uint2 *>(r)[13] = make_uint2(fm, 0u);
+        reinterpret_cast<uint2 *>(r)[13] = make_uint2(fm, perm[e]);
         reinterpret_cast<uint4 *>(r)[7] = nd4;
     }
 }
 
 // Per assembly: each element's Lame constants, density and thermal factor (before the volume),
-// as the colour kernel computes them.
-__global__ void k_stress_coef(const uint32_t *__restrict__ perm, const double *__restrict__ E,
-                              const double *__restrict__ nu, const double *__restrict__ rho,
-                              const double *__restrict__ alpha_t, const double *__restrict__ dT, uint32_t ne,
-                              double *__restrict__ coef)
+// as the colour kernel computes them, in the caller's element order (coalesced reads and
+// writes; the row kernel finds an element's entry through the original id in its record).
+__global__ void k_stress_coef(const double *__restrict__ E, const double *__restrict__ nu,
+                              const double *__restrict__ rho, const double *__restrict__ alpha_t,
+                              const double *__restrict__ dT, uint32_t ne, double *__restrict__ coef)
 {
     for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock) {
-        const uint32_t orig = perm[e];
-        const double Ee = E[orig], v = nu[orig];
+        const double Ee = E[e], v = nu[e];
         double4 c;
         c.x = Ee * v / ((1.0 + v) * (1.0 - 2.0 * v));
         c.y = Ee / (2.0 * (1.0 + v));
-        c.z = rho[orig];
-        c.w = alpha_t ? alpha_t[orig] * dT[orig] * Ee / (1.0 - 2.0 * v) : 0.0;
+        c.z = rho[e];
+        c.w = alpha_t ? alpha_t[e] * dT[e] * Ee / (1.0 - 2.0 * v) : 0.0;
         reinterpret_cast<double4 *>(coef)[e] = c;
     }
 }
@@ -1132,7 +1166,7 @@ __global__ __launch_bounds__(kBlock) void k_stress_rows(StressRowArgs a)
     __shared__ uint4 enode[kStressWaves][kStressMaxInc];
     __shared__ uint32_t elocal[kStressWaves][kStressMaxInc];
     const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const uint32_t k = blockIdx.x * kStressWaves + w;
+    const uint32_t k = xcd_remap(blockIdx.x, gridDim.x) * kStressWaves + w;
     const bool live = k < a.n_nodes;                  // whole wavefronts
     const uint32_t node = live ? a.nodes[k] : 0;
     const uint32_t p0 = live ? a.inc_ptr[node] : 0, cnt = live ? a.inc_ptr[node + 1] - p0 : 0;
@@ -1143,8 +1177,10 @@ __global__ __launch_bounds__(kBlock) void k_stress_rows(StressRowArgs a)
 #pragma unroll
         for (int q = 0; q < 7; q++) t[q] = r[q];
         const uint4 nd4 = reinterpret_cast<const uint4 *>(r)[7];
-        const double4 cf = reinterpret_cast<const double4 *>(a.coef)[e];
-        const uint32_t fm = reinterpret_cast<const uint2 *>(r)[13].x;
+        // word 13: the nodes' free-component masks and the original element id
+        const uint64_t w13 = (uint64_t)__double_as_longlong(t[6].y);
+        const uint32_t fm = (uint32_t)w13;
+        const double4 cf = reinterpret_cast<const double4 *>(a.coef)[w13 >> 32];
         double *x = el[w][lane];
 #pragma unroll
         for (int q = 0; q < 6; q++) {
@@ -1300,6 +1336,14 @@ __global__ void k_gather(const double *__restrict__ src, const uint32_t *__restr
 static uint32_t grid_for(uint64_t items, uint32_t per_block)
 {
     return (uint32_t)std::max<uint64_t>(1, (items + per_block - 1) / per_block);
+}
+
+// A grid whose size is a multiple of the XCD count, for kernels that map blockIdx through
+// xcd_remap (extra workgroups find no rows and exit).
+static uint32_t grid_xcd(uint64_t items, uint32_t per_block)
+{
+    const uint32_t g = grid_for(items, per_block);
+    return (g + kXcd - 1) / kXcd * kXcd;
 }
 
 static uint32_t vblocks(uint64_t n)
@@ -1579,7 +1623,7 @@ extern "C" int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t 
         if (n_nodes) RS_HIP(hipMemcpyAsync(S->s_fmask.p, s_fmask.data(), n_nodes, hipMemcpyHostToDevice, st));
         if (n_tets)
             k_stress_geometry<<<vblocks(n_tets), kBlock, 0, st>>>(S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p),
-                                                                 S->s_fmask.p, n_tets, S->s_geo.p);
+                                                                 S->perm.p, S->s_fmask.p, n_tets, S->s_geo.p);
         RS_HIP(hipGetLastError());
     }
     if ((rc = matrix_set_layout(A, pat.rowptr.data()))) return rc;
@@ -1656,7 +1700,7 @@ extern "C" int rs_assemble_heat_device(rs_assembly *S, const double *d_k, const 
         RowRecords rec{ S->rec_grad.p, S->rec_vol.p, S->rec_elem.p, S->rec_slots.p, S->rec_nodes.p, S->rec_base.p,
                         S->rec_count.p };
         HeatRowArgs ra{ a, S->A.rowptr.p, rec, S->n_dof };
-        if (S->n_dof) k_heat_rows<<<grid_for(S->n_dof, kBlock), kBlock, 0, st>>>(ra);
+        if (S->n_dof) k_heat_rows<<<grid_xcd(S->n_dof, kBlock), kBlock, 0, st>>>(ra);
     } else {
         if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
         if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
@@ -1829,9 +1873,9 @@ static int stress_assemble(rs_assembly *S, const rs_stress_conf &conf, const dou
         r.u_node = a.u_node;
         r.val = a.val;
         r.b = a.b;
-        if (ne) k_stress_coef<<<vblocks(ne), kBlock, 0, st>>>(a.perm, a.E, a.nu, a.rho, a.alpha_t, a.dT, (uint32_t)ne,
+        if (ne) k_stress_coef<<<vblocks(ne), kBlock, 0, st>>>(a.E, a.nu, a.rho, a.alpha_t, a.dT, (uint32_t)ne,
                                                             S->s_coef.p);
-        if (S->s_count) k_stress_rows<<<grid_for(S->s_count, kStressWaves), kBlock, 0, st>>>(r);
+        if (S->s_count) k_stress_rows<<<grid_xcd(S->s_count, kStressWaves), kBlock, 0, st>>>(r);
     } else {
         constexpr uint32_t EPB = kBlock / 16;
         for (uint32_t c = 0; c < S->n_colours; c++) {
