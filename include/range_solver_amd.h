@@ -231,6 +231,10 @@ int rs_assembly_create(uint32_t n_nodes, const double *xyz, uint32_t n_tets,
 int rs_assembly_info(const rs_assembly *S, uint32_t *n_dof, uint64_t *nnz,
                      uint32_t *n_colours);
 int rs_assembly_book(const rs_assembly *S, uint32_t *book);
+/* The assembly's device copies of the mesh coordinates (3 * n_nodes) and of the DOF book
+ * (n_nodes * dofs_per_node), valid while S lives (input of rs_learned_create_by_bins). */
+int rs_assembly_device_mesh(const rs_assembly *S, const double **d_xyz, const uint32_t **d_book,
+                            uint32_t *n_nodes, uint32_t *dofs_per_node);
 /* Device time (ms, stream events) of the last rs_assemble_* call, colour launches only. */
 int rs_assembly_last_ms(const rs_assembly *S, double *ms);
 /* The assembled matrix (owned by the assembly; do not destroy). */
@@ -484,6 +488,16 @@ int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_
 /* Hierarchy for the pattern of A: n_levels coarse levels given by their maps (as above). */
 int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const uint32_t *agg,
                       const uint32_t *n_agg, rs_learned **out);
+/* The same hierarchy built on the GPU from the mesh in device memory: the nested coordinate
+ * bins of rs_hierarchy_by_bins (identical level maps) and every list of rs_learned_create.
+ * d_xyz: 3 * n_nodes coordinates; d_book: n_nodes * dofs_per_node positions, numbering the
+ * rows of A (0 .. n-1) or RS_EOD.  max_coarse <= 512, max_levels <= 16. */
+int rs_learned_create_by_bins(const rs_matrix *A, uint32_t n_nodes, const double *d_xyz,
+                              uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx,
+                              uint32_t nby, uint32_t nbz, uint32_t max_coarse,
+                              uint32_t max_levels, rs_learned **out);
+/* Level map of level `level` (0 .. n_levels-2) to the next: one entry per row of the level. */
+int rs_learned_level_map(const rs_learned *L, uint32_t level, uint32_t *agg, uint64_t capacity);
 /* Network parameters, FP32, 49,921 values (layout: W_in[4][64], b_in[64], 6 x {W_s[64][64],
  * W_n[64][64], b[64]}, w_out[64], b_out). */
 int rs_learned_set_params(rs_learned *L, const float *params, uint64_t count);

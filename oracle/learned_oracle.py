@@ -168,6 +168,45 @@ def prolongation(m, n_next):
     return sp.csr_matrix((np.ones(m.size), (np.arange(m.size), m)), shape=(m.size, n_next))
 
 
+def bins_hierarchy(xyz, book, dpn, bins, max_coarse, max_levels=16, eod=0xFFFFFFFF):
+    """Nested coordinate bins over the enabled positions of `book` (numpy restatement that pins
+    the library's host and GPU builders).  Level k bins the nodes of level k (level 0: the DOFs,
+    each at its node's coordinates; a coarser node at the coordinates of its first member) into
+    bins >> k cells per axis of the bounding box of all nodes; ids in first-appearance order;
+    a level that merges nothing is skipped (the bins halve again).  Returns the level maps."""
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+    book = np.asarray(book, dtype=np.uint64).ravel()
+    pos = np.nonzero(book != eod)[0]
+    n_dof = pos.size
+    node_of = np.empty(n_dof, dtype=np.int64)
+    node_of[book[pos].astype(np.int64)] = pos // dpn
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+    rep = np.arange(n_dof)
+    cur, maps, k = n_dof, [], 0
+    while cur > max_coarse or not maps:
+        if len(maps) == max_levels:
+            raise ValueError("coarsest level still too large")
+        nb = np.maximum(1, np.asarray(bins, dtype=np.int64) >> min(k, 31))
+        k += 1
+        p = xyz[node_of[rep]]
+        span = hi - lo
+        t = np.where(span > 0, (p - lo) / np.where(span > 0, span, 1.0), 0.0)
+        c = np.minimum(nb - 1, (t * nb).astype(np.uint64).astype(np.int64))
+        b = c[:, 0] + nb[0] * (c[:, 1] + nb[1] * c[:, 2])
+        _, first, inv = np.unique(b, return_index=True, return_inverse=True)
+        order = np.argsort(first)
+        rank = np.empty_like(order)
+        rank[order] = np.arange(order.size)
+        if order.size == cur:
+            if (nb == 1).all():
+                raise ValueError("cannot coarsen further")
+            continue
+        maps.append(rank[inv.ravel()].astype(np.uint32))
+        rep = rep[first[order]]
+        cur = order.size
+    return maps
+
+
 def hierarchy(A, maps):
     """Level operators [A_0, A_1, ...] and prolongations for level maps (scipy sparse A)."""
     levels, Ps = [A.tocsr()], []

@@ -173,6 +173,10 @@ _SIGS = {
     "rs_hierarchy_by_bins": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                              C.c_uint32, C.c_uint32, _P, C.c_uint64, _P, _u32p],
     "rs_learned_create": [_P, C.c_uint32, _P, _P, C.POINTER(_P)],
+    "rs_learned_create_by_bins": [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_uint32, C.c_uint32, C.POINTER(_P)],
+    "rs_learned_level_map": [_P, C.c_uint32, _P, C.c_uint64],
+    "rs_assembly_device_mesh": [_P, C.POINTER(_P), C.POINTER(_P), _u32p, _u32p],
     "rs_learned_set_params": [_P, _P, C.c_uint64],
     "rs_learned_set_cycle": [_P, C.c_uint32, C.c_uint32, C.c_int],
     "rs_learned_set_compressed": [_P, C.c_uint32],
@@ -999,6 +1003,32 @@ class LearnedPreconditioner:
         self._nagg = np.asarray(sizes, np.uint32)
         self.h = C.c_void_p()
         _check(_lib.rs_learned_create(A.h, len(maps), _ptr(self._agg), _ptr(self._nagg), C.byref(self.h)))
+        self._init_info(params)
+
+    @classmethod
+    def from_bins(cls, A: SparseMatrix, asm, bins, max_coarse: int = 512, max_levels: int = 16,
+                  params=None) -> "LearnedPreconditioner":
+        """The hierarchy of hierarchy_by_bins(xyz, book, ...) over the assembly's mesh, built on the
+        GPU from the assembly's device coordinates and DOF book (no host work over the mesh)."""
+        load()
+        self = cls.__new__(cls)
+        self.A = A
+        xp, bp = _P(), _P()
+        nn, dpn = C.c_uint32(), C.c_uint32()
+        _check(_lib.rs_assembly_device_mesh(asm.h, C.byref(xp), C.byref(bp), C.byref(nn), C.byref(dpn)))
+        self.h = C.c_void_p()
+        _check(_lib.rs_learned_create_by_bins(A.h, nn.value, xp, dpn.value, bp, int(bins[0]), int(bins[1]),
+                                              int(bins[2]), max_coarse, max_levels, C.byref(self.h)))
+        self._init_info(params)
+        return self
+
+    def level_map(self, level: int) -> np.ndarray:
+        """Map of level `level` (0 = the matrix rows) to the next level."""
+        out = np.empty(max(int(self.level_rows[level]), 1), np.uint32)
+        _check(_lib.rs_learned_level_map(self.h, level, _ptr(out), out.size))
+        return out[:int(self.level_rows[level])]
+
+    def _init_info(self, params) -> None:
         nl = C.c_uint32()
         rows = np.zeros(32, np.uint32)
         nnz = np.zeros(32, np.uint64)

@@ -1663,6 +1663,17 @@ extern "C" int rs_assembly_book(const rs_assembly *S, uint32_t *book)
     return RS_OK;
 }
 
+extern "C" int rs_assembly_device_mesh(const rs_assembly *S, const double **d_xyz, const uint32_t **d_book,
+                                      uint32_t *n_nodes, uint32_t *dofs_per_node)
+{
+    if (!S || !d_xyz || !d_book || !n_nodes || !dofs_per_node) return fail(RS_ERR_INVALID_ARG, "rs_assembly_device_mesh: null");
+    *d_xyz = S->xyz.p;
+    *d_book = S->dbook.p;
+    *n_nodes = S->n_nodes;
+    *dofs_per_node = S->dpn;
+    return RS_OK;
+}
+
 extern "C" int rs_assembly_matrix(rs_assembly *S, rs_matrix **A)
 {
     if (!S || !A) return fail(RS_ERR_INVALID_ARG, "rs_assembly_matrix: null");

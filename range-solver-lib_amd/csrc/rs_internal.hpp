@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -153,6 +154,25 @@ int matrix_frobenius(Matrix &A);   // device reduction, sets A.frob
 int matrix_refresh_fp32(Matrix &A);
 int matrix_set_layout(Matrix &A, const uint32_t *host_rowptr);
 int matrix_tune(Matrix &A, bool fp32);
+// A matrix with the given device pattern (taken over: rowptr and col are left empty) and zero
+// values.
+int matrix_create_device(uint32_t n, DBuf<uint32_t> &rowptr, DBuf<uint32_t> &col, uint64_t nnz, rs_matrix **out);
+
+// ----------------------------------------------------------------- aggregation hierarchy (GPU)
+// Nested coordinate bins over the DOFs of a mesh (device coordinates and DOF book), halved per
+// level until a level has <= max_coarse nodes: maps[k] sends the nodes of level k to level k + 1.
+int hier_bins(uint32_t n_nodes, const double *d_xyz, uint32_t dpn, const uint32_t *d_book, uint32_t n_dof,
+              const uint32_t bins[3], uint32_t max_coarse, uint32_t max_levels,
+              std::vector<std::unique_ptr<DBuf<uint32_t>>> &maps, std::vector<uint32_t> &n_agg);
+// Members of each of the nn aggregates of d_map (n entries), rows ascending: ptr (nn + 1), rows (n).
+int hier_members(const uint32_t *d_map, uint32_t n, uint32_t nn, DBuf<uint32_t> &ptr, DBuf<uint32_t> &rows);
+// Next-level pattern (cptr, ccol) of a level matrix under d_map, and the gather lists: the entries
+// of the level matrix summed into each next-level entry, by position (gptr: cnnz + 1, gsrc: nnz).
+int hier_level_lists(const uint32_t *d_rowptr, const uint32_t *d_col, uint32_t n, uint64_t nnz, const uint32_t *d_map,
+                     uint32_t nn, DBuf<uint32_t> &cptr, DBuf<uint32_t> &ccol, DBuf<uint32_t> &gptr,
+                     DBuf<uint32_t> &gsrc, uint32_t &cnnz);
+int hier_compose(uint32_t *d_total, uint64_t n, const uint32_t *d_map);   // total[i] = map[total[i]]
+int hier_identity(uint32_t *d_a, uint64_t n);
 
 }  // namespace rs
 

@@ -1696,6 +1696,107 @@ extern "C" int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_
     return RS_OK;
 }
 
+namespace rs {
+
+// The hierarchy from device level maps (maps[l] sends the nodes of level l to level l + 1):
+// members, next-level patterns and gather lists on the GPU; only the coarsest pattern (at most
+// 512 rows) comes to the host, for the network's 16 x 16 blocks.
+static int learned_build(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<uint32_t>>> &maps,
+                         const std::vector<uint32_t> &n_agg, rs_learned **out)
+{
+    const uint32_t n_levels = (uint32_t)maps.size();
+    if (n_levels == 0 || n_levels != n_agg.size()) return fail(RS_ERR_INVALID_ARG, "learned hierarchy: no levels");
+    const uint32_t nc = n_agg.back();
+    if (nc == 0 || nc > kMaxCoarse) return fail(RS_ERR_UNSUPPORTED, "rs_learned_create: coarsest level must have 1..512 nodes");
+    int rc;
+    hipStream_t st = runtime().stream;
+    std::unique_ptr<rs_learned> L(new rs_learned());
+    L->n = A->n;
+    L->nc = nc;
+    L->nlev = n_levels + 1;
+    L->owner = A;
+    L->lv.emplace_back(new CycLevel());
+    L->lv[0]->M = const_cast<rs_matrix *>(A);
+    L->lv[0]->n = A->n;
+    DBuf<uint32_t> total;
+    if ((rc = total.alloc(std::max(A->n, 1u))) || (rc = hier_identity(total.p, A->n))) return rc;
+    for (uint32_t l = 0; l < n_levels; l++) {
+        CycLevel &V = *L->lv[l];
+        const uint32_t nn = n_agg[l];
+        V.nnext = nn;
+        std::swap(V.agg.p, maps[l]->p);
+        std::swap(V.agg.n, maps[l]->n);
+        DBuf<uint32_t> cptr, ccol;
+        uint32_t cnnz = 0;
+        if ((rc = hier_members(V.agg.p, V.n, nn, V.mptr, V.mrows))) return rc;
+        if ((rc = hier_level_lists(V.M->rowptr.p, V.M->col.p, V.n, V.M->nnz, V.agg.p, nn, cptr, ccol, V.gptr, V.gsrc,
+                                   cnnz)))
+            return rc;
+        std::unique_ptr<CycLevel> C(new CycLevel());
+        if ((rc = matrix_create_device(nn, cptr, ccol, cnnz, &C->own))) return rc;
+        C->M = C->own;
+        C->n = nn;
+        const double avg = (double)V.n / nn;
+        V.rgroup = avg <= 6 ? 1 : avg <= 24 ? 4 : avg <= 96 ? 16 : 64;
+        if ((rc = hier_compose(total.p, A->n, V.agg.p))) return rc;
+        L->lv.push_back(std::move(C));
+    }
+    for (auto &lp : L->lv) {
+        CycLevel &V = *lp;
+        const size_t n = std::max(V.n, 1u);
+        if ((rc = V.diag.alloc(n)) || (rc = V.dinv.alloc(n)) || (rc = V.r.alloc(n)) || (rc = V.za.alloc(n)) || (rc = V.zb.alloc(n)) ||
+            (rc = V.d.alloc(n)) || (rc = V.res.alloc(n)))
+            return rc;
+    }
+    // fine rows of each coarsest node
+    if ((rc = hier_members(total.p, A->n, nc, L->tot_mptr, L->tot_mrows))) return rc;
+    // coarsest graph blocks, from the coarsest pattern
+    const Matrix &Cc = *L->lv.back()->M;
+    std::vector<uint32_t> rp(nc + 1ull), ci(Cc.nnz);
+    RS_HIP(hipMemcpyAsync(rp.data(), Cc.rowptr.p, sizeof(uint32_t) * rp.size(), hipMemcpyDeviceToHost, st));
+    if (Cc.nnz) RS_HIP(hipMemcpyAsync(ci.data(), Cc.col.p, sizeof(uint32_t) * Cc.nnz, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    const uint32_t nstrips = (nc + 15) / 16;
+    std::vector<uint32_t> gbptr(nstrips + 1ull, 0), gbcol, gbidx(ci.size());
+    for (uint32_t s = 0; s < nstrips; s++) {
+        std::vector<uint32_t> ts;
+        for (uint32_t I = s * 16; I < std::min(nc, s * 16 + 16); I++)
+            for (uint32_t p = rp[I]; p < rp[I + 1]; p++) ts.push_back(ci[p] / 16);
+        std::sort(ts.begin(), ts.end());
+        ts.erase(std::unique(ts.begin(), ts.end()), ts.end());
+        const uint32_t b0 = (uint32_t)gbcol.size();
+        gbcol.insert(gbcol.end(), ts.begin(), ts.end());
+        gbptr[s + 1] = (uint32_t)gbcol.size();
+        for (uint32_t I = s * 16; I < std::min(nc, s * 16 + 16); I++)
+            for (uint32_t p = rp[I]; p < rp[I + 1]; p++) {
+                const uint32_t t = ci[p] / 16;
+                const uint32_t b = b0 + (uint32_t)(std::lower_bound(ts.begin(), ts.end(), t) - ts.begin());
+                gbidx[p] = b * 256 + (I % 16) * 16 + ci[p] % 16;
+            }
+    }
+    rs_learned &R = *L;
+    if ((rc = R.gbptr.alloc(gbptr.size())) || (rc = R.gbcol.alloc(std::max<size_t>(gbcol.size(), 1))) ||
+        (rc = R.gbidx.alloc(std::max<size_t>(gbidx.size(), 1))) || (rc = R.gbval.alloc(std::max<size_t>(gbcol.size(), 1) * 256)) ||
+        (rc = R.gh.alloc(2ull * nstrips * 16 * kHid)) || (rc = R.ght.alloc(2ull * nstrips * 16 * kHid)) ||
+        (rc = R.gu.alloc(nstrips * 16)) || (rc = R.gnorm.alloc(2)) || (rc = R.s16bad.alloc(R.nlev)))
+        return rc;
+    RS_HIP(hipMemcpyAsync(R.gbptr.p, gbptr.data(), sizeof(uint32_t) * gbptr.size(), hipMemcpyHostToDevice, st));
+    if (!gbcol.empty())
+        RS_HIP(hipMemcpyAsync(R.gbcol.p, gbcol.data(), sizeof(uint32_t) * gbcol.size(), hipMemcpyHostToDevice, st));
+    if (!gbidx.empty())
+        RS_HIP(hipMemcpyAsync(R.gbidx.p, gbidx.data(), sizeof(uint32_t) * gbidx.size(), hipMemcpyHostToDevice, st));
+    if ((rc = R.cw.alloc(std::max<uint64_t>(Cc.nnz, 1))) || (rc = R.cdiag.alloc(nc)) ||
+        (rc = R.coef.alloc((size_t)R.nlev * kMaxDeg * 2)) || (rc = R.tmp_rc.alloc(nc)) ||
+        (rc = R.part.alloc((size_t)R.nlev * 4096)) || (rc = R.gersh.alloc(R.nlev)) || (rc = R.theta.alloc(kNumParams)) ||
+        (rc = R.wbf.alloc(kLayers * kBfPerLayer)))
+        return rc;
+    RS_HIP(hipStreamSynchronize(st));
+    *out = L.release();
+    return RS_OK;
+}
+
+}  // namespace rs
+
 extern "C" int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const uint32_t *agg, const uint32_t *n_agg,
                                  rs_learned **out)
 {
@@ -1721,148 +1822,51 @@ extern "C" int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const ui
     }
     int rc = require_runtime();
     if (rc) return rc;
-    std::unique_ptr<rs_learned> L(new rs_learned());
-    L->n = A->n;
-    L->nc = nc;
-    L->nlev = n_levels + 1;
-    L->owner = A;
     hipStream_t st = runtime().stream;
-    std::vector<uint32_t> rp(A->n + 1ull), ci(A->nnz);
-    RS_HIP(hipMemcpyAsync(rp.data(), A->rowptr.p, sizeof(uint32_t) * rp.size(), hipMemcpyDeviceToHost, st));
-    if (A->nnz) RS_HIP(hipMemcpyAsync(ci.data(), A->col.p, sizeof(uint32_t) * A->nnz, hipMemcpyDeviceToHost, st));
-    RS_HIP(hipStreamSynchronize(st));
-    L->lv.emplace_back(new CycLevel());
-    L->lv[0]->M = const_cast<rs_matrix *>(A);
-    L->lv[0]->n = A->n;
-    std::vector<uint32_t> total(A->n);
-    for (uint32_t i = 0; i < A->n; i++) total[i] = i;
+    std::vector<std::unique_ptr<DBuf<uint32_t>>> maps;
     uint64_t off = 0;
+    uint32_t nprev = A->n;
     for (uint32_t l = 0; l < n_levels; l++) {
-        CycLevel &V = *L->lv[l];
-        const uint32_t nn = n_agg[l];
-        const uint32_t *m = agg + off;
-        off += V.n;
-        V.nnext = nn;
-        std::vector<uint32_t> mptr(nn + 1ull, 0), mrows(V.n);
-        for (uint32_t i = 0; i < V.n; i++) mptr[m[i] + 1]++;
-        for (uint32_t I = 0; I < nn; I++) mptr[I + 1] += mptr[I];
-        {
-            std::vector<uint32_t> fill(mptr.begin(), mptr.end() - 1);
-            for (uint32_t i = 0; i < V.n; i++) mrows[fill[m[i]]++] = i;
-        }
-        // next-level pattern and gather lists: the entries (agg(i), agg(j)) of the member rows,
-        // sorted by (coarse column, entry position)
-        std::vector<std::vector<uint32_t>> ccols(nn), cstart(nn), csrc(nn);
-        parallel_for(nn, [&](uint64_t I0, uint64_t I1) {
-            std::vector<std::pair<uint32_t, uint32_t>> e;
-            for (uint64_t I = I0; I < I1; I++) {
-                e.clear();
-                for (uint32_t k = mptr[I]; k < mptr[I + 1]; k++) {
-                    const uint32_t i = mrows[k];
-                    for (uint32_t p = rp[i]; p < rp[i + 1]; p++) e.emplace_back(m[ci[p]], p);
-                }
-                std::sort(e.begin(), e.end());
-                for (size_t t = 0; t < e.size(); t++) {
-                    if (t == 0 || e[t].first != e[t - 1].first) {
-                        ccols[I].push_back(e[t].first);
-                        cstart[I].push_back((uint32_t)t);
-                    }
-                    csrc[I].push_back(e[t].second);
-                }
-            }
-        });
-        std::vector<uint32_t> cptr(nn + 1ull, 0), ccol, gptr(1, 0), gsrc;
-        gsrc.reserve(rp[V.n]);
-        for (uint32_t I = 0; I < nn; I++) {
-            for (size_t c = 0; c < ccols[I].size(); c++) {
-                ccol.push_back(ccols[I][c]);
-                size_t t1 = c + 1 < ccols[I].size() ? cstart[I][c + 1] : csrc[I].size();
-                for (size_t t = cstart[I][c]; t < t1; t++) gsrc.push_back(csrc[I][t]);
-                gptr.push_back((uint32_t)gsrc.size());
-            }
-            cptr[I + 1] = (uint32_t)ccol.size();
-        }
-        std::vector<double> zeros(std::max<size_t>(ccol.size(), 1), 0.0);
-        std::unique_ptr<CycLevel> C(new CycLevel());
-        if ((rc = rs_matrix_create(nn, cptr.data(), ccol.data(), zeros.data(), &C->own))) return rc;
-        C->M = C->own;
-        C->n = nn;
-        if ((rc = V.agg.alloc(V.n)) || (rc = V.mptr.alloc(mptr.size())) || (rc = V.mrows.alloc(V.n)) ||
-            (rc = V.gptr.alloc(gptr.size())) || (rc = V.gsrc.alloc(std::max<size_t>(gsrc.size(), 1))))
-            return rc;
-        RS_HIP(hipMemcpyAsync(V.agg.p, m, sizeof(uint32_t) * V.n, hipMemcpyHostToDevice, st));
-        RS_HIP(hipMemcpyAsync(V.mptr.p, mptr.data(), sizeof(uint32_t) * mptr.size(), hipMemcpyHostToDevice, st));
-        RS_HIP(hipMemcpyAsync(V.mrows.p, mrows.data(), sizeof(uint32_t) * V.n, hipMemcpyHostToDevice, st));
-        RS_HIP(hipMemcpyAsync(V.gptr.p, gptr.data(), sizeof(uint32_t) * gptr.size(), hipMemcpyHostToDevice, st));
-        if (!gsrc.empty())
-            RS_HIP(hipMemcpyAsync(V.gsrc.p, gsrc.data(), sizeof(uint32_t) * gsrc.size(), hipMemcpyHostToDevice, st));
-        RS_HIP(hipStreamSynchronize(st));
-        const double avg = (double)V.n / nn;
-        V.rgroup = avg <= 6 ? 1 : avg <= 24 ? 4 : avg <= 96 ? 16 : 64;
-        for (uint32_t i = 0; i < A->n; i++) total[i] = m[total[i]];
-        rp.swap(cptr);
-        ci.swap(ccol);
-        L->lv.push_back(std::move(C));
+        maps.emplace_back(new DBuf<uint32_t>());
+        if ((rc = maps.back()->alloc(nprev))) return rc;
+        if (nprev) RS_HIP(hipMemcpyAsync(maps.back()->p, agg + off, sizeof(uint32_t) * nprev, hipMemcpyHostToDevice, st));
+        off += nprev;
+        nprev = n_agg[l];
     }
-    for (auto &lp : L->lv) {
-        CycLevel &V = *lp;
-        const size_t n = std::max(V.n, 1u);
-        if ((rc = V.diag.alloc(n)) || (rc = V.dinv.alloc(n)) || (rc = V.r.alloc(n)) || (rc = V.za.alloc(n)) || (rc = V.zb.alloc(n)) ||
-            (rc = V.d.alloc(n)) || (rc = V.res.alloc(n)))
-            return rc;
-    }
-    // fine rows of each coarsest node
-    std::vector<uint32_t> tptr(nc + 1ull, 0), trows(A->n);
-    for (uint32_t i = 0; i < A->n; i++) tptr[total[i] + 1]++;
-    for (uint32_t I = 0; I < nc; I++) tptr[I + 1] += tptr[I];
-    {
-        std::vector<uint32_t> fill(tptr.begin(), tptr.end() - 1);
-        for (uint32_t i = 0; i < A->n; i++) trows[fill[total[i]]++] = i;
-    }
-    // coarsest graph blocks: rp/ci hold the coarsest pattern here
-    const uint32_t nstrips = (nc + 15) / 16;
-    std::vector<uint32_t> gbptr(nstrips + 1ull, 0), gbcol, gbidx(ci.size());
-    for (uint32_t s = 0; s < nstrips; s++) {
-        std::vector<uint32_t> ts;
-        for (uint32_t I = s * 16; I < std::min(nc, s * 16 + 16); I++)
-            for (uint32_t p = rp[I]; p < rp[I + 1]; p++) ts.push_back(ci[p] / 16);
-        std::sort(ts.begin(), ts.end());
-        ts.erase(std::unique(ts.begin(), ts.end()), ts.end());
-        const uint32_t b0 = (uint32_t)gbcol.size();
-        gbcol.insert(gbcol.end(), ts.begin(), ts.end());
-        gbptr[s + 1] = (uint32_t)gbcol.size();
-        for (uint32_t I = s * 16; I < std::min(nc, s * 16 + 16); I++)
-            for (uint32_t p = rp[I]; p < rp[I + 1]; p++) {
-                const uint32_t t = ci[p] / 16;
-                const uint32_t b = b0 + (uint32_t)(std::lower_bound(ts.begin(), ts.end(), t) - ts.begin());
-                gbidx[p] = b * 256 + (I % 16) * 16 + ci[p] % 16;
-            }
-    }
-    rs_learned &R = *L;
-    const Matrix &Cc = *R.lv.back()->M;
-    if ((rc = R.gbptr.alloc(gbptr.size())) || (rc = R.gbcol.alloc(std::max<size_t>(gbcol.size(), 1))) ||
-        (rc = R.gbidx.alloc(std::max<size_t>(gbidx.size(), 1))) || (rc = R.gbval.alloc(std::max<size_t>(gbcol.size(), 1) * 256)) ||
-        (rc = R.gh.alloc(2ull * nstrips * 16 * kHid)) || (rc = R.ght.alloc(2ull * nstrips * 16 * kHid)) ||
-        (rc = R.gu.alloc(nstrips * 16)) || (rc = R.gnorm.alloc(2)) || (rc = R.s16bad.alloc(R.nlev)))
-        return rc;
-    RS_HIP(hipMemcpyAsync(R.gbptr.p, gbptr.data(), sizeof(uint32_t) * gbptr.size(), hipMemcpyHostToDevice, st));
-    if (!gbcol.empty())
-        RS_HIP(hipMemcpyAsync(R.gbcol.p, gbcol.data(), sizeof(uint32_t) * gbcol.size(), hipMemcpyHostToDevice, st));
-    if (!gbidx.empty())
-        RS_HIP(hipMemcpyAsync(R.gbidx.p, gbidx.data(), sizeof(uint32_t) * gbidx.size(), hipMemcpyHostToDevice, st));
-    if ((rc = R.tot_mptr.alloc(tptr.size())) || (rc = R.tot_mrows.alloc(std::max(A->n, 1u))) ||
-        (rc = R.cw.alloc(std::max<uint64_t>(Cc.nnz, 1))) || (rc = R.cdiag.alloc(nc)) ||
-        (rc = R.coef.alloc((size_t)R.nlev * kMaxDeg * 2)) || (rc = R.tmp_rc.alloc(nc)) ||
-        (rc = R.part.alloc((size_t)R.nlev * 4096)) || (rc = R.gersh.alloc(R.nlev)) || (rc = R.theta.alloc(kNumParams)) ||
-        (rc = R.wbf.alloc(kLayers * kBfPerLayer)))
-        return rc;
-    RS_HIP(hipMemcpyAsync(R.tot_mptr.p, tptr.data(), sizeof(uint32_t) * tptr.size(), hipMemcpyHostToDevice, st));
-    if (A->n) RS_HIP(hipMemcpyAsync(R.tot_mrows.p, trows.data(), sizeof(uint32_t) * A->n, hipMemcpyHostToDevice, st));
     RS_HIP(hipStreamSynchronize(st));
-    *out = L.release();
-    return RS_OK;
+    return learned_build(A, maps, std::vector<uint32_t>(n_agg, n_agg + n_levels), out);
 }
 
+extern "C" int rs_learned_create_by_bins(const rs_matrix *A, uint32_t n_nodes, const double *d_xyz,
+                                         uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx, uint32_t nby,
+                                         uint32_t nbz, uint32_t max_coarse, uint32_t max_levels, rs_learned **out)
+{
+    if (!A || !d_xyz || !d_book || !out || !n_nodes || !dofs_per_node || !nbx || !nby || !nbz || !max_coarse ||
+        !max_levels || max_levels > 16)
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_create_by_bins: bad arguments");
+    if (max_coarse > kMaxCoarse) return fail(RS_ERR_UNSUPPORTED, "rs_learned_create_by_bins: max_coarse above 512");
+    int rc = require_runtime();
+    if (rc) return rc;
+    std::vector<std::unique_ptr<DBuf<uint32_t>>> maps;
+    std::vector<uint32_t> n_agg;
+    const uint32_t bins[3] = { nbx, nby, nbz };
+    if ((rc = hier_bins(n_nodes, d_xyz, dofs_per_node, d_book, A->n, bins, max_coarse, max_levels, maps, n_agg)))
+        return rc;
+    return learned_build(A, maps, n_agg, out);
+}
+
+extern "C" int rs_learned_level_map(const rs_learned *L, uint32_t level, uint32_t *agg, uint64_t capacity)
+{
+    if (!L || !agg || level + 1 >= L->nlev) return fail(RS_ERR_INVALID_ARG, "rs_learned_level_map: bad arguments");
+    const CycLevel &V = *L->lv[level];
+    if (capacity < V.n) return fail(RS_ERR_INVALID_ARG, "rs_learned_level_map: capacity below the level's rows");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    if (V.n) RS_HIP(hipMemcpyAsync(agg, V.agg.p, sizeof(uint32_t) * V.n, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
 extern "C" int rs_learned_set_params(rs_learned *L, const float *params, uint64_t count)
 {
     if (!L || !params) return fail(RS_ERR_INVALID_ARG, "rs_learned_set_params: null argument");

@@ -1245,6 +1245,36 @@ extern "C" int rs_matrix_create(uint32_t n_rows, const uint32_t *row_ptr, const 
     return RS_OK;
 }
 
+namespace rs {
+
+int matrix_create_device(uint32_t n, DBuf<uint32_t> &rowptr, DBuf<uint32_t> &col, uint64_t nnz, rs_matrix **out)
+{
+    if (!out || rowptr.n != n + 1ull || (nnz && col.n < nnz)) return fail(RS_ERR_INVALID_ARG, "matrix_create_device: bad pattern");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    std::unique_ptr<rs_matrix> A(new rs_matrix());
+    A->n = n;
+    A->nnz = nnz;
+    std::swap(A->rowptr.p, rowptr.p);
+    std::swap(A->rowptr.n, rowptr.n);
+    std::swap(A->col.p, col.p);
+    std::swap(A->col.n, col.n);
+    if ((rc = A->val.alloc(nnz))) return rc;
+    if (nnz) RS_HIP(hipMemsetAsync(A->val.p, 0, sizeof(double) * nnz, st));
+    A->frob = 0.0;
+    std::vector<uint32_t> hrp(n + 1ull);
+    RS_HIP(hipMemcpyAsync(hrp.data(), A->rowptr.p, sizeof(uint32_t) * hrp.size(), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    if (hrp[0] != 0 || hrp[n] != nnz) return fail(RS_ERR_INVALID_ARG, "matrix_create_device: row pointers do not span the entries");
+    if ((rc = matrix_set_layout(*A, hrp.data()))) return rc;
+    RS_HIP(hipStreamSynchronize(st));
+    *out = A.release();
+    return RS_OK;
+}
+
+}  // namespace rs
+
 extern "C" int rs_matrix_update_values(rs_matrix *A, const double *val)
 {
     if (!A || (!val && A->nnz)) return fail(RS_ERR_INVALID_ARG, "rs_matrix_update_values: null argument");

@@ -102,6 +102,67 @@ def test_hierarchy_operators_match_galerkin_chain(gpu, bins, max_coarse):
         assert np.array_equal(L2.level_matrix(l).data, L.level_matrix(l).data)
 
 
+def _stress_assembly(rs, nx, ny, nz):
+    xyz, tets = rs.kuhn_box(nx, ny, nz, 4.0, 1.0, 1.0)
+    nn, ne = xyz.shape[0], tets.shape[0]
+    fixed = np.repeat(np.abs(xyz[:, 0]) < 1e-12, 3).astype(np.uint8)
+    E = rs.field_lognormal(3, ne, 0.0, 1.0)
+    tris = rs.plane_faces(xyz, tets, 0, 4.0)
+    trac = np.tile([0.0, 0.0, -1.0], (tris.shape[0], 1))
+    asm = rs.Assembly(xyz, tets, 3, fixed)
+    b = asm.assemble_stress(E, np.full(ne, 0.3), np.ones(ne), [0.0, 0.0, 0.0], np.zeros(3 * nn), tris, trac)
+    return asm, b, xyz
+
+
+@pytest.mark.parametrize("case", ["heat16-8^3-64", "heat20-10x7x5-200", "stress12x4x4-6^3-100"])
+def test_hierarchy_built_on_gpu(gpu, case):
+    """rs_learned_create_by_bins: the level maps built on the GPU are the oracle's nested bins,
+    and every level operator and bound is bit-identical to those of the hierarchy built from the
+    host maps; a learned solve with either converges alike."""
+    rs = gpu
+    _say(f"GPU hierarchy {case}")
+    if case.startswith("heat16"):
+        (asm, b, xyz), dpn, bins, mc = _heat(rs, 16, 1.5), 1, (8, 8, 8), 64
+    elif case.startswith("heat20"):
+        (asm, b, xyz), dpn, bins, mc = _heat(rs, 20, 2.0), 1, (10, 7, 5), 200
+    else:
+        (asm, b, xyz), dpn, bins, mc = _stress_assembly(rs, 12, 4, 4), 3, (6, 6, 6), 100
+    ref = lo.bins_hierarchy(xyz, asm.book(), dpn, bins, mc)
+    Lg = rs.LearnedPreconditioner.from_bins(asm.matrix, asm, bins, max_coarse=mc, params=lo.init_params(1))
+    maps = rs.hierarchy_by_bins(xyz, asm.book(), dpn, bins, max_coarse=mc)
+    Lh = rs.LearnedPreconditioner(asm.matrix, maps, params=lo.init_params(1))
+    assert Lg.n_levels == len(ref) + 1
+    for l, r in enumerate(ref):
+        assert np.array_equal(Lg.level_map(l), r), l
+    assert np.array_equal(Lg.level_rows, Lh.level_rows) and np.array_equal(Lg.level_nnz, Lh.level_nnz)
+    Lg.setup()
+    Lh.setup()
+    for l in range(Lg.n_levels):
+        G, H = Lg.level_matrix(l), Lh.level_matrix(l)
+        assert np.array_equal(G.indptr, H.indptr) and np.array_equal(G.indices, H.indices), l
+        assert np.array_equal(G.data, H.data), l
+    assert np.array_equal(Lg.level_lambda(), Lh.level_lambda())
+    if dpn == 1:
+        # each preconditioner autotunes its own coarse levels' FP32 SpMV shapes (timing-based),
+        # so the two solves may sum coarse rows in different orders: same bar, not the same bits
+        xg, ig = _learned_solve(rs, asm, b, Lg)
+        xh, ih = _learned_solve(rs, asm, b, Lh)
+        assert ig.converged == 1 and ih.converged == 1 and abs(ig.iterations - ih.iterations) <= 1
+        assert _rel(xg, xh) <= 1e-6
+
+
+def test_hierarchy_on_gpu_invalid_arguments(gpu):
+    rs = gpu
+    asm, _, _ = _heat(rs, 6, 0.0)
+    with pytest.raises(rs.RSError):
+        rs.LearnedPreconditioner.from_bins(asm.matrix, asm, (4, 4, 4), max_coarse=600)
+    with pytest.raises(rs.RSError):   # one level allowed, but 4^3 bins leave more than 8 nodes
+        rs.LearnedPreconditioner.from_bins(asm.matrix, asm, (4, 4, 4), max_coarse=8, max_levels=1)
+    other, _, _ = _heat(rs, 5, 0.0)   # a book that does not number this matrix's rows
+    with pytest.raises(rs.RSError):
+        rs.LearnedPreconditioner.from_bins(asm.matrix, other, (4, 4, 4), max_coarse=64)
+
+
 def test_single_level_coarse_operator(gpu):
     rs = gpu
     asm, _, agg, nagg = _heat(rs, 12, 1.0, (3, 3, 4))

@@ -114,6 +114,26 @@ def test_config_counts(rs):
     assert n_dof == 8379
 
 
+@pytest.mark.parametrize("dpn,bins,max_coarse", [(1, (16, 16, 16), 512), (1, (10, 7, 5), 40), (3, (9, 9, 9), 100)])
+def test_hierarchy_by_bins_matches_oracle(rs, orc, dpn, bins, max_coarse):
+    """The host builder of the nested bins against the numpy restatement in the oracle (uneven
+    bins, a 3-DOF book with fixed nodes)."""
+    import learned_oracle as lo
+    xyz, tets = rs.kuhn_box(16, 16, 16)
+    x = xyz[:, 0]
+    fixed = ((np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12)).astype(np.uint8)
+    book, _ = orc.node_book(xyz.shape[0], tets, dpn, fixed)
+    maps = rs.hierarchy_by_bins(xyz, book, dpn, bins, max_coarse=max_coarse)
+    ref = lo.bins_hierarchy(xyz, book, dpn, bins, max_coarse)
+    assert len(maps) == len(ref) >= 1
+    for m, r in zip(maps, ref):
+        assert np.array_equal(m, r)
+    # ids are in first-appearance order: the first occurrences of 0, 1, 2, ... are increasing
+    for r in ref:
+        first = np.unique(r, return_index=True)[1]
+        assert np.all(np.diff(first) > 0)
+
+
 def test_hierarchy_by_bins_nested(rs, orc):
     """Nested coordinate bins: each level map halves the bins; composing the maps gives the
     direct binning of the finest positions at the coarser resolution."""
