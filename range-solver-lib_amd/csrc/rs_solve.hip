@@ -364,6 +364,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CsrView A, const double *__rest
     Shape::template run<double>(A, A.val, x, e);
 }
 
+// y = A x unless the GMRES inner cycle has stopped
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_gm_spmv(CsrView A, const double *__restrict__ x, double *__restrict__ y,
+                                                    const uint32_t *stop)
+{
+    if (*stop) return;
+    EpiStore64 e{ y };
+    Shape::template run<double>(A, A.val, x, e);
+}
+
 template <class Shape>
 __global__ __launch_bounds__(kBlock) void k_spmv32(CsrView A, const float *__restrict__ x, float *__restrict__ y)
 {
@@ -573,8 +583,9 @@ __device__ __forceinline__ void pc_apply_block(int ptype, uint32_t w, const PT *
 // y = P x (RMatrixPreconditioner::compute, and GMRES's z_k = P v_k)
 template <int W>
 __global__ void k_pc_apply(int ptype, uint32_t w, const double *__restrict__ pd, uint32_t n,
-                           const double *__restrict__ x, double *__restrict__ y)
+                           const double *__restrict__ x, double *__restrict__ y, const uint32_t *stop)
 {
+    if (stop && *stop) return;
     constexpr int WM = W == 0 ? 16 : W;
     const uint32_t ww = W == 0 ? w : W;
     uint64_t nb = n / ww;
@@ -875,8 +886,10 @@ __global__ __launch_bounds__(kBlock) void k_mp_rho(MpArgs a)
 // h_i = w . v_i for i in [i0, i0+8) -> partials[i][block]; final sum in k_multidot_final
 __global__ __launch_bounds__(kBlock) void k_multidot(const double *__restrict__ w,
                                                      const double *__restrict__ V, uint64_t ld,
-                                                     uint32_t n, uint32_t cnt, double *partials)
+                                                     uint32_t n, uint32_t cnt, double *partials,
+                                                     const uint32_t *stop)
 {
+    if (*stop) return;
     __shared__ double lds[8 * (kBlock / kWave)];
     uint32_t i0 = blockIdx.y * 8;
     double v[8];
@@ -897,8 +910,9 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const double *__restrict__ 
 }
 
 __global__ __launch_bounds__(kBlock) void k_multidot_final(const double *partials, uint32_t nblk,
-                                                           double *out)
+                                                           double *out, const uint32_t *stop)
 {
+    if (*stop) return;
     __shared__ double lds[kBlock / kWave];
     double v[1] = { 0.0 };
     for (uint32_t j = threadIdx.x; j < nblk; j += kBlock) v[0] += partials[(uint64_t)blockIdx.x * nblk + j];
@@ -910,8 +924,10 @@ __global__ __launch_bounds__(kBlock) void k_multidot_final(const double *partial
 __global__ __launch_bounds__(kBlock) void k_gs_subtract(double *__restrict__ w,
                                                         const double *__restrict__ V, uint64_t ld,
                                                         uint32_t n, uint32_t cnt,
-                                                        const double *__restrict__ h, double *partials)
+                                                        const double *__restrict__ h, double *partials,
+                                                        const uint32_t *stop)
 {
+    if (*stop) return;
     double v[1] = { 0.0 };
     for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (uint64_t)gridDim.x * kBlock) {
         double value = w[j];
@@ -924,8 +940,9 @@ __global__ __launch_bounds__(kBlock) void k_gs_subtract(double *__restrict__ w,
 
 // v = (h == 0) ? 0 : w / h with h = st->t0
 __global__ void k_normalize(const double *__restrict__ w, double *__restrict__ v, uint64_t n,
-                            const SolveState *st)
+                            const SolveState *st, const uint32_t *stop)
 {
+    if (*stop) return;
     double h = st->t0;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
         v[i] = (h == 0.0) ? 0.0 : w[i] / h;
@@ -938,15 +955,132 @@ __global__ void k_scale_by(const double *__restrict__ a, double *__restrict__ ou
         out[i] = (beta == 0.0) ? 0.0 : a[i] / beta;
 }
 
-// x += sum_{j<cnt} Z_j y_j   (src/rmatrixsolver.cpp:571-578)
+// x += sum_{j<cnt} Z_j y_j   (src/rmatrixsolver.cpp:571-578), cnt = the inner cycle's length
 __global__ void k_update_x(double *__restrict__ x, const double *__restrict__ Z, uint64_t ld,
-                           uint32_t n, uint32_t cnt, const double *__restrict__ y)
+                           uint32_t n, const uint32_t *__restrict__ cnt_dev, const double *__restrict__ y)
 {
+    const uint32_t cnt = *cnt_dev;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         double xi = x[i];
         for (uint32_t j = 0; j < cnt; j++) xi += Z[j * ld + i] * y[j];
         x[i] = xi;
     }
+}
+
+__global__ void k_zero_u32(uint32_t *__restrict__ a, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0u;
+}
+
+// GMRES inner state (device, one outer cycle): H ((k+1) x k, row-major), the rotations c, s,
+// the rotated right-hand side pp and y (k+1 each), and beta = |r_0|; flag[0] = stop,
+// flag[1] = the cycle's length.
+struct GmView {
+    double *H, *c, *s, *pp, *y;
+    uint32_t k;
+    __host__ __device__ GmView(double *base, uint32_t k_) : k(k_)
+    {
+        H = base;
+        c = H + (uint64_t)(k + 1) * k;
+        s = c + k + 1;
+        pp = s + k + 1;
+        y = pp + k + 1;
+    }
+    __host__ __device__ double &h(uint32_t i, uint32_t j) const { return H[(uint64_t)i * k + j]; }
+    __host__ __device__ double &beta() const { return y[k + 1]; }
+    static uint64_t doubles(uint32_t k) { return (uint64_t)(k + 1) * k + 4ull * (k + 1) + 1; }
+};
+
+constexpr uint32_t kGmLds = 2048;     // inner cycles up to this length keep their vectors in LDS
+constexpr uint32_t kGmBlock = 256;
+
+// Column it of H from the Gram-Schmidt coefficients and |w|, the previous rotations applied,
+// the new rotation, and the stopping test of src/rmatrixsolver.cpp:521-550.  The column and the
+// rotations are staged in LDS by the workgroup; one thread runs the rotation chain with the
+// running entry in a register.  The host loop's expressions without contraction, so the
+// rotations agree with it bit for bit.
+__global__ __launch_bounds__(kGmBlock) void k_gm_givens(double *__restrict__ base, uint32_t k, uint32_t *__restrict__ flag,
+                                                        const double *__restrict__ hcol,
+                                                        const SolveState *__restrict__ st, uint32_t it)
+{
+#pragma clang fp contract(off)
+    if (flag[0]) return;
+    __shared__ double col[kGmLds], cs[kGmLds], sn[kGmLds];
+    GmView g(base, k);
+    const bool lds = it < kGmLds;
+    const double *cc = lds ? cs : g.c, *ss = lds ? sn : g.s;
+    const uint64_t cstride = lds ? 1 : k;   // column it of H: stride k in place
+    double *cvi = lds ? col : g.H + it;
+    if (lds) {
+        for (uint32_t i = threadIdx.x; i <= it; i += kGmBlock) {
+            col[i] = hcol[i];
+            if (i < it) {
+                cs[i] = g.c[i];
+                sn[i] = g.s[i];
+            }
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i <= it; i += kGmBlock) g.h(i, it) = hcol[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double prev = cvi[0];
+        for (uint32_t i = 1; i <= it; i++) {
+            const double cur = cvi[i * cstride], c = cc[i - 1], s = ss[i - 1];
+            cvi[(i - 1) * cstride] = c * prev + s * cur;
+            prev = -s * prev + c * cur;
+        }
+        const double sub = st->t0;   // h(it + 1, it)
+        const double gg = __dsqrt_rn(prev * prev + sub * sub);
+        double cn = 0.0, sg = 0.0;
+        if (gg != 0.0) {
+            cn = prev / gg;
+            sg = sub / gg;
+        }
+        g.c[it] = cn;
+        g.s[it] = sg;
+        cvi[it * cstride] = gg;
+        g.h(it + 1, it) = 0.0;
+        g.pp[it + 1] = -sg * g.pp[it];
+        g.pp[it] = cn * g.pp[it];
+        if (fabs(g.pp[it + 1]) < g.beta() * 1.0e-4 || it + 1 == k) {
+            flag[0] = 1u;
+            flag[1] = it + 1;
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (uint32_t i = threadIdx.x; i <= it; i += kGmBlock) g.h(i, it) = col[i];
+}
+
+// Back substitution of the cycle (src/rmatrixsolver.cpp:557-566) by columns: y_i from pp_i,
+// then every pp_j (j < i) less y_i h(j, i) in parallel -- for each pp_j the same subtractions in
+// the same order (i = m-1 down to j+1) as the row loop, so the same bits.
+__global__ __launch_bounds__(kGmBlock) void k_gm_backsub(double *__restrict__ base, uint32_t k,
+                                                         const uint32_t *__restrict__ flag)
+{
+#pragma clang fp contract(off)
+    __shared__ double lpp[kGmLds], ly[kGmLds];
+    GmView g(base, k);
+    const int m = (int)flag[1];
+    if (m == 0) return;
+    const bool lds = m <= (int)kGmLds;
+    double *pp = lds ? lpp : g.pp, *y = lds ? ly : g.y;
+    if (lds)
+        for (int j = threadIdx.x; j < m; j += kGmBlock) lpp[j] = g.pp[j];
+    __syncthreads();
+    for (int i = m - 1; i >= 0; i--) {
+        if (threadIdx.x == 0) {
+            const double d = g.h(i, i);
+            y[i] = d == 0.0 ? 0.0 : pp[i] / d;
+        }
+        __syncthreads();
+        const double yi = y[i];
+        for (int j = threadIdx.x; j < i; j += kGmBlock) pp[j] -= yi * g.h(j, i);
+        __syncthreads();
+    }
+    if (lds)
+        for (int j = threadIdx.x; j < m; j += kGmBlock) g.y[j] = ly[j];
 }
 
 // ============================================================================ launch helpers
@@ -1541,7 +1675,7 @@ extern "C" int rs_precond_compute(const rs_precond *P, const double *x, double *
     hipStream_t st = runtime().stream;
     RS_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * P->n, hipMemcpyHostToDevice, st));
     uint32_t w = P->width ? P->width : 1;
-    RS_DISPATCH_W(w, k_pc_apply, vec_blocks(P->n / w), P->type, w, P->data.p, P->n, dx.p, dy.p);
+    RS_DISPATCH_W(w, k_pc_apply, vec_blocks(P->n / w), P->type, w, P->data.p, P->n, dx.p, dy.p, nullptr);
     RS_HIP(hipGetLastError());
     RS_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * P->n, hipMemcpyDeviceToHost, st));
     RS_HIP(hipStreamSynchronize(st));
@@ -1587,7 +1721,8 @@ struct rs_solver {
     uint32_t last_iterations = 0;
     std::vector<double> history;
     // device workspace
-    DBuf<double> y, r, p, q, ro, w, V, Z, hbuf;
+    DBuf<double> y, r, p, q, ro, w, V, Z, hbuf, gm;
+    DBuf<uint32_t> gmflag;      // GMRES inner cycle: stop, length
     DBuf<float> r32, p32, q32, dx32, z32;
     DBuf<double> partials, hist;
     DBuf<SolveState> st;
@@ -1923,9 +2058,13 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
     const uint64_t ld = n;
     SolveState *dst = S.st.p;
     SolveState h;
-    std::vector<double> H((size_t)(ninner + 1) * ninner), c(ninner + 1), s(ninner + 1), pp(ninner + 1), y(ninner + 1),
-        hv(ninner + 2);
-    auto Hm = [&](uint32_t i, uint32_t j) -> double & { return H[(size_t)i * ninner + j]; };
+    // the inner cycle runs on the device (Gram-Schmidt, Hessenberg column, Givens rotation,
+    // stopping test, back substitution); the host launches it in batches of kGmBatch
+    // iterations and reads only the stop flag between batches
+    constexpr uint32_t kGmBatch = 8;
+    if ((rc = S.gm.alloc(GmView::doubles(ninner))) || (rc = S.gmflag.alloc(2))) return rc;
+    const GmView gv(S.gm.p, ninner);
+    std::vector<double> init(ninner + 2, 0.0);   // pp (k + 1 values) ... and beta, uploaded per cycle
     // bn = |b| (src/rmatrixsolver.cpp:375)
     k_norm2<<<nvb, kBlock, 0, stream>>>(S.y.p, n, S.partials.p);
     fin(S.partials.p, nvb, 1, FIN_NORM0, 0, dst, nullptr);
@@ -1938,8 +2077,8 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
     if ((rc = mpart.alloc((size_t)(ninner + 1) * nbm))) return rc;
     outer = 0;
     inner_total = 0;
+    const uint32_t *stop = S.gmflag.p;
     for (uint32_t ito = 0; ito < nouter; ito++) {
-        std::fill(H.begin(), H.end(), 0.0);
         RS_SHAPE_SWITCH(plan, k_gmres_resid, nb, stream, v, S.y.p, d_x, S.ro.p, S.partials.p);
         fin(S.partials.p, nb, 2, FIN_NORM01, 0, dst, nullptr);
         RS_HIP(hipGetLastError());
@@ -1960,56 +2099,35 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
                     std::fabs(S.trend) < kEps * kEps;
         if (conv) break;
         k_scale_by<<<nvb, kBlock, 0, stream>>>(S.ro.p, S.V.p, n, beta);
-        std::fill(pp.begin(), pp.end(), 0.0);
-        pp[0] = beta;
-        uint32_t iti;
-        for (iti = 0; iti < ninner; iti++) {
-            inner_total++;
-            double *vi = S.V.p + (uint64_t)iti * ld, *zi = S.Z.p + (uint64_t)iti * ld;
-            RS_DISPATCH_W(w, k_pc_apply, vec_blocks(n / w), P.type, w, P.data.p, n, vi, zi);
-            RS_SHAPE_SWITCH(plan, k_spmv, nb, stream, v, zi, S.w.p);
-            uint32_t cnt = iti + 1;
-            dim3 g(nbm, (cnt + 7) / 8);
-            k_multidot<<<g, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, mpart.p);
-            k_multidot_final<<<cnt, kBlock, 0, stream>>>(mpart.p, nbm, S.hbuf.p);
-            k_gs_subtract<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, S.hbuf.p, S.partials.p);
-            fin(S.partials.p, nvb, 1, FIN_NORM0, 0, dst, nullptr);
-            k_normalize<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p + (uint64_t)(iti + 1) * ld, n, dst);
+        // pp = (beta, 0, ...), beta; stop = 0
+        init[0] = beta;
+        init[ninner + 1] = beta;
+        RS_HIP(hipMemcpyAsync(gv.pp, init.data(), sizeof(double) * (ninner + 1), hipMemcpyHostToDevice, stream));
+        RS_HIP(hipMemcpyAsync(&gv.beta(), &init[ninner + 1], sizeof(double), hipMemcpyHostToDevice, stream));
+        k_zero_u32<<<1, 64, 0, stream>>>(S.gmflag.p, 2);
+        uint32_t hflag[2] = { 0, 0 };
+        for (uint32_t iti = 0; iti < ninner;) {
+            for (const uint32_t end = std::min(ninner, iti + kGmBatch); iti < end; iti++) {
+                double *vi = S.V.p + (uint64_t)iti * ld, *zi = S.Z.p + (uint64_t)iti * ld;
+                RS_DISPATCH_W(w, k_pc_apply, vec_blocks(n / w), P.type, w, P.data.p, n, vi, zi, stop);
+                RS_SHAPE_SWITCH(plan, k_gm_spmv, nb, stream, v, zi, S.w.p, stop);
+                uint32_t cnt = iti + 1;
+                dim3 g(nbm, (cnt + 7) / 8);
+                k_multidot<<<g, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, mpart.p, stop);
+                k_multidot_final<<<cnt, kBlock, 0, stream>>>(mpart.p, nbm, S.hbuf.p, stop);
+                k_gs_subtract<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, S.hbuf.p, S.partials.p, stop);
+                fin(S.partials.p, nvb, 1, FIN_NORM0, 0, dst, nullptr);
+                k_normalize<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p + (uint64_t)(iti + 1) * ld, n, dst, stop);
+                k_gm_givens<<<1, kGmBlock, 0, stream>>>(S.gm.p, ninner, S.gmflag.p, S.hbuf.p, dst, iti);
+            }
             RS_HIP(hipGetLastError());
-            RS_HIP(hipMemcpyAsync(hv.data(), S.hbuf.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, stream));
-            RS_HIP(hipMemcpyAsync(&h, dst, sizeof(h), hipMemcpyDeviceToHost, stream));
+            RS_HIP(hipMemcpyAsync(hflag, S.gmflag.p, sizeof(hflag), hipMemcpyDeviceToHost, stream));
             RS_HIP(hipStreamSynchronize(stream));
-            for (uint32_t i = 0; i <= iti; i++) Hm(i, iti) = hv[i];
-            Hm(iti + 1, iti) = h.t0;
-            for (uint32_t i = 1; i <= iti; i++) {           // src/rmatrixsolver.cpp:521-544
-                double hsave = Hm(i - 1, iti);
-                Hm(i - 1, iti) = c[i - 1] * hsave + s[i - 1] * Hm(i, iti);
-                Hm(i, iti) = -s[i - 1] * hsave + c[i - 1] * Hm(i, iti);
-            }
-            double gg = std::sqrt(Hm(iti, iti) * Hm(iti, iti) + Hm(iti + 1, iti) * Hm(iti + 1, iti));
-            if (gg == 0.0) {
-                c[iti] = s[iti] = 0.0;
-            } else {
-                c[iti] = Hm(iti, iti) / gg;
-                s[iti] = Hm(iti + 1, iti) / gg;
-            }
-            Hm(iti, iti) = gg;
-            Hm(iti + 1, iti) = 0.0;
-            pp[iti + 1] = -s[iti] * pp[iti];
-            pp[iti] = c[iti] * pp[iti];
-            if (std::fabs(pp[iti + 1]) < beta * 1.0e-4) {   // src/rmatrixsolver.cpp:546-550
-                iti++;
-                break;
-            }
+            if (hflag[0]) break;
         }
-        // back substitution (src/rmatrixsolver.cpp:557-566)
-        y[iti - 1] = (Hm(iti - 1, iti - 1) == 0.0) ? 0.0 : pp[iti - 1] / Hm(iti - 1, iti - 1);
-        for (int i = (int)iti - 2; i >= 0; i--) {
-            for (int j = (int)iti - 1; j > i; j--) pp[i] -= y[j] * Hm(i, j);
-            y[i] = (Hm(i, i) == 0.0) ? 0.0 : pp[i] / Hm(i, i);
-        }
-        RS_HIP(hipMemcpyAsync(S.hbuf.p, y.data(), sizeof(double) * iti, hipMemcpyHostToDevice, stream));
-        k_update_x<<<nvb, kBlock, 0, stream>>>(d_x, S.Z.p, ld, n, iti, S.hbuf.p);
+        inner_total += hflag[1];
+        k_gm_backsub<<<1, kGmBlock, 0, stream>>>(S.gm.p, ninner, S.gmflag.p);
+        k_update_x<<<nvb, kBlock, 0, stream>>>(d_x, S.Z.p, ld, n, S.gmflag.p + 1, gv.y);
         RS_HIP(hipGetLastError());
         RS_HIP(hipStreamSynchronize(stream));
     }
