@@ -1025,7 +1025,23 @@ __global__ __launch_bounds__(kGmBlock) void k_gm_givens(double *__restrict__ bas
     __syncthreads();
     if (threadIdx.x == 0) {
         double prev = cvi[0];
-        for (uint32_t i = 1; i <= it; i++) {
+        uint32_t i = 1;
+        if (lds)   // eight steps per trip, their operands loaded ahead of the chain
+            for (; i + 7 <= it; i += 8) {
+                double cur[8], c[8], s[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    cur[u] = col[i + u];
+                    c[u] = cs[i - 1 + u];
+                    s[u] = sn[i - 1 + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    col[i - 1 + u] = c[u] * prev + s[u] * cur[u];
+                    prev = -s[u] * prev + c[u] * cur[u];
+                }
+            }
+        for (; i <= it; i++) {
             const double cur = cvi[i * cstride], c = cc[i - 1], s = ss[i - 1];
             cvi[(i - 1) * cstride] = c * prev + s * cur;
             prev = -s * prev + c * cur;
