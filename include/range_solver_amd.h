@@ -257,10 +257,11 @@ int rs_assemble_stress(rs_assembly *S, const double *E, const double *nu, const 
                        const double *gravity, const double *u_node, uint32_t n_tri,
                        const uint32_t *tris, const double *traction, double *b);
 /* Assembly kernels: row_gather 1 (default) sums each free row's incident elements in one place
- * and writes the row whole -- heat: one thread per row (rows of at most 16 entries); static,
- * unrotated stress: nine threads per node (rows of at most 48 entries); otherwise, and for the
- * transient, modal or rotated stress problems, the colour launches are used.  0 = one launch per
- * node-disjoint colour with a scatter.  Both add the matrix values in the same element order. */
+ * and writes the row whole -- heat and scalar transport: one thread per row (rows of at most 16
+ * entries); static, unrotated stress: one wavefront per node (rows of at most 48 entries, at
+ * most 32 incident elements); otherwise, and for the transient, modal or rotated stress
+ * problems, the colour launches are used.  0 = one launch per node-disjoint colour with a
+ * scatter.  Both add the matrix values in the same element order. */
 int rs_assembly_set_method(rs_assembly *S, int32_t row_gather);
 /* Write-back (src/rsolverheat.cpp:473-480): node[i*dpn+c] = x[book] where enabled. */
 int rs_assembly_dofs_to_nodes(rs_assembly *S, const double *d_x, double *d_node);

@@ -663,7 +663,8 @@ class Assembly:
         self.matrix = SparseMatrix(0, None, None, None, _handle=mh, _owner=self)
 
     def set_method(self, row_gather: bool) -> None:
-        """Heat assembly kernel: row gather (default) or one launch per colour."""
+        """Heat, transport and static stress assembly kernels: row gather (default) or one launch
+        per colour."""
         _check(_lib.rs_assembly_set_method(self.h, 1 if row_gather else 0))
 
     def book(self) -> np.ndarray:

@@ -73,6 +73,16 @@ __device__ __forceinline__ double tet_gradients(const double (&X)[4][3], double 
     return det / 6.0;
 }
 
+// a.b of two 3-vectors with its roundings spelled out (a0 b0, then two fused multiply-adds) and
+// no further contraction: the same bits in every kernel that inlines it, whatever the compiler
+// would fuse around an open-coded sum in one kernel and not in another.  The row-gather and
+// colour-launch assemblies rely on it to add identical element terms.
+__device__ __forceinline__ double dot3(const double *a, const double *b)
+{
+#pragma clang fp contract(off)
+    return fma(a[2], b[2], fma(a[1], b[1], a[0] * b[0]));
+}
+
 __global__ __launch_bounds__(kBlock) void k_heat_colour(HeatArgs a, uint32_t e0, uint32_t e1)
 {
     uint32_t e = e0 + blockIdx.x * kBlock + threadIdx.x;
@@ -99,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_colour(HeatArgs a, uint32_t e0,
             be[m] = a.dt * fq;
 #pragma unroll
             for (int n = 0; n < 4; n++) {
-                double K = kV * (g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]);
+                double K = kV * dot3(g[m], g[n]);
                 double M = mc * (m == n ? 2.0 : 1.0);
                 Ae[m][n] = M + a.alpha * a.dt * K;
                 be[m] += (M - (1.0 - a.alpha) * a.dt * K) * T[n];
@@ -110,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_colour(HeatArgs a, uint32_t e0,
         for (int m = 0; m < 4; m++) {
             be[m] = fq;
 #pragma unroll
-            for (int n = 0; n < 4; n++) Ae[m][n] = kV * (g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]);
+            for (int n = 0; n < 4; n++) Ae[m][n] = kV * dot3(g[m], g[n]);
         }
     }
     uint32_t dof[4];
@@ -206,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_records(HeatArgs h, const uint3
         uint32_t sl = 0;
 #pragma unroll
         for (int n = 0; n < 4; n++) {
-            grad[4 * r + n] = g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2];
+            grad[4 * r + n] = dot3(g[m], g[n]);
             sl |= (pp[n] == RS_EOD ? kSlotFixed : (pp[n] - r0)) << (8 * n);
         }
         vol[r] = V;
@@ -672,8 +682,56 @@ struct TransportArgs {
     int unsteady;
 };
 
+// The transport element's row-independent quantities (RSolverFluidHeat, src/rsolverfluidheat.cpp:
+// 669-880): v.B_n, the stream-line length h = 2 / sum_m |s.B_m| and the SUPG parameter, with
+// the roundings of dot3 and no contraction, shared by the colour and row-gather kernels.
+struct TransportElem {
+    double vdiv[4], Tsupg;
+};
+__device__ __forceinline__ TransportElem transport_elem(const double (&B)[4][3], const double (&ve)[3], double k)
+{
+#pragma clang fp contract(off)
+    TransportElem t;
+    const double mvh = sqrt(dot3(ve, ve));
+    double s[3] = { 0, 0, 0 };
+    if (mvh > 0.0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) s[c] = ve[c] / mvh;
+    }
+    double h = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        t.vdiv[m] = dot3(ve, B[m]);
+        h += fabs(dot3(s, B[m]));
+    }
+    if (h != 0.0) h = 2.0 / h;
+    const double Re = k == 0.0 ? 0.0 : mvh * h / (2.0 * k);
+    t.Tsupg = 0.0;
+    if (mvh > 0.0) {
+        t.Tsupg = h / (2.0 * mvh);
+        if (Re > 0.0 && Re <= 3.0) t.Tsupg *= Re / 3.0;
+    }
+    return t;
+}
+
+// Entry (m, n) of the element matrix and its right-hand-side term, without contraction:
+// ss = ca iN vd_n - k wt (B_m.B_n) + Tsupg ca vd_m vd_n wt + Tsupg vd_m wt; A = (me + a dt ss) detJ
+// (steady: ss detJ); rhs term (me - (1 - a) dt ss) T_n.
+__device__ __forceinline__ double transport_entry(double ca, double k, double vdm, double vdn, double gmn, double Tsupg,
+                                                  bool unsteady, bool diag, double alpha, double dt, double detJ,
+                                                  double Tn, double &rhs)
+{
+#pragma clang fp contract(off)
+    const double wt = 1.0 / 6.0, iN = 1.0 / 24.0;
+    const double me = unsteady ? ca * ((diag ? 2.0 : 1.0) / 120.0) : 0.0;
+    const double ss = ca * iN * vdn - k * wt * gmn + Tsupg * ca * vdm * vdn * wt + Tsupg * vdm * wt;
+    rhs = unsteady ? (me - (1.0 - alpha) * dt * ss) * Tn : 0.0;
+    return (unsteady ? me + alpha * dt * ss : ss) * detJ;
+}
+
 __global__ __launch_bounds__(kBlock) void k_transport_colour(TransportArgs a, uint32_t e0, uint32_t e1)
 {
+#pragma clang fp contract(off)
     const uint32_t e = e0 + blockIdx.x * kBlock + threadIdx.x;
     if (e >= e1) return;
     const uint4 nd4 = a.conn[e];
@@ -685,28 +743,10 @@ __global__ __launch_bounds__(kBlock) void k_transport_colour(TransportArgs a, ui
 #pragma unroll
         for (int c = 0; c < 3; c++) X[q][c] = a.xyz[3 * (uint64_t)nd[q] + c];
     const double V = tet_gradients(X, B);
-    const double detJ = 6.0 * V, wt = 1.0 / 6.0, iN = 1.0 / 24.0;
+    const double detJ = 6.0 * V, iN = 1.0 / 24.0;
     const double ca = a.ca[orig], k = a.k[orig];
     const double ve[3] = { a.vel[3 * (uint64_t)orig], a.vel[3 * (uint64_t)orig + 1], a.vel[3 * (uint64_t)orig + 2] };
-    const double mvh = sqrt(ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
-    double s[3] = { 0, 0, 0 };
-    if (mvh > 0.0) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) s[c] = ve[c] / mvh;
-    }
-    double vdiv[4], h = 0.0;
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-        vdiv[m] = ve[0] * B[m][0] + ve[1] * B[m][1] + ve[2] * B[m][2];
-        h += fabs(s[0] * B[m][0] + s[1] * B[m][1] + s[2] * B[m][2]);
-    }
-    if (h != 0.0) h = 2.0 / h;
-    const double Re = k == 0.0 ? 0.0 : mvh * h / (2.0 * k);
-    double Tsupg = 0.0;
-    if (mvh > 0.0) {
-        Tsupg = h / (2.0 * mvh);
-        if (Re > 0.0 && Re <= 3.0) Tsupg *= Re / 3.0;
-    }
+    const TransportElem te = transport_elem(B, ve, k);
     const bool unsteady = a.unsteady != 0;
     double T[4];
 #pragma unroll
@@ -718,11 +758,10 @@ __global__ __launch_bounds__(kBlock) void k_transport_colour(TransportArgs a, ui
         double sum = 0.0;
 #pragma unroll
         for (int n = 0; n < 4; n++) {
-            const double me = unsteady ? ca * ((m == n ? 2.0 : 1.0) / 120.0) : 0.0;
-            const double ss = ca * iN * vdiv[n] - k * wt * (B[m][0] * B[n][0] + B[m][1] * B[n][1] + B[m][2] * B[n][2])
-                            + Tsupg * ca * vdiv[m] * vdiv[n] * wt + Tsupg * vdiv[m] * wt;
-            Ae[m][n] = (unsteady ? me + a.alpha * a.dt * ss : ss) * detJ;
-            if (unsteady) sum += (me - (1.0 - a.alpha) * a.dt * ss) * T[n];
+            double rhs;
+            Ae[m][n] = transport_entry(ca, k, te.vdiv[m], te.vdiv[n], dot3(B[m], B[n]), te.Tsupg, unsteady, m == n,
+                                       a.alpha, a.dt, detJ, T[n], rhs);
+            if (unsteady) sum += rhs;
         }
         be[m] = (unsteady ? a.dt * fv + sum : fv) * detJ;
     }
@@ -746,6 +785,111 @@ __global__ __launch_bounds__(kBlock) void k_transport_colour(TransportArgs a, ui
         for (int n = 0; n < 4; n++)
             if (pp[n] != RS_EOD) a.val[pp[n]] += Ae[m][n];
     }
+}
+
+// Row gather of the transport element, on the heat assembly's row records (the gradient
+// products, volume, element and column slots of every (row, incident element) pair, once per
+// mesh).  Per assembly, k_transport_elem first computes what does not depend on the row, in the
+// caller's element order: v.B_n (n = 0..3), the SUPG parameter, ca and k -- 64 bytes per element;
+// k_transport_rows then sums, per free row, its entries over the incident elements in colour
+// order with k_transport_colour's expressions, and writes the row and b_i once.  Same terms in
+// the same order as the colour launches, so the same bits.
+constexpr int kTElemWords = 8;
+
+__global__ __launch_bounds__(kBlock) void k_transport_elem(TransportArgs a, uint32_t ne, double *__restrict__ telem)
+{
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < ne; e += gridDim.x * kBlock) {
+        const uint4 nd4 = a.conn[e];
+        const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+        const uint32_t orig = a.perm[e];
+        double X[4][3], B[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) X[q][c] = a.xyz[3 * (uint64_t)nd[q] + c];
+        (void)tet_gradients(X, B);
+        const double ca = a.ca[orig], k = a.k[orig];
+        const double ve[3] = { a.vel[3 * (uint64_t)orig], a.vel[3 * (uint64_t)orig + 1], a.vel[3 * (uint64_t)orig + 2] };
+        const TransportElem te = transport_elem(B, ve, k);
+        double4 *o = reinterpret_cast<double4 *>(telem + (uint64_t)kTElemWords * orig);
+        o[0] = make_double4(te.vdiv[0], te.vdiv[1], te.vdiv[2], te.vdiv[3]);
+        o[1] = make_double4(te.Tsupg, ca, k, 0.0);
+    }
+}
+
+struct TransportRowArgs {
+    const uint32_t *rowptr, *dof_node;
+    RowRecords rec;
+    uint32_t n_rows;
+    const double *telem;
+    const double *q, *field;        // per node
+    double *val, *b;
+    double alpha, dt;
+    int unsteady;
+};
+
+__global__ __launch_bounds__(kBlock) void k_transport_rows(TransportRowArgs a)
+{
+#pragma clang fp contract(off)
+    __shared__ double acc[kRowCap][kBlock];
+    const uint32_t t = threadIdx.x;
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + t;
+    if (i >= a.n_rows) return;
+    const RowRecords &R = a.rec;
+    const uint32_t r0 = a.rowptr[i], len = a.rowptr[i + 1] - r0;
+#pragma unroll
+    for (int j = 0; j < kRowCap; j++) acc[j][t] = 0.0;
+    const bool unsteady = a.unsteady != 0;
+    const double iN = 1.0 / 24.0;
+    const double fv = a.q[a.dof_node[i]] * iN;
+    double bi = 0.0;
+    const uint64_t b = R.base[i / kWave] + (i % kWave);
+    const uint32_t cnt = R.count[i];
+    for (uint32_t kk = 0; kk < cnt; kk++) {
+        const uint64_t r = b + (uint64_t)kk * kWave;
+        const rec_d2 g01 = __builtin_nontemporal_load(reinterpret_cast<const rec_d2 *>(R.grad) + 2 * r);
+        const rec_d2 g23 = __builtin_nontemporal_load(reinterpret_cast<const rec_d2 *>(R.grad) + 2 * r + 1);
+        const double gg[4] = { g01.x, g01.y, g23.x, g23.y };
+        const double detJ = 6.0 * __builtin_nontemporal_load(R.vol + r);
+        const uint32_t em = __builtin_nontemporal_load(R.elem + r), orig = em & 0x3FFFFFFFu, m = em >> 30;
+        const uint32_t sl = __builtin_nontemporal_load(R.slots + r);
+        const double4 *te = reinterpret_cast<const double4 *>(a.telem + (uint64_t)kTElemWords * orig);
+        const double4 t0 = te[0], t1 = te[1];
+        const double vdiv[4] = { t0.x, t0.y, t0.z, t0.w };
+        const double Tsupg = t1.x, ca = t1.y, k = t1.z;
+        const double vm = m == 0 ? vdiv[0] : m == 1 ? vdiv[1] : m == 2 ? vdiv[2] : vdiv[3];
+        const bool lift = ((sl & 0xFFu) == kSlotFixed) | (((sl >> 8) & 0xFFu) == kSlotFixed) |
+                          (((sl >> 16) & 0xFFu) == kSlotFixed) | ((sl >> 24) == kSlotFixed);
+        double T[4] = { 0.0, 0.0, 0.0, 0.0 };
+        if (unsteady || lift) {
+            const uint4 nd4 = R.nodes[r];
+            T[0] = a.field[nd4.x];
+            T[1] = a.field[nd4.y];
+            T[2] = a.field[nd4.z];
+            T[3] = a.field[nd4.w];
+        }
+        double Ae[4], sum = 0.0;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            double rhs;
+            Ae[n] = transport_entry(ca, k, vm, vdiv[n], gg[n], Tsupg, unsteady, (uint32_t)n == m, a.alpha, a.dt, detJ,
+                                    T[n], rhs);
+            if (unsteady) sum += rhs;
+        }
+        double be = (unsteady ? a.dt * fv + sum : fv) * detJ;
+        // lifting of the prescribed columns, in the colour kernel's order
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+            if (((sl >> (8 * n)) & 0xFFu) == kSlotFixed) be -= Ae[n] * T[n];
+        bi += be;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const uint32_t slot = (sl >> (8 * n)) & 0xFFu;
+            if (slot != kSlotFixed) acc[slot][t] += Ae[n];
+        }
+    }
+    for (uint32_t j = 0; j < len; j++) a.val[r0 + j] = acc[j][t];
+    a.b[i] = bi;
 }
 
 // RSolverFluidHeat::solve / RSolverFluidParticle::solve write-back: field = max(x, 0) on the
@@ -1364,6 +1508,7 @@ struct rs_assembly {
     DBuf<uint32_t> conn, perm, map, dbook;
     DBuf<uint32_t> conn0;                   // connectivity in the caller's order (post-processing)
     DBuf<double> coeff;                     // mapped element coefficients (electrostatics)
+    DBuf<double> telem;                     // per-assembly transport element quantities (row gather)
     DBuf<double> part;                      // block partials of the fluid write-back norms
     DBuf<uint32_t> dof_node, inc_ptr, inc;  // row-gather heat assembly
     DBuf<double> rec_grad, rec_vol;         // row records (built once per mesh)
@@ -2197,14 +2342,26 @@ extern "C" int rs_assemble_transport_device(rs_assembly *S, const rs_transport_c
     hipStream_t st = runtime().stream;
     EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
-    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
-    if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
     TransportArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p), S->perm.p, S->map.p, S->dbook.p, d_ca, d_k,
                      d_velocity, d_node_source, d_node_field, S->A.val.p, d_b, conf->alpha, conf->dt,
                      conf->unsteady ? 1 : 0 };
-    for (uint32_t c = 0; c < S->n_colours; c++) {
-        const uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
-        if (e1 > e0) k_transport_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+    if (S->rows && S->use_rows) {
+        if (S->telem.n < (uint64_t)kTElemWords * S->n_elems &&
+            (rc = S->telem.alloc((uint64_t)kTElemWords * std::max(S->n_elems, 1u))))
+            return rc;
+        if (S->n_elems) k_transport_elem<<<vblocks(S->n_elems), kBlock, 0, st>>>(a, S->n_elems, S->telem.p);
+        RowRecords rec{ S->rec_grad.p, S->rec_vol.p, S->rec_elem.p, S->rec_slots.p, S->rec_nodes.p, S->rec_base.p,
+                        S->rec_count.p };
+        TransportRowArgs ra{ S->A.rowptr.p, S->dof_node.p, rec, S->n_dof, S->telem.p, d_node_source, d_node_field,
+                             S->A.val.p, d_b, conf->alpha, conf->dt, conf->unsteady ? 1 : 0 };
+        if (S->n_dof) k_transport_rows<<<grid_xcd(S->n_dof, kBlock), kBlock, 0, st>>>(ra);
+    } else {
+        if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+        if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
+        for (uint32_t c = 0; c < S->n_colours; c++) {
+            const uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+            if (e1 > e0) k_transport_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+        }
     }
     RS_HIP(hipGetLastError());
     if ((rc = timed_end(ev, S->last_ms))) return rc;

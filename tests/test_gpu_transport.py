@@ -59,3 +59,25 @@ def test_transport_matches_oracle(gpu, orc, unsteady, speed):
     ref = orc.transport_update(book, dx.download(), f_ref)
     np.testing.assert_array_equal(dfield.download(), f_ref)
     np.testing.assert_allclose(norms, ref, rtol=1e-13)
+
+
+@pytest.mark.parametrize("unsteady", [0, 1])
+@pytest.mark.parametrize("speed", [0.1, 50.0])
+def test_transport_row_gather_matches_colour_launches(gpu, unsteady, speed):
+    """The row gather (default) and the colour launches assemble the same bits: the same element
+    terms, summed per row in the same (colour) order."""
+    rs = gpu
+    xyz, tets, ca, k, vel, q, field, fixed = problem(rs, 7, speed, 60 + unsteady)
+    asm = rs.Assembly(xyz, tets, 1, fixed)
+    conf = rs.TransportConf(0.6, 0.02, unsteady)
+    args = [rs.DeviceArray.from_host(a.reshape(-1)) for a in (ca, k, vel, q, field)]
+    out = {}
+    for rows in (True, False):
+        asm.set_method(rows)
+        db = rs.DeviceArray(asm.n_dof)
+        asm.assemble_transport_device(conf, *args, db)
+        out[rows] = (asm.matrix.to_scipy(), db.download())
+    A_rows, A_col = out[True][0], out[False][0]
+    assert np.array_equal(A_rows.indptr, A_col.indptr) and np.array_equal(A_rows.indices, A_col.indices)
+    assert np.array_equal(A_rows.data, A_col.data)
+    assert np.array_equal(out[True][1], out[False][1])
