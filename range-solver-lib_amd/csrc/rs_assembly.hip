@@ -49,9 +49,12 @@ __global__ void k_zero(double *__restrict__ a, uint64_t n)
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) a[i] = 0.0;
 }
 
-// Gradients of the 4 linear shape functions and the volume of tet (X0..X3).
+// Gradients of the 4 linear shape functions and the volume of tet (X0..X3).  No contraction:
+// every kernel that inlines it gets the same bits (the geometry records of the row-gather
+// assemblies are computed by other kernels than the colour launches they must match).
 __device__ __forceinline__ double tet_gradients(const double (&X)[4][3], double (&g)[4][3])
 {
+#pragma clang fp contract(off)
     double a[3], b[3], c[3];
     for (int i = 0; i < 3; i++) {
         a[i] = X[1][i] - X[0][i];
@@ -370,8 +373,26 @@ struct MagnetoArgs {
     double *val, *b;
 };
 
+// The magnetostatic element's terms, without contraction (shared by the colour launches and
+// the row gather): K_mn = -(grad N_m . grad N_n) V on the three component diagonals, and the
+// source of node m, component c: V/4 eps0 (curl-type product of grad N_m with J_m).
+__device__ __forceinline__ double magneto_k(const double *gm, const double *gn, double V)
+{
+#pragma clang fp contract(off)
+    return -(dot3(gm, gn) * V);
+}
+__device__ __forceinline__ void magneto_source(const double *gm, const double *Jm, double V, double (&fe)[3])
+{
+#pragma clang fp contract(off)
+    const double fv = V * 0.25 * 8.854187817e-12;
+    fe[0] = fv * (-gm[2] * Jm[1] + gm[1] * Jm[2]);
+    fe[1] = fv * (gm[2] * Jm[0] - gm[0] * Jm[2]);
+    fe[2] = fv * (-gm[1] * Jm[0] + gm[0] * Jm[1]);
+}
+
 __global__ __launch_bounds__(kBlock) void k_magneto_colour(MagnetoArgs a, uint32_t e0, uint32_t e1)
 {
+#pragma clang fp contract(off)
     const uint32_t e = e0 + blockIdx.x * kBlock + threadIdx.x;
     if (e >= e1) return;
     const uint4 nd4 = a.conn[e];
@@ -386,12 +407,11 @@ __global__ __launch_bounds__(kBlock) void k_magneto_colour(MagnetoArgs a, uint32
 #pragma unroll
     for (int m = 0; m < 4; m++) {
 #pragma unroll
-        for (int n = 0; n < 4; n++) K[m][n] = -((g[m][0] * g[n][0] + g[m][1] * g[n][1] + g[m][2] * g[n][2]) * V);
-        const double *Jm = a.current + 3 * (uint64_t)nd[m];
-        const double fv = V * 0.25 * 8.854187817e-12;
-        fe[3 * m + 0] = fv * (-g[m][2] * Jm[1] + g[m][1] * Jm[2]);
-        fe[3 * m + 1] = fv * (g[m][2] * Jm[0] - g[m][0] * Jm[2]);
-        fe[3 * m + 2] = fv * (-g[m][1] * Jm[0] + g[m][0] * Jm[1]);
+        for (int n = 0; n < 4; n++) K[m][n] = magneto_k(g[m], g[n], V);
+        double f3[3];
+        magneto_source(g[m], a.current + 3 * (uint64_t)nd[m], V, f3);
+#pragma unroll
+        for (int c = 0; c < 3; c++) fe[3 * m + c] = f3[c];
     }
     bool free_[4];
 #pragma unroll
@@ -1412,6 +1432,90 @@ __global__ __launch_bounds__(kBlock) void k_stress_rows(StressRowArgs a)
     }
 }
 
+// Row gather of the magnetostatic element (3 unknowns per node, the component diagonals of each
+// 3 x 3 block), on the stress assembly's node lists and geometry records: one thread per node
+// with a free component sums K_mn of its incident elements (colour order) per neighbour block in
+// LDS, and the sources and the lifting of prescribed neighbours; then writes its three rows
+// whole (zeros off the diagonals) and b.  A node counts as prescribed when its first component
+// is, as in k_magneto_colour; same terms in the same order, so the same bits.
+struct MagnetoRowArgs {
+    const uint32_t *nodes;
+    uint32_t n_nodes;
+    const uint32_t *inc_ptr, *inc;
+    const uint32_t *nbr_ptr;
+    const uint2 *nbr;           // (node, offset of its first free column | free mask << 8)
+    const uint8_t *fmask;       // per node: free components
+    const double *geo;
+    const uint32_t *book, *rowptr;
+    const double *current, *field;
+    double *val, *b;
+};
+
+__global__ __launch_bounds__(kBlock) void k_magneto_rows(MagnetoRowArgs a)
+{
+#pragma clang fp contract(off)
+    __shared__ double acc[kStressMaxNbr][kBlock];
+    __shared__ uint32_t nbn[kStressMaxNbr][kBlock];
+    const uint32_t t = threadIdx.x;
+    const uint32_t k = xcd_remap(blockIdx.x, gridDim.x) * kBlock + t;
+    if (k >= a.n_nodes) return;
+    const uint32_t v = a.nodes[k];
+    const uint32_t b0 = a.nbr_ptr[k], nb = a.nbr_ptr[k + 1] - b0;
+    for (uint32_t j = 0; j < nb; j++) {
+        nbn[j][t] = a.nbr[b0 + j].x;
+        acc[j][t] = 0.0;
+    }
+    const bool vfree = (a.fmask[v] & 1u) != 0;
+    double bsum[3] = { 0.0, 0.0, 0.0 };
+    if (vfree) {
+        const double *Jv = a.current + 3 * (uint64_t)v;
+        for (uint32_t q = a.inc_ptr[v]; q < a.inc_ptr[v + 1]; q++) {
+            const uint32_t wd = a.inc[q], e = wd >> 2, m = wd & 3u;
+            const double *r = a.geo + (uint64_t)kGeoWords * e;
+            double g[4][3];
+#pragma unroll
+            for (int i = 0; i < 12; i++) g[i / 3][i % 3] = r[i];
+            const double V = r[12];
+            const uint32_t fm = reinterpret_cast<const uint2 *>(r)[13].x;
+            const uint4 nd4 = reinterpret_cast<const uint4 *>(r)[7];
+            const uint32_t nd[4] = { nd4.x, nd4.y, nd4.z, nd4.w };
+            double gm[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) gm[d] = m == 0 ? g[0][d] : m == 1 ? g[1][d] : m == 2 ? g[2][d] : g[3][d];
+            double fe[3];
+            magneto_source(gm, Jv, V, fe);
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const double K = magneto_k(gm, g[n], V);
+                if (!((fm >> (8 * n)) & 1u)) {          // prescribed neighbour: lifting
+                    const double *H = a.field + 3 * (uint64_t)nd[n];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) fe[c] -= K * H[c];
+                } else {
+                    uint32_t j = 0;
+                    while (j + 1 < nb && nbn[j][t] != nd[n]) j++;
+                    acc[j][t] += K;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) bsum[c] += fe[c];
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        const uint32_t row = a.book[3 * (uint64_t)v + c];
+        if (row == RS_EOD) continue;
+        double *dst = a.val + a.rowptr[row];
+        for (uint32_t j = 0; j < nb; j++) {
+            const uint32_t w = a.nbr[b0 + j].y, off = w & 0xFFu, fmu = w >> 8;
+            const double kv = (fmu & 1u) ? acc[j][t] : 0.0;
+            uint32_t p = off;
+            for (int c2 = 0; c2 < 3; c2++)
+                if ((fmu >> c2) & 1u) dst[p++] = c2 == c ? kv : 0.0;
+        }
+        a.b[row] = bsum[c];
+    }
+}
+
 // RSolverStress::setDisplacement (src/rsolverstress.cpp:800-836): du = x on the enabled
 // components, 0 on the disabled ones; an active local rotation writes iR^T du
 // (RLocalRotation::rotateResultsVector, src/rlocalrotation.cpp:60-77).
@@ -1509,6 +1613,7 @@ struct rs_assembly {
     DBuf<uint32_t> conn0;                   // connectivity in the caller's order (post-processing)
     DBuf<double> coeff;                     // mapped element coefficients (electrostatics)
     DBuf<double> telem;                     // per-assembly transport element quantities (row gather)
+    int whole_nodes = -1;                   // 3-DOF book prescribes whole nodes only (-1: not checked)
     DBuf<double> part;                      // block partials of the fluid write-back norms
     DBuf<uint32_t> dof_node, inc_ptr, inc;  // row-gather heat assembly
     DBuf<double> rec_grad, rec_vol;         // row records (built once per mesh)
@@ -2244,18 +2349,40 @@ extern "C" int rs_assemble_magnetostatics_device(rs_assembly *S, const double *d
     if (!S || !d_current_node || !d_field_node || !d_b)
         return fail(RS_ERR_INVALID_ARG, "rs_assemble_magnetostatics_device: null argument");
     if (S->dpn != 3) return fail(RS_ERR_INVALID_ARG, "rs_assemble_magnetostatics_device: assembly needs 3 dofs per node");
+    if (S->whole_nodes < 0) {
+        // the reference's magnetostatic node book is per node (src/rsolvermagnetostatics.cpp:
+        // 241-261): a node's field is prescribed or free as a whole
+        S->whole_nodes = 1;
+        for (uint32_t v = 0; v < S->n_nodes && S->whole_nodes; v++) {
+            const uint32_t nfix = (S->book[3ull * v] == RS_EOD) + (S->book[3ull * v + 1] == RS_EOD) +
+                                  (S->book[3ull * v + 2] == RS_EOD);
+            if (nfix != 0 && nfix != 3) S->whole_nodes = 0;
+        }
+    }
+    if (!S->whole_nodes)
+        return fail(RS_ERR_INVALID_ARG,
+                    "rs_assemble_magnetostatics_device: a node has some field components prescribed and others free; "
+                    "magnetostatics prescribes whole nodes");
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;
     EventPair ev;
     if ((rc = timed_begin(ev))) return rc;
-    if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
-    if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
-    MagnetoArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p), S->map.p, S->dbook.p, d_current_node,
-                   d_field_node, S->A.val.p, d_b };
-    for (uint32_t c = 0; c < S->n_colours; c++) {
-        const uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
-        if (e1 > e0) k_magneto_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+    if (S->srows && S->use_rows) {
+        MagnetoRowArgs r{ S->s_nodes.p, S->s_count, S->inc_ptr.p, S->inc.p, S->s_nbr_ptr.p,
+                          reinterpret_cast<const uint2 *>(S->s_nbr.p), S->s_fmask.p, S->s_geo.p, S->dbook.p,
+                          S->A.rowptr.p, d_current_node, d_field_node, S->A.val.p, d_b };
+        if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);   // rows of no free node
+        if (S->s_count) k_magneto_rows<<<grid_xcd(S->s_count, kBlock), kBlock, 0, st>>>(r);
+    } else {
+        if (S->A.nnz) k_zero<<<vblocks(S->A.nnz), kBlock, 0, st>>>(S->A.val.p, S->A.nnz);
+        if (S->n_dof) k_zero<<<vblocks(S->n_dof), kBlock, 0, st>>>(d_b, S->n_dof);
+        MagnetoArgs a{ S->xyz.p, reinterpret_cast<const uint4 *>(S->conn.p), S->map.p, S->dbook.p, d_current_node,
+                       d_field_node, S->A.val.p, d_b };
+        for (uint32_t c = 0; c < S->n_colours; c++) {
+            const uint32_t e0 = S->colour_off[c], e1 = S->colour_off[c + 1];
+            if (e1 > e0) k_magneto_colour<<<grid_for(e1 - e0, kBlock), kBlock, 0, st>>>(a, e0, e1);
+        }
     }
     RS_HIP(hipGetLastError());
     if ((rc = timed_end(ev, S->last_ms))) return rc;

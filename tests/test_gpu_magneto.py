@@ -38,3 +38,45 @@ def test_magnetostatics_matches_oracle_and_solves(gpu, orc):
     x_ref = spla.spsolve(A_ref.tocsc(), b_ref)
     xg = dx.download()
     assert np.linalg.norm(xg - x_ref) <= 1e-8 * np.linalg.norm(x_ref)
+
+
+def test_magnetostatics_row_gather_matches_colour_launches(gpu):
+    """The row gather (default) and the colour launches assemble the same bits."""
+    rs = gpu
+    n = 7
+    xyz, tets = rs.kuhn_box(n, n, n)
+    nn = xyz.shape[0]
+    x = xyz[:, 0]
+    fixed_node = (np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12)
+    fixed = np.repeat(fixed_node, 3).astype(np.uint8)
+    rng = np.random.default_rng(8)
+    J = rng.standard_normal((nn, 3)) * 1e6
+    H = rng.standard_normal((nn, 3))
+    asm = rs.Assembly(xyz, tets, 3, fixed)
+    dJ, dH = rs.DeviceArray.from_host(J.reshape(-1)), rs.DeviceArray.from_host(H.reshape(-1))
+    out = {}
+    for rows in (True, False):
+        asm.set_method(rows)
+        db = rs.DeviceArray(asm.n_dof)
+        asm.assemble_magnetostatics_device(dJ, dH, db)
+        out[rows] = (asm.matrix.to_scipy(), db.download())
+    A_rows, A_col = out[True][0], out[False][0]
+    assert np.array_equal(A_rows.indptr, A_col.indptr) and np.array_equal(A_rows.indices, A_col.indices)
+    assert np.array_equal(A_rows.data, A_col.data)
+    assert np.array_equal(out[True][1], out[False][1])
+
+
+def test_magnetostatics_rejects_partly_prescribed_nodes(gpu):
+    """The reference's magnetostatic book is per node: a 3-DOF book that prescribes some
+    components of a node and not others is refused on the host, before any launch."""
+    rs = gpu
+    xyz, tets = rs.kuhn_box(4, 4, 4)
+    nn = xyz.shape[0]
+    fixed = np.repeat(np.abs(xyz[:, 0]) < 1e-12, 3).astype(np.uint8)
+    fixed[3 * 31 + 2] = 1          # node 31 is at x = 0.25: free before
+    asm = rs.Assembly(xyz, tets, 3, fixed)
+    zeros = rs.DeviceArray.from_host(np.zeros(3 * nn))
+    for rows in (True, False):
+        asm.set_method(rows)
+        with pytest.raises(rs.RSError, match="whole nodes"):
+            asm.assemble_magnetostatics_device(zeros, zeros, rs.DeviceArray(asm.n_dof))
