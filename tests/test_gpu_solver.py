@@ -227,6 +227,29 @@ def test_gmres_matches_oracle(gpu, orc, ptype):
     assert S.info.true_relres < 1e-6
 
 
+@pytest.mark.parametrize("n", [300, 2100])
+def test_gmres_full_cycle_on_the_cyclic_shift(gpu, n):
+    """GMRES(n) on the cyclic shift (A e_i = e_(i+1 mod n)) with b = e_0: the Krylov space is
+    e_0 .. e_(k-1), the residual stays 1 until the n-th inner iteration, and x = e_(n-1) exactly.
+    n = 2100 runs the device rotation chain and back substitution past their 2048-entry LDS
+    staging (the in-place path)."""
+    rs = gpu
+    _say(f"gmres cyclic shift n={n}")
+    rows = np.arange(n)
+    A = sp.csr_matrix((np.ones(n), ((rows + 1) % n, rows)), shape=(n, n))
+    A.sort_indices()
+    b = np.zeros(n)
+    b[0] = 1.0
+    conf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=3, n_inner_iterations=n,
+                               solver_cvg_value=1e-12)
+    S = rs.RMatrixSolver(conf)
+    x = S.solve(rs.SparseMatrix.from_scipy(A), b, None, rs.PRECOND_NONE, 1)
+    expect = np.zeros(n)
+    expect[n - 1] = 1.0
+    assert S.info.inner_iterations == n
+    assert np.abs(x - expect).max() < 1e-12
+
+
 def test_convergence_log_format(gpu, tmp_path):
     """RIterationInfo::writeToFile: header line, then `it+1|error|trend` in %.7e."""
     rs = gpu
