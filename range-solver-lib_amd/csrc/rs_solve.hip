@@ -608,7 +608,31 @@ struct CgArgs {
     double *partials;
     SolveState *st;
     double *hist;
+    uint32_t stream;      // the update and direction kernels stream their vectors non-temporally
 };
+
+// FP64 CG vectors of at least this size are streamed non-temporally by the update and direction
+// kernels: their ~7 vector passes per iteration then leave the L2 and the 256 MB MALL to the
+// matrix and to the direction the SpMV gathers (128^3 heat, 17 MB per vector: FP64 Jacobi-CG
+// 116-117 -> 112-114 ms).  Smaller vectors stay cached: they fit the 8 XCDs' L2s, and streaming
+// them costs (cantilever, 3.3 MB: 146 -> 152 ms).  The FP32 vectors of the mixed and flexible CG
+// stay cached too (streaming them made the mixed solve 74 -> 78 ms).
+constexpr uint64_t kStreamVectorBytes = 8ull << 20;
+
+// (A runtime choice between the two loads of one address lets the compiler merge them into one
+// plain load, so the choice is a template parameter and the kernels branch once, on entry.)
+template <bool NT, typename T>
+__device__ __forceinline__ T vload(const T *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void vstore(T *p, T v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 __global__ void k_state_init(SolveState *st, double an, double cvg, double prev_error,
                              uint32_t n_outer, uint32_t stop_rule, uint32_t hist_cap, uint32_t mode)
@@ -694,10 +718,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_rho(CgArgs a)
 }
 
 // p = z (first iteration) or p*beta + z   (src/rmatrixsolver.cpp:278-289)
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs a)
+template <int W, bool NT>
+__device__ __forceinline__ void cg_dir(const CgArgs &a)
 {
-    if (a.st->done) return;
     constexpr int WM = W == 0 ? 16 : W;
     const uint32_t ww = W == 0 ? a.pw : W;
     const uint64_t nb = a.A.n / ww;
@@ -705,13 +728,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs a)
     const double beta = a.st->beta;
     for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
         double r[WM], z[WM];
-        for (uint32_t k = 0; k < ww; k++) r[k] = a.r[ib * ww + k];
+        for (uint32_t k = 0; k < ww; k++) r[k] = vload<NT>(a.r + ib * ww + k);
         pc_apply_block<W, double, double>(a.ptype, ww, a.pd, ib, r, z);
         for (uint32_t k = 0; k < ww; k++) {
             uint64_t i = ib * ww + k;
-            a.p[i] = first ? z[k] : a.p[i] * beta + z[k];
+            a.p[i] = first ? z[k] : vload<NT>(a.p + i) * beta + z[k];
         }
     }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs a)
+{
+    if (a.st->done) return;
+    if (a.stream) cg_dir<W, true>(a);
+    else cg_dir<W, false>(a);
 }
 
 // q = A p, partial p.q   (src/rmatrixsolver.cpp:295-300)
@@ -725,11 +756,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs a)
 }
 
 // x += alpha p, r -= alpha q, partial |x|^2, |r|^2 and r.P r  (src/rmatrixsolver.cpp:324-331)
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
+template <int W, bool NT>
+__device__ __forceinline__ void cg_update(const CgArgs &a)
 {
     SolveState *st = a.st;
-    if (st->done) return;
     constexpr int WM = W == 0 ? 16 : W;
     const uint32_t ww = W == 0 ? a.pw : W;
     const uint64_t nb = a.A.n / ww;
@@ -739,10 +769,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
         double r[WM], z[WM];
         for (uint32_t k = 0; k < ww; k++) {
             uint64_t i = ib * ww + k;
-            double xi = a.x[i] + alpha * a.p[i];
-            double ri = a.r[i] - alpha * a.q[i];
-            a.x[i] = xi;
-            a.r[i] = ri;
+            double xi = vload<NT>(a.x + i) + alpha * vload<NT>(a.p + i);
+            double ri = vload<NT>(a.r + i) - alpha * vload<NT>(a.q + i);
+            vstore<NT>(a.x + i, xi);
+            vstore<NT>(a.r + i, ri);
             v[0] += xi * xi;
             v[1] += ri * ri;
             r[k] = ri;
@@ -751,6 +781,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
         for (uint32_t k = 0; k < ww; k++) v[2] += r[k] * z[k];
     }
     store_partials<3>(v, a.partials);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
+{
+    if (a.st->done) return;
+    if (a.stream) cg_update<W, true>(a);
+    else cg_update<W, false>(a);
 }
 
 // ============================================================================ CG (mixed)
@@ -1952,6 +1990,7 @@ static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, ui
     a.partials = S.partials.p;
     a.st = S.st.p;
     a.hist = S.hist.p;
+    a.stream = (uint64_t)A.n * sizeof(double) >= kStreamVectorBytes;
     uint32_t w = a.pw;
     uint32_t vb = vec_blocks(A.n / w);
     hipStream_t stream = runtime().stream;
@@ -2432,6 +2471,7 @@ static int solve_device_noalloc(rs_solver &S, rs_matrix &A, const double *d_b, d
     a.partials = partials;
     a.st = st;
     a.hist = hist;
+    a.stream = (uint64_t)n * sizeof(double) >= kStreamVectorBytes;
     const uint32_t vb = vec_blocks(n / w);
     SolveState h{};
     if (n > 0) {
