@@ -16,9 +16,10 @@
 //             inverse (the ceiling a coarse solve can reach).
 // Network: features [u/sigma, log d - mean, 1, 0] with u = r_c / diag(A_c), sigma = rms(u);
 // messages are neighbour features weighted by -A_c[I,J]/A_c[I,I]; Y = H Ws + M Wn + b,
-// H += relu(Y); BF16 weights and activations on MFMA (v_mfma_f32_16x16x16_bf16), FP32
-// accumulation; output e = u + sigma * (H wo + bo).  Inference runs in one workgroup with the
-// coarsest graph resident in LDS.
+// H += relu(Y); BF16 weights and activations on gfx950's K = 32 MFMA (v_mfma_f32_16x16x32_bf16;
+// the K = 16 form for an odd last 16 of K), FP32 accumulation; output e = u + sigma * (H wo + bo).
+// Inference: the embedding and one launch per layer, one workgroup per 16-node strip of the
+// coarsest graph.
 // Training (one workgroup per sample, activations in global memory): forward, the loss
 // (e.A_c.e - 2 e.r_c)/s + 1 with s = r_c.A_c^-1.r_c -- the A_c-norm error fraction the
 // correction leaves -- backward, per-sample gradients, a fixed-order batch mean and Adam.
@@ -42,6 +43,7 @@ __host__ __device__ constexpr uint32_t off_wn(int l) { return off_ws(l) + kHid *
 __host__ __device__ constexpr uint32_t off_b(int l) { return off_ws(l) + 2 * kHid * kHid; }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950).
@@ -49,18 +51,29 @@ __device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_c
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 
 // One 16x16 output tile: acc += A(16 x K) B(K x 16), K a multiple of 16, operands fetched as
-// BF16 bit patterns through a(row, k) and b(k, col).  Lane l supplies A[l%16][4(l/16)+i] and
-// B[4(l/16)+i][l%16]; acc[i] is C[4(l/16)+i][l%16].
+// BF16 bit patterns through a(row, k) and b(k, col): 32 of K per gfx950 K = 32 MFMA (lane l
+// supplies A[l%16][8(l/16)+i] and B[8(l/16)+i][l%16], i < 8), a last 16 on the K = 16 form
+// (A[l%16][4(l/16)+i], B[4(l/16)+i][l%16], i < 4); acc[i] is C[4(l/16)+i][l%16].
 template <class FA, class FB>
 __device__ __forceinline__ f32x4 mfma_tile(FA a, FB b, int K, f32x4 acc)
 {
-    const int lane = threadIdx.x & 63, rc = lane & 15, kq = (lane >> 4) * 4;
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    const int lane = threadIdx.x & 63, rc = lane & 15, q = lane >> 4;
+    int k0 = 0;
+    for (; k0 + 32 <= K; k0 += 32) {
+        bf16x8 av, bv;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            av[i] = __builtin_bit_cast(__bf16, (unsigned short)a(rc, k0 + 8 * q + i));
+            bv[i] = __builtin_bit_cast(__bf16, (unsigned short)b(k0 + 8 * q + i, rc));
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+    if (k0 < K) {
         bf16x4 av, bv;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            av[i] = (short)a(rc, k0 + kq + i);
-            bv[i] = (short)b(k0 + kq + i, rc);
+            av[i] = (short)a(rc, k0 + 4 * q + i);
+            bv[i] = (short)b(k0 + 4 * q + i, rc);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
     }
@@ -694,7 +707,16 @@ struct GnnArgs {
 
 __device__ __forceinline__ bf16x4 bf16x4_of(uint2 w) { return __builtin_bit_cast(bf16x4, w); }
 
-constexpr int kInferMaxBlk = 9;     // graph blocks per strip prefetched into registers
+// gfx950's K = 32 BF16 MFMA (v_mfma_f32_16x16x32_bf16): lane l supplies A[l%16][8(l/16)+i] and
+// B[8(l/16)+i][l%16] (i = 0..7, one 16-byte load each); the accumulator layout is that of the
+// K = 16 form, acc[i] = C[4(l/16)+i][l%16].
+__device__ __forceinline__ bf16x8 bf16x8_of(uint4 w) { return __builtin_bit_cast(bf16x8, w); }
+__device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 acc)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf16x8_of(a), bf16x8_of(b), acc, 0, 0, 0);
+}
+
+constexpr int kInferMaxPairs = 5;   // pairs of graph blocks per strip prefetched into registers
 
 // Input features and the first activations of one strip of 16 nodes (64 threads, lane = feature).
 __global__ __launch_bounds__(kWave) void k_gnn_embed(GnnArgs g)
@@ -747,35 +769,46 @@ __global__ __launch_bounds__(4 * kWave) void k_gnn_layer(GnnArgs g, int l)
     const unsigned short *H = g.h[l & 1], *Ht = g.ht[l & 1];
     unsigned short *Hn = g.h[(l + 1) & 1], *Htn = g.ht[(l + 1) & 1];
     const float *P = g.params;
-    // operands of the output tile first: they do not depend on the messages
+    // operands of the output tile first: they do not depend on the messages (K = 32 MFMA: two
+    // 32-feature halves, 8 consecutive features per lane)
     const unsigned short *Wl = g.wbt + (size_t)l * kBfPerLayer;
-    const int ct = w;
-    uint2 ws[4], wn[4], ha[4];
+    const int ct = w, q = lane >> 4, k8 = 8 * q;
+    uint4 ws[2], wn[2], ha[2];
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++) {
-        ws[kt] = *reinterpret_cast<const uint2 *>(Wl + (ct * 16 + rc) * kHid + kt * 16 + kq);
-        wn[kt] = *reinterpret_cast<const uint2 *>(Wl + 4096 + (ct * 16 + rc) * kHid + kt * 16 + kq);
-        ha[kt] = *reinterpret_cast<const uint2 *>(H + (r0 + rc) * kHid + kt * 16 + kq);
+    for (int kb = 0; kb < 2; kb++) {
+        ws[kb] = *reinterpret_cast<const uint4 *>(Wl + (ct * 16 + rc) * kHid + 32 * kb + k8);
+        wn[kb] = *reinterpret_cast<const uint4 *>(Wl + 4096 + (ct * 16 + rc) * kHid + 32 * kb + k8);
+        ha[kb] = *reinterpret_cast<const uint4 *>(H + (r0 + rc) * kHid + 32 * kb + k8);
     }
     float hold[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) hold[i] = bf2f(H[(r0 + kq + i) * kHid + ct * 16 + rc]);
     const float bias = P[off_b(l) + ct * 16 + rc];
-    // messages of feature block kt = w
-    const uint32_t b0 = g.bptr[s], nb = g.bptr[s + 1] - b0, last = b0 + nb - 1;
-    const int kt = w;
-    uint2 bv[kInferMaxBlk], av[kInferMaxBlk];
-#pragma unroll
-    for (int j = 0; j < kInferMaxBlk; j++) {
-        const uint32_t b = min(b0 + j, last);
-        bv[j] = *reinterpret_cast<const uint2 *>(g.bval + (uint64_t)b * 256 + rc * 16 + kq);
-        av[j] = *reinterpret_cast<const uint2 *>(Ht + (kt * 16 + rc) * npad + g.bcol[b] * 16 + kq);
-    }
+    // messages of feature block kt = w, graph blocks two at a time (lanes q < 2 take the pair's
+    // first block, q >= 2 its second, 8 of its 16 nodes each); an odd last block on the K = 16 form
+    const uint32_t b0 = g.bptr[s], nb = g.bptr[s + 1] - b0, npair = nb / 2;
+    const int kt = w, h8 = 8 * (q & 1), second = q >> 1;
     f32x4 macc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+    if (npair) {
+        uint4 bv[kInferMaxPairs], av[kInferMaxPairs];
 #pragma unroll
-    for (int j = 0; j < kInferMaxBlk; j++)
-        if ((uint32_t)j < nb) macc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4_of(av[j]), bf16x4_of(bv[j]), macc, 0, 0, 0);
-    for (uint32_t b = b0 + kInferMaxBlk; b < b0 + nb; b++) {
+        for (int j = 0; j < kInferMaxPairs; j++) {
+            const uint32_t b = b0 + 2 * min((uint32_t)j, npair - 1) + second;
+            bv[j] = *reinterpret_cast<const uint4 *>(g.bval + (uint64_t)b * 256 + rc * 16 + h8);
+            av[j] = *reinterpret_cast<const uint4 *>(Ht + (kt * 16 + rc) * npad + g.bcol[b] * 16 + h8);
+        }
+#pragma unroll
+        for (int j = 0; j < kInferMaxPairs; j++)
+            if ((uint32_t)j < npair) macc = mfma32(av[j], bv[j], macc);
+        for (uint32_t j = kInferMaxPairs; j < npair; j++) {
+            const uint32_t b = b0 + 2 * j + second;
+            const uint4 bb = *reinterpret_cast<const uint4 *>(g.bval + (uint64_t)b * 256 + rc * 16 + h8);
+            const uint4 aa = *reinterpret_cast<const uint4 *>(Ht + (kt * 16 + rc) * npad + g.bcol[b] * 16 + h8);
+            macc = mfma32(aa, bb, macc);
+        }
+    }
+    if (nb & 1u) {
+        const uint32_t b = b0 + nb - 1;
         const uint2 bb = *reinterpret_cast<const uint2 *>(g.bval + (uint64_t)b * 256 + rc * 16 + kq);
         const uint2 aa = *reinterpret_cast<const uint2 *>(Ht + (kt * 16 + rc) * npad + g.bcol[b] * 16 + kq);
         macc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4_of(aa), bf16x4_of(bb), macc, 0, 0, 0);
@@ -790,10 +823,10 @@ __global__ __launch_bounds__(4 * kWave) void k_gnn_layer(GnnArgs g, int l)
     __syncthreads();
     f32x4 acc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const uint2 ma = *reinterpret_cast<const uint2 *>(&sM[rc * kHid + k * 16 + kq]);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4_of(ha[k]), bf16x4_of(ws[k]), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4_of(ma), bf16x4_of(wn[k]), acc, 0, 0, 0);
+    for (int kb = 0; kb < 2; kb++) {
+        const uint4 ma = *reinterpret_cast<const uint4 *>(&sM[rc * kHid + 32 * kb + k8]);
+        acc = mfma32(ha[kb], ws[kb], acc);
+        acc = mfma32(ma, wn[kb], acc);
     }
     const uint32_t f = ct * 16 + rc;
     float hv[4];
