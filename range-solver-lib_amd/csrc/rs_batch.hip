@@ -353,10 +353,19 @@ __global__ __launch_bounds__(W * kWave) void k_bt_spmv(BtArgs a)
     const auto cols = bt_cols<OFF>(a);
     constexpr uint32_t R = kWave / G;
     const uint32_t sub = lane / R;
+    // the next slice's system and row range are loaded while this slice's rows run
+    uint32_t nsys = lw < a.nslices ? a.slice_sys[lw] : 0;
+    uint32_t nb0 = lw < a.nslices ? a.sell_ptr[lw] : 0, nb1 = lw < a.nslices ? a.sell_ptr[lw + 1] : 0;
     for (uint32_t s = lw; s < a.nslices; s += nw) {
-        if (!bt_iterating(a.sys[a.slice_sys[s]].flags)) continue;
+        const uint32_t sys = nsys, base = nb0, width = (nb1 - nb0) / kSlice;
+        const uint32_t sn = s + nw;
+        if (sn < a.nslices) {
+            nsys = a.slice_sys[sn];
+            nb0 = a.sell_ptr[sn];
+            nb1 = a.sell_ptr[sn + 1];
+        }
+        if (!bt_iterating(a.sys[sys].flags)) continue;
         const uint32_t sbase = s * kSlice;
-        const uint32_t base = a.sell_ptr[s], width = (a.sell_ptr[s + 1] - base) / kSlice;
         double acc = 0.0;
 #pragma unroll
         for (int pass = 0; pass < G; pass++) {
