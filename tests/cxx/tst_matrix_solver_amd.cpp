@@ -1,6 +1,6 @@
 // Drop-in check of the C++ call surface: the known-answer cases of the reference's
 // tests/tst_matrix_solver.cpp:33-77 (CG, preconditioner NONE, cvg 1e-13, 1000 iterations,
-// tolerance 1e-6) through rsamd::RMatrixSolver, plus the error behaviour of an unknown
+// tolerance 1e-6) through rsamd::RMatrixSolver (and the first through GMRES), plus the error behaviour of an unknown
 // preconditioner type, the copy semantics of RMatrixPreconditioner, the bounded device-matrix
 // cache, and an eigenvalue solve through rsamd::REigenValueSolver.  Exit status 0
 // when every check passes.
@@ -51,6 +51,18 @@ int main()
         EXPECT(std::fabs(x[0] - 1.0 / 11.0) < tol);
         EXPECT(std::fabs(x[1] - 7.0 / 11.0) < tol);
         std::printf("spd 2x2: x = [%.12f, %.12f] in %u iterations\n", x[0], x[1], solver.lastInfo().iterations);
+        // the same known answer through GMRES (RMatrixSolverConf::GMRES, src/rmatrixsolver.cpp:357-581;
+        // the reference's tests solve with CG only)
+        RMatrixSolverConf gconf(RMatrixSolverConf::GMRES);
+        gconf.setNInnerIterations(10);
+        gconf.setNOuterIterations(100);
+        gconf.setSolverCvgValue(1.0e-13);
+        gconf.setOutputFrequency(0);
+        RMatrixSolver gsolver(gconf);
+        RRVector xg;
+        gsolver.solve(a, b, xg, R_MATRIX_PRECONDITIONER_JACOBI, 1);
+        EXPECT(std::fabs(xg[0] - 1.0 / 11.0) < tol);
+        EXPECT(std::fabs(xg[1] - 7.0 / 11.0) < tol);
     }
     {
         RSparseMatrix a;
