@@ -1174,7 +1174,9 @@ struct FcgArgs {
     uint32_t n;
     float *r, *z, *p, *q, *dx;
     double *partials;
+    uint32_t *arrive;     // kArriveWords arrival counters, zero between launches
     SolveState *st;
+    double *hist;
 };
 
 // p = z (first iteration of a segment) or z + beta p
@@ -1187,7 +1189,7 @@ __global__ void k_fcg_p(FcgArgs a)
         a.p[i] = first ? a.z[i] : fmaf(beta, a.p[i], a.z[i]);
 }
 
-// dx += alpha p, r -= alpha q, partial |r|^2
+// dx += alpha p, r -= alpha q, |r|^2, then the stopping rule and a replacement request
 __global__ __launch_bounds__(kBlock) void k_fcg_update(FcgArgs a)
 {
     if (a.st->done) return;
@@ -1199,7 +1201,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(FcgArgs a)
         a.r[i] = ri;
         v[0] += (double)ri * (double)ri;
     }
-    store_partials<1>(v, a.partials);
+    finalize_in_launch<1, FIN_FCG_UPDATE>(v, a.partials, a.arrive, a.st, a.hist);
 }
 
 }  // namespace rs
@@ -1571,6 +1573,7 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
     a.pw = 1;
     a.pd = P.data32.p;
     a.partials = S.partials.p;
+    a.arrive = reinterpret_cast<uint32_t *>(S.partials.p + kPartialDoubles);
     a.st = S.st.p;
     a.hist = S.hist.p;
     MpArgs a64 = a;
@@ -1582,7 +1585,9 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
     f.q = S.q32.p;
     f.dx = S.dx32.p;
     f.partials = S.partials.p;
+    f.arrive = a.arrive;
     f.st = S.st.p;
+    f.hist = S.hist.p;
     const uint32_t nvb = vec_blocks(A.n);
     hipStream_t stream = runtime().stream;
     L.r0 = S.r32.p;
@@ -1591,6 +1596,7 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
     const uint32_t nbz = step_blocks(*L.lv[0]);
     k_fill32<<<nvb, kBlock, 0, stream>>>(S.dx32.p, A.n, 0.0f);
     k_fill32<<<nvb, kBlock, 0, stream>>>(S.q32.p, A.n, 0.0f);
+    k_zero_u32<<<1, 64, 0, stream>>>(a.arrive, kArriveWords);
     RS_SHAPE_SWITCH(plan64, k_mp_true_resid, nb64, stream, a64);
     fin(S.partials.p, nb64, 2, FIN_MP_TRUE, 1, S.st.p, S.hist.p);
     RS_HIP(hipGetLastError());
@@ -1605,14 +1611,12 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
             FcgArgs fz = f;
             fz.z = const_cast<float *>(z);
             k_fcg_p<<<nvb, kBlock, 0, stream>>>(fz);
-            RS_SHAPE_SWITCH(plan32, k_mp_spmv_dot, nb32, stream, a);
-            fin(S.partials.p, nb32, 1, FIN_DOT, 0, S.st.p, S.hist.p);
-            k_fcg_update<<<nvb, kBlock, 0, stream>>>(f);
-            fin(S.partials.p, nvb, 1, FIN_FCG_UPDATE, 0, S.st.p, S.hist.p);
-            // FP64 residual replacement when requested, inside the graph (no host round trip)
+            RS_SHAPE_SWITCH(plan32, k_mp_spmv_dot, nb32, stream, a);   // finalizes alpha
+            k_fcg_update<<<nvb, kBlock, 0, stream>>>(f);                // finalizes the stopping rule
+            // FP64 residual replacement when requested, inside the graph (no host round trip);
+            // the true-residual launch finalizes it
             k_mp_fold_cond<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n, S.st.p);
             RS_SHAPE_SWITCH(plan64, k_mp_true_resid_cond, nb64, stream, a64);
-            fin(S.partials.p, nb64, 2, FIN_MP_REPLACE, 0, S.st.p, S.hist.p);
         }
     };
     rc = capture(S, body, graph_key_of(3, a, plan32, nb32, nb64, nvb) + learned_key(L) + std::to_string(nbz));

@@ -87,32 +87,9 @@ enum FinOp {
 
 constexpr int kFinBlock = 1024;
 
-__global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ partials, uint32_t nparts,
-                                                   int nv, int op, int flag, SolveState *st,
-                                                   double *__restrict__ hist)
+// The solver scalars from the finalized sums v (thread 0 of the finalizing workgroup).
+__device__ __forceinline__ void fin_apply(int op, int flag, const double (&v)[4], SolveState *st, double *__restrict__ hist)
 {
-    if ((op == FIN_DOT || op == FIN_UPDATE || op == FIN_MP_UPDATE || op == FIN_FCG_BETA || op == FIN_FCG_UPDATE) &&
-        st->done)
-        return;
-    if (op == FIN_MP_REPLACE && (st->done || !st->replace)) return;
-    __shared__ double lds[4 * (kFinBlock / kWave)];
-    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
-    // all loads of a thread are independent and issued before the sums (up to 4 partials per
-    // value and thread at the launch sizes used here)
-    double ld[4][4];
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            uint32_t i = threadIdx.x + u * kFinBlock;
-            ld[k][u] = (k < nv && i < nparts) ? partials[(uint64_t)k * nparts + i] : 0.0;
-        }
-#pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = (ld[k][0] + ld[k][1]) + (ld[k][2] + ld[k][3]);
-    for (uint32_t i = threadIdx.x + 4 * kFinBlock; i < nparts; i += kFinBlock)
-        for (int k = 0; k < nv; k++) v[k] += partials[(uint64_t)k * nparts + i];
-    block_sum<4, kFinBlock>(v, lds);
-    if (threadIdx.x != 0) return;
     const double eps = kEps;
     switch (op) {
         case FIN_NORM0: st->t0 = sqrt(v[0]); break;
@@ -273,6 +250,110 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ pa
             }
             break;
         }
+    }
+}
+
+
+__global__ __launch_bounds__(kFinBlock) void k_fin(const double *__restrict__ partials, uint32_t nparts,
+                                                   int nv, int op, int flag, SolveState *st,
+                                                   double *__restrict__ hist)
+{
+    if ((op == FIN_DOT || op == FIN_UPDATE || op == FIN_MP_UPDATE || op == FIN_FCG_BETA || op == FIN_FCG_UPDATE) &&
+        st->done)
+        return;
+    if (op == FIN_MP_REPLACE && (st->done || !st->replace)) return;
+    __shared__ double lds[4 * (kFinBlock / kWave)];
+    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+    // all loads of a thread are independent and issued before the sums (up to 4 partials per
+    // value and thread at the launch sizes used here)
+    double ld[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            uint32_t i = threadIdx.x + u * kFinBlock;
+            ld[k][u] = (k < nv && i < nparts) ? partials[(uint64_t)k * nparts + i] : 0.0;
+        }
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (ld[k][0] + ld[k][1]) + (ld[k][2] + ld[k][3]);
+    for (uint32_t i = threadIdx.x + 4 * kFinBlock; i < nparts; i += kFinBlock)
+        for (int k = 0; k < nv; k++) v[k] += partials[(uint64_t)k * nparts + i];
+    block_sum<4, kFinBlock>(v, lds);
+    if (threadIdx.x == 0) fin_apply(op, flag, v, st, hist);
+}
+
+// k_fin folded into the producing launch.  Thread 0 of every workgroup publishes the NV block
+// sums at partials[k*gridDim.x + blockIdx.x] with device-coherent stores, waits until they are
+// written and counts its workgroup in the arrival counter of its group (workgroup index mod
+// kArriveGroups); the workgroup that completes a group counts the group in the last counter,
+// and the workgroup that completes that one adds all partials in index order (the same order on
+// every launch, so results are reproducible) and applies OP.  Each workgroup that completes a
+// counter resets it, so the counters are zero between launches.  No workgroup waits for another,
+// and every store that the last workgroup reads is device-coherent, so no cache write-back is
+// needed.  The groups spread the arrivals over kArriveGroups cache lines: with one counter, the
+// 2048 workgroups of a launch serialise on it.
+constexpr uint32_t kArriveGroups = 8;
+constexpr uint32_t kArriveStride = 64;                                  // words: 256 bytes apart
+constexpr uint32_t kArriveWords = (kArriveGroups + 1) * kArriveStride;
+constexpr uint64_t kPartialDoubles = 4 * 4096;                          // partial sums per solve
+// the partial sums of one solve, then its arrival counters
+constexpr uint64_t kPartialBytes = kPartialDoubles * sizeof(double) + kArriveWords * sizeof(uint32_t);
+
+template <int NV, int OP>
+__device__ __forceinline__ void finalize_in_launch(double (&v)[NV], double *partials, uint32_t *arrive,
+                                                   SolveState *st, double *__restrict__ hist)
+{
+    __shared__ double lds[NV * (kBlock / kWave)];
+    __shared__ uint32_t last;
+    block_sum<NV, kBlock>(v, lds);
+    const uint32_t nparts = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            __hip_atomic_store(partials + (uint64_t)k * nparts + blockIdx.x, v[k], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t g = blockIdx.x % kArriveGroups;
+        const uint32_t members = (nparts - g + kArriveGroups - 1) / kArriveGroups;
+        const uint32_t groups = nparts < kArriveGroups ? nparts : kArriveGroups;
+        uint32_t *gc = arrive + g * kArriveStride, *tc = arrive + kArriveGroups * kArriveStride;
+        bool fin = false;
+        if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+            __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            fin = __hip_atomic_fetch_add(tc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == groups - 1;
+            if (fin) __hip_atomic_store(tc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        last = fin;
+    }
+    __syncthreads();
+    if (!last) return;
+    // U partials per value and thread in flight before they are added (one pass at 2048 workgroups)
+    constexpr int U = 8;
+    double s[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) s[k] = 0.0;
+    for (uint32_t base = 0; base < nparts; base += U * kBlock) {
+        double t[NV][U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t i = base + u * kBlock + threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < NV; k++)
+                t[k][u] = i < nparts ? __hip_atomic_load(partials + (uint64_t)k * nparts + i, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)
+                                     : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int k = 0; k < NV; k++) s[k] += t[k][u];
+    }
+    block_sum<NV, kBlock>(s, lds);
+    if (threadIdx.x == 0) {
+        double w[4] = { 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+        for (int k = 0; k < NV; k++) w[k] = s[k];
+        fin_apply(OP, 0, w, st, hist);
     }
 }
 
@@ -606,6 +687,7 @@ struct CgArgs {
     uint32_t pw;
     const double *pd;
     double *partials;
+    uint32_t *arrive;     // kArriveWords arrival counters, zero between launches
     SolveState *st;
     double *hist;
     uint32_t stream;      // the update and direction kernels stream their vectors non-temporally
@@ -745,17 +827,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(CgArgs a)
     else cg_dir<W, false>(a);
 }
 
-// q = A p, partial p.q   (src/rmatrixsolver.cpp:295-300)
+// q = A p and p.q, then alpha   (src/rmatrixsolver.cpp:295-321)
 template <class Shape>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs a)
 {
     if (a.st->done) return;
     EpiDot64 e{ a.p, a.q, { 0.0 } };
     Shape::template run<double>(a.A, a.A.val, a.p, e);
-    store_partials<1>(e.acc, a.partials);
+    finalize_in_launch<1, FIN_DOT>(e.acc, a.partials, a.arrive, a.st, a.hist);
 }
 
-// x += alpha p, r -= alpha q, partial |x|^2, |r|^2 and r.P r  (src/rmatrixsolver.cpp:324-331)
+// x += alpha p, r -= alpha q, |x|^2, |r|^2 and r.P r, then the stopping rule and beta
+// (src/rmatrixsolver.cpp:324-352)
 template <int W, bool NT>
 __device__ __forceinline__ void cg_update(const CgArgs &a)
 {
@@ -780,7 +863,7 @@ __device__ __forceinline__ void cg_update(const CgArgs &a)
         pc_apply_block<W, double, double>(a.ptype, ww, a.pd, ib, r, z);
         for (uint32_t k = 0; k < ww; k++) v[2] += r[k] * z[k];
     }
-    store_partials<3>(v, a.partials);
+    finalize_in_launch<3, FIN_UPDATE>(v, a.partials, a.arrive, st, a.hist);
 }
 
 template <int W>
@@ -806,6 +889,7 @@ struct MpArgs {
     uint32_t pw;
     const float *pd;
     double *partials;
+    uint32_t *arrive;     // kArriveWords arrival counters, zero between launches
     SolveState *st;
     double *hist;
 };
@@ -836,7 +920,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_spmv_dot(MpArgs a)
     if (a.st->done) return;
     EpiDot32 e{ a.p, a.q, { 0.0 } };
     Shape::template run<float>(a.A, a.A.val32, a.p, e);
-    store_partials<1>(e.acc, a.partials);
+    finalize_in_launch<1, FIN_DOT>(e.acc, a.partials, a.arrive, a.st, a.hist);
 }
 
 template <int W>
@@ -862,7 +946,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_update(MpArgs a)
         pc_apply_block<W, float, float>(a.ptype, ww, a.pd, ib, r, z);
         for (uint32_t k = 0; k < ww; k++) v[1] += (double)r[k] * (double)z[k];
     }
-    store_partials<2>(v, a.partials);
+    finalize_in_launch<2, FIN_MP_UPDATE>(v, a.partials, a.arrive, st, a.hist);
 }
 
 // x += dx, dx = 0
@@ -883,8 +967,9 @@ __global__ __launch_bounds__(kBlock) void k_mp_true_resid(MpArgs a)
     store_partials<2>(e.acc, a.partials);
 }
 
-// The same two steps of a replacement, executed only when the finalize kernel requested one
-// (flexible CG keeps them inside its captured iterations instead of pausing for the host).
+// The same two steps of a replacement, executed only when the stopping rule requested one
+// (flexible CG keeps them inside its captured iterations instead of pausing for the host); the
+// true-residual launch finalizes the replacement itself.
 __global__ void k_mp_fold_cond(double *__restrict__ x, float *__restrict__ dx, uint64_t n, const SolveState *st)
 {
     if (st->done || !st->replace) return;
@@ -900,7 +985,7 @@ __global__ __launch_bounds__(kBlock) void k_mp_true_resid_cond(MpArgs a)
     if (a.st->done || !a.st->replace) return;
     EpiTrueResid32 e{ a.b, a.r, { 0.0, 0.0 } };
     Shape::template run<double>(a.A, a.A.val, a.x, e);
-    store_partials<2>(e.acc, a.partials);
+    finalize_in_launch<2, FIN_MP_REPLACE>(e.acc, a.partials, a.arrive, a.st, a.hist);
 }
 
 // partial rho = r . P r after a residual replacement
@@ -1564,7 +1649,9 @@ extern "C" int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t re
     // partial sums of p.q, on a running solver state
     DBuf<double> partials;
     DBuf<SolveState> state;
-    if ((rc = partials.alloc(4 * 4096)) || (rc = state.alloc(1)) || (rc = matrix_sync_plan(*A, id, false))) return rc;
+    if ((rc = partials.alloc(kPartialBytes / sizeof(double))) || (rc = state.alloc(1)) ||
+        (rc = matrix_sync_plan(*A, id, false)))
+        return rc;
     k_state_init<<<1, 64, 0, st>>>(state.p, A->frob, 0.0, 0.0, 1, RS_STOP_RELRES, 0, 0);
     uint32_t nb = 0;
     CgArgs a{};
@@ -1572,7 +1659,9 @@ extern "C" int rs_matrix_spmv_timed(rs_matrix *A, int32_t precision, uint32_t re
     a.p = x.p;
     a.q = y.p;
     a.partials = partials.p;
+    a.arrive = reinterpret_cast<uint32_t *>(partials.p + kPartialDoubles);
     a.st = state.p;
+    k_zero_u32<<<1, 64, 0, st>>>(a.arrive, kArriveWords);
     RS_SHAPE_SWITCH(id, k_cg_spmv_dot, nb, st, a);
     EventPair ev;
     RS_HIP(ev.create());
@@ -1806,7 +1895,8 @@ static int ensure_ws(rs_solver &S, uint32_t n, bool mixed, bool gmres)
 {
     int rc;
     size_t N = std::max(n, 1u);
-    if ((rc = S.y.alloc(N)) || (rc = S.partials.alloc(4 * 4096)) || (rc = S.st.alloc(1))) return rc;
+    if ((rc = S.y.alloc(N)) || (rc = S.partials.alloc(kPartialBytes / sizeof(double))) || (rc = S.st.alloc(1)))
+        return rc;
     uint32_t hc = std::min(kHistCap, std::max(1u, S.conf.n_outer_iterations));
     if ((rc = S.hist.alloc(2ull * hc))) return rc;
     if (!mixed && !gmres) {
@@ -1957,16 +2047,14 @@ static int capture(rs_solver &S, const std::function<void()> &body, const std::s
     return RS_OK;
 }
 
-// kGraphIters FP64 CG iterations (src/rmatrixsolver.cpp:248-353), 5 launches each:
-//   dir, spmv_dot, fin DOT, update, fin UPDATE.
+// kGraphIters FP64 CG iterations (src/rmatrixsolver.cpp:248-353), 3 launches each:
+//   dir, spmv_dot (finalizes alpha), update (finalizes the stopping rule and beta).
 static void cg64_iterations(const CgArgs &a, int plan, uint32_t nb, uint32_t w, uint32_t vb, hipStream_t stream)
 {
     for (uint32_t i = 0; i < kGraphIters; i++) {
         RS_DISPATCH_W(w, k_cg_dir, vb, a);
         RS_SHAPE_SWITCH(plan, k_cg_spmv_dot, nb, stream, a);
-        fin(a.partials, nb, 1, FIN_DOT, 0, a.st, a.hist);
         RS_DISPATCH_W(w, k_cg_update, vb, a);
-        fin(a.partials, vb, 3, FIN_UPDATE, 0, a.st, a.hist);
     }
 }
 
@@ -1988,12 +2076,14 @@ static int solve_cg64(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, ui
     a.pw = P.width ? P.width : 1;
     a.pd = P.data.p;
     a.partials = S.partials.p;
+    a.arrive = reinterpret_cast<uint32_t *>(S.partials.p + kPartialDoubles);
     a.st = S.st.p;
     a.hist = S.hist.p;
     a.stream = (uint64_t)A.n * sizeof(double) >= kStreamVectorBytes;
     uint32_t w = a.pw;
     uint32_t vb = vec_blocks(A.n / w);
     hipStream_t stream = runtime().stream;
+    k_zero_u32<<<1, 64, 0, stream>>>(a.arrive, kArriveWords);
     RS_SHAPE_SWITCH(plan, k_cg_init, nb, stream, a);
     fin(S.partials.p, nb, 1, FIN_CG_INIT, 0, S.st.p, S.hist.p);
     RS_DISPATCH_W(w, k_cg_rho, vb, a);
@@ -2027,6 +2117,7 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
     a.pw = P.width ? P.width : 1;
     a.pd = P.data32.p;
     a.partials = S.partials.p;
+    a.arrive = reinterpret_cast<uint32_t *>(S.partials.p + kPartialDoubles);
     a.st = S.st.p;
     a.hist = S.hist.p;
     MpArgs a64 = a;
@@ -2036,6 +2127,7 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
     uint32_t nvb = vec_blocks(A.n);
     hipStream_t stream = runtime().stream;
     k_fill32<<<nvb, kBlock, 0, stream>>>(S.dx32.p, A.n, 0.0f);
+    k_zero_u32<<<1, 64, 0, stream>>>(a.arrive, kArriveWords);
     RS_SHAPE_SWITCH(plan64, k_mp_true_resid, nb64, stream, a64);
     fin(S.partials.p, nb64, 2, FIN_MP_TRUE, 1, S.st.p, S.hist.p);
     RS_DISPATCH_W(w, k_mp_rho, vb, a);
@@ -2044,10 +2136,9 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
     const std::function<void()> body = [&] {
         for (uint32_t i = 0; i < kGraphIters; i++) {
             RS_DISPATCH_W(w, k_mp_dir, vb, a);
-            RS_SHAPE_SWITCH(plan32, k_mp_spmv_dot, nb32, stream, a);
-            fin(S.partials.p, nb32, 1, FIN_DOT, 0, S.st.p, S.hist.p);
-            RS_DISPATCH_W(w, k_mp_update, vb, a);
-            fin(S.partials.p, vb, 2, FIN_MP_UPDATE, 0, S.st.p, S.hist.p);
+            RS_SHAPE_SWITCH(plan32, k_mp_spmv_dot, nb32, stream, a);   // finalizes alpha
+            RS_DISPATCH_W(w, k_mp_update, vb, a);                      // finalizes the stopping rule, beta
+
         }
     };
     rc = capture(S, body, graph_key_of(2, a, plan32, nb32, w, vb));
@@ -2396,7 +2487,7 @@ static NoAllocLayout noalloc_layout(uint64_t n, uint32_t w, bool precond, uint32
     const uint64_t N = std::max<uint64_t>(n, 1);
     L.hist_cap = std::min(kNoAllocHistCap, std::max(1u, n_outer));
     L.st = take(sizeof(SolveState));
-    L.partials = take(sizeof(double) * 4 * 4096);
+    L.partials = take(kPartialBytes);
     L.hist = take(sizeof(double) * 2 * L.hist_cap);
     L.y = take(sizeof(double) * N);
     L.r = take(sizeof(double) * N);
@@ -2469,12 +2560,14 @@ static int solve_device_noalloc(rs_solver &S, rs_matrix &A, const double *d_b, d
     a.pw = w;
     a.pd = pd;
     a.partials = partials;
+    a.arrive = reinterpret_cast<uint32_t *>(partials + kPartialDoubles);
     a.st = st;
     a.hist = hist;
     a.stream = (uint64_t)n * sizeof(double) >= kStreamVectorBytes;
     const uint32_t vb = vec_blocks(n / w);
     SolveState h{};
     if (n > 0) {
+        k_zero_u32<<<1, 64, 0, stream>>>(a.arrive, kArriveWords);
         RS_SHAPE_SWITCH(plan, k_cg_init, nb, stream, a);
         fin(partials, nb, 1, FIN_CG_INIT, 0, st, hist);
         RS_DISPATCH_W(w, k_cg_rho, vb, a);
