@@ -472,6 +472,13 @@ enum {
     RS_COARSE_JACOBI = 1,    /* coarse Jacobi: what the untrained network computes */
     RS_COARSE_EXACT = 2      /* dense inverse of the coarsest operator */
 };
+/* How the bins group the DOFs of a node (dofs_per_node > 1). */
+enum {
+    RS_AGG_NODES = 0,       /* the DOFs of a node share their bin's aggregate */
+    RS_AGG_COMPONENTS = 1   /* one aggregate per bin and component: the unknown-based
+                               aggregation for vector problems (elasticity), whose coarse
+                               spaces keep the translations of each component */
+};
 /* Aggregates from node coordinates: nbx*nby*nbz bins over the bounding box; agg_of_dof gets
  * the bin of each enabled position of the node book (ids compacted to 0..n_agg-1). */
 int rs_aggregates_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node,
@@ -486,6 +493,14 @@ int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_
                          const uint32_t *book, uint32_t nbx, uint32_t nby, uint32_t nbz,
                          uint32_t max_coarse, uint32_t max_levels, uint32_t *agg,
                          uint64_t agg_capacity, uint32_t *n_agg, uint32_t *n_levels);
+/* The same with a choice of aggregation (RS_AGG_NODES as above, or RS_AGG_COMPONENTS: bin b
+ * and component c form aggregate key dofs_per_node * b + c; every coarse node keeps the
+ * component of its members). */
+int rs_hierarchy_by_bins_ex(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node,
+                            const uint32_t *book, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                            uint32_t max_coarse, uint32_t max_levels, int32_t aggregation,
+                            uint32_t *agg, uint64_t agg_capacity, uint32_t *n_agg,
+                            uint32_t *n_levels);
 /* Hierarchy for the pattern of A: n_levels coarse levels given by their maps (as above). */
 int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const uint32_t *agg,
                       const uint32_t *n_agg, rs_learned **out);
@@ -497,6 +512,11 @@ int rs_learned_create_by_bins(const rs_matrix *A, uint32_t n_nodes, const double
                               uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx,
                               uint32_t nby, uint32_t nbz, uint32_t max_coarse,
                               uint32_t max_levels, rs_learned **out);
+/* The same with a choice of aggregation (the level maps of rs_hierarchy_by_bins_ex). */
+int rs_learned_create_by_bins_ex(const rs_matrix *A, uint32_t n_nodes, const double *d_xyz,
+                                 uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx,
+                                 uint32_t nby, uint32_t nbz, uint32_t max_coarse,
+                                 uint32_t max_levels, int32_t aggregation, rs_learned **out);
 /* Level map of level `level` (0 .. n_levels-2) to the next: one entry per row of the level. */
 int rs_learned_level_map(const rs_learned *L, uint32_t level, uint32_t *agg, uint64_t capacity);
 /* Network parameters, FP32, 49,921 values (layout: W_in[4][64], b_in[64], 6 x {W_s[64][64],

@@ -168,18 +168,22 @@ def prolongation(m, n_next):
     return sp.csr_matrix((np.ones(m.size), (np.arange(m.size), m)), shape=(m.size, n_next))
 
 
-def bins_hierarchy(xyz, book, dpn, bins, max_coarse, max_levels=16, eod=0xFFFFFFFF):
+def bins_hierarchy(xyz, book, dpn, bins, max_coarse, max_levels=16, eod=0xFFFFFFFF, by_component=False):
     """Nested coordinate bins over the enabled positions of `book` (numpy restatement that pins
     the library's host and GPU builders).  Level k bins the nodes of level k (level 0: the DOFs,
     each at its node's coordinates; a coarser node at the coordinates of its first member) into
     bins >> k cells per axis of the bounding box of all nodes; ids in first-appearance order;
-    a level that merges nothing is skipped (the bins halve again).  Returns the level maps."""
+    a level that merges nothing is skipped (the bins halve again).  by_component: the key of a
+    node is dpn * bin + the component of its first member, so an aggregate holds one component
+    only (unknown-based aggregation).  Returns the level maps."""
     xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
     book = np.asarray(book, dtype=np.uint64).ravel()
     pos = np.nonzero(book != eod)[0]
     n_dof = pos.size
     node_of = np.empty(n_dof, dtype=np.int64)
     node_of[book[pos].astype(np.int64)] = pos // dpn
+    comp_of = np.empty(n_dof, dtype=np.int64)
+    comp_of[book[pos].astype(np.int64)] = pos % dpn
     lo, hi = xyz.min(axis=0), xyz.max(axis=0)
     rep = np.arange(n_dof)
     cur, maps, k = n_dof, [], 0
@@ -193,6 +197,8 @@ def bins_hierarchy(xyz, book, dpn, bins, max_coarse, max_levels=16, eod=0xFFFFFF
         t = np.where(span > 0, (p - lo) / np.where(span > 0, span, 1.0), 0.0)
         c = np.minimum(nb - 1, (t * nb).astype(np.uint64).astype(np.int64))
         b = c[:, 0] + nb[0] * (c[:, 1] + nb[1] * c[:, 2])
+        if by_component:
+            b = b * dpn + comp_of[rep]
         _, first, inv = np.unique(b, return_index=True, return_inverse=True)
         order = np.argsort(first)
         rank = np.empty_like(order)
@@ -289,3 +295,28 @@ def coarse_exact(Ac, r):
 
 def coarse_jacobi(Ac, r):
     return r / Ac.diagonal()
+
+
+def fcg(A, b, prec, tol=1e-8, maxit=100000):
+    """Flexible CG, FCG(1) (Notay 2000): p = z - (z.q_prev / p_prev.q_prev) p_prev, alpha =
+    p.r / p.q, stop when ||r|| / ||b|| < tol.  FP64 throughout; returns (x, iterations).  Pinned
+    by its closed forms: B = A^-1 converges in one iteration, B = I is plain CG (finite
+    termination on few distinct eigenvalues)."""
+    b = np.asarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    r = b.copy()
+    nb = np.linalg.norm(b)
+    if nb == 0:
+        return x, 0
+    p_prev = q_prev = None
+    for k in range(1, maxit + 1):
+        z = prec(r)
+        p = z if p_prev is None else z - (z @ q_prev) / (p_prev @ q_prev) * p_prev
+        q = A @ p
+        a = (p @ r) / (p @ q)
+        x = x + a * p
+        r = r - a * q
+        if np.linalg.norm(r) / nb < tol:
+            return x, k
+        p_prev, q_prev = p, q
+    return x, maxit

@@ -175,6 +175,10 @@ _SIGS = {
     "rs_learned_create": [_P, C.c_uint32, _P, _P, C.POINTER(_P)],
     "rs_learned_create_by_bins": [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32, C.c_uint32, C.POINTER(_P)],
+    "rs_hierarchy_by_bins_ex": [C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.c_uint32, C.c_uint32, C.c_int32, _P, C.c_uint64, _P, _u32p],
+    "rs_learned_create_by_bins_ex": [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)],
     "rs_learned_level_map": [_P, C.c_uint32, _P, C.c_uint64],
     "rs_assembly_device_mesh": [_P, C.POINTER(_P), C.POINTER(_P), _u32p, _u32p],
     "rs_learned_set_params": [_P, _P, C.c_uint64],
@@ -919,6 +923,7 @@ class Batch:
 N_PARAMS = 49921
 MAX_COARSE = 512
 COARSE_NETWORK, COARSE_JACOBI, COARSE_EXACT = 0, 1, 2
+AGG_NODES, AGG_COMPONENTS = 0, 1   # rs_*_by_bins_ex: a node's DOFs share an aggregate / one per component
 
 
 NET_HID, NET_LAYERS, NET_IN = 64, 6, 4
@@ -964,9 +969,10 @@ def aggregates_by_bins(xyz, book, dofs_per_node: int, bins) -> tuple[np.ndarray,
 
 
 def hierarchy_by_bins(xyz, book, dofs_per_node: int, bins, max_coarse: int = MAX_COARSE,
-                      max_levels: int = 16) -> list[np.ndarray]:
+                      max_levels: int = 16, by_component: bool = False) -> list[np.ndarray]:
     """Nested coordinate bins, halved per level until a level has <= max_coarse nodes.
-    Returns the level maps: maps[k] sends the nodes of level k (level 0: the DOFs) to level k+1."""
+    Returns the level maps: maps[k] sends the nodes of level k (level 0: the DOFs) to level k+1.
+    by_component: one aggregate per bin and DOF component (RS_AGG_COMPONENTS)."""
     load()
     xyz = _f64(xyz).reshape(-1)
     book = _u32(book)
@@ -975,9 +981,10 @@ def hierarchy_by_bins(xyz, book, dofs_per_node: int, bins, max_coarse: int = MAX
     agg = np.empty(cap, np.uint32)
     n_agg = np.empty(max_levels, np.uint32)
     nl = C.c_uint32()
-    _check(_lib.rs_hierarchy_by_bins(xyz.size // 3, _ptr(xyz), dofs_per_node, _ptr(book), bins[0],
-                                     bins[1], bins[2], max_coarse, max_levels, _ptr(agg), cap,
-                                     _ptr(n_agg), C.byref(nl)))
+    _check(_lib.rs_hierarchy_by_bins_ex(xyz.size // 3, _ptr(xyz), dofs_per_node, _ptr(book), bins[0],
+                                        bins[1], bins[2], max_coarse, max_levels,
+                                        AGG_COMPONENTS if by_component else AGG_NODES, _ptr(agg), cap,
+                                        _ptr(n_agg), C.byref(nl)))
     maps, off, n = [], 0, n_dof
     for k in range(nl.value):
         maps.append(agg[off:off + n].copy())
@@ -1008,9 +1015,10 @@ class LearnedPreconditioner:
 
     @classmethod
     def from_bins(cls, A: SparseMatrix, asm, bins, max_coarse: int = 512, max_levels: int = 16,
-                  params=None) -> "LearnedPreconditioner":
+                  params=None, by_component: bool = False) -> "LearnedPreconditioner":
         """The hierarchy of hierarchy_by_bins(xyz, book, ...) over the assembly's mesh, built on the
-        GPU from the assembly's device coordinates and DOF book (no host work over the mesh)."""
+        GPU from the assembly's device coordinates and DOF book (no host work over the mesh).
+        by_component: one aggregate per bin and DOF component (vector problems)."""
         load()
         self = cls.__new__(cls)
         self.A = A
@@ -1018,8 +1026,9 @@ class LearnedPreconditioner:
         nn, dpn = C.c_uint32(), C.c_uint32()
         _check(_lib.rs_assembly_device_mesh(asm.h, C.byref(xp), C.byref(bp), C.byref(nn), C.byref(dpn)))
         self.h = C.c_void_p()
-        _check(_lib.rs_learned_create_by_bins(A.h, nn.value, xp, dpn.value, bp, int(bins[0]), int(bins[1]),
-                                              int(bins[2]), max_coarse, max_levels, C.byref(self.h)))
+        _check(_lib.rs_learned_create_by_bins_ex(A.h, nn.value, xp, dpn.value, bp, int(bins[0]), int(bins[1]),
+                                                 int(bins[2]), max_coarse, max_levels,
+                                                 AGG_COMPONENTS if by_component else AGG_NODES, C.byref(self.h)))
         self._init_info(params)
         return self
 

@@ -38,10 +38,10 @@ __global__ void k_iota_u32(uint32_t *a, uint64_t n)
     HIER_LOOP(i, n) a[i] = (uint32_t)i;
 }
 
-// node_of[d]: the node of DOF d.  bad: 1 = a DOF number >= n_dof, 2 = a DOF numbered twice,
-// 4 = a DOF without a node.
+// node_of[d]: the node of DOF d; comp_of[d]: its component (position within the node).
+// bad: 1 = a DOF number >= n_dof, 2 = a DOF numbered twice, 4 = a DOF without a node.
 __global__ void k_node_of(const uint32_t *__restrict__ book, uint64_t npos, uint32_t dpn, uint32_t n_dof,
-                          uint32_t *__restrict__ node_of, uint32_t *__restrict__ bad)
+                          uint32_t *__restrict__ node_of, uint32_t *__restrict__ comp_of, uint32_t *__restrict__ bad)
 {
     HIER_LOOP(q, npos)
     {
@@ -52,6 +52,7 @@ __global__ void k_node_of(const uint32_t *__restrict__ book, uint64_t npos, uint
             continue;
         }
         if (atomicExch(&node_of[d], (uint32_t)(q / dpn)) != RS_EOD) atomicOr(bad, 2u);
+        comp_of[d] = (uint32_t)(q % dpn);
     }
 }
 
@@ -106,12 +107,15 @@ __global__ __launch_bounds__(kHierBlock) void k_bbox_fin(const double *__restric
 struct BinArgs {
     const double *xyz;
     const uint32_t *node_of, *rep;
+    const uint32_t *comp_of;    // per component: the component of each fine DOF; nullptr: per node
     const double *box;          // lo[3], hi[3]
-    uint32_t cur, nb0, nb1, nb2;
+    uint32_t cur, nb0, nb1, nb2, dpn;
 };
 
 // Bin of each node J of the current level (the coordinates of a fine node it contains), and the
-// first J of every bin.  The expressions are those of the host builder, so the bins agree.
+// first J of every bin.  Per component, bin b holds one aggregate per component c (key dpn b + c),
+// so an aggregate never mixes components.  The expressions are those of the host builder, so the
+// bins agree.
 __global__ void k_bin(BinArgs a, uint32_t *__restrict__ bin, uint32_t *__restrict__ first)
 {
     const double lo[3] = { a.box[0], a.box[1], a.box[2] }, hi[3] = { a.box[3], a.box[4], a.box[5] };
@@ -125,7 +129,8 @@ __global__ void k_bin(BinArgs a, uint32_t *__restrict__ bin, uint32_t *__restric
             const uint64_t q = (uint64_t)(t * nb[d]);
             c[d] = q < nb[d] - 1ull ? q : nb[d] - 1ull;
         }
-        const uint32_t b = (uint32_t)(c[0] + nb[0] * (c[1] + (uint64_t)nb[1] * c[2]));
+        uint32_t b = (uint32_t)(c[0] + nb[0] * (c[1] + (uint64_t)nb[1] * c[2]));
+        if (a.comp_of) b = b * a.dpn + a.comp_of[a.rep[J]];
         bin[J] = b;
         atomicMin(&first[b], (uint32_t)J);
     }
@@ -332,7 +337,7 @@ int hier_identity(uint32_t *d_a, uint64_t n)
 }
 
 int hier_bins(uint32_t n_nodes, const double *d_xyz, uint32_t dpn, const uint32_t *d_book, uint32_t n_dof,
-              const uint32_t bins[3], uint32_t max_coarse, uint32_t max_levels,
+              const uint32_t bins[3], uint32_t max_coarse, uint32_t max_levels, bool by_component,
               std::vector<std::unique_ptr<DBuf<uint32_t>>> &maps, std::vector<uint32_t> &n_agg)
 {
     maps.clear();
@@ -342,17 +347,17 @@ int hier_bins(uint32_t n_nodes, const double *d_xyz, uint32_t dpn, const uint32_
     hipStream_t st = runtime().stream;
     Temp tmp;
     int rc;
-    DBuf<uint32_t> node_of, bad, rep, next_rep, bin, first, flag, rank;
+    DBuf<uint32_t> node_of, comp_of, bad, rep, next_rep, bin, first, flag, rank;
     DBuf<double> part, box;
     const uint64_t npos = (uint64_t)n_nodes * dpn;
     const uint32_t nbb = hier_grid(n_nodes);
-    if ((rc = node_of.alloc(n_dof)) || (rc = bad.alloc(1)) || (rc = rep.alloc(n_dof)) || (rc = next_rep.alloc(n_dof)) ||
-        (rc = bin.alloc(n_dof)) || (rc = flag.alloc(n_dof)) || (rc = rank.alloc(n_dof)) || (rc = part.alloc(6ull * nbb)) ||
-        (rc = box.alloc(6)))
+    if ((rc = node_of.alloc(n_dof)) || (rc = comp_of.alloc(n_dof)) || (rc = bad.alloc(1)) || (rc = rep.alloc(n_dof)) ||
+        (rc = next_rep.alloc(n_dof)) || (rc = bin.alloc(n_dof)) || (rc = flag.alloc(n_dof)) || (rc = rank.alloc(n_dof)) ||
+        (rc = part.alloc(6ull * nbb)) || (rc = box.alloc(6)))
         return rc;
     k_fill_u32<<<hier_grid(n_dof), kHierBlock, 0, st>>>(node_of.p, n_dof, RS_EOD);
     k_fill_u32<<<1, kHierBlock, 0, st>>>(bad.p, 1, 0u);
-    k_node_of<<<hier_grid(npos), kHierBlock, 0, st>>>(d_book, npos, dpn, n_dof, node_of.p, bad.p);
+    k_node_of<<<hier_grid(npos), kHierBlock, 0, st>>>(d_book, npos, dpn, n_dof, node_of.p, comp_of.p, bad.p);
     k_check_assigned<<<hier_grid(n_dof), kHierBlock, 0, st>>>(node_of.p, n_dof, bad.p);
     k_bbox<<<nbb, kHierBlock, 0, st>>>(d_xyz, n_nodes, part.p);
     k_bbox_fin<<<1, kHierBlock, 0, st>>>(part.p, nbb, box.p);
@@ -367,11 +372,11 @@ int hier_bins(uint32_t n_nodes, const double *d_xyz, uint32_t dpn, const uint32_
         if (levels == max_levels) return fail(RS_ERR_UNSUPPORTED, "hierarchy by bins: coarsest level still too large");
         const uint32_t sh = std::min(k, 31u);
         const uint32_t nb[3] = { std::max(1u, bins[0] >> sh), std::max(1u, bins[1] >> sh), std::max(1u, bins[2] >> sh) };
-        const uint64_t nbins = (uint64_t)nb[0] * nb[1] * nb[2];
+        const uint64_t nbins = (uint64_t)nb[0] * nb[1] * nb[2] * (by_component ? dpn : 1u);
         if (nbins >= 0xFFFFFFFFull) return fail(RS_ERR_UNSUPPORTED, "hierarchy by bins: 2^32 bins or more");
         if ((rc = first.alloc(nbins))) return rc;
         k_fill_u32<<<hier_grid(nbins), kHierBlock, 0, st>>>(first.p, nbins, RS_EOD);
-        BinArgs a{ d_xyz, node_of.p, rep.p, box.p, cur, nb[0], nb[1], nb[2] };
+        BinArgs a{ d_xyz, node_of.p, rep.p, by_component ? comp_of.p : nullptr, box.p, cur, nb[0], nb[1], nb[2], dpn };
         k_bin<<<hier_grid(cur), kHierBlock, 0, st>>>(a, bin.p, first.p);
         k_is_first<<<hier_grid(cur), kHierBlock, 0, st>>>(bin.p, first.p, cur, flag.p);
         RS_HIP(hipGetLastError());

@@ -161,8 +161,11 @@ int matrix_create_device(uint32_t n, DBuf<uint32_t> &rowptr, DBuf<uint32_t> &col
 // ----------------------------------------------------------------- aggregation hierarchy (GPU)
 // Nested coordinate bins over the DOFs of a mesh (device coordinates and DOF book), halved per
 // level until a level has <= max_coarse nodes: maps[k] sends the nodes of level k to level k + 1.
+// by_component: one aggregate per bin and DOF component (the DOFs of a node go to dpn different
+// aggregates, every coarse node keeps the component of its members); else the DOFs of a node share
+// their bin's aggregate.
 int hier_bins(uint32_t n_nodes, const double *d_xyz, uint32_t dpn, const uint32_t *d_book, uint32_t n_dof,
-              const uint32_t bins[3], uint32_t max_coarse, uint32_t max_levels,
+              const uint32_t bins[3], uint32_t max_coarse, uint32_t max_levels, bool by_component,
               std::vector<std::unique_ptr<DBuf<uint32_t>>> &maps, std::vector<uint32_t> &n_agg);
 // Members of each of the nn aggregates of d_map (n entries), rows ascending: ptr (nn + 1), rows (n).
 int hier_members(const uint32_t *d_map, uint32_t n, uint32_t nn, DBuf<uint32_t> &ptr, DBuf<uint32_t> &rows);

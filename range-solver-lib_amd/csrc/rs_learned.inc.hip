@@ -1670,19 +1670,25 @@ extern "C" int rs_aggregates_by_bins(uint32_t n_nodes, const double *xyz, uint32
     return RS_OK;
 }
 
-extern "C" int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node, const uint32_t *book,
-                                    uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t max_coarse, uint32_t max_levels,
-                                    uint32_t *agg, uint64_t agg_capacity, uint32_t *n_agg, uint32_t *n_levels)
+extern "C" int rs_hierarchy_by_bins_ex(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node, const uint32_t *book,
+                                       uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t max_coarse, uint32_t max_levels,
+                                       int32_t aggregation, uint32_t *agg, uint64_t agg_capacity, uint32_t *n_agg,
+                                       uint32_t *n_levels)
 {
     if (!xyz || !book || !agg || !n_agg || !n_levels || !nbx || !nby || !nbz || !dofs_per_node || !max_coarse ||
-        !max_levels)
+        !max_levels || (aggregation != RS_AGG_NODES && aggregation != RS_AGG_COMPONENTS))
         return fail(RS_ERR_INVALID_ARG, "rs_hierarchy_by_bins: bad arguments");
-    // enabled positions and their nodes
-    std::vector<uint32_t> node_of;
+    const bool by_component = aggregation == RS_AGG_COMPONENTS;
+    // enabled positions, their nodes and components
+    std::vector<uint32_t> node_of, comp_of;
     for (uint64_t i = 0; i < (uint64_t)n_nodes * dofs_per_node; i++)
         if (book[i] != RS_EOD) {
-            if (book[i] >= node_of.size()) node_of.resize(book[i] + 1ull, RS_EOD);
+            if (book[i] >= node_of.size()) {
+                node_of.resize(book[i] + 1ull, RS_EOD);
+                comp_of.resize(book[i] + 1ull, 0);
+            }
             node_of[book[i]] = (uint32_t)(i / dofs_per_node);
+            comp_of[book[i]] = (uint32_t)(i % dofs_per_node);
         }
     const uint32_t n_dof = (uint32_t)node_of.size();
     for (uint32_t d = 0; d < n_dof; d++)
@@ -1703,7 +1709,8 @@ extern "C" int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_
         const uint32_t sh = std::min(k, 31u);
         const uint32_t nb[3] = { std::max(1u, nbx >> sh), std::max(1u, nby >> sh), std::max(1u, nbz >> sh) };
         if (used + cur > agg_capacity) return fail(RS_ERR_INVALID_ARG, "rs_hierarchy_by_bins: agg capacity too small");
-        std::vector<uint32_t> id((uint64_t)nb[0] * nb[1] * nb[2], RS_EOD), next_rep;
+        std::vector<uint32_t> id((uint64_t)nb[0] * nb[1] * nb[2] * (by_component ? dofs_per_node : 1u), RS_EOD),
+            next_rep;
         uint32_t count = 0;
         for (uint32_t J = 0; J < cur; J++) {
             const double *p = xyz + 3ull * node_of[rep[J]];
@@ -1713,6 +1720,7 @@ extern "C" int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_
                 c[a] = std::min<uint64_t>(nb[a] - 1, (uint64_t)(t * nb[a]));
             }
             uint64_t b = c[0] + nb[0] * (c[1] + (uint64_t)nb[1] * c[2]);
+            if (by_component) b = b * dofs_per_node + comp_of[rep[J]];
             if (id[b] == RS_EOD) {
                 id[b] = count++;
                 next_rep.push_back(rep[J]);
@@ -1731,6 +1739,14 @@ extern "C" int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_
     }
     *n_levels = levels;
     return RS_OK;
+}
+
+extern "C" int rs_hierarchy_by_bins(uint32_t n_nodes, const double *xyz, uint32_t dofs_per_node, const uint32_t *book,
+                                    uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t max_coarse, uint32_t max_levels,
+                                    uint32_t *agg, uint64_t agg_capacity, uint32_t *n_agg, uint32_t *n_levels)
+{
+    return rs_hierarchy_by_bins_ex(n_nodes, xyz, dofs_per_node, book, nbx, nby, nbz, max_coarse, max_levels,
+                                   RS_AGG_NODES, agg, agg_capacity, n_agg, n_levels);
 }
 
 namespace rs {
@@ -1874,12 +1890,13 @@ extern "C" int rs_learned_create(const rs_matrix *A, uint32_t n_levels, const ui
     return learned_build(A, maps, std::vector<uint32_t>(n_agg, n_agg + n_levels), out);
 }
 
-extern "C" int rs_learned_create_by_bins(const rs_matrix *A, uint32_t n_nodes, const double *d_xyz,
-                                         uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx, uint32_t nby,
-                                         uint32_t nbz, uint32_t max_coarse, uint32_t max_levels, rs_learned **out)
+extern "C" int rs_learned_create_by_bins_ex(const rs_matrix *A, uint32_t n_nodes, const double *d_xyz,
+                                            uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx, uint32_t nby,
+                                            uint32_t nbz, uint32_t max_coarse, uint32_t max_levels, int32_t aggregation,
+                                            rs_learned **out)
 {
     if (!A || !d_xyz || !d_book || !out || !n_nodes || !dofs_per_node || !nbx || !nby || !nbz || !max_coarse ||
-        !max_levels || max_levels > 16)
+        !max_levels || max_levels > 16 || (aggregation != RS_AGG_NODES && aggregation != RS_AGG_COMPONENTS))
         return fail(RS_ERR_INVALID_ARG, "rs_learned_create_by_bins: bad arguments");
     if (max_coarse > kMaxCoarse) return fail(RS_ERR_UNSUPPORTED, "rs_learned_create_by_bins: max_coarse above 512");
     int rc = require_runtime();
@@ -1887,9 +1904,18 @@ extern "C" int rs_learned_create_by_bins(const rs_matrix *A, uint32_t n_nodes, c
     std::vector<std::unique_ptr<DBuf<uint32_t>>> maps;
     std::vector<uint32_t> n_agg;
     const uint32_t bins[3] = { nbx, nby, nbz };
-    if ((rc = hier_bins(n_nodes, d_xyz, dofs_per_node, d_book, A->n, bins, max_coarse, max_levels, maps, n_agg)))
+    if ((rc = hier_bins(n_nodes, d_xyz, dofs_per_node, d_book, A->n, bins, max_coarse, max_levels,
+                        aggregation == RS_AGG_COMPONENTS, maps, n_agg)))
         return rc;
     return learned_build(A, maps, n_agg, out);
+}
+
+extern "C" int rs_learned_create_by_bins(const rs_matrix *A, uint32_t n_nodes, const double *d_xyz,
+                                         uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx, uint32_t nby,
+                                         uint32_t nbz, uint32_t max_coarse, uint32_t max_levels, rs_learned **out)
+{
+    return rs_learned_create_by_bins_ex(A, n_nodes, d_xyz, dofs_per_node, d_book, nbx, nby, nbz, max_coarse,
+                                        max_levels, RS_AGG_NODES, out);
 }
 
 extern "C" int rs_learned_level_map(const rs_learned *L, uint32_t level, uint32_t *agg, uint64_t capacity)

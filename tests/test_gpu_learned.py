@@ -114,7 +114,8 @@ def _stress_assembly(rs, nx, ny, nz):
     return asm, b, xyz
 
 
-@pytest.mark.parametrize("case", ["heat16-8^3-64", "heat20-10x7x5-200", "stress12x4x4-6^3-100"])
+@pytest.mark.parametrize("case", ["heat16-8^3-64", "heat20-10x7x5-200", "stress12x4x4-6^3-100",
+                                  "stress12x4x4-components-6x2x2-100"])
 def test_hierarchy_built_on_gpu(gpu, case):
     """rs_learned_create_by_bins: the level maps built on the GPU are the oracle's nested bins,
     and every level operator and bound is bit-identical to those of the hierarchy built from the
@@ -125,11 +126,15 @@ def test_hierarchy_built_on_gpu(gpu, case):
         (asm, b, xyz), dpn, bins, mc = _heat(rs, 16, 1.5), 1, (8, 8, 8), 64
     elif case.startswith("heat20"):
         (asm, b, xyz), dpn, bins, mc = _heat(rs, 20, 2.0), 1, (10, 7, 5), 200
+    elif "components" in case:
+        (asm, b, xyz), dpn, bins, mc = _stress_assembly(rs, 12, 4, 4), 3, (6, 2, 2), 100
     else:
         (asm, b, xyz), dpn, bins, mc = _stress_assembly(rs, 12, 4, 4), 3, (6, 6, 6), 100
-    ref = lo.bins_hierarchy(xyz, asm.book(), dpn, bins, mc)
-    Lg = rs.LearnedPreconditioner.from_bins(asm.matrix, asm, bins, max_coarse=mc, params=lo.init_params(1))
-    maps = rs.hierarchy_by_bins(xyz, asm.book(), dpn, bins, max_coarse=mc)
+    comp = "components" in case
+    ref = lo.bins_hierarchy(xyz, asm.book(), dpn, bins, mc, by_component=comp)
+    Lg = rs.LearnedPreconditioner.from_bins(asm.matrix, asm, bins, max_coarse=mc, params=lo.init_params(1),
+                                            by_component=comp)
+    maps = rs.hierarchy_by_bins(xyz, asm.book(), dpn, bins, max_coarse=mc, by_component=comp)
     Lh = rs.LearnedPreconditioner(asm.matrix, maps, params=lo.init_params(1))
     assert Lg.n_levels == len(ref) + 1
     for l, r in enumerate(ref):

@@ -134,6 +134,22 @@ def test_hierarchy_by_bins_matches_oracle(rs, orc, dpn, bins, max_coarse):
         assert np.all(np.diff(first) > 0)
 
 
+@pytest.mark.parametrize("bins,max_coarse", [((9, 9, 9), 100), ((8, 3, 5), 300)])
+def test_hierarchy_by_bins_per_component_matches_oracle(rs, orc, bins, max_coarse):
+    """RS_AGG_COMPONENTS: the host builder against the oracle's per-component nested bins on a
+    3-DOF book with a clamped face."""
+    import learned_oracle as lo
+    xyz, tets = rs.kuhn_box(16, 8, 8, 4.0, 1.0, 1.0)
+    fixed = np.repeat(np.abs(xyz[:, 0]) < 1e-12, 3).astype(np.uint8)
+    book, _ = orc.node_book(xyz.shape[0], tets, 3, fixed)
+    maps = rs.hierarchy_by_bins(xyz, book, 3, bins, max_coarse=max_coarse, by_component=True)
+    ref = lo.bins_hierarchy(xyz, book, 3, bins, max_coarse, by_component=True)
+    assert len(maps) == len(ref) >= 2
+    for m, r in zip(maps, ref):
+        assert np.array_equal(m, r)
+    assert int(maps[-1].max()) + 1 <= max_coarse
+
+
 def test_hierarchy_by_bins_nested(rs, orc):
     """Nested coordinate bins: each level map halves the bins; composing the maps gives the
     direct binning of the finest positions at the coarser resolution."""

@@ -175,3 +175,58 @@ def test_cycle_preconditions_cg():
     jac = its(sla.LinearOperator(A.shape, matvec=lambda r: r / A.diagonal()))
     mg = its(sla.LinearOperator(A.shape, matvec=lambda r: lo.vcycle(levels, Ps, r, 1, 1, lo.coarse_exact)))
     assert mg < 0.5 * jac
+
+
+def test_fcg_closed_forms():
+    """The oracle's flexible CG: an exact preconditioner converges in one iteration; the identity
+    gives plain CG, which terminates after as many iterations as A has distinct eigenvalues."""
+    rng = np.random.default_rng(7)
+    Q, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    A = Q @ np.diag(np.repeat([1.0, 3.0, 10.0], [10, 15, 15])) @ Q.T
+    b = rng.standard_normal(40)
+    x, its = lo.fcg(A, b, lambda r: np.linalg.solve(A, r), tol=1e-12)
+    assert its == 1 and np.allclose(A @ x, b, atol=1e-10)
+    x, its = lo.fcg(A, b, lambda r: r, tol=1e-10)
+    assert its == 3 and np.allclose(A @ x, b, atol=1e-8)
+
+
+def _cantilever_like(nx=8, ny=3, nz=3, seed=0):
+    """A 3-DOF-per-node SPD operator on a box of nodes: three scalar Laplacians coupled by a
+    small symmetric term, clamped (removed) at x = 0; coordinates and node book alongside."""
+    g = np.stack(np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij"), -1)
+    xyz = g.reshape(-1, 3).astype(np.float64) / nx
+    nn = xyz.shape[0]
+    fixed = xyz[:, 0] == 0
+    book = np.full(3 * nn, 0xFFFFFFFF, dtype=np.uint64)
+    free = np.nonzero(~fixed)[0]
+    for k, i in enumerate(free):
+        book[3 * i:3 * i + 3] = 3 * k + np.arange(3)
+    return xyz, book, free
+
+
+def test_component_bins_keep_components_apart():
+    """Per-component bins: level 0 puts two DOFs in one aggregate exactly when they have the same
+    component and their nodes share a bin (brute force over all pairs), and every level's
+    aggregate holds a single component."""
+    xyz, book, free = _cantilever_like()
+    bins = (4, 2, 2)
+    maps = lo.bins_hierarchy(xyz, book, 3, bins, 40, by_component=True)
+    book1 = np.full(xyz.shape[0], 0xFFFFFFFF, dtype=np.uint64)   # the free nodes, one DOF each
+    book1[free] = np.arange(free.size)
+    node_maps = lo.bins_hierarchy(xyz, book1, 1, bins, 40 // 3)
+    n_dof = 3 * free.size
+    comp = np.arange(n_dof) % 3
+    node = np.arange(n_dof) // 3
+    m0, nb0 = maps[0], node_maps[0]
+    same = m0[:, None] == m0[None, :]
+    want = (comp[:, None] == comp[None, :]) & (nb0[node][:, None] == nb0[node][None, :])
+    assert np.array_equal(same, want)
+    # composed maps: still one component per aggregate on every level
+    cur = np.arange(n_dof)
+    for m in maps:
+        cur = m[cur]
+        for a in np.unique(cur):
+            assert np.unique(comp[cur == a]).size == 1
+    # the per-node aggregation of the same book mixes the components of a node
+    mixed = lo.bins_hierarchy(xyz, book, 3, bins, 40)
+    assert mixed[0][0] == mixed[0][1] == mixed[0][2]
