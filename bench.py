@@ -295,36 +295,40 @@ def elasticity(rs, args) -> int:
     rs.synchronize()
     t_hier = time.perf_counter() - t0
     Lm.set_cycle(args.pre, args.post, rs.COARSE_EXACT)
-    conf_ml = rs.MatrixSolverConf(n_outer_iterations=20000, solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES,
-                                  precision=rs.PRECISION_MIXED)
-    sml = rs.RMatrixSolver(conf_ml)
+    for name, prec, krylov in (("mixed_multilevel_fcg", rs.PRECISION_MIXED, "FP32 vectors and SpMV, FP64 residual "
+                                                                          "replacement"),
+                               ("mixed_multilevel_fcg_fp64_krylov", rs.PRECISION_FP64, "FP64 vectors and SpMV")):
+        conf_ml = rs.MatrixSolverConf(n_outer_iterations=20000, solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES,
+                                      precision=prec)
+        sml = rs.RMatrixSolver(conf_ml)
 
-    def solve_ml():
-        rs._check(lib.rs_memcpy_dtod(dx.ptr, dzero.ptr, 8 * n))
-        t0 = time.perf_counter()
-        info = sml.solve_learned_device(asm.matrix, Lm, db, dx)
-        rs.synchronize()
-        return time.perf_counter() - t0, info
+        def solve_ml():
+            rs._check(lib.rs_memcpy_dtod(dx.ptr, dzero.ptr, 8 * n))
+            t0 = time.perf_counter()
+            info = sml.solve_learned_device(asm.matrix, Lm, db, dx)
+            rs.synchronize()
+            return time.perf_counter() - t0, info
 
-    solve_ml()                                     # warm-up (level tuning, graph capture)
-    times, info = [], None
-    for _ in range(max(1, args.steps)):
-        t, info = solve_ml()
-        times.append(t)
-    x = dx.download()
-    res["mixed_multilevel_fcg"] = {
-        "time_s": round(float(np.median(times)), 4), "iterations": info.iterations, "true_relres": info.true_relres,
-        "reason": rs.REASON_NAMES.get(info.reason), "fallback": info.fallback,
-        "solve_ms": round(info.solve_ms, 2), "fp64_residual_replacements": info.inner_iterations,
-        "solution_error_vs_oracle_fp64": float(np.linalg.norm(x - x_star) / np.linalg.norm(x_star)),
-        "levels": [int(v) for v in Lm.level_rows], "level_nnz": [int(v) for v in Lm.level_nnz],
-        "cycle": {"pre": args.pre, "post": args.post, "aggregation": "nested coordinate bins per component",
-                  "smoother": "Chebyshev in D^-1 A, FP32 (FP16 copy of D^-1 A on levels >= 65,536 rows)",
-                  "coarsest": "exact inverse (FP64 sweep on the GPU, applied in FP32)"},
-        "hierarchy_build_s": round(t_hier, 4),
-        "time_with_hierarchy_build_s": round(float(np.median(times)) + t_hier, 4)}
-    log(f"mixed_multilevel_fcg: {res['mixed_multilevel_fcg']['time_s'] * 1e3:.1f} ms, {info.iterations} its, "
-        f"relres {info.true_relres:.2e}, levels {res['mixed_multilevel_fcg']['levels']}, hierarchy {t_hier:.3f} s")
+        solve_ml()                                     # warm-up (level tuning, graph capture)
+        times, info = [], None
+        for _ in range(max(1, args.steps)):
+            t, info = solve_ml()
+            times.append(t)
+        x = dx.download()
+        res[name] = {
+            "time_s": round(float(np.median(times)), 4), "iterations": info.iterations,
+            "true_relres": info.true_relres, "reason": rs.REASON_NAMES.get(info.reason), "fallback": info.fallback,
+            "solve_ms": round(info.solve_ms, 2), "fp64_residual_replacements": info.inner_iterations,
+            "solution_error_vs_oracle_fp64": float(np.linalg.norm(x - x_star) / np.linalg.norm(x_star)),
+            "levels": [int(v) for v in Lm.level_rows], "level_nnz": [int(v) for v in Lm.level_nnz],
+            "cycle": {"pre": args.pre, "post": args.post, "aggregation": "nested coordinate bins per component",
+                      "smoother": "Chebyshev in D^-1 A, FP32 (FP16 copy of D^-1 A on levels >= 65,536 rows)",
+                      "coarsest": "exact inverse (FP64 sweep on the GPU, applied in FP32)"},
+            "krylov": krylov, "hierarchy_build_s": round(t_hier, 4),
+            "time_with_hierarchy_build_s": round(float(np.median(times)) + t_hier, 4)}
+        log(f"{name}: {res[name]['time_s'] * 1e3:.1f} ms, {info.iterations} its, "
+            f"relres {info.true_relres:.2e}, levels {res[name]['levels']}, hierarchy {t_hier:.3f} s")
+    best_ml = min(("mixed_multilevel_fcg", "mixed_multilevel_fcg_fp64_krylov"), key=lambda k: res[k]["time_s"])
     spmv = {}
     for name, prec in (("fp64", rs.PRECISION_FP64), ("fp32", rs.PRECISION_MIXED)):
         plan, tune_ms = asm.matrix.spmv_plan(prec)
@@ -344,14 +348,15 @@ def elasticity(rs, args) -> int:
     log(f"stress processing: {spm:.3f} ms, max von Mises {vmax:.4g}")
     out = {"metric": "time-to-1e-8-residual per system (elasticity; mixed precision, multilevel flexible CG; "
                      "block-Jacobi-3 CG beside it)",
-           "value": res["mixed_multilevel_fcg"]["time_s"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+           "value": res[best_ml]["time_s"], "value_method": best_ml, "unit": "s", "n_gpus": 1,
+           "higher_is_better": False,
            "dtype": "f64 / f32 SpMV+precond with f64 residuals", "data": "synthetic",
            "config": {"workload": f"cantilever {nx}x{ny}x{nz} cells x 6 tets, E lognormal(0,1), nu 0.3",
                       "nodes": int(nn), "tets": int(ne), "dofs": int(n), "nnz": int(asm.nnz)},
            "assembly_ms": round(asm_kernel_ms, 3), "methods": res,
            "speedup_mixed_vs_fp64": round(res["fp64_blockjacobi3_cg"]["time_s"] / res["mixed_blockjacobi3_cg"]["time_s"], 3),
            "speedup_mixed_multilevel_vs_fp64_blockjacobi": round(
-               res["fp64_blockjacobi3_cg"]["time_s"] / res["mixed_multilevel_fcg"]["time_s"], 3),
+               res["fp64_blockjacobi3_cg"]["time_s"] / res[best_ml]["time_s"], 3),
            "reference_solution": xstar_info, "cpu_baseline": cpu, "spmv": spmv,
            "post_processing": {"stress_process_ms": round(spm, 4), "max_von_mises": vmax}}
     print(json.dumps(out), flush=True)

@@ -550,7 +550,10 @@ int rs_learned_apply(rs_learned *L, rs_matrix *A, const float *r_host, float *z_
 int rs_learned_record(rs_learned *L, uint32_t capacity);
 int rs_learned_records(const rs_learned *L, uint32_t count, float *out);
 int rs_learned_destroy(rs_learned *L);
-/* Flexible CG with the learned preconditioner (solver conf: CG; b, x device arrays). */
+/* Flexible CG with the learned preconditioner (solver conf: CG; b, x device arrays).  The
+ * cycle runs in FP32 either way; conf precision RS_PRECISION_MIXED keeps the Krylov vectors
+ * and product in FP32 with FP64 residual replacement (stopping rule: relres), RS_PRECISION_FP64
+ * keeps them in FP64 (the configured stopping rule). */
 int rs_solver_solve_learned_device(rs_solver *S, rs_matrix *A, rs_learned *L, const double *d_b,
                                    double *d_x, rs_info *info);
 

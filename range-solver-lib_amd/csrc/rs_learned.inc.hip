@@ -1204,6 +1204,61 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(FcgArgs a)
     finalize_in_launch<1, FIN_FCG_UPDATE>(v, a.partials, a.arrive, a.st, a.hist);
 }
 
+// ---------------------------------------------------------------------------- flexible CG, FP64
+// The Krylov vectors in FP64, the cycle in FP32: p = z (first iteration of a segment) or
+// p beta + z, with z the cycle's FP32 output.
+__global__ void k_fcg64_p(const float *__restrict__ z, double *__restrict__ p, uint32_t n, const SolveState *st)
+{
+    if (st->done) return;
+    const bool first = st->it == st->it0;
+    const double beta = st->beta;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        p[i] = first ? (double)z[i] : p[i] * beta + (double)z[i];
+}
+
+// x += alpha p, r -= alpha q in FP64; the FP32 copies of r (the cycle's right-hand side) and of
+// q (for z.q of the next beta); |x|^2 and |r|^2, then the stopping rule of the configuration
+// (the FP64 CG's, src/riterationinfo.cpp:90-109).
+struct Fcg64Args {
+    uint32_t n;
+    double *x, *r;
+    const double *p, *q;
+    float *r32, *q32;
+    double *partials;
+    uint32_t *arrive;
+    SolveState *st;
+    double *hist;
+};
+
+__global__ __launch_bounds__(kBlock) void k_fcg64_update(Fcg64Args a)
+{
+    if (a.st->done) return;
+    const double alpha = a.st->alpha;
+    double v[3] = { 0.0, 0.0, 0.0 };
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * kBlock) {
+        const double qi = a.q[i];
+        const double xi = a.x[i] + alpha * a.p[i];
+        const double ri = a.r[i] - alpha * qi;
+        a.x[i] = xi;
+        a.r[i] = ri;
+        a.r32[i] = (float)ri;
+        a.q32[i] = (float)qi;
+        v[0] += xi * xi;
+        v[1] += ri * ri;
+    }
+    // v[2] = 0: FIN_UPDATE's beta is the plain CG's; the next FIN_FCG_BETA replaces it
+    finalize_in_launch<3, FIN_UPDATE>(v, a.partials, a.arrive, a.st, a.hist);
+}
+
+// r32 = r, q32 = 0 before the first iteration
+__global__ void k_fcg64_start(const double *__restrict__ r, float *__restrict__ r32, float *__restrict__ q32, uint32_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        r32[i] = (float)r[i];
+        q32[i] = 0.0f;
+    }
+}
+
 }  // namespace rs
 
 // ============================================================================ hierarchy (host)
@@ -1633,6 +1688,62 @@ static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learn
     if (h.reason != RS_REASON_NONNORMAL) k_mp_fold<<<nvb, kBlock, 0, stream>>>(d_x, S.dx32.p, A.n);
     RS_HIP(hipGetLastError());
     return RS_OK;
+}
+
+// Flexible CG with FP64 Krylov vectors and FP64 SpMV around the FP32 cycle: the cycle's
+// rounding does not enter the recursion, so operators whose FP32 rounding is amplified
+// (elasticity: near rigid-body cancellation) keep the iteration count of an FP64 cycle.  Per
+// iteration: the cycle (its last step leaves r.z and z.q), FIN_FCG_BETA, p, the FP64 product
+// (k_cg_spmv_dot finalizes alpha = r.z / p.q) and the update (finalizes the stopping rule).
+static int solve_fcg64_learned(rs_solver &S, rs_matrix &A, rs_learned &L, double *d_x, uint32_t &done)
+{
+    int rc;
+    if ((rc = matrix_refresh_fp32(A)) || (rc = matrix_tune(A, true)) || (rc = matrix_tune(A, false))) return rc;
+    const int plan64 = A.plan[0];
+    if ((rc = matrix_sync_plan(A, A.plan[1], true)) || (rc = matrix_sync_plan(A, plan64, false))) return rc;
+    if ((rc = learned_setup(L, A)) || (rc = learned_plans(L))) return rc;
+    const size_t N = std::max(A.n, 1u);
+    if ((rc = S.z32.alloc(N)) || (rc = S.r32.alloc(N)) || (rc = S.q32.alloc(N))) return rc;
+    uint32_t nb64 = 0;
+    CgArgs a{};
+    a.A = plan_view(A, plan64, nb64);
+    a.b = S.y.p;
+    a.x = d_x;
+    a.r = S.r.p;
+    a.p = S.p.p;
+    a.q = S.q.p;
+    a.ptype = RS_PRECOND_NONE;
+    a.pw = 1;
+    a.partials = S.partials.p;
+    a.arrive = reinterpret_cast<uint32_t *>(S.partials.p + kPartialDoubles);
+    a.st = S.st.p;
+    a.hist = S.hist.p;
+    Fcg64Args u{ A.n, d_x, S.r.p, S.p.p, S.q.p, S.r32.p, S.q32.p, a.partials, a.arrive, S.st.p, S.hist.p };
+    const uint32_t nvb = vec_blocks(A.n);
+    hipStream_t stream = runtime().stream;
+    L.r0 = S.r32.p;
+    const uint32_t nbz = step_blocks(*L.lv[0]);
+    k_zero_u32<<<1, 64, 0, stream>>>(a.arrive, kArriveWords);
+    RS_SHAPE_SWITCH(plan64, k_cg_init, nb64, stream, a);
+    fin(S.partials.p, nb64, 1, FIN_CG_INIT, 0, S.st.p, S.hist.p);
+    k_fcg64_start<<<nvb, kBlock, 0, stream>>>(S.r.p, S.r32.p, S.q32.p, A.n);
+    RS_HIP(hipGetLastError());
+    const std::function<void()> body = [&] {
+        for (uint32_t i = 0; i < kGraphIters; i++) {
+            const float *z = cycle_level(L, 0, S.st.p, S.q32.p, S.partials.p, Prolong{});
+            if (L.rec_cap) {
+                CycLevel &Cl = *L.lv.back();
+                k_record<<<vec_blocks(L.nc), kBlock, 0, stream>>>(Cl.r.p, L.rec.p, L.nc, L.rec_cap, S.st.p);
+            }
+            fin(S.partials.p, nbz, 2, FIN_FCG_BETA, 0, S.st.p, S.hist.p);
+            k_fcg64_p<<<nvb, kBlock, 0, stream>>>(z, S.p.p, A.n, S.st.p);
+            RS_SHAPE_SWITCH(plan64, k_cg_spmv_dot, nb64, stream, a);   // finalizes alpha
+            k_fcg64_update<<<nvb, kBlock, 0, stream>>>(u);               // finalizes the stopping rule
+        }
+    };
+    rc = capture(S, body, graph_key_of(4, a, plan64, nb64, nbz, nvb) + learned_key(L));
+    if (rc) return rc;
+    return run_graph_loop(S, done, body);
 }
 
 }  // namespace rs

@@ -2285,6 +2285,7 @@ struct rs_learned;
 namespace rs {
 static int solve_fcg_learned(rs_solver &S, rs_matrix &A, rs_precond &P, rs_learned &L, double *d_x, uint32_t &done,
                              uint32_t &refinements);
+static int solve_fcg64_learned(rs_solver &S, rs_matrix &A, rs_learned &L, double *d_x, uint32_t &done);
 
 static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, double *d_x, int32_t ptype,
                             uint32_t block_size, rs_learned *L, rs_info *info)
@@ -2296,7 +2297,9 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
     if (!learned && (ptype < RS_PRECOND_NONE || ptype > RS_PRECOND_BLOCK_JACOBI))
         return fail(RS_ERR_INVALID_PRECOND, "Invalid matrix preconditioner type '" + std::to_string(ptype) + "'");
     if (learned) ptype = RS_PRECOND_JACOBI;   // the smoother part and the fallback
-    bool mixed = c.precision == RS_PRECISION_MIXED || learned;
+    // the learned cycle always runs in FP32; PRECISION_MIXED also keeps the flexible CG's
+    // vectors and product in FP32 (FP64 residual replacement), PRECISION_FP64 keeps them in FP64
+    bool mixed = c.precision == RS_PRECISION_MIXED;
     bool gmres = c.solver_type == RS_SOLVER_GMRES;
     if (mixed && gmres) return fail(RS_ERR_UNSUPPORTED, "mixed precision is implemented for CG only");
     int rc;
@@ -2329,7 +2332,8 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
         if (gmres) {
             rc = solve_gmres(S, A, P, d_x, iters, inner);
         } else if (learned) {
-            rc = solve_fcg_learned(S, A, P, *L, d_x, done, refinements);
+            rc = mixed ? solve_fcg_learned(S, A, P, *L, d_x, done, refinements)
+                       : solve_fcg64_learned(S, A, *L, d_x, done);
             if (rc) return rc;
             SolveState hs;
             RS_HIP(hipMemcpyAsync(&hs, S.st.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
@@ -2341,7 +2345,7 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
                 if ((rc = precond_build(A, ptype, block_size, P))) return rc;
                 k_state_restart<<<1, 64, 0, stream>>>(S.st.p);
                 uint32_t more = 0;
-                rc = solve_cg_mixed(S, A, P, d_x, done, more);
+                rc = mixed ? solve_cg_mixed(S, A, P, d_x, done, more) : solve_cg64(S, A, P, d_x, done);
                 refinements += more;
             }
         } else if (mixed) {

@@ -48,9 +48,9 @@ def _gpu_solve(rs, M, b, precision, precond, width, cvg=RELRES):
     return dx.download(), info
 
 
-def _learned_solve(rs, M, b, L):
+def _learned_solve(rs, M, b, L, precision=None):
     conf = rs.MatrixSolverConf(n_outer_iterations=5000, solver_cvg_value=RELRES, stop_rule=rs.STOP_RELRES,
-                               precision=rs.PRECISION_MIXED)
+                               precision=rs.PRECISION_MIXED if precision is None else precision)
     S = rs.RMatrixSolver(conf)
     db, dx = rs.DeviceArray.from_host(b), rs.DeviceArray.from_host(np.zeros_like(b))
     info = S.solve_learned_device(M, L, db, dx)
@@ -125,12 +125,13 @@ def test_heat_mixed_cg_against_oracle_solution(gpu, heat):
     assert err <= ERR_BAR
 
 
-@pytest.mark.parametrize("variant", ["default", "fp16_smoother", "fp16_smoother_vecrows_fp32"])
+@pytest.mark.parametrize("variant", ["default", "fp16_smoother", "fp16_smoother_vecrows_fp32", "fp64_krylov"])
 def test_heat_learned_fcg_against_oracle_solution(gpu, heat, variant):
     """The trained network in the flexible CG.  `fp16_smoother` forces the fine level's FP16
     smoother copy below its default size threshold; with `vecrows_fp32` the FP32 SpMV plan is
     also forced to a non-SELL shape, so the smoother and the SpMV run different grids (the
-    flexible CG's r.z and z.q partial sums come from the smoother's grid)."""
+    flexible CG's r.z and z.q partial sums come from the smoother's grid); `fp64_krylov` runs
+    the flexible CG's vectors and product in FP64 around the FP32 cycle (PRECISION_FP64)."""
     rs = gpu
     params, _ = rs.load_weights()
     assert params is not None, "the committed weights are missing"
@@ -143,7 +144,7 @@ def test_heat_learned_fcg_against_oracle_solution(gpu, heat, variant):
     if variant == "fp16_smoother_vecrows_fp32":
         M.force_plan(rs.PRECISION_MIXED, 1)
     try:
-        x, info = _learned_solve(rs, M, heat.b, L)
+        x, info = _learned_solve(rs, M, heat.b, L, rs.PRECISION_FP64 if variant == "fp64_krylov" else None)
         if variant == "fp16_smoother_vecrows_fp32":
             assert M.spmv_plan(rs.PRECISION_MIXED)[0] == 1
     finally:
@@ -233,6 +234,8 @@ def test_cantilever_multilevel_fcg_against_oracle_solution(gpu, cantilever):
         assert np.array_equal(L.level_map(l), m), l
     x, info = _learned_solve(rs, c.asm.matrix, c.b, L)
     err = _rel(x, c.x_star)
+    x64, info64 = _learned_solve(rs, c.asm.matrix, c.b, L, rs.PRECISION_FP64)
+    err64 = _rel(x64, c.x_star)
     levels, Ps = lo.hierarchy(c.A_ref, maps)
     lams = [lo.gershgorin(A) for A in levels[:-1]]
     cinv = np.linalg.inv(levels[-1].toarray())
@@ -240,14 +243,20 @@ def test_cantilever_multilevel_fcg_against_oracle_solution(gpu, cantilever):
     _, oits = lo.fcg(c.A_ref, c.b_ref, lambda r: lo.vcycle(levels, Ps, r, 1, 1, lambda A, rc: cinv @ rc, lams, 0,
                                                             fp16), tol=RELRES)
     _, bj = _gpu_solve(rs, c.asm.matrix, c.b, rs.PRECISION_FP64, rs.PRECOND_BLOCK_JACOBI, 3)
-    _say(f"cantilever {c.cells} multilevel FCG (per-component bins, exact coarsest): {info.iterations} its "
-         f"(oracle FP64 cycle {oits}, block-Jacobi(3) CG {bj.iterations}), levels {list(map(int, L.level_rows))}, "
-         f"error vs oracle {err:.2e}")
+    _say(f"cantilever {c.cells} multilevel FCG (per-component bins, exact coarsest): {info.iterations} its, "
+         f"FP64 Krylov {info64.iterations} its (oracle FP64 cycle {oits}, block-Jacobi(3) CG {bj.iterations}), "
+         f"levels {list(map(int, L.level_rows))}, error vs oracle {err:.2e} / {err64:.2e}")
     _check_converged(c.A, c.b, x, info)
     assert info.fallback == 0
     assert err <= ERR_BAR
     assert info.iterations <= 2 * oits
     assert 2 * info.iterations < bj.iterations
+    # FP64 Krylov vectors around the FP32 cycle: the oracle's iterations (the cycle's FP32
+    # rounding only perturbs the preconditioner, which the flexible CG absorbs)
+    _check_converged(c.A, c.b, x64, info64)
+    assert info64.fallback == 0
+    assert err64 <= ERR_BAR
+    assert abs(info64.iterations - oits) <= 0.1 * oits + 2
 
 
 def test_trained_network_beats_coarse_jacobi_on_held_out_problem(gpu, orc):
