@@ -289,15 +289,25 @@ def elasticity(rs, args) -> int:
     # (unknown-based aggregation), Chebyshev smoothing in D^-1 A, the exact coarsest inverse, in
     # flexible CG with mixed precision (FP32 cycle and SpMV, FP64 residual replacement)
     bins = (max(1, nx // 2), max(1, ny // 2), max(1, nz // 2))
-    rs.synchronize()
-    t0 = time.perf_counter()
-    Lm = rs.LearnedPreconditioner.from_bins(asm.matrix, asm, bins, max_coarse=rs.MAX_COARSE, by_component=True)
-    rs.synchronize()
-    t_hier = time.perf_counter() - t0
-    Lm.set_cycle(args.pre, args.post, rs.COARSE_EXACT)
-    for name, prec, krylov in (("mixed_multilevel_fcg", rs.PRECISION_MIXED, "FP32 vectors and SpMV, FP64 residual "
-                                                                          "replacement"),
-                               ("mixed_multilevel_fcg_fp64_krylov", rs.PRECISION_FP64, "FP64 vectors and SpMV")):
+    hier = {}
+    for agg_name, kw in (("components", {"by_component": True}), ("rigid", {"rigid": True})):
+        rs.synchronize()
+        t0 = time.perf_counter()
+        Lh = rs.LearnedPreconditioner.from_bins(asm.matrix, asm, bins, max_coarse=rs.MAX_COARSE, **kw)
+        rs.synchronize()
+        Lh.set_cycle(args.pre, args.post, rs.COARSE_EXACT)
+        hier[agg_name] = (Lh, time.perf_counter() - t0)
+    aggregation_text = {"components": "nested coordinate bins per displacement component (translations)",
+                        "rigid": "nested coordinate bins of nodes, six rigid-body modes per aggregate "
+                                 "(orthonormalised, 6 coarse rows per aggregate)"}
+    for name, prec, krylov, agg_name in (
+            ("mixed_multilevel_fcg", rs.PRECISION_MIXED, "FP32 vectors and SpMV, FP64 residual replacement",
+             "components"),
+            ("mixed_multilevel_fcg_fp64_krylov", rs.PRECISION_FP64, "FP64 vectors and SpMV", "components"),
+            ("mixed_rigid_multilevel_fcg", rs.PRECISION_MIXED, "FP32 vectors and SpMV, FP64 residual replacement",
+             "rigid"),
+            ("mixed_rigid_multilevel_fcg_fp64_krylov", rs.PRECISION_FP64, "FP64 vectors and SpMV", "rigid")):
+        Lm, t_hier = hier[agg_name]
         conf_ml = rs.MatrixSolverConf(n_outer_iterations=20000, solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES,
                                       precision=prec)
         sml = rs.RMatrixSolver(conf_ml)
@@ -321,14 +331,15 @@ def elasticity(rs, args) -> int:
             "solve_ms": round(info.solve_ms, 2), "fp64_residual_replacements": info.inner_iterations,
             "solution_error_vs_oracle_fp64": float(np.linalg.norm(x - x_star) / np.linalg.norm(x_star)),
             "levels": [int(v) for v in Lm.level_rows], "level_nnz": [int(v) for v in Lm.level_nnz],
-            "cycle": {"pre": args.pre, "post": args.post, "aggregation": "nested coordinate bins per component",
+            "cycle": {"pre": args.pre, "post": args.post, "aggregation": aggregation_text[agg_name],
                       "smoother": "Chebyshev in D^-1 A, FP32 (FP16 copy of D^-1 A on levels >= 65,536 rows)",
                       "coarsest": "exact inverse (FP64 sweep on the GPU, applied in FP32)"},
             "krylov": krylov, "hierarchy_build_s": round(t_hier, 4),
             "time_with_hierarchy_build_s": round(float(np.median(times)) + t_hier, 4)}
         log(f"{name}: {res[name]['time_s'] * 1e3:.1f} ms, {info.iterations} its, "
             f"relres {info.true_relres:.2e}, levels {res[name]['levels']}, hierarchy {t_hier:.3f} s")
-    best_ml = min(("mixed_multilevel_fcg", "mixed_multilevel_fcg_fp64_krylov"), key=lambda k: res[k]["time_s"])
+    best_ml = min(("mixed_multilevel_fcg", "mixed_multilevel_fcg_fp64_krylov", "mixed_rigid_multilevel_fcg",
+                   "mixed_rigid_multilevel_fcg_fp64_krylov"), key=lambda k: res[k]["time_s"])
     spmv = {}
     for name, prec in (("fp64", rs.PRECISION_FP64), ("fp32", rs.PRECISION_MIXED)):
         plan, tune_ms = asm.matrix.spmv_plan(prec)

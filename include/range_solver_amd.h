@@ -475,9 +475,13 @@ enum {
 /* How the bins group the DOFs of a node (dofs_per_node > 1). */
 enum {
     RS_AGG_NODES = 0,       /* the DOFs of a node share their bin's aggregate */
-    RS_AGG_COMPONENTS = 1   /* one aggregate per bin and component: the unknown-based
+    RS_AGG_COMPONENTS = 1,  /* one aggregate per bin and component: the unknown-based
                                aggregation for vector problems (elasticity), whose coarse
                                spaces keep the translations of each component */
+    RS_AGG_RIGID = 2        /* rs_learned_create_by_bins_ex only, 3 DOFs per node: node
+                               aggregates whose coarse spaces are their six rigid-body modes
+                               (translations and rotations), orthonormalised per aggregate; 6
+                               coarse rows per aggregate, coarsest at most max_coarse rows */
 };
 /* Aggregates from node coordinates: nbx*nby*nbz bins over the bounding box; agg_of_dof gets
  * the bin of each enabled position of the node book (ids compacted to 0..n_agg-1). */
@@ -512,11 +516,18 @@ int rs_learned_create_by_bins(const rs_matrix *A, uint32_t n_nodes, const double
                               uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx,
                               uint32_t nby, uint32_t nbz, uint32_t max_coarse,
                               uint32_t max_levels, rs_learned **out);
-/* The same with a choice of aggregation (the level maps of rs_hierarchy_by_bins_ex). */
+/* The same with a choice of aggregation: RS_AGG_NODES / RS_AGG_COMPONENTS give the level maps of
+ * rs_hierarchy_by_bins_ex; RS_AGG_RIGID (dofs_per_node 3, max_coarse >= 6) builds the rigid-body
+ * hierarchy on the node aggregates (at most max_coarse / 6 of them on the coarsest level; at most
+ * 256 rows per aggregate), whose level maps send each row to its aggregate (row 6 J + a of a
+ * coarser level belongs to the aggregate of node J). */
 int rs_learned_create_by_bins_ex(const rs_matrix *A, uint32_t n_nodes, const double *d_xyz,
                                  uint32_t dofs_per_node, const uint32_t *d_book, uint32_t nbx,
                                  uint32_t nby, uint32_t nbz, uint32_t max_coarse,
                                  uint32_t max_levels, int32_t aggregation, rs_learned **out);
+/* Rigid-body levels (RS_AGG_RIGID): the orthonormal basis rows of level `level` (6 doubles per
+ * row: row i of aggregate J holds row i of Q_J, the coarse rows 6 J .. 6 J + 5). */
+int rs_learned_level_basis(const rs_learned *L, uint32_t level, double *q, uint64_t capacity);
 /* Level map of level `level` (0 .. n_levels-2) to the next: one entry per row of the level. */
 int rs_learned_level_map(const rs_learned *L, uint32_t level, uint32_t *agg, uint64_t capacity);
 /* Network parameters, FP32, 49,921 values (layout: W_in[4][64], b_in[64], 6 x {W_s[64][64],

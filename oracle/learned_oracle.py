@@ -320,3 +320,122 @@ def fcg(A, b, prec, tol=1e-8, maxit=100000):
             return x, k
         p_prev, q_prev = p, q
     return x, maxit
+
+
+# ------------------------------------------------------------------------ rigid-body aggregation
+def rigid_modes(xyz, comp):
+    """The six rigid-body modes at DOFs with coordinates xyz (m x 3, relative to a reference
+    point) and components comp: translations x, y, z, then the rotations about z, x and y
+    (u = w x r)."""
+    m = xyz.shape[0]
+    B = np.zeros((m, 6))
+    B[np.arange(m), comp] = 1.0
+    x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+    B[:, 3] = np.where(comp == 0, -y, np.where(comp == 1, x, 0.0))
+    B[:, 4] = np.where(comp == 1, -z, np.where(comp == 2, y, 0.0))
+    B[:, 5] = np.where(comp == 2, -x, np.where(comp == 0, z, 0.0))
+    return B
+
+
+def frame_shift(p):
+    """T (6 x 6) with modes(r + p) = modes(r) T: the rigid-body modes about the origin from those
+    about the point p (translations unchanged, each rotation gains the translation p x w)."""
+    T = np.eye(6)
+    x0, y0, z0 = p
+    T[0, 3], T[1, 3] = -y0, x0
+    T[1, 4], T[2, 4] = -z0, y0
+    T[0, 5], T[2, 5] = z0, -x0
+    return T
+
+
+def mgs2(B, tol=1e-10):
+    """Modified Gram-Schmidt, every column orthogonalised twice against the previous live
+    columns, in column order.  Returns Q (m x 6, dead columns zero), R (6 x 6 upper triangular,
+    the rows of dead columns zero) and the dead flags: a column whose norm after the two passes
+    is at most tol times its norm before (or zero) is dead."""
+    m, k = B.shape
+    Q = np.zeros((m, k))
+    R = np.zeros((k, k))
+    dead = np.zeros(k, dtype=bool)
+    for c in range(k):
+        v = B[:, c].copy()
+        n0 = np.sqrt(v @ v)
+        for _ in range(2):
+            for cp in range(c):
+                if dead[cp]:
+                    continue
+                d = Q[:, cp] @ v
+                v = v - d * Q[:, cp]
+                R[cp, c] += d
+        nv = np.sqrt(v @ v)
+        if n0 == 0.0 or nv <= tol * n0:
+            dead[c] = True
+        else:
+            Q[:, c] = v / nv
+            R[c, c] = nv
+    return Q, R, dead
+
+
+def rigid_hierarchy(A, xyz, book, bins, max_coarse, max_levels=16, eod=0xFFFFFFFF):
+    """Smoothed-aggregation's tentative prolongations with the rigid-body modes (Vanek, Mandel,
+    Brezina 1996), unsmoothed, for 3-DOF-per-node elasticity: the node aggregates of the nested
+    bins (bins_hierarchy with the DOFs of a node in one aggregate, coarsest at most max_coarse /
+    6 aggregates); each aggregate J has a reference point p_J (the node of its first fine DOF);
+    its members' rigid-body modes about p_J (level 0: from the node coordinates; coarser levels:
+    the near-null-space rows B times frame_shift(-p_J)) are orthonormalised by mgs2 over the
+    members in row order, B_J = Q_J R_J; the 6 coarse rows of J (6 J + a) take the rows of
+    R_J frame_shift(p_J), the modes about the origin again, as their near-null-space rows, so that
+    all aggregates share one frame.  A dead column leaves a zero prolongation column and a unit
+    diagonal on its coarse row.
+    Returns the level operators [A_0, A_1, ...], the prolongations and the dead flags per level."""
+    import scipy.sparse as sp
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+    book = np.asarray(book, dtype=np.uint64).ravel()
+    maps = bins_hierarchy(xyz, book, 3, bins, max(1, max_coarse // 6), max_levels, eod)
+    pos = np.nonzero(book != eod)[0]
+    n0 = pos.size
+    node = np.empty(n0, dtype=np.int64)
+    comp = np.empty(n0, dtype=np.int64)
+    node[book[pos].astype(np.int64)] = pos // 3
+    comp[book[pos].astype(np.int64)] = pos % 3
+    levels, Ps, deads = [A.tocsr()], [], []
+    Bl = None
+    ref = xyz[node]          # reference point of every row of the current level
+    n = n0
+    for lvl, m in enumerate(maps):
+        m = np.asarray(m, dtype=np.int64)
+        rowmap = m if lvl == 0 else np.repeat(m, 6)
+        na = int(m.max()) + 1
+        rows, cols, vals = [], [], []
+        Bn = np.zeros((6 * na, 6))
+        dead_n = np.zeros(6 * na, dtype=bool)
+        ref_n = np.zeros((6 * na, 3))
+        order = np.argsort(rowmap, kind="stable")
+        bounds = np.searchsorted(rowmap[order], np.arange(na + 1))
+        for J in range(na):
+            ids = order[bounds[J]:bounds[J + 1]]
+            pJ = ref[ids[0]]
+            if lvl == 0:
+                B = rigid_modes(xyz[node[ids]] - pJ, comp[ids])
+            else:
+                B = Bl[ids] @ frame_shift(-pJ)
+            Q, R, dead = mgs2(B)
+            for a in range(6):
+                rows.append(ids)
+                cols.append(np.full(ids.size, 6 * J + a))
+                vals.append(Q[:, a])
+            Bn[6 * J:6 * J + 6] = R @ frame_shift(pJ)
+            dead_n[6 * J:6 * J + 6] = dead
+            ref_n[6 * J:6 * J + 6] = pJ
+        P = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, 6 * na))
+        P.eliminate_zeros()
+        Ac = (P.T @ levels[-1] @ P).tolil()
+        for r in np.nonzero(dead_n)[0]:
+            Ac[r, r] = 1.0
+        Ps.append(P.tocsr())
+        levels.append(Ac.tocsr())
+        deads.append(dead_n)
+        Bl = Bn
+        ref = ref_n
+        n = 6 * na
+    return levels, Ps, deads

@@ -180,6 +180,7 @@ _SIGS = {
     "rs_learned_create_by_bins_ex": [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)],
     "rs_learned_level_map": [_P, C.c_uint32, _P, C.c_uint64],
+    "rs_learned_level_basis": [_P, C.c_uint32, _P, C.c_uint64],
     "rs_assembly_device_mesh": [_P, C.POINTER(_P), C.POINTER(_P), _u32p, _u32p],
     "rs_learned_set_params": [_P, _P, C.c_uint64],
     "rs_learned_set_cycle": [_P, C.c_uint32, C.c_uint32, C.c_int],
@@ -923,7 +924,8 @@ class Batch:
 N_PARAMS = 49921
 MAX_COARSE = 512
 COARSE_NETWORK, COARSE_JACOBI, COARSE_EXACT = 0, 1, 2
-AGG_NODES, AGG_COMPONENTS = 0, 1   # rs_*_by_bins_ex: a node's DOFs share an aggregate / one per component
+# rs_*_by_bins_ex: a node's DOFs share an aggregate / one aggregate per component / rigid-body modes
+AGG_NODES, AGG_COMPONENTS, AGG_RIGID = 0, 1, 2
 
 
 NET_HID, NET_LAYERS, NET_IN = 64, 6, 4
@@ -1015,10 +1017,11 @@ class LearnedPreconditioner:
 
     @classmethod
     def from_bins(cls, A: SparseMatrix, asm, bins, max_coarse: int = 512, max_levels: int = 16,
-                  params=None, by_component: bool = False) -> "LearnedPreconditioner":
+                  params=None, by_component: bool = False, rigid: bool = False) -> "LearnedPreconditioner":
         """The hierarchy of hierarchy_by_bins(xyz, book, ...) over the assembly's mesh, built on the
         GPU from the assembly's device coordinates and DOF book (no host work over the mesh).
-        by_component: one aggregate per bin and DOF component (vector problems)."""
+        by_component: one aggregate per bin and DOF component (vector problems); rigid: node
+        aggregates with their six rigid-body modes as coarse space (3-DOF elasticity)."""
         load()
         self = cls.__new__(cls)
         self.A = A
@@ -1028,9 +1031,17 @@ class LearnedPreconditioner:
         self.h = C.c_void_p()
         _check(_lib.rs_learned_create_by_bins_ex(A.h, nn.value, xp, dpn.value, bp, int(bins[0]), int(bins[1]),
                                                  int(bins[2]), max_coarse, max_levels,
-                                                 AGG_COMPONENTS if by_component else AGG_NODES, C.byref(self.h)))
+                                                 AGG_RIGID if rigid else AGG_COMPONENTS if by_component else AGG_NODES,
+                                                 C.byref(self.h)))
         self._init_info(params)
         return self
+
+    def level_basis(self, level: int) -> np.ndarray:
+        """Rigid-body levels: the basis rows of level `level` (rows x 6)."""
+        n = int(self.level_rows[level])
+        out = np.empty(max(6 * n, 1))
+        _check(_lib.rs_learned_level_basis(self.h, level, _ptr(out), out.size))
+        return out[:6 * n].reshape(n, 6)
 
     def level_map(self, level: int) -> np.ndarray:
         """Map of level `level` (0 = the matrix rows) to the next level."""

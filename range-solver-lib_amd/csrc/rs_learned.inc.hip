@@ -1169,6 +1169,315 @@ __global__ void k_weights_bf16(const float *__restrict__ theta, unsigned short *
     }
 }
 
+// ---------------------------------------------------------------------------- rigid-body aggregation
+// For 3-DOF elasticity the transfer from level l to level l + 1 can take the six rigid-body modes
+// of every aggregate as its coarse space: the tentative prolongation of smoothed aggregation
+// (Vanek, Mandel, Brezina 1996), unsmoothed here.  Row i of aggregate J carries the row
+// q[6 i .. 6 i + 5] of the orthonormal basis Q_J of the modes of J's members; level l + 1 has
+// the rows 6 J + a.  The modes are taken about a reference point p_J (the node of the first fine
+// DOF of J) and orthonormalised by modified Gram-Schmidt, every column twice, B_J = Q_J R_J; the
+// rows of R_J shifted back to the origin's frame are the next level's near-null-space rows.
+
+constexpr uint32_t kRigidPerLane = 4;
+constexpr uint32_t kRigidMaxMembers = kRigidPerLane * kWave;   // rows per aggregate
+constexpr double kRigidDeadTol = 1e-10;                        // relative norm of a dependent mode
+
+// Reference point (node coordinates) and component of every DOF: level 0 of a rigid hierarchy.
+__global__ void k_rigid_dofs(const uint32_t *__restrict__ book, uint64_t npos, const double *__restrict__ xyz,
+                             double *__restrict__ ref, uint32_t *__restrict__ comp)
+{
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < npos; q += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t d = book[q];
+        if (d == RS_EOD) continue;
+        const uint64_t node = q / 3;
+        ref[3ull * d] = xyz[3 * node];
+        ref[3ull * d + 1] = xyz[3 * node + 1];
+        ref[3ull * d + 2] = xyz[3 * node + 2];
+        comp[d] = (uint32_t)(q % 3);
+    }
+}
+
+struct RigidQrArgs {
+    uint32_t nagg;
+    const uint32_t *mptr, *mrows;   // members of each aggregate, rows ascending
+    const double *ref;              // reference point of every row of the level (3 per row)
+    const uint32_t *comp;           // level 0: component of every row; nullptr on coarser levels
+    const double *bsrc;             // coarser levels: near-null-space rows (6 per row, origin's frame)
+    double *q64;                    // the basis rows of the level (6 per row)
+    float *q32;
+    double *bnext, *refnext;        // next level: near-null-space rows (6 per row), reference points
+    uint32_t *dead;                 // next level: 1 on the rows of dependent modes
+    uint32_t *bad;                  // an aggregate with more than kRigidMaxMembers rows
+};
+
+// One wavefront per aggregate J (kRigidPerLane members per lane, in row order lane + 64 u):
+// B_J = Q_J R_J by modified Gram-Schmidt, each column orthogonalised twice against the previous
+// live columns; wavefront sums by xor butterflies (every lane holds the same bits).  A column whose
+// norm falls to kRigidDeadTol of its norm before is dead: zero basis column, zero row of R.
+__global__ __launch_bounds__(kBlock) void k_rigid_qr(RigidQrArgs a)
+{
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / kWave);
+    // J is the same on every lane of a wavefront
+    for (uint64_t J = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; J < a.nagg; J += stride) {
+        const uint32_t m0 = a.mptr[J], m = a.mptr[J + 1] - m0;
+        if (m > kRigidMaxMembers) {
+            if (lane == 0) atomicOr(a.bad, 1u);
+            continue;
+        }
+        const uint32_t first = a.mrows[m0];
+        const double px = a.ref[3ull * first], py = a.ref[3ull * first + 1], pz = a.ref[3ull * first + 2];
+        double B[kRigidPerLane][6];
+        uint32_t row[kRigidPerLane];
+#pragma unroll
+        for (uint32_t u = 0; u < kRigidPerLane; u++) {
+            const uint32_t k = lane + u * kWave;
+            row[u] = k < m ? a.mrows[m0 + k] : 0u;
+#pragma unroll
+            for (int c = 0; c < 6; c++) B[u][c] = 0.0;
+            if (k >= m) continue;
+            const uint64_t i = row[u];
+            if (a.comp) {
+                // the modes at a DOF of component cm, relative to p: translations, rotations about z, x, y
+                const double rx = a.ref[3 * i] - px, ry = a.ref[3 * i + 1] - py, rz = a.ref[3 * i + 2] - pz;
+                const uint32_t cm = a.comp[i];
+                B[u][0] = cm == 0 ? 1.0 : 0.0;
+                B[u][1] = cm == 1 ? 1.0 : 0.0;
+                B[u][2] = cm == 2 ? 1.0 : 0.0;
+                B[u][3] = cm == 0 ? -ry : cm == 1 ? rx : 0.0;
+                B[u][4] = cm == 1 ? -rz : cm == 2 ? ry : 0.0;
+                B[u][5] = cm == 2 ? -rx : cm == 0 ? rz : 0.0;
+            } else {
+                // origin's frame -> p's frame: each rotation loses the translation p x w
+                const double *b = a.bsrc + 6 * i;
+                B[u][0] = b[0];
+                B[u][1] = b[1];
+                B[u][2] = b[2];
+                B[u][3] = b[3] + py * b[0] - px * b[1];
+                B[u][4] = b[4] + pz * b[1] - py * b[2];
+                B[u][5] = b[5] - pz * b[0] + px * b[2];
+            }
+        }
+        double R[6][6];
+        bool dead[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            dead[c] = false;
+#pragma unroll
+            for (int k = 0; k < 6; k++) R[c][k] = 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            double t = 0.0;
+#pragma unroll
+            for (uint32_t u = 0; u < kRigidPerLane; u++) t += B[u][c] * B[u][c];
+            const double n0 = sqrt(group_sum<kWave>(t));
+#pragma unroll
+            for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+                for (int cp = 0; cp < c; cp++) {
+                    if (dead[cp]) continue;
+                    double d = 0.0;
+#pragma unroll
+                    for (uint32_t u = 0; u < kRigidPerLane; u++) d += B[u][cp] * B[u][c];
+                    d = group_sum<kWave>(d);
+#pragma unroll
+                    for (uint32_t u = 0; u < kRigidPerLane; u++) B[u][c] -= d * B[u][cp];
+                    R[cp][c] += d;
+                }
+            }
+            t = 0.0;
+#pragma unroll
+            for (uint32_t u = 0; u < kRigidPerLane; u++) t += B[u][c] * B[u][c];
+            const double nv = sqrt(group_sum<kWave>(t));
+            if (n0 == 0.0 || nv <= kRigidDeadTol * n0) {
+                dead[c] = true;
+#pragma unroll
+                for (uint32_t u = 0; u < kRigidPerLane; u++) B[u][c] = 0.0;
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < kRigidPerLane; u++) B[u][c] /= nv;
+                R[c][c] = nv;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kRigidPerLane; u++) {
+            if (lane + u * kWave >= m) continue;
+            const uint64_t i = row[u];
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                a.q64[6 * i + c] = B[u][c];
+                a.q32[6 * i + c] = (float)B[u][c];
+            }
+        }
+        if (lane < 6) {
+            // row `lane` of R_J in the origin's frame: R T(p), each rotation gains the translation p x w
+            const int r = (int)lane;
+            double Rr[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) Rr[k] = 0.0;
+#pragma unroll
+            for (int rr = 0; rr < 6; rr++)
+                if (rr == r)
+#pragma unroll
+                    for (int k = 0; k < 6; k++) Rr[k] = R[rr][k];
+            double *o = a.bnext + 6 * (6 * J + r);
+            o[0] = Rr[0];
+            o[1] = Rr[1];
+            o[2] = Rr[2];
+            o[3] = Rr[3] - py * Rr[0] + px * Rr[1];
+            o[4] = Rr[4] - pz * Rr[1] + py * Rr[2];
+            o[5] = Rr[5] + pz * Rr[0] - px * Rr[2];
+            double *pr = a.refnext + 3 * (6 * J + r);
+            pr[0] = px;
+            pr[1] = py;
+            pr[2] = pz;
+            bool dr = false;
+#pragma unroll
+            for (int rr = 0; rr < 6; rr++)
+                if (rr == r) dr = dead[rr];
+            a.dead[6 * J + r] = dr ? 1u : 0u;
+        }
+    }
+}
+
+// Row of every entry of a CSR pattern.
+__global__ void k_entry_rows(const uint32_t *__restrict__ rowptr, uint32_t n, uint32_t *__restrict__ erow)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        for (uint32_t p = rowptr[i]; p < rowptr[i + 1]; p++) erow[p] = (uint32_t)i;
+}
+
+// out[6 I + a] = m[I]: the map of a level's rows when its nodes carry 6 rows each.
+__global__ void k_expand6(const uint32_t *__restrict__ m, uint64_t n6, uint32_t *__restrict__ out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n6; i += (uint64_t)gridDim.x * kBlock)
+        out[i] = m[i / 6];
+}
+
+// The next level's pattern from the aggregate pattern (cptr, ccol): every aggregate entry
+// (J, J') becomes the 6 x 6 block of rows 6 J + a and columns 6 J' + b, columns ascending
+// within a row; crow[k] is the aggregate row of entry k.
+__global__ void k_rigid_expand(const uint32_t *__restrict__ cptr, const uint32_t *__restrict__ ccol, uint32_t nn,
+                               uint32_t *__restrict__ erowptr, uint32_t *__restrict__ ecol, uint32_t *__restrict__ crow)
+{
+    for (uint64_t J = (uint64_t)blockIdx.x * kBlock + threadIdx.x; J < nn; J += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t k0 = cptr[J], len = cptr[J + 1] - k0;
+        for (uint32_t r = 0; r < 6; r++) {
+            const uint32_t base = 36 * k0 + 6 * r * len;
+            erowptr[6 * J + r] = base;
+            for (uint32_t k = 0; k < len; k++)
+                for (uint32_t b = 0; b < 6; b++) ecol[base + 6 * k + b] = 6 * ccol[k0 + k] + b;
+        }
+        for (uint32_t k = 0; k < len; k++) crow[k0 + k] = (uint32_t)J;
+        if (J + 1 == nn) erowptr[6ull * nn] = 36 * cptr[nn];
+    }
+}
+
+struct RigidGalArgs {
+    const double *fval;                     // level l values
+    const uint32_t *fcol, *frow;            // level l columns and rows of its entries
+    const double *q64;                      // level l basis rows
+    const uint32_t *gptr, *gsrc;            // aggregate entry k -> level l entries, in list order
+    const uint32_t *cptr, *ccol, *crow;     // aggregate pattern
+    const uint32_t *erowptr;                // level l + 1 row pointers (the expanded pattern)
+    const uint32_t *dead;                   // level l + 1 rows of dependent modes
+    uint64_t cnnz;
+    double *cval;                           // level l + 1 values
+};
+
+// Level l + 1 values Q^T A Q block by block: one wavefront per aggregate entry (J, J'), lane
+// 6 a + b sums q_i[a] a_ij q_j[b] over the listed entries (i, j) in list order; the diagonal of a
+// dead row is 1.
+__global__ __launch_bounds__(kBlock) void k_rigid_galerkin(RigidGalArgs a)
+{
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t ra = lane / 6, cb = lane % 6;
+    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / kWave);
+    for (uint64_t k = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; k < a.cnnz; k += stride) {
+        if (lane >= 36) continue;
+        uint32_t t = a.gptr[k];
+        const uint32_t e = a.gptr[k + 1];
+        double s = 0.0;
+        // four list entries' loads in flight, summed in list order
+        for (; t + 4 <= e; t += 4) {
+            uint32_t p[4];
+            double v[4], qi[4], qj[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) p[u] = a.gsrc[t + u];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                v[u] = a.fval[p[u]];
+                qi[u] = a.q64[6ull * a.frow[p[u]] + ra];
+                qj[u] = a.q64[6ull * a.fcol[p[u]] + cb];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) s += qi[u] * v[u] * qj[u];
+        }
+        for (; t < e; t++) {
+            const uint32_t p = a.gsrc[t];
+            s += a.q64[6ull * a.frow[p] + ra] * a.fval[p] * a.q64[6ull * a.fcol[p] + cb];
+        }
+        const uint32_t J = a.crow[k];
+        if (a.ccol[k] == J && ra == cb && a.dead[6 * J + ra]) s = 1.0;
+        a.cval[a.erowptr[6 * J + ra] + 6 * (k - a.cptr[J]) + cb] = s;
+    }
+}
+
+// r_next[6 J + a] = sum over the rows i of J of q_i[a] res_i (G lanes per aggregate, FP64 sums
+// in a fixed order), with the next level's first Chebyshev step fused as in k_cyc_restrict.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_rigid_restrict(const float *__restrict__ res, const uint32_t *__restrict__ mptr,
+                                                           const uint32_t *__restrict__ mrows, const float *__restrict__ q32,
+                                                           uint32_t nagg, float *__restrict__ rnext, const SolveState *st,
+                                                           FirstStep f)
+{
+    if (cyc_idle(st)) return;
+    const uint32_t gl = threadIdx.x % G;
+    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / G);
+    const float c2 = f.coef ? f.coef[1] : 0.0f;
+    for (uint64_t J = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G; J < nagg; J += stride) {
+        double s[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+        for (uint32_t k = mptr[J] + gl; k < mptr[J + 1]; k += G) {
+            const uint64_t i = mrows[k];
+            const double r = (double)res[i];
+#pragma unroll
+            for (int c = 0; c < 6; c++) s[c] += (double)q32[6 * i + c] * r;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) s[c] = group_sum<G>(s[c]);
+        if (gl == 0) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const uint64_t I = 6 * J + c;
+                const float rv = (float)s[c];
+                rnext[I] = rv;
+                if (f.coef) {
+                    const float z = c2 * f.dinv[I] * rv;
+                    f.z[I] = z;
+                    f.d[I] = z;
+                }
+            }
+        }
+    }
+}
+
+// z_out[i] = z_pre[i] + sum_a q_i[a] e[6 agg(i) + a] (z_pre nullptr: zero).
+__global__ void k_rigid_prolong(uint32_t n, const uint32_t *__restrict__ agg, const float *__restrict__ q32,
+                                const float *__restrict__ e, const float *__restrict__ zpre, float *__restrict__ zout,
+                                const SolveState *st)
+{
+    if (cyc_idle(st)) return;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        const float *q = q32 + 6 * i;
+        const float *ej = e + 6ull * agg[i];
+        float s = q[0] * ej[0];
+#pragma unroll
+        for (int c = 1; c < 6; c++) s = fmaf(q[c], ej[c], s);
+        zout[i] = (zpre ? zpre[i] : 0.0f) + s;
+    }
+}
+
 // ---------------------------------------------------------------------------- flexible CG
 struct FcgArgs {
     uint32_t n;
@@ -1274,6 +1583,14 @@ struct CycLevel {
     // to the next level
     uint32_t nnext = 0, rgroup = 1;
     DBuf<uint32_t> agg, mptr, mrows, gptr, gsrc;
+    // rigid-body transfer to the next level (6 rows per aggregate): this level's reference points
+    // (3 per row), components (level 0) or near-null-space rows (6 per row, coarser levels),
+    // dependent-mode flags (1 per row, coarser levels), basis rows, entry rows and the aggregate
+    // pattern of the next level's blocks
+    bool rigid = false;
+    DBuf<double> ref, bsrc, q64;
+    DBuf<float> q32;
+    DBuf<uint32_t> comp, dead, erow, ncptr, nccol, ncrow;
     ~CycLevel()
     {
         delete own;
@@ -1307,6 +1624,7 @@ struct rs_learned {
     bool inverse_valid = false;
     std::vector<double> lam;                         // Gershgorin bound per level
     const float *r0 = nullptr;                       // finest right-hand side of the cycle
+    bool rigid = false;                              // rigid-body transfers (6 rows per aggregate)
 };
 
 namespace rs {
@@ -1354,8 +1672,15 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     L.lv[0]->M = &A;
     for (uint32_t l = 0; l + 1 < L.nlev; l++) {
         CycLevel &F = *L.lv[l], &C = *L.lv[l + 1];
-        k_galerkin_gather<<<vec_blocks(std::max<uint64_t>(C.M->nnz, 1)), kBlock, 0, st>>>(F.M->val.p, F.gptr.p, F.gsrc.p,
-                                                                                         C.M->nnz, C.M->val.p);
+        if (F.rigid) {
+            const uint64_t cnnz = F.ncrow.n;
+            RigidGalArgs g{ F.M->val.p, F.M->col.p, F.erow.p, F.q64.p, F.gptr.p, F.gsrc.p, F.ncptr.p, F.nccol.p,
+                            F.ncrow.p, C.M->rowptr.p, C.dead.p, cnnz, C.M->val.p };
+            k_rigid_galerkin<<<vec_blocks(std::max<uint64_t>(cnnz, 1) * kWave), kBlock, 0, st>>>(g);
+        } else {
+            k_galerkin_gather<<<vec_blocks(std::max<uint64_t>(C.M->nnz, 1)), kBlock, 0, st>>>(F.M->val.p, F.gptr.p,
+                                                                                             F.gsrc.p, C.M->nnz, C.M->val.p);
+        }
         matrix_values_changed(*C.M);
     }
     const uint32_t kParts = 4096;
@@ -1562,10 +1887,23 @@ static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st,
     FirstStep fs{};
     if (L.pre >= 1 && l + 2 < L.nlev) fs = FirstStep{ L.coef.p + (size_t)(l + 1) * kMaxDeg * 2, C.dinv.p, C.za.p, C.d.p };
     const uint32_t G = V.rgroup;
-    launch_restrict(G, vec_blocks((uint64_t)V.nnext * G), rres, V, C.r.p, st, fs);
-    // the coarser level leaves z_pre + P e in buf[nxt]
     const int nxt = cur >= 0 ? cur ^ 1 : 0;
-    cycle_level(L, l + 1, st, nullptr, nullptr, Prolong{ V.mptr.p, V.mrows.p, cur >= 0 ? buf[cur] : nullptr, buf[nxt] });
+    if (V.rigid) {
+        // restriction Q^T res, the coarser cycle, then z_pre + Q e in buf[nxt] (one thread per row)
+        const uint32_t nagg = V.nnext / 6;
+        const uint32_t nbr = vec_blocks((uint64_t)nagg * G);
+        switch (G) {
+            case 4: k_rigid_restrict<4><<<nbr, kBlock, 0, s>>>(rres, V.mptr.p, V.mrows.p, V.q32.p, nagg, C.r.p, st, fs); break;
+            case 16: k_rigid_restrict<16><<<nbr, kBlock, 0, s>>>(rres, V.mptr.p, V.mrows.p, V.q32.p, nagg, C.r.p, st, fs); break;
+            default: k_rigid_restrict<64><<<nbr, kBlock, 0, s>>>(rres, V.mptr.p, V.mrows.p, V.q32.p, nagg, C.r.p, st, fs); break;
+        }
+        const float *e = cycle_level(L, l + 1, st, nullptr, nullptr, Prolong{});
+        k_rigid_prolong<<<nvb, kBlock, 0, s>>>(V.n, V.agg.p, V.q32.p, e, cur >= 0 ? buf[cur] : nullptr, buf[nxt], st);
+    } else {
+        launch_restrict(G, vec_blocks((uint64_t)V.nnext * G), rres, V, C.r.p, st, fs);
+        // the coarser level leaves z_pre + P e in buf[nxt]
+        cycle_level(L, l + 1, st, nullptr, nullptr, Prolong{ V.mptr.p, V.mrows.p, cur >= 0 ? buf[cur] : nullptr, buf[nxt] });
+    }
     cur = nxt;
     for (uint32_t k = 0; k < L.post; k++) {
         const bool last = k + 1 == L.post;
@@ -1865,6 +2203,8 @@ namespace rs {
 // The hierarchy from device level maps (maps[l] sends the nodes of level l to level l + 1):
 // members, next-level patterns and gather lists on the GPU; only the coarsest pattern (at most
 // 512 rows) comes to the host, for the network's 16 x 16 blocks.
+static int learned_finish(std::unique_ptr<rs_learned> &L, DBuf<uint32_t> *total, rs_learned **out);
+
 static int learned_build(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<uint32_t>>> &maps,
                          const std::vector<uint32_t> &n_agg, rs_learned **out)
 {
@@ -1873,7 +2213,6 @@ static int learned_build(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<ui
     const uint32_t nc = n_agg.back();
     if (nc == 0 || nc > kMaxCoarse) return fail(RS_ERR_UNSUPPORTED, "rs_learned_create: coarsest level must have 1..512 nodes");
     int rc;
-    hipStream_t st = runtime().stream;
     std::unique_ptr<rs_learned> L(new rs_learned());
     L->n = A->n;
     L->nc = nc;
@@ -1905,6 +2244,16 @@ static int learned_build(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<ui
         if ((rc = hier_compose(total.p, A->n, V.agg.p))) return rc;
         L->lv.push_back(std::move(C));
     }
+    return learned_finish(L, &total, out);
+}
+
+// Level buffers, the fine rows of each coarsest node (scalar hierarchies: total maps every fine
+// row to its coarsest node) and the coarsest graph's blocks for the network.
+static int learned_finish(std::unique_ptr<rs_learned> &L, DBuf<uint32_t> *total, rs_learned **out)
+{
+    int rc;
+    hipStream_t st = runtime().stream;
+    const uint32_t nc = L->nc;
     for (auto &lp : L->lv) {
         CycLevel &V = *lp;
         const size_t n = std::max(V.n, 1u);
@@ -1913,7 +2262,7 @@ static int learned_build(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<ui
             return rc;
     }
     // fine rows of each coarsest node
-    if ((rc = hier_members(total.p, A->n, nc, L->tot_mptr, L->tot_mrows))) return rc;
+    if (total && (rc = hier_members(total->p, L->n, nc, L->tot_mptr, L->tot_mrows))) return rc;
     // coarsest graph blocks, from the coarsest pattern
     const Matrix &Cc = *L->lv.back()->M;
     std::vector<uint32_t> rp(nc + 1ull), ci(Cc.nnz);
@@ -1957,6 +2306,95 @@ static int learned_build(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<ui
     RS_HIP(hipStreamSynchronize(st));
     *out = L.release();
     return RS_OK;
+}
+
+// The rigid-body hierarchy of a 3-DOF matrix from node-aggregate maps (maps[0]: DOFs -> aggregates,
+// maps[l]: aggregates of level l -> level l + 1) and the mesh in device memory.  Level l + 1 has
+// 6 rows per aggregate of maps[l]; everything is built on the GPU and depends on the mesh only.
+static int learned_build_rigid(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<uint32_t>>> &maps,
+                               const std::vector<uint32_t> &n_agg, uint32_t n_nodes, const double *d_xyz,
+                               const uint32_t *d_book, rs_learned **out)
+{
+    const uint32_t n_levels = (uint32_t)maps.size();
+    if (n_levels == 0 || n_levels != n_agg.size()) return fail(RS_ERR_INVALID_ARG, "rigid hierarchy: no levels");
+    const uint64_t nc64 = 6ull * n_agg.back();
+    if (nc64 == 0 || nc64 > kMaxCoarse) return fail(RS_ERR_UNSUPPORTED, "rigid hierarchy: coarsest level must have 1..512 rows");
+    int rc;
+    hipStream_t st = runtime().stream;
+    std::unique_ptr<rs_learned> L(new rs_learned());
+    L->n = A->n;
+    L->nc = (uint32_t)nc64;
+    L->nlev = n_levels + 1;
+    L->owner = A;
+    L->rigid = true;
+    L->lv.emplace_back(new CycLevel());
+    CycLevel &V0 = *L->lv[0];
+    V0.M = const_cast<rs_matrix *>(A);
+    V0.n = A->n;
+    if ((rc = V0.ref.alloc(3ull * std::max(A->n, 1u))) || (rc = V0.comp.alloc(std::max(A->n, 1u)))) return rc;
+    k_rigid_dofs<<<vec_blocks(3ull * n_nodes), kBlock, 0, st>>>(d_book, 3ull * n_nodes, d_xyz, V0.ref.p, V0.comp.p);
+    DBuf<uint32_t> bad;
+    if ((rc = bad.alloc(1))) return rc;
+    k_fill_u32<<<1, 64, 0, st>>>(bad.p, 1, 0u);
+    RS_HIP(hipGetLastError());
+    for (uint32_t l = 0; l < n_levels; l++) {
+        CycLevel &V = *L->lv[l];
+        const uint32_t nn = n_agg[l];
+        V.rigid = true;
+        V.nnext = 6 * nn;
+        if (l == 0) {
+            std::swap(V.agg.p, maps[0]->p);
+            std::swap(V.agg.n, maps[0]->n);
+        } else {
+            if ((rc = V.agg.alloc(std::max(V.n, 1u)))) return rc;
+            k_expand6<<<vec_blocks(V.n), kBlock, 0, st>>>(maps[l]->p, V.n, V.agg.p);
+        }
+        uint32_t cnnz = 0;
+        DBuf<uint32_t> cptr, ccol;
+        if ((rc = hier_members(V.agg.p, V.n, nn, V.mptr, V.mrows))) return rc;
+        if ((rc = hier_level_lists(V.M->rowptr.p, V.M->col.p, V.n, V.M->nnz, V.agg.p, nn, cptr, ccol, V.gptr, V.gsrc,
+                                   cnnz)))
+            return rc;
+        if (36ull * cnnz >= 0xFFFFFFFFull) return fail(RS_ERR_UNSUPPORTED, "rigid hierarchy: coarse level too large");
+        // members per aggregate within one wavefront's reach
+        std::vector<uint32_t> hm(nn + 1ull);
+        RS_HIP(hipMemcpyAsync(hm.data(), V.mptr.p, sizeof(uint32_t) * hm.size(), hipMemcpyDeviceToHost, st));
+        RS_HIP(hipStreamSynchronize(st));
+        uint32_t mmax = 0;
+        for (uint32_t J = 0; J < nn; J++) mmax = std::max(mmax, hm[J + 1] - hm[J]);
+        if (mmax > kRigidMaxMembers)
+            return fail(RS_ERR_UNSUPPORTED, "rigid hierarchy: an aggregate has more than 256 rows (use finer bins)");
+        std::unique_ptr<CycLevel> C(new CycLevel());
+        C->n = 6 * nn;
+        if ((rc = C->bsrc.alloc(36ull * nn)) || (rc = C->ref.alloc(18ull * nn)) || (rc = C->dead.alloc(6ull * nn)) ||
+            (rc = V.q64.alloc(6ull * std::max(V.n, 1u))) || (rc = V.q32.alloc(6ull * std::max(V.n, 1u))))
+            return rc;
+        RigidQrArgs qa{ nn, V.mptr.p, V.mrows.p, V.ref.p, l == 0 ? V.comp.p : nullptr, l == 0 ? nullptr : V.bsrc.p,
+                        V.q64.p, V.q32.p, C->bsrc.p, C->ref.p, C->dead.p, bad.p };
+        k_rigid_qr<<<vec_blocks((uint64_t)nn * kWave), kBlock, 0, st>>>(qa);
+        DBuf<uint32_t> erowptr, ecol;
+        if ((rc = erowptr.alloc(6ull * nn + 1)) || (rc = ecol.alloc(std::max<uint64_t>(36ull * cnnz, 1))) ||
+            (rc = V.ncrow.alloc(std::max(cnnz, 1u))) || (rc = V.erow.alloc(std::max<uint64_t>(V.M->nnz, 1))))
+            return rc;
+        k_rigid_expand<<<vec_blocks(nn), kBlock, 0, st>>>(cptr.p, ccol.p, nn, erowptr.p, ecol.p, V.ncrow.p);
+        k_entry_rows<<<vec_blocks(V.n), kBlock, 0, st>>>(V.M->rowptr.p, V.n, V.erow.p);
+        RS_HIP(hipGetLastError());
+        V.ncrow.n = cnnz;
+        std::swap(V.ncptr.p, cptr.p);
+        std::swap(V.ncptr.n, cptr.n);
+        std::swap(V.nccol.p, ccol.p);
+        std::swap(V.nccol.n, ccol.n);
+        if ((rc = matrix_create_device(6 * nn, erowptr, ecol, 36ull * cnnz, &C->own))) return rc;
+        C->M = C->own;
+        const double avg = (double)V.n / nn;
+        V.rgroup = avg <= 24 ? 4 : avg <= 96 ? 16 : 64;
+        L->lv.push_back(std::move(C));
+    }
+    uint32_t hbad = 0;
+    RS_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    if (hbad) return fail(RS_ERR_UNSUPPORTED, "rigid hierarchy: an aggregate has more than 256 rows");
+    return learned_finish(L, nullptr, out);
 }
 
 }  // namespace rs
@@ -2007,14 +2445,25 @@ extern "C" int rs_learned_create_by_bins_ex(const rs_matrix *A, uint32_t n_nodes
                                             rs_learned **out)
 {
     if (!A || !d_xyz || !d_book || !out || !n_nodes || !dofs_per_node || !nbx || !nby || !nbz || !max_coarse ||
-        !max_levels || max_levels > 16 || (aggregation != RS_AGG_NODES && aggregation != RS_AGG_COMPONENTS))
+        !max_levels || max_levels > 16 ||
+        (aggregation != RS_AGG_NODES && aggregation != RS_AGG_COMPONENTS && aggregation != RS_AGG_RIGID))
         return fail(RS_ERR_INVALID_ARG, "rs_learned_create_by_bins: bad arguments");
+    if (aggregation == RS_AGG_RIGID && (dofs_per_node != 3 || max_coarse < 6))
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_create_by_bins: rigid-body aggregation needs 3 DOFs per node "
+                                        "and max_coarse >= 6");
     if (max_coarse > kMaxCoarse) return fail(RS_ERR_UNSUPPORTED, "rs_learned_create_by_bins: max_coarse above 512");
     int rc = require_runtime();
     if (rc) return rc;
     std::vector<std::unique_ptr<DBuf<uint32_t>>> maps;
     std::vector<uint32_t> n_agg;
     const uint32_t bins[3] = { nbx, nby, nbz };
+    if (aggregation == RS_AGG_RIGID) {
+        // node aggregates (the DOFs of a node together), at most max_coarse / 6 on the coarsest level
+        if ((rc = hier_bins(n_nodes, d_xyz, dofs_per_node, d_book, A->n, bins, max_coarse / 6, max_levels, false, maps,
+                            n_agg)))
+            return rc;
+        return learned_build_rigid(A, maps, n_agg, n_nodes, d_xyz, d_book, out);
+    }
     if ((rc = hier_bins(n_nodes, d_xyz, dofs_per_node, d_book, A->n, bins, max_coarse, max_levels,
                         aggregation == RS_AGG_COMPONENTS, maps, n_agg)))
         return rc;
@@ -2126,10 +2575,26 @@ extern "C" int rs_learned_coarse_matrix(const rs_learned *L, double *dense)
     return RS_OK;
 }
 
+// The rigid-body basis rows of level `level` (6 per row of the level, FP64).
+extern "C" int rs_learned_level_basis(const rs_learned *L, uint32_t level, double *q, uint64_t capacity)
+{
+    if (!L || !q || level + 1 >= L->nlev) return fail(RS_ERR_INVALID_ARG, "rs_learned_level_basis: bad arguments");
+    const CycLevel &V = *L->lv[level];
+    if (!V.rigid) return fail(RS_ERR_INVALID_ARG, "rs_learned_level_basis: not a rigid-body level");
+    if (capacity < 6ull * V.n) return fail(RS_ERR_INVALID_ARG, "rs_learned_level_basis: capacity below 6 x the level's rows");
+    int rc = require_runtime();
+    if (rc) return rc;
+    hipStream_t st = runtime().stream;
+    if (V.n) RS_HIP(hipMemcpyAsync(q, V.q64.p, sizeof(double) * 6 * V.n, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+}
+
 // A fine vector (device, FP64) restricted straight to the coarsest level.
 extern "C" int rs_learned_restrict_device(rs_learned *L, const double *d_r, float *rc_host)
 {
     if (!L || !d_r || !rc_host) return fail(RS_ERR_INVALID_ARG, "rs_learned_restrict_device: null");
+    if (L->rigid) return fail(RS_ERR_UNSUPPORTED, "rs_learned_restrict_device: not for rigid-body hierarchies");
     int rc = require_runtime();
     if (rc) return rc;
     hipStream_t st = runtime().stream;

@@ -259,6 +259,56 @@ def test_cantilever_multilevel_fcg_against_oracle_solution(gpu, cantilever):
     assert abs(info64.iterations - oits) <= 0.1 * oits + 2
 
 
+def test_cantilever_rigid_multilevel_fcg_against_oracle(gpu, cantilever):
+    """Elasticity through the rigid-body hierarchy (RS_AGG_RIGID: node aggregates whose coarse
+    spaces are their six rigid-body modes, built on the GPU) against the oracle's restatement
+    (learned_oracle.rigid_hierarchy): level maps equal; basis rows and level operators within
+    1e-12; flexible CG with FP64 Krylov vectors takes the oracle's FP64 iterations (within 10% + 2)
+    and mixed precision at most twice as many, both within 1e-6 of the oracle's FP64 solution."""
+    import learned_oracle as lo
+    rs = gpu
+    c = cantilever
+    nx, ny, nz = c.cells
+    bins = (max(1, nx // 2), max(1, ny // 2), max(1, nz // 2))
+    _say(f"cantilever {c.cells}: rigid-body hierarchy on the GPU")
+    L = rs.LearnedPreconditioner.from_bins(c.asm.matrix, c.asm, bins, max_coarse=rs.MAX_COARSE, rigid=True)
+    L.set_cycle(1, 1, rs.COARSE_EXACT)
+    xyz = rs.kuhn_box(nx, ny, nz, 4.0, 1.0, 1.0)[0]
+    book = c.asm.book()
+    levels, Ps, deads = lo.rigid_hierarchy(c.A_ref, xyz, book, bins, rs.MAX_COARSE)
+    maps = lo.bins_hierarchy(xyz, book, 3, bins, rs.MAX_COARSE // 6)
+    assert L.n_levels == len(levels)
+    assert [int(v) for v in L.level_rows] == [Lv.shape[0] for Lv in levels]
+    L.setup()
+    for l, P in enumerate(Ps):
+        rowmap = maps[l] if l == 0 else np.repeat(maps[l], 6)
+        assert np.array_equal(L.level_map(l), rowmap), l
+        q = L.level_basis(l)
+        Pd = P.tocsr()
+        ref = np.zeros_like(q)
+        for a in range(6):
+            ref[:, a] = np.asarray(Pd[np.arange(q.shape[0]), 6 * rowmap.astype(np.int64) + a]).ravel()
+        assert np.abs(q - ref).max() <= 1e-12, l
+        G, R = L.level_matrix(l + 1), levels[l + 1]
+        assert np.abs(G - R).max() <= 1e-12 * np.abs(R).max(), l
+    lams = [lo.gershgorin(Lv) for Lv in levels[:-1]]
+    cinv = np.linalg.inv(levels[-1].toarray())
+    fp16 = tuple(l for l in range(len(levels) - 1) if levels[l].shape[0] >= 65536)
+    _, oits = lo.fcg(c.A_ref, c.b_ref, lambda r: lo.vcycle(levels, Ps, r, 1, 1, lambda A, rc: cinv @ rc, lams, 0,
+                                                            fp16), tol=RELRES)
+    x64, i64 = _learned_solve(rs, c.asm.matrix, c.b, L, rs.PRECISION_FP64)
+    xmp, imp = _learned_solve(rs, c.asm.matrix, c.b, L)
+    e64, emp = _rel(x64, c.x_star), _rel(xmp, c.x_star)
+    _say(f"cantilever {c.cells} rigid multilevel FCG: FP64 Krylov {i64.iterations} its, mixed {imp.iterations} its "
+         f"(oracle FP64 cycle {oits}), levels {list(map(int, L.level_rows))}, error vs oracle {e64:.2e} / {emp:.2e}")
+    for x, info, err in ((x64, i64, e64), (xmp, imp, emp)):
+        _check_converged(c.A, c.b, x, info)
+        assert info.fallback == 0
+        assert err <= ERR_BAR
+    assert abs(i64.iterations - oits) <= 0.1 * oits + 2
+    assert imp.iterations <= 2 * oits
+
+
 def test_trained_network_beats_coarse_jacobi_on_held_out_problem(gpu, orc):
     """The committed weights (trained on 64^3 problems with lognormal(0, 2.5) conductivity, seeds
     10000-11999) on a problem none of them saw (seed 20002): fewer flexible-CG iterations than
