@@ -181,8 +181,12 @@ def learned_only(rs, args, asm, xyz, dk, dq, dT, db, dx, zero_x, conf) -> int:
 def elasticity(rs, args) -> int:
     """Linear elasticity cantilever (RSolverStress form): box 4 x 1 x 1 in nx x ny x nz cells
     (default 128 x 32 x 32) x 6 tets, E lognormal(0, 1) per tet (seed 3), nu = 0.3, clamped at
-    x = 0, traction (0, 0, -1) on x = 4; block-Jacobi (3 x 3 node blocks) CG in FP64 and in
-    mixed precision to relres 1e-8.  Prints one JSON line for this workload."""
+    x = 0, traction (0, 0, -1) on x = 4; to relres 1e-8: block-Jacobi (3 x 3 node blocks) CG in FP64
+    and in mixed precision, and flexible CG with the multilevel cycle over per-component and over
+    rigid-body aggregates (mixed precision: FP32 cycle; FP32 or FP64 Krylov vectors).  Every timed
+    solve follows a GPU re-assembly of the system and includes its kernel time, so the solver's
+    per-matrix set-up (FP32 copies, preconditioner, Galerkin levels, coarsest inverse) is part of
+    the time.  Prints one JSON line for this workload."""
     nx, ny, nz = args.elast_cells
     L = 4.0
     xyz, tets = rs.kuhn_box(nx, ny, nz, L, 1.0, 1.0)
@@ -223,15 +227,23 @@ def elasticity(rs, args) -> int:
     dzero = rs.DeviceArray.from_host(np.zeros(n))
     lib = rs.load()
 
+    def reassemble():
+        # every timed solve is a new system: the GPU assembly writes the matrix values again, so the
+        # solver's per-matrix work (FP32 copies, preconditioner, multilevel set-up) runs again; the
+        # assembly's kernel time is added to the solve (its host transfers are not)
+        asm.assemble_stress(E, nu, rho, [0.0, 0.0, 0.0], np.zeros(3 * nn), tris, trac)
+        return asm.last_ms * 1e-3
+
     def solve(precision, cvg):
         conf = rs.MatrixSolverConf(n_outer_iterations=200000, solver_cvg_value=cvg, stop_rule=rs.STOP_RELRES,
                                    precision=precision)
         S = rs.RMatrixSolver(conf)
+        t_asm = reassemble()
         rs._check(lib.rs_memcpy_dtod(dx.ptr, dzero.ptr, 8 * n))
         t0 = time.perf_counter()
         info = S.solve_device(asm.matrix, db, dx, rs.PRECOND_BLOCK_JACOBI, 3)
         rs.synchronize()
-        return time.perf_counter() - t0, info
+        return time.perf_counter() - t0 + t_asm, info
 
     cpu, x_star, xstar_info = None, None, None
     if not args.no_cpu_baseline:
@@ -313,11 +325,12 @@ def elasticity(rs, args) -> int:
         sml = rs.RMatrixSolver(conf_ml)
 
         def solve_ml():
+            t_asm = reassemble()
             rs._check(lib.rs_memcpy_dtod(dx.ptr, dzero.ptr, 8 * n))
             t0 = time.perf_counter()
             info = sml.solve_learned_device(asm.matrix, Lm, db, dx)
             rs.synchronize()
-            return time.perf_counter() - t0, info
+            return time.perf_counter() - t0 + t_asm, info
 
         solve_ml()                                     # warm-up (level tuning, graph capture)
         times, info = [], None

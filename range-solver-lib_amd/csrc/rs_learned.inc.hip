@@ -1386,41 +1386,50 @@ struct RigidGalArgs {
     double *cval;                           // level l + 1 values
 };
 
-// Level l + 1 values Q^T A Q block by block: one wavefront per aggregate entry (J, J'), lane
-// 6 a + b sums q_i[a] a_ij q_j[b] over the listed entries (i, j) in list order; the diagonal of a
-// dead row is 1.
+// Level l + 1 values Q^T A Q block by block: one wavefront per aggregate entry (J, J').  Lane t
+// takes the listed entries (i, j) t, t + 64, ... in list order and keeps the 36 products
+// q_i[a] a_ij q_j[b]; xor butterflies then sum them over the wavefront (a fixed order) and lane
+// 6 a + b writes entry (a, b).  The diagonal of a dead row is 1.
 __global__ __launch_bounds__(kBlock) void k_rigid_galerkin(RigidGalArgs a)
 {
     const uint32_t lane = threadIdx.x % kWave;
-    const uint32_t ra = lane / 6, cb = lane % 6;
     const uint64_t stride = (uint64_t)gridDim.x * (kBlock / kWave);
+    // k is the same on every lane of a wavefront
     for (uint64_t k = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; k < a.cnnz; k += stride) {
-        if (lane >= 36) continue;
-        uint32_t t = a.gptr[k];
+        double acc[36];
+#pragma unroll
+        for (int c = 0; c < 36; c++) acc[c] = 0.0;
         const uint32_t e = a.gptr[k + 1];
-        double s = 0.0;
-        // four list entries' loads in flight, summed in list order
-        for (; t + 4 <= e; t += 4) {
-            uint32_t p[4];
-            double v[4], qi[4], qj[4];
+        for (uint32_t t = a.gptr[k] + lane; t < e; t += kWave) {
+            const uint32_t p = a.gsrc[t];
+            const double v = a.fval[p];
+            const double *qi = a.q64 + 6ull * a.frow[p];
+            const double *qj = a.q64 + 6ull * a.fcol[p];
+            double vi[6], vj[6];
 #pragma unroll
-            for (int u = 0; u < 4; u++) p[u] = a.gsrc[t + u];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                v[u] = a.fval[p[u]];
-                qi[u] = a.q64[6ull * a.frow[p[u]] + ra];
-                qj[u] = a.q64[6ull * a.fcol[p[u]] + cb];
+            for (int c = 0; c < 6; c++) {
+                vi[c] = qi[c];
+                vj[c] = qj[c];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) s += qi[u] * v[u] * qj[u];
+            for (int ra = 0; ra < 6; ra++) {
+                const double w = vi[ra] * v;
+#pragma unroll
+                for (int cb = 0; cb < 6; cb++) acc[6 * ra + cb] += w * vj[cb];
+            }
         }
-        for (; t < e; t++) {
-            const uint32_t p = a.gsrc[t];
-            s += a.q64[6ull * a.frow[p] + ra] * a.fval[p] * a.q64[6ull * a.fcol[p] + cb];
+        double out = 0.0;
+#pragma unroll
+        for (int c = 0; c < 36; c++) {
+            const double t = group_sum<kWave>(acc[c]);
+            if (lane == (uint32_t)c) out = t;
         }
-        const uint32_t J = a.crow[k];
-        if (a.ccol[k] == J && ra == cb && a.dead[6 * J + ra]) s = 1.0;
-        a.cval[a.erowptr[6 * J + ra] + 6 * (k - a.cptr[J]) + cb] = s;
+        if (lane < 36) {
+            const uint32_t ra = lane / 6, cb = lane % 6;
+            const uint32_t J = a.crow[k];
+            if (a.ccol[k] == J && ra == cb && a.dead[6 * J + ra]) out = 1.0;
+            a.cval[a.erowptr[6 * J + ra] + 6 * (k - a.cptr[J]) + cb] = out;
+        }
     }
 }
 
