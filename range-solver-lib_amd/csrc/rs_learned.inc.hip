@@ -549,39 +549,57 @@ __global__ void k_dense_scatter(const uint32_t *__restrict__ rowptr, const uint3
         for (uint32_t p = rowptr[i]; p < rowptr[i + 1]; p++) m[(uint64_t)i * N + col[p]] = val[p];
 }
 
-// D = A_kk^-1 by sweeping the 64 scalar pivots of the block in LDS (one workgroup).  A pivot that
-// is not positive (not SPD) raises *bad.
+// D = A_kk^-1 by sweeping the 64 scalar pivots of the block (one workgroup; each thread keeps a
+// 4 x 4 block of the pivot block in registers, and before every pivot step the owners of row c
+// and column c publish them in LDS).  A pivot that is not positive (not SPD) raises *bad.
 __global__ __launch_bounds__(kBlock) void k_sweep_pivot(const double *__restrict__ m, uint32_t N, uint32_t k,
                                                         double *__restrict__ d, uint32_t *bad)
 {
-    __shared__ double a[kSweepB * kSweepLd];
     __shared__ double rowc[kSweepB], colc[kSweepB];
+    const uint32_t rb = threadIdx.x / 16, cb = threadIdx.x % 16;   // rows 4 rb + i, columns 4 cb + j
     const uint64_t o = (uint64_t)k * kSweepB;
-    for (uint32_t e = threadIdx.x; e < kSweepB * kSweepB; e += kBlock)
-        a[(e / kSweepB) * kSweepLd + e % kSweepB] = m[(o + e / kSweepB) * N + o + e % kSweepB];
-    __syncthreads();
+    double v[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[i][j] = m[(o + 4 * rb + i) * N + o + 4 * cb + j];
     for (uint32_t c = 0; c < kSweepB; c++) {
-        const double p = a[c * kSweepLd + c];
-        if (threadIdx.x < kSweepB) {
-            rowc[threadIdx.x] = a[c * kSweepLd + threadIdx.x];
-            colc[threadIdx.x] = a[threadIdx.x * kSweepLd + c];
-        }
-        if (threadIdx.x == 0 && !(p > 0.0)) *bad = 1u;
+        const uint32_t cr = c % 4;
+        if (c / 4 == rb)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if ((uint32_t)i == cr)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) rowc[4 * cb + j] = v[i][j];
+        if (c / 4 == cb)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((uint32_t)j == cr)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) colc[4 * rb + i] = v[i][j];
         __syncthreads();
+        const double p = rowc[c];
+        if (threadIdx.x == 0 && !(p > 0.0)) *bad = 1u;
         const double ip = p > 0.0 ? 1.0 / p : 0.0;
-        for (uint32_t e = threadIdx.x; e < kSweepB * kSweepB; e += kBlock) {
-            const uint32_t i = e / kSweepB, j = e % kSweepB;
-            double v;
-            if (i == c && j == c) v = -ip;
-            else if (i == c) v = rowc[j] * ip;
-            else if (j == c) v = colc[i] * ip;
-            else v = a[i * kSweepLd + j] - colc[i] * rowc[j] * ip;
-            a[i * kSweepLd + j] = v;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t gi = 4 * rb + i;
+            const double ci = colc[gi];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t gj = 4 * cb + j;
+                if (gi == c && gj == c) v[i][j] = -ip;
+                else if (gi == c) v[i][j] = rowc[gj] * ip;
+                else if (gj == c) v[i][j] = ci * ip;
+                else v[i][j] = v[i][j] - ci * rowc[gj] * ip;
+            }
         }
         __syncthreads();
     }
-    for (uint32_t e = threadIdx.x; e < kSweepB * kSweepB; e += kBlock)
-        d[e] = -a[(e / kSweepB) * kSweepLd + e % kSweepB];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) d[(4 * rb + i) * kSweepB + 4 * cb + j] = -v[i][j];
 }
 
 // C = X Y for two 64 x 64 LDS tiles; each thread owns a 4 x 4 block of C.
