@@ -535,6 +535,9 @@ int rs_learned_level_map(const rs_learned *L, uint32_t level, uint32_t *agg, uin
 int rs_learned_set_params(rs_learned *L, const float *params, uint64_t count);
 /* Smoothing steps before (0..4) and after (1..4) the coarse correction, coarsest solver. */
 int rs_learned_set_cycle(rs_learned *L, uint32_t pre, uint32_t post, int coarse_mode);
+/* Smoothing steps on the levels below the finest (rs_learned_set_cycle sets both to its own
+ * pre/post; this call then changes the coarser levels' only). */
+int rs_learned_set_coarse_cycle(rs_learned *L, uint32_t pre, uint32_t post);
 /* Levels with at least min_rows rows smooth with a compressed copy of D^-1 A (FP16 values,
  * int16 column offsets in the SELL-64 slots; 4 bytes per entry); 0 keeps every level on its
  * FP32 values.  Default 65536.  Levels whose offsets do not fit in 16 bits stay FP32. */

@@ -181,6 +181,7 @@ _SIGS = {
                                      C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)],
     "rs_learned_level_map": [_P, C.c_uint32, _P, C.c_uint64],
     "rs_learned_level_basis": [_P, C.c_uint32, _P, C.c_uint64],
+    "rs_learned_set_coarse_cycle": [_P, C.c_uint32, C.c_uint32],
     "rs_assembly_device_mesh": [_P, C.POINTER(_P), C.POINTER(_P), _u32p, _u32p],
     "rs_learned_set_params": [_P, _P, C.c_uint64],
     "rs_learned_set_cycle": [_P, C.c_uint32, C.c_uint32, C.c_int],
@@ -1069,6 +1070,10 @@ class LearnedPreconditioner:
 
     def set_cycle(self, pre: int = 1, post: int = 1, coarse: int = COARSE_NETWORK) -> None:
         _check(_lib.rs_learned_set_cycle(self.h, pre, post, coarse))
+
+    def set_coarse_cycle(self, pre: int = 1, post: int = 1) -> None:
+        """Smoothing steps on the levels below the finest (set_cycle resets them to its own)."""
+        _check(_lib.rs_learned_set_coarse_cycle(self.h, pre, post))
 
     def set_compressed(self, min_rows: int) -> None:
         """Levels with >= min_rows rows smooth with the FP16 copy of D^-1 A (0: none)."""

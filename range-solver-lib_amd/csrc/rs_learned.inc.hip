@@ -1642,7 +1642,8 @@ struct rs_learned {
     rs::DBuf<float> theta;
     rs::DBuf<unsigned short> wbf;
     bool have_params = false;
-    uint32_t pre = 1, post = 1;
+    uint32_t pre = 1, post = 1;                      // smoothing steps on the finest level
+    uint32_t cpre = 1, cpost = 1;                    // and on the coarser levels
     int coarse = RS_COARSE_NETWORK;
     uint32_t rec_cap = 0;
     uint32_t s16_min_rows = 65536;                   // levels at least this large smooth with the FP16 copy
@@ -1884,7 +1885,8 @@ static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st,
     const float *coef = L.coef.p + (size_t)l * kMaxDeg * 2;
     float *buf[2] = { V.za.p, V.zb.p };
     int cur = -1;
-    for (uint32_t k = 0; k < L.pre; k++) {
+    const uint32_t npre = l == 0 ? L.pre : L.cpre, npost = l == 0 ? L.post : L.cpost;
+    for (uint32_t k = 0; k < npre; k++) {
         a.coef = coef + 2 * k;
         if (k == 0) {
             if (l == 0) {
@@ -1912,7 +1914,7 @@ static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st,
         rres = V.res.p;
     }
     FirstStep fs{};
-    if (L.pre >= 1 && l + 2 < L.nlev) fs = FirstStep{ L.coef.p + (size_t)(l + 1) * kMaxDeg * 2, C.dinv.p, C.za.p, C.d.p };
+    if (L.cpre >= 1 && l + 2 < L.nlev) fs = FirstStep{ L.coef.p + (size_t)(l + 1) * kMaxDeg * 2, C.dinv.p, C.za.p, C.d.p };
     const uint32_t G = V.rgroup;
     const int nxt = cur >= 0 ? cur ^ 1 : 0;
     if (V.rigid) {
@@ -1932,8 +1934,8 @@ static const float *cycle_level(rs_learned &L, uint32_t l, const SolveState *st,
         cycle_level(L, l + 1, st, nullptr, nullptr, Prolong{ V.mptr.p, V.mrows.p, cur >= 0 ? buf[cur] : nullptr, buf[nxt] });
     }
     cur = nxt;
-    for (uint32_t k = 0; k < L.post; k++) {
-        const bool last = k + 1 == L.post;
+    for (uint32_t k = 0; k < npost; k++) {
+        const bool last = k + 1 == npost;
         a.coef = coef + 2 * k;
         a.zin = buf[cur];
         a.zout = buf[cur ^ 1];
@@ -1959,6 +1961,8 @@ static std::string learned_key(const rs_learned &L)
     }
     k.push_back(L.pre);
     k.push_back(L.post);
+    k.push_back(L.cpre);
+    k.push_back(L.cpost);
     k.push_back((uint64_t)L.coarse);
     k.push_back(L.rec_cap);
     k.push_back((uint64_t)(uintptr_t)L.rec.p);
@@ -2539,9 +2543,19 @@ extern "C" int rs_learned_set_cycle(rs_learned *L, uint32_t pre, uint32_t post, 
         return fail(RS_ERR_INVALID_ARG, "rs_learned_set_cycle: pre in 0..4 and post in 1..4 smoothing steps");
     if (coarse_mode < RS_COARSE_NETWORK || coarse_mode > RS_COARSE_EXACT)
         return fail(RS_ERR_INVALID_ARG, "rs_learned_set_cycle: unknown coarse mode");
-    L->pre = pre;
-    L->post = post;
+    L->pre = L->cpre = pre;
+    L->post = L->cpost = post;
     L->coarse = coarse_mode;
+    return RS_OK;
+}
+
+extern "C" int rs_learned_set_coarse_cycle(rs_learned *L, uint32_t pre, uint32_t post)
+{
+    if (!L) return fail(RS_ERR_INVALID_ARG, "rs_learned_set_coarse_cycle: null");
+    if (pre > kMaxDeg || post == 0 || post > kMaxDeg)
+        return fail(RS_ERR_INVALID_ARG, "rs_learned_set_coarse_cycle: pre in 0..4 and post in 1..4 smoothing steps");
+    L->cpre = pre;
+    L->cpost = post;
     return RS_OK;
 }
 
