@@ -271,12 +271,15 @@ def compressed(A):
     return (sp.diags(dg) @ S).tocsr()
 
 
-def vcycle(levels, Ps, r, pre, post, coarse, lams=None, level=0, fp16_levels=()):
+def vcycle(levels, Ps, r, pre, post, coarse, lams=None, level=0, fp16_levels=(), coarse_steps=None):
     """z = B r; coarse(Ac, r) solves the coarsest level; levels listed in fp16_levels smooth
-    (and form the restricted residual) with compressed(A_l)."""
+    (and form the restricted residual) with compressed(A_l); coarse_steps = (pre, post) on the
+    levels below the finest (None: the finest level's)."""
     A = levels[level]
     if level == len(levels) - 1:
         return coarse(A, r)
+    if coarse_steps is not None and level > 0:
+        pre, post = coarse_steps
     lam = gershgorin(A) if lams is None else lams[level]
     coefs = cheb_coefs(lam)
     dg = A.diagonal()
@@ -284,7 +287,7 @@ def vcycle(levels, Ps, r, pre, post, coarse, lams=None, level=0, fp16_levels=())
     As = compressed(A) if level in fp16_levels else A
     z = cheb_smooth(As, dinv, coefs, r, None, pre) if pre > 0 else None
     res = r if z is None else r - As @ z
-    e = vcycle(levels, Ps, Ps[level].T @ res, pre, post, coarse, lams, level + 1, fp16_levels)
+    e = vcycle(levels, Ps, Ps[level].T @ res, pre, post, coarse, lams, level + 1, fp16_levels, coarse_steps)
     z = Ps[level] @ e if z is None else z + Ps[level] @ e
     return cheb_smooth(As, dinv, coefs, r, z, post)
 

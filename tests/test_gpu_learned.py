@@ -219,6 +219,32 @@ def test_vcycle_matches_oracle(gpu, pre, post, mode, fp16):
     assert _rel(bad, ref) > 100 * CYC_RTOL
 
 
+@pytest.mark.parametrize("pre,post,cpre,cpost,mode", [(1, 1, 3, 3, "jacobi"), (1, 1, 0, 2, "exact"),
+                                                      (2, 1, 1, 4, "exact")])
+def test_vcycle_coarse_degrees_match_oracle(gpu, pre, post, cpre, cpost, mode):
+    """Smoothing steps on the coarse levels set apart from the finest (rs_learned_set_coarse_cycle)."""
+    rs = gpu
+    _say(f"vcycle pre={pre} post={post} coarse pre={cpre} post={cpost} coarse={mode}")
+    asm, _, xyz = _heat(rs, 20, 2.0)
+    maps = rs.hierarchy_by_bins(xyz, asm.book(), 1, (10, 10, 10), max_coarse=200)
+    L = rs.LearnedPreconditioner(asm.matrix, maps)
+    L.set_cycle(pre, post, rs.COARSE_EXACT if mode == "exact" else rs.COARSE_JACOBI)
+    L.set_coarse_cycle(cpre, cpost)
+    L.set_compressed(0)
+    r = np.random.default_rng(cpre * 10 + cpost).standard_normal(asm.n_dof).astype(np.float32)
+    z = L.apply(r)
+    levels = _levels_from_gpu(L)
+    assert len(levels) >= 3
+    _, Ps = lo.hierarchy(asm.matrix.to_scipy(), maps)
+    lams = L.level_lambda()
+    ref = lo.vcycle(levels, Ps, r.astype(np.float64), pre, post, _coarse_fn(mode), lams, coarse_steps=(cpre, cpost))
+    assert np.all(np.isfinite(z))
+    assert _rel(z, ref) < CYC_RTOL, _rel(z, ref)
+    # the same cycle with the finest level's steps everywhere is far outside the tolerance
+    bad = lo.vcycle(levels, Ps, r.astype(np.float64), pre, post, _coarse_fn(mode), lams)
+    assert _rel(bad, ref) > 100 * CYC_RTOL
+
+
 def test_inference_matches_oracle(gpu):
     rs = gpu
     _say("inference 8x8x8 aggregates")
