@@ -1188,10 +1188,11 @@ def main() -> int:
                     help="profiling mode: time only the learned method (no FP64/mixed CG, no ablation)")
     ap.add_argument("--pre", type=int, default=1, help="pre-smoothing steps of the V-cycle")
     ap.add_argument("--post", type=int, default=1, help="post-smoothing steps of the V-cycle")
-    ap.add_argument("--coarse-pre", type=int, default=3,
-                    help="heat's learned cycle: pre-smoothing steps on the levels below the finest (3 steps "
-                         "there measured 19.4 ms / 53 its against 20.0 ms / 65 its with 1 at 128^3)")
-    ap.add_argument("--coarse-post", type=int, default=3,
+    ap.add_argument("--coarse-pre", type=int, default=1,
+                    help="heat's learned cycle: pre-smoothing steps on the levels below the finest (3 there: "
+                         "20.8 ms / 53 its against 21.3 ms / 65 its with 1 at 128^3, but coarse Jacobi on the "
+                         "coarsest level then needs only 56 its, so the network gains nothing)")
+    ap.add_argument("--coarse-post", type=int, default=1,
                     help="heat's learned cycle: post-smoothing steps on the levels below the finest")
     ap.add_argument("--assembly-only", action="store_true",
                     help="elasticity profiling mode: a 256 MiB device copy (counter calibration), then 5 "
