@@ -1607,6 +1607,7 @@ struct CycLevel {
     DBuf<float> diag, dinv, r, za, zb, d, res;
     DBuf<uint32_t> s16;                  // compressed smoother copy (SELL-64 slots), when used
     bool use_s16 = false;
+    int s16_pref = 0;                    // timed against the FP32 plan: 0 not yet, 1 faster, 2 slower
     // to the next level
     uint32_t nnext = 0, rgroup = 1;
     DBuf<uint32_t> agg, mptr, mrows, gptr, gsrc;
@@ -1647,6 +1648,7 @@ struct rs_learned {
     int coarse = RS_COARSE_NETWORK;
     uint32_t rec_cap = 0;
     uint32_t s16_min_rows = 65536;                   // levels at least this large smooth with the FP16 copy
+    bool s16_auto = true;                            // ... when it times faster than the FP32 plan
     rs::DBuf<uint32_t> s16bad;
     uint64_t setup_version = ~0ull;
     bool inverse_valid = false;
@@ -1730,6 +1732,7 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     for (uint32_t l = 0; l + 1 < L.nlev; l++) {
         CycLevel &V = *L.lv[l];
         if (L.s16_min_rows == 0 || V.n < L.s16_min_rows || V.M->sell_col.n == 0) continue;
+        if (L.s16_auto && V.s16_pref == 2) continue;
         if ((rc = V.s16.alloc(V.M->sell_col.n)) || (rc = matrix_sync_plan(*V.M, kSellPlan, false))) return rc;
         uint32_t nb = 0;
         const CsrView sv = plan_view(*V.M, kSellPlan, nb);
@@ -1742,7 +1745,8 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     RS_HIP(hipStreamSynchronize(st));
     for (uint32_t l = 0; l < L.nlev; l++) {
         CycLevel &V = *L.lv[l];
-        V.use_s16 = l + 1 < L.nlev && L.s16_min_rows != 0 && V.n >= L.s16_min_rows && V.s16.p && bad[l] == 0;
+        V.use_s16 = l + 1 < L.nlev && L.s16_min_rows != 0 && V.n >= L.s16_min_rows && V.s16.p && bad[l] == 0 &&
+                    !(L.s16_auto && V.s16_pref == 2);
     }
     std::vector<float> coef((size_t)L.nlev * kMaxDeg * 2);
     for (uint32_t l = 0; l < L.nlev; l++) cheb_coefs(L.lam[l] > 0.0 ? L.lam[l] : 1.0, &coef[(size_t)l * kMaxDeg * 2]);
@@ -1776,6 +1780,10 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
 }
 
 // FP32 SpMV plans of every level (autotuned once per level).
+// and, under the default threshold, whether a level's compressed smoother copy is faster than
+// its FP32 plan: the two residual kernels timed back to back on the level's own vectors, once per
+// hierarchy (long rows favour the FP32 shapes with several lanes per row: the rigid-body
+// level 1 of the cantilever, 83 entries per row).
 static int learned_plans(rs_learned &L)
 {
     int rc;
@@ -1783,6 +1791,46 @@ static int learned_plans(rs_learned &L)
         Matrix &M = *L.lv[l]->M;
         if ((rc = matrix_refresh_fp32(M)) || (rc = matrix_tune(M, true)) || (rc = matrix_sync_plan(M, M.plan[1], true)))
             return rc;
+    }
+    if (!L.s16_auto) return RS_OK;
+    hipStream_t st = runtime().stream;
+    for (uint32_t l = 0; l + 1 < L.nlev; l++) {
+        CycLevel &V = *L.lv[l];
+        if (!V.use_s16 || V.s16_pref != 0) continue;
+        const int plan = V.M->plan[1];
+        uint32_t nb = 0, nbs = 0;
+        CycArgs a{};
+        a.A = plan_view(*V.M, plan, nb);
+        a.r = V.r.p;
+        a.zin = V.za.p;
+        a.res = V.res.p;
+        a.dinv = V.dinv.p;
+        a.n = V.n;
+        const CsrView v16 = plan_view(*V.M, kSellPlan, nbs);
+        const S16View sv{ V.n, v16.nslices, v16.spb, v16.sell_ptr, V.s16.p };
+        k_fill32<<<vec_blocks(V.n), kBlock, 0, st>>>(V.r.p, V.n, 1.0f);
+        k_fill32<<<vec_blocks(V.n), kBlock, 0, st>>>(V.za.p, V.n, 1.0f);
+        RS_HIP(hipGetLastError());
+        double ms[2] = { 0.0, 0.0 };
+        for (int k = 0; k < 2; k++) {
+            auto launch = [&]() {
+                if (k == 0) k_cyc_resid_s16<<<nbs, kBlock, 0, st>>>(sv, a, V.diag.p);
+                else RS_SHAPE_SWITCH(plan, k_cyc_resid, nb, st, a);
+            };
+            launch();
+            EventPair ev;
+            RS_HIP(ev.create());
+            RS_HIP(hipEventRecord(ev.e[0], st));
+            for (int i = 0; i < 10; i++) launch();
+            RS_HIP(hipEventRecord(ev.e[1], st));
+            RS_HIP(hipEventSynchronize(ev.e[1]));
+            RS_HIP(hipGetLastError());
+            float f = 0.0f;
+            RS_HIP(hipEventElapsedTime(&f, ev.e[0], ev.e[1]));
+            ms[k] = f;
+        }
+        V.s16_pref = ms[0] <= ms[1] ? 1 : 2;
+        if (V.s16_pref == 2) V.use_s16 = false;
     }
     return RS_OK;
 }
@@ -2563,6 +2611,7 @@ extern "C" int rs_learned_set_compressed(rs_learned *L, uint32_t min_rows)
 {
     if (!L) return fail(RS_ERR_INVALID_ARG, "rs_learned_set_compressed: null");
     L->s16_min_rows = min_rows;
+    L->s16_auto = false;   // an explicit threshold is followed as given
     L->setup_version = ~0ull;   // rebuilt at the next set-up
     return RS_OK;
 }
