@@ -894,28 +894,46 @@ def fluid(rs, args) -> int:
     n = asm.n_dof
     db, dx, dz = rs.DeviceArray(n), rs.DeviceArray(n), rs.DeviceArray.from_host(np.zeros(n))
     restart = args.fluid_restart
-    sconf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=60, n_inner_iterations=restart,
-                                solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES)
-    solver = rs.RMatrixSolver(sconf)
     lib = rs.load()
-    steps = []
-    for it in range(args.warmup + args.steps):
-        rs.synchronize()
-        t0 = time.perf_counter()
-        asm.assemble_fluid_device(conf, drho, dmu, dg, ds, dvo, dacc, db)
-        a_ms = asm.last_ms
-        rs._check(lib.rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * n))
-        info = solver.solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
-        norms = asm.fluid_update_device(dx, False, 1.0, ds, dvo, dacc)
-        rs.synchronize()
-        t = time.perf_counter() - t0
-        if it >= args.warmup:
-            steps.append((t, a_ms, info.solve_ms, info.inner_iterations, info.true_relres, info.iterations,
-                          info.converged))
-        log(f"fluid step {it}: {t * 1e3:.1f} ms (assembly {a_ms:.2f} ms, GMRES {info.solve_ms:.1f} ms, "
-            f"{info.inner_iterations} inner its, relres {info.true_relres:.2e}, converged {info.converged}), "
-            f"|v| {norms[1]:.4f} |p| {norms[3]:.4f}")
+
+    def picard_steps(precision):
+        # the same Picard steps from the same initial state, GMRES in FP64 or mixed precision
+        ds.upload(state.reshape(-1).copy())
+        dvo.upload(state[:, :3].reshape(-1).copy())
+        dacc.upload(np.zeros(3 * nn))
+        sconf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=60, n_inner_iterations=restart,
+                                    solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES, precision=precision)
+        solver = rs.RMatrixSolver(sconf)
+        out = []
+        for it in range(args.warmup + args.steps):
+            rs.synchronize()
+            t0 = time.perf_counter()
+            asm.assemble_fluid_device(conf, drho, dmu, dg, ds, dvo, dacc, db)
+            a_ms = asm.last_ms
+            rs._check(lib.rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * n))
+            info = solver.solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+            norms = asm.fluid_update_device(dx, False, 1.0, ds, dvo, dacc)
+            rs.synchronize()
+            t = time.perf_counter() - t0
+            if it >= args.warmup:
+                out.append((t, a_ms, info.solve_ms, info.inner_iterations, info.true_relres, info.iterations,
+                            info.converged))
+            log(f"fluid step {it} ({'mixed' if precision == rs.PRECISION_MIXED else 'fp64'}): {t * 1e3:.1f} ms "
+                f"(assembly {a_ms:.2f} ms, GMRES {info.solve_ms:.1f} ms, {info.inner_iterations} inner its, relres "
+                f"{info.true_relres:.2e}, converged {info.converged}), |v| {norms[1]:.4f} |p| {norms[3]:.4f}")
+        return out, ds.download()
+
+    steps_mp, state_mp = picard_steps(rs.PRECISION_MIXED)
+    steps, state_64 = picard_steps(rs.PRECISION_FP64)
     ts = np.array([x[0] for x in steps])
+    ts_mp = np.array([x[0] for x in steps_mp])
+    mixed = {"time_s": round(float(np.median(ts_mp)), 4), "gmres_ms": round(float(np.median([x[2] for x in steps_mp])), 2),
+             "inner_iterations": [int(x[3]) for x in steps_mp], "cycles": [int(x[5]) for x in steps_mp],
+             "all_converged": all(x[6] == 1 for x in steps_mp), "true_relres": max(x[4] for x in steps_mp),
+             "state_difference_vs_fp64": float(np.abs(state_mp - state_64).max() / np.abs(state_64).max()),
+             "speedup_vs_fp64": round(float(np.median(ts)) / float(np.median(ts_mp)), 3),
+             "method": ("GMRES inner cycle on an FP32 basis, FP32 SpMV and Jacobi; FP64 Gram-Schmidt products, "
+                        "rotations, restart residuals and stopping test")}
     cpu = None
     if not args.no_cpu_baseline:
         orc = ge.load_oracle()
@@ -966,6 +984,7 @@ def fluid(rs, args) -> int:
            "gmres_restart": restart, "inner_iterations": [int(x[3]) for x in steps],
            "cycles": [int(x[5]) for x in steps], "all_converged": all(x[6] == 1 for x in steps),
            "true_relres": max(x[4] for x in steps),
+           "mixed_precision_gmres": mixed,
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     return 0

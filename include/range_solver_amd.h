@@ -59,7 +59,10 @@ typedef enum rs_solver_type { RS_SOLVER_CG = 0, RS_SOLVER_GMRES = 1 } rs_solver_
 typedef enum rs_stop_rule { RS_STOP_REFERENCE = 0, RS_STOP_RELRES = 1 } rs_stop_rule;
 
 /* FP64 is the drop-in.  MIXED: FP32 matrix values, FP32 SpMV and preconditioner, FP64 dot
- * products, norms, solution and residual replacement (CG only, RELRES rule). */
+ * products, norms and solution.  CG: FP32 vectors with FP64 residual replacement (RELRES
+ * rule).  GMRES: the inner cycle's basis, preconditioned basis and w in FP32, its Gram-Schmidt
+ * products, Hessenberg column and rotations in FP64; each restart's residual and the stopping
+ * test in FP64 with the configured rule. */
 typedef enum rs_precision { RS_PRECISION_FP64 = 0, RS_PRECISION_MIXED = 1 } rs_precision;
 
 typedef enum rs_reason {
@@ -85,7 +88,7 @@ typedef struct rs_conf {
 
 typedef struct rs_info {
     uint32_t iterations;          /* CG iterations / GMRES outer iterations performed */
-    uint32_t inner_iterations;    /* GMRES inner steps in total; MIXED: FP64 residual replacements */
+    uint32_t inner_iterations;    /* GMRES inner steps in total; MIXED CG: FP64 residual replacements */
     int32_t reason;               /* rs_reason */
     int32_t converged;            /* reason is CONVERGED */
     double error;                 /* last RIterationInfo error */

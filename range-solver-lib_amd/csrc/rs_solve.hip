@@ -455,6 +455,16 @@ __global__ __launch_bounds__(kBlock) void k_gm_spmv(CsrView A, const double *__r
     Shape::template run<double>(A, A.val, x, e);
 }
 
+// mixed-precision GMRES: y = A x on the FP32 values and basis, unless the inner cycle has stopped
+template <class Shape>
+__global__ __launch_bounds__(kBlock) void k_gm_spmv32(CsrView A, const float *__restrict__ x, float *__restrict__ y,
+                                                      const uint32_t *stop)
+{
+    if (*stop) return;
+    EpiStore32 e{ y };
+    Shape::template run<float>(A, A.val32, x, e);
+}
+
 template <class Shape>
 __global__ __launch_bounds__(kBlock) void k_spmv32(CsrView A, const float *__restrict__ x, float *__restrict__ y)
 {
@@ -658,6 +668,23 @@ __device__ __forceinline__ void pc_apply_block(int ptype, uint32_t w, const PT *
         for (int k = 0; k < ww; k++) z[k] = (T)pd[ib * ww + k] * r[k];
     } else {
         for (int k = 0; k < ww; k++) z[k] = r[k];
+    }
+}
+
+// mixed-precision GMRES: z_k = P v_k on the FP32 preconditioner data and basis
+template <int W>
+__global__ void k_pc_apply32(int ptype, uint32_t w, const float *__restrict__ pd, uint32_t n,
+                             const float *__restrict__ x, float *__restrict__ y, const uint32_t *stop)
+{
+    if (stop && *stop) return;
+    constexpr int WM = W == 0 ? 16 : W;
+    const uint32_t ww = W == 0 ? w : W;
+    uint64_t nb = n / ww;
+    for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
+        float r[WM], z[WM];
+        for (uint32_t k = 0; k < ww; k++) r[k] = x[ib * ww + k];
+        pc_apply_block<W, float, float>(ptype, ww, pd, ib, r, z);
+        for (uint32_t k = 0; k < ww; k++) y[ib * ww + k] = z[k];
     }
 }
 
@@ -1006,23 +1033,66 @@ __global__ __launch_bounds__(kBlock) void k_mp_rho(MpArgs a)
 }
 
 // ============================================================================ GMRES kernels
+// The GMRES vector kernels take the basis type T: double (FP64), or float for the
+// mixed-precision GMRES (FP32 basis, products and norms accumulated in FP64).  The Gram-Schmidt
+// kernels read the basis in 16-byte loads (2 FP64 or 4 FP32 entries per lane; basis rows start at
+// multiples of 16 bytes, ld % (16 / sizeof(T)) == 0): they are bound by the number of loads in
+// flight, so 8- or 4-byte loads per entry would leave the FP32 basis, half the bytes, barely faster
+// (measured on the duct flow: 134 against 161 us for the products at 4 bytes per entry).
+template <typename T>
+struct Load16 {
+    static constexpr int kN = 16 / sizeof(T);
+    __device__ __forceinline__ static void get(const T *p, T (&e)[kN])
+    {
+        if constexpr (sizeof(T) == 4) {
+            const float4 v = *reinterpret_cast<const float4 *>(p);
+            e[0] = v.x;
+            e[1] = v.y;
+            e[2] = v.z;
+            e[3] = v.w;
+        } else {
+            const double2 v = *reinterpret_cast<const double2 *>(p);
+            e[0] = v.x;
+            e[1] = v.y;
+        }
+    }
+    __device__ __forceinline__ static void put(T *p, const T (&e)[kN])
+    {
+        if constexpr (sizeof(T) == 4) *reinterpret_cast<float4 *>(p) = make_float4(e[0], e[1], e[2], e[3]);
+        else *reinterpret_cast<double2 *>(p) = make_double2(e[0], e[1]);
+    }
+};
+
 // h_i = w . v_i for i in [i0, i0+8) -> partials[i][block]; final sum in k_multidot_final
-__global__ __launch_bounds__(kBlock) void k_multidot(const double *__restrict__ w,
-                                                     const double *__restrict__ V, uint64_t ld,
-                                                     uint32_t n, uint32_t cnt, double *partials,
-                                                     const uint32_t *stop)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_multidot(const T *__restrict__ w, const T *__restrict__ V, uint64_t ld,
+                                                     uint32_t n, uint32_t cnt, double *partials, const uint32_t *stop)
 {
     if (*stop) return;
+    constexpr int N = Load16<T>::kN;
     __shared__ double lds[8 * (kBlock / kWave)];
-    uint32_t i0 = blockIdx.y * 8;
+    const uint32_t i0 = blockIdx.y * 8;
     double v[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = 0.0;
-    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (uint64_t)gridDim.x * kBlock) {
-        double wj = w[j];
+    const uint64_t nq = n / N, stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+        T wq[N];
+        Load16<T>::get(w + N * q, wq);
 #pragma unroll
         for (int k = 0; k < 8; k++)
-            if (i0 + k < cnt) v[k] += wj * V[(i0 + k) * ld + j];
+            if (i0 + k < cnt) {
+                T vq[N];
+                Load16<T>::get(V + (i0 + k) * ld + N * q, vq);
+#pragma unroll
+                for (int e = 0; e < N; e++) v[k] += (double)wq[e] * (double)vq[e];
+            }
+    }
+    for (uint64_t j = N * nq + (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
+        const double wj = (double)w[j];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (i0 + k < cnt) v[k] += wj * (double)V[(i0 + k) * ld + j];
     }
     block_sum<8, kBlock>(v, lds);
     if (threadIdx.x == 0) {
@@ -1043,49 +1113,75 @@ __global__ __launch_bounds__(kBlock) void k_multidot_final(const double *partial
     if (threadIdx.x == 0) out[blockIdx.x] = v[0];
 }
 
-// w -= sum_i h_i v_i ; partial |w|^2  (classical Gram-Schmidt, src/rmatrixsolver.cpp:486-508)
-__global__ __launch_bounds__(kBlock) void k_gs_subtract(double *__restrict__ w,
-                                                        const double *__restrict__ V, uint64_t ld,
-                                                        uint32_t n, uint32_t cnt,
-                                                        const double *__restrict__ h, double *partials,
-                                                        const uint32_t *stop)
+// w -= sum_i h_i v_i ; partial |w|^2  (classical Gram-Schmidt, src/rmatrixsolver.cpp:486-508);
+// the sums in FP64, the stored w in T and its norm from the stored values
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gs_subtract(T *__restrict__ w, const T *__restrict__ V, uint64_t ld,
+                                                        uint32_t n, uint32_t cnt, const double *__restrict__ h,
+                                                        double *partials, const uint32_t *stop)
 {
     if (*stop) return;
+    constexpr int N = Load16<T>::kN;
     double v[1] = { 0.0 };
-    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (uint64_t)gridDim.x * kBlock) {
-        double value = w[j];
-        for (uint32_t i = 0; i < cnt; i++) value -= h[i] * V[i * ld + j];
-        w[j] = value;
-        v[0] += value * value;
+    const uint64_t nq = n / N, stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+        T wq[N];
+        Load16<T>::get(w + N * q, wq);
+        double a[N];
+#pragma unroll
+        for (int e = 0; e < N; e++) a[e] = (double)wq[e];
+        for (uint32_t i = 0; i < cnt; i++) {
+            T vq[N];
+            Load16<T>::get(V + i * ld + N * q, vq);
+            const double hi = h[i];
+#pragma unroll
+            for (int e = 0; e < N; e++) a[e] -= hi * (double)vq[e];
+        }
+#pragma unroll
+        for (int e = 0; e < N; e++) {
+            wq[e] = (T)a[e];
+            v[0] += (double)wq[e] * (double)wq[e];
+        }
+        Load16<T>::put(w + N * q, wq);
+    }
+    for (uint64_t j = N * nq + (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
+        double value = (double)w[j];
+        for (uint32_t i = 0; i < cnt; i++) value -= h[i] * (double)V[i * ld + j];
+        const T stored = (T)value;
+        w[j] = stored;
+        v[0] += (double)stored * (double)stored;
     }
     store_partials<1>(v, partials);
 }
 
 // v = (h == 0) ? 0 : w / h with h = st->t0
-__global__ void k_normalize(const double *__restrict__ w, double *__restrict__ v, uint64_t n,
-                            const SolveState *st, const uint32_t *stop)
+template <typename T>
+__global__ void k_normalize(const T *__restrict__ w, T *__restrict__ v, uint64_t n, const SolveState *st,
+                            const uint32_t *stop)
 {
     if (*stop) return;
     double h = st->t0;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
-        v[i] = (h == 0.0) ? 0.0 : w[i] / h;
+        v[i] = (h == 0.0) ? (T)0 : (T)((double)w[i] / h);
 }
 
 // v0 = ro / beta (0 if beta == 0)
-__global__ void k_scale_by(const double *__restrict__ a, double *__restrict__ out, uint64_t n, double beta)
+template <typename T>
+__global__ void k_scale_by(const double *__restrict__ a, T *__restrict__ out, uint64_t n, double beta)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
-        out[i] = (beta == 0.0) ? 0.0 : a[i] / beta;
+        out[i] = (beta == 0.0) ? (T)0 : (T)(a[i] / beta);
 }
 
 // x += sum_{j<cnt} Z_j y_j   (src/rmatrixsolver.cpp:571-578), cnt = the inner cycle's length
-__global__ void k_update_x(double *__restrict__ x, const double *__restrict__ Z, uint64_t ld,
+template <typename T>
+__global__ void k_update_x(double *__restrict__ x, const T *__restrict__ Z, uint64_t ld,
                            uint32_t n, const uint32_t *__restrict__ cnt_dev, const double *__restrict__ y)
 {
     const uint32_t cnt = *cnt_dev;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         double xi = x[i];
-        for (uint32_t j = 0; j < cnt; j++) xi += Z[j * ld + i] * y[j];
+        for (uint32_t j = 0; j < cnt; j++) xi += (double)Z[j * ld + i] * y[j];
         x[i] = xi;
     }
 }
@@ -1867,6 +1963,7 @@ struct rs_solver {
     DBuf<double> y, r, p, q, ro, w, V, Z, hbuf, gm;
     DBuf<uint32_t> gmflag;      // GMRES inner cycle: stop, length
     DBuf<float> r32, p32, q32, dx32, z32;
+    DBuf<float> V32, Z32, w32;  // mixed-precision GMRES: FP32 basis, preconditioned basis, w
     DBuf<double> partials, hist;
     DBuf<SolveState> st;
     // captured iteration graph (or, with RS_NO_GRAPH=1 in the environment, the same S
@@ -1902,14 +1999,15 @@ static int ensure_ws(rs_solver &S, uint32_t n, bool mixed, bool gmres)
     if (!mixed && !gmres) {
         if ((rc = S.r.alloc(N)) || (rc = S.p.alloc(N)) || (rc = S.q.alloc(N))) return rc;
     }
-    if (mixed) {
+    if (mixed && !gmres) {
         if ((rc = S.r32.alloc(N)) || (rc = S.p32.alloc(N)) || (rc = S.q32.alloc(N)) || (rc = S.dx32.alloc(N))) return rc;
     }
     if (gmres) {
+        // the mixed-precision GMRES keeps its basis in FP32 buffers of its own (solve_gmres)
         uint64_t k = S.conf.n_inner_iterations;
-        if ((rc = S.ro.alloc(N)) || (rc = S.w.alloc(N)) || (rc = S.V.alloc((k + 1) * N)) ||
-            (rc = S.Z.alloc((k + 1) * N)) || (rc = S.hbuf.alloc(2 * (k + 2))))
-            return rc;
+        if ((rc = S.ro.alloc(N)) || (rc = S.hbuf.alloc(2 * (k + 2)))) return rc;
+        const uint64_t ld = (N + 1) / 2 * 2;   // basis rows at multiples of 16 bytes
+        if (!mixed && ((rc = S.w.alloc(N)) || (rc = S.V.alloc((k + 1) * ld)) || (rc = S.Z.alloc((k + 1) * ld)))) return rc;
     }
     return RS_OK;
 }
@@ -2186,8 +2284,14 @@ static int solve_cg_mixed(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x
     return RS_OK;
 }
 
+// Restarted GMRES (src/rmatrixsolver.cpp:357-581).  mixed: the inner cycle keeps its basis V,
+// the preconditioned basis Z and w in FP32 and runs the SpMV and the preconditioner on FP32
+// values (the Gram-Schmidt products, norms, Hessenberg column, rotations and back substitution
+// stay FP64); each restart's residual b - A x and the solution stay FP64, so the outer stopping
+// test is the FP64 one.  The basis reads of the Gram-Schmidt steps, which bound an inner
+// iteration, are halved.
 static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, uint32_t &outer,
-                       uint32_t &inner_total)
+                       uint32_t &inner_total, bool mixed)
 {
     hipStream_t stream = runtime().stream;
     const uint32_t n = A.n;
@@ -2199,9 +2303,23 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
     if ((rc = matrix_sync_plan(A, plan, false))) return rc;
     uint32_t nb = 0;
     CsrView v = plan_view(A, plan, nb);
+    int plan32 = -1;
+    uint32_t nb32 = 0;
+    CsrView v32{};
+    const uint64_t ld32 = (n + 3ull) / 4 * 4;   // FP32 basis rows start at multiples of 4 entries (16 bytes)
+    if (mixed) {
+        const size_t N = std::max(n, 1u);
+        if ((rc = matrix_refresh_fp32(A)) || (rc = matrix_tune(A, true)) || (rc = precond_fp32(P))) return rc;
+        plan32 = A.plan[1];
+        if ((rc = matrix_sync_plan(A, plan32, true))) return rc;
+        v32 = plan_view(A, plan32, nb32);
+        if ((rc = S.V32.alloc((ninner + 1ull) * ld32)) || (rc = S.Z32.alloc((ninner + 1ull) * ld32)) ||
+            (rc = S.w32.alloc(N)))
+            return rc;
+    }
     uint32_t nvb = vec_blocks(n);
     uint32_t w = P.width ? P.width : 1;
-    const uint64_t ld = n;
+    const uint64_t ld = (n + 1ull) / 2 * 2;     // FP64 basis rows start at multiples of 2 entries (16 bytes)
     SolveState *dst = S.st.p;
     SolveState h;
     // the inner cycle runs on the device (Gram-Schmidt, Hessenberg column, Givens rotation,
@@ -2244,7 +2362,8 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
         bool conv = !std::isnormal(S.error) || !std::isnormal(S.trend) || S.error < S.conf.solver_cvg_value ||
                     std::fabs(S.trend) < kEps * kEps;
         if (conv) break;
-        k_scale_by<<<nvb, kBlock, 0, stream>>>(S.ro.p, S.V.p, n, beta);
+        if (mixed) k_scale_by<float><<<nvb, kBlock, 0, stream>>>(S.ro.p, S.V32.p, n, beta);   // row 0
+        else k_scale_by<double><<<nvb, kBlock, 0, stream>>>(S.ro.p, S.V.p, n, beta);
         // pp = (beta, 0, ...), beta; stop = 0
         init[0] = beta;
         init[ninner + 1] = beta;
@@ -2254,16 +2373,31 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
         uint32_t hflag[2] = { 0, 0 };
         for (uint32_t iti = 0; iti < ninner;) {
             for (const uint32_t end = std::min(ninner, iti + kGmBatch); iti < end; iti++) {
-                double *vi = S.V.p + (uint64_t)iti * ld, *zi = S.Z.p + (uint64_t)iti * ld;
-                RS_DISPATCH_W(w, k_pc_apply, vec_blocks(n / w), P.type, w, P.data.p, n, vi, zi, stop);
-                RS_SHAPE_SWITCH(plan, k_gm_spmv, nb, stream, v, zi, S.w.p, stop);
-                uint32_t cnt = iti + 1;
-                dim3 g(nbm, (cnt + 7) / 8);
-                k_multidot<<<g, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, mpart.p, stop);
-                k_multidot_final<<<cnt, kBlock, 0, stream>>>(mpart.p, nbm, S.hbuf.p, stop);
-                k_gs_subtract<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, S.hbuf.p, S.partials.p, stop);
-                fin(S.partials.p, nvb, 1, FIN_NORM0, 0, dst, nullptr);
-                k_normalize<<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p + (uint64_t)(iti + 1) * ld, n, dst, stop);
+                const uint32_t cnt = iti + 1;
+                const dim3 g(nbm, (cnt + 7) / 8);
+                if (mixed) {
+                    float *vi = S.V32.p + (uint64_t)iti * ld32, *zi = S.Z32.p + (uint64_t)iti * ld32;
+                    RS_DISPATCH_W(w, k_pc_apply32, vec_blocks(n / w), P.type, w, P.data32.p, n, vi, zi, stop);
+                    RS_SHAPE_SWITCH(plan32, k_gm_spmv32, nb32, stream, v32, zi, S.w32.p, stop);
+                    k_multidot<float><<<g, kBlock, 0, stream>>>(S.w32.p, S.V32.p, ld32, n, cnt, mpart.p, stop);
+                    k_multidot_final<<<cnt, kBlock, 0, stream>>>(mpart.p, nbm, S.hbuf.p, stop);
+                    k_gs_subtract<float><<<nvb, kBlock, 0, stream>>>(S.w32.p, S.V32.p, ld32, n, cnt, S.hbuf.p,
+                                                                      S.partials.p, stop);
+                    fin(S.partials.p, nvb, 1, FIN_NORM0, 0, dst, nullptr);
+                    k_normalize<float><<<nvb, kBlock, 0, stream>>>(S.w32.p, S.V32.p + (uint64_t)(iti + 1) * ld32, n,
+                                                                    dst, stop);
+                } else {
+                    double *vi = S.V.p + (uint64_t)iti * ld, *zi = S.Z.p + (uint64_t)iti * ld;
+                    RS_DISPATCH_W(w, k_pc_apply, vec_blocks(n / w), P.type, w, P.data.p, n, vi, zi, stop);
+                    RS_SHAPE_SWITCH(plan, k_gm_spmv, nb, stream, v, zi, S.w.p, stop);
+                    k_multidot<double><<<g, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, mpart.p, stop);
+                    k_multidot_final<<<cnt, kBlock, 0, stream>>>(mpart.p, nbm, S.hbuf.p, stop);
+                    k_gs_subtract<double><<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p, ld, n, cnt, S.hbuf.p,
+                                                                       S.partials.p, stop);
+                    fin(S.partials.p, nvb, 1, FIN_NORM0, 0, dst, nullptr);
+                    k_normalize<double><<<nvb, kBlock, 0, stream>>>(S.w.p, S.V.p + (uint64_t)(iti + 1) * ld, n, dst,
+                                                                     stop);
+                }
                 k_gm_givens<<<1, kGmBlock, 0, stream>>>(S.gm.p, ninner, S.gmflag.p, S.hbuf.p, dst, iti);
             }
             RS_HIP(hipGetLastError());
@@ -2273,7 +2407,8 @@ static int solve_gmres(rs_solver &S, rs_matrix &A, rs_precond &P, double *d_x, u
         }
         inner_total += hflag[1];
         k_gm_backsub<<<1, kGmBlock, 0, stream>>>(S.gm.p, ninner, S.gmflag.p);
-        k_update_x<<<nvb, kBlock, 0, stream>>>(d_x, S.Z.p, ld, n, S.gmflag.p + 1, gv.y);
+        if (mixed) k_update_x<float><<<nvb, kBlock, 0, stream>>>(d_x, S.Z32.p, ld32, n, S.gmflag.p + 1, gv.y);
+        else k_update_x<double><<<nvb, kBlock, 0, stream>>>(d_x, S.Z.p, ld, n, S.gmflag.p + 1, gv.y);
         RS_HIP(hipGetLastError());
         RS_HIP(hipStreamSynchronize(stream));
     }
@@ -2301,7 +2436,6 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
     // vectors and product in FP32 (FP64 residual replacement), PRECISION_FP64 keeps them in FP64
     bool mixed = c.precision == RS_PRECISION_MIXED;
     bool gmres = c.solver_type == RS_SOLVER_GMRES;
-    if (mixed && gmres) return fail(RS_ERR_UNSUPPORTED, "mixed precision is implemented for CG only");
     int rc;
     // tune outside the timed region (once per matrix)
     if (A.n && (rc = matrix_tune(A, false))) return rc;
@@ -2330,7 +2464,7 @@ static int solve_device_ext(rs_solver &S, rs_matrix &A, const double *d_b, doubl
     S.history.assign(2ull * std::min(hist_cap, std::max(1u, c.n_outer_iterations)), 0.0);
     if (n > 0) {
         if (gmres) {
-            rc = solve_gmres(S, A, P, d_x, iters, inner);
+            rc = solve_gmres(S, A, P, d_x, iters, inner, mixed);
         } else if (learned) {
             rc = mixed ? solve_fcg_learned(S, A, P, *L, d_x, done, refinements)
                        : solve_fcg64_learned(S, A, *L, d_x, done);

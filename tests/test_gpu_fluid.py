@@ -113,3 +113,38 @@ def test_fluid_channel_picard(gpu, orc):
     assert res[-1] < 1e-4 * res[0]
     sg = ds.download().reshape(nn, 4)
     assert np.abs(sg - st).max() <= 1e-8 * np.abs(st).max()
+
+
+def test_fluid_duct_mixed_gmres(gpu, orc):
+    """One Picard system of the duct flow (16 x 4 x 4 cells, mu 0.02) solved by restarted GMRES +
+    Jacobi to relres 1e-8 in FP64 and in mixed precision (FP32 basis, SpMV and preconditioner):
+    both within 1e-6 of the oracle's direct solution, the mixed one within 10% of the FP64
+    one's inner iterations."""
+    rs = gpu
+    xyz, tets, fixed, inflow = channel(rs, 16, 4, 4, lx=4.0)
+    nn, ne = xyz.shape[0], tets.shape[0]
+    y, z = xyz[:, 1], xyz[:, 2]
+    state = np.zeros((nn, 4))
+    state[inflow, 0] = 16 * y[inflow] * (1 - y[inflow]) * z[inflow] * (1 - z[inflow])
+    rho, mu, grav = np.ones(ne), np.full(ne, 0.02), np.zeros((ne, 3))
+    asm = rs.Assembly(xyz, tets, 4, fixed)
+    conf = rs.FluidConf(1.0, 1.0, 1.0, 0)
+    dev = [rs.DeviceArray.from_host(np.ascontiguousarray(a).reshape(-1)) for a in
+           (rho, mu, grav, state, state[:, :3].copy(), np.zeros((nn, 3)))]
+    db = rs.DeviceArray(asm.n_dof)
+    asm.assemble_fluid_device(conf, *dev, db)
+    A_ref, b_ref = orc.fluid_assemble(xyz, tets, rho, mu, grav, state, state[:, :3].copy(), np.zeros((nn, 3)), 1.0,
+                                      1.0, 1.0, 0, asm.book(), asm.n_dof)
+    x_star = spla.spsolve(A_ref.tocsc(), b_ref)
+    its, xs = {}, {}
+    for name, prec in (("fp64", rs.PRECISION_FP64), ("mixed", rs.PRECISION_MIXED)):
+        sconf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=100, n_inner_iterations=100,
+                                    solver_cvg_value=1e-8, stop_rule=rs.STOP_RELRES, precision=prec)
+        dx = rs.DeviceArray.from_host(np.zeros(asm.n_dof))
+        info = rs.RMatrixSolver(sconf).solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+        assert info.converged == 1, info
+        assert info.true_relres < 1e-8
+        xs[name], its[name] = dx.download(), info.inner_iterations
+        assert np.linalg.norm(xs[name] - x_star) / np.linalg.norm(x_star) <= 1e-6
+    print(f"duct 16x4x4 GMRES inner iterations: fp64 {its['fp64']}, mixed {its['mixed']}", flush=True)
+    assert its["mixed"] <= 1.1 * its["fp64"] + 2

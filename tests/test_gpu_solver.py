@@ -227,6 +227,31 @@ def test_gmres_matches_oracle(gpu, orc, ptype):
     assert S.info.true_relres < 1e-6
 
 
+@pytest.mark.parametrize("ptype", [0, 1, 2])
+def test_gmres_mixed_against_oracle(gpu, orc, ptype):
+    """Mixed-precision GMRES (FP32 basis, SpMV and preconditioner; FP64 Gram-Schmidt products,
+    rotations and restart residuals) reaches the FP64 tolerance and the oracle's FP64 solution,
+    in at most two restarts more than the oracle's GMRES."""
+    _say(f"mixed gmres precond={ptype}")
+    rs = gpu
+    rng = np.random.default_rng(24)
+    n = 1500
+    A = (sp.random(n, n, density=0.004, random_state=rng) + sp.eye(n) * 3.0).tocsr()
+    A.sort_indices()
+    b = rng.standard_normal(n)
+    M = rs.SparseMatrix.from_scipy(A)
+    conf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=100, n_inner_iterations=20,
+                               solver_cvg_value=1e-10, stop_rule=rs.STOP_RELRES, precision=rs.PRECISION_MIXED)
+    S = rs.RMatrixSolver(conf)
+    x = S.solve(M, b, None, ptype, 3)
+    x_ref, info_ref = orc.solve(A, b, precond=ptype, block_size=3, solver=orc.SOLVER_GMRES,
+                                n_outer=100, n_inner=20, cvg=1e-10, stop_rule=orc.STOP_RELRES)
+    assert S.info.converged == 1
+    assert S.info.true_relres < 1e-10
+    assert S.info.iterations <= info_ref["iterations"] + 2
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-8
+
+
 @pytest.mark.parametrize("n", [300, 2100])
 def test_gmres_full_cycle_on_the_cyclic_shift(gpu, n):
     """GMRES(n) on the cyclic shift (A e_i = e_(i+1 mod n)) with b = e_0: the Krylov space is
