@@ -245,11 +245,20 @@ struct CycArgs {
 __device__ __forceinline__ bool cyc_idle(const SolveState *st) { return st && st->done; }
 
 // First Chebyshev step from a zero iterate: d = c2 D^-1 r, z = d.
+// (4 entries per lane in 16-byte loads and stores, as the flexible CG's vector kernels: with
+// 4-byte accesses these FP32 passes are bound by the number of loads, not by their bytes)
 __global__ void k_cyc_first(CycArgs a)
 {
     if (cyc_idle(a.st)) return;
     const float c2 = a.coef[1];
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t nq = a.n / 4, stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+        const float4 dv = reinterpret_cast<const float4 *>(a.dinv)[q], rv = reinterpret_cast<const float4 *>(a.r)[q];
+        const float4 z = make_float4(c2 * dv.x * rv.x, c2 * dv.y * rv.y, c2 * dv.z * rv.z, c2 * dv.w * rv.w);
+        reinterpret_cast<float4 *>(a.zout)[q] = z;
+        reinterpret_cast<float4 *>(a.d)[q] = z;
+    }
+    for (uint64_t i = 4 * nq + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
         float z = c2 * a.dinv[i] * a.r[i];
         a.zout[i] = z;
         a.d[i] = z;
@@ -1516,12 +1525,23 @@ struct FcgArgs {
 };
 
 // p = z (first iteration of a segment) or z + beta p
+// (both in 16-byte loads and stores, 4 entries per lane, then the remaining n % 4 one per lane)
 __global__ void k_fcg_p(FcgArgs a)
 {
     if (a.st->done) return;
     const bool first = a.st->it == a.st->it0;
     const float beta = (float)a.st->beta;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * kBlock)
+    const uint64_t nq = a.n / 4, stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+        const float4 z = reinterpret_cast<const float4 *>(a.z)[q];
+        float4 o = z;
+        if (!first) {
+            const float4 p = reinterpret_cast<const float4 *>(a.p)[q];
+            o = make_float4(fmaf(beta, p.x, z.x), fmaf(beta, p.y, z.y), fmaf(beta, p.z, z.z), fmaf(beta, p.w, z.w));
+        }
+        reinterpret_cast<float4 *>(a.p)[q] = o;
+    }
+    for (uint64_t i = 4 * nq + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride)
         a.p[i] = first ? a.z[i] : fmaf(beta, a.p[i], a.z[i]);
 }
 
@@ -1531,7 +1551,21 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(FcgArgs a)
     if (a.st->done) return;
     const float alpha = (float)a.st->alpha;
     double v[1] = { 0.0 };
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t nq = a.n / 4, stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+        const float4 p = reinterpret_cast<const float4 *>(a.p)[q], dx = reinterpret_cast<const float4 *>(a.dx)[q];
+        const float4 qq = reinterpret_cast<const float4 *>(a.q)[q], r = reinterpret_cast<const float4 *>(a.r)[q];
+        reinterpret_cast<float4 *>(a.dx)[q] =
+            make_float4(fmaf(alpha, p.x, dx.x), fmaf(alpha, p.y, dx.y), fmaf(alpha, p.z, dx.z), fmaf(alpha, p.w, dx.w));
+        const float4 rn = make_float4(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z),
+                                      fmaf(-alpha, qq.w, r.w));
+        reinterpret_cast<float4 *>(a.r)[q] = rn;
+        v[0] += (double)rn.x * (double)rn.x;
+        v[0] += (double)rn.y * (double)rn.y;
+        v[0] += (double)rn.z * (double)rn.z;
+        v[0] += (double)rn.w * (double)rn.w;
+    }
+    for (uint64_t i = 4 * nq + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
         a.dx[i] = fmaf(alpha, a.p[i], a.dx[i]);
         float ri = fmaf(-alpha, a.q[i], a.r[i]);
         a.r[i] = ri;
