@@ -921,10 +921,81 @@ struct MpArgs {
     double *hist;
 };
 
+// Jacobi or no preconditioner (width 1): 4 rows per lane in 16-byte loads and stores, then the
+// remaining n % 4 one per lane; the same expressions as pc_apply_block (z = d r).
+__device__ __forceinline__ float4 mp_pc4(const MpArgs &a, uint64_t q, const float4 &r)
+{
+    if (a.ptype != RS_PRECOND_JACOBI) return r;
+    const float4 d = reinterpret_cast<const float4 *>(a.pd)[q];
+    return make_float4(d.x * r.x, d.y * r.y, d.z * r.z, d.w * r.w);
+}
+
+__device__ __forceinline__ float mp_pc1(const MpArgs &a, uint64_t i, float r)
+{
+    return a.ptype == RS_PRECOND_JACOBI ? a.pd[i] * r : r;
+}
+
+__device__ __forceinline__ void mp_dir_jacobi4(const MpArgs &a)
+{
+    const uint64_t n = a.A.n, nq = n / 4, stride = (uint64_t)gridDim.x * kBlock;
+    const bool first = a.st->it == a.st->it0;
+    const float beta = (float)a.st->beta;
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+        const float4 z = mp_pc4(a, q, reinterpret_cast<const float4 *>(a.r)[q]);
+        float4 o = z;
+        if (!first) {
+            const float4 p = reinterpret_cast<const float4 *>(a.p)[q];
+            o = make_float4(fmaf(p.x, beta, z.x), fmaf(p.y, beta, z.y), fmaf(p.z, beta, z.z), fmaf(p.w, beta, z.w));
+        }
+        reinterpret_cast<float4 *>(a.p)[q] = o;
+    }
+    for (uint64_t i = 4 * nq + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const float z = mp_pc1(a, i, a.r[i]);
+        a.p[i] = first ? z : fmaf(a.p[i], beta, z);
+    }
+}
+
+__device__ __forceinline__ void mp_update_jacobi4(const MpArgs &a, double (&v)[2])
+{
+    const uint64_t n = a.A.n, nq = n / 4, stride = (uint64_t)gridDim.x * kBlock;
+    const float alpha = (float)a.st->alpha;
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+        const float4 p = reinterpret_cast<const float4 *>(a.p)[q], dx = reinterpret_cast<const float4 *>(a.dx)[q];
+        const float4 qq = reinterpret_cast<const float4 *>(a.q)[q], r = reinterpret_cast<const float4 *>(a.r)[q];
+        reinterpret_cast<float4 *>(a.dx)[q] =
+            make_float4(fmaf(alpha, p.x, dx.x), fmaf(alpha, p.y, dx.y), fmaf(alpha, p.z, dx.z), fmaf(alpha, p.w, dx.w));
+        const float4 rn = make_float4(fmaf(-alpha, qq.x, r.x), fmaf(-alpha, qq.y, r.y), fmaf(-alpha, qq.z, r.z),
+                                      fmaf(-alpha, qq.w, r.w));
+        reinterpret_cast<float4 *>(a.r)[q] = rn;
+        const float4 z = mp_pc4(a, q, rn);
+        v[0] += (double)rn.x * (double)rn.x;
+        v[0] += (double)rn.y * (double)rn.y;
+        v[0] += (double)rn.z * (double)rn.z;
+        v[0] += (double)rn.w * (double)rn.w;
+        v[1] += (double)rn.x * (double)z.x;
+        v[1] += (double)rn.y * (double)z.y;
+        v[1] += (double)rn.z * (double)z.z;
+        v[1] += (double)rn.w * (double)z.w;
+    }
+    for (uint64_t i = 4 * nq + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        a.dx[i] = fmaf(alpha, a.p[i], a.dx[i]);
+        const float ri = fmaf(-alpha, a.q[i], a.r[i]);
+        a.r[i] = ri;
+        v[0] += (double)ri * (double)ri;
+        v[1] += (double)ri * (double)mp_pc1(a, i, ri);
+    }
+}
+
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_mp_dir(MpArgs a)
 {
     if (a.st->done) return;
+    if constexpr (W == 1) {
+        if (a.ptype != RS_PRECOND_BLOCK_JACOBI) {
+            mp_dir_jacobi4(a);
+            return;
+        }
+    }
     constexpr int WM = W == 0 ? 16 : W;
     const uint32_t ww = W == 0 ? a.pw : W;
     const uint64_t nb = a.A.n / ww;
@@ -960,6 +1031,13 @@ __global__ __launch_bounds__(kBlock) void k_mp_update(MpArgs a)
     const uint64_t nb = a.A.n / ww;
     const float alpha = (float)st->alpha;
     double v[2] = { 0.0, 0.0 };
+    if constexpr (W == 1) {
+        if (a.ptype != RS_PRECOND_BLOCK_JACOBI) {
+            mp_update_jacobi4(a, v);
+            finalize_in_launch<2, FIN_MP_UPDATE>(v, a.partials, a.arrive, st, a.hist);
+            return;
+        }
+    }
     for (uint64_t ib = (uint64_t)blockIdx.x * kBlock + threadIdx.x; ib < nb; ib += (uint64_t)gridDim.x * kBlock) {
         float r[WM], z[WM];
         for (uint32_t k = 0; k < ww; k++) {
