@@ -1208,7 +1208,20 @@ __global__ __launch_bounds__(kBlock) void k_gs_subtract(T *__restrict__ w, const
         double a[N];
 #pragma unroll
         for (int e = 0; e < N; e++) a[e] = (double)wq[e];
-        for (uint32_t i = 0; i < cnt; i++) {
+        // eight basis rows' loads in flight, subtracted in ascending order
+        uint32_t i = 0;
+        for (; i + 8 <= cnt; i += 8) {
+            T vq[8][N];
+#pragma unroll
+            for (int k = 0; k < 8; k++) Load16<T>::get(V + (i + k) * ld + N * q, vq[k]);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const double hi = h[i + k];
+#pragma unroll
+                for (int e = 0; e < N; e++) a[e] -= hi * (double)vq[k][e];
+            }
+        }
+        for (; i < cnt; i++) {
             T vq[N];
             Load16<T>::get(V + i * ld + N * q, vq);
             const double hi = h[i];
