@@ -1024,28 +1024,42 @@ def transport(rs, args) -> int:
                                                                             np.zeros(nn)))
     dfield = rs.DeviceArray.from_host(field.copy())
     db, dx, dz = rs.DeviceArray(n), rs.DeviceArray(n), rs.DeviceArray.from_host(np.zeros(n))
-    sconf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=100, n_inner_iterations=100,
-                                solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES)
-    solver = rs.RMatrixSolver(sconf)
     lib = rs.load()
-    steps = []
-    for it in range(args.warmup + args.steps):
-        rs.synchronize()
-        t0 = time.perf_counter()
-        asm.assemble_transport_device(conf, dca, dk, dv, dq, dfield, db)
-        a_ms = asm.last_ms
-        rs._check(lib.rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * n))       # prepare clears x every step
-        info = solver.solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
-        norms = asm.transport_update_device(dx, dfield)
-        rs.synchronize()
-        t = time.perf_counter() - t0
-        if it >= args.warmup:
-            steps.append((t, a_ms, info.solve_ms, info.inner_iterations, info.true_relres, info.iterations,
-                          info.converged))
-        log(f"transport step {it}: {t * 1e3:.1f} ms (assembly {a_ms:.2f} ms, GMRES {info.solve_ms:.1f} ms, "
-            f"{info.inner_iterations} inner its, relres {info.true_relres:.2e}, converged {info.converged}), "
-            f"|T| {norms[1]:.3f}")
+
+    def time_steps(precision):
+        # the same time steps from the same initial field, GMRES in FP64 or mixed precision
+        dfield.upload(field.copy())
+        sconf = rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES, n_outer_iterations=100, n_inner_iterations=100,
+                                    solver_cvg_value=args.tol, stop_rule=rs.STOP_RELRES, precision=precision)
+        solver = rs.RMatrixSolver(sconf)
+        out = []
+        for it in range(args.warmup + args.steps):
+            rs.synchronize()
+            t0 = time.perf_counter()
+            asm.assemble_transport_device(conf, dca, dk, dv, dq, dfield, db)
+            a_ms = asm.last_ms
+            rs._check(lib.rs_memcpy_dtod(dx.ptr, dz.ptr, 8 * n))       # prepare clears x every step
+            info = solver.solve_device(asm.matrix, db, dx, rs.PRECOND_JACOBI, 1)
+            norms = asm.transport_update_device(dx, dfield)
+            rs.synchronize()
+            t = time.perf_counter() - t0
+            if it >= args.warmup:
+                out.append((t, a_ms, info.solve_ms, info.inner_iterations, info.true_relres, info.iterations,
+                            info.converged))
+            log(f"transport step {it} ({'mixed' if precision == rs.PRECISION_MIXED else 'fp64'}): {t * 1e3:.1f} ms "
+                f"(assembly {a_ms:.2f} ms, GMRES {info.solve_ms:.1f} ms, {info.inner_iterations} inner its, relres "
+                f"{info.true_relres:.2e}, converged {info.converged}), |T| {norms[1]:.3f}")
+        return out, dfield.download()
+
+    steps_mp, field_mp = time_steps(rs.PRECISION_MIXED)
+    steps, field_64 = time_steps(rs.PRECISION_FP64)
     ts = np.array([x[0] for x in steps])
+    ts_mp = np.array([x[0] for x in steps_mp])
+    mixed = {"time_s": round(float(np.median(ts_mp)), 5), "gmres_ms": round(float(np.median([x[2] for x in steps_mp])), 3),
+             "inner_iterations": [int(x[3]) for x in steps_mp], "all_converged": all(x[6] == 1 for x in steps_mp),
+             "true_relres": max(x[4] for x in steps_mp),
+             "field_difference_vs_fp64": float(np.abs(field_mp - field_64).max() / max(np.abs(field_64).max(), 1e-300)),
+             "speedup_vs_fp64": round(float(np.median(ts)) / float(np.median(ts_mp)), 3)}
     cpu = None
     if not args.no_cpu_baseline:
         orc = ge.load_oracle()
@@ -1086,7 +1100,7 @@ def transport(rs, args) -> int:
            "assembly_ms": round(float(np.median([x[1] for x in steps])), 3),
            "gmres_ms": round(float(np.median([x[2] for x in steps])), 3),
            "inner_iterations": [int(x[3]) for x in steps], "all_converged": all(x[6] == 1 for x in steps),
-           "true_relres": max(x[4] for x in steps), "cpu_baseline": cpu}
+           "true_relres": max(x[4] for x in steps), "mixed_precision_gmres": mixed, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     return 0
 
