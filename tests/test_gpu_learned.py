@@ -168,6 +168,21 @@ def test_hierarchy_on_gpu_invalid_arguments(gpu):
         rs.LearnedPreconditioner.from_bins(asm.matrix, other, (4, 4, 4), max_coarse=64)
 
 
+def test_rigid_hierarchy_invalid_arguments(gpu):
+    """RS_AGG_RIGID needs 3 DOFs per node, and refuses aggregates of more than 256 rows."""
+    rs = gpu
+    asm, _, _ = _heat(rs, 6, 0.0)
+    with pytest.raises(rs.RSError):   # 1 DOF per node
+        rs.LearnedPreconditioner.from_bins(asm.matrix, asm, (3, 3, 3), max_coarse=64, rigid=True)
+    sasm, _, _ = _stress_assembly(rs, 16, 4, 4)
+    with pytest.raises(rs.RSError):   # bins of 8 x 5 x 5 nodes: 600 rows per aggregate
+        rs.LearnedPreconditioner.from_bins(sasm.matrix, sasm, (2, 1, 1), max_coarse=512, rigid=True)
+    L = rs.LearnedPreconditioner.from_bins(sasm.matrix, sasm, (8, 2, 2), max_coarse=512, rigid=True)
+    assert L.level_rows[1] % 6 == 0 and L.level_rows[-1] <= 512
+    with pytest.raises(rs.RSError):   # not a rigid-body level: no basis rows
+        rs.LearnedPreconditioner.from_bins(sasm.matrix, sasm, (8, 2, 2), max_coarse=512).level_basis(0)
+
+
 def test_single_level_coarse_operator(gpu):
     rs = gpu
     asm, _, agg, nagg = _heat(rs, 12, 1.0, (3, 3, 4))
