@@ -390,7 +390,6 @@ int rs_stress_process_device(rs_assembly *S, const double *d_E, const double *d_
                              double *d_normal, double *d_shear, double *d_von_mises, double *d_node_force);
 int rs_assembly_destroy(rs_assembly *S);
 
-/* ---------------------------------------------------------------- learned preconditioner */
 /* ---------------------------------------------------------------- eigenvalues
  * REigenValueSolver (include/reigenvaluesolver.h:9-63, src/reigenvaluesolver.cpp:41-592): the
  * generalized problem K x = lambda M x through K^-1 M (every inner solve is CG with Jacobi on the
@@ -418,6 +417,16 @@ typedef struct rs_eigen_info {
 int rs_eigen_solve(const rs_eigen_conf *econf, const rs_conf *mconf, rs_matrix *M, rs_matrix *K, double *d,
                    uint32_t d_capacity, uint32_t *n_d, double *ev, uint64_t ev_capacity, uint32_t *n_ev_rows,
                    rs_eigen_info *info);
+/* rs_eigen_solve with every inner solve with K (all of Lanczos's and Arnoldi's, Rayleigh's
+ * K d = f) run as rs_solver_solve_learned_device with the multilevel preconditioner L, which
+ * must have been created for K (mconf: CG; its precision and stopping rule as for that call).
+ * Rayleigh's solves with M + mu K and with M stay Jacobi-CG.  No counterpart in the reference,
+ * whose inner solves are all Jacobi-CG (src/reigenvaluesolver.cpp:166,250,357,365,392): the
+ * eigenvalues agree with rs_eigen_solve's to the inner solves' tolerance. */
+typedef struct rs_learned rs_learned;
+int rs_eigen_solve_learned(const rs_eigen_conf *econf, const rs_conf *mconf, rs_matrix *M, rs_matrix *K,
+                           rs_learned *L, double *d, uint32_t d_capacity, uint32_t *n_d, double *ev,
+                           uint64_t ev_capacity, uint32_t *n_ev_rows, rs_eigen_info *info);
 
 /* ---------------------------------------------------------------- batched systems
  * Many independent systems solved together (no counterpart in the reference, which solves
@@ -464,12 +473,12 @@ int rs_batch_info_layout(const rs_batch *B, uint32_t *padded_rows, uint64_t *nnz
 int rs_batch_set_replacement(rs_batch *B, double ratio);
 int rs_batch_destroy(rs_batch *B);
 
-/* No counterpart in the reference (a new preconditioner type, RS_PRECOND_LEARNED): one V-cycle
- * over an aggregation hierarchy (piecewise-constant prolongations, Galerkin operators
- * P^T A P, Chebyshev smoothing in FP32) whose coarsest level (<= 512 nodes) is solved by a
- * 6-layer, 64-wide message-passing network, applied inside flexible CG in mixed precision
- * with Jacobi-CG as the fallback. */
-typedef struct rs_learned rs_learned;
+/* ---------------------------------------------------------------- learned preconditioner
+ * No counterpart in the reference (a new preconditioner type, RS_PRECOND_LEARNED): one V-cycle
+ * over an aggregation hierarchy (piecewise-constant or rigid-body-mode prolongations, Galerkin
+ * operators P^T A P, Chebyshev smoothing in FP32) whose coarsest level (<= 512 rows) is solved
+ * by a 6-layer, 64-wide message-passing network, coarse Jacobi or its exact inverse, applied
+ * inside flexible CG (Krylov vectors in FP32 or FP64) with Jacobi-CG as the fallback. */
 enum {
     RS_COARSE_NETWORK = 0,   /* the learned network */
     RS_COARSE_JACOBI = 1,    /* coarse Jacobi: what the untrained network computes */

@@ -212,6 +212,8 @@ _SIGS = {
     "rs_batch_destroy": [_P],
     "rs_eigen_solve": [C.POINTER(EigenConf), C.POINTER(Conf), _P, _P, _P, C.c_uint32, _u32p, _P, C.c_uint64,
                        _u32p, C.POINTER(EigenInfo)],
+    "rs_eigen_solve_learned": [C.POINTER(EigenConf), C.POINTER(Conf), _P, _P, _P, _P, C.c_uint32, _u32p, _P,
+                               C.c_uint64, _u32p, C.POINTER(EigenInfo)],
 }
 
 _lib = None
@@ -823,7 +825,9 @@ class REigenValueSolver:
         self.matrix_conf = matrix_conf
         self.info: EigenInfo | None = None
 
-    def solve(self, M: SparseMatrix, K: SparseMatrix):
+    def solve(self, M: SparseMatrix, K: SparseMatrix, preconditioner: "LearnedPreconditioner | None" = None):
+        """preconditioner: a multilevel preconditioner built for K, used for every inner solve
+        with K (rs_eigen_solve_learned; no counterpart in the reference)."""
         n = M.n
         ne = max(1, min(self.conf.n_eigen_values, n))
         d = np.zeros(ne)
@@ -833,8 +837,14 @@ class REigenValueSolver:
                        self.conf.solver_cvg_value)
         mc = self.matrix_conf.to_c()
         info = EigenInfo()
-        _check(_lib.rs_eigen_solve(C.byref(ec), C.byref(mc), M.h, K.h, _ptr(d), ne, C.byref(nd), _ptr(ev),
-                                   ev.size, C.byref(nr), C.byref(info)))
+        if preconditioner is None:
+            _check(_lib.rs_eigen_solve(C.byref(ec), C.byref(mc), M.h, K.h, _ptr(d), ne, C.byref(nd), _ptr(ev),
+                                       ev.size, C.byref(nr), C.byref(info)))
+        else:
+            if preconditioner.A is not K:
+                raise RSError(RS_ERR_INVALID_ARG, "REigenValueSolver: the preconditioner was built for another matrix")
+            _check(_lib.rs_eigen_solve_learned(C.byref(ec), C.byref(mc), M.h, K.h, preconditioner.h, _ptr(d), ne,
+                                               C.byref(nd), _ptr(ev), ev.size, C.byref(nr), C.byref(info)))
         self.info = info
         rows = ev[: nr.value * n].reshape(nr.value, n) if nr.value else None
         return d[: nd.value], rows

@@ -174,8 +174,12 @@ int random_unit(EvWork &w, double *d_v, std::vector<double> &h)
     return RS_OK;
 }
 
+// Every inner solve is Jacobi-CG as in the reference; with a multilevel preconditioner built
+// for K (rs_eigen_solve_learned) the solves with K run flexible CG with its V-cycle instead.
 struct InnerSolver {
     rs_solver *S = nullptr;
+    rs_learned *L = nullptr;
+    const rs_matrix *K = nullptr;
     uint64_t iterations = 0;
     ~InnerSolver()
     {
@@ -190,7 +194,8 @@ struct InnerSolver {
     int solve(rs_matrix *A, const double *b, double *x)
     {
         rs_info info;
-        int rc = rs_solver_solve_device(S, A, b, x, RS_PRECOND_JACOBI, 1, &info);
+        int rc = (L && A == K) ? rs_solver_solve_learned_device(S, A, L, b, x, &info)
+                               : rs_solver_solve_device(S, A, b, x, RS_PRECOND_JACOBI, 1, &info);
         iterations += info.iterations;
         return rc;
     }
@@ -280,9 +285,9 @@ int check_pair(const rs_matrix *M, const rs_matrix *K)
 
 using namespace rs;
 
-extern "C" int rs_eigen_solve(const rs_eigen_conf *ec, const rs_conf *mc, rs_matrix *M, rs_matrix *K, double *d,
-                              uint32_t d_capacity, uint32_t *n_d, double *ev, uint64_t ev_capacity,
-                              uint32_t *n_ev_rows, rs_eigen_info *info)
+static int eigen_solve(const rs_eigen_conf *ec, const rs_conf *mc, rs_matrix *M, rs_matrix *K, rs_learned *L,
+                       double *d, uint32_t d_capacity, uint32_t *n_d, double *ev, uint64_t ev_capacity,
+                       uint32_t *n_ev_rows, rs_eigen_info *info)
 {
     if (!ec || !mc || !M || !K || !d || !n_d || !n_ev_rows)
         return fail(RS_ERR_INVALID_ARG, "rs_eigen_solve: null argument");
@@ -297,6 +302,8 @@ extern "C" int rs_eigen_solve(const rs_eigen_conf *ec, const rs_conf *mc, rs_mat
     if ((rc = w.init(n))) return rc;
     InnerSolver solver;
     if ((rc = solver.create(mc))) return rc;
+    solver.L = L;
+    solver.K = K;
     std::vector<double> h(n);
     const uint32_t grid = ev_blocks(n);
     EventPair t0t1_;
@@ -475,4 +482,21 @@ extern "C" int rs_eigen_solve(const rs_eigen_conf *ec, const rs_conf *mc, rs_mat
         info->solve_ms = ms;
     }
     return RS_OK;
+}
+
+extern "C" int rs_eigen_solve(const rs_eigen_conf *ec, const rs_conf *mc, rs_matrix *M, rs_matrix *K, double *d,
+                              uint32_t d_capacity, uint32_t *n_d, double *ev, uint64_t ev_capacity,
+                              uint32_t *n_ev_rows, rs_eigen_info *info)
+{
+    return eigen_solve(ec, mc, M, K, nullptr, d, d_capacity, n_d, ev, ev_capacity, n_ev_rows, info);
+}
+
+extern "C" int rs_eigen_solve_learned(const rs_eigen_conf *ec, const rs_conf *mc, rs_matrix *M, rs_matrix *K,
+                                      rs_learned *L, double *d, uint32_t d_capacity, uint32_t *n_d, double *ev,
+                                      uint64_t ev_capacity, uint32_t *n_ev_rows, rs_eigen_info *info)
+{
+    if (!L) return fail(RS_ERR_INVALID_ARG, "rs_eigen_solve_learned: null preconditioner");
+    if (mc && mc->solver_type != RS_SOLVER_CG)
+        return fail(RS_ERR_UNSUPPORTED, "rs_eigen_solve_learned: the inner solver is flexible CG");
+    return eigen_solve(ec, mc, M, K, L, d, d_capacity, n_d, ev, ev_capacity, n_ev_rows, info);
 }
