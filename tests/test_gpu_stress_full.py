@@ -139,6 +139,62 @@ def test_modal_cantilever_matches_dense(gpu, orc):
     np.testing.assert_allclose(d, d_ref, rtol=1e-7)
 
 
+@pytest.mark.parametrize("method,ne", [(1, 24), (0, 24), (2, 1)])
+def test_modal_cantilever_multilevel_inner_solves(gpu, orc, method, ne):
+    """rs_eigen_solve_learned: the inner solves with K through flexible CG with the rigid-body
+    V-cycle give the eigenvalues of the Jacobi-CG run (both from the same start vector) in a
+    fraction of its inner iterations; Arnoldi's lowest four also match a dense eigensolve to 1e-9.
+    The two runs' inner solves stop at different residuals below 1e-13; the upper Ritz values of a
+    24-dimensional Krylov space are not converged and move with that (5e-8 measured), hence 1e-6
+    over all values."""
+    rtol = 1e-6
+    rs = gpu
+    xyz, tets, E, nu, rho, fixed, _, _ = cantilever(rs, 16, 4, 4, seed=6)
+    nn = xyz.shape[0]
+    asm = rs.Assembly(xyz, tets, 3, fixed)
+    asm.assemble_stress_ex(rs.StressConf(2, 1.0, 1.0, (0.0, 0.0, 0.0)), E, nu, rho, np.zeros(3 * nn))
+    K, M = asm.matrix, asm.mass_matrix()
+    L = rs.LearnedPreconditioner.from_bins(K, asm, (8, 2, 2), max_coarse=rs.MAX_COARSE, rigid=True)
+    L.set_cycle(1, 1, rs.COARSE_EXACT)
+    assert [int(v) for v in L.level_rows] == [asm.n_dof, 8 * 2 * 2 * 6]
+    econf = rs.EigenValueSolverConf(method=method, n_eigen_values=ne, n_iterations=500 if method != 2 else 50,
+                                    solver_cvg_value=1e-12)
+    mconf = rs.MatrixSolverConf(n_outer_iterations=100000, solver_cvg_value=1e-13)
+    runs = []
+    for P in (None, L):
+        rs.seed_c_rand(4)
+        S = rs.REigenValueSolver(econf, mconf)
+        d, ev = S.solve(M, K, preconditioner=P)
+        runs.append((d, ev, S.info))
+    (dj, evj, ij), (dl, evl, il) = runs
+    np.testing.assert_allclose(dl, dj, rtol=rtol)
+    if method != 2:
+        assert il.outer_iterations == ij.outer_iterations == ne
+        # Rayleigh's solves with M + mu K and with M stay Jacobi-CG; Lanczos's and Arnoldi's are all with K
+        assert il.cg_iterations * 4 < ij.cg_iterations, (il.cg_iterations, ij.cg_iterations)
+    if method == 1:
+        lam = sla.eigh(K.to_scipy().toarray(), M.to_scipy().toarray(), eigvals_only=True)[:4]
+        np.testing.assert_allclose(np.sort(dl)[:4], lam, rtol=1e-9)
+        for a, b in zip(evl[:4], evj[:4]):
+            assert min(np.linalg.norm(a - b), np.linalg.norm(a + b)) <= 1e-6 * np.linalg.norm(b)
+
+
+def test_modal_multilevel_invalid(gpu):
+    rs = gpu
+    xyz, tets, E, nu, rho, fixed, _, _ = cantilever(rs, 8, 2, 2, seed=7)
+    nn = xyz.shape[0]
+    asm = rs.Assembly(xyz, tets, 3, fixed)
+    asm.assemble_stress_ex(rs.StressConf(2, 1.0, 1.0, (0.0, 0.0, 0.0)), E, nu, rho, np.zeros(3 * nn))
+    K, M = asm.matrix, asm.mass_matrix()
+    L = rs.LearnedPreconditioner.from_bins(K, asm, (4, 1, 1), max_coarse=rs.MAX_COARSE, rigid=True)
+    L.set_cycle(1, 1, rs.COARSE_EXACT)
+    econf = rs.EigenValueSolverConf(method=rs.EIGEN_ARNOLDI, n_eigen_values=4, n_iterations=100)
+    with pytest.raises(rs.RSError, match="another matrix"):
+        rs.REigenValueSolver(econf, rs.MatrixSolverConf()).solve(K, M, preconditioner=L)
+    with pytest.raises(rs.RSError, match="flexible CG"):
+        rs.REigenValueSolver(econf, rs.MatrixSolverConf(solver_type=rs.SOLVER_GMRES)).solve(M, K, preconditioner=L)
+
+
 def _rotations(rng, nn, frac=0.4):
     rot = np.tile(np.eye(3), (nn, 1, 1))
     active = (rng.random(nn) < frac).astype(np.uint8)
