@@ -212,10 +212,38 @@ struct Prolong {
     const uint32_t *mptr, *mrows;    // finer rows of each node of this level
     const float *zpre;
     float *zout;
+    // One lane: the members eight at a time, every load of a batch issued before its stores (the
+    // member rows of a node are distinct, so a store never feeds a later load of the batch).
     __device__ __forceinline__ void apply(uint32_t I, float e) const
     {
         if (!mptr) return;
-        for (uint32_t k = mptr[I]; k < mptr[I + 1]; k++) {
+        const uint32_t *__restrict__ mr = mrows;
+        const float *__restrict__ zp = zpre;
+        float *__restrict__ zo = zout;
+        uint32_t k = mptr[I];
+        const uint32_t k1 = mptr[I + 1];
+        for (; k + 8 <= k1; k += 8) {
+            uint32_t i[8];
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) i[u] = mr[k + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = (zp ? zp[i[u]] : 0.0f) + e;
+#pragma unroll
+            for (int u = 0; u < 8; u++) zo[i[u]] = v[u];
+        }
+        for (; k < k1; k++) {
+            const uint32_t i = mr[k];
+            zo[i] = (zp ? zp[i] : 0.0f) + e;
+        }
+    }
+    // The G lanes of a row group (each holding e): member k on lane k mod G.
+    template <int G>
+    __device__ __forceinline__ void apply_group(uint32_t I, float e, uint32_t gl) const
+    {
+        if (!mptr) return;
+        const uint32_t k1 = mptr[I + 1];
+        for (uint32_t k = mptr[I] + gl; k < k1; k += G) {
             const uint32_t i = mrows[k];
             zout[i] = (zpre ? zpre[i] : 0.0f) + e;
         }
@@ -287,6 +315,27 @@ struct EpiCheb {
             acc[0] += (double)ri * (double)z;
             acc[1] += (double)z * (double)q[i];
         }
+    }
+    // Shapes with a lane group per row: every lane holds the row's sum (a butterfly sum, the same
+    // bits on every lane) and computes the same z; lane 0 stores, the group spreads the
+    // prolongation over its lanes.
+    static constexpr bool kGroupRow = true;
+    template <int G>
+    __device__ void row_group(uint32_t i, float s, uint32_t gl)
+    {
+        const float ri = r[i];
+        float dd = c2 * dinv[i] * (ri - s);
+        if (c1 != 0.0f) dd = fmaf(c1, d[i], dd);
+        const float z = zin[i] + dd;
+        if (gl == 0) {
+            d[i] = dd;
+            zout[i] = z;
+            if (dots) {
+                acc[0] += (double)ri * (double)z;
+                acc[1] += (double)z * (double)q[i];
+            }
+        }
+        up.template apply_group<G>(i, z, gl);
     }
 };
 
@@ -526,10 +575,8 @@ __global__ __launch_bounds__(kBlock) void k_coarse_exact(const float *__restrict
         float s = 0.0f;
         for (uint32_t j = lane; j < n; j += kWave) s += cinv[(uint64_t)row * n + j] * r[j];
         s = group_sum<kWave>(s);
-        if (lane == 0) {
-            z[row] = s;
-            up.apply(row, s);
-        }
+        if (lane == 0) z[row] = s;
+        up.apply_group<kWave>(row, s, lane);
     }
 }
 
@@ -767,19 +814,37 @@ __global__ __launch_bounds__(kWave) void k_gnn_embed(GnnArgs g)
     }
     const float *P = g.params;
     const float w0 = P[kOffWin + lane], w1 = P[kOffWin + kHid + lane], c = P[kOffWin + 2 * kHid + lane] + P[kOffBin + lane];
-    for (uint32_t I = r0; I < r0 + 16; I++) {
-        float z = 0.0f;
-        if (I < nc) {
-            const float d = g.diag[I], uu = g.rc[I] / d;
-            z = (uu / sigma) * w0 + (logf(d) - meanlog) * w1 + c;
-            if (lane == 0) g.u[I] = uu;
-        } else if (lane == 0) {
-            g.u[I] = 0.0f;
+    // the strip's 16 nodes on lanes 0..15 (one load round), then every lane (a feature) reads
+    // them by shuffles; lane f's 16 values of ht are contiguous and go out as two 16-byte stores
+    const uint32_t Il = r0 + (lane & 15);
+    float ul = 0.0f, ll = 0.0f;
+    if (lane < 16) {
+        if (Il < nc) {
+            const float d = g.diag[Il];
+            ul = g.rc[Il] / d;
+            ll = logf(d);
         }
-        const unsigned short hv = f2bf(fmaxf(z, 0.0f));
-        g.h[0][I * kHid + lane] = hv;
-        g.ht[0][lane * npad + I] = hv;
+        g.u[Il] = ul;
     }
+    unsigned short hs[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const float uu = __shfl(ul, t, kWave), ld = __shfl(ll, t, kWave);
+        const float z = r0 + t < nc ? (uu / sigma) * w0 + (ld - meanlog) * w1 + c : 0.0f;
+        hs[t] = f2bf(fmaxf(z, 0.0f));
+        g.h[0][(r0 + t) * kHid + lane] = hs[t];
+    }
+    uint4 v[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        v[k].x = hs[8 * k + 0] | ((uint32_t)hs[8 * k + 1] << 16);
+        v[k].y = hs[8 * k + 2] | ((uint32_t)hs[8 * k + 3] << 16);
+        v[k].z = hs[8 * k + 4] | ((uint32_t)hs[8 * k + 5] << 16);
+        v[k].w = hs[8 * k + 6] | ((uint32_t)hs[8 * k + 7] << 16);
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(g.ht[0] + (size_t)lane * npad + r0);
+    dst[0] = v[0];
+    dst[1] = v[1];
 }
 
 // Layer l for one strip of 16 nodes, 4 wavefronts.  Wavefront w computes the messages of feature
@@ -857,14 +922,17 @@ __global__ __launch_bounds__(4 * kWave) void k_gnn_layer(GnnArgs g, int l)
     }
     const uint32_t f = ct * 16 + rc;
     float hv[4];
+    unsigned short b16[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const uint32_t row = r0 + kq + i;
-        const unsigned short b16 = row < nc ? f2bf(hold[i] + fmaxf(acc[i] + bias, 0.0f)) : (unsigned short)0;
-        hv[i] = bf2f(b16);
-        Hn[row * kHid + f] = b16;
-        Htn[f * npad + row] = b16;
+        b16[i] = row < nc ? f2bf(hold[i] + fmaxf(acc[i] + bias, 0.0f)) : (unsigned short)0;
+        hv[i] = bf2f(b16[i]);
+        Hn[row * kHid + f] = b16[i];
     }
+    // rows r0 + kq .. + 3 of feature f are contiguous in ht: one 8-byte store
+    *reinterpret_cast<uint2 *>(Htn + (size_t)f * npad + r0 + kq) =
+        make_uint2(b16[0] | ((uint32_t)b16[1] << 16), b16[2] | ((uint32_t)b16[3] << 16));
     if (l == kLayers - 1) {
         // e = u + sigma (H wo + bo) for the strip's nodes
         const float wo = P[kOffWo + f];
@@ -875,13 +943,14 @@ __global__ __launch_bounds__(4 * kWave) void k_gnn_layer(GnnArgs g, int l)
 #pragma unroll
             for (int i = 0; i < 4; i++) sO[w][kq + i] = part[i];
         __syncthreads();
-        if (threadIdx.x < 16) {
-            const uint32_t row = r0 + threadIdx.x;
+        if (threadIdx.x < 64) {
+            // four lanes per node: the same ev on each, the prolongation spread over them
+            const uint32_t t = threadIdx.x >> 2, sub = threadIdx.x & 3u, row = r0 + t;
             if (row < nc) {
-                const float o = ((sO[0][threadIdx.x] + sO[1][threadIdx.x]) + (sO[2][threadIdx.x] + sO[3][threadIdx.x])) + P[kOffBo];
+                const float o = ((sO[0][t] + sO[1][t]) + (sO[2][t] + sO[3][t])) + P[kOffBo];
                 const float ev = g.u[row] + g.norm[0] * o;
-                g.e[row] = ev;
-                g.up.apply(row, ev);
+                if (sub == 0) g.e[row] = ev;
+                g.up.apply_group<4>(row, ev, sub);
             }
         }
     }

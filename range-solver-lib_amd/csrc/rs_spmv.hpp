@@ -18,15 +18,30 @@
 //                product loop has no branch.
 //
 // All call epi.row(r, s) exactly once per row with the row's dot product s (accumulated in
-// AT: double for the FP64 path, float for the FP32 path); the epilogue fuses whatever the
-// caller needs (store, residual, dot products).  Sums are in a fixed order (deterministic).
+// AT: double for the FP64 path, float for the FP32 path; VecRows calls a group epilogue's
+// row_group on each lane of the row's group instead); the epilogue fuses whatever the caller
+// needs (store, residual, dot products).  Sums are in a fixed order (deterministic).
 #pragma once
+
+#include <type_traits>
 
 #include "rs_device.hpp"
 
 namespace rs {
 
 constexpr uint32_t kTileRows = 256;
+
+// An epilogue with kGroupRow = true is called by every lane of a VecRows group, as
+// epi.row_group<G>(r, s, lane in group) with the group's sum on every lane, instead of
+// epi.row(r, s) on lane 0 only; it stores each row once all the same.
+template <class E, class = void>
+struct GroupRow {
+    static constexpr bool value = false;
+};
+template <class E>
+struct GroupRow<E, std::void_t<decltype(E::kGroupRow)>> {
+    static constexpr bool value = E::kGroupRow;
+};
 
 struct CsrView {
     uint32_t n;
@@ -96,7 +111,8 @@ struct VecRows {
                 AT s = 0;
                 for (uint32_t j = b + gl; j < e; j += G) s += (AT)val[j] * (AT)x[col[j]];
                 s = group_sum<G>(s);
-                if (gl == 0) epi.row(r, s);
+                if constexpr (GroupRow<Epi>::value) epi.template row_group<G>(r, s, gl);
+                else if (gl == 0) epi.row(r, s);
             }
         }
     }
