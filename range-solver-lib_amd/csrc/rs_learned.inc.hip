@@ -1482,21 +1482,30 @@ struct RigidGalArgs {
     double *cval;                           // level l + 1 values
 };
 
-// Level l + 1 values Q^T A Q block by block: one wavefront per aggregate entry (J, J').  Lane t
-// takes the listed entries (i, j) t, t + 64, ... in list order and keeps the 36 products
-// q_i[a] a_ij q_j[b]; xor butterflies then sum them over the wavefront (a fixed order) and lane
-// 6 a + b writes entry (a, b).  The diagonal of a dead row is 1.
+// Level l + 1 values Q^T A Q block by block: a group of G lanes (32 or 64) per aggregate entry
+// (J, J'); G = 32 where the entries list few level-l entries (~80 on a mesh's finest level), 64
+// where they list hundreds (the coarser levels).  Lane t of the group takes the listed entries
+// (i, j) t, t + G, ... in list order and keeps the 36 products q_i[a] a_ij q_j[b]; a
+// reduce-scatter over the group then sums them in a fixed order: the xor-G/2 and xor-G/4
+// exchanges leave each lane 9 of the 36 sums (index base (t & G/2 ? 18 : 0) + (t & G/4 ? 9 : 0)),
+// partial over the G/4 lanes of its subgroup, and a subgroup sum completes them (63 or 54 FP64
+// shuffles, against 36 x 6 for a butterfly per sum).  Lane c < 9 of a subgroup writes sum c (with
+// 8 lanes, lane 0 also sum 8).  The diagonal of a dead row is 1.
+template <int G>
 __global__ __launch_bounds__(kBlock) void k_rigid_galerkin(RigidGalArgs a)
 {
-    const uint32_t lane = threadIdx.x % kWave;
-    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / kWave);
-    // k is the same on every lane of a wavefront
-    for (uint64_t k = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; k < a.cnnz; k += stride) {
+    static_assert(G == 32 || G == 64, "32 or 64 lanes per aggregate entry");
+    constexpr uint32_t kSub = G / 4;
+    const uint32_t hl = threadIdx.x % G;
+    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / G);
+    // k is the same on every lane of a group, so every exchange below (xor offsets below G)
+    // stays among lanes of one k
+    for (uint64_t k = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G; k < a.cnnz; k += stride) {
         double acc[36];
 #pragma unroll
         for (int c = 0; c < 36; c++) acc[c] = 0.0;
         const uint32_t e = a.gptr[k + 1];
-        for (uint32_t t = a.gptr[k] + lane; t < e; t += kWave) {
+        for (uint32_t t = a.gptr[k] + hl; t < e; t += G) {
             const uint32_t p = a.gsrc[t];
             const double v = a.fval[p];
             const double *qi = a.q64 + 6ull * a.frow[p];
@@ -1514,18 +1523,40 @@ __global__ __launch_bounds__(kBlock) void k_rigid_galerkin(RigidGalArgs a)
                 for (int cb = 0; cb < 6; cb++) acc[6 * ra + cb] += w * vj[cb];
             }
         }
-        double out = 0.0;
+        const bool hi = (hl & (G / 2)) != 0, lo = (hl & kSub) != 0;
+        double v18[18];
 #pragma unroll
-        for (int c = 0; c < 36; c++) {
-            const double t = group_sum<kWave>(acc[c]);
-            if (lane == (uint32_t)c) out = t;
+        for (int c = 0; c < 18; c++) {
+            const double send = hi ? acc[c] : acc[18 + c];
+            const double keep = hi ? acc[18 + c] : acc[c];
+            v18[c] = keep + __shfl_xor(send, G / 2, kWave);
         }
-        if (lane < 36) {
-            const uint32_t ra = lane / 6, cb = lane % 6;
-            const uint32_t J = a.crow[k];
-            if (a.ccol[k] == J && ra == cb && a.dead[6 * J + ra]) out = 1.0;
-            a.cval[a.erowptr[6 * J + ra] + 6 * (k - a.cptr[J]) + cb] = out;
+        double v9[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            const double send = lo ? v18[c] : v18[9 + c];
+            const double keep = lo ? v18[9 + c] : v18[c];
+            v9[c] = keep + __shfl_xor(send, kSub, kWave);
         }
+        const uint32_t sub = hl % kSub;
+        double out = 0.0, out8 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            const double t = group_sum<kSub>(v9[c]);
+            if (sub == (uint32_t)c) out = t;
+            if (c == 8) out8 = t;
+        }
+        const uint32_t base = (hi ? 18u : 0u) + (lo ? 9u : 0u);
+        const uint32_t J = a.crow[k];
+        const bool diag_block = a.ccol[k] == J;
+        const uint32_t col0 = 6 * (uint32_t)(k - a.cptr[J]);
+        auto put = [&](uint32_t idx, double val) {
+            const uint32_t ra = idx / 6, cb = idx % 6;
+            if (diag_block && ra == cb && a.dead[6 * J + ra]) val = 1.0;
+            a.cval[a.erowptr[6 * J + ra] + col0 + cb] = val;
+        };
+        if (sub < 9) put(base + sub, out);
+        if (kSub < 9 && sub == 0) put(base + 8, out8);
     }
 }
 
@@ -1809,7 +1840,12 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
             const uint64_t cnnz = F.ncrow.n;
             RigidGalArgs g{ F.M->val.p, F.M->col.p, F.erow.p, F.q64.p, F.gptr.p, F.gsrc.p, F.ncptr.p, F.nccol.p,
                             F.ncrow.p, C.M->rowptr.p, C.dead.p, cnnz, C.M->val.p };
-            k_rigid_galerkin<<<vec_blocks(std::max<uint64_t>(cnnz, 1) * kWave), kBlock, 0, st>>>(g);
+            // 32 lanes per aggregate entry while the entries list at most ~3 trips of 32 level-l
+            // entries on average, 64 above
+            if (F.gsrc.n <= 96 * std::max<uint64_t>(cnnz, 1))
+                k_rigid_galerkin<32><<<vec_blocks(std::max<uint64_t>(cnnz, 1) * 32), kBlock, 0, st>>>(g);
+            else
+                k_rigid_galerkin<64><<<vec_blocks(std::max<uint64_t>(cnnz, 1) * 64), kBlock, 0, st>>>(g);
         } else {
             k_galerkin_gather<<<vec_blocks(std::max<uint64_t>(C.M->nnz, 1)), kBlock, 0, st>>>(F.M->val.p, F.gptr.p,
                                                                                              F.gsrc.p, C.M->nnz, C.M->val.p);
