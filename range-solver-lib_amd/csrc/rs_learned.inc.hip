@@ -179,16 +179,22 @@ __global__ __launch_bounds__(kBlock) void k_level_diag(const uint32_t *__restric
 }
 
 // Coarsest graph for the network: diagonal and message weights -A[I,J]/A[I,I] (0 on the diagonal).
-__global__ void k_coarse_weights(const uint32_t *__restrict__ cptr, const uint32_t *__restrict__ ccol,
-                                 const double *__restrict__ cval, uint32_t nc, float *__restrict__ cw,
-                                 float *__restrict__ cdiag)
+// 16 lanes per row; a row's one diagonal entry plus zeros sums to itself exactly.
+__global__ __launch_bounds__(kBlock) void k_coarse_weights(const uint32_t *__restrict__ cptr,
+                                                           const uint32_t *__restrict__ ccol,
+                                                           const double *__restrict__ cval, uint32_t nc,
+                                                           float *__restrict__ cw, float *__restrict__ cdiag)
 {
-    for (uint32_t I = blockIdx.x * kBlock + threadIdx.x; I < nc; I += gridDim.x * kBlock) {
+    constexpr int G = 16;
+    const uint32_t gl = threadIdx.x % G;
+    for (uint64_t I = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G; I < nc; I += (uint64_t)gridDim.x * (kBlock / G)) {
+        const uint32_t p0 = cptr[I], p1 = cptr[I + 1];
         double d = 0.0;
-        for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++)
+        for (uint32_t p = p0 + gl; p < p1; p += G)
             if (ccol[p] == I) d += cval[p];
-        cdiag[I] = (float)d;
-        for (uint32_t p = cptr[I]; p < cptr[I + 1]; p++)
+        d = group_sum<G>(d);
+        if (gl == 0) cdiag[I] = (float)d;
+        for (uint32_t p = p0 + gl; p < p1; p += G)
             cw[p] = (ccol[p] == I || d == 0.0) ? 0.0f : (float)(-cval[p] / d);
     }
 }
@@ -598,11 +604,16 @@ __global__ void k_dense_pad(double *__restrict__ m, uint32_t N, uint32_t n)
     }
 }
 
-__global__ void k_dense_scatter(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ col,
-                                const double *__restrict__ val, uint32_t n, uint32_t N, double *__restrict__ m)
+// 16 lanes per row.
+__global__ __launch_bounds__(kBlock) void k_dense_scatter(const uint32_t *__restrict__ rowptr,
+                                                          const uint32_t *__restrict__ col,
+                                                          const double *__restrict__ val, uint32_t n, uint32_t N,
+                                                          double *__restrict__ m)
 {
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        for (uint32_t p = rowptr[i]; p < rowptr[i + 1]; p++) m[(uint64_t)i * N + col[p]] = val[p];
+    constexpr uint32_t G = 16;
+    const uint32_t gl = threadIdx.x % G;
+    for (uint64_t i = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / G; i < n; i += (uint64_t)gridDim.x * (kBlock / G))
+        for (uint32_t p = rowptr[i] + gl; p < rowptr[i + 1]; p += G) m[i * N + col[p]] = val[p];
 }
 
 // D = A_kk^-1 by sweeping the 64 scalar pivots of the block (one workgroup; each thread keeps a
@@ -1864,13 +1875,14 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
     const uint32_t kParts = 4096;
     for (uint32_t l = 0; l < L.nlev; l++) {
         CycLevel &V = *L.lv[l];
-        uint32_t nb = std::min(vec_blocks(std::max(V.n, 1u)), kParts);
+        // 16 lanes per row: a workgroup per 16 rows, up to the partials' capacity
+        uint32_t nb = std::min(vec_blocks(std::max<uint64_t>(V.n, 1) * 16), kParts);
         k_level_diag<<<nb, kBlock, 0, st>>>(V.M->rowptr.p, V.M->col.p, V.M->val.p, V.n, V.diag.p, V.dinv.p,
                                             L.part.p + (uint64_t)l * kParts);
         k_max_fin<<<1, kMaxFinBlock, 0, st>>>(L.part.p + (uint64_t)l * kParts, nb, L.gersh.p + l);
     }
     const Matrix &C = *L.lv.back()->M;
-    k_coarse_weights<<<vec_blocks(L.nc), kBlock, 0, st>>>(C.rowptr.p, C.col.p, C.val.p, L.nc, L.cw.p, L.cdiag.p);
+    k_coarse_weights<<<vec_blocks((uint64_t)L.nc * 16), kBlock, 0, st>>>(C.rowptr.p, C.col.p, C.val.p, L.nc, L.cw.p, L.cdiag.p);
     k_fill_u16<<<vec_blocks(L.gbval.n), kBlock, 0, st>>>(L.gbval.p, L.gbval.n);
     k_gnn_blocks<<<vec_blocks(std::max<uint64_t>(C.nnz, 1)), kBlock, 0, st>>>(L.cw.p, L.gbidx.p, C.nnz, L.gbval.p);
     RS_HIP(hipGetLastError());
@@ -1909,7 +1921,7 @@ static int learned_setup(rs_learned &L, rs_matrix &A)
             return rc;
         k_fill_u32<<<1, 64, 0, st>>>(L.sweep_bad.p, 1, 0u);
         k_dense_pad<<<vec_blocks((uint64_t)N * N), kBlock, 0, st>>>(L.sweep[0].p, N, n);
-        k_dense_scatter<<<vec_blocks(n), kBlock, 0, st>>>(C.rowptr.p, C.col.p, C.val.p, n, N, L.sweep[0].p);
+        k_dense_scatter<<<vec_blocks((uint64_t)n * 16), kBlock, 0, st>>>(C.rowptr.p, C.col.p, C.val.p, n, N, L.sweep[0].p);
         for (uint32_t k = 0; k < nbk; k++) {
             k_sweep_pivot<<<1, kBlock, 0, st>>>(L.sweep[k & 1].p, N, k, L.sweep_d.p, L.sweep_bad.p);
             k_sweep_tiles<<<nbk * nbk, kBlock, 0, st>>>(L.sweep[k & 1].p, L.sweep[(k + 1) & 1].p, N, k, L.sweep_d.p);
