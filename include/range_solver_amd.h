@@ -553,8 +553,8 @@ int rs_learned_set_coarse_cycle(rs_learned *L, uint32_t pre, uint32_t post);
 /* Levels with at least min_rows rows smooth with a compressed copy of D^-1 A (FP16 values,
  * int16 column offsets in the SELL-64 slots; 4 bytes per entry); 0 keeps every level on its
  * FP32 values.  Levels whose offsets do not fit in 16 bits stay FP32.  Default: levels of at
- * least 65536 rows use the copy where a residual pass with it timed faster than with the level's
- * FP32 plan (decided once per hierarchy); after this call min_rows is followed as given. */
+ * least 65536 rows use the copy where a residual pass with it timed at least 10% faster than with
+ * the level's FP32 plan (decided once per hierarchy); after this call min_rows is followed as given. */
 int rs_learned_set_compressed(rs_learned *L, uint32_t min_rows);
 /* Levels including the finest; level_rows / level_nnz hold n_levels entries when given. */
 int rs_learned_info(const rs_learned *L, uint32_t *n_levels, uint32_t *level_rows,

@@ -1989,7 +1989,9 @@ static int learned_plans(rs_learned &L)
             RS_HIP(hipEventElapsedTime(&f, ev.e[0], ev.e[1]));
             ms[k] = f;
         }
-        V.s16_pref = ms[0] <= ms[1] ? 1 : 2;
+        // the copy rounds the smoother's values to FP16, so it is taken only for a clear gain:
+        // with a near tie the timing noise alone would pick it on some runs and not on others
+        V.s16_pref = ms[0] <= 0.9 * ms[1] ? 1 : 2;
         if (V.s16_pref == 2) V.use_s16 = false;
     }
     return RS_OK;
