@@ -216,10 +216,24 @@ uint32_t key_bits(uint64_t max_key)
 }
 
 // Workspace of the library routines, grown on demand.
+// rocPRIM temporary storage: its own buffer, or a HierWork's when one is given.
 struct Temp {
-    DBuf<uint8_t> buf;
-    int fit(size_t bytes) { return bytes <= buf.n && buf.p ? RS_OK : buf.alloc(std::max<size_t>(bytes, 256)); }
+    DBuf<uint8_t> own;
+    DBuf<uint8_t> *ext = nullptr;
+    DBuf<uint8_t> &buf() { return ext ? *ext : own; }
+    int fit(size_t bytes)
+    {
+        DBuf<uint8_t> &b = buf();
+        return bytes <= b.n && b.p ? RS_OK : b.alloc(std::max<size_t>(bytes, 256));
+    }
 };
+
+// At least count elements (a HierWork buffer keeps a larger earlier allocation).
+template <typename T>
+int grow(DBuf<T> &b, size_t count)
+{
+    return b.p && b.n >= count ? RS_OK : b.alloc(count);
+}
 
 template <typename K>
 int sort_pairs(Temp &tmp, const K *kin, K *kout, const uint32_t *vin, uint32_t *vout, uint64_t m, uint32_t bits,
@@ -229,8 +243,8 @@ int sort_pairs(Temp &tmp, const K *kin, K *kout, const uint32_t *vin, uint32_t *
     RS_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, m, 0u, bits, st));
     int rc = tmp.fit(bytes);
     if (rc) return rc;
-    bytes = tmp.buf.n;
-    RS_HIP(rocprim::radix_sort_pairs(tmp.buf.p, bytes, kin, kout, vin, vout, m, 0u, bits, st));
+    bytes = tmp.buf().n;
+    RS_HIP(rocprim::radix_sort_pairs(tmp.buf().p, bytes, kin, kout, vin, vout, m, 0u, bits, st));
     return RS_OK;
 }
 
@@ -240,8 +254,8 @@ int scan_u32(Temp &tmp, const uint32_t *in, uint32_t *out, uint64_t m, hipStream
     RS_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, (size_t)m, rocprim::plus<uint32_t>(), st));
     int rc = tmp.fit(bytes);
     if (rc) return rc;
-    bytes = tmp.buf.n;
-    RS_HIP(rocprim::exclusive_scan(tmp.buf.p, bytes, in, out, 0u, (size_t)m, rocprim::plus<uint32_t>(), st));
+    bytes = tmp.buf().n;
+    RS_HIP(rocprim::exclusive_scan(tmp.buf().p, bytes, in, out, 0u, (size_t)m, rocprim::plus<uint32_t>(), st));
     return RS_OK;
 }
 
@@ -256,15 +270,19 @@ int counts_to_ptr(Temp &tmp, DBuf<uint32_t> &count, uint32_t nn, DBuf<uint32_t> 
 
 }  // namespace
 
-int hier_members(const uint32_t *d_map, uint32_t n, uint32_t nn, DBuf<uint32_t> &ptr, DBuf<uint32_t> &rows)
+int hier_members(const uint32_t *d_map, uint32_t n, uint32_t nn, DBuf<uint32_t> &ptr, DBuf<uint32_t> &rows,
+                 HierWork *work)
 {
     if (nn == 0) return fail(RS_ERR_INVALID_ARG, "hierarchy: empty level");
     hipStream_t st = runtime().stream;
     Temp tmp;
-    DBuf<uint32_t> kout, vin, count;
+    HierWork local;
+    HierWork &w = work ? *work : local;
+    if (work) tmp.ext = &work->tmp;
+    DBuf<uint32_t> &kout = w.kout32, &vin = w.vin, &count = w.count;
     int rc;
-    if ((rc = rows.alloc(std::max(n, 1u))) || (rc = kout.alloc(std::max(n, 1u))) || (rc = vin.alloc(std::max(n, 1u))) ||
-        (rc = count.alloc(nn + 1ull)))
+    if ((rc = rows.alloc(std::max(n, 1u))) || (rc = grow(kout, std::max(n, 1u))) || (rc = grow(vin, std::max(n, 1u))) ||
+        (rc = grow(count, nn + 1ull)))
         return rc;
     k_fill_u32<<<hier_grid(nn + 1ull), kHierBlock, 0, st>>>(count.p, nn + 1ull, 0u);
     if (n) {
@@ -280,19 +298,22 @@ int hier_members(const uint32_t *d_map, uint32_t n, uint32_t nn, DBuf<uint32_t> 
 
 int hier_level_lists(const uint32_t *d_rowptr, const uint32_t *d_col, uint32_t n, uint64_t nnz, const uint32_t *d_map,
                      uint32_t nn, DBuf<uint32_t> &cptr, DBuf<uint32_t> &ccol, DBuf<uint32_t> &gptr,
-                     DBuf<uint32_t> &gsrc, uint32_t &cnnz)
+                     DBuf<uint32_t> &gsrc, uint32_t &cnnz, HierWork *work)
 {
     if (nn == 0) return fail(RS_ERR_INVALID_ARG, "hierarchy: empty level");
     if (nnz >= 0xFFFFFFFFull) return fail(RS_ERR_UNSUPPORTED, "hierarchy: level matrix has 2^32 entries or more");
     hipStream_t st = runtime().stream;
     Temp tmp;
+    HierWork local;
+    HierWork &w = work ? *work : local;
+    if (work) tmp.ext = &work->tmp;
     int rc;
     const uint64_t m = nnz;
-    DBuf<uint64_t> kin, kout;
-    DBuf<uint32_t> pin, flag, run, row_runs;
+    DBuf<uint64_t> &kin = w.kin, &kout = w.kout;
+    DBuf<uint32_t> &pin = w.pin, &flag = w.flag, &run = w.run, &row_runs = w.count;
     const size_t m1 = std::max<uint64_t>(m, 1);
-    if ((rc = kin.alloc(m1)) || (rc = kout.alloc(m1)) || (rc = pin.alloc(m1)) || (rc = gsrc.alloc(m1)) ||
-        (rc = flag.alloc(m1)) || (rc = run.alloc(m1)) || (rc = row_runs.alloc(nn + 1ull)))
+    if ((rc = grow(kin, m1)) || (rc = grow(kout, m1)) || (rc = grow(pin, m1)) || (rc = gsrc.alloc(m1)) ||
+        (rc = grow(flag, m1)) || (rc = grow(run, m1)) || (rc = grow(row_runs, nn + 1ull)))
         return rc;
     k_fill_u32<<<hier_grid(nn + 1ull), kHierBlock, 0, st>>>(row_runs.p, nn + 1ull, 0u);
     uint32_t nruns = 0;

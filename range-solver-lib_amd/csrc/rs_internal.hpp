@@ -167,13 +167,23 @@ int matrix_create_device(uint32_t n, DBuf<uint32_t> &rowptr, DBuf<uint32_t> &col
 int hier_bins(uint32_t n_nodes, const double *d_xyz, uint32_t dpn, const uint32_t *d_book, uint32_t n_dof,
               const uint32_t bins[3], uint32_t max_coarse, uint32_t max_levels, bool by_component,
               std::vector<std::unique_ptr<DBuf<uint32_t>>> &maps, std::vector<uint32_t> &n_agg);
+// Scratch of the two builders below, kept across the levels of one hierarchy build so that the
+// finest level's sort buffers (hundreds of MB at 128^3) are allocated once and reused by the
+// coarser levels instead of a hipMalloc / hipFree pair per buffer and level.  Grow-only; the
+// builders pass every size explicitly.  Null: each call allocates its own.
+struct HierWork {
+    DBuf<uint64_t> kin, kout;
+    DBuf<uint32_t> pin, flag, run, count, vin, kout32;
+    DBuf<uint8_t> tmp;
+};
 // Members of each of the nn aggregates of d_map (n entries), rows ascending: ptr (nn + 1), rows (n).
-int hier_members(const uint32_t *d_map, uint32_t n, uint32_t nn, DBuf<uint32_t> &ptr, DBuf<uint32_t> &rows);
+int hier_members(const uint32_t *d_map, uint32_t n, uint32_t nn, DBuf<uint32_t> &ptr, DBuf<uint32_t> &rows,
+                 HierWork *work = nullptr);
 // Next-level pattern (cptr, ccol) of a level matrix under d_map, and the gather lists: the entries
 // of the level matrix summed into each next-level entry, by position (gptr: cnnz + 1, gsrc: nnz).
 int hier_level_lists(const uint32_t *d_rowptr, const uint32_t *d_col, uint32_t n, uint64_t nnz, const uint32_t *d_map,
                      uint32_t nn, DBuf<uint32_t> &cptr, DBuf<uint32_t> &ccol, DBuf<uint32_t> &gptr,
-                     DBuf<uint32_t> &gsrc, uint32_t &cnnz);
+                     DBuf<uint32_t> &gsrc, uint32_t &cnnz, HierWork *work = nullptr);
 int hier_compose(uint32_t *d_total, uint64_t n, const uint32_t *d_map);   // total[i] = map[total[i]]
 int hier_identity(uint32_t *d_a, uint64_t n);
 

@@ -2464,6 +2464,7 @@ static int learned_build(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<ui
     L->lv[0]->n = A->n;
     DBuf<uint32_t> total;
     if ((rc = total.alloc(std::max(A->n, 1u))) || (rc = hier_identity(total.p, A->n))) return rc;
+    HierWork work;   // the finest level sizes the lists' scratch, the coarser levels reuse it
     for (uint32_t l = 0; l < n_levels; l++) {
         CycLevel &V = *L->lv[l];
         const uint32_t nn = n_agg[l];
@@ -2472,9 +2473,9 @@ static int learned_build(const rs_matrix *A, std::vector<std::unique_ptr<DBuf<ui
         std::swap(V.agg.n, maps[l]->n);
         DBuf<uint32_t> cptr, ccol;
         uint32_t cnnz = 0;
-        if ((rc = hier_members(V.agg.p, V.n, nn, V.mptr, V.mrows))) return rc;
+        if ((rc = hier_members(V.agg.p, V.n, nn, V.mptr, V.mrows, &work))) return rc;
         if ((rc = hier_level_lists(V.M->rowptr.p, V.M->col.p, V.n, V.M->nnz, V.agg.p, nn, cptr, ccol, V.gptr, V.gsrc,
-                                   cnnz)))
+                                   cnnz, &work)))
             return rc;
         std::unique_ptr<CycLevel> C(new CycLevel());
         if ((rc = matrix_create_device(nn, cptr, ccol, cnnz, &C->own))) return rc;
@@ -2578,6 +2579,7 @@ static int learned_build_rigid(const rs_matrix *A, std::vector<std::unique_ptr<D
     if ((rc = bad.alloc(1))) return rc;
     k_fill_u32<<<1, 64, 0, st>>>(bad.p, 1, 0u);
     RS_HIP(hipGetLastError());
+    HierWork work;   // the finest level sizes the lists' scratch, the coarser levels reuse it
     for (uint32_t l = 0; l < n_levels; l++) {
         CycLevel &V = *L->lv[l];
         const uint32_t nn = n_agg[l];
@@ -2592,9 +2594,9 @@ static int learned_build_rigid(const rs_matrix *A, std::vector<std::unique_ptr<D
         }
         uint32_t cnnz = 0;
         DBuf<uint32_t> cptr, ccol;
-        if ((rc = hier_members(V.agg.p, V.n, nn, V.mptr, V.mrows))) return rc;
+        if ((rc = hier_members(V.agg.p, V.n, nn, V.mptr, V.mrows, &work))) return rc;
         if ((rc = hier_level_lists(V.M->rowptr.p, V.M->col.p, V.n, V.M->nnz, V.agg.p, nn, cptr, ccol, V.gptr, V.gsrc,
-                                   cnnz)))
+                                   cnnz, &work)))
             return rc;
         if (36ull * cnnz >= 0xFFFFFFFFull) return fail(RS_ERR_UNSUPPORTED, "rigid hierarchy: coarse level too large");
         // members per aggregate within one wavefront's reach
