@@ -70,3 +70,39 @@ def test_parallel_plates_patch(gpu):
     field = outs[0].download().reshape(ne, 3)
     V = 1.0 / (6 * 6 ** 3)
     np.testing.assert_allclose(field, np.tile([-5.0 * 6 * V, 0.0, 0.0], (ne, 1)), atol=1e-9)
+
+
+def test_electrostatics_multilevel_against_oracle(gpu, orc):
+    """The electrostatics system through the multilevel cycle (node bins, exact coarsest inverse)
+    in mixed-precision flexible CG: the oracle's direct solution of its own assembly to 1e-6, in
+    a fraction of Jacobi-CG's iterations on the same system (no counterpart in the reference)."""
+    import scipy.sparse.linalg as spla
+    rs = gpu
+    c = 24
+    xyz, tets, fixed, phi = _plates(rs, c)
+    ne = tets.shape[0]
+    eps_r = rs.field_lognormal(7, ne, 0.0, 1.0)
+    rho = (rs.field_uniform(8, ne) - 0.5) * 1e-9
+    asm = rs.Assembly(xyz, tets, 1, fixed)
+    n = asm.n_dof
+    db = rs.DeviceArray(n)
+    asm.assemble_electrostatics_device(*(rs.DeviceArray.from_host(a) for a in (eps_r, rho, phi)), db)
+    book, n_dof = orc.node_book(xyz.shape[0], tets, 1, fixed)
+    A_ref, b_ref = orc.electrostatics_assemble(xyz, tets, eps_r, rho, phi, book, n_dof)
+    x_ref = spla.spsolve(A_ref.tocsc(), b_ref)
+    L = rs.LearnedPreconditioner.from_bins(asm.matrix, asm, (c // 2, c // 2, c // 2), max_coarse=rs.MAX_COARSE)
+    L.set_cycle(1, 1, rs.COARSE_EXACT)
+    assert L.n_levels >= 2
+    conf = rs.MatrixSolverConf(n_outer_iterations=5000, solver_cvg_value=1e-8, stop_rule=rs.STOP_RELRES,
+                               precision=rs.PRECISION_MIXED)
+    dx = rs.DeviceArray.from_host(np.zeros(n))
+    info = rs.RMatrixSolver(conf).solve_learned_device(asm.matrix, L, db, dx)
+    assert info.converged == 1 and info.true_relres <= 1e-8
+    x = dx.download()
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-6
+    dj = rs.DeviceArray.from_host(np.zeros(n))
+    jac = rs.RMatrixSolver(rs.MatrixSolverConf(n_outer_iterations=100000, solver_cvg_value=1e-8,
+                                               stop_rule=rs.STOP_RELRES)).solve_device(asm.matrix, db, dj,
+                                                                                       rs.PRECOND_JACOBI, 1)
+    print(f"electrostatics {c}^3: multilevel {info.iterations} its, Jacobi-CG {jac.iterations} its", flush=True)
+    assert info.iterations * 3 < jac.iterations
